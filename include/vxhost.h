@@ -1,0 +1,149 @@
+/*
+ * vxhost.h -- C ABI of the host-side world store and per-view setup.
+ *
+ * This is the C++ mirror of the reference's CPU side of the path, so that a
+ * caller without the Rust crate can produce the exact pool bytes, chunk lists
+ * and frame records the GPU library consumes:
+ *
+ *   vxw_world_*           <- voxel::World               (src/voxel/world.rs:16-638)
+ *   vxw_insert_dense      <- build_local + insert_local (src/voxel/tree.rs:89-204,
+ *                                                        src/voxel/world.rs:63-153)
+ *   vxw_get/set_block     <- World::get_block/set_block (src/voxel/world.rs:258-529)
+ *   vxw_extract_dense     <- World::extract_dense       (src/voxel/world.rs:594-629)
+ *   vxw_pool_*            <- RunPool/WordPool data() + take_dirty()
+ *                            (src/voxel/geometry.rs:74-221, src/voxel/attributes.rs:6-88)
+ *   vxw_build_view        <- Frustum::from_camera + Renderer::build_chunk_list
+ *                            (src/math.rs:69-117, src/render/mod.rs:82-138,1447-1564)
+ *   vxw_pack_frame        <- the GpuFrame packing in Renderer::render
+ *                            (src/render/mod.rs:1612-1710)
+ *   vxw_generate          <- a seeded terrain generator with the reference's
+ *                            structure (heightfield, sea, caves, trees); it is
+ *                            fixture support and does not reproduce worldgen.rs bits.
+ *
+ * All calls are synchronous, CPU only, and return VXW_OK (0) or a negative code.
+ * A vxw_world is not thread safe; distinct worlds may be used concurrently.
+ */
+#ifndef VXHOST_H
+#define VXHOST_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "vxmarch.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    VXW_OK = 0,
+    VXW_E_ARG = -3,
+    VXW_E_MISSING = -7
+};
+
+typedef struct vxw_world vxw_world;
+
+/* ChunkRecord fields a caller inspects (src/voxel/chunk.rs:73-111). */
+typedef struct vxw_chunk_info {
+    int32_t key[3];
+    uint32_t root[4];            /* Inner as vec4<u32> */
+    uint32_t attr_uniform;       /* 1 if AttrRef::Uniform */
+    uint32_t attr_base;          /* uniform id or table base */
+    uint32_t attr_len;
+    int32_t aabb_min[3], aabb_max[3];
+    uint32_t solid_count, leaf_count;
+    uint32_t has_water, has_foliage, has_cutout, has_glass, has_emitter, emitters;
+    uint64_t dry_mask;
+} vxw_chunk_info;
+
+typedef struct vxw_pool_stats {
+    uint64_t live_elems, referenced_elems, live_runs, len;
+} vxw_pool_stats;
+
+vxw_world *vxw_world_new(void);
+void vxw_world_free(vxw_world *w);
+
+/* dense: 64^3 block ids indexed x | z<<6 | y<<12 (src/voxel/tree.rs:8-10). */
+int vxw_insert_dense(vxw_world *w, int32_t cx, int32_t cy, int32_t cz, const uint16_t *dense);
+int vxw_remove_chunk(vxw_world *w, int32_t cx, int32_t cy, int32_t cz);
+uint16_t vxw_get_block(const vxw_world *w, int32_t x, int32_t y, int32_t z);
+/* 1 if the chunk is resident (the edit applied or was a no-op), 0 otherwise. */
+int vxw_set_block(vxw_world *w, int32_t x, int32_t y, int32_t z, uint16_t id);
+/* Apply n 16-byte edit records (journal Record, src/journal.rs:116-122) in order.
+ * Returns the number of records that landed in a resident chunk. */
+int64_t vxw_apply_edits(vxw_world *w, const vx_edit *edits, size_t n);
+int vxw_extract_dense(const vxw_world *w, int32_t cx, int32_t cy, int32_t cz, uint16_t *out);
+int vxw_chunk_info_get(const vxw_world *w, int32_t cx, int32_t cy, int32_t cz, vxw_chunk_info *out);
+size_t vxw_chunk_count(const vxw_world *w);
+/* Resident chunk keys in insertion order (the render set order); returns the count. */
+size_t vxw_chunk_keys(const vxw_world *w, int32_t *keys_xyz, size_t cap);
+uint32_t vxw_emitter_chunks(const vxw_world *w);
+
+/* Pools: element sizes 16 (inners), 8 (leaves), 4 (bricks). */
+const void *vxw_pool_data(const vxw_world *w, int pool);
+size_t vxw_pool_bytes(const vxw_world *w, int pool);
+/* Moves up to cap pending dirty ranges (element units) into out and clears them.
+ * With out == NULL returns the pending count without clearing. */
+size_t vxw_take_dirty(vxw_world *w, int pool, vx_range *out, size_t cap);
+int vxw_pool_stats_get(const vxw_world *w, int pool, vxw_pool_stats *out);
+
+/* ---- terrain generator (fixture support) ---- */
+typedef struct vxw_gen_params {
+    int32_t seed;
+    int32_t sea_level;          /* water fills air columns below this y */
+    int32_t chunk_min[3];       /* inclusive chunk key range to generate */
+    int32_t chunk_max[3];       /* exclusive */
+    float cave_fraction;        /* target carved fraction of the subsurface, 0 = none */
+    float tree_density;         /* trees per column */
+    float glass_fraction;       /* fraction of exposed surface voxels turned to glass */
+    float leaf_extra;           /* fraction of exposed surface voxels turned to leaves */
+    int32_t threads;            /* <= 0: hardware threads */
+} vxw_gen_params;
+
+int vxw_generate(vxw_world *w, const vxw_gen_params *p);
+/* Surface height (first air y) of column (x, z) for these parameters. */
+int32_t vxw_height(const vxw_gen_params *p, int32_t x, int32_t z);
+/* Generates one chunk's dense grid (x | z<<6 | y<<12) without inserting it. */
+int vxw_generate_dense(const vxw_gen_params *p, int32_t cx, int32_t cy, int32_t cz, uint16_t *out);
+
+/* ---- per-view setup ---- */
+typedef struct vxw_camera {
+    float pos[3];       /* eye position */
+    float fwd[3], right[3], up[3];
+    float fov_y;        /* radians */
+    uint32_t width, height;
+    float far_;         /* view distance */
+    float sun_dir[3];
+    float daylight;
+    float sea_level;
+    float time;
+    float jitter[2];
+    uint32_t flags;     /* FLAG_* word (src/render/mod.rs:144-270) */
+    uint32_t frame_index;
+} vxw_camera;
+
+/* Player::look_dir + scene::camera_basis (src/player.rs:122-126, src/scene.rs:334-339). */
+void vxw_camera_basis(float yaw, float pitch, float fwd[3], float right[3], float up[3]);
+/* scene::sun (src/scene.rs:10-15). */
+void vxw_sun(float time_of_day, float dir[3], float *daylight);
+
+typedef struct vxw_view {
+    uint32_t n_visible;     /* frame.chunk_count */
+    uint32_t n_total;       /* visible + off-screen shadow casters */
+    uint32_t camera_chunk;  /* index among visible, or 0xFFFFFFFF */
+    int32_t grid_min[3];
+} vxw_view;
+
+/* Builds the distance-sorted chunk list and the 32x8x32 chunk grid for a camera.
+ * chunks must hold VX_MAX_CHUNKS entries, grid VX_GRID_LEN. */
+int vxw_build_view(const vxw_world *w, const vxw_camera *cam, vx_gpu_chunk *chunks, uint32_t *grid,
+                   vxw_view *out);
+/* Packs the GpuFrame record for a camera and view (max_pairs 0 = pairs_for(tiles)). */
+int vxw_pack_frame(const vxw_camera *cam, const vxw_view *view, uint32_t max_pairs, vx_gpu_frame *out);
+/* render::pairs_for (src/render/mod.rs:27-30). */
+uint32_t vxw_pairs_for(uint32_t tiles_x, uint32_t tiles_y);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

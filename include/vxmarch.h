@@ -1,0 +1,194 @@
+/*
+ * vxmarch.h -- C ABI of the MI355X (gfx950) voxel ray-march library.
+ *
+ * Drop-in replacement for the GPU side of the reference renderer's hot path:
+ * the primary-ray march writing the 64-bit visibility key, two-phase Hi-Z tile
+ * culling, and the sun-shadow ray. Inputs are the reference's own byte layouts
+ * (GpuFrame, GpuChunk, the three tree pools and the 32x8x32 chunk grid), so a
+ * host that already builds them for wgpu hands the same bytes to these calls.
+ *
+ * Entry point                 replaces (reference file:line)
+ *   vx_ctx_create/destroy     init_gpu + Renderer::new      (src/render/mod.rs:672-719, :851-1221)
+ *   vx_resize                 Renderer::resize              (src/render/mod.rs:1232-1275)
+ *   vx_pool_sync              GpuMirror::sync, via Renderer::sync_world
+ *                                                           (src/render/pools.rs:44-98, src/render/mod.rs:1355-1396)
+ *   vx_set_chunks             the chunk/grid write_buffer in build_chunk_list
+ *                                                           (src/render/mod.rs:1555-1562)
+ *   vx_render                 Renderer::render, passes tile_select..primary_shadow
+ *                                                           (src/render/mod.rs:1593-1795, :1821-1840)
+ *   vx_sync                   device.poll(Wait)             (src/headless.rs:489-496)
+ *   vx_read_vis/shadow/hiz    read_texture / buffer readback (src/render/mod.rs:2062-2112)
+ *   vx_read_decoded           shadow_guide + hit_t decode    (src/render/shaders/shadow.wgsl:8-30,
+ *                                                            src/render/shaders/common.wgsl:1300-1318)
+ *   vx_pair_demand            Renderer::pair_demand          (src/render/mod.rs:1948-1975)
+ *   vx_timings                Readback per-pass timestamps   (src/render/timer.rs:150-176)
+ *   vx_batch_*                no counterpart: many views per launch (the reference renders
+ *                             one view per render() call, src/headless.rs:680-837)
+ *   vx_serve_*                no counterpart: continuous-batching request loop with
+ *                             in-place edits (the reference's loop is headless.rs:419-497)
+ *
+ * Errors are int status codes; no exception crosses the ABI. A context owns one
+ * HIP stream and is not re-entrant; several contexts may coexist.
+ */
+#ifndef VXMARCH_H
+#define VXMARCH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    VX_OK = 0,
+    VX_E_NODEV = -1,
+    VX_E_OOM = -2,
+    VX_E_ARG = -3,
+    VX_E_OVERFLOW = -4,
+    VX_E_UNSUPPORTED = -5,
+    VX_E_HIP = -6,
+    VX_E_BUSY = -8
+};
+
+/* Limits (src/render/mod.rs:20-37, :82-91). */
+#define VX_TILE 8u
+#define VX_MAX_CHUNKS 8191u
+#define VX_MAX_PAIRS (1u << 20)
+#define VX_MAX_TILES (1u << 19)
+#define VX_GRID_X 32u
+#define VX_GRID_Y 8u
+#define VX_GRID_Z 32u
+#define VX_GRID_LEN (VX_GRID_X * VX_GRID_Y * VX_GRID_Z)
+#define VX_NO_CHUNK 0xFFFFFFFFu
+
+/* Frame flag bits the passes read (src/render/mod.rs:144-270). */
+#define VX_FLAG_HIZ 1u
+#define VX_FLAG_HEATMAP 2u
+#define VX_FLAG_SHADOWS 4u
+#define VX_FLAG_UNDERWATER 64u
+#define VX_FLAG_FOLIAGE 256u
+#define VX_FLAG_LEAF_CUTOUT 65536u
+#define VX_FLAG_GLASS 131072u
+#define VX_FLAG_LEAF_THIN 524288u
+#define VX_FLAG_WATER_FAR 2097152u
+#define VX_FLAG_WATER_DARK 4194304u
+/* spec_hi bits (src/render/mod.rs:220-270). */
+#define VX_FLAG_HI_FULL_MARCH 2u
+#define VX_FLAG_HI_EMITTER_WORLD 4u
+#define VX_FLAG_HI_SNELL_BEND 256u
+#define VX_FLAG_HI_WIND_SWAY 32768u
+#define VX_FLAG_HI_LEGACY_WATER 2097152u
+
+/* GpuFrame, 352 bytes (src/render/mod.rs:398-466). */
+typedef struct vx_gpu_frame {
+    float cam_pos[3];      float time;
+    float cam_fwd[3];      float tan_half_fov;
+    float cam_right[3];    float aspect;
+    float cam_up[3];       float far_;
+    float sun_dir[3];      float daylight;
+    uint32_t res[2];       uint32_t tiles[2];
+    uint32_t chunk_count;  uint32_t camera_chunk;
+    uint32_t flags;        uint32_t max_pairs;
+    float fog_density, fog_falloff, shadow_dist, ambient, fog_scatter, fog_g, fog_height;
+    float mip_bias;
+    int32_t grid_min[3];   float sea_level;
+    uint32_t grid_dim[3];  float water_absorb;
+    float jitter[2];       float taa_feedback;
+    uint32_t frame_index;
+    float shading[24];     /* cloud_cover .. zenith_deep (offsets 192-287), unused here */
+    float prev_view_proj[16];
+} vx_gpu_frame;
+
+/* GpuChunk, 96 bytes (src/render/mod.rs:468-489). */
+typedef struct vx_gpu_chunk {
+    float aabb_min[3];  float voxel_size;
+    float aabb_max[3];  uint32_t lod;
+    float origin[3];    float fade;
+    uint32_t root[4];   /* Inner: mask lo, mask hi, run_ptr | FULL, leaf_prefix | water_line << 28 */
+    uint32_t attr_base, attr_flags, light_base, light_flags;
+    uint32_t dry_mask[2];
+    uint32_t reserved[2];
+} vx_gpu_chunk;
+
+/* Dirty range in pool elements (RunPool/WordPool take_dirty, src/voxel/geometry.rs:213). */
+typedef struct vx_range {
+    uint32_t off, len;
+} vx_range;
+
+/* Edit journal record, 16 bytes (src/journal.rs:116-122). */
+typedef struct vx_edit {
+    int32_t x, y, z;
+    uint16_t id, pad;
+} vx_edit;
+
+enum { VX_POOL_INNERS = 0, VX_POOL_LEAVES = 1, VX_POOL_BRICKS = 2 };
+
+/* Algorithm used for the primary march.
+ * VX_PATH_PAIRS: the reference's decomposition -- tile_select emits (tile, chunk)
+ *   pairs, one wavefront marches one pair, then Hi-Z build/recover and a second march.
+ * VX_PATH_TILES: one wavefront owns an 8x8 tile: culls the chunk list in registers,
+ *   marches the surviving chunks front to back with exact per-lane early exit, and
+ *   runs the same two-phase Hi-Z selection inside the wave. Same output bits. */
+enum { VX_PATH_TILES = 0, VX_PATH_PAIRS = 1 };
+
+typedef struct vx_ctx_desc {
+    uint32_t width, height;   /* internal resolution */
+    uint32_t max_pairs;       /* 0 = render::pairs_for(tiles) */
+    uint32_t path;            /* VX_PATH_* */
+    int32_t device;           /* HIP device ordinal, -1 = current */
+} vx_ctx_desc;
+
+typedef struct vx_ctx vx_ctx;
+
+/* Counter readback (Renderer::pair_demand + march stats, src/render/mod.rs:1948-2045). */
+typedef struct vx_pair_demand {
+    uint32_t select, deferred, recover, cap;
+    uint32_t truncated;       /* 1 if any list overflowed cap (TRUNCATION_MARK) */
+    uint32_t shadow_rays;     /* pixels whose shadow_guide was valid */
+} vx_pair_demand;
+
+/* Per-pass GPU milliseconds of the last vx_render (hipEvent pairs on the context stream). */
+typedef struct vx_timings {
+    float tile_select, march, hiz, recover, march2, shadow, total;
+} vx_timings;
+
+/* Decoded visibility (voxel coordinate, face, material id, hit distance). */
+typedef struct vx_hit {
+    int32_t voxel[3];     /* world voxel (origin + v * voxel_size) */
+    uint32_t face;        /* normal id 0..7, valid when t >= 0 */
+    uint32_t material;    /* block id */
+    uint32_t ci;          /* chunk index in the view list */
+    float t;              /* hit distance (hit_t), -1 on a miss */
+    uint32_t shadow;      /* 1 if the sun ray was blocked */
+} vx_hit;
+
+int vx_ctx_create(const vx_ctx_desc *desc, vx_ctx **out);
+int vx_ctx_destroy(vx_ctx *ctx);
+int vx_resize(vx_ctx *ctx, uint32_t width, uint32_t height);
+int vx_pool_sync(vx_ctx *ctx, int pool, const void *data, size_t bytes, const vx_range *dirty, size_t n_dirty);
+int vx_set_chunks(vx_ctx *ctx, const vx_gpu_chunk *chunks, uint32_t n_total, const uint32_t *grid);
+/* spec = frame flags & SPEC_MASK, spec_hi = spec_hi word (the pipeline key of
+ * Renderer::render, src/render/mod.rs:1597). Foliage tufts and the Snell bend need
+ * the texture atlas / refract and return VX_E_UNSUPPORTED. */
+int vx_render(vx_ctx *ctx, const vx_gpu_frame *frame, uint32_t spec, uint32_t spec_hi);
+int vx_sync(vx_ctx *ctx);
+int vx_read_vis(vx_ctx *ctx, uint64_t *out, size_t n);
+int vx_read_shadow(vx_ctx *ctx, uint16_t *out, size_t n);
+int vx_read_hiz(vx_ctx *ctx, uint32_t *out, size_t n);
+int vx_write_hiz(vx_ctx *ctx, const uint32_t *in, size_t n);
+/* Pair list + 16 counter words of the last frame (VX_PATH_PAIRS contexts only): the
+ * reference's pairs/counters storage buffers (src/render/mod.rs:996-1037). */
+int vx_read_pairs(vx_ctx *ctx, uint32_t *out, size_t n, uint32_t *counters16);
+int vx_read_decoded(vx_ctx *ctx, vx_hit *out, size_t n);
+int vx_pair_demand_get(vx_ctx *ctx, vx_pair_demand *out);
+int vx_timings_get(vx_ctx *ctx, vx_timings *out);
+/* Device pointers of the current outputs (for callers that keep data on the GPU). */
+int vx_device_outputs(vx_ctx *ctx, uint64_t **vis, uint16_t **shadow);
+const char *vx_error_string(int code);
+int vx_device_name(char *buf, size_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

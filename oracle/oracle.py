@@ -1,0 +1,166 @@
+"""ctypes binding of the CPU oracle (oracle/vxoracle.c).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker. The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libvxoracle.so")
+
+
+class Pools(C.Structure):
+    _fields_ = [("inners", C.c_void_p), ("n_inners", C.c_size_t),
+                ("leaves", C.c_void_p), ("n_leaves", C.c_size_t),
+                ("bricks", C.c_void_p), ("n_bricks", C.c_size_t)]
+
+
+class Spec(C.Structure):
+    _fields_ = [("leaf_cutout", C.c_uint32), ("glass", C.c_uint32), ("full_march", C.c_uint32),
+                ("emitter_gather", C.c_uint32), ("water_repair", C.c_uint32), ("leaf_fill", C.c_float)]
+
+
+_STAT_FIELDS = ["pairs_select", "pairs_deferred", "pairs_recover", "rays_primary", "rays_shadow",
+                "chunk_marches", "iters", "l1_loads", "leaf_loads", "attr_loads", "brick_words",
+                "grid_loads", "chunk_loads", "node_bytes", "capped", "oob"]
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in _STAT_FIELDS]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n in _STAT_FIELDS}
+
+
+class Hit(C.Structure):
+    _fields_ = [("hit", C.c_uint32), ("t", C.c_float), ("normal", C.c_uint32),
+                ("voxel", C.c_uint32 * 3), ("micro", C.c_uint32 * 3), ("iters", C.c_uint32)]
+
+
+DECODED = np.dtype([("voxel", "<i4", 3), ("face", "<u4"), ("material", "<u4"), ("ci", "<u4"), ("t", "<f4")])
+
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with the recipe in oracle/Makefile."""
+    if force or not os.path.exists(LIB):
+        subprocess.run(["make", "-C", HERE, "-s"], check=True)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        V = C.c_void_p
+        L.vxo_render.restype = C.c_int
+        L.vxo_render.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, C.POINTER(Spec), V, V, V,
+                                 C.POINTER(Stats), C.c_int]
+        L.vxo_primary_shadow.restype = C.c_int
+        L.vxo_primary_shadow.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, C.POINTER(Spec), V, V,
+                                         C.POINTER(Stats), C.c_int]
+        L.vxo_decode.restype = C.c_int
+        L.vxo_decode.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, V, C.c_int]
+        L.vxo_ray_dir.restype = None
+        L.vxo_ray_dir.argtypes = [V, C.c_float, C.c_float, V]
+        L.vxo_depth_key.restype = C.c_uint32
+        L.vxo_depth_key.argtypes = [V, C.c_float]
+        L.vxo_make_key.restype = C.c_uint64
+        L.vxo_make_key.argtypes = [V, C.c_float, C.c_uint32, C.c_uint32, V, V]
+        L.vxo_hash_u32.restype = C.c_uint32
+        L.vxo_hash_u32.argtypes = [C.c_uint32]
+        L.vxo_f32_to_f16.restype = C.c_uint16
+        L.vxo_f32_to_f16.argtypes = [C.c_float]
+        L.vxo_block_at.restype = C.c_uint32
+        L.vxo_block_at.argtypes = [C.POINTER(Pools), V, C.c_uint32, V]
+        L.vxo_march_chunk.restype = None
+        L.vxo_march_chunk.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, C.POINTER(Spec), C.c_uint32,
+                                      V, V, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(Hit)]
+        L.vxo_shadow_ray_t.restype = C.c_float
+        L.vxo_shadow_ray_t.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, C.POINTER(Spec), V, V,
+                                       C.c_float]
+        L.vxo_hardware_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def spec_from(flags: int, spec_hi: int) -> Spec:
+    """WGSL override values for a (flags, spec_hi) pipeline key (src/render/mod.rs:2355-2689)."""
+    return Spec(
+        leaf_cutout=1 if flags & 65536 else 0,
+        glass=1 if flags & 131072 else 0,
+        full_march=1 if spec_hi & 2 else 0,
+        emitter_gather=1 if spec_hi & 4 else 0,
+        water_repair=0 if spec_hi & 2097152 else 1,
+        leaf_fill=0.62 if flags & 524288 else 0.72,
+    )
+
+
+class Scene:
+    """Pools + chunk list + grid as numpy arrays, kept alive for the oracle calls."""
+
+    def __init__(self, inners: np.ndarray, leaves: np.ndarray, bricks: np.ndarray,
+                 chunks: np.ndarray, grid: np.ndarray):
+        self.inners = np.ascontiguousarray(inners, dtype=np.uint32).reshape(-1)
+        self.leaves = np.ascontiguousarray(leaves, dtype=np.uint32).reshape(-1)
+        self.bricks = np.ascontiguousarray(bricks, dtype=np.uint32).reshape(-1)
+        self.chunks = np.ascontiguousarray(chunks)
+        self.n_chunks = self.chunks.nbytes // 96
+        self.grid = np.ascontiguousarray(grid, dtype=np.uint32)
+        self.pools = Pools(_p(self.inners) if self.inners.size else None, self.inners.size // 4,
+                           _p(self.leaves) if self.leaves.size else None, self.leaves.size // 2,
+                           _p(self.bricks) if self.bricks.size else None, self.bricks.size)
+
+
+def render(scene: Scene, frame_bytes: bytes, spec: Spec, hiz: np.ndarray, shadow: bool = True,
+           threads: int = 0):
+    """Run the reference pass sequence on the CPU. Returns (vis, shadow_f16, hiz_out, stats)."""
+    f = np.frombuffer(frame_bytes, dtype=np.uint8).copy()
+    res = np.frombuffer(frame_bytes[80:88], dtype=np.uint32)
+    w, h = int(res[0]), int(res[1])
+    vis = np.zeros(w * h, dtype=np.uint64)
+    sh = np.zeros(w * h, dtype=np.uint16) if shadow else None
+    hz = np.ascontiguousarray(hiz, dtype=np.uint32).copy()
+    st = Stats()
+    rc = lib().vxo_render(C.byref(scene.pools), _p(scene.chunks), scene.n_chunks, _p(scene.grid), _p(f),
+                          C.byref(spec), _p(hz), _p(vis), _p(sh), C.byref(st), threads)
+    if rc != 0:
+        raise RuntimeError("oracle render rejected its arguments")
+    return vis, sh, hz, st.as_dict()
+
+
+def primary_shadow(scene: Scene, frame_bytes: bytes, spec: Spec, vis: np.ndarray, threads: int = 0):
+    f = np.frombuffer(frame_bytes, dtype=np.uint8).copy()
+    sh = np.zeros(vis.size, dtype=np.uint16)
+    st = Stats()
+    v = np.ascontiguousarray(vis, dtype=np.uint64)
+    rc = lib().vxo_primary_shadow(C.byref(scene.pools), _p(scene.chunks), scene.n_chunks, _p(scene.grid), _p(f),
+                                  C.byref(spec), _p(v), _p(sh), C.byref(st), threads)
+    if rc != 0:
+        raise RuntimeError("oracle shadow rejected its arguments")
+    return sh, st.as_dict()
+
+
+def decode(scene: Scene, frame_bytes: bytes, vis: np.ndarray):
+    f = np.frombuffer(frame_bytes, dtype=np.uint8).copy()
+    out = np.zeros(vis.size, dtype=DECODED)
+    v = np.ascontiguousarray(vis, dtype=np.uint64)
+    lib().vxo_decode(C.byref(scene.pools), _p(scene.chunks), scene.n_chunks, _p(f), _p(v), _p(out), 1)
+    return out
+
+
+def hardware_threads() -> int:
+    return int(lib().vxo_hardware_threads())

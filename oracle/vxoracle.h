@@ -1,0 +1,155 @@
+/*
+ * vxoracle.h -- CPU oracle for the voxel ray-march hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is the checker: a plain-C restatement of the
+ * reference's WGSL passes (primary march, Hi-Z tile culling, sun shadow) in the
+ * reference's f32 operation order, compiled with -ffp-contract=off. Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it. The
+ * product library never links or calls it.
+ *
+ * Every struct here is an independent restatement of the reference layouts:
+ *   vxo_frame  <- GpuFrame  (src/render/mod.rs:398-466, 352 bytes)
+ *   vxo_chunk  <- GpuChunk  (src/render/mod.rs:468-489, 96 bytes)
+ * Pools are the reference's three storage buffers
+ *   inners: array<vec4<u32>>, leaves: array<vec2<u32>>, bricks: array<u32>
+ *   (src/render/shaders/common.wgsl:100-104).
+ */
+#ifndef VXORACLE_H
+#define VXORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vxo_frame {
+    float cam_pos[3];      float time;
+    float cam_fwd[3];      float tan_half_fov;
+    float cam_right[3];    float aspect;
+    float cam_up[3];       float far_;
+    float sun_dir[3];      float daylight;
+    uint32_t res[2];       uint32_t tiles[2];
+    uint32_t chunk_count;  uint32_t camera_chunk;
+    uint32_t flags;        uint32_t max_pairs;
+    float fog_density, fog_falloff, shadow_dist, ambient, fog_scatter, fog_g, fog_height;
+    float mip_bias;
+    int32_t grid_min[3];   float sea_level;
+    uint32_t grid_dim[3];  float water_absorb;
+    float jitter[2];       float taa_feedback;
+    uint32_t frame_index;
+    float rest[40];        /* cloud_cover .. prev_view_proj: not read by these passes */
+} vxo_frame;
+
+typedef struct vxo_chunk {
+    float aabb_min[3];  float voxel_size;
+    float aabb_max[3];  uint32_t lod;
+    float origin[3];    float fade;
+    uint32_t root[4];
+    uint32_t attr_base, attr_flags, light_base, light_flags;
+    uint32_t dry_mask[2];
+    uint32_t reserved[2];
+} vxo_chunk;
+
+typedef struct vxo_pools {
+    const uint32_t *inners;  size_t n_inners;   /* count of vec4<u32> */
+    const uint32_t *leaves;  size_t n_leaves;   /* count of vec2<u32> */
+    const uint32_t *bricks;  size_t n_bricks;   /* count of u32 */
+} vxo_pools;
+
+/* WGSL override constants the march reads (src/render/shaders/common.wgsl:180-263,
+ * set per pipeline in src/render/mod.rs:2355-2689). */
+typedef struct vxo_spec {
+    uint32_t leaf_cutout;     /* SPEC_LEAF_CUTOUT */
+    uint32_t glass;           /* SPEC_GLASS */
+    uint32_t full_march;      /* SPEC_FULL_MARCH */
+    uint32_t emitter_gather;  /* SPEC_EMITTER_GATHER */
+    uint32_t water_repair;    /* SPEC_WATER_REPAIR */
+    float leaf_fill;          /* SPEC_LEAF_FILL */
+} vxo_spec;
+
+/* Algorithmic traffic and work of one frame, counted exactly by the restatement. */
+typedef struct vxo_stats {
+    uint64_t pairs_select;     /* counters[4]: raw tile_select emits */
+    uint64_t pairs_deferred;   /* counters[1]: raw Hi-Z deferrals */
+    uint64_t pairs_recover;    /* counters[5]: raw recovered pairs */
+    uint64_t rays_primary;     /* (pixel, pair) marches issued */
+    uint64_t rays_shadow;      /* pixels whose shadow_guide is valid */
+    uint64_t chunk_marches;    /* march_chunk calls (primary + shadow) */
+    uint64_t iters;            /* DDA iterations */
+    uint64_t l1_loads, leaf_loads, attr_loads, brick_words, grid_loads, chunk_loads;
+    uint64_t node_bytes;       /* 96*chunk + 16*l1 + 8*leaf + 4*(attr+brick+grid) */
+    uint64_t capped;           /* marches stopped by MAX_ITERS */
+    uint64_t oob;              /* pool reads outside the supplied arrays */
+} vxo_stats;
+
+/* Single-function restatements exposed for pinning tests. */
+void     vxo_ray_dir(const vxo_frame *f, float px, float py, float out[3]);
+uint32_t vxo_depth_key(const vxo_frame *f, float t);
+uint64_t vxo_make_key(const vxo_frame *f, float t, uint32_t normal, uint32_t ci,
+                      const uint32_t v[3], const uint32_t micro[3]);
+uint32_t vxo_hash_u32(uint32_t x);
+uint16_t vxo_f32_to_f16(float x);
+uint32_t vxo_block_at(const vxo_pools *p, const vxo_chunk *chunks, uint32_t ci,
+                      const uint32_t v[3]);
+
+typedef struct vxo_hit {
+    uint32_t hit;
+    float t;
+    uint32_t normal;
+    uint32_t voxel[3];
+    uint32_t micro[3];
+    uint32_t iters;
+} vxo_hit;
+
+void vxo_march_chunk(const vxo_pools *p, const vxo_chunk *chunks, uint32_t n_chunks,
+                     const uint32_t *grid, const vxo_frame *f, const vxo_spec *s,
+                     uint32_t ci, const float ro_w[3], const float rd[3], float t_limit,
+                     int skip_water, int skip_foliage, int skip_glass, int sun_ray,
+                     vxo_hit *out);
+
+float vxo_shadow_ray_t(const vxo_pools *p, const vxo_chunk *chunks, uint32_t n_chunks,
+                       const uint32_t *grid, const vxo_frame *f, const vxo_spec *s,
+                       const float ro[3], const float rd[3], float max_dist);
+
+/*
+ * One frame of the reference pass sequence (src/render/mod.rs:1716-1795,1821-1840):
+ * clear -> tile_select -> march -> [build_hiz -> recover_select -> march] -> primary_shadow.
+ * hiz: in = previous frame's Hi-Z (T entries; ignored and zeroed when FLAG_HIZ is off),
+ *      out = this frame's Hi-Z as the reference leaves it (built after the first march).
+ * The pair list is never truncated here; the raw demand is reported in stats so a
+ * caller can check it against frame->max_pairs.
+ * vis: W*H keys. shadow: W*H f16 bit patterns (R16F target). Either may be NULL to skip.
+ * threads <= 0 selects the hardware thread count.
+ * Returns 0, or -1 on bad arguments.
+ */
+int vxo_render(const vxo_pools *p, const vxo_chunk *chunks, uint32_t n_chunks,
+               const uint32_t *grid, const vxo_frame *f, const vxo_spec *s,
+               uint32_t *hiz, uint64_t *vis, uint16_t *shadow, vxo_stats *stats,
+               int threads);
+
+/* Shadow pass alone, from a given visibility buffer. */
+int vxo_primary_shadow(const vxo_pools *p, const vxo_chunk *chunks, uint32_t n_chunks,
+                       const uint32_t *grid, const vxo_frame *f, const vxo_spec *s,
+                       const uint64_t *vis, uint16_t *shadow, vxo_stats *stats, int threads);
+
+/* Decode one visibility key into world voxel, face, material and f32 distance
+ * (shadow_guide/hit_t, src/render/shaders/shadow.wgsl:8-30, common.wgsl:1300-1318). */
+typedef struct vxo_decoded {
+    int32_t voxel[3];     /* world voxel coordinate (origin + v*vs) */
+    uint32_t face;        /* normal id 0..7 */
+    uint32_t material;    /* block id */
+    uint32_t ci;
+    float t;              /* hit distance re-derived by hit_t */
+} vxo_decoded;
+
+int vxo_decode(const vxo_pools *p, const vxo_chunk *chunks, uint32_t n_chunks,
+               const vxo_frame *f, const uint64_t *vis, vxo_decoded *out, int threads);
+
+int vxo_hardware_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

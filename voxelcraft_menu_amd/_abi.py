@@ -1,0 +1,272 @@
+"""ctypes view of the two C ABIs (include/vxhost.h, include/vxmarch.h).
+
+The shared objects are built in-tree by ``voxelcraft_menu_amd.build`` and loaded
+from this directory. Loading fails loudly: there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HOST_LIB = os.path.join(HERE, "libvxhost.so")
+MARCH_LIB = os.path.join(HERE, "libvxmarch.so")
+
+# ---- constants (include/vxmarch.h; src/render/mod.rs:20-37, :144-270) ----
+VX_OK = 0
+VX_TILE = 8
+VX_MAX_CHUNKS = 8191
+VX_MAX_PAIRS = 1 << 20
+VX_MAX_TILES = 1 << 19
+VX_GRID = (32, 8, 32)
+VX_GRID_LEN = 32 * 8 * 32
+VX_NO_CHUNK = 0xFFFFFFFF
+POOL_INNERS, POOL_LEAVES, POOL_BRICKS = 0, 1, 2
+POOL_ELEM = {POOL_INNERS: 16, POOL_LEAVES: 8, POOL_BRICKS: 4}
+PATH_TILES, PATH_PAIRS = 0, 1
+
+FLAG_HIZ = 1
+FLAG_HEATMAP = 2
+FLAG_SHADOWS = 4
+FLAG_AO = 8
+FLAG_WATER_REFLECT = 16
+FLAG_WATER_REFRACT = 32
+FLAG_UNDERWATER = 64
+FLAG_TINT = 128
+FLAG_FOLIAGE = 256
+FLAG_DISTANT_SHADOWS = 32768
+FLAG_LEAF_CUTOUT = 65536
+FLAG_GLASS = 131072
+FLAG_WATER_SHADOW_CUT = 262144
+FLAG_LEAF_THIN = 524288
+FLAG_WATER_FAR = 2097152
+FLAG_WATER_DARK = 4194304
+FLAG_SNELL = 8388608
+FLAG_HI_SKY_SPECULAR = 1
+FLAG_HI_FULL_MARCH = 2
+FLAG_HI_EMITTER_WORLD = 4
+FLAG_HI_SNELL_BEND = 256
+FLAG_HI_WIND_SWAY = 32768
+FLAG_HI_SHADOW_PASS = 524288
+FLAG_HI_LEGACY_WATER = 2097152
+
+ERRORS = {0: "ok", -1: "no device", -2: "out of memory", -3: "bad argument", -4: "overflow",
+          -5: "unsupported", -6: "hip error", -7: "missing", -8: "busy"}
+
+
+class Frame(C.Structure):
+    """GpuFrame, 352 bytes (src/render/mod.rs:398-466)."""
+    _fields_ = [
+        ("cam_pos", C.c_float * 3), ("time", C.c_float),
+        ("cam_fwd", C.c_float * 3), ("tan_half_fov", C.c_float),
+        ("cam_right", C.c_float * 3), ("aspect", C.c_float),
+        ("cam_up", C.c_float * 3), ("far_", C.c_float),
+        ("sun_dir", C.c_float * 3), ("daylight", C.c_float),
+        ("res", C.c_uint32 * 2), ("tiles", C.c_uint32 * 2),
+        ("chunk_count", C.c_uint32), ("camera_chunk", C.c_uint32),
+        ("flags", C.c_uint32), ("max_pairs", C.c_uint32),
+        ("fog_density", C.c_float), ("fog_falloff", C.c_float), ("shadow_dist", C.c_float),
+        ("ambient", C.c_float), ("fog_scatter", C.c_float), ("fog_g", C.c_float),
+        ("fog_height", C.c_float), ("mip_bias", C.c_float),
+        ("grid_min", C.c_int32 * 3), ("sea_level", C.c_float),
+        ("grid_dim", C.c_uint32 * 3), ("water_absorb", C.c_float),
+        ("jitter", C.c_float * 2), ("taa_feedback", C.c_float),
+        ("frame_index", C.c_uint32),
+        ("shading", C.c_float * 24),
+        ("prev_view_proj", C.c_float * 16),
+    ]
+
+
+class Chunk(C.Structure):
+    """GpuChunk, 96 bytes (src/render/mod.rs:468-489)."""
+    _fields_ = [
+        ("aabb_min", C.c_float * 3), ("voxel_size", C.c_float),
+        ("aabb_max", C.c_float * 3), ("lod", C.c_uint32),
+        ("origin", C.c_float * 3), ("fade", C.c_float),
+        ("root", C.c_uint32 * 4),
+        ("attr_base", C.c_uint32), ("attr_flags", C.c_uint32),
+        ("light_base", C.c_uint32), ("light_flags", C.c_uint32),
+        ("dry_mask", C.c_uint32 * 2), ("reserved", C.c_uint32 * 2),
+    ]
+
+
+assert C.sizeof(Frame) == 352 and Frame.prev_view_proj.offset == 288 and Frame.grid_min.offset == 144
+assert C.sizeof(Chunk) == 96 and Chunk.root.offset == 48 and Chunk.dry_mask.offset == 80
+
+
+class Range(C.Structure):
+    _fields_ = [("off", C.c_uint32), ("len", C.c_uint32)]
+
+
+class Edit(C.Structure):
+    """Journal edit record, 16 bytes (src/journal.rs:116-122)."""
+    _fields_ = [("x", C.c_int32), ("y", C.c_int32), ("z", C.c_int32), ("id", C.c_uint16), ("pad", C.c_uint16)]
+
+
+assert C.sizeof(Edit) == 16
+
+EDIT_DTYPE = np.dtype([("x", "<i4"), ("y", "<i4"), ("z", "<i4"), ("id", "<u2"), ("pad", "<u2")])
+
+
+class ChunkInfo(C.Structure):
+    _fields_ = [
+        ("key", C.c_int32 * 3), ("root", C.c_uint32 * 4), ("attr_uniform", C.c_uint32),
+        ("attr_base", C.c_uint32), ("attr_len", C.c_uint32),
+        ("aabb_min", C.c_int32 * 3), ("aabb_max", C.c_int32 * 3),
+        ("solid_count", C.c_uint32), ("leaf_count", C.c_uint32),
+        ("has_water", C.c_uint32), ("has_foliage", C.c_uint32), ("has_cutout", C.c_uint32),
+        ("has_glass", C.c_uint32), ("has_emitter", C.c_uint32), ("emitters", C.c_uint32),
+        ("dry_mask", C.c_uint64),
+    ]
+
+
+class PoolStats(C.Structure):
+    _fields_ = [("live_elems", C.c_uint64), ("referenced_elems", C.c_uint64),
+                ("live_runs", C.c_uint64), ("len", C.c_uint64)]
+
+
+class GenParams(C.Structure):
+    _fields_ = [
+        ("seed", C.c_int32), ("sea_level", C.c_int32),
+        ("chunk_min", C.c_int32 * 3), ("chunk_max", C.c_int32 * 3),
+        ("cave_fraction", C.c_float), ("tree_density", C.c_float),
+        ("glass_fraction", C.c_float), ("leaf_extra", C.c_float), ("threads", C.c_int32),
+    ]
+
+
+class Camera(C.Structure):
+    _fields_ = [
+        ("pos", C.c_float * 3), ("fwd", C.c_float * 3), ("right", C.c_float * 3), ("up", C.c_float * 3),
+        ("fov_y", C.c_float), ("width", C.c_uint32), ("height", C.c_uint32), ("far_", C.c_float),
+        ("sun_dir", C.c_float * 3), ("daylight", C.c_float), ("sea_level", C.c_float), ("time", C.c_float),
+        ("jitter", C.c_float * 2), ("flags", C.c_uint32), ("frame_index", C.c_uint32),
+    ]
+
+
+class View(C.Structure):
+    _fields_ = [("n_visible", C.c_uint32), ("n_total", C.c_uint32), ("camera_chunk", C.c_uint32),
+                ("grid_min", C.c_int32 * 3)]
+
+
+class CtxDesc(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("max_pairs", C.c_uint32),
+                ("path", C.c_uint32), ("device", C.c_int32)]
+
+
+class PairDemand(C.Structure):
+    _fields_ = [("select", C.c_uint32), ("deferred", C.c_uint32), ("recover", C.c_uint32),
+                ("cap", C.c_uint32), ("truncated", C.c_uint32), ("shadow_rays", C.c_uint32)]
+
+
+class Timings(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("tile_select", "march", "hiz", "recover", "march2", "shadow", "total")]
+
+
+class Hit(C.Structure):
+    _fields_ = [("voxel", C.c_int32 * 3), ("face", C.c_uint32), ("material", C.c_uint32),
+                ("ci", C.c_uint32), ("t", C.c_float), ("shadow", C.c_uint32)]
+
+
+HIT_DTYPE = np.dtype([("voxel", "<i4", 3), ("face", "<u4"), ("material", "<u4"), ("ci", "<u4"),
+                      ("t", "<f4"), ("shadow", "<u4")])
+assert HIT_DTYPE.itemsize == C.sizeof(Hit)
+
+P = C.c_void_p
+_sig_host = {
+    "vxw_world_new": (P, []),
+    "vxw_world_free": (None, [P]),
+    "vxw_insert_dense": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, P]),
+    "vxw_remove_chunk": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32]),
+    "vxw_get_block": (C.c_uint16, [P, C.c_int32, C.c_int32, C.c_int32]),
+    "vxw_set_block": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_uint16]),
+    "vxw_apply_edits": (C.c_int64, [P, P, C.c_size_t]),
+    "vxw_extract_dense": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, P]),
+    "vxw_chunk_info_get": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.POINTER(ChunkInfo)]),
+    "vxw_chunk_count": (C.c_size_t, [P]),
+    "vxw_chunk_keys": (C.c_size_t, [P, P, C.c_size_t]),
+    "vxw_emitter_chunks": (C.c_uint32, [P]),
+    "vxw_pool_data": (P, [P, C.c_int]),
+    "vxw_pool_bytes": (C.c_size_t, [P, C.c_int]),
+    "vxw_take_dirty": (C.c_size_t, [P, C.c_int, P, C.c_size_t]),
+    "vxw_pool_stats_get": (C.c_int, [P, C.c_int, C.POINTER(PoolStats)]),
+    "vxw_generate": (C.c_int, [P, C.POINTER(GenParams)]),
+    "vxw_height": (C.c_int32, [C.POINTER(GenParams), C.c_int32, C.c_int32]),
+    "vxw_generate_dense": (C.c_int, [C.POINTER(GenParams), C.c_int32, C.c_int32, C.c_int32, P]),
+    "vxw_camera_basis": (None, [C.c_float, C.c_float, P, P, P]),
+    "vxw_sun": (None, [C.c_float, P, P]),
+    "vxw_build_view": (C.c_int, [P, C.POINTER(Camera), P, P, C.POINTER(View)]),
+    "vxw_pack_frame": (C.c_int, [C.POINTER(Camera), C.POINTER(View), C.c_uint32, C.POINTER(Frame)]),
+    "vxw_pairs_for": (C.c_uint32, [C.c_uint32, C.c_uint32]),
+}
+
+_sig_march = {
+    "vx_ctx_create": (C.c_int, [C.POINTER(CtxDesc), C.POINTER(P)]),
+    "vx_ctx_destroy": (C.c_int, [P]),
+    "vx_resize": (C.c_int, [P, C.c_uint32, C.c_uint32]),
+    "vx_pool_sync": (C.c_int, [P, C.c_int, P, C.c_size_t, P, C.c_size_t]),
+    "vx_set_chunks": (C.c_int, [P, P, C.c_uint32, P]),
+    "vx_render": (C.c_int, [P, C.POINTER(Frame), C.c_uint32, C.c_uint32]),
+    "vx_sync": (C.c_int, [P]),
+    "vx_read_vis": (C.c_int, [P, P, C.c_size_t]),
+    "vx_read_shadow": (C.c_int, [P, P, C.c_size_t]),
+    "vx_read_hiz": (C.c_int, [P, P, C.c_size_t]),
+    "vx_write_hiz": (C.c_int, [P, P, C.c_size_t]),
+    "vx_read_pairs": (C.c_int, [P, P, C.c_size_t, P]),
+    "vx_read_decoded": (C.c_int, [P, P, C.c_size_t]),
+    "vx_pair_demand_get": (C.c_int, [P, C.POINTER(PairDemand)]),
+    "vx_timings_get": (C.c_int, [P, C.POINTER(Timings)]),
+    "vx_device_outputs": (C.c_int, [P, P, P]),
+    "vx_error_string": (C.c_char_p, [C.c_int]),
+    "vx_device_name": (C.c_int, [C.c_char_p, C.c_size_t]),
+}
+
+_host = None
+_march = None
+
+
+def _bind(lib, sigs):
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def host():
+    """The host world-store library (CPU only)."""
+    global _host
+    if _host is None:
+        if not os.path.exists(HOST_LIB):
+            raise RuntimeError(f"{HOST_LIB} is not built; run voxelcraft_menu_amd.build.build()")
+        _host = _bind(C.CDLL(HOST_LIB), _sig_host)
+    return _host
+
+
+def march():
+    """The HIP ray-march library. Raises if it is not built: there is no CPU fallback."""
+    global _march
+    if _march is None:
+        if not os.path.exists(MARCH_LIB):
+            raise RuntimeError(f"{MARCH_LIB} is not built; run voxelcraft_menu_amd.build.build()")
+        _march = _bind(C.CDLL(MARCH_LIB), _sig_march)
+    return _march
+
+
+def host_symbols():
+    return list(_sig_host)
+
+
+def march_symbols():
+    return list(_sig_march)
+
+
+def check(rc, what="call"):
+    if rc < 0:
+        raise RuntimeError(f"{what} failed: {ERRORS.get(rc, rc)}")
+    return rc
+
+
+def ptr(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data)

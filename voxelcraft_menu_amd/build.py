@@ -1,0 +1,74 @@
+"""Build recipe for the two in-tree shared objects.
+
+  libvxhost.so   host world store + view setup (g++, CPU only)
+  libvxmarch.so  HIP kernels + C ABI for gfx950 (hipcc)
+
+Floating point: the kernels are compiled without FMA contraction, with correctly
+rounded f32 division and square root and with denormals kept, so every f32 operation
+rounds once in source order. Packed-FP32 instruction selection is disabled.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+INC = os.path.join(ROOT, "include")
+CSRC = os.path.join(HERE, "csrc")
+
+HOST_SOURCES = ["host/world.cpp", "host/worldgen.cpp", "host/view.cpp", "host/capi.cpp"]
+HIP_SOURCES = ["kernels.hip", "vxmarch.hip"]
+HIP_EXTRA = []  # further .hip translation units are appended by later modules
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt", "-fno-gpu-flush-denormals-to-zero",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-command-line-argument",
+]
+
+
+def _hipcc() -> str:
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"build step failed: {cmd[0]} ... -o {cmd[-1]}")
+    return r
+
+
+def build_host(verbose: bool = False) -> str:
+    out = os.path.join(HERE, "libvxhost.so")
+    cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-Wextra", f"-I{INC}"]
+    cmd += [os.path.join(CSRC, s) for s in HOST_SOURCES] + ["-lpthread", "-o", out]
+    _run(cmd, verbose)
+    return out
+
+
+def build_march(verbose: bool = False) -> str:
+    out = os.path.join(HERE, "libvxmarch.so")
+    tmp = out + ".tmp"
+    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES + HIP_EXTRA]
+    cmd = [_hipcc()] + HIPCC_FLAGS + [f"-I{INC}", f"-I{CSRC}"] + srcs + ["-o", tmp]
+    _run(cmd, verbose)
+    os.replace(tmp, out)
+    return out
+
+
+def build(verbose: bool = False):
+    return build_host(verbose), build_march(verbose)
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,756 @@
+// march_core.hpp -- device-side voxel DDA for gfx950.
+//
+// The arithmetic reproduces the reference WGSL bit for bit: f32, IEEE division and
+// square root (this file is compiled with -ffp-contract=off and correctly rounded
+// div/sqrt), WGSL operation order, minNum/maxNum for min/max. Each function cites
+// the WGSL it computes. Pool reads are bounds-guarded: an index outside a pool reads
+// zeros, so malformed input cannot take the GPU outside its buffers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vxmarch.h"
+
+namespace vxd {
+
+constexpr uint32_t FULL_BIT = 0x80000000u;
+constexpr uint32_t PTR_MASK = 0x7FFFFFFFu;
+constexpr uint32_t UNIFORM_BIT = 0x80000000u;
+constexpr uint32_t NO_CHUNK = 0xFFFFFFFFu;
+constexpr uint32_t MAX_ITERS = 512u;
+constexpr uint32_t WATER_ID = 13u, GLASS_ID = 10u, GLOWSTONE_ID = 11u, LEAVES_ID = 7u, PINE_LEAVES_ID = 16u;
+constexpr uint32_t ATTR_UNIFORM = 1u, ATTR_HAS_WATER = 2u, ATTR_HAS_CUTOUT = 8u, ATTR_HAS_GLASS = 16u,
+                   ATTR_HAS_EMITTER = 32u;
+constexpr uint32_t NORMAL_CROSS_A = 6u, NORMAL_CROSS_B = 7u;
+
+// Pipeline specialisation bits (the WGSL override constants the march reads).
+constexpr uint32_t SPEC_CUTOUT = 1u, SPEC_GLASS = 2u, SPEC_FULL_MARCH = 4u, SPEC_EMITTER = 8u,
+                   SPEC_WATER_REPAIR = 16u;
+
+struct Spec {
+    uint32_t bits;
+    float leaf_fill;
+};
+
+struct Pools {
+    const uint4 *__restrict__ inners;
+    const uint2 *__restrict__ leaves;
+    const uint32_t *__restrict__ bricks;
+    const vx_gpu_chunk *__restrict__ chunks;
+    const uint32_t *__restrict__ grid;
+    uint32_t n_inners, n_leaves, n_bricks, n_chunks;
+};
+
+// Frame fields read by the passes, packed for kernel arguments.
+struct FrameK {
+    float cam[3], fwd[3], right[3], up[3];
+    float tan_half_fov, aspect, far_, daylight;
+    float sun[3], shadow_dist, sea_level;
+    float jitter[2];
+    uint32_t res[2], tiles[2];
+    uint32_t chunk_count, camera_chunk, flags, max_pairs, frame_index;
+    int32_t grid_min[3];
+    uint32_t grid_dim[3];
+};
+
+__device__ __forceinline__ uint4 ld_inner(const Pools &p, uint32_t i) {
+    return i < p.n_inners ? p.inners[i] : make_uint4(0, 0, 0, 0);
+}
+__device__ __forceinline__ uint2 ld_leaf(const Pools &p, uint32_t i) {
+    return i < p.n_leaves ? p.leaves[i] : make_uint2(0, 0);
+}
+__device__ __forceinline__ uint32_t ld_brick(const Pools &p, uint32_t i) {
+    return i < p.n_bricks ? p.bricks[i] : 0u;
+}
+
+// The chunk fields the march reads (GpuChunk, 96 B) as four 16-byte loads.
+struct ChunkK {
+    float bmin[3], vs, bmax[3], origin[3], fade;
+    uint32_t root[4];
+    uint32_t attr_base, attr_flags;
+    uint32_t dry[2];
+};
+
+__device__ __forceinline__ ChunkK ld_chunk(const Pools &p, uint32_t ci) {
+    ChunkK c;
+    if (ci >= p.n_chunks) ci = 0;  // callers guard; keeps the read inside the buffer
+    const uint4 *q = reinterpret_cast<const uint4 *>(p.chunks + ci);
+    uint4 a = q[0], b = q[1], o = q[2], r = q[3], at = q[4], d = q[5];
+    c.bmin[0] = __uint_as_float(a.x); c.bmin[1] = __uint_as_float(a.y); c.bmin[2] = __uint_as_float(a.z);
+    c.vs = __uint_as_float(a.w);
+    c.bmax[0] = __uint_as_float(b.x); c.bmax[1] = __uint_as_float(b.y); c.bmax[2] = __uint_as_float(b.z);
+    c.origin[0] = __uint_as_float(o.x); c.origin[1] = __uint_as_float(o.y); c.origin[2] = __uint_as_float(o.z);
+    c.fade = __uint_as_float(o.w);
+    c.root[0] = r.x; c.root[1] = r.y; c.root[2] = r.z; c.root[3] = r.w;
+    c.attr_base = at.x; c.attr_flags = at.y;
+    c.dry[0] = d.x; c.dry[1] = d.y;
+    return c;
+}
+
+// mask_has / mask_below / cell_bit (common.wgsl:299-318) on a u64 mask.
+__device__ __forceinline__ bool mask_has(uint64_t m, uint32_t bit) { return (m >> bit) & 1ull; }
+__device__ __forceinline__ uint32_t mask_below(uint64_t m, uint32_t bit) {
+    return (uint32_t)__popcll(m & ((1ull << bit) - 1ull));  // bit < 64 on every call
+}
+__device__ __forceinline__ uint32_t cell_bit(uint32_t x, uint32_t y, uint32_t z) { return x | (y << 2) | (z << 4); }
+__device__ __forceinline__ uint64_t mk64(uint32_t lo, uint32_t hi) { return (uint64_t)lo | ((uint64_t)hi << 32); }
+
+// leaf_block (common.wgsl:320-334)
+__device__ __forceinline__ uint32_t leaf_block(const Pools &p, uint32_t entry, uint32_t vx, uint32_t vy, uint32_t vz) {
+    if (entry & UNIFORM_BIT) return entry & 0xFFFFu;
+    uint32_t hdr = ld_brick(p, entry);
+    uint32_t n = hdr & 0xFFu;
+    uint32_t bits = (hdr >> 8) & 0xFFu;
+    uint32_t pal_words = (n + 1u) / 2u;
+    uint32_t vb = cell_bit(vx & 3u, vy & 3u, vz & 3u);
+    uint32_t bitpos = vb * bits;
+    uint32_t w = ld_brick(p, entry + 1u + pal_words + (bitpos >> 5));
+    uint32_t idx = (w >> (bitpos & 31u)) & ((1u << (bits & 31u)) - 1u);
+    uint32_t pw = ld_brick(p, entry + 1u + idx / 2u);
+    return (pw >> ((idx & 1u) * 16u)) & 0xFFFFu;
+}
+
+// leaf_attr + block_at (common.wgsl:336-356)
+__device__ __forceinline__ uint32_t block_at(const Pools &p, const ChunkK &c, uint32_t vx, uint32_t vy, uint32_t vz) {
+    if (c.attr_flags & ATTR_UNIFORM) return c.attr_base;
+    uint64_t root_mask = mk64(c.root[0], c.root[1]);
+    uint32_t l1b = cell_bit(vx >> 4, vy >> 4, vz >> 4);
+    uint4 l1;
+    if (c.root[2] & FULL_BIT) l1 = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, FULL_BIT, l1b * 64u);
+    else l1 = ld_inner(p, (c.root[2] & PTR_MASK) + mask_below(root_mask, l1b));
+    uint32_t lfb = cell_bit((vx >> 2) & 3u, (vy >> 2) & 3u, (vz >> 2) & 3u);
+    uint32_t entry = ld_brick(p, c.attr_base + (l1.w & 0x0FFFFFFFu) + mask_below(mk64(l1.x, l1.y), lfb));
+    return leaf_block(p, entry, vx, vy, vz);
+}
+
+// voxel_occupied (common.wgsl:912-940)
+__device__ inline bool voxel_occupied(const Pools &p, const ChunkK &c, int32_t x, int32_t y, int32_t z) {
+    if (x < 0 || y < 0 || z < 0 || x > 63 || y > 63 || z > 63) return false;
+    uint32_t ux = (uint32_t)x, uy = (uint32_t)y, uz = (uint32_t)z;
+    uint64_t root_mask = mk64(c.root[0], c.root[1]);
+    uint32_t l1b = cell_bit(ux >> 4, uy >> 4, uz >> 4);
+    if (!mask_has(root_mask, l1b)) return false;
+    if (c.root[2] & FULL_BIT) return true;
+    uint4 l1 = ld_inner(p, (c.root[2] & PTR_MASK) + mask_below(root_mask, l1b));
+    if (l1.z & FULL_BIT) return true;
+    uint64_t l1m = mk64(l1.x, l1.y);
+    uint32_t lfb = cell_bit((ux >> 2) & 3u, (uy >> 2) & 3u, (uz >> 2) & 3u);
+    if (!mask_has(l1m, lfb)) return false;
+    uint2 lf = ld_leaf(p, (l1.z & PTR_MASK) + mask_below(l1m, lfb));
+    return mask_has(mk64(lf.x, lf.y), cell_bit(ux & 3u, uy & 3u, uz & 3u));
+}
+
+__device__ __forceinline__ uint32_t grid_index(const FrameK &f, int32_t cx, int32_t cy, int32_t cz, bool &ok) {
+    int32_t rx = cx - f.grid_min[0], ry = cy - f.grid_min[1], rz = cz - f.grid_min[2];
+    ok = rx >= 0 && ry >= 0 && rz >= 0 && rx < (int32_t)f.grid_dim[0] && ry < (int32_t)f.grid_dim[1] &&
+         rz < (int32_t)f.grid_dim[2];
+    return (uint32_t)rx + (uint32_t)ry * f.grid_dim[0] + (uint32_t)rz * f.grid_dim[0] * f.grid_dim[1];
+}
+
+__device__ __forceinline__ uint32_t ld_grid(const Pools &p, const FrameK &f, uint32_t idx) {
+    return idx < f.grid_dim[0] * f.grid_dim[1] * f.grid_dim[2] && idx < VX_GRID_LEN ? p.grid[idx] : NO_CHUNK;
+}
+
+// world_block_at (common.wgsl:266-277)
+__device__ inline uint32_t world_block_at(const Pools &p, const FrameK &f, float px, float py, float pz) {
+    bool ok;
+    uint32_t gi = grid_index(f, (int32_t)floorf(px / 64.0f), (int32_t)floorf(py / 64.0f), (int32_t)floorf(pz / 64.0f), ok);
+    if (!ok) return 0u;
+    uint32_t ci = ld_grid(p, f, gi);
+    if (ci == NO_CHUNK || ci >= p.n_chunks) return 0u;
+    ChunkK c = ld_chunk(p, ci);
+    int32_t vx = (int32_t)floorf((px - c.origin[0]) / c.vs);
+    int32_t vy = (int32_t)floorf((py - c.origin[1]) / c.vs);
+    int32_t vz = (int32_t)floorf((pz - c.origin[2]) / c.vs);
+    if (vx < 0 || vy < 0 || vz < 0 || vx >= 64 || vy >= 64 || vz >= 64) return 0u;
+    if (!voxel_occupied(p, c, vx, vy, vz)) return 0u;
+    return block_at(p, c, (uint32_t)vx, (uint32_t)vy, (uint32_t)vz);
+}
+
+// ray_dir (common.wgsl:358-368): normalize = v / sqrt(dot(v, v)).
+__device__ __forceinline__ void ray_dir(const FrameK &f, float px, float py, float &dx, float &dy, float &dz) {
+    float ndc_x = ((px + f.jitter[0]) / (float)f.res[0]) * 2.0f - 1.0f;
+    float ndc_y = 1.0f - ((py + f.jitter[1]) / (float)f.res[1]) * 2.0f;
+    float sx = ndc_x * f.tan_half_fov * f.aspect;
+    float sy = ndc_y * f.tan_half_fov;
+    float x = f.fwd[0] + f.right[0] * sx + f.up[0] * sy;
+    float y = f.fwd[1] + f.right[1] * sx + f.up[1] * sy;
+    float z = f.fwd[2] + f.right[2] * sx + f.up[2] * sy;
+    float len = sqrtf(x * x + y * y + z * z);
+    x = x / len;
+    y = y / len;
+    z = z / len;
+    auto guard = [](float d) {
+        if (fabsf(d) < 1e-6f) {
+            float s = d + 1e-9f;
+            d = 1e-6f * (s > 0.0f ? 1.0f : (s < 0.0f ? -1.0f : 0.0f));
+        }
+        return d;
+    };
+    dx = guard(x);
+    dy = guard(y);
+    dz = guard(z);
+}
+
+// ray_skips with dither (common.wgsl:370-385)
+__device__ __forceinline__ bool ray_skips(const FrameK &f, float fade, uint32_t px, uint32_t py) {
+    if (fade >= 1.0f) return false;
+    float p0 = (float)px + 5.588238f * (float)f.frame_index;
+    float p1 = (float)py;
+    float a = p0 * 0.06711056f + p1 * 0.00583715f;
+    a = a - floorf(a);
+    float d = 52.9829189f * a;
+    d = d - floorf(d);
+    if (fade >= 0.0f) return d >= fade;
+    return d < -fade;
+}
+
+// hash_u32 (common.wgsl:1422-1430)
+__device__ __forceinline__ uint32_t hash_u32(uint32_t h) {
+    h ^= h >> 16;
+    h *= 0x7feb352du;
+    h ^= h >> 15;
+    h *= 0x846ca68bu;
+    h ^= h >> 16;
+    return h;
+}
+
+// leaf_solid (common.wgsl:514-518)
+__device__ __forceinline__ bool leaf_solid(float fill, int32_t wx, int32_t wy, int32_t wz, uint32_t mx, uint32_t my,
+                                           uint32_t mz) {
+    uint32_t qx = (uint32_t)wx * 4u + mx, qy = (uint32_t)wy * 4u + my, qz = (uint32_t)wz * 4u + mz;
+    uint32_t h = hash_u32((qx * 73856093u) ^ (qy * 19349663u) ^ (qz * 83492791u));
+    return (float)(h >> 8) * (1.0f / 16777216.0f) < fill;
+}
+
+__device__ __forceinline__ int32_t clampi(int32_t v, int32_t lo, int32_t hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// micro_entry (common.wgsl:520-528); returns the three 2-bit coordinates packed x | y<<2 | z<<4.
+__device__ __forceinline__ uint32_t micro_entry(uint32_t ux, uint32_t uy, uint32_t uz, const float ro[3],
+                                                const float rd[3], float t, uint32_t axis, const bool pd[3],
+                                                bool snap) {
+    int32_t m[3];
+    uint32_t u[3] = {ux, uy, uz};
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        float q = (ro[i] + rd[i] * t - (float)u[i]) * 4.0f;
+        m[i] = clampi((int32_t)floorf(q), 0, 3);
+    }
+    if (snap) {
+        int32_t s = pd[axis] ? 0 : 3;
+        if (axis == 0) m[0] = s;
+        else if (axis == 1) m[1] = s;
+        else m[2] = s;
+    }
+    return (uint32_t)m[0] | ((uint32_t)m[1] << 2) | ((uint32_t)m[2] << 4);
+}
+
+struct MicroHit {
+    bool hit;
+    float t;
+    uint32_t normal;
+    uint32_t micro;  // packed x | y<<2 | z<<4
+};
+
+// cutout_march (common.wgsl:537-577)
+__device__ inline MicroHit cutout_march(float fill, int32_t wx, int32_t wy, int32_t wz, uint32_t ux, uint32_t uy,
+                                        uint32_t uz, const float ro[3], const float rd[3], const float inv[3],
+                                        float t_in, uint32_t axis_in, const bool pd[3], bool snap) {
+    MicroHit r{false, t_in, 0u, 0u};
+    const float base[3] = {(float)ux, (float)uy, (float)uz};
+    // inv is 1/rd computed exactly as cutout_march's own `let inv = 1.0 / rd`.
+    uint32_t me = micro_entry(ux, uy, uz, ro, rd, t_in, axis_in, pd, snap);
+    int32_t m[3] = {(int32_t)(me & 3u), (int32_t)((me >> 2) & 3u), (int32_t)((me >> 4) & 3u)};
+    float t = t_in;
+    uint32_t axis = axis_in;
+    for (uint32_t i = 0; i < 12u; i++) {
+        if (m[0] < 0 || m[1] < 0 || m[2] < 0 || m[0] > 3 || m[1] > 3 || m[2] > 3) return r;
+        if (leaf_solid(fill, wx, wy, wz, (uint32_t)m[0], (uint32_t)m[1], (uint32_t)m[2])) {
+            r.hit = true;
+            r.t = t;
+            r.normal = axis * 2u + (pd[axis] ? 0u : 1u);
+            r.micro = (uint32_t)m[0] | ((uint32_t)m[1] << 2) | ((uint32_t)m[2] << 4);
+            return r;
+        }
+        float tn[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            float bound = (float)(pd[k] ? m[k] + 1 : m[k]);
+            tn[k] = (base[k] + bound * 0.25f - ro[k]) * inv[k];
+        }
+        if (tn[0] <= tn[1] && tn[0] <= tn[2]) { t = tn[0]; axis = 0; }
+        else if (tn[1] <= tn[2]) { t = tn[1]; axis = 1; }
+        else { t = tn[2]; axis = 2; }
+        int32_t step = pd[axis] ? 1 : -1;
+        if (axis == 0) m[0] += step;
+        else if (axis == 1) m[1] += step;
+        else m[2] += step;
+    }
+    return r;
+}
+
+struct Hit {
+    bool hit;
+    float t;
+    uint32_t normal;
+    uint32_t voxel;  // x | y<<6 | z<<12
+    uint32_t micro;  // x | y<<2 | z<<4
+    uint32_t iters;
+};
+
+// march_chunk (common.wgsl:579-910). Foliage tufts are not handled here: the
+// context refuses SPEC_FOLIAGE, so foliage_aware is false on every call.
+__device__ inline Hit march_chunk(const Pools &p, const FrameK &f, const Spec &s, const ChunkK &c, const float ro_w[3],
+                                  const float rd[3], float t_limit, bool skip_water, bool skip_glass, bool sun_ray) {
+    Hit h{false, 0.0f, 0u, 0u, 0u, 0u};
+    const float vs = c.vs;
+    float ro[3], inv[3], bmin[3], bmax[3], tmin3[3], tmax3[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        ro[i] = (ro_w[i] - c.origin[i]) / vs;
+        inv[i] = 1.0f / rd[i];
+        bmin[i] = (c.bmin[i] - c.origin[i]) / vs;
+        bmax[i] = (c.bmax[i] - c.origin[i]) / vs;
+        float t0 = (bmin[i] - ro[i]) * inv[i];
+        float t1 = (bmax[i] - ro[i]) * inv[i];
+        tmin3[i] = fminf(t0, t1);
+        tmax3[i] = fmaxf(t0, t1);
+    }
+    const float t_enter = fmaxf(fmaxf(tmin3[0], tmin3[1]), tmin3[2]);
+    const float t_exit = fminf(fminf(tmax3[0], tmax3[1]), tmax3[2]);
+    const float t_end = fminf(t_exit, t_limit / vs);
+    float t = fmaxf(t_enter, 0.0f);
+    if (t >= t_end) return h;
+    uint32_t axis;
+    if (tmin3[0] >= tmin3[1] && tmin3[0] >= tmin3[2]) axis = 0;
+    else if (tmin3[1] >= tmin3[2]) axis = 1;
+    else axis = 2;
+    const bool pd[3] = {rd[0] > 0.0f, rd[1] > 0.0f, rd[2] > 0.0f};
+    int32_t cell[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) cell[i] = (int32_t)floorf(ro[i] + rd[i] * t);
+    if (t_enter > 0.0f) {
+        int32_t v = pd[axis] ? (int32_t)floorf(bmin[axis]) : (int32_t)ceilf(bmax[axis]) - 1;
+        if (axis == 0) cell[0] = v;
+        else if (axis == 1) cell[1] = v;
+        else cell[2] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) cell[i] = clampi(cell[i], 0, 63);
+    const uint64_t root_mask = mk64(c.root[0], c.root[1]);
+    const uint32_t root_ptr = c.root[2] & PTR_MASK;
+    const bool attr_uniform = (c.attr_flags & ATTR_UNIFORM) != 0u;
+
+    bool water_aware = skip_water && (c.attr_flags & ATTR_HAS_WATER) != 0u;
+    if (water_aware && attr_uniform) {
+        if (c.attr_base == WATER_ID) return h;
+        water_aware = false;
+    }
+    bool cutout_aware = false;
+    if (s.bits & SPEC_CUTOUT) {
+        cutout_aware = (c.attr_flags & ATTR_HAS_CUTOUT) != 0u && vs == 1.0f;
+        if (cutout_aware && attr_uniform && c.attr_base != LEAVES_ID && c.attr_base != PINE_LEAVES_ID)
+            cutout_aware = false;
+    }
+    bool glass_aware = false;
+    if (s.bits & SPEC_GLASS) {
+        glass_aware = skip_glass && (c.attr_flags & ATTR_HAS_GLASS) != 0u;
+        if (glass_aware && attr_uniform) {
+            if (c.attr_base == GLASS_ID) return h;
+            glass_aware = false;
+        }
+    }
+    bool noncaster_aware = false;
+    if (s.bits & SPEC_EMITTER) noncaster_aware = sun_ray && (c.attr_flags & ATTR_HAS_EMITTER) != 0u;
+    if (noncaster_aware && attr_uniform && c.attr_base == GLOWSTONE_ID) return h;
+    const bool water_interface =
+        (s.bits & SPEC_WATER_REPAIR) && !skip_water && (c.attr_flags & ATTR_HAS_WATER) != 0u;
+    const bool attr_needed = (noncaster_aware && !attr_uniform) || (water_interface && !attr_uniform) ||
+                             water_aware || glass_aware || (cutout_aware && !attr_uniform);
+    const uint64_t trav_mask = water_aware ? mk64(c.dry[0], c.dry[1]) : root_mask;
+    const bool root_full = (c.root[2] & FULL_BIT) != 0u;
+    if (root_full && !(s.bits & SPEC_FULL_MARCH) && !noncaster_aware && !water_interface) {
+        h.hit = true;
+        h.t = t * vs;
+        h.normal = axis * 2u + (pd[axis] ? 0u : 1u);
+        h.voxel = (uint32_t)cell[0] | ((uint32_t)cell[1] << 6) | ((uint32_t)cell[2] << 12);
+        h.micro = micro_entry((uint32_t)cell[0], (uint32_t)cell[1], (uint32_t)cell[2], ro, rd, t, axis, pd, t > 0.0f);
+        return h;
+    }
+    if (trav_mask == 0ull) return h;
+
+    uint32_t l1_cached = 0xFFFFFFFFu, leaf_cached = 0xFFFFFFFFu;
+    uint4 l1 = make_uint4(0, 0, 0, 0);
+    uint64_t leaf = 0;
+    uint32_t attr = 0;
+    uint32_t iters = 0;
+    for (;;) {
+        iters++;
+        if (iters > MAX_ITERS) break;
+        if ((uint32_t)cell[0] > 63u || (uint32_t)cell[1] > 63u || (uint32_t)cell[2] > 63u) break;
+        const uint32_t ux = (uint32_t)cell[0], uy = (uint32_t)cell[1], uz = (uint32_t)cell[2];
+        int32_t skip = 16;
+        bool solid = false;
+        const uint32_t l1b = cell_bit(ux >> 4, uy >> 4, uz >> 4);
+        if (mask_has(trav_mask, l1b)) {
+            if (l1b != l1_cached) {
+                l1_cached = l1b;
+                if (root_full) l1 = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, FULL_BIT, (l1b * 64u) | 0xF0000000u);
+                else l1 = ld_inner(p, root_ptr + mask_below(root_mask, l1b));
+                leaf_cached = 0xFFFFFFFFu;
+            }
+            bool line_skip = false;
+            if (water_aware) {
+                uint32_t wl = l1.w >> 28;
+                float ly = (float)(uy & 15u);
+                if (wl < 15u && ly > (float)wl) {
+                    line_skip = rd[1] >= 0.0f;
+                    if (!line_skip) {
+                        float hh = fmaxf(fabsf(rd[0]), fabsf(rd[2]));
+                        if (hh > 1e-6f && (ly - (float)wl) > 16.0f * (fabsf(rd[1]) / hh)) line_skip = true;
+                    }
+                }
+            }
+            if (line_skip) {
+                skip = 16;
+            } else if (l1.z & FULL_BIT) {
+                solid = true;
+                if (attr_needed) {
+                    uint32_t lfb = cell_bit((ux >> 2) & 3u, (uy >> 2) & 3u, (uz >> 2) & 3u);
+                    if (lfb != leaf_cached) {
+                        leaf_cached = lfb;
+                        attr = ld_brick(p, c.attr_base + (l1.w & 0x0FFFFFFFu) + lfb);
+                    }
+                }
+            } else {
+                const uint64_t l1m = mk64(l1.x, l1.y);
+                const uint32_t lfb = cell_bit((ux >> 2) & 3u, (uy >> 2) & 3u, (uz >> 2) & 3u);
+                if (mask_has(l1m, lfb)) {
+                    if (lfb != leaf_cached) {
+                        leaf_cached = lfb;
+                        uint32_t li = mask_below(l1m, lfb);
+                        uint2 lf = ld_leaf(p, (l1.z & PTR_MASK) + li);
+                        leaf = mk64(lf.x, lf.y);
+                        if (attr_needed) attr = ld_brick(p, c.attr_base + (l1.w & 0x0FFFFFFFu) + li);
+                    }
+                    const uint32_t vb = cell_bit(ux & 3u, uy & 3u, uz & 3u);
+                    if (mask_has(leaf, vb)) {
+                        solid = true;
+                    } else {
+                        // 2x2x2 window starting at the even corner: bits (vb & 42) + {0,1,4,5,16,17,20,21}.
+                        uint32_t lo = (uint32_t)(leaf >> (vb & 42u));
+                        skip = ((lo & 0x00330033u) == 0u) ? 2 : 1;
+                    }
+                } else {
+                    skip = 4;
+                }
+            }
+        } else {
+            uint32_t rlo = (uint32_t)(trav_mask >> (l1b & 42u));
+            skip = ((rlo & 0x00330033u) == 0u) ? 32 : 16;
+        }
+        if (solid && noncaster_aware) {
+            uint32_t id = attr_uniform ? c.attr_base : leaf_block(p, attr, ux, uy, uz);
+            if (id == GLOWSTONE_ID) { solid = false; skip = 1; }
+        }
+        if (solid && water_interface) {
+            uint32_t id = attr_uniform ? c.attr_base : leaf_block(p, attr, ux, uy, uz);
+            if (id == WATER_ID) {
+                // normal_of(axis*2 + (pos ? 0 : 1)): unit vector along axis, -1 if the ray goes +.
+                float o[3] = {0.0f, 0.0f, 0.0f};
+                float sgn = pd[axis] ? -1.0f : 1.0f;
+                if (axis == 0) o[0] = sgn;
+                else if (axis == 1) o[1] = sgn;
+                else o[2] = sgn;
+                float tw = t * vs;
+                float off = 0.002f * vs;
+                float b0 = ro_w[0] + rd[0] * tw, b1 = ro_w[1] + rd[1] * tw, b2 = ro_w[2] + rd[2] * tw;
+                if (world_block_at(p, f, b0 + o[0] * off, b1 + o[1] * off, b2 + o[2] * off) == WATER_ID) {
+                    solid = false;
+                    skip = 1;
+                }
+            }
+        }
+        if (solid && water_aware) {
+            if (leaf_block(p, attr, ux, uy, uz) == WATER_ID) {
+                solid = false;
+                skip = (attr & UNIFORM_BIT) ? 4 : 1;
+            }
+        }
+        if (solid && glass_aware) {
+            if (leaf_block(p, attr, ux, uy, uz) == GLASS_ID) {
+                solid = false;
+                skip = (attr & UNIFORM_BIT) ? 4 : 1;
+            }
+        }
+        if (solid && cutout_aware) {
+            uint32_t bid = attr_uniform ? c.attr_base : leaf_block(p, attr, ux, uy, uz);
+            if (bid == LEAVES_ID || bid == PINE_LEAVES_ID) {
+                int32_t wx = (int32_t)c.origin[0] + (int32_t)ux, wy = (int32_t)c.origin[1] + (int32_t)uy,
+                        wz = (int32_t)c.origin[2] + (int32_t)uz;
+                MicroHit q = cutout_march(s.leaf_fill, wx, wy, wz, ux, uy, uz, ro, rd, inv, t, axis, pd, t > 0.0f);
+                if (q.hit) {
+                    h.hit = true;
+                    h.t = q.t * vs;
+                    h.normal = q.normal;
+                    h.voxel = ux | (uy << 6) | (uz << 12);
+                    h.micro = q.micro;
+                    h.iters = iters;
+                    return h;
+                }
+                solid = false;
+                skip = 1;
+            }
+        }
+        if (solid) {
+            h.hit = true;
+            h.t = t * vs;
+            h.normal = axis * 2u + (pd[axis] ? 0u : 1u);
+            h.voxel = ux | (uy << 6) | (uz << 12);
+            h.micro = micro_entry(ux, uy, uz, ro, rd, t, axis, pd, t > 0.0f);
+            h.iters = iters;
+            return h;
+        }
+        int32_t bmin_i[3], bmax_i[3];
+        float tn[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            bmin_i[i] = cell[i] - (cell[i] & (skip - 1));
+            bmax_i[i] = bmin_i[i] + skip;
+            float bound = pd[i] ? (float)bmax_i[i] : (float)bmin_i[i];
+            tn[i] = (bound - ro[i]) * inv[i];
+        }
+        if (tn[0] <= tn[1] && tn[0] <= tn[2]) { t = tn[0]; axis = 0; }
+        else if (tn[1] <= tn[2]) { t = tn[1]; axis = 1; }
+        else { t = tn[2]; axis = 2; }
+        if (t >= t_end) break;
+#pragma unroll
+        for (int i = 0; i < 3; i++) cell[i] = (int32_t)floorf(ro[i] + rd[i] * t);
+        int32_t forced = pd[axis] ? bmax_i[axis] : bmin_i[axis] - 1;
+        if (axis == 0) cell[0] = forced;
+        else if (axis == 1) cell[1] = forced;
+        else cell[2] = forced;
+    }
+    h.iters = iters;
+    return h;
+}
+
+// depth_key / make_key (common.wgsl:1320-1330)
+__device__ __forceinline__ uint32_t depth_key(float far_, float t) {
+    float x = 1.0f - t / far_;
+    x = fminf(fmaxf(x, 0.0f), 1.0f);
+    return (uint32_t)(x * 16777215.0f);
+}
+
+__device__ __forceinline__ uint64_t make_key(float far_, const Hit &h, uint32_t ci) {
+    uint32_t d = depth_key(far_, h.t);
+    uint32_t vx = h.voxel & 63u, vy = (h.voxel >> 6) & 63u, vz = (h.voxel >> 12) & 63u;
+    uint32_t mx = h.micro & 3u, my = (h.micro >> 2) & 3u, mz = (h.micro >> 4) & 3u;
+    uint32_t b = ((vx | (mx << 6)) << 16) | ((vy | (my << 6)) << 8) | (vz | (mz << 6));
+    return ((uint64_t)d << 40) | ((uint64_t)h.normal << 37) | ((uint64_t)ci << 24) | (uint64_t)b;
+}
+
+// shadow_ray_t (common.wgsl:1156-1195): chunk-grid DDA over at most 24 cells.
+__device__ inline float shadow_ray_t(const Pools &p, const FrameK &f, const Spec &s, const float ro[3],
+                                     const float rd[3], float max_dist) {
+    float inv[3];
+    bool pd[3];
+    int32_t cell[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        inv[i] = 1.0f / rd[i];
+        pd[i] = rd[i] > 0.0f;
+        cell[i] = (int32_t)floorf(ro[i] / 64.0f);
+    }
+    uint32_t last = NO_CHUNK;
+    for (uint32_t i = 0; i < 24u; i++) {
+        bool ok;
+        uint32_t gi = grid_index(f, cell[0], cell[1], cell[2], ok);
+        if (!ok) return -1.0f;
+        uint32_t ci = ld_grid(p, f, gi);
+        if (ci != NO_CHUNK && ci != last) {
+            last = ci;
+            if (ci < p.n_chunks) {
+                ChunkK c = ld_chunk(p, ci);
+                Hit m = march_chunk(p, f, s, c, ro, rd, max_dist, true, true, true);
+                if (m.hit) return m.t;
+            }
+        }
+        float tn[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            float bound = (float)(pd[k] ? cell[k] + 1 : cell[k]) * 64.0f;
+            tn[k] = (bound - ro[k]) * inv[k];
+        }
+        uint32_t axis = 0;
+        float t;
+        if (tn[0] <= tn[1] && tn[0] <= tn[2]) t = tn[0];
+        else if (tn[1] <= tn[2]) { t = tn[1]; axis = 1; }
+        else { t = tn[2]; axis = 2; }
+        if (t >= max_dist) return -1.0f;
+        int32_t step = pd[axis] ? 1 : -1;
+        if (axis == 0) cell[0] += step;
+        else if (axis == 1) cell[1] += step;
+        else cell[2] += step;
+    }
+    return -1.0f;
+}
+
+// hit_t (common.wgsl:1300-1318)
+__device__ __forceinline__ float hit_t(const FrameK &f, const ChunkK &c, uint32_t vx, uint32_t vy, uint32_t vz,
+                                       uint32_t mx, uint32_t my, uint32_t mz, uint32_t nid, const float rd[3]) {
+    float bx = c.origin[0] + (float)vx * c.vs, by = c.origin[1] + (float)vy * c.vs, bz = c.origin[2] + (float)vz * c.vs;
+    if (nid >= NORMAL_CROSS_A) {
+        if (nid == NORMAL_CROSS_A) return ((bx - bz) - (f.cam[0] - f.cam[2])) / (rd[0] - rd[2]);
+        return ((bx + bz + c.vs) - (f.cam[0] + f.cam[2])) / (rd[0] + rd[2]);
+    }
+    uint32_t axis = nid >> 1;
+    float ms = c.vs / 4.0f;
+    uint32_t add = (nid & 1u) ? 1u : 0u;
+    float plane;
+    if (axis == 0) plane = bx + (float)(mx + add) * ms;
+    else if (axis == 1) plane = by + (float)(my + add) * ms;
+    else plane = bz + (float)(mz + add) * ms;
+    return (plane - f.cam[axis]) / rd[axis];
+}
+
+// f32 -> f16 bits, IEEE round-to-nearest-even (the R16F store of primary_shadow).
+__device__ __forceinline__ uint16_t f32_to_f16_bits(float x) {
+    _Float16 h = (_Float16)x;
+    return __builtin_bit_cast(uint16_t, h);
+}
+
+// normal_of (common.wgsl:387-398)
+__device__ __forceinline__ void normal_of(uint32_t id, float n[3]) {
+    if (id >= NORMAL_CROSS_A) {
+        float s = (id == NORMAL_CROSS_B) ? 1.0f : -1.0f;
+        n[0] = 0.70710678f; n[1] = 0.0f; n[2] = s * 0.70710678f;
+        return;
+    }
+    uint32_t axis = id >> 1;
+    float s = (id & 1u) ? 1.0f : -1.0f;
+    n[0] = axis == 0 ? s : 0.0f;
+    n[1] = axis == 1 ? s : 0.0f;
+    n[2] = axis == 2 ? s : 0.0f;
+}
+
+// primary_shadow for one pixel (shadow.wgsl:8-48). Returns the R16F bits and, through
+// traced, whether a sun ray was cast (the pixel's shadow_guide was valid).
+__device__ inline uint16_t shadow_pixel(const Pools &p, const FrameK &f, const Spec &s, uint64_t key, uint32_t px,
+                                        uint32_t py, bool &traced) {
+    traced = false;
+    float distance = 0.0f;
+    const bool active = (f.flags & VX_FLAG_SHADOWS) && !(f.flags & VX_FLAG_HEATMAP) && f.daylight > 0.01f;
+    if (active && key != 0ull) {
+        uint32_t ci = (uint32_t)(key >> 24) & 0x1FFFu;
+        uint32_t low = (uint32_t)(key & 0xFFFFFFull);
+        uint32_t vx = (low >> 16) & 63u, vy = (low >> 8) & 63u, vz = low & 63u;
+        if (ci < p.n_chunks) {
+            ChunkK c = ld_chunk(p, ci);
+            uint32_t id = block_at(p, c, vx, vy, vz);
+            if (!(id == WATER_ID || ((s.bits & SPEC_GLASS) && id == GLASS_ID))) {
+                uint32_t nid = (uint32_t)(key >> 37) & 7u;
+                uint32_t mx = (low >> 22) & 3u, my = (low >> 14) & 3u, mz = (low >> 6) & 3u;
+                float rd[3], n[3];
+                ray_dir(f, (float)px + 0.5f, (float)py + 0.5f, rd[0], rd[1], rd[2]);
+                normal_of(nid, n);
+                if (nid >= NORMAL_CROSS_A && (n[0] * rd[0] + n[1] * rd[1] + n[2] * rd[2]) > 0.0f) {
+                    n[0] = -n[0]; n[1] = -n[1]; n[2] = -n[2];
+                }
+                if ((n[0] * f.sun[0] + n[1] * f.sun[1] + n[2] * f.sun[2]) > 0.0f) {
+                    float t = hit_t(f, c, vx, vy, vz, mx, my, mz, nid, rd);
+                    if (t >= 0.0f) {
+                        float e = 0.02f * c.vs;
+                        float o[3];
+#pragma unroll
+                        for (int i = 0; i < 3; i++) o[i] = (f.cam[i] + rd[i] * t) + n[i] * e;
+                        traced = true;
+                        float blocker = shadow_ray_t(p, f, s, o, f.sun, f.shadow_dist);
+                        if (blocker >= 0.0f) distance = fminf(fmaxf(blocker, 0.001f), 65504.0f);
+                    }
+                }
+            }
+        }
+    }
+    return f32_to_f16_bits(distance);
+}
+
+// box_outside / box_distance (tile_select.wgsl:1-9)
+__device__ __forceinline__ bool box_outside(const float bmin[3], const float bmax[3], const float n[3],
+                                            const float cam[3]) {
+    float q0 = (n[0] > 0.0f ? bmax[0] : bmin[0]) - cam[0];
+    float q1 = (n[1] > 0.0f ? bmax[1] : bmin[1]) - cam[1];
+    float q2 = (n[2] > 0.0f ? bmax[2] : bmin[2]) - cam[2];
+    return (n[0] * q0 + n[1] * q1 + n[2] * q2) < 0.0f;
+}
+
+__device__ __forceinline__ float box_distance(const float p[3], const float bmin[3], const float bmax[3]) {
+    float q[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) q[i] = fmaxf(fmaxf(bmin[i] - p[i], p[i] - bmax[i]), 0.0f);
+    return sqrtf(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+}
+
+// The per-tile culling set-up of tile_select (tile_select.wgsl:16-57).
+struct TileCull {
+    float n[4][3];
+    float reach;
+};
+
+__device__ inline TileCull tile_cull_setup(const FrameK &f, uint32_t gx, uint32_t gy) {
+    TileCull tc;
+    float x0 = (float)(gx * 8u), y0 = (float)(gy * 8u);
+    float x1 = x0 + 8.0f, y1 = y0 + 8.0f;
+    float d[4][3], ce[3];
+    ray_dir(f, x0, y0, d[0][0], d[0][1], d[0][2]);
+    ray_dir(f, x1, y0, d[1][0], d[1][1], d[1][2]);
+    ray_dir(f, x1, y1, d[2][0], d[2][1], d[2][2]);
+    ray_dir(f, x0, y1, d[3][0], d[3][1], d[3][2]);
+    ray_dir(f, (x0 + x1) * 0.5f, (y0 + y1) * 0.5f, ce[0], ce[1], ce[2]);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const float *a = d[k];
+        const float *b = d[(k + 1) & 3];
+        float cx = a[1] * b[2] - a[2] * b[1];
+        float cy = a[2] * b[0] - a[0] * b[2];
+        float cz = a[0] * b[1] - a[1] * b[0];
+        bool keep = (cx * ce[0] + cy * ce[1] + cz * ce[2]) >= 0.0f;
+        tc.n[k][0] = keep ? cx : -cx;
+        tc.n[k][1] = keep ? cy : -cy;
+        tc.n[k][2] = keep ? cz : -cz;
+    }
+    tc.reach = f.far_;
+    const uint32_t uw = VX_FLAG_UNDERWATER | VX_FLAG_WATER_FAR;
+    if ((f.flags & uw) == uw) {
+        float up = fmaxf(fmaxf(d[0][1], d[1][1]), fmaxf(d[2][1], d[3][1]));
+        float submerged = fmaxf(f.sea_level - f.cam[1], 0.0f);
+        float escape = 1e30f;
+        if (up > 0.0f) escape = submerged / up;
+        if (escape >= 128.0f) {
+            tc.reach = fminf(f.far_, 128.0f);
+            float dep_far = submerged - up * 128.0f;
+            if ((f.flags & VX_FLAG_WATER_DARK) && fminf(submerged, dep_far) >= 15.0f) tc.reach = fminf(tc.reach, 64.0f);
+        }
+    }
+    return tc;
+}
+
+// Classification of one (tile, chunk) pair by tile_select (tile_select.wgsl:59-91):
+// 0 culled, 1 selected, 2 deferred by Hi-Z. dmin receives box_distance when computed.
+__device__ __forceinline__ uint32_t classify(const FrameK &f, const TileCull &tc, const float bmin[3],
+                                             const float bmax[3], uint32_t ci, uint32_t tile_far, bool use_hiz,
+                                             float &dmin) {
+    dmin = 0.0f;
+    if (ci == f.camera_chunk) return 1u;
+    if (box_outside(bmin, bmax, f.fwd, f.cam)) return 0u;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (box_outside(bmin, bmax, tc.n[k], f.cam)) return 0u;
+    dmin = box_distance(f.cam, bmin, bmax);
+    if (dmin >= tc.reach) return 0u;
+    if (use_hiz && depth_key(f.far_, dmin) < tile_far) return 2u;
+    return 1u;
+}
+
+}  // namespace vxd

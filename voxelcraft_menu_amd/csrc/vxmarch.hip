@@ -1,0 +1,509 @@
+// vxmarch.hip -- extern "C" entry points of the ray-march library (include/vxmarch.h).
+//
+// A context owns one HIP stream and the device buffers the reference Renderer owns:
+// the three tree pools (mirrored with GpuMirror::sync semantics), the chunk list and
+// chunk grid, the visibility buffer, Hi-Z, the pair list, counters and the R16F
+// shadow target.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "kernels.hpp"
+#include "vxmarch.h"
+
+using namespace vxd;
+
+#define LAUNCHED() HIPX(hipPeekAtLastError())
+
+#define HIPX(expr)                                  \
+    do {                                            \
+        hipError_t e_ = (expr);                     \
+        if (e_ != hipSuccess) return hip_fail(e_);  \
+    } while (0)
+
+static int hip_fail(hipError_t e) {
+    if (e == hipErrorOutOfMemory) return VX_E_OOM;
+    std::fprintf(stderr, "vxmarch: HIP error %d (%s)\n", (int)e, hipGetErrorString(e));
+    return VX_E_HIP;
+}
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+int ensure(DevBuf &b, size_t bytes) {
+    if (bytes <= b.cap) return VX_OK;
+    if (b.p) HIPX(hipFree(b.p));
+    b.p = nullptr;
+    b.cap = 0;
+    HIPX(hipMalloc(&b.p, bytes));
+    b.cap = bytes;
+    return VX_OK;
+}
+
+constexpr size_t MIN_CAP = 1u << 20;  // GpuMirror MIN_CAP (src/render/pools.rs:11)
+
+}  // namespace
+
+struct vx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t w = 0, h = 0, tx = 0, ty = 0, max_pairs = 0, max_pairs_req = 0, path = VX_PATH_TILES;
+    DevBuf pool[3];
+    size_t pool_len[3] = {0, 0, 0};
+    bool pool_synced[3] = {false, false, false};
+    DevBuf chunks, grid, vis, shadow, hiz, pairs, counters;
+    uint32_t n_chunks = 0;
+    hipEvent_t ev[8] = {};
+    int nev = 0;
+    bool have_frame = false;
+    FrameK frame{};
+    Spec spec{};
+};
+
+static uint32_t pairs_for(uint32_t tx, uint32_t ty) {
+    uint64_t want = (uint64_t)tx * ty * 16ull;
+    want = std::max<uint64_t>(want, VX_MAX_PAIRS);
+    want = std::min<uint64_t>(want, 0xFFFFFFFFull / 2);
+    return (uint32_t)want;
+}
+
+static int fill_u32(vx_ctx *c, void *p, uint32_t v, size_t n) {
+    if (!n) return VX_OK;
+    unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+    k_fill_u32<<<blocks, 256, 0, c->stream>>>((uint32_t *)p, v, n);
+    HIPX(hipGetLastError());
+    return VX_OK;
+}
+
+static int fill_u64(vx_ctx *c, void *p, uint64_t v, size_t n) {
+    if (!n) return VX_OK;
+    unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+    k_fill_u64<<<blocks, 256, 0, c->stream>>>((uint64_t *)p, v, n);
+    HIPX(hipGetLastError());
+    return VX_OK;
+}
+
+static int alloc_targets(vx_ctx *c, uint32_t w, uint32_t h) {
+    if (w == 0 || h == 0) return VX_E_ARG;
+    uint32_t tx = (w + 7) / 8, ty = (h + 7) / 8;
+    if ((uint64_t)tx * ty > VX_MAX_TILES) return VX_E_OVERFLOW;  // warn_if_oversized
+    c->w = w;
+    c->h = h;
+    c->tx = tx;
+    c->ty = ty;
+    c->max_pairs = c->max_pairs_req ? c->max_pairs_req : pairs_for(tx, ty);
+    int rc;
+    if ((rc = ensure(c->vis, (size_t)w * h * 8)) || (rc = ensure(c->shadow, (size_t)w * h * 2)) ||
+        (rc = ensure(c->hiz, (size_t)tx * ty * 4)))
+        return rc;
+    if (c->path == VX_PATH_PAIRS && (rc = ensure(c->pairs, (size_t)c->max_pairs * 2 * 4))) return rc;
+    if ((rc = fill_u64(c, c->vis.p, 0, (size_t)w * h)) || (rc = fill_u32(c, c->hiz.p, 0, (size_t)tx * ty))) return rc;
+    if ((rc = fill_u32(c, c->shadow.p, 0, ((size_t)w * h + 1) / 2))) return rc;
+    HIPX(hipStreamSynchronize(c->stream));
+    c->have_frame = false;
+    return VX_OK;
+}
+
+extern "C" {
+
+const char *vx_error_string(int code) {
+    switch (code) {
+        case VX_OK: return "ok";
+        case VX_E_NODEV: return "no HIP device";
+        case VX_E_OOM: return "out of device memory";
+        case VX_E_ARG: return "invalid argument";
+        case VX_E_OVERFLOW: return "capacity exceeded";
+        case VX_E_UNSUPPORTED: return "feature not supported by this library";
+        case VX_E_HIP: return "HIP runtime error";
+        case VX_E_BUSY: return "busy";
+        default: return "unknown error";
+    }
+}
+
+int vx_device_name(char *buf, size_t cap) {
+    if (!buf || !cap) return VX_E_ARG;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return VX_E_NODEV;
+    int dev = 0;
+    HIPX(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIPX(hipGetDeviceProperties(&prop, dev));
+    std::snprintf(buf, cap, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+    return VX_OK;
+}
+
+int vx_ctx_create(const vx_ctx_desc *d, vx_ctx **out) {
+    if (!d || !out) return VX_E_ARG;
+    *out = nullptr;
+    if (d->path != VX_PATH_TILES && d->path != VX_PATH_PAIRS) return VX_E_ARG;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return VX_E_NODEV;
+    vx_ctx *c = new (std::nothrow) vx_ctx();
+    if (!c) return VX_E_OOM;
+    int rc = VX_OK;
+    do {
+        if (d->device >= 0) {
+            if (d->device >= n) { rc = VX_E_ARG; break; }
+            if (hipSetDevice(d->device) != hipSuccess) { rc = VX_E_HIP; break; }
+            c->device = d->device;
+        } else if (hipGetDevice(&c->device) != hipSuccess) { rc = VX_E_HIP; break; }
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { rc = VX_E_HIP; break; }
+        for (auto &e : c->ev)
+            if (hipEventCreate(&e) != hipSuccess) { rc = VX_E_HIP; break; }
+        if (rc) break;
+        c->path = d->path;
+        c->max_pairs_req = d->max_pairs;
+        for (auto &b : c->pool)
+            if ((rc = ensure(b, MIN_CAP))) break;
+        if (rc) break;
+        if ((rc = ensure(c->grid, VX_GRID_LEN * 4)) || (rc = ensure(c->counters, CNT_LEN * 4)) ||
+            (rc = ensure(c->chunks, 96 * 64)))
+            break;
+        if ((rc = fill_u32(c, c->grid.p, VX_NO_CHUNK, VX_GRID_LEN))) break;
+        if ((rc = fill_u32(c, c->chunks.p, 0, 96 * 64 / 4))) break;
+        if ((rc = alloc_targets(c, d->width, d->height))) break;
+    } while (0);
+    if (rc) {
+        vx_ctx_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return VX_OK;
+}
+
+int vx_ctx_destroy(vx_ctx *c) {
+    if (!c) return VX_E_ARG;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    DevBuf *bufs[] = {&c->pool[0], &c->pool[1], &c->pool[2], &c->chunks, &c->grid, &c->vis,
+                      &c->shadow,  &c->hiz,     &c->pairs,   &c->counters};
+    for (DevBuf *b : bufs)
+        if (b->p) hipFree(b->p);
+    for (auto &e : c->ev)
+        if (e) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return VX_OK;
+}
+
+int vx_resize(vx_ctx *c, uint32_t w, uint32_t h) {
+    if (!c) return VX_E_ARG;
+    HIPX(hipSetDevice(c->device));
+    return alloc_targets(c, w, h);
+}
+
+// GpuMirror::sync (src/render/pools.rs:44-98): grow by doubling from 1 MiB with a
+// full upload; otherwise merge dirty ranges with a 4096-byte gap and upload them,
+// falling back to a full upload above 512 merged ranges.
+int vx_pool_sync(vx_ctx *c, int pool, const void *data, size_t bytes, const vx_range *dirty, size_t n_dirty) {
+    if (!c || pool < 0 || pool > 2 || (!data && bytes) || (!dirty && n_dirty)) return VX_E_ARG;
+    static const size_t elem[3] = {16, 8, 4};
+    if (bytes % elem[pool]) return VX_E_ARG;
+    HIPX(hipSetDevice(c->device));
+    DevBuf &b = c->pool[pool];
+    if (!c->pool_synced[pool] && bytes <= b.cap) {
+        // A new context has no mirror contents yet: its first sync is a full upload
+        // whatever the dirty list says (another mirror may have drained it already).
+        if (bytes) HIPX(hipMemcpyAsync(b.p, data, bytes, hipMemcpyHostToDevice, c->stream));
+        c->pool_len[pool] = bytes;
+        c->pool_synced[pool] = true;
+        HIPX(hipStreamSynchronize(c->stream));
+        return VX_OK;
+    }
+    c->pool_synced[pool] = true;
+    if (bytes > b.cap) {
+        size_t cap = b.cap ? b.cap : MIN_CAP;
+        while (cap < bytes) cap *= 2;
+        HIPX(hipStreamSynchronize(c->stream));
+        if (b.p) HIPX(hipFree(b.p));
+        b.p = nullptr;
+        b.cap = 0;
+        HIPX(hipMalloc(&b.p, cap));
+        b.cap = cap;
+        HIPX(hipMemcpyAsync(b.p, data, bytes, hipMemcpyHostToDevice, c->stream));
+        c->pool_len[pool] = bytes;
+        HIPX(hipStreamSynchronize(c->stream));
+        return VX_OK;
+    }
+    c->pool_len[pool] = bytes;
+    if (n_dirty == 0) return VX_OK;
+    std::vector<std::pair<uint64_t, uint64_t>> d(n_dirty);
+    for (size_t i = 0; i < n_dirty; i++)
+        d[i] = {(uint64_t)dirty[i].off * elem[pool], ((uint64_t)dirty[i].off + dirty[i].len) * elem[pool]};
+    std::sort(d.begin(), d.end());
+    std::vector<std::pair<uint64_t, uint64_t>> merged;
+    for (auto &r : d) {
+        if (!merged.empty() && r.first <= merged.back().second + 4096)
+            merged.back().second = std::max(merged.back().second, r.second);
+        else
+            merged.push_back(r);
+    }
+    if (merged.size() > 512) {
+        HIPX(hipMemcpyAsync(b.p, data, bytes, hipMemcpyHostToDevice, c->stream));
+    } else {
+        for (auto &r : merged) {
+            uint64_t s = r.first & ~3ull, e = std::min<uint64_t>((r.second + 3) & ~3ull, bytes);
+            if (e > s)
+                HIPX(hipMemcpyAsync((char *)b.p + s, (const char *)data + s, e - s, hipMemcpyHostToDevice,
+                                    c->stream));
+        }
+    }
+    HIPX(hipStreamSynchronize(c->stream));
+    return VX_OK;
+}
+
+int vx_set_chunks(vx_ctx *c, const vx_gpu_chunk *chunks, uint32_t n, const uint32_t *grid) {
+    if (!c || (!chunks && n) || !grid || n > VX_MAX_CHUNKS) return VX_E_ARG;
+    HIPX(hipSetDevice(c->device));
+    size_t bytes = (size_t)std::max<uint32_t>(n, 1) * sizeof(vx_gpu_chunk);
+    if (bytes > c->chunks.cap) {
+        HIPX(hipStreamSynchronize(c->stream));
+        int rc = ensure(c->chunks, std::max<size_t>(bytes, c->chunks.cap * 2));
+        if (rc) return rc;
+    }
+    if (n) HIPX(hipMemcpyAsync(c->chunks.p, chunks, (size_t)n * sizeof(vx_gpu_chunk), hipMemcpyHostToDevice, c->stream));
+    HIPX(hipMemcpyAsync(c->grid.p, grid, VX_GRID_LEN * 4, hipMemcpyHostToDevice, c->stream));
+    HIPX(hipStreamSynchronize(c->stream));
+    c->n_chunks = n;
+    return VX_OK;
+}
+
+static Pools make_pools(const vx_ctx *c) {
+    Pools p;
+    p.inners = (const uint4 *)c->pool[0].p;
+    p.leaves = (const uint2 *)c->pool[1].p;
+    p.bricks = (const uint32_t *)c->pool[2].p;
+    p.chunks = (const vx_gpu_chunk *)c->chunks.p;
+    p.grid = (const uint32_t *)c->grid.p;
+    p.n_inners = (uint32_t)(c->pool_len[0] / 16);
+    p.n_leaves = (uint32_t)(c->pool_len[1] / 8);
+    p.n_bricks = (uint32_t)(c->pool_len[2] / 4);
+    p.n_chunks = c->n_chunks;
+    return p;
+}
+
+static FrameK make_frame(const vx_gpu_frame *g) {
+    FrameK f;
+    std::memcpy(f.cam, g->cam_pos, 12);
+    std::memcpy(f.fwd, g->cam_fwd, 12);
+    std::memcpy(f.right, g->cam_right, 12);
+    std::memcpy(f.up, g->cam_up, 12);
+    std::memcpy(f.sun, g->sun_dir, 12);
+    f.tan_half_fov = g->tan_half_fov;
+    f.aspect = g->aspect;
+    f.far_ = g->far_;
+    f.daylight = g->daylight;
+    f.shadow_dist = g->shadow_dist;
+    f.sea_level = g->sea_level;
+    f.jitter[0] = g->jitter[0];
+    f.jitter[1] = g->jitter[1];
+    f.res[0] = g->res[0];
+    f.res[1] = g->res[1];
+    f.tiles[0] = g->tiles[0];
+    f.tiles[1] = g->tiles[1];
+    f.chunk_count = g->chunk_count;
+    f.camera_chunk = g->camera_chunk;
+    f.flags = g->flags;
+    f.max_pairs = g->max_pairs;
+    f.frame_index = g->frame_index;
+    std::memcpy(f.grid_min, g->grid_min, 12);
+    std::memcpy(f.grid_dim, g->grid_dim, 12);
+    return f;
+}
+
+// Override values for a (flags & SPEC_MASK, spec_hi) pipeline key (src/render/mod.rs:2355-2689).
+static int make_spec(uint32_t spec, uint32_t spec_hi, uint32_t frame_flags, Spec &s) {
+    if (spec & VX_FLAG_FOLIAGE) return VX_E_UNSUPPORTED;  // cross_quad samples the texture atlas
+    if ((spec_hi & VX_FLAG_HI_SNELL_BEND) && (frame_flags & VX_FLAG_UNDERWATER)) return VX_E_UNSUPPORTED;
+    s.bits = 0;
+    if (spec & VX_FLAG_LEAF_CUTOUT) s.bits |= SPEC_CUTOUT;
+    if (spec & VX_FLAG_GLASS) s.bits |= SPEC_GLASS;
+    if (spec_hi & VX_FLAG_HI_FULL_MARCH) s.bits |= SPEC_FULL_MARCH;
+    if (spec_hi & VX_FLAG_HI_EMITTER_WORLD) s.bits |= SPEC_EMITTER;
+    if (!(spec_hi & VX_FLAG_HI_LEGACY_WATER)) s.bits |= SPEC_WATER_REPAIR;
+    s.leaf_fill = (spec & VX_FLAG_LEAF_THIN) ? 0.62f : 0.72f;
+    return VX_OK;
+}
+
+static unsigned pad8(unsigned n) { return (n + 7u) & ~7u; }
+
+int vx_render(vx_ctx *c, const vx_gpu_frame *g, uint32_t spec, uint32_t spec_hi) {
+    if (!c || !g) return VX_E_ARG;
+    if (g->res[0] != c->w || g->res[1] != c->h) return VX_E_ARG;
+    if (g->tiles[0] != c->tx || g->tiles[1] != c->ty) return VX_E_ARG;
+    if (g->chunk_count > c->n_chunks || g->chunk_count > VX_MAX_CHUNKS) return VX_E_ARG;
+    if ((uint64_t)g->grid_dim[0] * g->grid_dim[1] * g->grid_dim[2] > VX_GRID_LEN) return VX_E_ARG;
+    if (c->path == VX_PATH_PAIRS && g->max_pairs != c->max_pairs) return VX_E_ARG;
+    Spec s;
+    int rc = make_spec(spec, spec_hi, g->flags, s);
+    if (rc) return rc;
+    HIPX(hipSetDevice(c->device));
+    FrameK f = make_frame(g);
+    Pools p = make_pools(c);
+    const bool hiz_on = (f.flags & VX_FLAG_HIZ) != 0u;
+    const uint32_t ntiles = c->tx * c->ty;
+    uint32_t *counters = (uint32_t *)c->counters.p;
+    unsigned long long *vis = (unsigned long long *)c->vis.p;
+    uint32_t *hiz = (uint32_t *)c->hiz.p;
+    hipStream_t st = c->stream;
+
+    HIPX(hipEventRecord(c->ev[0], st));
+    if ((rc = fill_u32(c, counters, 0, CNT_LEN))) return rc;
+    if (c->path == VX_PATH_PAIRS) {
+        // clear vis (+ hiz when Hi-Z is off), tile_select, finalize, march,
+        // [build_hiz, recover_select, finalize_recover, march] (src/render/mod.rs:1716-1795)
+        if ((rc = fill_u64(c, vis, 0, (size_t)c->w * c->h))) return rc;
+        if (!hiz_on && (rc = fill_u32(c, hiz, 0, ntiles))) return rc;
+        uint32_t *pairs = (uint32_t *)c->pairs.p;
+        dim3 sel_grid((c->tx + 15) / 16, (c->ty + 15) / 16);
+        k_tile_select<<<sel_grid, 256, 0, st>>>(p, f, hiz, pairs, counters);
+        LAUNCHED();
+        k_finalize<<<1, 1, 0, st>>>(f, counters);
+        LAUNCHED();
+        HIPX(hipEventRecord(c->ev[1], st));
+        k_march_pairs<<<2048, 256, 0, st>>>(p, f, s, pairs, counters, vis);
+        LAUNCHED();
+        HIPX(hipEventRecord(c->ev[2], st));
+        if (hiz_on) {
+            k_build_hiz<<<(ntiles + 3) / 4, 256, 0, st>>>(f, vis, hiz);
+            LAUNCHED();
+            HIPX(hipEventRecord(c->ev[3], st));
+            k_recover_select<<<(c->max_pairs + 255) / 256, 256, 0, st>>>(p, f, hiz, pairs, counters);
+            LAUNCHED();
+            k_finalize_recover<<<1, 1, 0, st>>>(f, counters);
+            LAUNCHED();
+            HIPX(hipEventRecord(c->ev[4], st));
+            k_march_pairs<<<2048, 256, 0, st>>>(p, f, s, pairs, counters, vis);
+            LAUNCHED();
+        } else {
+            HIPX(hipEventRecord(c->ev[3], st));
+            HIPX(hipEventRecord(c->ev[4], st));
+        }
+        HIPX(hipEventRecord(c->ev[5], st));
+    } else {
+        HIPX(hipEventRecord(c->ev[1], st));
+        k_march_tiles<<<pad8((ntiles + 3) / 4), 256, 0, st>>>(p, f, s, hiz, vis, counters);
+        LAUNCHED();
+        HIPX(hipEventRecord(c->ev[2], st));
+        HIPX(hipEventRecord(c->ev[3], st));
+        HIPX(hipEventRecord(c->ev[4], st));
+        HIPX(hipEventRecord(c->ev[5], st));
+    }
+    k_shadow<<<pad8((ntiles + 3) / 4), 256, 0, st>>>(p, f, s, vis, (uint16_t *)c->shadow.p, counters);
+    LAUNCHED();
+    HIPX(hipEventRecord(c->ev[6], st));
+    HIPX(hipGetLastError());
+    c->frame = f;
+    c->spec = s;
+    c->have_frame = true;
+    return VX_OK;
+}
+
+int vx_sync(vx_ctx *c) {
+    if (!c) return VX_E_ARG;
+    HIPX(hipStreamSynchronize(c->stream));
+    return VX_OK;
+}
+
+static int read_dev(vx_ctx *c, const DevBuf &b, void *out, size_t bytes) {
+    HIPX(hipSetDevice(c->device));
+    HIPX(hipMemcpyAsync(out, b.p, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPX(hipStreamSynchronize(c->stream));
+    return VX_OK;
+}
+
+int vx_read_vis(vx_ctx *c, uint64_t *out, size_t n) {
+    if (!c || !out || n != (size_t)c->w * c->h) return VX_E_ARG;
+    return read_dev(c, c->vis, out, n * 8);
+}
+
+int vx_read_shadow(vx_ctx *c, uint16_t *out, size_t n) {
+    if (!c || !out || n != (size_t)c->w * c->h) return VX_E_ARG;
+    return read_dev(c, c->shadow, out, n * 2);
+}
+
+int vx_read_hiz(vx_ctx *c, uint32_t *out, size_t n) {
+    if (!c || !out || n != (size_t)c->tx * c->ty) return VX_E_ARG;
+    return read_dev(c, c->hiz, out, n * 4);
+}
+
+int vx_read_pairs(vx_ctx *c, uint32_t *out, size_t n, uint32_t *counters16) {
+    if (!c || (!out && n) || c->path != VX_PATH_PAIRS || n > (size_t)c->max_pairs * 2) return VX_E_ARG;
+    int rc = VX_OK;
+    if (n && (rc = read_dev(c, c->pairs, out, n * 4))) return rc;
+    if (counters16) rc = read_dev(c, c->counters, counters16, CNT_LEN * 4);
+    return rc;
+}
+
+int vx_write_hiz(vx_ctx *c, const uint32_t *in, size_t n) {
+    if (!c || !in || n != (size_t)c->tx * c->ty) return VX_E_ARG;
+    HIPX(hipSetDevice(c->device));
+    HIPX(hipMemcpyAsync(c->hiz.p, in, n * 4, hipMemcpyHostToDevice, c->stream));
+    HIPX(hipStreamSynchronize(c->stream));
+    return VX_OK;
+}
+
+int vx_read_decoded(vx_ctx *c, vx_hit *out, size_t n) {
+    if (!c || !out || n != (size_t)c->w * c->h) return VX_E_ARG;
+    if (!c->have_frame) return VX_E_ARG;
+    HIPX(hipSetDevice(c->device));
+    vx_hit *d = nullptr;
+    HIPX(hipMallocAsync((void **)&d, n * sizeof(vx_hit), c->stream));
+    k_decode<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(make_pools(c), c->frame,
+                                                                 (const unsigned long long *)c->vis.p,
+                                                                 (const uint16_t *)c->shadow.p, d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d, n * sizeof(vx_hit), hipMemcpyDeviceToHost, c->stream);
+    hipFreeAsync(d, c->stream);
+    hipError_t e2 = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(e);
+    if (e2 != hipSuccess) return hip_fail(e2);
+    return VX_OK;
+}
+
+int vx_pair_demand_get(vx_ctx *c, vx_pair_demand *out) {
+    if (!c || !out) return VX_E_ARG;
+    uint32_t cnt[CNT_LEN];
+    int rc = read_dev(c, c->counters, cnt, sizeof cnt);
+    if (rc) return rc;
+    out->select = cnt[CNT_SELECT_RAW];
+    out->deferred = cnt[CNT_DEFERRED];
+    out->recover = cnt[CNT_RECOVER_RAW];
+    out->cap = c->max_pairs;
+    out->truncated = cnt[CNT_TRUNC];
+    out->shadow_rays = cnt[CNT_SHADOW];
+    return VX_OK;
+}
+
+int vx_timings_get(vx_ctx *c, vx_timings *t) {
+    if (!c || !t) return VX_E_ARG;
+    HIPX(hipSetDevice(c->device));
+    HIPX(hipStreamSynchronize(c->stream));
+    float ms[6];
+    for (int i = 0; i < 6; i++) HIPX(hipEventElapsedTime(&ms[i], c->ev[i], c->ev[i + 1]));
+    t->tile_select = ms[0];
+    t->march = ms[1];
+    t->hiz = ms[2];
+    t->recover = ms[3];
+    t->march2 = ms[4];
+    t->shadow = ms[5];
+    HIPX(hipEventElapsedTime(&t->total, c->ev[0], c->ev[6]));
+    return VX_OK;
+}
+
+int vx_device_outputs(vx_ctx *c, uint64_t **vis, uint16_t **shadow) {
+    if (!c) return VX_E_ARG;
+    if (vis) *vis = (uint64_t *)c->vis.p;
+    if (shadow) *shadow = (uint16_t *)c->shadow.p;
+    return VX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,131 @@
+"""GPU renderer handle: the reference ``Renderer`` surface for the march passes.
+
+``Renderer`` mirrors src/render/mod.rs: ``sync_world`` (:1355-1396) uploads the three
+pools with GpuMirror semantics, ``build_chunk_list`` (:1447-1564) builds and uploads the
+per-view chunk list and grid, ``render`` (:1593-1840) runs tile_select / march / Hi-Z /
+recover / march / primary_shadow, and the ``read_*`` calls read results back
+(:1948-2112). Every call goes to libvxmarch.so; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _abi as A
+from .world import CameraParams, ViewData, World, build_view
+
+
+class Renderer:
+    def __init__(self, width: int, height: int, path: int = A.PATH_TILES, max_pairs: int = 0, device: int = -1):
+        self._L = A.march()
+        self._c = C.c_void_p()
+        desc = A.CtxDesc(width, height, max_pairs, path, device)
+        A.check(self._L.vx_ctx_create(C.byref(desc), C.byref(self._c)), "vx_ctx_create")
+        self.size = (width, height)
+        self.path = path
+        self.view: ViewData | None = None
+        self._frame = None
+
+    def close(self):
+        if self._c:
+            self._L.vx_ctx_destroy(self._c)
+            self._c = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._c
+
+    def resize(self, width: int, height: int):
+        A.check(self._L.vx_resize(self._c, width, height), "vx_resize")
+        self.size = (width, height)
+
+    def sync_world(self, world: World):
+        """Upload the pools' dirty ranges (or the whole pool when it outgrows the mirror)."""
+        for pool in (A.POOL_INNERS, A.POOL_LEAVES, A.POOL_BRICKS):
+            dirty = world.take_dirty(pool)
+            ptr, nbytes = world.pool_view(pool)
+            A.check(self._L.vx_pool_sync(self._c, pool, ptr, nbytes, A.ptr(dirty) if len(dirty) else None,
+                                         len(dirty)), "vx_pool_sync")
+
+    def upload_pools(self, inners: np.ndarray, leaves: np.ndarray, bricks: np.ndarray):
+        """Full upload of caller-held pool arrays (u32 words)."""
+        for pool, arr in ((A.POOL_INNERS, inners), (A.POOL_LEAVES, leaves), (A.POOL_BRICKS, bricks)):
+            a = np.ascontiguousarray(arr, dtype=np.uint32)
+            n_elem = a.nbytes // A.POOL_ELEM[pool]
+            rng = np.array([[0, n_elem]], dtype=np.uint32)
+            A.check(self._L.vx_pool_sync(self._c, pool, A.ptr(a) if a.size else None, a.nbytes,
+                                         A.ptr(rng) if n_elem else None, 1 if n_elem else 0), "vx_pool_sync")
+
+    def set_view(self, view: ViewData):
+        A.check(self._L.vx_set_chunks(self._c, A.ptr(view.chunks), view.n_total, A.ptr(view.grid)), "vx_set_chunks")
+        self.view = view
+
+    def build_chunk_list(self, world: World, cam: CameraParams, max_pairs: int = 0) -> ViewData:
+        v = build_view(world, cam, max_pairs)
+        self.set_view(v)
+        return v
+
+    def render(self, frame: A.Frame, spec: int, spec_hi: int):
+        A.check(self._L.vx_render(self._c, C.byref(frame), spec, spec_hi), "vx_render")
+        self._frame = frame
+
+    def sync(self):
+        A.check(self._L.vx_sync(self._c), "vx_sync")
+
+    def read_vis(self) -> np.ndarray:
+        w, h = self.size
+        out = np.zeros(w * h, dtype=np.uint64)
+        A.check(self._L.vx_read_vis(self._c, A.ptr(out), out.size), "vx_read_vis")
+        return out
+
+    def read_shadow(self) -> np.ndarray:
+        w, h = self.size
+        out = np.zeros(w * h, dtype=np.uint16)
+        A.check(self._L.vx_read_shadow(self._c, A.ptr(out), out.size), "vx_read_shadow")
+        return out
+
+    def read_hiz(self) -> np.ndarray:
+        w, h = self.size
+        out = np.zeros(((w + 7) // 8) * ((h + 7) // 8), dtype=np.uint32)
+        A.check(self._L.vx_read_hiz(self._c, A.ptr(out), out.size), "vx_read_hiz")
+        return out
+
+    def write_hiz(self, hiz: np.ndarray):
+        a = np.ascontiguousarray(hiz, dtype=np.uint32)
+        A.check(self._L.vx_write_hiz(self._c, A.ptr(a), a.size), "vx_write_hiz")
+
+    def read_pairs(self, n: int):
+        """(pair words[0:n], counters[16]) of the last frame; pairs-path contexts only."""
+        out = np.zeros(max(n, 1), dtype=np.uint32)
+        cnt = np.zeros(16, dtype=np.uint32)
+        A.check(self._L.vx_read_pairs(self._c, A.ptr(out), n, A.ptr(cnt)), "vx_read_pairs")
+        return out[:n], cnt
+
+    def read_decoded(self) -> np.ndarray:
+        w, h = self.size
+        out = np.zeros(w * h, dtype=A.HIT_DTYPE)
+        A.check(self._L.vx_read_decoded(self._c, A.ptr(out), w * h), "vx_read_decoded")
+        return out
+
+    def pair_demand(self) -> dict:
+        d = A.PairDemand()
+        A.check(self._L.vx_pair_demand_get(self._c, C.byref(d)), "vx_pair_demand_get")
+        return {k: int(getattr(d, k)) for k, _ in A.PairDemand._fields_}
+
+    def timings(self) -> dict:
+        t = A.Timings()
+        A.check(self._L.vx_timings_get(self._c, C.byref(t)), "vx_timings_get")
+        return {k: float(getattr(t, k)) for k, _ in A.Timings._fields_}
+
+
+def device_name() -> str:
+    buf = C.create_string_buffer(256)
+    rc = A.march().vx_device_name(buf, 256)
+    return buf.value.decode() if rc == 0 else "none"
