@@ -183,6 +183,10 @@ int vx_read_pairs(vx_ctx *ctx, uint32_t *out, size_t n, uint32_t *counters16);
 int vx_read_decoded(vx_ctx *ctx, vx_hit *out, size_t n);
 int vx_pair_demand_get(vx_ctx *ctx, vx_pair_demand *out);
 int vx_timings_get(vx_ctx *ctx, vx_timings *out);
+/* Per-pass milliseconds summed over the frames rendered since vx_timings_reset (the
+ * most recent 256 at most; *frames receives the count). */
+int vx_timings_reset(vx_ctx *ctx);
+int vx_timings_sum(vx_ctx *ctx, vx_timings *out, uint32_t *frames);
 /* Device pointers of the current outputs (for callers that keep data on the GPU). */
 int vx_device_outputs(vx_ctx *ctx, uint64_t **vis, uint16_t **shadow);
 const char *vx_error_string(int code);

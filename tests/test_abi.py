@@ -1,0 +1,65 @@
+"""The C ABI: every declared symbol is exported, struct layouts match the reference,
+and host-side argument checks behave without a GPU."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from voxelcraft_menu_amd import _abi as A
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    src = open(os.path.join(ROOT, "include", header)).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b((?:vx|vxw)_[a-z0-9_]+)\s*\(", src)))
+
+
+@pytest.mark.parametrize("header,loader", [("vxhost.h", A.host), ("vxmarch.h", A.march)])
+def test_every_declared_symbol_is_exported(header, loader):
+    lib = loader()
+    names = _declared(header)
+    assert len(names) > 10
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_bindings_cover_the_headers():
+    assert set(_declared("vxhost.h")) <= set(A.host_symbols())
+    assert set(_declared("vxmarch.h")) <= set(A.march_symbols())
+
+
+def test_struct_layouts_match_the_reference():
+    # GpuFrame offsets asserted in src/render/mod.rs:458-466 and validation/check_compact.py:45
+    assert C.sizeof(A.Frame) == 352
+    assert A.Frame.cloud_cover.offset == 192 if hasattr(A.Frame, "cloud_cover") else A.Frame.shading.offset == 192
+    assert A.Frame.prev_view_proj.offset == 288
+    assert A.Frame.grid_min.offset == 144 and A.Frame.sea_level.offset == 156
+    assert A.Frame.grid_dim.offset == 160 and A.Frame.jitter.offset == 176 and A.Frame.frame_index.offset == 188
+    # GpuChunk (src/render/mod.rs:468-489)
+    assert C.sizeof(A.Chunk) == 96
+    assert A.Chunk.root.offset == 48 and A.Chunk.attr_base.offset == 64 and A.Chunk.dry_mask.offset == 80
+    assert C.sizeof(A.Edit) == 16
+
+
+def test_error_strings_and_no_device_path():
+    L = A.march()
+    assert L.vx_error_string(0) == b"ok"
+    assert L.vx_error_string(-5).startswith(b"feature")
+    # argument checks run before any device work
+    assert L.vx_ctx_create(None, None) == -3
+    assert L.vx_render(None, None, 0, 0) == -3
+    assert L.vx_pool_sync(None, 0, None, 0, None, 0) == -3
+    if os.environ.get("HIP_VISIBLE_DEVICES") == "" or not os.path.exists("/dev/kfd"):
+        ctx = C.c_void_p()
+        desc = A.CtxDesc(64, 64, 0, 0, -1)
+        assert L.vx_ctx_create(C.byref(desc), C.byref(ctx)) == -1  # VX_E_NODEV
+
+
+def test_host_argument_checks():
+    L = A.host()
+    assert L.vxw_insert_dense(None, 0, 0, 0, None) == -3
+    assert L.vxw_build_view(None, None, None, None, None) == -3
+    assert L.vxw_pack_frame(None, None, 0, None) == -3

@@ -1,0 +1,186 @@
+"""GPU parity: the HIP path against the oracle through the C ABI, bit for bit.
+
+Visibility keys, Hi-Z words, R16F shadow words and the pair-demand counters must be
+identical; the decoded hit distance is the f32 hit_t and must agree within 1e-6
+relative (here it is also bit-identical, since both sides run the same f32 ops).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import helpers as H
+from voxelcraft_menu_amd import _abi as A
+from voxelcraft_menu_amd import workloads as W
+from voxelcraft_menu_amd.renderer import Renderer
+from voxelcraft_menu_amd.world import build_view
+
+pytestmark = pytest.mark.gpu
+PATHS = [A.PATH_TILES, A.PATH_PAIRS]
+
+
+@pytest.fixture(scope="module")
+def small_world():
+    return W.make_world(W.WORKLOADS[0])
+
+
+@pytest.fixture(scope="module")
+def big_world():
+    return W.make_world(W.WORKLOADS[1])
+
+
+def _check_frames(world, cams, path, frames=2, label=""):
+    r = Renderer(cams[0].width, cams[0].height, path=path)
+    r.sync_world(world)
+    for ci, cam in enumerate(cams):
+        if (cam.width, cam.height) != r.size:
+            r.resize(cam.width, cam.height)
+        v = build_view(world, cam)
+        r.set_view(v)
+        spec, hi = W.spec_words(cam.flags, world)
+        hz = r.read_hiz()
+        ref = H.oracle_frames(world, v, spec, hi, frames=frames, hiz=hz)
+        for f in range(frames):
+            r.render(v.frame, spec, hi)
+            r.sync()
+            vis, sh, hz_o, st = ref[f]
+            gv, gs, gh = r.read_vis(), r.read_shadow(), r.read_hiz()
+            d = r.pair_demand()
+            tag = f"{label} cam{ci} frame{f} path{path} pos={cam.pos}"
+            assert np.array_equal(gv, vis), f"{tag}: {(gv != vis).sum()} vis words differ"
+            assert np.array_equal(gh, hz_o), f"{tag}: {(gh != hz_o).sum()} hiz words differ"
+            assert np.array_equal(gs, sh), f"{tag}: {(gs != sh).sum()} shadow words differ"
+            assert (d["select"], d["deferred"], d["recover"]) == (
+                st["pairs_select"], st["pairs_deferred"], st["pairs_recover"]), tag
+            assert d["shadow_rays"] == st["rays_shadow"], tag
+            assert not d["truncated"]
+    r.close()
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_small_default_camera_three_frames(small_world, path):
+    wl = W.WORKLOADS[0]
+    _check_frames(small_world, [W.default_camera(wl)], path, frames=3, label="w0")
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_small_random_views_odd_sizes(small_world, path):
+    wl = W.WORKLOADS[0]
+    cams = W.random_cameras(wl, 5, seed=21, resolutions=[(333, 197), (64, 64), (120, 250)])
+    _check_frames(small_world, cams, path, frames=2, label="w0-random")
+
+
+def test_camera_inside_terrain_and_underwater(small_world):
+    wl = W.WORKLOADS[0]
+    flags = W.DEFAULT_FLAGS
+    inside = H.camera((60.5, 120.3, 70.5), 30.0, -5.0, 160, 96, flags)  # below the surface: camera chunk path
+    h = wl.gen.height(200, 40)
+    under = H.camera((200.5, min(h, 127) - 0.5, 40.5), 120.0, 10.0, 160, 96, flags | A.FLAG_UNDERWATER)
+    up = H.camera((128.5, 150.0, 128.5), 0.0, 60.0, 160, 96, flags)  # looking up: sky, off-frame chunks
+    _check_frames(small_world, [inside, under, up], A.PATH_TILES, frames=2, label="special")
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_full_size_1080p_default_camera(big_world, path):
+    wl = W.WORKLOADS[1]
+    _check_frames(big_world, [W.default_camera(wl)], path, frames=2, label="w1")
+
+
+def test_full_size_random_views(big_world):
+    wl = W.WORKLOADS[1]
+    cams = W.random_cameras(wl, 3, seed=2, resolutions=[(640, 360), (1280, 720)])
+    _check_frames(big_world, cams, A.PATH_TILES, frames=1, label="w1-random")
+
+
+def test_decoded_hits_match_oracle_decode(small_world):
+    wl = W.WORKLOADS[0]
+    cam = W.default_camera(wl)
+    v = build_view(small_world, cam)
+    spec, hi = W.spec_words(cam.flags, small_world)
+    r = Renderer(cam.width, cam.height)
+    r.sync_world(small_world)
+    r.set_view(v)
+    r.render(v.frame, spec, hi)
+    r.sync()
+    dec = r.read_decoded()
+    vis, sh = r.read_vis(), r.read_shadow()
+    sc = O.Scene(small_world.pool(0), small_world.pool(1), small_world.pool(2), v.chunks, v.grid)
+    ref = O.decode(sc, v.frame_bytes, vis)
+    hit = vis != 0
+    assert hit.mean() > 0.2
+    assert np.array_equal(dec["voxel"][hit], ref["voxel"][hit])
+    assert np.array_equal(dec["face"][hit], ref["face"][hit])
+    assert np.array_equal(dec["material"][hit], ref["material"][hit])
+    rel = np.abs(dec["t"][hit] - ref["t"][hit]) / np.maximum(np.abs(ref["t"][hit]), 1e-6)
+    assert rel.max() <= 1e-6
+    assert np.all(dec["t"][~hit] == -1.0)
+    assert np.array_equal(dec["shadow"].astype(bool), sh != 0)
+    # the decoded material is the world's block at the decoded voxel
+    idx = np.nonzero(hit)[0][:500]
+    for i in idx:
+        x, y, z = (int(c) for c in dec["voxel"][i])
+        assert small_world.get_block(x, y, z) == dec["material"][i]
+    r.close()
+
+
+def test_edits_sync_incrementally_and_stay_exact():
+    wl = W.WORKLOADS[0]
+    world = W.make_world(wl)
+    cam = W.default_camera(wl, width=320, height=180)
+    r = Renderer(cam.width, cam.height)
+    r.sync_world(world)
+    rng = np.random.default_rng(9)
+    for batch in range(3):
+        e = np.zeros(1000, A.EDIT_DTYPE)
+        e["x"] = rng.integers(0, 256, 1000)
+        e["y"] = rng.integers(64, 192, 1000)
+        e["z"] = rng.integers(0, 256, 1000)
+        e["id"] = rng.choice([0, 1, 2, 8, 10, 11], 1000)
+        assert world.apply_edits(e) == 1000
+        r.sync_world(world)  # dirty ranges only
+        v = build_view(world, cam)
+        r.set_view(v)
+        spec, hi = W.spec_words(cam.flags, world)
+        assert hi & A.FLAG_HI_EMITTER_WORLD  # glowstone edits switch the emitter specialisation on
+        ref = H.oracle_frames(world, v, spec, hi, frames=1, hiz=r.read_hiz())
+        r.render(v.frame, spec, hi)
+        r.sync()
+        assert np.array_equal(r.read_vis(), ref[0][0]), f"batch {batch}"
+        assert np.array_equal(r.read_shadow(), ref[0][1]), f"batch {batch}"
+    r.close()
+
+
+def test_pair_list_overflow_is_flagged(small_world):
+    wl = W.WORKLOADS[0]
+    cam = W.default_camera(wl, flags=W.DEFAULT_FLAGS & ~A.FLAG_HIZ)
+    v = build_view(small_world, cam, max_pairs=1000)
+    spec, hi = W.spec_words(cam.flags, small_world)
+    r = Renderer(cam.width, cam.height, path=A.PATH_PAIRS, max_pairs=1000)
+    r.sync_world(small_world)
+    r.set_view(v)
+    r.render(v.frame, spec, hi)
+    r.sync()
+    d = r.pair_demand()
+    assert d["truncated"] == 1 and d["select"] > 1000 and d["cap"] == 1000
+    r.close()
+
+
+def test_tiles_and_pairs_paths_agree_and_are_deterministic(big_world):
+    wl = W.WORKLOADS[1]
+    cams = W.random_cameras(wl, 2, seed=5, resolutions=[(1280, 720)])
+    outs = []
+    for path in PATHS:
+        r = Renderer(1280, 720, path=path)
+        r.sync_world(big_world)
+        res = []
+        for cam in cams:
+            v = build_view(big_world, cam)
+            r.set_view(v)
+            spec, hi = W.spec_words(cam.flags, big_world)
+            for _ in range(2):
+                r.render(v.frame, spec, hi)
+            r.sync()
+            res.append((r.read_vis(), r.read_shadow()))
+        outs.append(res)
+        r.close()
+    for (a, b) in zip(outs[0], outs[1]):
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])

@@ -217,6 +217,8 @@ _sig_march = {
     "vx_read_decoded": (C.c_int, [P, P, C.c_size_t]),
     "vx_pair_demand_get": (C.c_int, [P, C.POINTER(PairDemand)]),
     "vx_timings_get": (C.c_int, [P, C.POINTER(Timings)]),
+    "vx_timings_reset": (C.c_int, [P]),
+    "vx_timings_sum": (C.c_int, [P, C.POINTER(Timings), C.POINTER(C.c_uint32)]),
     "vx_device_outputs": (C.c_int, [P, P, P]),
     "vx_error_string": (C.c_char_p, [C.c_int]),
     "vx_device_name": (C.c_int, [C.c_char_p, C.c_size_t]),

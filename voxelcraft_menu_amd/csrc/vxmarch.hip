@@ -61,8 +61,12 @@ struct vx_ctx {
     bool pool_synced[3] = {false, false, false};
     DevBuf chunks, grid, vis, shadow, hiz, pairs, counters;
     uint32_t n_chunks = 0;
-    hipEvent_t ev[8] = {};
-    int nev = 0;
+    // Ring of per-frame event sets: [0] start, [1..5] pass boundaries, [6] end.
+    static constexpr int RING = 256;
+    hipEvent_t ring[RING][7] = {};
+    uint32_t ring_next = 0;     // set the next frame records into
+    uint32_t ring_count = 0;    // frames recorded since the last reset (saturates at RING)
+    hipEvent_t *ev = nullptr;   // the set of the frame being recorded / last recorded
     bool have_frame = false;
     FrameK frame{};
     Spec spec{};
@@ -156,9 +160,11 @@ int vx_ctx_create(const vx_ctx_desc *d, vx_ctx **out) {
             c->device = d->device;
         } else if (hipGetDevice(&c->device) != hipSuccess) { rc = VX_E_HIP; break; }
         if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { rc = VX_E_HIP; break; }
-        for (auto &e : c->ev)
-            if (hipEventCreate(&e) != hipSuccess) { rc = VX_E_HIP; break; }
+        for (auto &set : c->ring)
+            for (auto &e : set)
+                if (rc == VX_OK && hipEventCreate(&e) != hipSuccess) rc = VX_E_HIP;
         if (rc) break;
+        c->ev = c->ring[0];
         c->path = d->path;
         c->max_pairs_req = d->max_pairs;
         for (auto &b : c->pool)
@@ -187,8 +193,9 @@ int vx_ctx_destroy(vx_ctx *c) {
                       &c->shadow,  &c->hiz,     &c->pairs,   &c->counters};
     for (DevBuf *b : bufs)
         if (b->p) hipFree(b->p);
-    for (auto &e : c->ev)
-        if (e) hipEventDestroy(e);
+    for (auto &set : c->ring)
+        for (auto &e : set)
+            if (e) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return VX_OK;
@@ -355,6 +362,9 @@ int vx_render(vx_ctx *c, const vx_gpu_frame *g, uint32_t spec, uint32_t spec_hi)
     uint32_t *hiz = (uint32_t *)c->hiz.p;
     hipStream_t st = c->stream;
 
+    c->ev = c->ring[c->ring_next];
+    c->ring_next = (c->ring_next + 1) % vx_ctx::RING;
+    if (c->ring_count < (uint32_t)vx_ctx::RING) c->ring_count++;
     HIPX(hipEventRecord(c->ev[0], st));
     if ((rc = fill_u32(c, counters, 0, CNT_LEN))) return rc;
     if (c->path == VX_PATH_PAIRS) {
@@ -496,6 +506,36 @@ int vx_timings_get(vx_ctx *c, vx_timings *t) {
     t->march2 = ms[4];
     t->shadow = ms[5];
     HIPX(hipEventElapsedTime(&t->total, c->ev[0], c->ev[6]));
+    return VX_OK;
+}
+
+int vx_timings_reset(vx_ctx *c) {
+    if (!c) return VX_E_ARG;
+    HIPX(hipSetDevice(c->device));
+    HIPX(hipStreamSynchronize(c->stream));
+    c->ring_count = 0;
+    return VX_OK;
+}
+
+int vx_timings_sum(vx_ctx *c, vx_timings *t, uint32_t *frames) {
+    if (!c || !t) return VX_E_ARG;
+    HIPX(hipSetDevice(c->device));
+    HIPX(hipStreamSynchronize(c->stream));
+    std::memset(t, 0, sizeof *t);
+    float *dst[6] = {&t->tile_select, &t->march, &t->hiz, &t->recover, &t->march2, &t->shadow};
+    for (uint32_t k = 0; k < c->ring_count; k++) {
+        uint32_t idx = (c->ring_next + vx_ctx::RING - 1 - k) % vx_ctx::RING;
+        hipEvent_t *e = c->ring[idx];
+        for (int i = 0; i < 6; i++) {
+            float ms = 0;
+            HIPX(hipEventElapsedTime(&ms, e[i], e[i + 1]));
+            *dst[i] += ms;
+        }
+        float tot = 0;
+        HIPX(hipEventElapsedTime(&tot, e[0], e[6]));
+        t->total += tot;
+    }
+    if (frames) *frames = c->ring_count;
     return VX_OK;
 }
 

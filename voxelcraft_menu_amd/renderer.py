@@ -124,6 +124,16 @@ class Renderer:
         A.check(self._L.vx_timings_get(self._c, C.byref(t)), "vx_timings_get")
         return {k: float(getattr(t, k)) for k, _ in A.Timings._fields_}
 
+    def timings_reset(self):
+        A.check(self._L.vx_timings_reset(self._c), "vx_timings_reset")
+
+    def timings_sum(self):
+        """(per-pass ms summed since timings_reset, frame count)."""
+        t = A.Timings()
+        n = C.c_uint32()
+        A.check(self._L.vx_timings_sum(self._c, C.byref(t), C.byref(n)), "vx_timings_sum")
+        return {k: float(getattr(t, k)) for k, _ in A.Timings._fields_}, int(n.value)
+
 
 def device_name() -> str:
     buf = C.create_string_buffer(256)
