@@ -87,6 +87,8 @@ def main():
     ap.add_argument("--path", choices=["tiles", "pairs"], default="tiles")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle work for cpu_baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run one 256 MiB device-to-device copy (scale reference for PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         _fail("bench.py measures one GPU: this build has no multi-GPU path (views would shard by GPU; "
@@ -98,6 +100,14 @@ def main():
     from voxelcraft_menu_amd import workloads as W
     from voxelcraft_menu_amd.renderer import Renderer, device_name
     from voxelcraft_menu_amd.world import build_view
+
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     wl = W.WORKLOADS[args.workload]
     world = W.make_world(wl)
@@ -127,8 +137,7 @@ def main():
     sums, nframes = r.timings_sum()
     dt = t1 - t0
     value = rays_per_step * args.steps / dt / 1e9
-    march_key = "march" if path == A.PATH_TILES else "march"
-    march_ms = sums[march_key] / max(nframes, 1)
+    march_ms = (sums["march"] + sums["march2"]) / max(nframes, 1)
     shadow_ms = sums["shadow"] / max(nframes, 1)
 
     out = {
@@ -170,25 +179,35 @@ def main():
 
     # Roofline of the dominant kernel: algorithmic bytes per launch (the oracle's exact
     # per-(ray, chunk) node accounting plus the 8-byte key per pixel and 4-byte Hi-Z per
-    # tile) over the kernel's average duration measured above with HIP events.
+    # tile; with the fused shadow also the shadow rays' node traffic and the 2-byte R16F
+    # word per pixel) over the kernel's average duration measured above with HIP events.
+    fused = path == A.PATH_TILES and (r.tuning()["variant"] & 4) != 0
     if prim is not None:
         tiles = view.tiles[0] * view.tiles[1]
-        alg = prim["node_bytes"] + 8 * wl.width * wl.height + 4 * tiles
-        achieved = alg / (march_ms * 1e-3) / 1e9
+        px = wl.width * wl.height
+        alg = prim["node_bytes"] + 8 * px + 4 * tiles
+        kernel_ms = march_ms
+        if fused:
+            alg += sh["node_bytes"] + 2 * px
+            kernel_ms = march_ms + shadow_ms
+        achieved = alg / (kernel_ms * 1e-3) / 1e9
         out["roofline"] = {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic_from_profiles("k_march_tiles", f"workload{args.workload}"),
+            "traffic": traffic_from_profiles("k_march_tiles", f"workload{args.workload}") if path == A.PATH_TILES
+            else None,
             "kernel": "k_march_tiles" if path == A.PATH_TILES else "k_march_pairs",
-            "kernel_ms": round(march_ms, 4),
+            "kernel_ms": round(kernel_ms, 4),
+            "fused_shadow": fused,
             "alg_bytes_per_launch": int(alg),
-            "alg_bytes_per_primary_ray": round(alg / (wl.width * wl.height), 2),
+            "alg_bytes_per_pixel": round(alg / px, 2),
         }
-        sh_alg = sh["node_bytes"] + 8 * wl.width * wl.height + 2 * wl.width * wl.height
-        out["roofline_shadow"] = {
-            "kernel": "k_shadow", "kernel_ms": round(shadow_ms, 4),
-            "achieved": round(sh_alg / (shadow_ms * 1e-3) / 1e9, 1), "unit": "GB/s",
-        }
+        if not fused:
+            sh_alg = sh["node_bytes"] + 8 * px + 2 * px
+            out["roofline_shadow"] = {
+                "kernel": "k_shadow", "kernel_ms": round(shadow_ms, 4),
+                "achieved": round(sh_alg / (shadow_ms * 1e-3) / 1e9, 1), "unit": "GB/s",
+            }
     else:
         out["roofline"] = None
     print(json.dumps(out), flush=True)

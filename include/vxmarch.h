@@ -163,7 +163,17 @@ typedef struct vx_hit {
     uint32_t shadow;      /* 1 if the sun ray was blocked */
 } vx_hit;
 
+/* Kernel variants chosen by the autotuner (no counterpart in the reference). */
+typedef struct vx_tuning {
+    uint32_t occupancy;       /* tile march built for 4 or 6 waves per SIMD */
+    uint32_t variant;         /* bit 0: classify the whole chunk list before marching;
+                                 bit 2: trace the sun-shadow ray in the tile's wave (fused) */
+    uint32_t ordered;         /* 1: launch tiles most expensive first (costs of the previous frame) */
+} vx_tuning;
+
 int vx_ctx_create(const vx_ctx_desc *desc, vx_ctx **out);
+int vx_set_tuning(vx_ctx *ctx, const vx_tuning *t);
+int vx_get_tuning(vx_ctx *ctx, vx_tuning *t);
 int vx_ctx_destroy(vx_ctx *ctx);
 int vx_resize(vx_ctx *ctx, uint32_t width, uint32_t height);
 int vx_pool_sync(vx_ctx *ctx, int pool, const void *data, size_t bytes, const vx_range *dirty, size_t n_dirty);

@@ -155,6 +155,10 @@ class CtxDesc(C.Structure):
                 ("path", C.c_uint32), ("device", C.c_int32)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("occupancy", C.c_uint32), ("variant", C.c_uint32), ("ordered", C.c_uint32)]
+
+
 class PairDemand(C.Structure):
     _fields_ = [("select", C.c_uint32), ("deferred", C.c_uint32), ("recover", C.c_uint32),
                 ("cap", C.c_uint32), ("truncated", C.c_uint32), ("shadow_rays", C.c_uint32)]
@@ -204,6 +208,8 @@ _sig_host = {
 _sig_march = {
     "vx_ctx_create": (C.c_int, [C.POINTER(CtxDesc), C.POINTER(P)]),
     "vx_ctx_destroy": (C.c_int, [P]),
+    "vx_set_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
+    "vx_get_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
     "vx_resize": (C.c_int, [P, C.c_uint32, C.c_uint32]),
     "vx_pool_sync": (C.c_int, [P, C.c_int, P, C.c_size_t, P, C.c_size_t]),
     "vx_set_chunks": (C.c_int, [P, P, C.c_uint32, P]),

@@ -19,13 +19,6 @@ namespace vxd {
 // 0: pairs to march   1: deferred (raw)   2: recover count   3: select (min cap)
 // 4: select (raw)     5: recover (raw)    6: shadow rays      7: truncated flag
 
-// blockIdx -> work block so that consecutive work blocks land on one XCD (one L2):
-// the dispatcher hands block b to XCD b % 8; grids are padded to a multiple of 8.
-__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
-    uint32_t per = nb >> 3;
-    return (b & 7u) * per + (b >> 3);
-}
-
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -37,15 +30,6 @@ __global__ void k_fill_u64(uint64_t *p, uint64_t v, size_t n) {
     size_t stride = (size_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) p[i] = v;
 }
-
-__device__ __forceinline__ uint64_t ballot(bool v) { return __ballot(v); }
-
-// Counters are produced by earlier kernels of the same frame; read them with an
-// agent-scope atomic load so the value comes from L2, never from a scalar-cache line.
-__device__ __forceinline__ uint32_t ld_counter(const uint32_t *c, uint32_t i) {
-    return __hip_atomic_load(c + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 
 // ---------------------------------------------------------------- pairs path
 // tile_select (tile_select.wgsl:11-92): one thread per tile; a wave's lanes walk the
@@ -113,9 +97,11 @@ __global__ __launch_bounds__(256) void k_march_pairs(Pools p, FrameK f, Spec s, 
         if (px >= f.res[0] || py >= f.res[1] || ci >= p.n_chunks) continue;
         ChunkK c = ld_chunk(p, ci);
         if (ray_skips(f, c.fade, px, py)) continue;
-        float rd[3];
+        float rd[3], inv[3];
         ray_dir(f, (float)px + 0.5f, (float)py + 0.5f, rd[0], rd[1], rd[2]);
-        Hit h = march_chunk(p, f, s, c, f.cam, rd, f.far_, underwater, false, false);
+        for (int i = 0; i < 3; i++) inv[i] = 1.0f / rd[i];
+        Hit h = underwater ? march_chunk<true, false, false>(p, f, s, c, f.cam, rd, inv, f.far_)
+                           : march_chunk<false, false, false>(p, f, s, c, f.cam, rd, inv, f.far_);
         if (h.hit) atomicMax(&vis[(size_t)py * f.res[0] + px], (unsigned long long)make_key(f.far_, h, ci));
     }
 }
@@ -172,124 +158,6 @@ __global__ void k_finalize_recover(FrameK f, uint32_t *counters) {
     if (raw > f.max_pairs) counters[7] = 1u;
 }
 
-// ---------------------------------------------------------------- tiles path
-// Conservative lower bound of the depth key a chunk at box distance dmin can produce:
-// every hit point lies in the chunk's box, so its distance is at least dmin; the
-// margin covers f32 rounding of the march's entry and step arithmetic.
-__device__ __forceinline__ uint32_t prune_key(float far_, float dmin) {
-    float lo = dmin - (0.25f + 1e-3f * dmin);
-    return depth_key(far_, fmaxf(lo, 0.0f));
-}
-
-template <bool kUseHiz>
-__device__ __forceinline__ void march_tile_body(const Pools &p, const FrameK &f, const Spec &s, uint32_t tile,
-                                                uint32_t *__restrict__ hiz, unsigned long long *__restrict__ vis,
-                                                uint32_t *__restrict__ counters, uint64_t *defmask) {
-    const uint32_t lane = lane_id();
-    const uint32_t gx = tile % f.tiles[0], gy = tile / f.tiles[0];
-    const uint32_t px = gx * 8u + (lane & 7u), py = gy * 8u + (lane >> 3);
-    const bool in_frame = px < f.res[0] && py < f.res[1];
-    const bool underwater = (f.flags & VX_FLAG_UNDERWATER) != 0u;
-    float rd[3];
-    ray_dir(f, (float)px + 0.5f, (float)py + 0.5f, rd[0], rd[1], rd[2]);
-    TileCull tc = tile_cull_setup(f, gx, gy);
-    const uint32_t tile_far = kUseHiz ? hiz[tile] : 0u;
-    uint64_t best = 0;
-    uint32_t n_sel = 0, n_def = 0, n_rec = 0;
-    const uint32_t nwords = (f.chunk_count + 63u) >> 6;
-
-    auto march_one = [&](uint32_t cj, uint32_t pk) {
-        // pk: prune key of chunk cj; a lane whose key already beats every hit the chunk
-        // can produce (pk < best depth) does not march it.
-        bool need = in_frame && pk >= (uint32_t)(best >> 40);
-        if (!ballot(need)) return;
-        ChunkK c = ld_chunk(p, cj);
-        if (need && !ray_skips(f, c.fade, px, py)) {
-            Hit h = march_chunk(p, f, s, c, f.cam, rd, f.far_, underwater, false, false);
-            if (h.hit) {
-                uint64_t k = make_key(f.far_, h, cj);
-                best = k > best ? k : best;
-            }
-        }
-    };
-
-    for (uint32_t w = 0; w < nwords; w++) {
-        uint32_t ci = (w << 6) + lane;
-        uint32_t cls = 0;
-        float dmin = 0.0f;
-        if (ci < f.chunk_count && ci < p.n_chunks) {
-            ChunkK c = ld_chunk(p, ci);
-            cls = classify(f, tc, c.bmin, c.bmax, ci, tile_far, kUseHiz, dmin);
-        }
-        uint32_t pk = prune_key(f.far_, dmin);
-        uint64_t sel = ballot(cls == 1u);
-        uint64_t def = ballot(cls == 2u);
-        n_sel += (uint32_t)__popcll(sel);
-        if (kUseHiz) {
-            n_def += (uint32_t)__popcll(def);
-            if (lane == 0) defmask[w] = def;
-        }
-        while (sel) {
-            uint32_t j = (uint32_t)__ffsll((long long)sel) - 1u;
-            sel &= sel - 1ull;
-            march_one((w << 6) + j, (uint32_t)__shfl((int)pk, (int)j));
-        }
-    }
-    if (kUseHiz) {
-        // defmask is written by lane 0 and read by every lane of this same wave only;
-        // LDS operations of one wave complete in order.
-        __builtin_amdgcn_wave_barrier();
-        // build_hiz on this tile: minimum 24-bit depth, 0 for pixels off the frame.
-        uint32_t d = in_frame ? (uint32_t)(best >> 40) : 0u;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) d = min(d, (uint32_t)__shfl_xor((int)d, o));
-        if (lane == 0) hiz[tile] = d;
-        // recover_select + the second march.
-        for (uint32_t w = 0; w < nwords; w++) {
-            uint64_t def = defmask[w];
-            if (!def) continue;
-            uint32_t ci = (w << 6) + lane;
-            bool rec = false;
-            uint32_t pk = 0;
-            if ((def >> lane) & 1ull) {
-                ChunkK c = ld_chunk(p, ci);
-                float dmin = box_distance(f.cam, c.bmin, c.bmax);
-                rec = depth_key(f.far_, dmin) >= d;
-                pk = prune_key(f.far_, dmin);
-            }
-            uint64_t r = ballot(rec);
-            n_rec += (uint32_t)__popcll(r);
-            while (r) {
-                uint32_t j = (uint32_t)__ffsll((long long)r) - 1u;
-                r &= r - 1ull;
-                march_one((w << 6) + j, (uint32_t)__shfl((int)pk, (int)j));
-            }
-        }
-    } else {
-        if (lane == 0) hiz[tile] = 0u;
-    }
-    if (in_frame) vis[(size_t)py * f.res[0] + px] = best;
-    if (lane == 0) {
-        atomicAdd(&counters[4], n_sel);
-        if (kUseHiz) {
-            atomicAdd(&counters[1], n_def);
-            atomicAdd(&counters[5], n_rec);
-        }
-    }
-}
-
-__global__ __launch_bounds__(256) void k_march_tiles(Pools p, FrameK f, Spec s, uint32_t *__restrict__ hiz,
-                                                     unsigned long long *__restrict__ vis,
-                                                     uint32_t *__restrict__ counters) {
-    __shared__ uint64_t defmask[4][128];
-    uint32_t b = xcd_block(blockIdx.x, gridDim.x);
-    uint32_t tile = b * 4u + (threadIdx.x >> 6);
-    if (tile >= f.tiles[0] * f.tiles[1]) return;  // whole wave exits together
-    uint64_t *dm = defmask[threadIdx.x >> 6];
-    if (f.flags & VX_FLAG_HIZ) march_tile_body<true>(p, f, s, tile, hiz, vis, counters, dm);
-    else march_tile_body<false>(p, f, s, tile, hiz, vis, counters, dm);
-}
-
 // ---------------------------------------------------------------- shadow
 // primary_shadow (shadow.wgsl:34-48): a wave covers one 8x8 tile.
 __global__ __launch_bounds__(256) void k_shadow(Pools p, FrameK f, Spec s, const unsigned long long *__restrict__ vis,
@@ -305,7 +173,7 @@ __global__ __launch_bounds__(256) void k_shadow(Pools p, FrameK f, Spec s, const
         out[i] = shadow_pixel(p, f, s, vis[i], px, py, traced);
     }
     uint64_t t = ballot(traced);
-    if (lane == 0 && t) atomicAdd(&counters[6], (uint32_t)__popcll(t));
+    if (lane == 0) stat_add(counters, tile, STAT_SHADOW, (uint32_t)__popcll(t));
 }
 
 // ---------------------------------------------------------------- decode

@@ -88,6 +88,11 @@ __device__ __forceinline__ ChunkK ld_chunk(const Pools &p, uint32_t ci) {
     return c;
 }
 
+// x / vs with the division skipped when vs is exactly 1.0 (x / 1.0f == x in IEEE f32,
+// so the result is bit-identical); LOD-0 chunks always have vs == 1.
+__device__ __forceinline__ float div_vs(float x, float vs) { return vs == 1.0f ? x : x / vs; }
+__device__ __forceinline__ float mul_vs(float x, float vs) { return vs == 1.0f ? x : x * vs; }
+
 // mask_has / mask_below / cell_bit (common.wgsl:299-318) on a u64 mask.
 __device__ __forceinline__ bool mask_has(uint64_t m, uint32_t bit) { return (m >> bit) & 1ull; }
 __device__ __forceinline__ uint32_t mask_below(uint64_t m, uint32_t bit) {
@@ -160,9 +165,9 @@ __device__ inline uint32_t world_block_at(const Pools &p, const FrameK &f, float
     uint32_t ci = ld_grid(p, f, gi);
     if (ci == NO_CHUNK || ci >= p.n_chunks) return 0u;
     ChunkK c = ld_chunk(p, ci);
-    int32_t vx = (int32_t)floorf((px - c.origin[0]) / c.vs);
-    int32_t vy = (int32_t)floorf((py - c.origin[1]) / c.vs);
-    int32_t vz = (int32_t)floorf((pz - c.origin[2]) / c.vs);
+    int32_t vx = (int32_t)floorf(div_vs(px - c.origin[0], c.vs));
+    int32_t vy = (int32_t)floorf(div_vs(py - c.origin[1], c.vs));
+    int32_t vz = (int32_t)floorf(div_vs(pz - c.origin[2], c.vs));
     if (vx < 0 || vy < 0 || vz < 0 || vx >= 64 || vy >= 64 || vz >= 64) return 0u;
     if (!voxel_occupied(p, c, vx, vy, vz)) return 0u;
     return block_at(p, c, (uint32_t)vx, (uint32_t)vy, (uint32_t)vz);
@@ -301,17 +306,23 @@ struct Hit {
 
 // march_chunk (common.wgsl:579-910). Foliage tufts are not handled here: the
 // context refuses SPEC_FOLIAGE, so foliage_aware is false on every call.
+// inv is the ray's 1.0 / rd (the same value march_chunk's own `let inv = 1.0 / rd`
+// computes for every chunk; callers compute it once per ray). With kLdsL1 the L1 nodes
+// of a non-full root are read from l1s, a copy of the chunk's inner run staged in LDS.
+// The ray kind (skip_water, skip_glass, sun_ray) is a compile-time parameter: primary
+// rays use <false,false,false> (or <true,false,false> under water), sun rays
+// <true,true,true>, so each kernel carries only the branches its rays can take.
+template <bool skip_water, bool skip_glass, bool sun_ray, bool kLdsL1 = false>
 __device__ inline Hit march_chunk(const Pools &p, const FrameK &f, const Spec &s, const ChunkK &c, const float ro_w[3],
-                                  const float rd[3], float t_limit, bool skip_water, bool skip_glass, bool sun_ray) {
+                                  const float rd[3], const float inv[3], float t_limit, const uint4 *l1s = nullptr) {
     Hit h{false, 0.0f, 0u, 0u, 0u, 0u};
     const float vs = c.vs;
-    float ro[3], inv[3], bmin[3], bmax[3], tmin3[3], tmax3[3];
+    float ro[3], bmin[3], bmax[3], tmin3[3], tmax3[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        ro[i] = (ro_w[i] - c.origin[i]) / vs;
-        inv[i] = 1.0f / rd[i];
-        bmin[i] = (c.bmin[i] - c.origin[i]) / vs;
-        bmax[i] = (c.bmax[i] - c.origin[i]) / vs;
+        ro[i] = div_vs(ro_w[i] - c.origin[i], vs);
+        bmin[i] = div_vs(c.bmin[i] - c.origin[i], vs);
+        bmax[i] = div_vs(c.bmax[i] - c.origin[i], vs);
         float t0 = (bmin[i] - ro[i]) * inv[i];
         float t1 = (bmax[i] - ro[i]) * inv[i];
         tmin3[i] = fminf(t0, t1);
@@ -319,7 +330,7 @@ __device__ inline Hit march_chunk(const Pools &p, const FrameK &f, const Spec &s
     }
     const float t_enter = fmaxf(fmaxf(tmin3[0], tmin3[1]), tmin3[2]);
     const float t_exit = fminf(fminf(tmax3[0], tmax3[1]), tmax3[2]);
-    const float t_end = fminf(t_exit, t_limit / vs);
+    const float t_end = fminf(t_exit, div_vs(t_limit, vs));
     float t = fmaxf(t_enter, 0.0f);
     if (t >= t_end) return h;
     uint32_t axis;
@@ -372,7 +383,7 @@ __device__ inline Hit march_chunk(const Pools &p, const FrameK &f, const Spec &s
     const bool root_full = (c.root[2] & FULL_BIT) != 0u;
     if (root_full && !(s.bits & SPEC_FULL_MARCH) && !noncaster_aware && !water_interface) {
         h.hit = true;
-        h.t = t * vs;
+        h.t = mul_vs(t, vs);
         h.normal = axis * 2u + (pd[axis] ? 0u : 1u);
         h.voxel = (uint32_t)cell[0] | ((uint32_t)cell[1] << 6) | ((uint32_t)cell[2] << 12);
         h.micro = micro_entry((uint32_t)cell[0], (uint32_t)cell[1], (uint32_t)cell[2], ro, rd, t, axis, pd, t > 0.0f);
@@ -397,6 +408,7 @@ __device__ inline Hit march_chunk(const Pools &p, const FrameK &f, const Spec &s
             if (l1b != l1_cached) {
                 l1_cached = l1b;
                 if (root_full) l1 = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, FULL_BIT, (l1b * 64u) | 0xF0000000u);
+                else if (kLdsL1) l1 = l1s[mask_below(root_mask, l1b)];
                 else l1 = ld_inner(p, root_ptr + mask_below(root_mask, l1b));
                 leaf_cached = 0xFFFFFFFFu;
             }
@@ -463,8 +475,8 @@ __device__ inline Hit march_chunk(const Pools &p, const FrameK &f, const Spec &s
                 if (axis == 0) o[0] = sgn;
                 else if (axis == 1) o[1] = sgn;
                 else o[2] = sgn;
-                float tw = t * vs;
-                float off = 0.002f * vs;
+                float tw = mul_vs(t, vs);
+                float off = mul_vs(0.002f, vs);
                 float b0 = ro_w[0] + rd[0] * tw, b1 = ro_w[1] + rd[1] * tw, b2 = ro_w[2] + rd[2] * tw;
                 if (world_block_at(p, f, b0 + o[0] * off, b1 + o[1] * off, b2 + o[2] * off) == WATER_ID) {
                     solid = false;
@@ -492,7 +504,7 @@ __device__ inline Hit march_chunk(const Pools &p, const FrameK &f, const Spec &s
                 MicroHit q = cutout_march(s.leaf_fill, wx, wy, wz, ux, uy, uz, ro, rd, inv, t, axis, pd, t > 0.0f);
                 if (q.hit) {
                     h.hit = true;
-                    h.t = q.t * vs;
+                    h.t = mul_vs(q.t, vs);
                     h.normal = q.normal;
                     h.voxel = ux | (uy << 6) | (uz << 12);
                     h.micro = q.micro;
@@ -505,7 +517,7 @@ __device__ inline Hit march_chunk(const Pools &p, const FrameK &f, const Spec &s
         }
         if (solid) {
             h.hit = true;
-            h.t = t * vs;
+            h.t = mul_vs(t, vs);
             h.normal = axis * 2u + (pd[axis] ? 0u : 1u);
             h.voxel = ux | (uy << 6) | (uz << 12);
             h.micro = micro_entry(ux, uy, uz, ro, rd, t, axis, pd, t > 0.0f);
@@ -573,7 +585,7 @@ __device__ inline float shadow_ray_t(const Pools &p, const FrameK &f, const Spec
             last = ci;
             if (ci < p.n_chunks) {
                 ChunkK c = ld_chunk(p, ci);
-                Hit m = march_chunk(p, f, s, c, ro, rd, max_dist, true, true, true);
+                Hit m = march_chunk<true, true, true>(p, f, s, c, ro, rd, inv, max_dist);
                 if (m.hit) return m.t;
             }
         }

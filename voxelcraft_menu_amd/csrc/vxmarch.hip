@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -60,6 +61,8 @@ struct vx_ctx {
     size_t pool_len[3] = {0, 0, 0};
     bool pool_synced[3] = {false, false, false};
     DevBuf chunks, grid, vis, shadow, hiz, pairs, counters;
+    DevBuf cost, order;         // per-tile cycles of the last frame, and the launch order
+    bool have_cost = false;
     uint32_t n_chunks = 0;
     // Ring of per-frame event sets: [0] start, [1..5] pass boundaries, [6] end.
     static constexpr int RING = 256;
@@ -67,6 +70,9 @@ struct vx_ctx {
     uint32_t ring_next = 0;     // set the next frame records into
     uint32_t ring_count = 0;    // frames recorded since the last reset (saturates at RING)
     hipEvent_t *ev = nullptr;   // the set of the frame being recorded / last recorded
+    uint32_t occ = 4;      // tile-march occupancy variant (waves per SIMD the kernel is built for)
+    uint32_t variant = VAR_FUSED_SHADOW;  // tile-march structure variant (VAR_* bits)
+    uint32_t ordered = 1;  // launch tiles most-expensive-first (1) or in raster order (0)
     bool have_frame = false;
     FrameK frame{};
     Spec spec{};
@@ -109,11 +115,59 @@ static int alloc_targets(vx_ctx *c, uint32_t w, uint32_t h) {
         (rc = ensure(c->hiz, (size_t)tx * ty * 4)))
         return rc;
     if (c->path == VX_PATH_PAIRS && (rc = ensure(c->pairs, (size_t)c->max_pairs * 2 * 4))) return rc;
+    if ((rc = ensure(c->cost, (size_t)tx * ty * 4)) || (rc = ensure(c->order, (size_t)tx * ty * 4))) return rc;
+    c->have_cost = false;
     if ((rc = fill_u64(c, c->vis.p, 0, (size_t)w * h)) || (rc = fill_u32(c, c->hiz.p, 0, (size_t)tx * ty))) return rc;
     if ((rc = fill_u32(c, c->shadow.p, 0, ((size_t)w * h + 1) / 2))) return rc;
     HIPX(hipStreamSynchronize(c->stream));
     c->have_frame = false;
     return VX_OK;
+}
+
+static unsigned pad8(unsigned n) { return (n + 7u) & ~7u; }
+
+struct TileArgs {
+    Pools p;
+    FrameK f;
+    Spec s;
+    uint32_t *hiz;
+    unsigned long long *vis;
+    uint16_t *shadow;
+    uint32_t *cnt;
+    const uint32_t *order;
+    uint32_t *cost;
+};
+
+template <int O, int V>
+static void launch_tiles_ov(bool hz, bool uw, unsigned nb, hipStream_t st, const TileArgs &a) {
+    if (hz && !uw)
+        k_march_tiles<O, V, true, false><<<nb, 64, 0, st>>>(a.p, a.f, a.s, a.hiz, a.vis, a.shadow, a.cnt, a.order, a.cost);
+    else if (!hz && !uw)
+        k_march_tiles<O, V, false, false><<<nb, 64, 0, st>>>(a.p, a.f, a.s, a.hiz, a.vis, a.shadow, a.cnt, a.order, a.cost);
+    else if (hz)
+        k_march_tiles<O, V, true, true><<<nb, 64, 0, st>>>(a.p, a.f, a.s, a.hiz, a.vis, a.shadow, a.cnt, a.order, a.cost);
+    else
+        k_march_tiles<O, V, false, true><<<nb, 64, 0, st>>>(a.p, a.f, a.s, a.hiz, a.vis, a.shadow, a.cnt, a.order, a.cost);
+}
+
+// Valid (occupancy, variant) pairs compiled into the library.
+static bool tuning_ok(uint32_t occ, uint32_t var) {
+    if (occ == 4) return var == 0 || var == 1 || var == 4 || var == 5;
+    if (occ == 6) return var == 0 || var == 4;
+    return false;
+}
+
+// One workgroup (one wave) per tile, launched in `order`.
+static void launch_tiles(uint32_t occ, uint32_t var, bool hz, bool uw, unsigned nb, hipStream_t st,
+                         const TileArgs &a) {
+    switch (occ * 16 + var) {
+        case 4 * 16 + 1: launch_tiles_ov<4, 1>(hz, uw, nb, st, a); break;
+        case 4 * 16 + 4: launch_tiles_ov<4, 4>(hz, uw, nb, st, a); break;
+        case 4 * 16 + 5: launch_tiles_ov<4, 5>(hz, uw, nb, st, a); break;
+        case 6 * 16 + 0: launch_tiles_ov<6, 0>(hz, uw, nb, st, a); break;
+        case 6 * 16 + 4: launch_tiles_ov<6, 4>(hz, uw, nb, st, a); break;
+        default: launch_tiles_ov<4, 0>(hz, uw, nb, st, a); break;
+    }
 }
 
 extern "C" {
@@ -167,6 +221,14 @@ int vx_ctx_create(const vx_ctx_desc *d, vx_ctx **out) {
         c->ev = c->ring[0];
         c->path = d->path;
         c->max_pairs_req = d->max_pairs;
+        if (const char *e = std::getenv("VX_TILES_OCC")) {
+            int v = std::atoi(e);
+            if (v == 4 || v == 6) c->occ = (uint32_t)v;
+        }
+        if (const char *e = std::getenv("VX_TILES_VARIANT")) {
+            int v = std::atoi(e);
+            if (v >= 0 && tuning_ok(c->occ, (uint32_t)v)) c->variant = (uint32_t)v;
+        }
         for (auto &b : c->pool)
             if ((rc = ensure(b, MIN_CAP))) break;
         if (rc) break;
@@ -189,8 +251,8 @@ int vx_ctx_destroy(vx_ctx *c) {
     if (!c) return VX_E_ARG;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
-    DevBuf *bufs[] = {&c->pool[0], &c->pool[1], &c->pool[2], &c->chunks, &c->grid, &c->vis,
-                      &c->shadow,  &c->hiz,     &c->pairs,   &c->counters};
+    DevBuf *bufs[] = {&c->pool[0], &c->pool[1], &c->pool[2], &c->chunks, &c->grid, &c->vis,   &c->shadow,
+                      &c->hiz,     &c->pairs,   &c->counters, &c->cost,  &c->order};
     for (DevBuf *b : bufs)
         if (b->p) hipFree(b->p);
     for (auto &set : c->ring)
@@ -340,8 +402,6 @@ static int make_spec(uint32_t spec, uint32_t spec_hi, uint32_t frame_flags, Spec
     return VX_OK;
 }
 
-static unsigned pad8(unsigned n) { return (n + 7u) & ~7u; }
-
 int vx_render(vx_ctx *c, const vx_gpu_frame *g, uint32_t spec, uint32_t spec_hi) {
     if (!c || !g) return VX_E_ARG;
     if (g->res[0] != c->w || g->res[1] != c->h) return VX_E_ARG;
@@ -400,15 +460,23 @@ int vx_render(vx_ctx *c, const vx_gpu_frame *g, uint32_t spec, uint32_t spec_hi)
         HIPX(hipEventRecord(c->ev[5], st));
     } else {
         HIPX(hipEventRecord(c->ev[1], st));
-        k_march_tiles<<<pad8((ntiles + 3) / 4), 256, 0, st>>>(p, f, s, hiz, vis, counters);
+        TileArgs a{p, f, s, hiz, vis, (uint16_t *)c->shadow.p, counters, (const uint32_t *)c->order.p,
+                   (uint32_t *)c->cost.p};
+        if (c->have_cost && c->ordered) k_order_tiles<<<1, 1024, 0, st>>>((const uint32_t *)c->cost.p, (uint32_t *)c->order.p, ntiles);
+        else k_iota<<<(ntiles + 255) / 256, 256, 0, st>>>((uint32_t *)c->order.p, ntiles);
         LAUNCHED();
+        launch_tiles(c->occ, c->variant, hiz_on, (f.flags & VX_FLAG_UNDERWATER) != 0u, ntiles, st, a);
+        LAUNCHED();
+        c->have_cost = true;
         HIPX(hipEventRecord(c->ev[2], st));
         HIPX(hipEventRecord(c->ev[3], st));
         HIPX(hipEventRecord(c->ev[4], st));
         HIPX(hipEventRecord(c->ev[5], st));
     }
-    k_shadow<<<pad8((ntiles + 3) / 4), 256, 0, st>>>(p, f, s, vis, (uint16_t *)c->shadow.p, counters);
-    LAUNCHED();
+    if (c->path == VX_PATH_PAIRS || !(c->variant & VAR_FUSED_SHADOW)) {
+        k_shadow<<<pad8((ntiles + 3) / 4), 256, 0, st>>>(p, f, s, vis, (uint16_t *)c->shadow.p, counters);
+        LAUNCHED();
+    }
     HIPX(hipEventRecord(c->ev[6], st));
     HIPX(hipGetLastError());
     c->frame = f;
@@ -449,7 +517,7 @@ int vx_read_pairs(vx_ctx *c, uint32_t *out, size_t n, uint32_t *counters16) {
     if (!c || (!out && n) || c->path != VX_PATH_PAIRS || n > (size_t)c->max_pairs * 2) return VX_E_ARG;
     int rc = VX_OK;
     if (n && (rc = read_dev(c, c->pairs, out, n * 4))) return rc;
-    if (counters16) rc = read_dev(c, c->counters, counters16, CNT_LEN * 4);
+    if (counters16) rc = read_dev(c, c->counters, counters16, 16 * 4);
     return rc;
 }
 
@@ -481,15 +549,18 @@ int vx_read_decoded(vx_ctx *c, vx_hit *out, size_t n) {
 
 int vx_pair_demand_get(vx_ctx *c, vx_pair_demand *out) {
     if (!c || !out) return VX_E_ARG;
-    uint32_t cnt[CNT_LEN];
-    int rc = read_dev(c, c->counters, cnt, sizeof cnt);
+    std::vector<uint32_t> cnt(CNT_LEN);
+    int rc = read_dev(c, c->counters, cnt.data(), CNT_LEN * 4);
     if (rc) return rc;
-    out->select = cnt[CNT_SELECT_RAW];
-    out->deferred = cnt[CNT_DEFERRED];
-    out->recover = cnt[CNT_RECOVER_RAW];
+    uint64_t st[4] = {0, 0, 0, 0};
+    for (uint32_t sl = 0; sl < CNT_STAT_SLOTS; sl++)
+        for (uint32_t k = 0; k < 4; k++) st[k] += cnt[CNT_STAT_BASE + sl * CNT_STAT_STRIDE + k];
+    out->select = cnt[CNT_SELECT_RAW] + (uint32_t)st[STAT_SELECT];
+    out->deferred = cnt[CNT_DEFERRED] + (uint32_t)st[STAT_DEFERRED];
+    out->recover = cnt[CNT_RECOVER_RAW] + (uint32_t)st[STAT_RECOVER];
     out->cap = c->max_pairs;
     out->truncated = cnt[CNT_TRUNC];
-    out->shadow_rays = cnt[CNT_SHADOW];
+    out->shadow_rays = (uint32_t)st[STAT_SHADOW];
     return VX_OK;
 }
 
@@ -506,6 +577,24 @@ int vx_timings_get(vx_ctx *c, vx_timings *t) {
     t->march2 = ms[4];
     t->shadow = ms[5];
     HIPX(hipEventElapsedTime(&t->total, c->ev[0], c->ev[6]));
+    return VX_OK;
+}
+
+int vx_set_tuning(vx_ctx *c, const vx_tuning *t) {
+    if (!c || !t) return VX_E_ARG;
+    if (!tuning_ok(t->occupancy, t->variant)) return VX_E_ARG;
+    if (t->ordered > 1) return VX_E_ARG;
+    c->occ = t->occupancy;
+    c->variant = t->variant;
+    c->ordered = t->ordered;
+    return VX_OK;
+}
+
+int vx_get_tuning(vx_ctx *c, vx_tuning *t) {
+    if (!c || !t) return VX_E_ARG;
+    t->occupancy = c->occ;
+    t->variant = c->variant;
+    t->ordered = c->ordered;
     return VX_OK;
 }
 

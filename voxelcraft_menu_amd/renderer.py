@@ -42,6 +42,14 @@ class Renderer:
     def handle(self):
         return self._c
 
+    def set_tuning(self, occupancy: int = 4, variant: int = 0, ordered: int = 1):
+        A.check(self._L.vx_set_tuning(self._c, C.byref(A.Tuning(occupancy, variant, ordered))), "vx_set_tuning")
+
+    def tuning(self) -> dict:
+        t = A.Tuning()
+        A.check(self._L.vx_get_tuning(self._c, C.byref(t)), "vx_get_tuning")
+        return {"occupancy": int(t.occupancy), "variant": int(t.variant), "ordered": int(t.ordered)}
+
     def resize(self, width: int, height: int):
         A.check(self._L.vx_resize(self._c, width, height), "vx_resize")
         self.size = (width, height)
