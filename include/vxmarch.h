@@ -199,6 +199,63 @@ int vx_timings_reset(vx_ctx *ctx);
 int vx_timings_sum(vx_ctx *ctx, vx_timings *out, uint32_t *frames);
 /* Device pointers of the current outputs (for callers that keep data on the GPU). */
 int vx_device_outputs(vx_ctx *ctx, uint64_t **vis, uint16_t **shadow);
+
+/* ---- View batches -----------------------------------------------------------------
+ * No counterpart in the reference, which renders one view per Renderer::render call and
+ * waits on it (src/headless.rs:680-837, :419-497). A batch renders many views of the
+ * context's world in one submission. It shares the context's pools and stream (the
+ * context must outlive it; vx_ctx_destroy returns VX_E_BUSY while batches exist), and
+ * each view keeps its own Hi-Z, visibility and shadow buffers. Per view, the output bits
+ * are those vx_render gives for the same frame, chunk list, grid and Hi-Z input. */
+typedef struct vx_view {
+    const vx_gpu_frame *frame;   /* the view's GpuFrame (res/tiles give its size) */
+    const vx_gpu_chunk *chunks;  /* its chunk list, n_total records (build_chunk_list order) */
+    uint32_t n_total;
+    const uint32_t *grid;        /* its 32x8x32 chunk grid */
+    uint32_t spec, spec_hi;      /* pipeline key, as for vx_render */
+} vx_view;
+
+/* PLAIN: one dispatch per view (the reference's pattern, kept as the baseline form);
+ * FLAT: one launch, one workgroup per tile of every view, most expensive tiles first;
+ * PERSISTENT: one launch of resident waves taking tiles from a global queue. */
+enum { VX_BATCH_PERSISTENT = 0, VX_BATCH_FLAT = 1, VX_BATCH_PLAIN = 2 };
+
+typedef struct vx_batch_tuning {
+    uint32_t mode;            /* VX_BATCH_* */
+    uint32_t occupancy;       /* waves per SIMD the march is built for: 4 or 6 */
+    uint32_t variant;         /* vx_tuning.variant bits; batched forms need the fused shadow (4, or 5 at occupancy 4) */
+    uint32_t ordered;         /* 1: tiles most expensive first (their cost in the previous frame) */
+    uint32_t queue_chunk;     /* PERSISTENT: tiles taken per queue grab, 1..64 */
+    uint32_t waves_per_simd;  /* PERSISTENT: grid = CUs * 4 * waves_per_simd, 1..occupancy */
+    uint32_t graph;           /* 1: capture the submission once into a hipGraph and replay it */
+} vx_batch_tuning;
+
+typedef struct vx_batch vx_batch;
+
+int vx_batch_create(vx_ctx *ctx, uint32_t max_views, vx_batch **out);
+int vx_batch_destroy(vx_batch *b);
+/* Replaces the view set (1..max_views views). Targets, Hi-Z and tile costs start at 0,
+ * as for a new Renderer. */
+int vx_batch_set_views(vx_batch *b, const vx_view *views, uint32_t n);
+int vx_batch_set_tuning(vx_batch *b, const vx_batch_tuning *t);
+int vx_batch_get_tuning(vx_batch *b, vx_batch_tuning *t);
+/* Enqueues one frame of every view on the context stream (asynchronous). */
+int vx_batch_render(vx_batch *b);
+int vx_batch_sync(vx_batch *b);
+/* Times each batched form for reps frames and keeps the fastest (the autotuner). */
+int vx_batch_autotune(vx_batch *b, uint32_t reps, vx_batch_tuning *best, float *best_ms);
+int vx_batch_reset_hiz(vx_batch *b);
+int vx_batch_read_vis(vx_batch *b, uint32_t view, uint64_t *out, size_t n);
+int vx_batch_read_shadow(vx_batch *b, uint32_t view, uint16_t *out, size_t n);
+int vx_batch_read_hiz(vx_batch *b, uint32_t view, uint32_t *out, size_t n);
+/* Totals over the views of the last frame (select/deferred/recover/shadow_rays). */
+int vx_batch_demand_get(vx_batch *b, vx_pair_demand *out);
+/* GPU milliseconds summed over the submissions since vx_batch_timings_reset (the last
+ * 256 at most): the whole submission, and the march kernel alone (FLAT / PERSISTENT
+ * without graph replay; otherwise the same span as the total). */
+int vx_batch_timings_reset(vx_batch *b);
+int vx_batch_timings_sum(vx_batch *b, float *total_ms, float *march_ms, uint32_t *frames);
+int vx_batch_device_outputs(vx_batch *b, uint32_t view, uint64_t **vis, uint16_t **shadow);
 const char *vx_error_string(int code);
 int vx_device_name(char *buf, size_t cap);
 

@@ -1,0 +1,185 @@
+"""GPU parity of view batches (vx_batch_*): every launch form against the oracle and
+against one-view-at-a-time rendering, bit for bit.
+
+A batch view's keys, shadow words and Hi-Z words must equal what the oracle gives for
+that view alone with the same Hi-Z input; the forms (PLAIN per-view dispatches, FLAT,
+PERSISTENT queue, each with and without hipGraph replay) must agree with each other.
+"""
+import numpy as np
+import pytest
+
+from tests import helpers as H
+from voxelcraft_menu_amd import _abi as A
+from voxelcraft_menu_amd import workloads as W
+from voxelcraft_menu_amd.renderer import Batch, Renderer
+from voxelcraft_menu_amd.world import build_view
+
+pytestmark = pytest.mark.gpu
+
+FORMS = [
+    dict(mode=A.BATCH_PLAIN, graph=0),
+    dict(mode=A.BATCH_PLAIN, graph=1, ordered=0),
+    dict(mode=A.BATCH_PLAIN, graph=0, variant=0),
+    dict(mode=A.BATCH_FLAT, graph=0),
+    dict(mode=A.BATCH_FLAT, graph=1, occupancy=6),
+    dict(mode=A.BATCH_PERSISTENT, graph=0, queue_chunk=1),
+    dict(mode=A.BATCH_PERSISTENT, graph=1, queue_chunk=3, waves_per_simd=2),
+    dict(mode=A.BATCH_PERSISTENT, graph=1, variant=5),
+    dict(mode=A.BATCH_PERSISTENT, graph=1, ordered=0, occupancy=6, queue_chunk=8),
+]
+
+
+def _form_id(f):
+    return "-".join(f"{k}{v}" for k, v in f.items())
+
+
+@pytest.fixture(scope="module")
+def small_world():
+    return W.make_world(W.WORKLOADS[0])
+
+
+@pytest.fixture(scope="module")
+def small_views(small_world):
+    wl = W.WORKLOADS[0]
+    flags = W.DEFAULT_FLAGS
+    cams = W.random_cameras(wl, 5, seed=31, resolutions=[(333, 197), (64, 64), (120, 250), (40, 24)])
+    h = wl.gen.height(200, 40)
+    cams.append(H.camera((200.5, min(h, 127) - 0.5, 40.5), 120.0, 10.0, 96, 72, flags | A.FLAG_UNDERWATER))
+    cams.append(H.camera((60.5, 120.3, 70.5), 30.0, -5.0, 72, 40, flags))
+    cams.append(H.camera((128.5, 150.0, 128.5), 0.0, 60.0, 80, 48, flags & ~A.FLAG_HIZ))
+    views = [build_view(small_world, c) for c in cams]
+    specs = [W.spec_words(c.flags, small_world) for c in cams]
+    refs = [H.oracle_frames(small_world, v, s, hi, frames=2) for v, (s, hi) in zip(views, specs)]
+    return views, specs, refs
+
+
+def _renderer(world, views):
+    r = Renderer(int(views[0].frame.res[0]), int(views[0].frame.res[1]))
+    r.sync_world(world)
+    return r
+
+
+@pytest.mark.parametrize("form", FORMS, ids=_form_id)
+def test_batch_forms_match_oracle(small_world, small_views, form):
+    views, specs, refs = small_views
+    r = _renderer(small_world, views)
+    b = Batch(r, 16)
+    b.set_tuning(**form)
+    b.set_views(views, specs)
+    for f in range(2):
+        print(f"form {_form_id(form)} frame {f}", flush=True)
+        b.render()
+        b.sync()
+        tot = {"select": 0, "deferred": 0, "recover": 0, "shadow_rays": 0}
+        for i in range(len(views)):
+            vis, sh, hz, st = refs[i][f]
+            tag = f"{_form_id(form)} view{i} frame{f} res={tuple(views[i].frame.res)}"
+            gv, gs, gh = b.read_vis(i), b.read_shadow(i), b.read_hiz(i)
+            assert np.array_equal(gv, vis), f"{tag}: {(gv != vis).sum()} vis words differ"
+            assert np.array_equal(gs, sh), f"{tag}: {(gs != sh).sum()} shadow words differ"
+            assert np.array_equal(gh, hz), f"{tag}: {(gh != hz).sum()} hiz words differ"
+            tot["select"] += st["pairs_select"]
+            tot["deferred"] += st["pairs_deferred"]
+            tot["recover"] += st["pairs_recover"]
+            tot["shadow_rays"] += st["rays_shadow"]
+        d = b.demand()
+        assert {k: d[k] for k in tot} == tot, (d, tot)
+    b.close()
+    r.close()
+
+
+def test_graph_replays_stay_exact_with_eager_work_between(small_world, small_views):
+    """Every replay of a captured batch equals the eager result, with single-view eager
+    renders and synchronisations on the same stream between the replays."""
+    views, specs, refs = small_views
+    r = _renderer(small_world, views)
+    b = Batch(r, 16)
+    b.set_tuning(mode=A.BATCH_PERSISTENT, graph=1, queue_chunk=2)
+    b.set_views(views, specs)
+    v0 = views[0]
+    r.resize(int(v0.frame.res[0]), int(v0.frame.res[1]))
+    r.set_view(v0)
+    for rep in range(5):
+        b.render()
+        r.render(v0.frame, *specs[0])
+        b.sync()
+        f = min(rep, 1)  # Hi-Z reaches its steady state after the first frame
+        for i in range(len(views)):
+            assert np.array_equal(b.read_vis(i), refs[i][f][0]), f"replay {rep} view {i}"
+            assert np.array_equal(b.read_shadow(i), refs[i][f][1]), f"replay {rep} view {i}"
+        assert np.array_equal(r.read_vis(), refs[0][0][0])
+    b.close()
+    r.close()
+
+
+def test_new_view_set_and_pool_edits_follow_into_the_graph():
+    """A captured persistent batch replays correctly after set_views with a different
+    view set and after edits change the pools (the graph is re-captured when a pool
+    moves or changes length)."""
+    wl = W.WORKLOADS[0]
+    world = W.make_world(wl)
+    cams_a = W.random_cameras(wl, 4, seed=41, resolutions=[(96, 64), (160, 90)])
+    cams_b = W.random_cameras(wl, 6, seed=42, resolutions=[(64, 48), (200, 120)])
+    r = Renderer(96, 64)
+    r.sync_world(world)
+    b = Batch(r, 8)
+    b.set_tuning(mode=A.BATCH_PERSISTENT, graph=1)
+    rng = np.random.default_rng(3)
+    for step, cams in enumerate([cams_a, cams_b, cams_a]):
+        if step == 2:
+            e = np.zeros(3000, A.EDIT_DTYPE)
+            e["x"] = rng.integers(0, 256, 3000)
+            e["y"] = rng.integers(64, 192, 3000)
+            e["z"] = rng.integers(0, 256, 3000)
+            e["id"] = rng.choice([0, 1, 2, 8, 10, 11], 3000)
+            assert world.apply_edits(e) == 3000
+            r.sync_world(world)
+        views = [build_view(world, c) for c in cams]
+        specs = [W.spec_words(c.flags, world) for c in cams]
+        b.set_views(views, specs)
+        b.render()
+        b.sync()
+        for i, (v, (s, hi)) in enumerate(zip(views, specs)):
+            ref = H.oracle_frames(world, v, s, hi, frames=1)[0]
+            assert np.array_equal(b.read_vis(i), ref[0]), f"step {step} view {i}"
+            assert np.array_equal(b.read_shadow(i), ref[1]), f"step {step} view {i}"
+    b.close()
+    r.close()
+
+
+def test_config_batch_64_views_all_forms_identical():
+    """The 64-view batch on the 1024x1024x256 world: batched forms and per-view plain
+    dispatches give identical bits for every view over two frames, and a sample of
+    views (reduced to the two smallest sizes) matches the oracle."""
+    wl = W.WORKLOADS[2]
+    world = W.make_world(wl)
+    cams = W.random_cameras(wl, wl.views, seed=2)
+    views = [build_view(world, c) for c in cams]
+    specs = [W.spec_words(c.flags, world) for c in cams]
+    r = Renderer(256, 144)
+    r.sync_world(world)
+    b = Batch(r, 64)
+    b.set_views(views, specs)
+    results = {}
+    for form in (dict(mode=A.BATCH_PLAIN, graph=0), dict(mode=A.BATCH_FLAT, graph=1),
+                 dict(mode=A.BATCH_PERSISTENT, graph=1, queue_chunk=2)):
+        print(f"64-view form {_form_id(form)}", flush=True)
+        b.set_tuning(**form)
+        b.reset_hiz()
+        b.render()
+        b.render()
+        b.sync()
+        results[_form_id(form)] = [(b.read_vis(i), b.read_shadow(i), b.read_hiz(i)) for i in range(len(views))]
+    keys = list(results)
+    for k in keys[1:]:
+        for i in range(len(views)):
+            for j, name in enumerate(("vis", "shadow", "hiz")):
+                assert np.array_equal(results[k][i][j], results[keys[0]][i][j]), f"{k} view {i} {name}"
+    small = [i for i, v in enumerate(views) if int(v.frame.res[0]) <= 640][:4]
+    for i in small:
+        s, hi = specs[i]
+        ref = H.oracle_frames(world, views[i], s, hi, frames=2)
+        assert np.array_equal(results[keys[0]][i][0], ref[1][0]), f"view {i} vs oracle"
+        assert np.array_equal(results[keys[0]][i][1], ref[1][1]), f"view {i} vs oracle"
+    b.close()
+    r.close()

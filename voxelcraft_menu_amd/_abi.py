@@ -159,6 +159,21 @@ class Tuning(C.Structure):
     _fields_ = [("occupancy", C.c_uint32), ("variant", C.c_uint32), ("ordered", C.c_uint32)]
 
 
+BATCH_PERSISTENT, BATCH_FLAT, BATCH_PLAIN = 0, 1, 2
+
+
+class ViewDesc(C.Structure):
+    """vx_view: one view of a batch (pointers into caller-owned arrays)."""
+    _fields_ = [("frame", C.c_void_p), ("chunks", C.c_void_p), ("n_total", C.c_uint32),
+                ("grid", C.c_void_p), ("spec", C.c_uint32), ("spec_hi", C.c_uint32)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("mode", C.c_uint32), ("occupancy", C.c_uint32), ("variant", C.c_uint32),
+                ("ordered", C.c_uint32), ("queue_chunk", C.c_uint32), ("waves_per_simd", C.c_uint32),
+                ("graph", C.c_uint32)]
+
+
 class PairDemand(C.Structure):
     _fields_ = [("select", C.c_uint32), ("deferred", C.c_uint32), ("recover", C.c_uint32),
                 ("cap", C.c_uint32), ("truncated", C.c_uint32), ("shadow_rays", C.c_uint32)]
@@ -226,6 +241,22 @@ _sig_march = {
     "vx_timings_reset": (C.c_int, [P]),
     "vx_timings_sum": (C.c_int, [P, C.POINTER(Timings), C.POINTER(C.c_uint32)]),
     "vx_device_outputs": (C.c_int, [P, P, P]),
+    "vx_batch_create": (C.c_int, [P, C.c_uint32, C.POINTER(P)]),
+    "vx_batch_destroy": (C.c_int, [P]),
+    "vx_batch_set_views": (C.c_int, [P, P, C.c_uint32]),
+    "vx_batch_set_tuning": (C.c_int, [P, C.POINTER(BatchTuning)]),
+    "vx_batch_get_tuning": (C.c_int, [P, C.POINTER(BatchTuning)]),
+    "vx_batch_render": (C.c_int, [P]),
+    "vx_batch_sync": (C.c_int, [P]),
+    "vx_batch_autotune": (C.c_int, [P, C.c_uint32, C.POINTER(BatchTuning), C.POINTER(C.c_float)]),
+    "vx_batch_reset_hiz": (C.c_int, [P]),
+    "vx_batch_read_vis": (C.c_int, [P, C.c_uint32, P, C.c_size_t]),
+    "vx_batch_read_shadow": (C.c_int, [P, C.c_uint32, P, C.c_size_t]),
+    "vx_batch_read_hiz": (C.c_int, [P, C.c_uint32, P, C.c_size_t]),
+    "vx_batch_demand_get": (C.c_int, [P, C.POINTER(PairDemand)]),
+    "vx_batch_timings_reset": (C.c_int, [P]),
+    "vx_batch_timings_sum": (C.c_int, [P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_uint32)]),
+    "vx_batch_device_outputs": (C.c_int, [P, C.c_uint32, P, P]),
     "vx_error_string": (C.c_char_p, [C.c_int]),
     "vx_device_name": (C.c_int, [C.c_char_p, C.c_size_t]),
 }

@@ -1,0 +1,521 @@
+// batch.hip -- extern "C" entry points of view batches (vx_batch_*, include/vxmarch.h).
+//
+// A batch renders many views of one world in one submission. It shares the context's
+// tree pools and stream; every view brings its own frame, chunk list and chunk grid
+// (the reference's per-view build_chunk_list output) and keeps its own Hi-Z, visibility
+// and shadow buffers, laid out back to back:
+//   chunks  [sum n_total]      GpuChunk records, view after view
+//   grids   [n_views * 8192]   u32 chunk grids
+//   vis     [sum W*H]          u64 keys        shadow [sum W*H] u16 (R16F bits)
+//   hiz/cost/order [sum tiles] u32
+// Three launch forms give the same bits (each tile runs the same body):
+//   PLAIN       per view: order kernels + the single-view tile march (one dispatch per
+//               view, the pattern of the reference's per-view Renderer::render);
+//   FLAT        one launch over every view's tiles, one workgroup per tile, in cost order;
+//   PERSISTENT  one launch of resident waves pulling tiles from a global queue.
+// With graph = 1 the submission is captured once into a hipGraph and replayed; the
+// kernels read the tile and view counts from a device header, so a replay follows the
+// current view set while no buffer is reallocated.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "ctx.hpp"
+
+using namespace vxd;
+using vxh::DevBuf;
+using vxh::ensure;
+
+#define HIPX VX_HIPX
+#define LAUNCHED VX_LAUNCHED
+
+struct vx_batch {
+    vx_ctx *ctx = nullptr;
+    uint32_t max_views = 0;
+    uint32_t n_views = 0, n_tiles = 0;
+    uint64_t n_px = 0;
+    int n_cu = 0;
+    std::vector<ViewK> vk;         // host copies of the view descriptors
+    std::vector<uint32_t> toff;    // n_views + 1 tile offsets
+    std::vector<uint64_t> poff;    // n_views + 1 pixel offsets
+    DevBuf views, tile_off, hdr, chunks, grids, vis, shadow, hiz, cost, order, counters;
+    vx_batch_tuning tn{};
+    // Graph of the current submission (captured lazily, dropped when it would go stale).
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint32_t graph_tiles = 0;
+    uint64_t graph_pool_gen = 0;
+    bool graph_valid = false;
+    static constexpr int RING = 256;
+    hipEvent_t ring[RING][4] = {};  // [0] call, [1] submission start, [2] march kernel start, [3] end
+    uint32_t ring_next = 0, ring_count = 0;
+    bool rendered = false;
+};
+
+namespace {
+
+// Per-view control blocks of the PLAIN form follow the shared counters.
+size_t counter_words(uint32_t n_views) { return (size_t)CNT_TOTAL + (size_t)n_views * CTL_LEN; }
+
+bool batch_tuning_ok(const vx_batch_tuning &t) {
+    if (t.mode > VX_BATCH_PLAIN || t.ordered > 1 || t.graph > 1) return false;
+    if (t.mode == VX_BATCH_PLAIN) return vxh::tuning_ok(t.occupancy, t.variant);
+    if (!((t.occupancy == 4 && (t.variant == 4 || t.variant == 5)) || (t.occupancy == 6 && t.variant == 4)))
+        return false;
+    if (t.mode == VX_BATCH_PERSISTENT) {
+        if (t.queue_chunk < 1 || t.queue_chunk > 64) return false;
+        if (t.waves_per_simd < 1 || t.waves_per_simd > t.occupancy) return false;
+    }
+    return true;
+}
+
+void drop_graph(vx_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->graph_valid = false;
+}
+
+BatchArgs batch_args(const vx_batch *b) {
+    BatchArgs a;
+    a.p = vxh::make_pools(b->ctx);
+    a.p.chunks = (const vx_gpu_chunk *)b->chunks.p;
+    a.p.grid = (const uint32_t *)b->grids.p;
+    a.p.n_chunks = 0;  // per view, from its descriptor
+    a.views = (const ViewK *)b->views.p;
+    a.hdr = (const uint32_t *)b->hdr.p;
+    a.hiz = (uint32_t *)b->hiz.p;
+    a.vis = (unsigned long long *)b->vis.p;
+    a.shadow = (uint16_t *)b->shadow.p;
+    a.counters = (uint32_t *)b->counters.p;
+    a.order = (const uint32_t *)b->order.p;
+    a.cost = (uint32_t *)b->cost.p;
+    return a;
+}
+
+template <int O, int V>
+void launch_batch_ov(const vx_batch *b, const BatchArgs &a, hipStream_t st) {
+    if (b->tn.mode == VX_BATCH_FLAT) {
+        k_batch_flat<O, V><<<b->n_tiles, 64, 0, st>>>(a);
+    } else {
+        unsigned nb = (unsigned)b->n_cu * 4u * b->tn.waves_per_simd;
+        k_batch_persistent<O, V><<<nb, 64, 0, st>>>(a, b->tn.queue_chunk);
+    }
+}
+
+// The whole submission of one batch frame, on the context stream (also what a graph
+// captures). PLAIN reads every view's geometry from the host copies, so its graph is
+// tied to the view set; the other forms read counts from the device header.
+int enqueue(vx_batch *b, hipStream_t st, hipEvent_t ev_march) {
+    int rc;
+    uint32_t *cnt = (uint32_t *)b->counters.p;
+    if ((rc = vxh::fill_u32(st, cnt, 0, counter_words(b->n_views)))) return rc;
+    if (b->tn.mode == VX_BATCH_PLAIN) {
+        vxh::TileArgs ta;
+        ta.p = vxh::make_pools(b->ctx);
+        ta.cnt = cnt;
+        for (uint32_t v = 0; v < b->n_views; v++) {
+            const ViewK &k = b->vk[v];
+            const uint32_t nt = k.f.tiles[0] * k.f.tiles[1];
+            const size_t px0 = b->poff[v];
+            uint32_t *order = (uint32_t *)b->order.p + k.tile_off;
+            uint32_t *cost = (uint32_t *)b->cost.p + k.tile_off;
+            if (b->tn.ordered)
+                vxh::launch_order(st, cost, nt, cnt + CNT_TOTAL + (size_t)v * CTL_LEN, order);
+            else
+                k_iota_views<<<(nt + 255) / 256, 256, 0, st>>>(order, nullptr, nt, nullptr, nullptr);
+            ta.p.chunks = (const vx_gpu_chunk *)b->chunks.p + k.chunk_off;
+            ta.p.grid = (const uint32_t *)b->grids.p + k.grid_off;
+            ta.p.n_chunks = k.n_chunks;
+            ta.f = k.f;
+            ta.s = k.s;
+            ta.hiz = (uint32_t *)b->hiz.p + k.tile_off;
+            ta.vis = (unsigned long long *)b->vis.p + px0;
+            ta.shadow = (uint16_t *)b->shadow.p + px0;
+            ta.order = order;
+            ta.cost = cost;
+            vxh::launch_tiles(b->tn.occupancy, b->tn.variant, (k.f.flags & VX_FLAG_HIZ) != 0u,
+                              (k.f.flags & VX_FLAG_UNDERWATER) != 0u, nt, st, ta);
+            if (!(b->tn.variant & VAR_FUSED_SHADOW))
+                k_shadow<<<((nt + 3) / 4 + 7u) & ~7u, 256, 0, st>>>(ta.p, k.f, k.s, ta.vis, ta.shadow, cnt);
+        }
+        LAUNCHED();
+        return VX_OK;
+    }
+    const uint32_t *hdr = (const uint32_t *)b->hdr.p;
+    if (b->tn.ordered) {
+        k_cost_hist<<<ORDER_BLOCKS, 256, 0, st>>>((const uint32_t *)b->cost.p, hdr + BH_TILES, 0, cnt + CNT_CTL);
+        k_cost_scatter<<<ORDER_BLOCKS, 256, 0, st>>>((const uint32_t *)b->cost.p, hdr + BH_TILES, 0, cnt + CNT_CTL,
+                                                     (const uint32_t *)b->tile_off.p, hdr + BH_VIEWS,
+                                                     (uint32_t *)b->order.p);
+    } else {
+        k_iota_views<<<ORDER_BLOCKS, 256, 0, st>>>((uint32_t *)b->order.p, hdr + BH_TILES, 0,
+                                                   (const uint32_t *)b->tile_off.p, hdr + BH_VIEWS);
+    }
+    LAUNCHED();
+    if (ev_march) HIPX(hipEventRecord(ev_march, st));
+    BatchArgs a = batch_args(b);
+    switch (b->tn.occupancy * 16 + b->tn.variant) {
+        case 4 * 16 + 5: launch_batch_ov<4, 5>(b, a, st); break;
+        case 6 * 16 + 4: launch_batch_ov<6, 4>(b, a, st); break;
+        default: launch_batch_ov<4, 4>(b, a, st); break;
+    }
+    LAUNCHED();
+    return VX_OK;
+}
+
+int capture(vx_batch *b) {
+    drop_graph(b);
+    hipStream_t st = b->ctx->stream;
+    HIPX(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue(b, st, nullptr);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    HIPX(e);
+    b->graph = g;
+    hipError_t ei = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        drop_graph(b);
+        return vxh_hip_fail(ei);
+    }
+    b->graph_tiles = b->n_tiles;
+    b->graph_pool_gen = b->ctx->pool_gen;
+    b->graph_valid = true;
+    return VX_OK;
+}
+
+int view_buffers(vx_batch *b, uint32_t n_views, uint32_t n_tiles, uint64_t n_px, size_t n_chunk_recs, bool &moved) {
+    const void *before[11] = {b->views.p, b->tile_off.p, b->hdr.p,  b->chunks.p, b->grids.p,   b->vis.p,
+                              b->shadow.p, b->hiz.p,     b->cost.p, b->order.p,  b->counters.p};
+    int rc;
+    if ((rc = ensure(b->views, (size_t)std::max(n_views, 1u) * sizeof(ViewK))) ||
+        (rc = ensure(b->tile_off, ((size_t)n_views + 1) * 4)) || (rc = ensure(b->hdr, BH_LEN * 4)) ||
+        (rc = ensure(b->chunks, std::max<size_t>(n_chunk_recs, 1) * sizeof(vx_gpu_chunk))) ||
+        (rc = ensure(b->grids, (size_t)std::max(n_views, 1u) * VX_GRID_LEN * 4)) ||
+        (rc = ensure(b->vis, std::max<uint64_t>(n_px, 1) * 8)) ||
+        (rc = ensure(b->shadow, std::max<uint64_t>(n_px, 2) * 2)) ||
+        (rc = ensure(b->hiz, (size_t)std::max(n_tiles, 1u) * 4)) ||
+        (rc = ensure(b->cost, (size_t)std::max(n_tiles, 1u) * 4)) ||
+        (rc = ensure(b->order, (size_t)std::max(n_tiles, 1u) * 4)) ||
+        (rc = ensure(b->counters, counter_words(n_views) * 4)))
+        return rc;
+    const void *after[11] = {b->views.p, b->tile_off.p, b->hdr.p,  b->chunks.p, b->grids.p,   b->vis.p,
+                             b->shadow.p, b->hiz.p,     b->cost.p, b->order.p,  b->counters.p};
+    moved = std::memcmp(before, after, sizeof before) != 0;
+    return VX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vx_batch_create(vx_ctx *c, uint32_t max_views, vx_batch **out) {
+    if (!c || !out || max_views == 0 || max_views > BATCH_MAX_VIEWS) return VX_E_ARG;
+    if (c->path != VX_PATH_TILES) return VX_E_UNSUPPORTED;
+    *out = nullptr;
+    HIPX(hipSetDevice(c->device));
+    vx_batch *b = new (std::nothrow) vx_batch();
+    if (!b) return VX_E_OOM;
+    b->ctx = c;
+    b->max_views = max_views;
+    int rc = VX_OK;
+    if (hipDeviceGetAttribute(&b->n_cu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
+        b->n_cu <= 0)
+        rc = VX_E_HIP;
+    for (auto &set : b->ring)
+        for (auto &e : set)
+            if (rc == VX_OK && hipEventCreate(&e) != hipSuccess) rc = VX_E_HIP;
+    if (rc) {
+        for (auto &set : b->ring)
+            for (auto &e : set)
+                if (e) (void)hipEventDestroy(e);
+        delete b;
+        return rc;
+    }
+    b->tn.mode = VX_BATCH_PERSISTENT;
+    b->tn.occupancy = 4;
+    b->tn.variant = VAR_FUSED_SHADOW;
+    b->tn.ordered = 1;
+    b->tn.queue_chunk = 1;
+    b->tn.waves_per_simd = 4;
+    b->tn.graph = 1;
+    c->n_batches++;
+    *out = b;
+    return VX_OK;
+}
+
+int vx_batch_destroy(vx_batch *b) {
+    if (!b) return VX_E_ARG;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    drop_graph(b);
+    DevBuf *bufs[] = {&b->views, &b->tile_off, &b->hdr,  &b->chunks, &b->grids,   &b->vis,
+                      &b->shadow, &b->hiz,     &b->cost, &b->order,  &b->counters};
+    for (DevBuf *d : bufs) vxh::release(*d);
+    for (auto &set : b->ring)
+        for (auto &e : set)
+            if (e) (void)hipEventDestroy(e);
+    b->ctx->n_batches--;
+    delete b;
+    return VX_OK;
+}
+
+int vx_batch_set_views(vx_batch *b, const vx_view *views, uint32_t n) {
+    if (!b || !views || n == 0 || n > b->max_views) return VX_E_ARG;
+    std::vector<ViewK> vk(n);
+    std::vector<uint32_t> toff(n + 1, 0);
+    std::vector<uint64_t> poff(n + 1, 0);
+    size_t recs = 0;
+    for (uint32_t v = 0; v < n; v++) {
+        const vx_view &w = views[v];
+        if (!w.frame || !w.grid || (!w.chunks && w.n_total) || w.n_total > VX_MAX_CHUNKS) return VX_E_ARG;
+        int rc = vxh::check_frame(w.frame, w.n_total);
+        if (rc) return rc;
+        ViewK &k = vk[v];
+        std::memset(&k, 0, sizeof k);
+        k.f = vxh::make_frame(w.frame);
+        if ((rc = vxh::make_spec(w.spec, w.spec_hi, w.frame->flags, k.s))) return rc;
+        k.chunk_off = (uint32_t)recs;
+        k.n_chunks = w.n_total;
+        k.grid_off = v * VX_GRID_LEN;
+        k.tile_off = toff[v];
+        k.px_off_lo = (uint32_t)poff[v];
+        k.px_off_hi = (uint32_t)(poff[v] >> 32);
+        const uint64_t nt = (uint64_t)k.f.tiles[0] * k.f.tiles[1];
+        if (toff[v] + nt > 0xFFFFFFFFull) return VX_E_OVERFLOW;
+        toff[v + 1] = toff[v] + (uint32_t)nt;
+        poff[v + 1] = poff[v] + (uint64_t)k.f.res[0] * k.f.res[1];
+        recs += w.n_total;
+    }
+    vx_ctx *c = b->ctx;
+    HIPX(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    HIPX(hipStreamSynchronize(st));
+    bool moved = false;
+    int rc = view_buffers(b, n, toff[n], poff[n], recs, moved);
+    if (rc) {
+        drop_graph(b);
+        b->n_views = 0;
+        return rc;
+    }
+    std::vector<vx_gpu_chunk> cat(std::max<size_t>(recs, 1));
+    std::vector<uint32_t> grids((size_t)n * VX_GRID_LEN);
+    for (uint32_t v = 0; v < n; v++) {
+        if (views[v].n_total)
+            std::memcpy(&cat[vk[v].chunk_off], views[v].chunks, (size_t)views[v].n_total * sizeof(vx_gpu_chunk));
+        std::memcpy(&grids[(size_t)v * VX_GRID_LEN], views[v].grid, VX_GRID_LEN * 4);
+    }
+    const uint32_t hdr[BH_LEN] = {toff[n], n, 0, 0};
+    HIPX(hipMemcpyAsync(b->views.p, vk.data(), (size_t)n * sizeof(ViewK), hipMemcpyHostToDevice, st));
+    HIPX(hipMemcpyAsync(b->tile_off.p, toff.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice, st));
+    HIPX(hipMemcpyAsync(b->hdr.p, hdr, sizeof hdr, hipMemcpyHostToDevice, st));
+    if (recs) HIPX(hipMemcpyAsync(b->chunks.p, cat.data(), recs * sizeof(vx_gpu_chunk), hipMemcpyHostToDevice, st));
+    HIPX(hipMemcpyAsync(b->grids.p, grids.data(), grids.size() * 4, hipMemcpyHostToDevice, st));
+    // New views start as a new Renderer would: empty targets, Hi-Z 0; costs 0.
+    if ((rc = vxh::fill_u64(st, b->vis.p, 0, poff[n])) || (rc = vxh::fill_u32(st, b->shadow.p, 0, (poff[n] + 1) / 2)) ||
+        (rc = vxh::fill_u32(st, b->hiz.p, 0, toff[n])) || (rc = vxh::fill_u32(st, b->cost.p, 0, toff[n])))
+        return rc;
+    HIPX(hipStreamSynchronize(st));
+    // A captured graph stays valid for another view set only if it reads every count from
+    // the header: not PLAIN (per-view launches) and not FLAT with a different grid size.
+    if (moved || b->tn.mode == VX_BATCH_PLAIN || (b->tn.mode == VX_BATCH_FLAT && b->graph_tiles != toff[n]))
+        drop_graph(b);
+    b->vk.swap(vk);
+    b->toff.swap(toff);
+    b->poff.swap(poff);
+    b->n_views = n;
+    b->n_tiles = b->toff[n];
+    b->n_px = b->poff[n];
+    b->rendered = false;
+    return VX_OK;
+}
+
+int vx_batch_set_tuning(vx_batch *b, const vx_batch_tuning *t) {
+    if (!b || !t || !batch_tuning_ok(*t)) return VX_E_ARG;
+    if (std::memcmp(&b->tn, t, sizeof *t) != 0) drop_graph(b);
+    b->tn = *t;
+    return VX_OK;
+}
+
+int vx_batch_get_tuning(vx_batch *b, vx_batch_tuning *t) {
+    if (!b || !t) return VX_E_ARG;
+    *t = b->tn;
+    return VX_OK;
+}
+
+int vx_batch_render(vx_batch *b) {
+    if (!b) return VX_E_ARG;
+    if (b->n_views == 0) return VX_E_ARG;
+    vx_ctx *c = b->ctx;
+    HIPX(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    hipEvent_t *ev = b->ring[b->ring_next];
+    b->ring_next = (b->ring_next + 1) % vx_batch::RING;
+    if (b->ring_count < (uint32_t)vx_batch::RING) b->ring_count++;
+    HIPX(hipEventRecord(ev[0], st));
+    int rc = VX_OK;
+    if (b->tn.graph) {
+        if ((!b->graph_valid || b->graph_pool_gen != c->pool_gen) && (rc = capture(b))) return rc;
+        HIPX(hipEventRecord(ev[1], st));
+        HIPX(hipEventRecord(ev[2], st));
+        HIPX(hipGraphLaunch(b->exec, st));
+    } else {
+        HIPX(hipEventRecord(ev[1], st));
+        const bool one_march = b->tn.mode != VX_BATCH_PLAIN;
+        if (!one_march) HIPX(hipEventRecord(ev[2], st));
+        if ((rc = enqueue(b, st, one_march ? ev[2] : nullptr))) return rc;
+    }
+    HIPX(hipEventRecord(ev[3], st));
+    b->rendered = true;
+    return VX_OK;
+}
+
+int vx_batch_sync(vx_batch *b) {
+    if (!b) return VX_E_ARG;
+    return vx_sync(b->ctx);
+}
+
+int vx_batch_reset_hiz(vx_batch *b) {
+    if (!b) return VX_E_ARG;
+    HIPX(hipSetDevice(b->ctx->device));
+    int rc = vxh::fill_u32(b->ctx->stream, b->hiz.p, 0, b->n_tiles);
+    if (rc) return rc;
+    HIPX(hipStreamSynchronize(b->ctx->stream));
+    return VX_OK;
+}
+
+static int batch_read(vx_batch *b, const DevBuf &buf, size_t off_bytes, void *out, size_t bytes) {
+    HIPX(hipSetDevice(b->ctx->device));
+    HIPX(hipMemcpyAsync(out, (const char *)buf.p + off_bytes, bytes, hipMemcpyDeviceToHost, b->ctx->stream));
+    HIPX(hipStreamSynchronize(b->ctx->stream));
+    return VX_OK;
+}
+
+int vx_batch_read_vis(vx_batch *b, uint32_t view, uint64_t *out, size_t n) {
+    if (!b || !out || view >= b->n_views || n != b->poff[view + 1] - b->poff[view]) return VX_E_ARG;
+    return batch_read(b, b->vis, b->poff[view] * 8, out, n * 8);
+}
+
+int vx_batch_read_shadow(vx_batch *b, uint32_t view, uint16_t *out, size_t n) {
+    if (!b || !out || view >= b->n_views || n != b->poff[view + 1] - b->poff[view]) return VX_E_ARG;
+    return batch_read(b, b->shadow, b->poff[view] * 2, out, n * 2);
+}
+
+int vx_batch_read_hiz(vx_batch *b, uint32_t view, uint32_t *out, size_t n) {
+    if (!b || !out || view >= b->n_views || n != (size_t)(b->toff[view + 1] - b->toff[view])) return VX_E_ARG;
+    return batch_read(b, b->hiz, (size_t)b->toff[view] * 4, out, n * 4);
+}
+
+int vx_batch_demand_get(vx_batch *b, vx_pair_demand *out) {
+    if (!b || !out) return VX_E_ARG;
+    std::vector<uint32_t> cnt(CNT_LEN);
+    int rc = batch_read(b, b->counters, 0, cnt.data(), CNT_LEN * 4);
+    if (rc) return rc;
+    uint64_t s[4] = {0, 0, 0, 0};
+    for (uint32_t sl = 0; sl < CNT_STAT_SLOTS; sl++)
+        for (uint32_t k = 0; k < 4; k++) s[k] += cnt[CNT_STAT_BASE + sl * CNT_STAT_STRIDE + k];
+    std::memset(out, 0, sizeof *out);
+    out->select = (uint32_t)s[STAT_SELECT];
+    out->deferred = (uint32_t)s[STAT_DEFERRED];
+    out->recover = (uint32_t)s[STAT_RECOVER];
+    out->shadow_rays = (uint32_t)s[STAT_SHADOW];
+    return VX_OK;
+}
+
+int vx_batch_timings_reset(vx_batch *b) {
+    if (!b) return VX_E_ARG;
+    HIPX(hipSetDevice(b->ctx->device));
+    HIPX(hipStreamSynchronize(b->ctx->stream));
+    b->ring_count = 0;
+    return VX_OK;
+}
+
+int vx_batch_timings_sum(vx_batch *b, float *total_ms, float *march_ms, uint32_t *frames) {
+    if (!b || !total_ms) return VX_E_ARG;
+    HIPX(hipSetDevice(b->ctx->device));
+    HIPX(hipStreamSynchronize(b->ctx->stream));
+    float tot = 0, mar = 0;
+    for (uint32_t k = 0; k < b->ring_count; k++) {
+        uint32_t idx = (b->ring_next + vx_batch::RING - 1 - k) % vx_batch::RING;
+        float ms = 0, mm = 0;
+        HIPX(hipEventElapsedTime(&ms, b->ring[idx][1], b->ring[idx][3]));
+        HIPX(hipEventElapsedTime(&mm, b->ring[idx][2], b->ring[idx][3]));
+        tot += ms;
+        mar += mm;
+    }
+    *total_ms = tot;
+    if (march_ms) *march_ms = mar;
+    if (frames) *frames = b->ring_count;
+    return VX_OK;
+}
+
+// Times every batched launch form on the current views (reps frames each, after one
+// warm frame that also refreshes the tile costs) and keeps the fastest. Outputs do not
+// depend on the choice; the Hi-Z words reach their steady state after the first frame.
+int vx_batch_autotune(vx_batch *b, uint32_t reps, vx_batch_tuning *best, float *best_ms) {
+    if (!b || b->n_views == 0 || reps == 0 || reps > 64) return VX_E_ARG;
+    std::vector<vx_batch_tuning> cand;
+    auto add = [&](uint32_t mode, uint32_t occ, uint32_t var, uint32_t chunk, uint32_t wps) {
+        vx_batch_tuning t{};
+        t.mode = mode;
+        t.occupancy = occ;
+        t.variant = var;
+        t.ordered = 1;
+        t.queue_chunk = chunk;
+        t.waves_per_simd = wps;
+        t.graph = 1;
+        cand.push_back(t);
+    };
+    for (uint32_t chunk : {1u, 2u, 4u}) {
+        add(VX_BATCH_PERSISTENT, 4, 4, chunk, 4);
+        add(VX_BATCH_PERSISTENT, 6, 4, chunk, 6);
+    }
+    add(VX_BATCH_PERSISTENT, 4, 5, 1, 4);
+    add(VX_BATCH_FLAT, 4, 4, 1, 4);
+    add(VX_BATCH_FLAT, 6, 4, 1, 6);
+    add(VX_BATCH_FLAT, 4, 5, 1, 4);
+    const vx_batch_tuning saved = b->tn;
+    float best_t = 1e30f;
+    vx_batch_tuning best_c = saved;
+    int rc = VX_OK;
+    for (const vx_batch_tuning &t : cand) {
+        if ((rc = vx_batch_set_tuning(b, &t))) break;
+        if ((rc = vx_batch_render(b))) break;
+        if ((rc = vx_batch_timings_reset(b))) break;
+        for (uint32_t r = 0; r < reps && rc == VX_OK; r++) rc = vx_batch_render(b);
+        if (rc) break;
+        float ms = 0;
+        uint32_t frames = 0;
+        if ((rc = vx_batch_timings_sum(b, &ms, nullptr, &frames))) break;
+        ms /= (float)std::max(frames, 1u);
+        if (ms < best_t) {
+            best_t = ms;
+            best_c = t;
+        }
+    }
+    if (rc) {
+        vx_batch_set_tuning(b, &saved);
+        return rc;
+    }
+    if ((rc = vx_batch_set_tuning(b, &best_c))) return rc;
+    if (best) *best = best_c;
+    if (best_ms) *best_ms = best_t;
+    return VX_OK;
+}
+
+int vx_batch_device_outputs(vx_batch *b, uint32_t view, uint64_t **vis, uint16_t **shadow) {
+    if (!b || view >= b->n_views) return VX_E_ARG;
+    if (vis) *vis = (uint64_t *)b->vis.p + b->poff[view];
+    if (shadow) *shadow = (uint16_t *)b->shadow.p + b->poff[view];
+    return VX_OK;
+}
+
+}  // extern "C"

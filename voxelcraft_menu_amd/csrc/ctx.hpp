@@ -1,0 +1,92 @@
+// ctx.hpp -- host-side state of a march context, shared by the C-ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+
+#include "kernels.hpp"
+#include "vxmarch.h"
+
+#define VX_HIPX(expr)                                    \
+    do {                                                 \
+        hipError_t e_ = (expr);                          \
+        if (e_ != hipSuccess) return vxh_hip_fail(e_);   \
+    } while (0)
+#define VX_LAUNCHED() VX_HIPX(hipPeekAtLastError())
+
+int vxh_hip_fail(hipError_t e);
+
+namespace vxh {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+// Grow-only device allocation (contents are not preserved).
+int ensure(DevBuf &b, size_t bytes);
+void release(DevBuf &b);
+
+int fill_u32(hipStream_t st, void *p, uint32_t v, size_t n);
+int fill_u64(hipStream_t st, void *p, uint64_t v, size_t n);
+
+vxd::FrameK make_frame(const vx_gpu_frame *g);
+// Kernel spec bits for a (flags & SPEC_MASK, spec_hi) pipeline key; VX_E_UNSUPPORTED for
+// the features that need the texture atlas or the refraction path.
+int make_spec(uint32_t spec, uint32_t spec_hi, uint32_t frame_flags, vxd::Spec &s);
+// Checks a frame against the fixed limits (tile counts, grid size, chunk count).
+int check_frame(const vx_gpu_frame *g, uint32_t n_chunks);
+
+// Launch of the tile-per-wave march for one view (one workgroup per tile, in `order`).
+struct TileArgs {
+    vxd::Pools p;
+    vxd::FrameK f;
+    vxd::Spec s;
+    uint32_t *hiz;
+    unsigned long long *vis;
+    uint16_t *shadow;
+    uint32_t *cnt;
+    const uint32_t *order;
+    uint32_t *cost;
+};
+bool tuning_ok(uint32_t occ, uint32_t var);
+void launch_tiles(uint32_t occ, uint32_t var, bool hz, bool uw, unsigned nb, hipStream_t st, const TileArgs &a);
+// Cost-bucket launch order of n tiles into `order` (entries are tile indices); `ctl`
+// is the CTL_LEN-word control block, cleared beforehand.
+void launch_order(hipStream_t st, const uint32_t *cost, uint32_t n, uint32_t *ctl, uint32_t *order);
+
+}  // namespace vxh
+
+struct vx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t w = 0, h = 0, tx = 0, ty = 0, max_pairs = 0, max_pairs_req = 0, path = VX_PATH_TILES;
+    vxh::DevBuf pool[3];
+    size_t pool_len[3] = {0, 0, 0};
+    bool pool_synced[3] = {false, false, false};
+    vxh::DevBuf chunks, grid, vis, shadow, hiz, pairs, counters;
+    vxh::DevBuf cost, order;  // per-tile cycles of the last frame, and the launch order
+    bool have_cost = false;
+    uint32_t n_chunks = 0;
+    // Ring of per-frame event sets: [0] start, [1..5] pass boundaries, [6] end.
+    static constexpr int RING = 256;
+    hipEvent_t ring[RING][7] = {};
+    uint32_t ring_next = 0;   // set the next frame records into
+    uint32_t ring_count = 0;  // frames recorded since the last reset (saturates at RING)
+    hipEvent_t *ev = nullptr; // the set of the frame being recorded / last recorded
+    uint32_t occ = 4;                          // tile-march occupancy variant (waves per SIMD)
+    uint32_t variant = vxd::VAR_FUSED_SHADOW;  // tile-march structure variant (VAR_* bits)
+    uint32_t ordered = 1;  // launch tiles most-expensive-first (1) or in raster order (0)
+    bool have_frame = false;
+    vxd::FrameK frame{};
+    vxd::Spec spec{};
+    int n_batches = 0;     // batches bound to this context (it must outlive them)
+    // Bumped whenever a pool's device address or length changes: captured graphs hold
+    // both in their kernel arguments and are re-captured when it moves.
+    uint64_t pool_gen = 0;
+};
+
+namespace vxh {
+vxd::Pools make_pools(const vx_ctx *c);
+}

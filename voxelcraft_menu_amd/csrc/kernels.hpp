@@ -30,6 +30,13 @@ constexpr uint32_t CNT_PAIRS = 0, CNT_DEFERRED = 1, CNT_RECOVER = 2, CNT_SELECT 
 constexpr uint32_t STAT_SELECT = 0, STAT_DEFERRED = 1, STAT_RECOVER = 2, STAT_SHADOW = 3;
 constexpr uint32_t CNT_STAT_BASE = 32, CNT_STAT_SLOTS = 64, CNT_STAT_STRIDE = 32;
 constexpr uint32_t CNT_LEN = CNT_STAT_BASE + CNT_STAT_SLOTS * CNT_STAT_STRIDE;
+// Control block after the statistics: the cost-bucket histogram and per-bucket cursors of
+// the launch ordering, and the work-queue head of the persistent batch march, each on its
+// own 128-byte line. Cleared with the counters at the start of every frame.
+constexpr uint32_t CNT_CTL = CNT_LEN;
+constexpr uint32_t CTL_HIST = 0, CTL_CURSOR = 32, CTL_HEAD = 64, CTL_LEN = 96;
+constexpr uint32_t CNT_TOTAL = CNT_LEN + CTL_LEN;
+constexpr uint32_t COST_BUCKETS = 32;
 
 __device__ __forceinline__ void stat_add(uint32_t *counters, uint32_t slot, uint32_t k, uint32_t v) {
     if (v) atomicAdd(&counters[CNT_STAT_BASE + (slot & (CNT_STAT_SLOTS - 1)) * CNT_STAT_STRIDE + k], v);
@@ -57,8 +64,55 @@ __global__ void k_march_tiles(Pools p, FrameK f, Spec s, uint32_t *__restrict__ 
                               unsigned long long *__restrict__ vis, uint16_t *__restrict__ shadow,
                               uint32_t *__restrict__ counters, const uint32_t *__restrict__ order,
                               uint32_t *__restrict__ cost);
-__global__ void k_order_tiles(const uint32_t *__restrict__ cost, uint32_t *__restrict__ order, uint32_t n);
-__global__ void k_iota(uint32_t *__restrict__ p, uint32_t n);
+
+// Launch ordering (most expensive cost bucket first). Entries are tile indices, or with a
+// view table (n_views > 0) packed view << ENTRY_VIEW_SHIFT | tile-in-view. The element
+// count is *n_dev when n_dev is non-null, else n.
+constexpr uint32_t ENTRY_VIEW_SHIFT = 20, ENTRY_TILE_MASK = (1u << ENTRY_VIEW_SHIFT) - 1u;
+constexpr uint32_t BATCH_MAX_VIEWS = 1024;
+constexpr uint32_t ORDER_BLOCKS = 256;
+__global__ void k_cost_hist(const uint32_t *__restrict__ cost, const uint32_t *__restrict__ n_dev, uint32_t n,
+                            uint32_t *__restrict__ ctl);
+__global__ void k_cost_scatter(const uint32_t *__restrict__ cost, const uint32_t *__restrict__ n_dev, uint32_t n,
+                               uint32_t *__restrict__ ctl, const uint32_t *__restrict__ tile_off,
+                               const uint32_t *__restrict__ n_views_dev, uint32_t *__restrict__ order);
+__global__ void k_iota_views(uint32_t *__restrict__ order, const uint32_t *__restrict__ n_dev, uint32_t n,
+                             const uint32_t *__restrict__ tile_off, const uint32_t *__restrict__ n_views_dev);
+
+// One view of a batch: its frame and spec, and where its inputs and outputs start in the
+// batch's concatenated buffers.
+struct ViewK {
+    FrameK f;
+    Spec s;
+    uint32_t chunk_off;  // first GpuChunk record of the view's list
+    uint32_t n_chunks;   // records in the view's list (visible first)
+    uint32_t grid_off;   // first word of the view's 32x8x32 chunk grid
+    uint32_t tile_off;   // first Hi-Z / cost word of the view
+    uint32_t px_off_lo, px_off_hi;  // first pixel of the view in vis / shadow
+};
+
+// Header words of a batch in device memory (read by graph-captured kernels, so a replay
+// follows the current view set).
+constexpr uint32_t BH_TILES = 0, BH_VIEWS = 1, BH_LEN = 4;
+
+struct BatchArgs {
+    Pools p;                  // tree pools; chunks/grid are the batch's concatenated arrays
+    const ViewK *__restrict__ views;
+    const uint32_t *__restrict__ hdr;
+    uint32_t *__restrict__ hiz;
+    unsigned long long *__restrict__ vis;
+    uint16_t *__restrict__ shadow;
+    uint32_t *__restrict__ counters;
+    const uint32_t *__restrict__ order;
+    uint32_t *__restrict__ cost;
+};
+
+// Batched tile march: FLAT = one workgroup per queue entry; PERSISTENT = resident waves
+// taking `chunk` entries at a time from the queue head in the control block.
+template <int kOcc, int kVar>
+__global__ void k_batch_flat(BatchArgs a);
+template <int kOcc, int kVar>
+__global__ void k_batch_persistent(BatchArgs a, uint32_t chunk);
 __global__ void k_shadow(Pools p, FrameK f, Spec s, const unsigned long long *__restrict__ vis,
                          uint16_t *__restrict__ out, uint32_t *__restrict__ counters);
 __global__ void k_decode(Pools p, FrameK f, const unsigned long long *__restrict__ vis,

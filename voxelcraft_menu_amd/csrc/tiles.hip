@@ -211,62 +211,163 @@ VX_TILES_INST(4, 5)
 VX_TILES_INST(6, 0)
 VX_TILES_INST(6, 4)
 
-// Launch order for the next frame: tiles grouped into 32 cost buckets (log2 of the
-// measured cycles), most expensive bucket first, raster order inside a bucket as far as
-// the atomics allow. One workgroup of 1024 threads; atomics are aggregated per wave and
-// bucket (a wave's 64 consecutive tiles usually fall in a handful of buckets).
-__device__ __forceinline__ uint32_t cost_bucket(uint32_t c) { return 31u - (uint32_t)__clz(c | 1u); }
-
-__global__ __launch_bounds__(1024) void k_order_tiles(const uint32_t *__restrict__ cost, uint32_t *__restrict__ order,
-                                                      uint32_t n) {
-    __shared__ uint32_t hist[32];
-    __shared__ uint32_t base[32];
-    const uint32_t t = threadIdx.x;
-    if (t < 32) hist[t] = 0;
-    __syncthreads();
-    const uint32_t n_pad = (n + 1023u) & ~1023u;
-    for (uint32_t i = t; i < n_pad; i += 1024u) {
-        uint32_t b = i < n ? cost_bucket(cost[i]) : 32u;
-        bool todo = b < 32u;
-        while (ballot(todo) != 0ull) {
-            uint32_t lead = (uint32_t)__shfl((int)b, __ffsll((long long)ballot(todo)) - 1);
-            uint64_t m = ballot(todo && b == lead);
-            if (todo && b == lead) {
-                if (lane_id() == (uint32_t)__ffsll((long long)m) - 1u) atomicAdd(&hist[lead], (uint32_t)__popcll(m));
-                todo = false;
-            }
-        }
+// ---- Batched march: many views, one launch -------------------------------------------
+// A queue entry names (view, tile). The view's frame, spec and buffer offsets are read
+// from its descriptor (wave-uniform, so scalar loads), and the tile runs the same body
+// as the single-view kernel, so a pixel's key, shadow word and Hi-Z word are the same
+// bits whichever way the views are launched.
+template <int kVar>
+__device__ __forceinline__ void batch_tile(const BatchArgs &a, uint32_t entry, uint32_t n_views, uint64_t *defmask,
+                                           uint64_t *selmask) {
+    entry = (uint32_t)__builtin_amdgcn_readfirstlane((int)entry);  // uniform: descriptor reads are scalar
+    const uint32_t v = entry >> ENTRY_VIEW_SHIFT, tile = entry & ENTRY_TILE_MASK;
+    if (v >= n_views) return;
+    const ViewK *vk = a.views + v;
+    const FrameK f = vk->f;
+    const Spec s = vk->s;
+    if (tile >= f.tiles[0] * f.tiles[1]) return;
+    Pools p = a.p;
+    p.chunks = a.p.chunks + vk->chunk_off;
+    p.grid = a.p.grid + vk->grid_off;
+    p.n_chunks = vk->n_chunks;
+    const size_t px0 = (size_t)vk->px_off_lo | ((size_t)vk->px_off_hi << 32);
+    const uint32_t toff = vk->tile_off;
+    uint32_t *hiz = a.hiz + toff;
+    unsigned long long *vis = a.vis + px0;
+    uint16_t *shadow = a.shadow + px0;
+    const uint64_t t0 = wall_clock64();
+    const bool hz = (f.flags & VX_FLAG_HIZ) != 0u, uw = (f.flags & VX_FLAG_UNDERWATER) != 0u;
+    if (uw) {
+        if (hz) march_tile_body<true, true, kVar>(p, f, s, tile, hiz, vis, shadow, a.counters, defmask, selmask);
+        else march_tile_body<false, true, kVar>(p, f, s, tile, hiz, vis, shadow, a.counters, defmask, selmask);
+    } else {
+        if (hz) march_tile_body<true, false, kVar>(p, f, s, tile, hiz, vis, shadow, a.counters, defmask, selmask);
+        else march_tile_body<false, false, kVar>(p, f, s, tile, hiz, vis, shadow, a.counters, defmask, selmask);
     }
-    __syncthreads();
-    if (t == 0) {
-        uint32_t acc = 0;
-        for (int b = 31; b >= 0; b--) {
-            base[b] = acc;
-            acc += hist[b];
-        }
-    }
-    __syncthreads();
-    for (uint32_t i = t; i < n_pad; i += 1024u) {
-        uint32_t b = i < n ? cost_bucket(cost[i]) : 32u;
-        bool todo = b < 32u;
-        while (ballot(todo) != 0ull) {
-            uint32_t lead = (uint32_t)__shfl((int)b, __ffsll((long long)ballot(todo)) - 1);
-            uint64_t m = ballot(todo && b == lead);
-            uint32_t first = (uint32_t)__ffsll((long long)m) - 1u;
-            uint32_t pos0 = 0;
-            if (todo && b == lead && lane_id() == first) pos0 = atomicAdd(&base[lead], (uint32_t)__popcll(m));
-            pos0 = (uint32_t)__shfl((int)pos0, (int)first);
-            if (todo && b == lead) {
-                order[pos0 + (uint32_t)__popcll(m & ((1ull << lane_id()) - 1ull))] = i;
-                todo = false;
-            }
-        }
+    if (lane_id() == 0) {
+        uint64_t dt = wall_clock64() - t0;
+        a.cost[toff + tile] = dt > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)dt;
     }
 }
 
-__global__ void k_iota(uint32_t *__restrict__ p, uint32_t n) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = i;
+template <int kOcc, int kVar>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kOcc))) void k_batch_flat(BatchArgs a) {
+    __shared__ uint64_t defmask[128];
+    __shared__ uint64_t selmask[(kVar & VAR_SPLIT) ? 128 : 1];
+    const uint32_t n = a.hdr[BH_TILES];
+    if (blockIdx.x >= n) return;
+    if (blockIdx.x < kPrioTiles) __builtin_amdgcn_s_setprio(3);
+    batch_tile<kVar>(a, a.order[blockIdx.x], a.hdr[BH_VIEWS], defmask, selmask);
+}
+
+// Persistent form: the grid is sized to the resident wave slots; every wave takes
+// `chunk` consecutive queue entries per atomic on the head word until the queue is
+// exhausted (the head only grows, so every wave reaches the exit).
+template <int kOcc, int kVar>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kOcc))) void k_batch_persistent(BatchArgs a,
+                                                                                                   uint32_t chunk) {
+    __shared__ uint64_t defmask[128];
+    __shared__ uint64_t selmask[(kVar & VAR_SPLIT) ? 128 : 1];
+    const uint32_t n = a.hdr[BH_TILES];
+    const uint32_t nv = a.hdr[BH_VIEWS];
+    uint32_t *head = a.counters + CNT_CTL + CTL_HEAD;
+    chunk = chunk ? chunk : 1u;
+    for (;;) {
+        uint32_t base = 0;
+        if (lane_id() == 0) base = atomicAdd(head, chunk);
+        base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);  // all lanes active: lane 0's value
+        if (base >= n) break;
+        if (base < kPrioTiles) __builtin_amdgcn_s_setprio(3);
+        else __builtin_amdgcn_s_setprio(0);
+        const uint32_t end = min(base + chunk, n);
+        for (uint32_t k = base; k < end; k++) batch_tile<kVar>(a, a.order[k], nv, defmask, selmask);
+    }
+}
+
+#define VX_BATCH_INST(O, V)                                        \
+    template __global__ void k_batch_flat<O, V>(BatchArgs);        \
+    template __global__ void k_batch_persistent<O, V>(BatchArgs, uint32_t);
+VX_BATCH_INST(4, 4)
+VX_BATCH_INST(4, 5)
+VX_BATCH_INST(6, 4)
+
+// Launch order for the next frame: tiles grouped into 32 buckets by the log2 of their
+// measured cycles, most expensive bucket first (longest-processing-time-first list
+// scheduling: the heavy tiles are the frame's critical path, so they start first).
+// k_cost_hist counts the buckets over the whole grid; k_cost_scatter gives every block a
+// range inside each bucket with one atomic per bucket and places its tiles there. Order
+// inside a bucket follows block arrival; it only schedules work, results do not depend
+// on it.
+__device__ __forceinline__ uint32_t cost_bucket(uint32_t c) { return 31u - (uint32_t)__clz(c | 1u); }
+
+// Entry of global tile i: i itself, or (view, tile-in-view) found in the view table.
+__device__ __forceinline__ uint32_t tile_entry(uint32_t i, const uint32_t *toff, uint32_t nv) {
+    if (nv == 0u) return i;
+    uint32_t lo = 0, hi = nv;  // toff[lo] <= i < toff[hi]
+    while (hi - lo > 1u) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (toff[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    return (lo << ENTRY_VIEW_SHIFT) | (i - toff[lo]);
+}
+
+__global__ __launch_bounds__(256) void k_cost_hist(const uint32_t *__restrict__ cost, const uint32_t *__restrict__ n_dev,
+                                                   uint32_t n, uint32_t *__restrict__ ctl) {
+    __shared__ uint32_t hist[COST_BUCKETS];
+    if (n_dev) n = *n_dev;
+    const uint32_t t = threadIdx.x;
+    if (t < COST_BUCKETS) hist[t] = 0;
+    __syncthreads();
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + t; i < n; i += stride) atomicAdd(&hist[cost_bucket(cost[i])], 1u);
+    __syncthreads();
+    if (t < COST_BUCKETS && hist[t]) atomicAdd(&ctl[CTL_HIST + t], hist[t]);
+}
+
+__global__ __launch_bounds__(256) void k_cost_scatter(const uint32_t *__restrict__ cost,
+                                                      const uint32_t *__restrict__ n_dev, uint32_t n,
+                                                      uint32_t *__restrict__ ctl, const uint32_t *__restrict__ tile_off,
+                                                      const uint32_t *__restrict__ n_views_dev,
+                                                      uint32_t *__restrict__ order) {
+    __shared__ uint32_t cnt[COST_BUCKETS];
+    __shared__ uint32_t base[COST_BUCKETS];
+    __shared__ uint32_t toff[BATCH_MAX_VIEWS + 1];
+    if (n_dev) n = *n_dev;
+    const uint32_t nv = n_views_dev ? min(*n_views_dev, BATCH_MAX_VIEWS) : 0u;
+    const uint32_t t = threadIdx.x;
+    if (t < COST_BUCKETS) cnt[t] = 0;
+    if (nv)
+        for (uint32_t v = t; v <= nv; v += blockDim.x) toff[v] = tile_off[v];
+    __syncthreads();
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + t; i < n; i += stride) atomicAdd(&cnt[cost_bucket(cost[i])], 1u);
+    __syncthreads();
+    if (t < COST_BUCKETS) {
+        uint32_t start = 0;
+        for (uint32_t b = t + 1u; b < COST_BUCKETS; b++) start += ctl[CTL_HIST + b];
+        const uint32_t c = cnt[t];
+        base[t] = c ? start + atomicAdd(&ctl[CTL_CURSOR + t], c) : 0u;
+        cnt[t] = 0;
+    }
+    __syncthreads();
+    for (uint32_t i = blockIdx.x * blockDim.x + t; i < n; i += stride) {
+        const uint32_t b = cost_bucket(cost[i]);
+        order[base[b] + atomicAdd(&cnt[b], 1u)] = tile_entry(i, toff, nv);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_iota_views(uint32_t *__restrict__ order, const uint32_t *__restrict__ n_dev,
+                                                    uint32_t n, const uint32_t *__restrict__ tile_off,
+                                                    const uint32_t *__restrict__ n_views_dev) {
+    __shared__ uint32_t toff[BATCH_MAX_VIEWS + 1];
+    if (n_dev) n = *n_dev;
+    const uint32_t nv = n_views_dev ? min(*n_views_dev, BATCH_MAX_VIEWS) : 0u;
+    if (nv)
+        for (uint32_t v = threadIdx.x; v <= nv; v += blockDim.x) toff[v] = tile_off[v];
+    __syncthreads();
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) order[i] = tile_entry(i, toff, nv);
 }
 
 }  // namespace vxd

@@ -13,31 +13,23 @@
 #include <new>
 #include <vector>
 
-#include "kernels.hpp"
-#include "vxmarch.h"
+#include "ctx.hpp"
 
 using namespace vxd;
+using vxh::DevBuf;
+using vxh::ensure;
 
-#define LAUNCHED() HIPX(hipPeekAtLastError())
+#define HIPX VX_HIPX
+#define LAUNCHED VX_LAUNCHED
 
-#define HIPX(expr)                                  \
-    do {                                            \
-        hipError_t e_ = (expr);                     \
-        if (e_ != hipSuccess) return hip_fail(e_);  \
-    } while (0)
-
-static int hip_fail(hipError_t e) {
+int vxh_hip_fail(hipError_t e) {
     if (e == hipErrorOutOfMemory) return VX_E_OOM;
     std::fprintf(stderr, "vxmarch: HIP error %d (%s)\n", (int)e, hipGetErrorString(e));
     return VX_E_HIP;
 }
+#define hip_fail vxh_hip_fail
 
-namespace {
-
-struct DevBuf {
-    void *p = nullptr;
-    size_t cap = 0;
-};
+namespace vxh {
 
 int ensure(DevBuf &b, size_t bytes) {
     if (bytes <= b.cap) return VX_OK;
@@ -49,34 +41,37 @@ int ensure(DevBuf &b, size_t bytes) {
     return VX_OK;
 }
 
-constexpr size_t MIN_CAP = 1u << 20;  // GpuMirror MIN_CAP (src/render/pools.rs:11)
+void release(DevBuf &b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
 
-}  // namespace
+int fill_u32(hipStream_t st, void *p, uint32_t v, size_t n) {
+    if (!n) return VX_OK;
+    unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+    k_fill_u32<<<blocks, 256, 0, st>>>((uint32_t *)p, v, n);
+    HIPX(hipGetLastError());
+    return VX_OK;
+}
 
-struct vx_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    uint32_t w = 0, h = 0, tx = 0, ty = 0, max_pairs = 0, max_pairs_req = 0, path = VX_PATH_TILES;
-    DevBuf pool[3];
-    size_t pool_len[3] = {0, 0, 0};
-    bool pool_synced[3] = {false, false, false};
-    DevBuf chunks, grid, vis, shadow, hiz, pairs, counters;
-    DevBuf cost, order;         // per-tile cycles of the last frame, and the launch order
-    bool have_cost = false;
-    uint32_t n_chunks = 0;
-    // Ring of per-frame event sets: [0] start, [1..5] pass boundaries, [6] end.
-    static constexpr int RING = 256;
-    hipEvent_t ring[RING][7] = {};
-    uint32_t ring_next = 0;     // set the next frame records into
-    uint32_t ring_count = 0;    // frames recorded since the last reset (saturates at RING)
-    hipEvent_t *ev = nullptr;   // the set of the frame being recorded / last recorded
-    uint32_t occ = 4;      // tile-march occupancy variant (waves per SIMD the kernel is built for)
-    uint32_t variant = VAR_FUSED_SHADOW;  // tile-march structure variant (VAR_* bits)
-    uint32_t ordered = 1;  // launch tiles most-expensive-first (1) or in raster order (0)
-    bool have_frame = false;
-    FrameK frame{};
-    Spec spec{};
-};
+int fill_u64(hipStream_t st, void *p, uint64_t v, size_t n) {
+    if (!n) return VX_OK;
+    unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+    k_fill_u64<<<blocks, 256, 0, st>>>((uint64_t *)p, v, n);
+    HIPX(hipGetLastError());
+    return VX_OK;
+}
+
+void launch_order(hipStream_t st, const uint32_t *cost, uint32_t n, uint32_t *ctl, uint32_t *order) {
+    unsigned nb = std::max(1u, std::min<unsigned>(ORDER_BLOCKS, (n + 255u) / 256u));
+    k_cost_hist<<<nb, 256, 0, st>>>(cost, nullptr, n, ctl);
+    k_cost_scatter<<<nb, 256, 0, st>>>(cost, nullptr, n, ctl, nullptr, nullptr, order);
+}
+
+}  // namespace vxh
+
+static const size_t MIN_CAP = 1u << 20;  // GpuMirror MIN_CAP (src/render/pools.rs:11)
 
 static uint32_t pairs_for(uint32_t tx, uint32_t ty) {
     uint64_t want = (uint64_t)tx * ty * 16ull;
@@ -85,21 +80,8 @@ static uint32_t pairs_for(uint32_t tx, uint32_t ty) {
     return (uint32_t)want;
 }
 
-static int fill_u32(vx_ctx *c, void *p, uint32_t v, size_t n) {
-    if (!n) return VX_OK;
-    unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
-    k_fill_u32<<<blocks, 256, 0, c->stream>>>((uint32_t *)p, v, n);
-    HIPX(hipGetLastError());
-    return VX_OK;
-}
-
-static int fill_u64(vx_ctx *c, void *p, uint64_t v, size_t n) {
-    if (!n) return VX_OK;
-    unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
-    k_fill_u64<<<blocks, 256, 0, c->stream>>>((uint64_t *)p, v, n);
-    HIPX(hipGetLastError());
-    return VX_OK;
-}
+static int fill_u32(vx_ctx *c, void *p, uint32_t v, size_t n) { return vxh::fill_u32(c->stream, p, v, n); }
+static int fill_u64(vx_ctx *c, void *p, uint64_t v, size_t n) { return vxh::fill_u64(c->stream, p, v, n); }
 
 static int alloc_targets(vx_ctx *c, uint32_t w, uint32_t h) {
     if (w == 0 || h == 0) return VX_E_ARG;
@@ -126,17 +108,7 @@ static int alloc_targets(vx_ctx *c, uint32_t w, uint32_t h) {
 
 static unsigned pad8(unsigned n) { return (n + 7u) & ~7u; }
 
-struct TileArgs {
-    Pools p;
-    FrameK f;
-    Spec s;
-    uint32_t *hiz;
-    unsigned long long *vis;
-    uint16_t *shadow;
-    uint32_t *cnt;
-    const uint32_t *order;
-    uint32_t *cost;
-};
+namespace vxh {
 
 template <int O, int V>
 static void launch_tiles_ov(bool hz, bool uw, unsigned nb, hipStream_t st, const TileArgs &a) {
@@ -151,15 +123,14 @@ static void launch_tiles_ov(bool hz, bool uw, unsigned nb, hipStream_t st, const
 }
 
 // Valid (occupancy, variant) pairs compiled into the library.
-static bool tuning_ok(uint32_t occ, uint32_t var) {
+bool tuning_ok(uint32_t occ, uint32_t var) {
     if (occ == 4) return var == 0 || var == 1 || var == 4 || var == 5;
     if (occ == 6) return var == 0 || var == 4;
     return false;
 }
 
 // One workgroup (one wave) per tile, launched in `order`.
-static void launch_tiles(uint32_t occ, uint32_t var, bool hz, bool uw, unsigned nb, hipStream_t st,
-                         const TileArgs &a) {
+void launch_tiles(uint32_t occ, uint32_t var, bool hz, bool uw, unsigned nb, hipStream_t st, const TileArgs &a) {
     switch (occ * 16 + var) {
         case 4 * 16 + 1: launch_tiles_ov<4, 1>(hz, uw, nb, st, a); break;
         case 4 * 16 + 4: launch_tiles_ov<4, 4>(hz, uw, nb, st, a); break;
@@ -169,6 +140,12 @@ static void launch_tiles(uint32_t occ, uint32_t var, bool hz, bool uw, unsigned 
         default: launch_tiles_ov<4, 0>(hz, uw, nb, st, a); break;
     }
 }
+
+}  // namespace vxh
+
+using vxh::TileArgs;
+using vxh::launch_tiles;
+using vxh::tuning_ok;
 
 extern "C" {
 
@@ -232,7 +209,7 @@ int vx_ctx_create(const vx_ctx_desc *d, vx_ctx **out) {
         for (auto &b : c->pool)
             if ((rc = ensure(b, MIN_CAP))) break;
         if (rc) break;
-        if ((rc = ensure(c->grid, VX_GRID_LEN * 4)) || (rc = ensure(c->counters, CNT_LEN * 4)) ||
+        if ((rc = ensure(c->grid, VX_GRID_LEN * 4)) || (rc = ensure(c->counters, CNT_TOTAL * 4)) ||
             (rc = ensure(c->chunks, 96 * 64)))
             break;
         if ((rc = fill_u32(c, c->grid.p, VX_NO_CHUNK, VX_GRID_LEN))) break;
@@ -249,16 +226,17 @@ int vx_ctx_create(const vx_ctx_desc *d, vx_ctx **out) {
 
 int vx_ctx_destroy(vx_ctx *c) {
     if (!c) return VX_E_ARG;
-    hipSetDevice(c->device);
-    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->n_batches > 0) return VX_E_BUSY;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     DevBuf *bufs[] = {&c->pool[0], &c->pool[1], &c->pool[2], &c->chunks, &c->grid, &c->vis,   &c->shadow,
                       &c->hiz,     &c->pairs,   &c->counters, &c->cost,  &c->order};
     for (DevBuf *b : bufs)
-        if (b->p) hipFree(b->p);
+        if (b->p) (void)hipFree(b->p);
     for (auto &set : c->ring)
         for (auto &e : set)
-            if (e) hipEventDestroy(e);
-    if (c->stream) hipStreamDestroy(c->stream);
+            if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return VX_OK;
 }
@@ -278,6 +256,18 @@ int vx_pool_sync(vx_ctx *c, int pool, const void *data, size_t bytes, const vx_r
     if (bytes % elem[pool]) return VX_E_ARG;
     HIPX(hipSetDevice(c->device));
     DevBuf &b = c->pool[pool];
+    const void *p0 = b.p;
+    const size_t len0 = c->pool_len[pool];
+    struct GenBump {
+        vx_ctx *c;
+        const DevBuf &b;
+        const void *p0;
+        size_t len0;
+        int pool;
+        ~GenBump() {
+            if (b.p != p0 || c->pool_len[pool] != len0) c->pool_gen++;
+        }
+    } bump{c, b, p0, len0, pool};
     if (!c->pool_synced[pool] && bytes <= b.cap) {
         // A new context has no mirror contents yet: its first sync is a full upload
         // whatever the dirty list says (another mirror may have drained it already).
@@ -345,7 +335,11 @@ int vx_set_chunks(vx_ctx *c, const vx_gpu_chunk *chunks, uint32_t n, const uint3
     return VX_OK;
 }
 
-static Pools make_pools(const vx_ctx *c) {
+}  // extern "C"
+
+namespace vxh {
+
+Pools make_pools(const vx_ctx *c) {
     Pools p;
     p.inners = (const uint4 *)c->pool[0].p;
     p.leaves = (const uint2 *)c->pool[1].p;
@@ -359,7 +353,7 @@ static Pools make_pools(const vx_ctx *c) {
     return p;
 }
 
-static FrameK make_frame(const vx_gpu_frame *g) {
+FrameK make_frame(const vx_gpu_frame *g) {
     FrameK f;
     std::memcpy(f.cam, g->cam_pos, 12);
     std::memcpy(f.fwd, g->cam_fwd, 12);
@@ -389,7 +383,7 @@ static FrameK make_frame(const vx_gpu_frame *g) {
 }
 
 // Override values for a (flags & SPEC_MASK, spec_hi) pipeline key (src/render/mod.rs:2355-2689).
-static int make_spec(uint32_t spec, uint32_t spec_hi, uint32_t frame_flags, Spec &s) {
+int make_spec(uint32_t spec, uint32_t spec_hi, uint32_t frame_flags, Spec &s) {
     if (spec & VX_FLAG_FOLIAGE) return VX_E_UNSUPPORTED;  // cross_quad samples the texture atlas
     if ((spec_hi & VX_FLAG_HI_SNELL_BEND) && (frame_flags & VX_FLAG_UNDERWATER)) return VX_E_UNSUPPORTED;
     s.bits = 0;
@@ -402,12 +396,28 @@ static int make_spec(uint32_t spec, uint32_t spec_hi, uint32_t frame_flags, Spec
     return VX_OK;
 }
 
+int check_frame(const vx_gpu_frame *g, uint32_t n_chunks) {
+    if (g->res[0] == 0 || g->res[1] == 0) return VX_E_ARG;
+    if (g->tiles[0] != (g->res[0] + 7) / 8 || g->tiles[1] != (g->res[1] + 7) / 8) return VX_E_ARG;
+    if ((uint64_t)g->tiles[0] * g->tiles[1] > VX_MAX_TILES) return VX_E_OVERFLOW;
+    if (g->chunk_count > n_chunks || g->chunk_count > VX_MAX_CHUNKS) return VX_E_ARG;
+    if ((uint64_t)g->grid_dim[0] * g->grid_dim[1] * g->grid_dim[2] > VX_GRID_LEN) return VX_E_ARG;
+    return VX_OK;
+}
+
+}  // namespace vxh
+
+using vxh::make_frame;
+using vxh::make_pools;
+using vxh::make_spec;
+
+extern "C" {
+
 int vx_render(vx_ctx *c, const vx_gpu_frame *g, uint32_t spec, uint32_t spec_hi) {
     if (!c || !g) return VX_E_ARG;
     if (g->res[0] != c->w || g->res[1] != c->h) return VX_E_ARG;
     if (g->tiles[0] != c->tx || g->tiles[1] != c->ty) return VX_E_ARG;
-    if (g->chunk_count > c->n_chunks || g->chunk_count > VX_MAX_CHUNKS) return VX_E_ARG;
-    if ((uint64_t)g->grid_dim[0] * g->grid_dim[1] * g->grid_dim[2] > VX_GRID_LEN) return VX_E_ARG;
+    if (int rc0 = vxh::check_frame(g, c->n_chunks)) return rc0;
     if (c->path == VX_PATH_PAIRS && g->max_pairs != c->max_pairs) return VX_E_ARG;
     Spec s;
     int rc = make_spec(spec, spec_hi, g->flags, s);
@@ -426,7 +436,7 @@ int vx_render(vx_ctx *c, const vx_gpu_frame *g, uint32_t spec, uint32_t spec_hi)
     c->ring_next = (c->ring_next + 1) % vx_ctx::RING;
     if (c->ring_count < (uint32_t)vx_ctx::RING) c->ring_count++;
     HIPX(hipEventRecord(c->ev[0], st));
-    if ((rc = fill_u32(c, counters, 0, CNT_LEN))) return rc;
+    if ((rc = fill_u32(c, counters, 0, CNT_TOTAL))) return rc;
     if (c->path == VX_PATH_PAIRS) {
         // clear vis (+ hiz when Hi-Z is off), tile_select, finalize, march,
         // [build_hiz, recover_select, finalize_recover, march] (src/render/mod.rs:1716-1795)
@@ -462,8 +472,10 @@ int vx_render(vx_ctx *c, const vx_gpu_frame *g, uint32_t spec, uint32_t spec_hi)
         HIPX(hipEventRecord(c->ev[1], st));
         TileArgs a{p, f, s, hiz, vis, (uint16_t *)c->shadow.p, counters, (const uint32_t *)c->order.p,
                    (uint32_t *)c->cost.p};
-        if (c->have_cost && c->ordered) k_order_tiles<<<1, 1024, 0, st>>>((const uint32_t *)c->cost.p, (uint32_t *)c->order.p, ntiles);
-        else k_iota<<<(ntiles + 255) / 256, 256, 0, st>>>((uint32_t *)c->order.p, ntiles);
+        if (c->have_cost && c->ordered)
+            vxh::launch_order(st, (const uint32_t *)c->cost.p, ntiles, counters + CNT_CTL, (uint32_t *)c->order.p);
+        else
+            k_iota_views<<<(ntiles + 255) / 256, 256, 0, st>>>((uint32_t *)c->order.p, nullptr, ntiles, nullptr, nullptr);
         LAUNCHED();
         launch_tiles(c->occ, c->variant, hiz_on, (f.flags & VX_FLAG_UNDERWATER) != 0u, ntiles, st, a);
         LAUNCHED();
@@ -540,7 +552,7 @@ int vx_read_decoded(vx_ctx *c, vx_hit *out, size_t n) {
                                                                  (const uint16_t *)c->shadow.p, d);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(out, d, n * sizeof(vx_hit), hipMemcpyDeviceToHost, c->stream);
-    hipFreeAsync(d, c->stream);
+    (void)hipFreeAsync(d, c->stream);
     hipError_t e2 = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(e);
     if (e2 != hipSuccess) return hip_fail(e2);

@@ -26,8 +26,11 @@ class Renderer:
         self.path = path
         self.view: ViewData | None = None
         self._frame = None
+        self._batches: list = []
 
     def close(self):
+        for b in list(self._batches):
+            b.close()
         if self._c:
             self._L.vx_ctx_destroy(self._c)
             self._c = C.c_void_p()
@@ -141,6 +144,116 @@ class Renderer:
         n = C.c_uint32()
         A.check(self._L.vx_timings_sum(self._c, C.byref(t), C.byref(n)), "vx_timings_sum")
         return {k: float(getattr(t, k)) for k, _ in A.Timings._fields_}, int(n.value)
+
+
+class Batch:
+    """Many views of the renderer's world per submission (vx_batch_*).
+
+    No counterpart in the reference, whose request loop renders one view per
+    ``Renderer::render`` call (src/headless.rs:680-837). Views share the renderer's
+    pools; each keeps its own Hi-Z, visibility and shadow buffers, and its output bits
+    equal ``Renderer.render`` on the same frame, chunk list, grid and Hi-Z input.
+    """
+
+    def __init__(self, renderer: Renderer, max_views: int):
+        self._L = renderer._L
+        self._r = renderer
+        self._b = C.c_void_p()
+        A.check(self._L.vx_batch_create(renderer.handle, max_views, C.byref(self._b)), "vx_batch_create")
+        renderer._batches.append(self)
+        self.views: list[ViewData] = []
+        self._keep = None
+
+    def close(self):
+        if self._b:
+            self._L.vx_batch_destroy(self._b)
+            self._b = C.c_void_p()
+            if self in self._r._batches:
+                self._r._batches.remove(self)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_views(self, views: list[ViewData], specs: list[tuple[int, int]]):
+        """Upload the views (their frames, chunk lists and grids) and their pipeline keys."""
+        descs = (A.ViewDesc * len(views))()
+        keep = []
+        for i, (v, (spec, spec_hi)) in enumerate(zip(views, specs)):
+            ch = np.ascontiguousarray(v.chunks[: v.n_total])
+            fr = A.Frame.from_buffer_copy(bytes(v.frame))
+            keep += [ch, fr, v.grid]
+            descs[i] = A.ViewDesc(C.addressof(fr), ch.ctypes.data if v.n_total else None, v.n_total,
+                                  v.grid.ctypes.data, spec, spec_hi)
+        A.check(self._L.vx_batch_set_views(self._b, descs, len(views)), "vx_batch_set_views")
+        self.views = list(views)
+        self._keep = keep
+
+    def set_tuning(self, mode: int = A.BATCH_PERSISTENT, occupancy: int = 4, variant: int = 4, ordered: int = 1,
+                   queue_chunk: int = 1, waves_per_simd: int | None = None, graph: int = 1):
+        wps = occupancy if waves_per_simd is None else waves_per_simd
+        t = A.BatchTuning(mode, occupancy, variant, ordered, queue_chunk, wps, graph)
+        A.check(self._L.vx_batch_set_tuning(self._b, C.byref(t)), "vx_batch_set_tuning")
+
+    def tuning(self) -> dict:
+        t = A.BatchTuning()
+        A.check(self._L.vx_batch_get_tuning(self._b, C.byref(t)), "vx_batch_get_tuning")
+        return {k: int(getattr(t, k)) for k, _ in A.BatchTuning._fields_}
+
+    def autotune(self, reps: int = 3) -> tuple[dict, float]:
+        t = A.BatchTuning()
+        ms = C.c_float()
+        A.check(self._L.vx_batch_autotune(self._b, reps, C.byref(t), C.byref(ms)), "vx_batch_autotune")
+        return {k: int(getattr(t, k)) for k, _ in A.BatchTuning._fields_}, float(ms.value)
+
+    def render(self):
+        A.check(self._L.vx_batch_render(self._b), "vx_batch_render")
+
+    def sync(self):
+        A.check(self._L.vx_batch_sync(self._b), "vx_batch_sync")
+
+    def reset_hiz(self):
+        A.check(self._L.vx_batch_reset_hiz(self._b), "vx_batch_reset_hiz")
+
+    def _size(self, i):
+        v = self.views[i]
+        return int(v.frame.res[0]), int(v.frame.res[1])
+
+    def read_vis(self, i: int) -> np.ndarray:
+        w, h = self._size(i)
+        out = np.zeros(w * h, dtype=np.uint64)
+        A.check(self._L.vx_batch_read_vis(self._b, i, A.ptr(out), out.size), "vx_batch_read_vis")
+        return out
+
+    def read_shadow(self, i: int) -> np.ndarray:
+        w, h = self._size(i)
+        out = np.zeros(w * h, dtype=np.uint16)
+        A.check(self._L.vx_batch_read_shadow(self._b, i, A.ptr(out), out.size), "vx_batch_read_shadow")
+        return out
+
+    def read_hiz(self, i: int) -> np.ndarray:
+        w, h = self._size(i)
+        out = np.zeros(((w + 7) // 8) * ((h + 7) // 8), dtype=np.uint32)
+        A.check(self._L.vx_batch_read_hiz(self._b, i, A.ptr(out), out.size), "vx_batch_read_hiz")
+        return out
+
+    def demand(self) -> dict:
+        d = A.PairDemand()
+        A.check(self._L.vx_batch_demand_get(self._b, C.byref(d)), "vx_batch_demand_get")
+        return {k: int(getattr(d, k)) for k, _ in A.PairDemand._fields_}
+
+    def timings_reset(self):
+        A.check(self._L.vx_batch_timings_reset(self._b), "vx_batch_timings_reset")
+
+    def timings_sum(self) -> tuple[float, float, int]:
+        """(submission ms, march-kernel ms, frames) summed since timings_reset."""
+        ms = C.c_float()
+        mm = C.c_float()
+        n = C.c_uint32()
+        A.check(self._L.vx_batch_timings_sum(self._b, C.byref(ms), C.byref(mm), C.byref(n)), "vx_batch_timings_sum")
+        return float(ms.value), float(mm.value), int(n.value)
 
 
 def device_name() -> str:
