@@ -1,15 +1,21 @@
 #!/usr/bin/env python3
 """Benchmark: Grays/s (primary + sun-shadow rays) of the voxel ray march on one MI355X.
 
-Default workload: the 1024x1024x256 world (seed 7) rendered by one 1920x1080 camera at
-the default pose, primary DDA + one sun shadow ray per lit pixel, two-phase Hi-Z on 8x8
-tiles. A step is one frame: march + Hi-Z + shadow over the resident pools; the camera is
-fixed, so frames after the first run in the Hi-Z steady state the reference's
---bench-frames reports. Inputs (pools, chunk list, grid, frame) are in HBM before the
-timed region starts.
+Default workload (--workload 2): the 1024x1024x256 world (seed 7) seen by 64 cameras of
+uneven resolution (uniform over 256x144, 640x360, 1280x720, 1920x1080; seeded poses
+20-60 blocks above the terrain), all rendered per step: primary DDA, one sun shadow ray
+per lit pixel, two-phase Hi-Z on 8x8 tiles. A step is one frame of every view. The
+batched form (the launch form picked by the autotuner, replayed from a hipGraph) is
+`value`; the same views rendered with one dispatch per view (inputs equally resident)
+are reported beside it as `plain`, with the ratio. Every view keeps its Hi-Z words
+between steps, so steps after the first run in the Hi-Z steady state.
 
-Output: one JSON line with the metric, a roofline object for the dominant kernel
-(the tile march) and a cpu_baseline object (the C oracle timed on the host cores).
+--workload 1 measures one 1920x1080 camera per step instead; --workload 0 the small
+640x360 case. Inputs (pools, chunk lists, grids, frames) are in HBM before the timed
+region starts.
+
+Output: one JSON line with the metric, a roofline object for the dominant kernel and a
+cpu_baseline object (the C oracle timed on the host cores).
 """
 from __future__ import annotations
 
@@ -25,11 +31,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "Grays/s primary+shadow, batched vs per-view plain; p50/p99 view latency at QPS"
 
 
 def _fail(msg: str, code: int = 2):
     print(msg, file=sys.stderr, flush=True)
     sys.exit(code)
+
+
+def _log(msg: str):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def traffic_from_profiles(kernel: str, workload: str):
@@ -47,67 +58,179 @@ def traffic_from_profiles(kernel: str, workload: str):
     return None
 
 
-def cpu_baseline(world, view, spec, spec_hi, budget_s: float):
+def _threads():
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return max(1, min(n, 64))
+
+
+def cpu_baseline(world, views, specs, budget_s: float):
     """The oracle (plain-C restatement of the WGSL passes) on this host's cores.
 
-    Runs whole frames (the same view, Hi-Z carried) until budget_s has elapsed and
-    returns the rate plus the oracle's algorithmic byte counts of one steady-state
-    frame, split into the primary pass and the shadow pass."""
+    First runs one frame of every view (Hi-Z from zero) and, per view, the steady-state
+    frame split into the primary pass and the shadow pass to get the exact algorithmic
+    byte counts; then times whole steady-state frames view after view until budget_s
+    has elapsed. Returns (cpu_baseline object, summed primary stats, summed shadow stats)."""
     from oracle import oracle as O
 
-    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-    threads = max(1, min(threads, 64))
-    sc = O.Scene(world.pool(0), world.pool(1), world.pool(2), view.chunks, view.grid)
-    sp = O.spec_from(spec, spec_hi)
-    hz = np.zeros(view.tiles[0] * view.tiles[1], np.uint32)
-    _, _, hz, _ = O.render(sc, view.frame_bytes, sp, hz, shadow=False, threads=threads)
-    vis, _, _, prim = O.render(sc, view.frame_bytes, sp, hz, shadow=False, threads=threads)
-    _, sh_stats = O.primary_shadow(sc, view.frame_bytes, sp, vis, threads=threads)
-    frames, rays, t0 = 0, 0, time.perf_counter()
+    threads = _threads()
+    scenes, hizs = [], []
+    prim = {"node_bytes": 0}
+    sh = {"node_bytes": 0, "rays_shadow": 0}
+    for v, (spec, spec_hi) in zip(views, specs):
+        sc = O.Scene(world.pool(0), world.pool(1), world.pool(2), v.chunks, v.grid)
+        sp = O.spec_from(spec, spec_hi)
+        hz = np.zeros(v.tiles[0] * v.tiles[1], np.uint32)
+        _, _, hz, _ = O.render(sc, v.frame_bytes, sp, hz, shadow=False, threads=threads)
+        vis, _, _, p = O.render(sc, v.frame_bytes, sp, hz, shadow=False, threads=threads)
+        _, s = O.primary_shadow(sc, v.frame_bytes, sp, vis, threads=threads)
+        prim["node_bytes"] += p["node_bytes"]
+        sh["node_bytes"] += s["node_bytes"]
+        sh["rays_shadow"] += s["rays_shadow"]
+        scenes.append((sc, sp))
+        hizs.append(hz)
+    _log("cpu baseline: byte counts done")
+    nviews, rays, t0 = 0, 0, time.perf_counter()
     while True:
-        _, _, hz, st = O.render(sc, view.frame_bytes, sp, hz, shadow=True, threads=threads)
-        frames += 1
-        rays += view.width * view.height + st["rays_shadow"]
+        i = nviews % len(views)
+        v = views[i]
+        sc, sp = scenes[i]
+        _, _, hizs[i], st = O.render(sc, v.frame_bytes, sp, hizs[i], shadow=True, threads=threads)
+        nviews += 1
+        rays += int(v.frame.res[0]) * int(v.frame.res[1]) + st["rays_shadow"]
         if time.perf_counter() - t0 >= budget_s:
             break
     dt = time.perf_counter() - t0
+    sizes = sorted({(int(v.frame.res[0]), int(v.frame.res[1])) for v in views})
     return {
         "value": rays / dt / 1e9, "unit": "Grays/s", "cores": threads, "kind": "port",
-        "sample": f"{frames} full {view.width}x{view.height} frames of the same view "
-                  f"(primary + shadow + Hi-Z) in {dt:.1f} s",
-    }, prim, sh_stats
+        "sample": f"{nviews} view-frames of the same {len(views)} views (sizes {sizes}; primary + shadow + "
+                  f"Hi-Z, steady state) in {dt:.1f} s",
+    }, prim, sh
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__)
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", type=int, default=1, choices=[0, 1])
-    ap.add_argument("--path", choices=["tiles", "pairs"], default="tiles")
-    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle work for cpu_baseline")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--calibrate-copy", action="store_true",
-                    help="first run one 256 MiB device-to-device copy (scale reference for PMC byte counters)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        _fail("bench.py measures one GPU: this build has no multi-GPU path (views would shard by GPU; "
-              "see the design notes). Run with --gpus 1.")
-    if args.steps < 1 or args.steps > 256:
-        _fail("--steps must be in [1, 256]")
+def _time_batch(b, steps):
+    b.sync()
+    b.timings_reset()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.render()
+    b.sync()
+    dt = time.perf_counter() - t0
+    tot, mar, n = b.timings_sum()
+    return dt, tot / max(n, 1), mar / max(n, 1)
 
+
+def run_batch(args, out):
+    from voxelcraft_menu_amd import _abi as A
+    from voxelcraft_menu_amd import workloads as W
+    from voxelcraft_menu_amd.renderer import Batch, Renderer, device_name
+    from voxelcraft_menu_amd.world import build_view
+
+    wl = W.WORKLOADS[2]
+    world = W.make_world(wl)
+    cams = W.random_cameras(wl, wl.views, seed=2)
+    views = [build_view(world, c) for c in cams]
+    specs = [W.spec_words(c.flags, world) for c in cams]
+    px = sum(int(v.frame.res[0]) * int(v.frame.res[1]) for v in views)
+    tiles = sum(v.tiles[0] * v.tiles[1] for v in views)
+    _log(f"{len(views)} views, {px} pixels")
+    r = Renderer(256, 144)
+    r.sync_world(world)
+    b = Batch(r, len(views))
+    b.set_views(views, specs)
+    best, best_ms = b.autotune(3)
+    _log(f"autotuned: {best} ({best_ms:.3f} ms)")
+    for _ in range(max(args.warmup, 1)):
+        b.render()
+    b.sync()
+    d = b.demand()
+    rays_per_step = px + d["shadow_rays"]
+
+    dt, _, _ = _time_batch(b, args.steps)
+    value = rays_per_step * args.steps / dt / 1e9
+    _log(f"batched: {dt / args.steps * 1e3:.3f} ms/step")
+
+    # The march kernel alone (HIP events around it on the context stream): the same
+    # launch form without graph replay, a few frames.
+    tuned = dict(best)
+    kt = dict(best)
+    kt["graph"] = 0
+    b.set_tuning(**kt)
+    _, _, march_ms = _time_batch(b, min(args.steps, 10))
+
+    # Per-view plain: one dispatch per view (cost-order kernels + the single-view tile
+    # march), the same resident inputs, no host synchronisation between views.
+    b.set_tuning(mode=A.BATCH_PLAIN, occupancy=4, variant=4, ordered=1, graph=0)
+    b.render()
+    pdt, _, _ = _time_batch(b, args.steps)
+    plain = rays_per_step * args.steps / pdt / 1e9
+    _log(f"plain: {pdt / args.steps * 1e3:.3f} ms/step")
+    b.set_tuning(**tuned)
+    b.close()
+
+    kname = {A.BATCH_FLAT: "k_batch_flat", A.BATCH_PERSISTENT: "k_batch_persistent"}.get(best["mode"], "k_march_tiles")
+    out.update({
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "Grays/s",
+        "ms_per_step": round(dt / args.steps * 1e3, 4),
+        "plain": {"value": round(plain, 4), "ms_per_step": round(pdt / args.steps * 1e3, 4),
+                  "form": "one dispatch per view (cost-order kernels + tile march), inputs resident"},
+        "batched_over_plain": round(value / plain, 3),
+        "config": {
+            "workload": wl.name, "views": len(views), "resolutions": "uniform over " + str(W.RESOLUTIONS),
+            "camera_seed": 2, "world_chunks": [wl.gen.chunk_max[i] - wl.gen.chunk_min[i] for i in range(3)],
+            "seed": wl.gen.seed, "hiz": True, "primary_rays": px, "shadow_rays": d["shadow_rays"],
+            "tiles": tiles, "tuning": tuned, "device": device_name(),
+        },
+    })
+
+    # Single 1920x1080 view (the plain-at-scale case), for reference.
+    wl1 = W.WORKLOADS[1]
+    cam1 = W.default_camera(wl1)
+    v1 = build_view(world, cam1)
+    s1 = W.spec_words(cam1.flags, world)
+    r1 = Renderer(wl1.width, wl1.height)
+    r1.sync_world(world)
+    r1.set_view(v1)
+    for _ in range(3):
+        r1.render(v1.frame, *s1)
+    r1.sync()
+    sr = r1.pair_demand()["shadow_rays"]
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r1.render(v1.frame, *s1)
+    r1.sync()
+    dt1 = time.perf_counter() - t0
+    out["single_view_1080p"] = {"value": round((wl1.width * wl1.height + sr) * args.steps / dt1 / 1e9, 4),
+                                "ms_per_step": round(dt1 / args.steps * 1e3, 4)}
+    r1.close()
+    r.close()
+
+    if args.no_cpu_baseline:
+        out["cpu_baseline"] = None
+        out["roofline"] = None
+        return
+    cb, prim, sh = cpu_baseline(world, views, specs, args.cpu_budget)
+    out["cpu_baseline"] = cb
+    # Algorithmic bytes of one batch launch: the oracle's exact per-(ray, chunk) node
+    # accounting of the primary and shadow rays of every view, plus per pixel the 8-byte
+    # key and 2-byte R16F shadow word, plus the 4-byte Hi-Z word per tile.
+    alg = prim["node_bytes"] + sh["node_bytes"] + 10 * px + 4 * tiles
+    achieved = alg / (march_ms * 1e-3) / 1e9
+    out["roofline"] = {
+        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(kname, "workload2"),
+        "kernel": kname, "kernel_ms": round(march_ms, 4), "alg_bytes_per_launch": int(alg),
+        "alg_bytes_per_pixel": round(alg / px, 2),
+    }
+
+
+def run_single(args, out):
     from voxelcraft_menu_amd import _abi as A
     from voxelcraft_menu_amd import workloads as W
     from voxelcraft_menu_amd.renderer import Renderer, device_name
     from voxelcraft_menu_amd.world import build_view
-
-    if args.calibrate_copy:
-        import torch
-        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
 
     wl = W.WORKLOADS[args.workload]
     world = W.make_world(wl)
@@ -133,83 +256,86 @@ def main():
     for _ in range(args.steps):
         r.render(view.frame, spec, spec_hi)
     r.sync()
-    t1 = time.perf_counter()
+    dt = time.perf_counter() - t0
     sums, nframes = r.timings_sum()
-    dt = t1 - t0
     value = rays_per_step * args.steps / dt / 1e9
     march_ms = (sums["march"] + sums["march2"]) / max(nframes, 1)
     shadow_ms = sums["shadow"] / max(nframes, 1)
-
-    out = {
-        "metric": "Grays/s primary+shadow (plain march, one 1920x1080 view per step)"
-        if args.workload == 1 else "Grays/s primary+shadow (plain march, 640x360)",
+    fused = path == A.PATH_TILES and (r.tuning()["variant"] & 4) != 0
+    r.close()
+    out.update({
+        "metric": METRIC,
         "value": round(value, 4),
         "unit": "Grays/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
         "ms_per_step": round(dt / args.steps * 1e3, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic seeded world (project terrain generator), fixed camera",
         "config": {
-            "workload": wl.name,
-            "resolution": [wl.width, wl.height],
+            "workload": wl.name, "resolution": [wl.width, wl.height],
             "world_chunks": [wl.gen.chunk_max[i] - wl.gen.chunk_min[i] for i in range(3)],
-            "seed": wl.gen.seed,
-            "hiz": True,
-            "path": args.path,
-            "primary_rays": wl.width * wl.height,
-            "shadow_rays": shadow_rays,
-            "chunks_visible": view.n_visible,
-            "chunks_total": view.n_total,
+            "seed": wl.gen.seed, "hiz": True, "path": args.path, "primary_rays": wl.width * wl.height,
+            "shadow_rays": shadow_rays, "chunks_visible": view.n_visible, "chunks_total": view.n_total,
             "device": device_name(),
         },
         "passes_ms": {k: round(v / max(nframes, 1), 4) for k, v in sums.items()},
+    })
+    if args.no_cpu_baseline:
+        out["cpu_baseline"] = None
+        out["roofline"] = None
+        return
+    cb, prim, sh = cpu_baseline(world, [view], [(spec, spec_hi)], args.cpu_budget)
+    out["cpu_baseline"] = cb
+    tiles = view.tiles[0] * view.tiles[1]
+    px = wl.width * wl.height
+    alg = prim["node_bytes"] + 8 * px + 4 * tiles
+    kernel_ms = march_ms
+    if fused:
+        alg += sh["node_bytes"] + 2 * px
+        kernel_ms = march_ms + shadow_ms
+    achieved = alg / (kernel_ms * 1e-3) / 1e9
+    out["roofline"] = {
+        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic_from_profiles("k_march_tiles", f"workload{args.workload}") if path == A.PATH_TILES
+        else None,
+        "kernel": "k_march_tiles" if path == A.PATH_TILES else "k_march_pairs",
+        "kernel_ms": round(kernel_ms, 4), "fused_shadow": fused, "alg_bytes_per_launch": int(alg),
+        "alg_bytes_per_pixel": round(alg / px, 2),
     }
 
-    prim = sh = None
-    if not args.no_cpu_baseline:
-        cb, prim, sh = cpu_baseline(world, view, spec, spec_hi, args.cpu_budget)
-        out["cpu_baseline"] = cb
-    else:
-        out["cpu_baseline"] = None
 
-    # Roofline of the dominant kernel: algorithmic bytes per launch (the oracle's exact
-    # per-(ray, chunk) node accounting plus the 8-byte key per pixel and 4-byte Hi-Z per
-    # tile; with the fused shadow also the shadow rays' node traffic and the 2-byte R16F
-    # word per pixel) over the kernel's average duration measured above with HIP events.
-    fused = path == A.PATH_TILES and (r.tuning()["variant"] & 4) != 0
-    if prim is not None:
-        tiles = view.tiles[0] * view.tiles[1]
-        px = wl.width * wl.height
-        alg = prim["node_bytes"] + 8 * px + 4 * tiles
-        kernel_ms = march_ms
-        if fused:
-            alg += sh["node_bytes"] + 2 * px
-            kernel_ms = march_ms + shadow_ms
-        achieved = alg / (kernel_ms * 1e-3) / 1e9
-        out["roofline"] = {
-            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic_from_profiles("k_march_tiles", f"workload{args.workload}") if path == A.PATH_TILES
-            else None,
-            "kernel": "k_march_tiles" if path == A.PATH_TILES else "k_march_pairs",
-            "kernel_ms": round(kernel_ms, 4),
-            "fused_shadow": fused,
-            "alg_bytes_per_launch": int(alg),
-            "alg_bytes_per_pixel": round(alg / px, 2),
-        }
-        if not fused:
-            sh_alg = sh["node_bytes"] + 8 * px + 2 * px
-            out["roofline_shadow"] = {
-                "kernel": "k_shadow", "kernel_ms": round(shadow_ms, 4),
-                "achieved": round(sh_alg / (shadow_ms * 1e-3) / 1e9, 1), "unit": "GB/s",
-            }
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", type=int, default=2, choices=[0, 1, 2])
+    ap.add_argument("--path", choices=["tiles", "pairs"], default="tiles", help="single-view workloads only")
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle work for cpu_baseline")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run one 256 MiB device-to-device copy (scale reference for PMC byte counters)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        _fail("bench.py measures one GPU: this build has no multi-GPU path (views would shard across GPUs by "
+              "view, with the pools replicated). Run with --gpus 1.")
+    if args.steps < 1 or args.steps > 256:
+        _fail("--steps must be in [1, 256]")
+
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
+    out = {"metric": METRIC, "value": None, "unit": "Grays/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic seeded world (project terrain generator), seeded cameras"}
+    if args.workload == 2:
+        run_batch(args, out)
     else:
-        out["roofline"] = None
+        run_single(args, out)
     print(json.dumps(out), flush=True)
 
 

@@ -52,6 +52,13 @@ def test_error_strings_and_no_device_path():
     assert L.vx_ctx_create(None, None) == -3
     assert L.vx_render(None, None, 0, 0) == -3
     assert L.vx_pool_sync(None, 0, None, 0, None, 0) == -3
+    out = C.c_void_p()
+    assert L.vx_batch_create(None, 4, C.byref(out)) == -3
+    assert L.vx_batch_set_views(None, None, 1) == -3
+    assert L.vx_batch_render(None) == -3
+    assert L.vx_batch_set_tuning(None, None) == -3
+    assert L.vx_batch_autotune(None, 3, None, None) == -3
+    assert L.vx_batch_read_vis(None, 0, None, 0) == -3
     if os.environ.get("HIP_VISIBLE_DEVICES") == "" or not os.path.exists("/dev/kfd"):
         ctx = C.c_void_p()
         desc = A.CtxDesc(64, 64, 0, 0, -1)

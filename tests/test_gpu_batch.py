@@ -25,6 +25,7 @@ FORMS = [
     dict(mode=A.BATCH_PERSISTENT, graph=0, queue_chunk=1),
     dict(mode=A.BATCH_PERSISTENT, graph=1, queue_chunk=3, waves_per_simd=2),
     dict(mode=A.BATCH_PERSISTENT, graph=1, variant=5),
+    dict(mode=A.BATCH_PERSISTENT, graph=1, queue_chunk=4, variant=5),
     dict(mode=A.BATCH_PERSISTENT, graph=1, ordered=0, occupancy=6, queue_chunk=8),
 ]
 

@@ -474,11 +474,9 @@ int vx_batch_autotune(vx_batch *b, uint32_t reps, vx_batch_tuning *best, float *
         t.graph = 1;
         cand.push_back(t);
     };
-    for (uint32_t chunk : {1u, 2u, 4u}) {
-        add(VX_BATCH_PERSISTENT, 4, 4, chunk, 4);
-        add(VX_BATCH_PERSISTENT, 6, 4, chunk, 6);
-    }
-    add(VX_BATCH_PERSISTENT, 4, 5, 1, 4);
+    for (uint32_t chunk : {1u, 2u, 4u, 8u}) add(VX_BATCH_PERSISTENT, 4, 4, chunk, 4);
+    add(VX_BATCH_PERSISTENT, 4, 5, 4u, 4);
+    add(VX_BATCH_PERSISTENT, 6, 4, 4u, 6);
     add(VX_BATCH_FLAT, 4, 4, 1, 4);
     add(VX_BATCH_FLAT, 6, 4, 1, 6);
     add(VX_BATCH_FLAT, 4, 5, 1, 4);

@@ -222,9 +222,11 @@ __device__ __forceinline__ void batch_tile(const BatchArgs &a, uint32_t entry, u
     entry = (uint32_t)__builtin_amdgcn_readfirstlane((int)entry);  // uniform: descriptor reads are scalar
     const uint32_t v = entry >> ENTRY_VIEW_SHIFT, tile = entry & ENTRY_TILE_MASK;
     if (v >= n_views) return;
+    // References into the descriptor, not copies: the march indexes frame arrays with
+    // run-time indices, and a local copy would be placed in per-lane scratch memory.
     const ViewK *vk = a.views + v;
-    const FrameK f = vk->f;
-    const Spec s = vk->s;
+    const FrameK &f = vk->f;
+    const Spec &s = vk->s;
     if (tile >= f.tiles[0] * f.tiles[1]) return;
     Pools p = a.p;
     p.chunks = a.p.chunks + vk->chunk_off;
@@ -262,7 +264,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kOcc))) void
 
 // Persistent form: the grid is sized to the resident wave slots; every wave takes
 // `chunk` consecutive queue entries per atomic on the head word until the queue is
-// exhausted (the head only grows, so every wave reaches the exit).
+// exhausted (the head only grows, so every wave reaches the exit). The queue is sorted
+// most expensive first, so small chunks keep the heavy head of the queue spread over
+// many waves.
 template <int kOcc, int kVar>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kOcc))) void k_batch_persistent(BatchArgs a,
                                                                                                    uint32_t chunk) {
@@ -271,15 +275,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kOcc))) void
     const uint32_t n = a.hdr[BH_TILES];
     const uint32_t nv = a.hdr[BH_VIEWS];
     uint32_t *head = a.counters + CNT_CTL + CTL_HEAD;
-    chunk = chunk ? chunk : 1u;
+    const uint32_t grab = chunk ? chunk : 1u;
     for (;;) {
         uint32_t base = 0;
-        if (lane_id() == 0) base = atomicAdd(head, chunk);
+        if (lane_id() == 0) base = atomicAdd(head, grab);
         base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);  // all lanes active: lane 0's value
         if (base >= n) break;
         if (base < kPrioTiles) __builtin_amdgcn_s_setprio(3);
         else __builtin_amdgcn_s_setprio(0);
-        const uint32_t end = min(base + chunk, n);
+        const uint32_t end = min(base + grab, n);
         for (uint32_t k = base; k < end; k++) batch_tile<kVar>(a, a.order[k], nv, defmask, selmask);
     }
 }
