@@ -231,6 +231,7 @@ typedef struct vx_batch_tuning {
 } vx_batch_tuning;
 
 typedef struct vx_batch vx_batch;
+typedef struct vx_serve vx_serve;
 
 int vx_batch_create(vx_ctx *ctx, uint32_t max_views, vx_batch **out);
 int vx_batch_destroy(vx_batch *b);
@@ -256,6 +257,54 @@ int vx_batch_demand_get(vx_batch *b, vx_pair_demand *out);
 int vx_batch_timings_reset(vx_batch *b);
 int vx_batch_timings_sum(vx_batch *b, float *total_ms, float *march_ms, uint32_t *frames);
 int vx_batch_device_outputs(vx_batch *b, uint32_t view, uint64_t **vis, uint16_t **shadow);
+
+/* ---- Serving: continuous batching of view requests --------------------------------
+ * No counterpart in the reference (its request loop renders and waits one view at a
+ * time, src/headless.rs:419-497). Requests are copied at submit and queued in order; a
+ * worker thread admits them onto free launch lanes (one HIP stream each) while view
+ * slots and the HBM budget allow, and launches each lane's admitted views as one batch,
+ * so requests arriving while other lanes run overlap with them on the device. Outputs
+ * stay in the request's region of the device output arena until vx_serve_release.
+ * Results are the bits vx_render gives for the same view with Hi-Z starting at zero.
+ * The context's pools may only change (vx_pool_sync) between vx_serve_barrier, which
+ * waits for every earlier request and holds later ones, and vx_serve_resume. */
+typedef struct vx_serve_desc {
+    uint32_t lanes;            /* concurrent batches on the device, 1..8 */
+    uint32_t max_views;        /* views per batch, 1..1024 */
+    uint64_t hbm_budget;       /* device bytes for in-flight request state: outputs
+                                  (10 B/pixel) + inputs (chunk list, grid, tile words) */
+    vx_batch_tuning tuning;    /* launch form of every batch (FLAT or PERSISTENT) */
+} vx_serve_desc;
+
+typedef struct vx_serve_result {
+    uint64_t id;               /* the caller's request id */
+    int32_t status;            /* VX_OK or the error of its batch */
+    uint32_t lane, batch_views, width, height, pad;
+    uint64_t batch;            /* batch sequence number (1, 2, ...) */
+    double t_arrival, t_submit, t_launch, t_done;  /* seconds, vx_serve_clock */
+} vx_serve_result;
+
+typedef struct vx_serve_stats {
+    uint64_t submitted, completed, launches, max_batch_views;
+    uint64_t shadow_rays;      /* sun rays traced by completed batches */
+    uint64_t pending, used_bytes, budget_bytes;
+} vx_serve_stats;
+
+double vx_serve_clock(void);  /* monotonic seconds, the clock of vx_serve_result */
+int vx_serve_create(vx_ctx *ctx, const vx_serve_desc *d, vx_serve **out);
+int vx_serve_destroy(vx_serve *s);
+/* t_arrival: when the request arrived (vx_serve_clock), 0 = now. VX_E_OVERFLOW if its
+ * footprint exceeds the whole budget. */
+int vx_serve_submit(vx_serve *s, uint64_t id, const vx_view *view, double t_arrival);
+/* Completed requests (up to cap), waiting up to wait_ms for the first. */
+int vx_serve_poll(vx_serve *s, vx_serve_result *out, uint32_t cap, uint32_t *n, int wait_ms);
+int vx_serve_read_vis(vx_serve *s, uint64_t id, uint64_t *out, size_t n);
+int vx_serve_read_shadow(vx_serve *s, uint64_t id, uint16_t *out, size_t n);
+/* Frees a completed request's outputs and budget (VX_E_BUSY while it is in flight). */
+int vx_serve_release(vx_serve *s, uint64_t id);
+int vx_serve_barrier(vx_serve *s);
+int vx_serve_resume(vx_serve *s);
+int vx_serve_stats_get(vx_serve *s, vx_serve_stats *out);
 const char *vx_error_string(int code);
 int vx_device_name(char *buf, size_t cap);
 

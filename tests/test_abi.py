@@ -70,3 +70,23 @@ def test_host_argument_checks():
     assert L.vxw_insert_dense(None, 0, 0, 0, None) == -3
     assert L.vxw_build_view(None, None, None, None, None) == -3
     assert L.vxw_pack_frame(None, None, 0, None) == -3
+
+
+def test_struct_sizes_match_the_c_header(tmp_path):
+    """ctypes mirrors against the C compiler's view of include/vxmarch.h."""
+    import subprocess
+
+    names = {"vx_gpu_frame": A.Frame, "vx_gpu_chunk": A.Chunk, "vx_edit": A.Edit, "vx_range": A.Range,
+             "vx_ctx_desc": A.CtxDesc, "vx_tuning": A.Tuning, "vx_pair_demand": A.PairDemand,
+             "vx_timings": A.Timings, "vx_hit": A.Hit, "vx_view": A.ViewDesc, "vx_batch_tuning": A.BatchTuning,
+             "vx_serve_desc": A.ServeDesc, "vx_serve_result": A.ServeResult, "vx_serve_stats": A.ServeStats}
+    src = tmp_path / "sz.c"
+    body = "".join(f'    printf("{n} %zu\\n", sizeof({n}));\n' for n in names)
+    src.write_text(f'#include <stdio.h>\n#include "vxmarch.h"\nint main(void) {{\n{body}    return 0;\n}}\n')
+    exe = tmp_path / "sz"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.run(["gcc", "-std=c99", f"-I{inc}", str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    got = dict(zip(out[0::2], (int(x) for x in out[1::2])))
+    for n, cls in names.items():
+        assert got[n] == C.sizeof(cls), (n, got[n], C.sizeof(cls))

@@ -1,0 +1,162 @@
+"""GPU tests of the continuous-batching view server (vx_serve_*).
+
+Each request's key and shadow words must be the same bits whether it runs alone or
+batched with many others on concurrent lanes, and must equal the oracle's first frame
+of that view (Hi-Z starting at zero). Admission must respect the HBM budget, and a
+barrier must order edits between the requests submitted before and after it.
+"""
+import numpy as np
+import pytest
+
+from tests import helpers as H
+from voxelcraft_menu_amd import _abi as A
+from voxelcraft_menu_amd import workloads as W
+from voxelcraft_menu_amd.renderer import Renderer, Server
+from voxelcraft_menu_amd.world import build_view
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small_world():
+    return W.make_world(W.WORKLOADS[0])
+
+
+def _views(world, n, seed, res):
+    cams = W.random_cameras(W.WORKLOADS[0], n, seed=seed, resolutions=res)
+    return [(build_view(world, c), W.spec_words(c.flags, world)) for c in cams]
+
+
+def _drain(srv, ids, timeout_polls=2000):
+    done = {}
+    polls = 0
+    while len(done) < len(ids):
+        for r in srv.poll(wait_ms=50):
+            done[r["id"]] = r
+        polls += 1
+        assert polls < timeout_polls, f"only {len(done)} of {len(ids)} requests completed"
+    return done
+
+
+def _read(srv, rid, v):
+    w, h = int(v.frame.res[0]), int(v.frame.res[1])
+    return srv.read_vis(rid, w, h), srv.read_shadow(rid, w, h)
+
+
+@pytest.mark.parametrize("mode", [A.BATCH_FLAT, A.BATCH_PERSISTENT])
+def test_results_same_alone_and_in_the_mix(small_world, mode):
+    probes = _views(small_world, 6, 51, [(333, 197), (64, 64), (120, 250), (40, 24)])
+    others = _views(small_world, 40, 52, [(256, 144), (96, 80), (200, 120)])
+    r = Renderer(64, 64)
+    r.sync_world(small_world)
+    srv = Server(r, lanes=3, max_views=8, hbm_budget=256 << 20, mode=mode)
+    alone = []
+    for i, (v, (s, hi)) in enumerate(probes):
+        print(f"alone {i}", flush=True)
+        srv.submit(i, v, s, hi)
+        res = _drain(srv, [i])
+        assert res[i]["status"] == 0 and res[i]["batch_views"] == 1
+        alone.append(_read(srv, i, v))
+        srv.release(i)
+        ref = H.oracle_frames(small_world, v, s, hi, frames=1)[0]
+        assert np.array_equal(alone[-1][0], ref[0]), f"probe {i} vis vs oracle"
+        assert np.array_equal(alone[-1][1], ref[1]), f"probe {i} shadow vs oracle"
+    # The mix: probes interleaved with other requests, submitted as fast as possible so
+    # they share batches and lanes run concurrently.
+    ids = []
+    rid = 100
+    for k in range(len(others)):
+        v, (s, hi) = others[k]
+        srv.submit(rid, v, s, hi)
+        ids.append(rid)
+        rid += 1
+        if k % 7 == 3 and (k // 7) < len(probes):
+            p = k // 7
+            v, (s, hi) = probes[p]
+            srv.submit(1000 + p, v, s, hi)
+            ids.append(1000 + p)
+    res = _drain(srv, ids)
+    assert max(x["batch_views"] for x in res.values()) > 1, "nothing was batched"
+    for p in range(len(probes)):
+        rid = 1000 + p
+        if rid not in res:
+            continue
+        v = probes[p][0]
+        gv, gs = _read(srv, rid, v)
+        assert np.array_equal(gv, alone[p][0]), f"probe {p}: vis differs in the mix (batch {res[rid]['batch_views']})"
+        assert np.array_equal(gs, alone[p][1]), f"probe {p}: shadow differs in the mix"
+    for i in ids:
+        srv.release(i)
+    st = srv.stats()
+    assert st["completed"] == st["submitted"] and st["used_bytes"] == 0
+    srv.close()
+    r.close()
+
+
+def test_admission_by_hbm_budget(small_world):
+    views = _views(small_world, 12, 61, [(320, 180)])
+    r = Renderer(64, 64)
+    r.sync_world(small_world)
+    # 320x180 needs 576 KB of outputs plus its inputs: the budget admits about two.
+    budget = 2 * (320 * 180 * 10 + 200 * 1024)
+    srv = Server(r, lanes=2, max_views=8, hbm_budget=budget)
+    for i, (v, (s, hi)) in enumerate(views):
+        srv.submit(i, v, s, hi)
+    done = {}
+    polls = 0
+    max_used = 0
+    while len(done) < len(views):
+        for x in srv.poll(wait_ms=50):
+            done[x["id"]] = x
+            assert x["status"] == 0
+            v, (s, hi) = views[x["id"]]
+            ref = H.oracle_frames(small_world, v, s, hi, frames=1)[0]
+            assert np.array_equal(_read(srv, x["id"], v)[0], ref[0])
+            srv.release(x["id"])
+        max_used = max(max_used, srv.stats()["used_bytes"])
+        polls += 1
+        assert polls < 2000
+    assert max_used <= budget
+    assert max(x["batch_views"] for x in done.values()) <= 2
+    srv.close()
+    r.close()
+
+
+def test_barrier_orders_edits_between_requests():
+    wl = W.WORKLOADS[0]
+    world = W.make_world(wl)
+    views = _views(world, 4, 71, [(200, 120)])
+    r = Renderer(64, 64)
+    r.sync_world(world)
+    srv = Server(r, lanes=2, max_views=4, hbm_budget=256 << 20)
+    before = [H.oracle_frames(world, v, s, hi, frames=1)[0] for v, (s, hi) in views]
+    for i, (v, (s, hi)) in enumerate(views):
+        srv.submit(i, v, s, hi)
+    srv.barrier()
+    rng = np.random.default_rng(8)
+    e = np.zeros(2000, A.EDIT_DTYPE)
+    e["x"] = rng.integers(0, 256, 2000)
+    e["y"] = rng.integers(64, 192, 2000)
+    e["z"] = rng.integers(0, 256, 2000)
+    e["id"] = rng.choice([0, 1, 2, 8, 10, 11], 2000)
+    assert world.apply_edits(e) == 2000
+    r.sync_world(world)
+    cams = W.random_cameras(wl, 4, seed=71, resolutions=[(200, 120)])
+    after_views = [(build_view(world, c), W.spec_words(c.flags, world)) for c in cams]
+    srv.resume()
+    for i, (v, (s, hi)) in enumerate(after_views):
+        srv.submit(10 + i, v, s, hi)
+    res = _drain(srv, list(range(4)) + [10 + i for i in range(4)])
+    for i, (v, _) in enumerate(views):
+        assert res[i]["t_done"] <= min(res[10 + k]["t_launch"] for k in range(4))
+        gv, gs = _read(srv, i, v)
+        assert np.array_equal(gv, before[i][0]) and np.array_equal(gs, before[i][1]), f"pre-edit request {i}"
+    changed = 0
+    for i, (v, (s, hi)) in enumerate(after_views):
+        ref = H.oracle_frames(world, v, s, hi, frames=1)[0]
+        gv, gs = _read(srv, 10 + i, v)
+        assert np.array_equal(gv, ref[0]) and np.array_equal(gs, ref[1]), f"post-edit request {i}"
+        changed += int((gv != before[i][0]).sum())
+    assert changed > 0, "the edits changed no pixel"
+    srv.close()
+    r.close()

@@ -174,6 +174,24 @@ class BatchTuning(C.Structure):
                 ("graph", C.c_uint32)]
 
 
+class ServeDesc(C.Structure):
+    _fields_ = [("lanes", C.c_uint32), ("max_views", C.c_uint32), ("hbm_budget", C.c_uint64),
+                ("tuning", BatchTuning)]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("status", C.c_int32), ("lane", C.c_uint32), ("batch_views", C.c_uint32),
+                ("width", C.c_uint32), ("height", C.c_uint32), ("pad", C.c_uint32), ("batch", C.c_uint64),
+                ("t_arrival", C.c_double), ("t_submit", C.c_double), ("t_launch", C.c_double),
+                ("t_done", C.c_double)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("launches", C.c_uint64),
+                ("max_batch_views", C.c_uint64), ("shadow_rays", C.c_uint64), ("pending", C.c_uint64),
+                ("used_bytes", C.c_uint64), ("budget_bytes", C.c_uint64)]
+
+
 class PairDemand(C.Structure):
     _fields_ = [("select", C.c_uint32), ("deferred", C.c_uint32), ("recover", C.c_uint32),
                 ("cap", C.c_uint32), ("truncated", C.c_uint32), ("shadow_rays", C.c_uint32)]
@@ -257,6 +275,17 @@ _sig_march = {
     "vx_batch_timings_reset": (C.c_int, [P]),
     "vx_batch_timings_sum": (C.c_int, [P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_uint32)]),
     "vx_batch_device_outputs": (C.c_int, [P, C.c_uint32, P, P]),
+    "vx_serve_clock": (C.c_double, []),
+    "vx_serve_create": (C.c_int, [P, C.POINTER(ServeDesc), C.POINTER(P)]),
+    "vx_serve_destroy": (C.c_int, [P]),
+    "vx_serve_submit": (C.c_int, [P, C.c_uint64, C.POINTER(ViewDesc), C.c_double]),
+    "vx_serve_poll": (C.c_int, [P, C.POINTER(ServeResult), C.c_uint32, C.POINTER(C.c_uint32), C.c_int]),
+    "vx_serve_read_vis": (C.c_int, [P, C.c_uint64, P, C.c_size_t]),
+    "vx_serve_read_shadow": (C.c_int, [P, C.c_uint64, P, C.c_size_t]),
+    "vx_serve_release": (C.c_int, [P, C.c_uint64]),
+    "vx_serve_barrier": (C.c_int, [P]),
+    "vx_serve_resume": (C.c_int, [P]),
+    "vx_serve_stats_get": (C.c_int, [P, C.POINTER(ServeStats)]),
     "vx_error_string": (C.c_char_p, [C.c_int]),
     "vx_device_name": (C.c_int, [C.c_char_p, C.c_size_t]),
 }

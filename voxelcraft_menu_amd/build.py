@@ -20,7 +20,7 @@ INC = os.path.join(ROOT, "include")
 CSRC = os.path.join(HERE, "csrc")
 
 HOST_SOURCES = ["host/world.cpp", "host/worldgen.cpp", "host/view.cpp", "host/capi.cpp"]
-HIP_SOURCES = ["kernels.hip", "tiles.hip", "vxmarch.hip", "batch.hip"]
+HIP_SOURCES = ["kernels.hip", "tiles.hip", "vxmarch.hip", "batch.hip", "serve.hip"]
 HIP_EXTRA = []  # further .hip translation units are appended by later modules
 
 HIPCC_FLAGS = [
@@ -60,7 +60,7 @@ def build_march(verbose: bool = False) -> str:
     out = os.path.join(HERE, "libvxmarch.so")
     tmp = out + ".tmp"
     srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES + HIP_EXTRA]
-    cmd = [_hipcc()] + HIPCC_FLAGS + [f"-I{INC}", f"-I{CSRC}"] + srcs + ["-o", tmp]
+    cmd = [_hipcc()] + HIPCC_FLAGS + [f"-I{INC}", f"-I{CSRC}"] + srcs + ["-pthread", "-o", tmp]
     _run(cmd, verbose)
     os.replace(tmp, out)
     return out

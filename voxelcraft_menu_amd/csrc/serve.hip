@@ -1,0 +1,571 @@
+// serve.hip -- continuous-batching view server (vx_serve_*, include/vxmarch.h).
+//
+// No counterpart in the reference, whose headless loop renders one view per
+// Renderer::render call and blocks on it (src/headless.rs:419-497, :680-837).
+//
+// Requests (one view each: frame, chunk list, grid, pipeline key) are copied at submit
+// and queued FIFO. A worker thread owns `lanes` launch lanes, each with its own HIP
+// stream and input buffers. Whenever a lane is free it admits the queued requests in
+// order while (a) the lane has view slots left and (b) the request's device footprint
+// fits the HBM budget and the output arena, then launches them as one batch (the tile
+// march of vx_batch, every tile of every admitted view in one launch). Requests that
+// arrive while other lanes are busy are admitted into the next free lane, so their
+// work overlaps on the device with the batches still running. A request's outputs
+// (key and shadow word per pixel) are written straight into its own region of the
+// output arena and stay there until the caller releases the request, which also
+// returns its footprint to the budget.
+//
+// The tree pools are the context's. A barrier (vx_serve_barrier) waits for every
+// request submitted before it and holds later ones back, so the caller can apply an
+// edit batch to the pools (vx_pool_sync of the dirty ranges) between frames; launches
+// resume with vx_serve_resume.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "ctx.hpp"
+
+using namespace vxd;
+using vxh::DevBuf;
+using vxh::ensure;
+
+#define HIPX VX_HIPX
+
+namespace {
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+constexpr uint64_t PX_UNIT = 256;  // output-arena granularity, pixels
+
+// First-fit allocator over arena units with coalescing.
+struct Arena {
+    std::map<uint64_t, uint64_t> free_;  // start -> length
+    void init(uint64_t units) {
+        free_.clear();
+        if (units) free_[0] = units;
+    }
+    bool alloc(uint64_t n, uint64_t &off) {
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < n) continue;
+            off = it->first;
+            const uint64_t rest = it->second - n, start = it->first + n;
+            free_.erase(it);
+            if (rest) free_[start] = rest;
+            return true;
+        }
+        return false;
+    }
+    void release(uint64_t off, uint64_t n) {
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+enum : int { RQ_PENDING = 0, RQ_RUNNING = 1, RQ_DONE = 2 };
+
+struct Req {
+    uint64_t id = 0, seq = 0;
+    ViewK vk{};
+    std::vector<vx_gpu_chunk> chunks;
+    std::vector<uint32_t> grid;
+    uint32_t w = 0, h = 0, tiles = 0;
+    uint64_t units = 0, px_off = 0, bytes = 0;
+    double t_arrival = 0, t_submit = 0, t_launch = 0, t_done = 0;
+    int state = RQ_PENDING, status = VX_OK;
+    uint32_t lane = 0, batch_views = 0;
+    uint64_t batch = 0;
+};
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct Lane {
+    hipStream_t st = nullptr;
+    hipEvent_t done = nullptr;
+    DevBuf in, hiz, cost, order, counters;
+    void *stage = nullptr;  // pinned staging of `in`
+    size_t stage_cap = 0;
+    uint32_t *stat = nullptr;  // host-mapped: shadow rays of the lane's last batch
+    uint32_t *stat_dev = nullptr;
+    std::vector<Req *> reqs;
+    uint64_t batch = 0;
+    bool busy = false;
+};
+
+}  // namespace
+
+namespace vxd {
+// Sums one statistic over the spread slots into a host-mapped word.
+__global__ void k_stat_sum(const uint32_t *__restrict__ counters, uint32_t k, uint32_t *__restrict__ out) {
+    uint32_t v = 0;
+    for (uint32_t s = threadIdx.x; s < CNT_STAT_SLOTS; s += blockDim.x)
+        v += counters[CNT_STAT_BASE + s * CNT_STAT_STRIDE + k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o);
+    if (threadIdx.x == 0) *out = v;
+}
+}  // namespace vxd
+
+struct vx_serve {
+    vx_ctx *ctx = nullptr;
+    vx_serve_desc d{};
+    DevBuf vis, shadow;  // output arena
+    Arena arena;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Req *> pending;
+    std::unordered_map<uint64_t, Req *> live;
+    std::deque<Req *> finished;
+    std::vector<Lane> lanes;
+    uint64_t used = 0, seq = 0, fence = ~0ull, batches = 0;
+    bool stop = false;
+    std::thread worker;
+    hipStream_t io = nullptr;
+    vx_serve_stats stats{};
+};
+
+namespace {
+
+template <int O, int V>
+void launch_march(uint32_t mode, uint32_t chunk, uint32_t wps, int n_cu, uint32_t tiles, const BatchArgs &a,
+                  hipStream_t st) {
+    if (mode == VX_BATCH_FLAT)
+        k_batch_flat<O, V><<<tiles, 64, 0, st>>>(a);
+    else
+        k_batch_persistent<O, V><<<(unsigned)n_cu * 4u * wps, 64, 0, st>>>(a, chunk);
+}
+
+int n_cus(int dev) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    return n;
+}
+
+// Enqueues one batch (the lane's admitted requests) on the lane stream.
+int launch(vx_serve *s, Lane &L) {
+    const uint32_t n = (uint32_t)L.reqs.size();
+    size_t recs = 0;
+    uint32_t tiles = 0;
+    for (Req *r : L.reqs) {
+        recs += r->chunks.size();
+        tiles += r->tiles;
+    }
+    const size_t o_views = 0, o_toff = align256(n * sizeof(ViewK)), o_hdr = align256(o_toff + (n + 1) * 4),
+                 o_chunks = align256(o_hdr + BH_LEN * 4), o_grids = align256(o_chunks + recs * sizeof(vx_gpu_chunk)),
+                 total = o_grids + (size_t)n * VX_GRID_LEN * 4;
+    int rc;
+    if ((rc = ensure(L.in, total)) || (rc = ensure(L.hiz, (size_t)tiles * 4)) ||
+        (rc = ensure(L.cost, (size_t)tiles * 4)) || (rc = ensure(L.order, (size_t)tiles * 4)))
+        return rc;
+    if (L.stage_cap < total) {
+        if (L.stage) (void)hipHostFree(L.stage);
+        L.stage = nullptr;
+        L.stage_cap = 0;
+        HIPX(hipHostMalloc(&L.stage, total + total / 2, hipHostMallocDefault));
+        L.stage_cap = total + total / 2;
+    }
+    char *stg = (char *)L.stage;
+    ViewK *vk = (ViewK *)(stg + o_views);
+    uint32_t *toff = (uint32_t *)(stg + o_toff);
+    uint32_t *hdr = (uint32_t *)(stg + o_hdr);
+    vx_gpu_chunk *ch = (vx_gpu_chunk *)(stg + o_chunks);
+    uint32_t *grids = (uint32_t *)(stg + o_grids);
+    uint32_t t = 0;
+    size_t c = 0;
+    for (uint32_t v = 0; v < n; v++) {
+        const Req *r = L.reqs[v];
+        vk[v] = r->vk;
+        vk[v].chunk_off = (uint32_t)c;
+        vk[v].grid_off = v * VX_GRID_LEN;
+        vk[v].tile_off = t;
+        vk[v].px_off_lo = (uint32_t)r->px_off;
+        vk[v].px_off_hi = (uint32_t)(r->px_off >> 32);
+        toff[v] = t;
+        if (!r->chunks.empty()) std::memcpy(ch + c, r->chunks.data(), r->chunks.size() * sizeof(vx_gpu_chunk));
+        std::memcpy(grids + (size_t)v * VX_GRID_LEN, r->grid.data(), VX_GRID_LEN * 4);
+        c += r->chunks.size();
+        t += r->tiles;
+    }
+    toff[n] = t;
+    hdr[BH_TILES] = tiles;
+    hdr[BH_VIEWS] = n;
+    hdr[2] = hdr[3] = 0;
+    hipStream_t st = L.st;
+    char *din = (char *)L.in.p;
+    HIPX(hipMemcpyAsync(din, stg, total, hipMemcpyHostToDevice, st));
+    if ((rc = vxh::fill_u32(st, L.counters.p, 0, CNT_TOTAL)) || (rc = vxh::fill_u32(st, L.hiz.p, 0, tiles)))
+        return rc;
+    const uint32_t *hdr_d = (const uint32_t *)(din + o_hdr);
+    const uint32_t *toff_d = (const uint32_t *)(din + o_toff);
+    k_iota_views<<<ORDER_BLOCKS, 256, 0, st>>>((uint32_t *)L.order.p, hdr_d + BH_TILES, 0, toff_d, hdr_d + BH_VIEWS);
+    BatchArgs a;
+    a.p = vxh::make_pools(s->ctx);
+    a.p.chunks = (const vx_gpu_chunk *)(din + o_chunks);
+    a.p.grid = (const uint32_t *)(din + o_grids);
+    a.p.n_chunks = 0;
+    a.views = (const ViewK *)(din + o_views);
+    a.hdr = hdr_d;
+    a.hiz = (uint32_t *)L.hiz.p;
+    a.vis = (unsigned long long *)s->vis.p;
+    a.shadow = (uint16_t *)s->shadow.p;
+    a.counters = (uint32_t *)L.counters.p;
+    a.order = (const uint32_t *)L.order.p;
+    a.cost = (uint32_t *)L.cost.p;
+    const vx_batch_tuning &tn = s->d.tuning;
+    const int ncu = n_cus(s->ctx->device);
+    switch (tn.occupancy * 16 + tn.variant) {
+        case 4 * 16 + 5: launch_march<4, 5>(tn.mode, tn.queue_chunk, tn.waves_per_simd, ncu, tiles, a, st); break;
+        case 6 * 16 + 4: launch_march<6, 4>(tn.mode, tn.queue_chunk, tn.waves_per_simd, ncu, tiles, a, st); break;
+        default: launch_march<4, 4>(tn.mode, tn.queue_chunk, tn.waves_per_simd, ncu, tiles, a, st); break;
+    }
+    k_stat_sum<<<1, 64, 0, st>>>((const uint32_t *)L.counters.p, STAT_SHADOW, L.stat_dev);
+    VX_LAUNCHED();
+    HIPX(hipEventRecord(L.done, st));
+    return VX_OK;
+}
+
+bool lanes_idle(const vx_serve *s) {
+    for (const Lane &L : s->lanes)
+        if (L.busy) return false;
+    return true;
+}
+
+void finish(vx_serve *s, Lane &L, int status, double t) {
+    const uint32_t sr = status == VX_OK ? *(volatile uint32_t *)L.stat : 0u;
+    for (Req *r : L.reqs) {
+        r->t_done = t;
+        r->state = RQ_DONE;
+        r->status = status;
+        s->finished.push_back(r);
+    }
+    s->stats.completed += L.reqs.size();
+    s->stats.shadow_rays += sr;
+    L.reqs.clear();
+    L.busy = false;
+}
+
+void worker_main(vx_serve *s) {
+    (void)hipSetDevice(s->ctx->device);
+    std::unique_lock<std::mutex> lk(s->mu);
+    while (!s->stop) {
+        bool progressed = false, busy = false;
+        for (Lane &L : s->lanes) {
+            if (!L.busy) continue;
+            hipError_t q = hipEventQuery(L.done);
+            if (q == hipErrorNotReady) {
+                busy = true;
+                continue;
+            }
+            finish(s, L, q == hipSuccess ? VX_OK : VX_E_HIP, now_s());
+            progressed = true;
+        }
+        for (uint32_t li = 0; li < s->lanes.size(); li++) {
+            Lane &L = s->lanes[li];
+            if (L.busy || s->pending.empty()) continue;
+            while (!s->pending.empty() && L.reqs.size() < s->d.max_views) {
+                Req *r = s->pending.front();
+                if (r->seq >= s->fence) break;              // held back by a barrier
+                if (s->used + r->bytes > s->d.hbm_budget) break;  // FIFO: wait for releases
+                uint64_t off = 0;
+                if (!s->arena.alloc(r->units, off)) break;
+                r->px_off = off * PX_UNIT;
+                s->used += r->bytes;
+                r->state = RQ_RUNNING;
+                r->lane = li;
+                L.reqs.push_back(r);
+                s->pending.pop_front();
+            }
+            if (L.reqs.empty()) break;
+            L.busy = true;
+            L.batch = ++s->batches;
+            const double t = now_s();
+            for (Req *r : L.reqs) {
+                r->t_launch = t;
+                r->batch_views = (uint32_t)L.reqs.size();
+                r->batch = L.batch;
+            }
+            s->stats.launches++;
+            s->stats.max_batch_views = std::max<uint64_t>(s->stats.max_batch_views, L.reqs.size());
+            lk.unlock();
+            const int rc = launch(s, L);
+            lk.lock();
+            if (rc) finish(s, L, rc, now_s());
+            else busy = true;
+            progressed = true;
+        }
+        if (progressed) s->cv_done.notify_all();
+        if (!busy) {
+            if (!progressed) s->cv_work.wait_for(lk, std::chrono::milliseconds(2));
+        } else if (!progressed) {
+            lk.unlock();
+            std::this_thread::yield();
+            lk.lock();
+        }
+    }
+}
+
+void free_req(vx_serve *s, Req *r) {
+    if (r->state == RQ_DONE && r->units) {
+        s->arena.release(r->px_off / PX_UNIT, r->units);
+        s->used -= r->bytes;
+    }
+    s->live.erase(r->id);
+    delete r;
+}
+
+}  // namespace
+
+extern "C" {
+
+double vx_serve_clock(void) { return now_s(); }
+
+int vx_serve_create(vx_ctx *c, const vx_serve_desc *d, vx_serve **out) {
+    if (!c || !d || !out) return VX_E_ARG;
+    *out = nullptr;
+    if (c->path != VX_PATH_TILES) return VX_E_UNSUPPORTED;
+    if (d->lanes < 1 || d->lanes > 8 || d->max_views < 1 || d->max_views > BATCH_MAX_VIEWS) return VX_E_ARG;
+    if (d->hbm_budget < ((uint64_t)1 << 20)) return VX_E_ARG;
+    const vx_batch_tuning &t = d->tuning;
+    if (t.mode == VX_BATCH_PLAIN || t.mode > VX_BATCH_PLAIN) return VX_E_ARG;
+    if (!((t.occupancy == 4 && (t.variant == 4 || t.variant == 5)) || (t.occupancy == 6 && t.variant == 4)))
+        return VX_E_ARG;
+    if (t.mode == VX_BATCH_PERSISTENT &&
+        (t.queue_chunk < 1 || t.queue_chunk > 64 || t.waves_per_simd < 1 || t.waves_per_simd > t.occupancy))
+        return VX_E_ARG;
+    HIPX(hipSetDevice(c->device));
+    vx_serve *s = new (std::nothrow) vx_serve();
+    if (!s) return VX_E_OOM;
+    s->ctx = c;
+    s->d = *d;
+    int rc = VX_OK;
+    // The output arena takes the whole budget at 10 bytes per pixel (8-byte key, 2-byte
+    // shadow word); inputs are charged against the same budget.
+    const uint64_t units = d->hbm_budget / (10 * PX_UNIT);
+    s->lanes.resize(d->lanes);
+    do {
+        if ((rc = ensure(s->vis, units * PX_UNIT * 8)) || (rc = ensure(s->shadow, units * PX_UNIT * 2))) break;
+        s->arena.init(units);
+        if (hipStreamCreateWithFlags(&s->io, hipStreamNonBlocking) != hipSuccess) { rc = VX_E_HIP; break; }
+        for (Lane &L : s->lanes) {
+            if (hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&L.done, hipEventDisableTiming) != hipSuccess) {
+                rc = VX_E_HIP;
+                break;
+            }
+            if ((rc = ensure(L.counters, CNT_TOTAL * 4))) break;
+            if (hipHostMalloc((void **)&L.stat, 64, hipHostMallocMapped) != hipSuccess ||
+                hipHostGetDevicePointer((void **)&L.stat_dev, L.stat, 0) != hipSuccess) {
+                rc = VX_E_HIP;
+                break;
+            }
+            *L.stat = 0;
+        }
+    } while (0);
+    if (rc) {
+        s->stop = true;
+        vx_serve_destroy(s);
+        return rc;
+    }
+    c->n_batches++;
+    s->worker = std::thread(worker_main, s);
+    *out = s;
+    return VX_OK;
+}
+
+int vx_serve_destroy(vx_serve *s) {
+    if (!s) return VX_E_ARG;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    const bool started = s->worker.joinable();
+    if (started) s->worker.join();
+    (void)hipSetDevice(s->ctx->device);
+    for (Lane &L : s->lanes) {
+        if (L.st) (void)hipStreamSynchronize(L.st);
+        DevBuf *bufs[] = {&L.in, &L.hiz, &L.cost, &L.order, &L.counters};
+        for (DevBuf *b : bufs) vxh::release(*b);
+        if (L.stage) (void)hipHostFree(L.stage);
+        if (L.stat) (void)hipHostFree(L.stat);
+        if (L.done) (void)hipEventDestroy(L.done);
+        if (L.st) (void)hipStreamDestroy(L.st);
+        for (Req *r : L.reqs) delete r;
+    }
+    for (Req *r : s->pending) delete r;
+    for (auto &kv : s->live)
+        if (kv.second->state == RQ_DONE) delete kv.second;
+    if (s->io) (void)hipStreamDestroy(s->io);
+    vxh::release(s->vis);
+    vxh::release(s->shadow);
+    if (started) s->ctx->n_batches--;
+    delete s;
+    return VX_OK;
+}
+
+int vx_serve_submit(vx_serve *s, uint64_t id, const vx_view *v, double t_arrival) {
+    if (!s || !v || !v->frame || !v->grid || (!v->chunks && v->n_total) || v->n_total > VX_MAX_CHUNKS)
+        return VX_E_ARG;
+    int rc = vxh::check_frame(v->frame, v->n_total);
+    if (rc) return rc;
+    Req *r = new (std::nothrow) Req();
+    if (!r) return VX_E_OOM;
+    r->id = id;
+    r->vk.f = vxh::make_frame(v->frame);
+    if ((rc = vxh::make_spec(v->spec, v->spec_hi, v->frame->flags, r->vk.s))) {
+        delete r;
+        return rc;
+    }
+    r->vk.n_chunks = v->n_total;
+    r->w = v->frame->res[0];
+    r->h = v->frame->res[1];
+    r->tiles = v->frame->tiles[0] * v->frame->tiles[1];
+    r->chunks.assign(v->chunks, v->chunks + v->n_total);
+    r->grid.assign(v->grid, v->grid + VX_GRID_LEN);
+    const uint64_t px = (uint64_t)r->w * r->h;
+    r->units = (px + PX_UNIT - 1) / PX_UNIT;
+    r->bytes = r->units * PX_UNIT * 10 + (uint64_t)v->n_total * sizeof(vx_gpu_chunk) + VX_GRID_LEN * 4 +
+               (uint64_t)r->tiles * 12 + sizeof(ViewK);
+    r->t_submit = now_s();
+    r->t_arrival = t_arrival > 0 ? t_arrival : r->t_submit;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (r->bytes > s->d.hbm_budget) {
+            delete r;
+            return VX_E_OVERFLOW;
+        }
+        if (s->live.count(id)) {
+            delete r;
+            return VX_E_ARG;
+        }
+        r->seq = s->seq++;
+        s->live[id] = r;
+        s->pending.push_back(r);
+        s->stats.submitted++;
+    }
+    s->cv_work.notify_all();
+    return VX_OK;
+}
+
+int vx_serve_poll(vx_serve *s, vx_serve_result *out, uint32_t cap, uint32_t *n, int wait_ms) {
+    if (!s || !n || (!out && cap)) return VX_E_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (s->finished.empty() && wait_ms > 0)
+        s->cv_done.wait_for(lk, std::chrono::milliseconds(wait_ms), [&] { return !s->finished.empty(); });
+    uint32_t k = 0;
+    while (k < cap && !s->finished.empty()) {
+        Req *r = s->finished.front();
+        s->finished.pop_front();
+        vx_serve_result &o = out[k++];
+        o.id = r->id;
+        o.status = r->status;
+        o.lane = r->lane;
+        o.batch_views = r->batch_views;
+        o.batch = r->batch;
+        o.width = r->w;
+        o.height = r->h;
+        o.t_arrival = r->t_arrival;
+        o.t_submit = r->t_submit;
+        o.t_launch = r->t_launch;
+        o.t_done = r->t_done;
+    }
+    *n = k;
+    return VX_OK;
+}
+
+static int serve_read(vx_serve *s, uint64_t id, const DevBuf &buf, size_t elem, void *out, size_t n) {
+    size_t off = 0;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->live.find(id);
+        if (it == s->live.end() || it->second->state != RQ_DONE) return VX_E_ARG;
+        if (it->second->status != VX_OK) return it->second->status;
+        if (n != (size_t)it->second->w * it->second->h) return VX_E_ARG;
+        off = it->second->px_off;
+    }
+    HIPX(hipSetDevice(s->ctx->device));
+    HIPX(hipMemcpyAsync(out, (const char *)buf.p + off * elem, n * elem, hipMemcpyDeviceToHost, s->io));
+    HIPX(hipStreamSynchronize(s->io));
+    return VX_OK;
+}
+
+int vx_serve_read_vis(vx_serve *s, uint64_t id, uint64_t *out, size_t n) {
+    if (!s || !out) return VX_E_ARG;
+    return serve_read(s, id, s->vis, 8, out, n);
+}
+
+int vx_serve_read_shadow(vx_serve *s, uint64_t id, uint16_t *out, size_t n) {
+    if (!s || !out) return VX_E_ARG;
+    return serve_read(s, id, s->shadow, 2, out, n);
+}
+
+int vx_serve_release(vx_serve *s, uint64_t id) {
+    if (!s) return VX_E_ARG;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->live.find(id);
+        if (it == s->live.end()) return VX_E_ARG;
+        if (it->second->state != RQ_DONE) return VX_E_BUSY;
+        free_req(s, it->second);
+    }
+    s->cv_work.notify_all();
+    return VX_OK;
+}
+
+int vx_serve_barrier(vx_serve *s) {
+    if (!s) return VX_E_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->fence = s->seq;
+    s->cv_work.notify_all();
+    s->cv_done.wait(lk, [&] {
+        if (!lanes_idle(s)) return false;
+        for (const Req *r : s->pending)
+            if (r->seq < s->fence) return false;
+        return true;
+    });
+    return VX_OK;
+}
+
+int vx_serve_resume(vx_serve *s) {
+    if (!s) return VX_E_ARG;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->fence = ~0ull;
+    }
+    s->cv_work.notify_all();
+    return VX_OK;
+}
+
+int vx_serve_stats_get(vx_serve *s, vx_serve_stats *out) {
+    if (!s || !out) return VX_E_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    *out = s->stats;
+    out->pending = s->pending.size();
+    out->used_bytes = s->used;
+    out->budget_bytes = s->d.hbm_budget;
+    return VX_OK;
+}
+
+}  // extern "C"

@@ -256,6 +256,83 @@ class Batch:
         return float(ms.value), float(mm.value), int(n.value)
 
 
+class Server:
+    """Continuous-batching view server over the renderer's resident world (vx_serve_*).
+
+    No counterpart in the reference (its request loop renders and waits one view at a
+    time, src/headless.rs:419-497). ``submit`` queues a view; a native worker thread
+    admits queued views onto free launch lanes by view slots and HBM budget and launches
+    each lane's views as one batch. ``poll`` returns completions with their timestamps;
+    ``release`` frees a completed request's outputs. Pool updates (``Renderer.sync_world``
+    after edits) go between ``barrier`` and ``resume``.
+    """
+
+    def __init__(self, renderer: Renderer, lanes: int = 2, max_views: int = 64, hbm_budget: int = 4 << 30,
+                 mode: int = A.BATCH_FLAT, occupancy: int = 4, variant: int = 4, queue_chunk: int = 4,
+                 waves_per_simd: int | None = None):
+        self._L = renderer._L
+        self._r = renderer
+        self._s = C.c_void_p()
+        wps = occupancy if waves_per_simd is None else waves_per_simd
+        t = A.BatchTuning(mode, occupancy, variant, 0, queue_chunk, wps, 0)
+        d = A.ServeDesc(lanes, max_views, hbm_budget, t)
+        A.check(self._L.vx_serve_create(renderer.handle, C.byref(d), C.byref(self._s)), "vx_serve_create")
+        renderer._batches.append(self)
+        self._res = (A.ServeResult * 256)()
+
+    def close(self):
+        if self._s:
+            self._L.vx_serve_destroy(self._s)
+            self._s = C.c_void_p()
+            if self in self._r._batches:
+                self._r._batches.remove(self)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def clock(self) -> float:
+        return float(self._L.vx_serve_clock())
+
+    def submit(self, rid: int, view: ViewData, spec: int, spec_hi: int, t_arrival: float = 0.0):
+        ch = np.ascontiguousarray(view.chunks[: view.n_total])
+        fr = A.Frame.from_buffer_copy(bytes(view.frame))
+        d = A.ViewDesc(C.addressof(fr), ch.ctypes.data if view.n_total else None, view.n_total,
+                       view.grid.ctypes.data, spec, spec_hi)
+        A.check(self._L.vx_serve_submit(self._s, rid, C.byref(d), t_arrival), "vx_serve_submit")
+
+    def poll(self, wait_ms: int = 0) -> list[dict]:
+        n = C.c_uint32()
+        A.check(self._L.vx_serve_poll(self._s, self._res, len(self._res), C.byref(n), wait_ms), "vx_serve_poll")
+        return [{k: getattr(self._res[i], k) for k, _ in A.ServeResult._fields_} for i in range(n.value)]
+
+    def read_vis(self, rid: int, width: int, height: int) -> np.ndarray:
+        out = np.zeros(width * height, dtype=np.uint64)
+        A.check(self._L.vx_serve_read_vis(self._s, rid, A.ptr(out), out.size), "vx_serve_read_vis")
+        return out
+
+    def read_shadow(self, rid: int, width: int, height: int) -> np.ndarray:
+        out = np.zeros(width * height, dtype=np.uint16)
+        A.check(self._L.vx_serve_read_shadow(self._s, rid, A.ptr(out), out.size), "vx_serve_read_shadow")
+        return out
+
+    def release(self, rid: int):
+        A.check(self._L.vx_serve_release(self._s, rid), "vx_serve_release")
+
+    def barrier(self):
+        A.check(self._L.vx_serve_barrier(self._s), "vx_serve_barrier")
+
+    def resume(self):
+        A.check(self._L.vx_serve_resume(self._s), "vx_serve_resume")
+
+    def stats(self) -> dict:
+        st = A.ServeStats()
+        A.check(self._L.vx_serve_stats_get(self._s, C.byref(st)), "vx_serve_stats_get")
+        return {k: int(getattr(st, k)) for k, _ in A.ServeStats._fields_}
+
+
 def device_name() -> str:
     buf = C.create_string_buffer(256)
     rc = A.march().vx_device_name(buf, 256)
