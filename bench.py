@@ -10,6 +10,11 @@ batched form (the launch form picked by the autotuner, replayed from a hipGraph)
 are reported beside it as `plain`, with the ratio. Every view keeps its Hi-Z words
 between steps, so steps after the first run in the Hi-Z steady state.
 
+The same line carries `serving`: p50/p99 request latency and Grays/s of an open-loop
+Poisson stream of view requests (1000/s by default, the four resolutions) mixed with
+edit batches of 1000 voxel ops (2/s) on the device-resident 2048x2048x512 world,
+through the continuous-batching server.
+
 --workload 1 measures one 1920x1080 camera per step instead; --workload 0 the small
 640x360 case. Inputs (pools, chunk lists, grids, frames) are in HBM before the timed
 region starts.
@@ -207,6 +212,9 @@ def run_batch(args, out):
     r1.close()
     r.close()
 
+    if not args.no_serving:
+        out["serving"] = run_serving(args)
+
     if args.no_cpu_baseline:
         out["cpu_baseline"] = None
         out["roofline"] = None
@@ -224,6 +232,28 @@ def run_batch(args, out):
         "kernel": kname, "kernel_ms": round(march_ms, 4), "alg_bytes_per_launch": int(alg),
         "alg_bytes_per_pixel": round(alg / px, 2),
     }
+
+
+def run_serving(args):
+    """The serving workload: the 2048x2048x512 world resident on the device, an open-loop
+    Poisson stream of view requests (resolutions uniform over the four sizes, seeded
+    poses) mixed with Poisson edit batches of 1000 random voxel set/clear ops, through
+    the continuous-batching server. Latency = completion - scheduled arrival."""
+    from voxelcraft_menu_amd import serving as S
+    from voxelcraft_menu_amd import workloads as W
+    from voxelcraft_menu_amd.renderer import Renderer
+
+    wl = W.WORKLOADS[4]
+    world = W.make_world(wl)
+    r = Renderer(64, 64)
+    r.sync_world(world)
+    S.run(world, r, wl, qps=200, duration_s=1.0, edit_rate=0, seed=99)  # warm-up stream
+    res = S.run(world, r, wl, qps=args.serve_qps, duration_s=args.serve_seconds, edit_rate=args.serve_edit_rate,
+                seed=4)
+    r.close()
+    res["workload"] = wl.name
+    _log(f"serving: p50 {res['p50_ms']} ms p99 {res['p99_ms']} ms at {args.serve_qps} qps")
+    return res
 
 
 def run_single(args, out):
@@ -311,6 +341,10 @@ def main():
     ap.add_argument("--path", choices=["tiles", "pairs"], default="tiles", help="single-view workloads only")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle work for cpu_baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving stream (workload 2 only)")
+    ap.add_argument("--serve-qps", type=float, default=1000.0, help="offered view requests per second")
+    ap.add_argument("--serve-seconds", type=float, default=4.0, help="seconds of request arrivals")
+    ap.add_argument("--serve-edit-rate", type=float, default=2.0, help="edit batches (1000 ops) per second")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one 256 MiB device-to-device copy (scale reference for PMC byte counters)")
     args = ap.parse_args()

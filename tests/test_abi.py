@@ -59,6 +59,11 @@ def test_error_strings_and_no_device_path():
     assert L.vx_batch_set_tuning(None, None) == -3
     assert L.vx_batch_autotune(None, 3, None, None) == -3
     assert L.vx_batch_read_vis(None, 0, None, 0) == -3
+    assert L.vx_serve_create(None, None, C.byref(out)) == -3
+    assert L.vx_serve_submit(None, 0, None, 0.0) == -3
+    assert L.vx_serve_release(None, 0) == -3
+    assert L.vx_serve_barrier(None) == -3
+    assert L.vx_serve_clock() > 0.0
     if os.environ.get("HIP_VISIBLE_DEVICES") == "" or not os.path.exists("/dev/kfd"):
         ctx = C.c_void_p()
         desc = A.CtxDesc(64, 64, 0, 0, -1)

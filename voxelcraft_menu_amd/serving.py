@@ -1,0 +1,182 @@
+"""Open-loop request stream for the view server: Poisson view requests and edit batches.
+
+The reference's request loop (src/headless.rs:419-497) takes one view at a time: build
+the chunk list, render, wait. Here requests arrive on a seeded Poisson clock whether or
+not earlier ones have finished; each is stamped with its scheduled arrival time, so
+latency = completion - arrival includes every host and queueing delay.
+
+  view request  host chunk list (build_view, under a shared lock on the host world),
+                then Server.submit; the native server batches it with whatever else is
+                queued and launches it on a free lane.
+  edit batch    1000 random voxel set/clear ops: applied to the host world under the
+                exclusive lock (the GPU keeps rendering submitted views meanwhile), then
+                barrier -> dirty-range pool upload -> resume, so views submitted before
+                the edit see the old world and views after it the new one.
+
+A poller thread collects completions and releases them (returning their HBM budget).
+"""
+from __future__ import annotations
+
+import threading
+import time
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+from . import _abi as A
+from . import workloads as W
+from .renderer import Renderer, Server
+from .world import World, build_view
+
+EDIT_IDS = [0, 1, 2, 5, 8, 11]  # air, stone, dirt, planks, glass, glowstone
+
+
+class _RWLock:
+    def __init__(self):
+        self._c = threading.Condition()
+        self._readers = 0
+        self._writer = False
+
+    def acquire_read(self):
+        with self._c:
+            while self._writer:
+                self._c.wait()
+            self._readers += 1
+
+    def release_read(self):
+        with self._c:
+            self._readers -= 1
+            if self._readers == 0:
+                self._c.notify_all()
+
+    def acquire_write(self):
+        with self._c:
+            while self._writer:
+                self._c.wait()
+            self._writer = True
+            while self._readers:
+                self._c.wait()
+
+    def release_write(self):
+        with self._c:
+            self._writer = False
+            self._c.notify_all()
+
+
+def random_edits(rng, wl: W.Workload, n: int = 1000) -> np.ndarray:
+    e = np.zeros(n, A.EDIT_DTYPE)
+    g = wl.gen
+    e["x"] = rng.integers(g.chunk_min[0] * 64, g.chunk_max[0] * 64, n)
+    e["y"] = rng.integers(g.chunk_min[1] * 64, g.chunk_max[1] * 64, n)
+    e["z"] = rng.integers(g.chunk_min[2] * 64, g.chunk_max[2] * 64, n)
+    e["id"] = rng.choice(EDIT_IDS, n)
+    return e
+
+
+def run(world: World, renderer: Renderer, wl: W.Workload, qps: float, duration_s: float, edit_rate: float,
+        seed: int = 4, workers: int = 6, lanes: int = 3, max_views: int = 32, hbm_budget: int = 8 << 30,
+        mode: int = A.BATCH_FLAT, log=None) -> dict:
+    """Runs the stream for duration_s seconds of arrivals and returns latency and
+    throughput figures of the completed view requests."""
+    rng = np.random.default_rng(seed)
+    n_views = max(1, int(qps * duration_s * 1.5) + 16)
+    t_views = np.cumsum(rng.exponential(1.0 / qps, n_views))
+    t_views = t_views[t_views < duration_s]
+    t_edits = np.cumsum(rng.exponential(1.0 / edit_rate, max(2, int(edit_rate * duration_s * 2) + 4))) \
+        if edit_rate > 0 else np.zeros(0)
+    t_edits = t_edits[t_edits < duration_s]
+    cams = W.random_cameras(wl, len(t_views), seed=seed)
+    edits = [random_edits(rng, wl) for _ in range(len(t_edits))]
+    events = sorted([(float(t), 0, i) for i, t in enumerate(t_views)] + [(float(t), 1, i) for i, t in enumerate(t_edits)])
+
+    srv = Server(renderer, lanes=lanes, max_views=max_views, hbm_budget=hbm_budget, mode=mode)
+    lock = _RWLock()
+    lat = {}
+    px = {}
+    batches = {}
+    errors = []
+    done_evt = threading.Event()
+    submitted = [0]
+    sub_lock = threading.Lock()
+
+    def poller():
+        while True:
+            for r in srv.poll(wait_ms=5):
+                if r["status"] != 0:
+                    errors.append(r)
+                lat[r["id"]] = (r["t_done"] - r["t_arrival"], r["t_launch"] - r["t_arrival"], r["t_done"])
+                batches[r["batch"]] = r["batch_views"]
+                srv.release(r["id"])
+            if done_evt.is_set() and len(lat) >= submitted[0]:
+                return
+
+    def prep_submit(i: int, t_arr: float):
+        lock.acquire_read()
+        try:
+            cam = cams[i]
+            v = build_view(world, cam)
+            spec, hi = W.spec_words(cam.flags, world)
+            srv.submit(i, v, spec, hi, t_arr)
+            px[i] = cam.width * cam.height
+            with sub_lock:
+                submitted[0] += 1
+        finally:
+            lock.release_read()
+
+    def apply_edit(e: np.ndarray):
+        lock.acquire_write()
+        try:
+            world.apply_edits(e)
+            srv.barrier()
+            renderer.sync_world(world)
+            srv.resume()
+        finally:
+            lock.release_write()
+
+    pt = threading.Thread(target=poller, daemon=True)
+    pt.start()
+    futs = []
+    edit_ms = []
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        t0 = srv.clock() + 0.05
+        for t, kind, i in events:
+            target = t0 + t
+            while True:
+                now = srv.clock()
+                if now >= target:
+                    break
+                time.sleep(min(target - now, 0.0005))
+            if kind == 0:
+                futs.append(pool.submit(prep_submit, i, target))
+            else:
+                te = srv.clock()
+                apply_edit(edits[i])
+                edit_ms.append((srv.clock() - te) * 1e3)
+        for f in futs:
+            f.result()
+    done_evt.set()
+    pt.join()
+    t_end = max(v[2] for v in lat.values())
+    st = srv.stats()
+    srv.close()
+    if errors:
+        raise RuntimeError(f"{len(errors)} requests failed, first: {errors[0]}")
+    l = np.array([v[0] for v in lat.values()]) * 1e3
+    q = np.array([v[1] for v in lat.values()]) * 1e3
+    span = t_end - t0
+    total_px = sum(px.values())
+    rays = total_px + st["shadow_rays"]
+    out = {
+        "qps_offered": qps, "duration_s": duration_s, "requests": len(lat),
+        "p50_ms": round(float(np.percentile(l, 50)), 3), "p99_ms": round(float(np.percentile(l, 99)), 3),
+        "mean_ms": round(float(l.mean()), 3), "max_ms": round(float(l.max()), 3),
+        "queue_p50_ms": round(float(np.percentile(q, 50)), 3), "queue_p99_ms": round(float(np.percentile(q, 99)), 3),
+        "edit_batches": len(edit_ms), "edit_ops_per_batch": 1000,
+        "edit_apply_ms_mean": round(float(np.mean(edit_ms)), 3) if edit_ms else None,
+        "batches": len(batches), "mean_batch_views": round(float(np.mean(list(batches.values()))), 2),
+        "max_batch_views": int(st["max_batch_views"]), "lanes": lanes, "hbm_budget_bytes": hbm_budget,
+        "Grays_per_s": round(rays / span / 1e9, 4), "primary_rays": int(total_px), "shadow_rays": int(st["shadow_rays"]),
+    }
+    if log:
+        log(f"serving @ {qps} qps: {out}")
+    return out
