@@ -8,6 +8,9 @@
 //   grids   [n_views * 8192]   u32 chunk grids
 //   vis     [sum W*H]          u64 keys        shadow [sum W*H] u16 (R16F bits)
 //   hiz/cost/order [sum tiles] u32
+// Views are laid out class by class (Hi-Z on/off x camera under water or not; the slot
+// of a view is its place in that layout), so each class is one contiguous tile range
+// marched by a kernel specialised for it.
 // Three launch forms give the same bits (each tile runs the same body):
 //   PLAIN       per view: order kernels + the single-view tile march (one dispatch per
 //               view, the pattern of the reference's per-view Renderer::render);
@@ -38,15 +41,18 @@ struct vx_batch {
     uint32_t n_views = 0, n_tiles = 0;
     uint64_t n_px = 0;
     int n_cu = 0;
-    std::vector<ViewK> vk;         // host copies of the view descriptors
-    std::vector<uint32_t> toff;    // n_views + 1 tile offsets
-    std::vector<uint64_t> poff;    // n_views + 1 pixel offsets
+    std::vector<ViewK> vk;         // host copies of the view descriptors, by slot
+    std::vector<uint32_t> toff;    // n_views + 1 tile offsets, by slot
+    std::vector<uint64_t> poff;    // n_views + 1 pixel offsets, by slot
+    std::vector<uint32_t> slot_of; // caller's view index -> slot
+    uint32_t cb[N_CLASSES + 1] = {0, 0, 0, 0, 0};  // class boundaries of the tile range
     DevBuf views, tile_off, hdr, chunks, grids, vis, shadow, hiz, cost, order, counters;
     vx_batch_tuning tn{};
     // Graph of the current submission (captured lazily, dropped when it would go stale).
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     uint32_t graph_tiles = 0;
+    uint32_t graph_cb[N_CLASSES + 1] = {0, 0, 0, 0, 0};
     uint64_t graph_pool_gen = 0;
     bool graph_valid = false;
     static constexpr int RING = 256;
@@ -63,8 +69,7 @@ size_t counter_words(uint32_t n_views) { return (size_t)CNT_TOTAL + (size_t)n_vi
 bool batch_tuning_ok(const vx_batch_tuning &t) {
     if (t.mode > VX_BATCH_PLAIN || t.ordered > 1 || t.graph > 1) return false;
     if (t.mode == VX_BATCH_PLAIN) return vxh::tuning_ok(t.occupancy, t.variant);
-    if (!((t.occupancy == 4 && (t.variant == 4 || t.variant == 5)) || (t.occupancy == 6 && t.variant == 4)))
-        return false;
+    if (!batch_form_ok(t.mode, t.occupancy, t.variant)) return false;
     if (t.mode == VX_BATCH_PERSISTENT) {
         if (t.queue_chunk < 1 || t.queue_chunk > 64) return false;
         if (t.waves_per_simd < 1 || t.waves_per_simd > t.occupancy) return false;
@@ -95,16 +100,6 @@ BatchArgs batch_args(const vx_batch *b) {
     a.order = (const uint32_t *)b->order.p;
     a.cost = (uint32_t *)b->cost.p;
     return a;
-}
-
-template <int O, int V>
-void launch_batch_ov(const vx_batch *b, const BatchArgs &a, hipStream_t st) {
-    if (b->tn.mode == VX_BATCH_FLAT) {
-        k_batch_flat<O, V><<<b->n_tiles, 64, 0, st>>>(a);
-    } else {
-        unsigned nb = (unsigned)b->n_cu * 4u * b->tn.waves_per_simd;
-        k_batch_persistent<O, V><<<nb, 64, 0, st>>>(a, b->tn.queue_chunk);
-    }
 }
 
 // The whole submission of one batch frame, on the context stream (also what a graph
@@ -148,9 +143,10 @@ int enqueue(vx_batch *b, hipStream_t st, hipEvent_t ev_march) {
     }
     const uint32_t *hdr = (const uint32_t *)b->hdr.p;
     if (b->tn.ordered) {
-        k_cost_hist<<<ORDER_BLOCKS, 256, 0, st>>>((const uint32_t *)b->cost.p, hdr + BH_TILES, 0, cnt + CNT_CTL);
-        k_cost_scatter<<<ORDER_BLOCKS, 256, 0, st>>>((const uint32_t *)b->cost.p, hdr + BH_TILES, 0, cnt + CNT_CTL,
-                                                     (const uint32_t *)b->tile_off.p, hdr + BH_VIEWS,
+        k_cost_hist<<<ORDER_BLOCKS, 256, 0, st>>>((const uint32_t *)b->cost.p, hdr + BH_TILES, 0, hdr + BH_CB,
+                                                  cnt + CNT_CTL);
+        k_cost_scatter<<<ORDER_BLOCKS, 256, 0, st>>>((const uint32_t *)b->cost.p, hdr + BH_TILES, 0, hdr + BH_CB,
+                                                     cnt + CNT_CTL, (const uint32_t *)b->tile_off.p, hdr + BH_VIEWS,
                                                      (uint32_t *)b->order.p);
     } else {
         k_iota_views<<<ORDER_BLOCKS, 256, 0, st>>>((uint32_t *)b->order.p, hdr + BH_TILES, 0,
@@ -158,12 +154,8 @@ int enqueue(vx_batch *b, hipStream_t st, hipEvent_t ev_march) {
     }
     LAUNCHED();
     if (ev_march) HIPX(hipEventRecord(ev_march, st));
-    BatchArgs a = batch_args(b);
-    switch (b->tn.occupancy * 16 + b->tn.variant) {
-        case 4 * 16 + 5: launch_batch_ov<4, 5>(b, a, st); break;
-        case 6 * 16 + 4: launch_batch_ov<6, 4>(b, a, st); break;
-        default: launch_batch_ov<4, 4>(b, a, st); break;
-    }
+    launch_batch_classes(b->tn.mode, b->tn.occupancy, b->tn.variant, b->tn.queue_chunk, b->tn.waves_per_simd, b->n_cu,
+                         b->cb, batch_args(b), st);
     LAUNCHED();
     return VX_OK;
 }
@@ -187,6 +179,7 @@ int capture(vx_batch *b) {
         return vxh_hip_fail(ei);
     }
     b->graph_tiles = b->n_tiles;
+    std::memcpy(b->graph_cb, b->cb, sizeof b->cb);
     b->graph_pool_gen = b->ctx->pool_gen;
     b->graph_valid = true;
     return VX_OK;
@@ -273,12 +266,27 @@ int vx_batch_set_views(vx_batch *b, const vx_view *views, uint32_t n) {
     std::vector<ViewK> vk(n);
     std::vector<uint32_t> toff(n + 1, 0);
     std::vector<uint64_t> poff(n + 1, 0);
+    std::vector<uint32_t> slot_of(n), view_at(n);
+    uint32_t cb[N_CLASSES + 1] = {0, 0, 0, 0, 0};
+    {
+        // Slots: the views of class 0, then class 1, ... (caller order inside a class).
+        uint32_t next = 0;
+        for (uint32_t c = 0; c < N_CLASSES; c++)
+            for (uint32_t v = 0; v < n; v++) {
+                if (!views[v].frame) return VX_E_ARG;
+                const uint32_t fl = views[v].frame->flags;
+                if (view_class((fl & VX_FLAG_HIZ) != 0u, (fl & VX_FLAG_UNDERWATER) != 0u) != c) continue;
+                slot_of[v] = next;
+                view_at[next++] = v;
+            }
+    }
     size_t recs = 0;
     for (uint32_t v = 0; v < n; v++) {
-        const vx_view &w = views[v];
+        const vx_view &w = views[view_at[v]];
         if (!w.frame || !w.grid || (!w.chunks && w.n_total) || w.n_total > VX_MAX_CHUNKS) return VX_E_ARG;
         int rc = vxh::check_frame(w.frame, w.n_total);
         if (rc) return rc;
+        const uint32_t cls = view_class((w.frame->flags & VX_FLAG_HIZ) != 0u, (w.frame->flags & VX_FLAG_UNDERWATER) != 0u);
         ViewK &k = vk[v];
         std::memset(&k, 0, sizeof k);
         k.f = vxh::make_frame(w.frame);
@@ -294,6 +302,7 @@ int vx_batch_set_views(vx_batch *b, const vx_view *views, uint32_t n) {
         toff[v + 1] = toff[v] + (uint32_t)nt;
         poff[v + 1] = poff[v] + (uint64_t)k.f.res[0] * k.f.res[1];
         recs += w.n_total;
+        for (uint32_t c = cls + 1; c <= N_CLASSES; c++) cb[c] = toff[v + 1];
     }
     vx_ctx *c = b->ctx;
     HIPX(hipSetDevice(c->device));
@@ -309,11 +318,14 @@ int vx_batch_set_views(vx_batch *b, const vx_view *views, uint32_t n) {
     std::vector<vx_gpu_chunk> cat(std::max<size_t>(recs, 1));
     std::vector<uint32_t> grids((size_t)n * VX_GRID_LEN);
     for (uint32_t v = 0; v < n; v++) {
-        if (views[v].n_total)
-            std::memcpy(&cat[vk[v].chunk_off], views[v].chunks, (size_t)views[v].n_total * sizeof(vx_gpu_chunk));
-        std::memcpy(&grids[(size_t)v * VX_GRID_LEN], views[v].grid, VX_GRID_LEN * 4);
+        const vx_view &w = views[view_at[v]];
+        if (w.n_total) std::memcpy(&cat[vk[v].chunk_off], w.chunks, (size_t)w.n_total * sizeof(vx_gpu_chunk));
+        std::memcpy(&grids[(size_t)v * VX_GRID_LEN], w.grid, VX_GRID_LEN * 4);
     }
-    const uint32_t hdr[BH_LEN] = {toff[n], n, 0, 0};
+    uint32_t hdr[BH_LEN] = {};
+    hdr[BH_TILES] = toff[n];
+    hdr[BH_VIEWS] = n;
+    for (uint32_t c = 0; c <= N_CLASSES; c++) hdr[BH_CB + c] = cb[c];
     HIPX(hipMemcpyAsync(b->views.p, vk.data(), (size_t)n * sizeof(ViewK), hipMemcpyHostToDevice, st));
     HIPX(hipMemcpyAsync(b->tile_off.p, toff.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice, st));
     HIPX(hipMemcpyAsync(b->hdr.p, hdr, sizeof hdr, hipMemcpyHostToDevice, st));
@@ -325,12 +337,21 @@ int vx_batch_set_views(vx_batch *b, const vx_view *views, uint32_t n) {
         return rc;
     HIPX(hipStreamSynchronize(st));
     // A captured graph stays valid for another view set only if it reads every count from
-    // the header: not PLAIN (per-view launches) and not FLAT with a different grid size.
-    if (moved || b->tn.mode == VX_BATCH_PLAIN || (b->tn.mode == VX_BATCH_FLAT && b->graph_tiles != toff[n]))
+    // the header and launches the same class kernels: not PLAIN (per-view launches), not
+    // FLAT with different class ranges (grid sizes), not PERSISTENT with a different set
+    // of non-empty classes.
+    bool same_classes = true, same_ranges = true;
+    for (uint32_t c = 0; c < N_CLASSES; c++) {
+        same_classes &= ((cb[c + 1] - cb[c]) != 0u) == ((b->graph_cb[c + 1] - b->graph_cb[c]) != 0u);
+        same_ranges &= (cb[c + 1] - cb[c]) == (b->graph_cb[c + 1] - b->graph_cb[c]);
+    }
+    if (moved || b->tn.mode == VX_BATCH_PLAIN || (b->tn.mode == VX_BATCH_FLAT && !same_ranges) || !same_classes)
         drop_graph(b);
     b->vk.swap(vk);
     b->toff.swap(toff);
     b->poff.swap(poff);
+    b->slot_of.swap(slot_of);
+    std::memcpy(b->cb, cb, sizeof cb);
     b->n_views = n;
     b->n_tiles = b->toff[n];
     b->n_px = b->poff[n];
@@ -400,18 +421,24 @@ static int batch_read(vx_batch *b, const DevBuf &buf, size_t off_bytes, void *ou
 }
 
 int vx_batch_read_vis(vx_batch *b, uint32_t view, uint64_t *out, size_t n) {
-    if (!b || !out || view >= b->n_views || n != b->poff[view + 1] - b->poff[view]) return VX_E_ARG;
-    return batch_read(b, b->vis, b->poff[view] * 8, out, n * 8);
+    if (!b || !out || view >= b->n_views) return VX_E_ARG;
+    const uint32_t v = b->slot_of[view];
+    if (n != b->poff[v + 1] - b->poff[v]) return VX_E_ARG;
+    return batch_read(b, b->vis, b->poff[v] * 8, out, n * 8);
 }
 
 int vx_batch_read_shadow(vx_batch *b, uint32_t view, uint16_t *out, size_t n) {
-    if (!b || !out || view >= b->n_views || n != b->poff[view + 1] - b->poff[view]) return VX_E_ARG;
-    return batch_read(b, b->shadow, b->poff[view] * 2, out, n * 2);
+    if (!b || !out || view >= b->n_views) return VX_E_ARG;
+    const uint32_t v = b->slot_of[view];
+    if (n != b->poff[v + 1] - b->poff[v]) return VX_E_ARG;
+    return batch_read(b, b->shadow, b->poff[v] * 2, out, n * 2);
 }
 
 int vx_batch_read_hiz(vx_batch *b, uint32_t view, uint32_t *out, size_t n) {
-    if (!b || !out || view >= b->n_views || n != (size_t)(b->toff[view + 1] - b->toff[view])) return VX_E_ARG;
-    return batch_read(b, b->hiz, (size_t)b->toff[view] * 4, out, n * 4);
+    if (!b || !out || view >= b->n_views) return VX_E_ARG;
+    const uint32_t v = b->slot_of[view];
+    if (n != (size_t)(b->toff[v + 1] - b->toff[v])) return VX_E_ARG;
+    return batch_read(b, b->hiz, (size_t)b->toff[v] * 4, out, n * 4);
 }
 
 int vx_batch_demand_get(vx_batch *b, vx_pair_demand *out) {
@@ -474,12 +501,13 @@ int vx_batch_autotune(vx_batch *b, uint32_t reps, vx_batch_tuning *best, float *
         t.graph = 1;
         cand.push_back(t);
     };
-    for (uint32_t chunk : {1u, 2u, 4u, 8u}) add(VX_BATCH_PERSISTENT, 4, 4, chunk, 4);
-    add(VX_BATCH_PERSISTENT, 4, 5, 4u, 4);
-    add(VX_BATCH_PERSISTENT, 6, 4, 4u, 6);
-    add(VX_BATCH_FLAT, 4, 4, 1, 4);
-    add(VX_BATCH_FLAT, 6, 4, 1, 6);
-    add(VX_BATCH_FLAT, 4, 5, 1, 4);
+    for (uint32_t occ : {4u, 6u})
+        for (uint32_t var : {4u, 5u}) {
+            add(VX_BATCH_FLAT, occ, var, 1, occ);
+            add(VX_BATCH_PERSISTENT, occ, var, 4, occ);
+        }
+    add(VX_BATCH_PERSISTENT, 6, 4, 2, 6);
+    add(VX_BATCH_PERSISTENT, 6, 4, 8, 6);
     const vx_batch_tuning saved = b->tn;
     float best_t = 1e30f;
     vx_batch_tuning best_c = saved;
@@ -511,8 +539,9 @@ int vx_batch_autotune(vx_batch *b, uint32_t reps, vx_batch_tuning *best, float *
 
 int vx_batch_device_outputs(vx_batch *b, uint32_t view, uint64_t **vis, uint16_t **shadow) {
     if (!b || view >= b->n_views) return VX_E_ARG;
-    if (vis) *vis = (uint64_t *)b->vis.p + b->poff[view];
-    if (shadow) *shadow = (uint16_t *)b->shadow.p + b->poff[view];
+    const uint32_t v = b->slot_of[view];
+    if (vis) *vis = (uint64_t *)b->vis.p + b->poff[v];
+    if (shadow) *shadow = (uint16_t *)b->shadow.p + b->poff[v];
     return VX_OK;
 }
 

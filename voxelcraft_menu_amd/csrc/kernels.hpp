@@ -30,13 +30,21 @@ constexpr uint32_t CNT_PAIRS = 0, CNT_DEFERRED = 1, CNT_RECOVER = 2, CNT_SELECT 
 constexpr uint32_t STAT_SELECT = 0, STAT_DEFERRED = 1, STAT_RECOVER = 2, STAT_SHADOW = 3;
 constexpr uint32_t CNT_STAT_BASE = 32, CNT_STAT_SLOTS = 64, CNT_STAT_STRIDE = 32;
 constexpr uint32_t CNT_LEN = CNT_STAT_BASE + CNT_STAT_SLOTS * CNT_STAT_STRIDE;
-// Control block after the statistics: the cost-bucket histogram and per-bucket cursors of
-// the launch ordering, and the work-queue head of the persistent batch march, each on its
-// own 128-byte line. Cleared with the counters at the start of every frame.
+// Tiles are ordered by (class, cost bucket): a batch holds up to four view classes
+// (Hi-Z on/off x camera under water or not), each marched by its own specialised kernel
+// over a contiguous range of the queue; inside a class, 32 buckets of log2 cycles, most
+// expensive first.
+constexpr uint32_t COST_BUCKETS = 32, N_CLASSES = 4, ORDER_BINS = COST_BUCKETS * N_CLASSES;
+__host__ __device__ constexpr uint32_t view_class(bool hiz, bool underwater) {
+    return (underwater ? 2u : 0u) + (hiz ? 0u : 1u);
+}
+// Control block after the statistics: the bin histogram and per-bin cursors of the
+// launch ordering, and one work-queue head per class for the persistent batch march
+// (each head on its own 128-byte line). Cleared with the counters at every frame.
 constexpr uint32_t CNT_CTL = CNT_LEN;
-constexpr uint32_t CTL_HIST = 0, CTL_CURSOR = 32, CTL_HEAD = 64, CTL_LEN = 96;
+constexpr uint32_t CTL_HIST = 0, CTL_CURSOR = ORDER_BINS, CTL_HEAD = 2 * ORDER_BINS, CTL_HEAD_STRIDE = 32,
+                   CTL_LEN = 2 * ORDER_BINS + N_CLASSES * CTL_HEAD_STRIDE;
 constexpr uint32_t CNT_TOTAL = CNT_LEN + CTL_LEN;
-constexpr uint32_t COST_BUCKETS = 32;
 
 __device__ __forceinline__ void stat_add(uint32_t *counters, uint32_t slot, uint32_t k, uint32_t v) {
     if (v) atomicAdd(&counters[CNT_STAT_BASE + (slot & (CNT_STAT_SLOTS - 1)) * CNT_STAT_STRIDE + k], v);
@@ -65,17 +73,20 @@ __global__ void k_march_tiles(Pools p, FrameK f, Spec s, uint32_t *__restrict__ 
                               uint32_t *__restrict__ counters, const uint32_t *__restrict__ order,
                               uint32_t *__restrict__ cost);
 
-// Launch ordering (most expensive cost bucket first). Entries are tile indices, or with a
-// view table (n_views > 0) packed view << ENTRY_VIEW_SHIFT | tile-in-view. The element
-// count is *n_dev when n_dev is non-null, else n.
+// Launch ordering (class, then most expensive cost bucket first). Entries are tile
+// indices, or with a view table (n_views > 0) packed view << ENTRY_VIEW_SHIFT |
+// tile-in-view. The element count is *n_dev when n_dev is non-null, else n. cb, when
+// non-null, holds the class boundaries cb[0..4] of the tile index space (views are laid
+// out class by class); without it every tile is class 0.
 constexpr uint32_t ENTRY_VIEW_SHIFT = 20, ENTRY_TILE_MASK = (1u << ENTRY_VIEW_SHIFT) - 1u;
 constexpr uint32_t BATCH_MAX_VIEWS = 1024;
 constexpr uint32_t ORDER_BLOCKS = 256;
 __global__ void k_cost_hist(const uint32_t *__restrict__ cost, const uint32_t *__restrict__ n_dev, uint32_t n,
-                            uint32_t *__restrict__ ctl);
+                            const uint32_t *__restrict__ cb, uint32_t *__restrict__ ctl);
 __global__ void k_cost_scatter(const uint32_t *__restrict__ cost, const uint32_t *__restrict__ n_dev, uint32_t n,
-                               uint32_t *__restrict__ ctl, const uint32_t *__restrict__ tile_off,
-                               const uint32_t *__restrict__ n_views_dev, uint32_t *__restrict__ order);
+                               const uint32_t *__restrict__ cb, uint32_t *__restrict__ ctl,
+                               const uint32_t *__restrict__ tile_off, const uint32_t *__restrict__ n_views_dev,
+                               uint32_t *__restrict__ order);
 __global__ void k_iota_views(uint32_t *__restrict__ order, const uint32_t *__restrict__ n_dev, uint32_t n,
                              const uint32_t *__restrict__ tile_off, const uint32_t *__restrict__ n_views_dev);
 
@@ -92,8 +103,8 @@ struct ViewK {
 };
 
 // Header words of a batch in device memory (read by graph-captured kernels, so a replay
-// follows the current view set).
-constexpr uint32_t BH_TILES = 0, BH_VIEWS = 1, BH_LEN = 4;
+// follows the current view set): tile and view counts, class boundaries cb[0..4].
+constexpr uint32_t BH_TILES = 0, BH_VIEWS = 1, BH_CB = 4, BH_LEN = 12;
 
 struct BatchArgs {
     Pools p;                  // tree pools; chunks/grid are the batch's concatenated arrays
@@ -107,12 +118,19 @@ struct BatchArgs {
     uint32_t *__restrict__ cost;
 };
 
-// Batched tile march: FLAT = one workgroup per queue entry; PERSISTENT = resident waves
-// taking `chunk` entries at a time from the queue head in the control block.
-template <int kOcc, int kVar>
+// Batched tile march over the queue range of one view class (kHiz, kUnderwater; the
+// range is the header's cb[c]..cb[c+1]): FLAT = one workgroup per entry; PERSISTENT =
+// resident waves taking `chunk` entries at a time from the class's queue head.
+template <int kOcc, int kVar, bool kHiz, bool kUnderwater>
 __global__ void k_batch_flat(BatchArgs a);
-template <int kOcc, int kVar>
+template <int kOcc, int kVar, bool kHiz, bool kUnderwater>
 __global__ void k_batch_persistent(BatchArgs a, uint32_t chunk);
+
+// Launches the batched march over every non-empty class range cb[c]..cb[c+1] (host
+// copies of the header's class boundaries).
+bool batch_form_ok(uint32_t mode, uint32_t occ, uint32_t var);
+void launch_batch_classes(uint32_t mode, uint32_t occ, uint32_t var, uint32_t chunk, uint32_t wps, int n_cu,
+                          const uint32_t cb[N_CLASSES + 1], const BatchArgs &a, hipStream_t st);
 __global__ void k_shadow(Pools p, FrameK f, Spec s, const unsigned long long *__restrict__ vis,
                          uint16_t *__restrict__ out, uint32_t *__restrict__ counters);
 __global__ void k_decode(Pools p, FrameK f, const unsigned long long *__restrict__ vis,

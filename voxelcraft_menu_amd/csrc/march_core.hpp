@@ -64,7 +64,7 @@ __device__ __forceinline__ uint32_t ld_brick(const Pools &p, uint32_t i) {
     return i < p.n_bricks ? p.bricks[i] : 0u;
 }
 
-// The chunk fields the march reads (GpuChunk, 96 B) as four 16-byte loads.
+// The chunk fields the march reads from a GpuChunk record (96 B).
 struct ChunkK {
     float bmin[3], vs, bmax[3], origin[3], fade;
     uint32_t root[4];
@@ -72,19 +72,39 @@ struct ChunkK {
     uint32_t dry[2];
 };
 
+// Read-only data of a launch (chunk records, view descriptors) is read through the
+// constant address space: with a wave-uniform index the loads become scalar loads into
+// SGPRs instead of one vector load per lane.
+template <typename T>
+using const_as = const __attribute__((address_space(4))) T;
+template <typename T>
+__device__ __forceinline__ const_as<T> *as_const(const T *p) {
+    return (const_as<T> *)p;
+}
+// Copy of a trivially copyable record, read word by word through the constant space.
+template <typename T>
+__device__ __forceinline__ T ld_const(const T *p) {
+    static_assert(sizeof(T) % 4 == 0, "record of whole words");
+    T out;
+    const_as<uint32_t> *src = as_const(reinterpret_cast<const uint32_t *>(p));
+    uint32_t *dst = reinterpret_cast<uint32_t *>(&out);
+#pragma unroll
+    for (uint32_t i = 0; i < sizeof(T) / 4; i++) dst[i] = src[i];
+    return out;
+}
+
 __device__ __forceinline__ ChunkK ld_chunk(const Pools &p, uint32_t ci) {
     ChunkK c;
     if (ci >= p.n_chunks) ci = 0;  // callers guard; keeps the read inside the buffer
-    const uint4 *q = reinterpret_cast<const uint4 *>(p.chunks + ci);
-    uint4 a = q[0], b = q[1], o = q[2], r = q[3], at = q[4], d = q[5];
-    c.bmin[0] = __uint_as_float(a.x); c.bmin[1] = __uint_as_float(a.y); c.bmin[2] = __uint_as_float(a.z);
-    c.vs = __uint_as_float(a.w);
-    c.bmax[0] = __uint_as_float(b.x); c.bmax[1] = __uint_as_float(b.y); c.bmax[2] = __uint_as_float(b.z);
-    c.origin[0] = __uint_as_float(o.x); c.origin[1] = __uint_as_float(o.y); c.origin[2] = __uint_as_float(o.z);
-    c.fade = __uint_as_float(o.w);
-    c.root[0] = r.x; c.root[1] = r.y; c.root[2] = r.z; c.root[3] = r.w;
-    c.attr_base = at.x; c.attr_flags = at.y;
-    c.dry[0] = d.x; c.dry[1] = d.y;
+    const_as<uint32_t> *w = as_const(reinterpret_cast<const uint32_t *>(p.chunks + ci));
+    c.bmin[0] = __uint_as_float(w[0]); c.bmin[1] = __uint_as_float(w[1]); c.bmin[2] = __uint_as_float(w[2]);
+    c.vs = __uint_as_float(w[3]);
+    c.bmax[0] = __uint_as_float(w[4]); c.bmax[1] = __uint_as_float(w[5]); c.bmax[2] = __uint_as_float(w[6]);
+    c.origin[0] = __uint_as_float(w[8]); c.origin[1] = __uint_as_float(w[9]); c.origin[2] = __uint_as_float(w[10]);
+    c.fade = __uint_as_float(w[11]);
+    c.root[0] = w[12]; c.root[1] = w[13]; c.root[2] = w[14]; c.root[3] = w[15];
+    c.attr_base = w[16]; c.attr_flags = w[17];
+    c.dry[0] = w[20]; c.dry[1] = w[21];
     return c;
 }
 
@@ -307,22 +327,25 @@ struct Hit {
 // march_chunk (common.wgsl:579-910). Foliage tufts are not handled here: the
 // context refuses SPEC_FOLIAGE, so foliage_aware is false on every call.
 // inv is the ray's 1.0 / rd (the same value march_chunk's own `let inv = 1.0 / rd`
-// computes for every chunk; callers compute it once per ray). With kLdsL1 the L1 nodes
-// of a non-full root are read from l1s, a copy of the chunk's inner run staged in LDS.
+// computes for every chunk; callers compute it once per ray). kUnit: the chunk's voxel
+// size is exactly 1.0f, so the divisions and products by it are the identity.
 // The ray kind (skip_water, skip_glass, sun_ray) is a compile-time parameter: primary
 // rays use <false,false,false> (or <true,false,false> under water), sun rays
 // <true,true,true>, so each kernel carries only the branches its rays can take.
-template <bool skip_water, bool skip_glass, bool sun_ray, bool kLdsL1 = false>
-__device__ inline Hit march_chunk(const Pools &p, const FrameK &f, const Spec &s, const ChunkK &c, const float ro_w[3],
-                                  const float rd[3], const float inv[3], float t_limit, const uint4 *l1s = nullptr) {
+template <bool skip_water, bool skip_glass, bool sun_ray, bool kUnit>
+__device__ __forceinline__ Hit march_chunk_vs(const Pools &p, const FrameK &f, const Spec &s, const ChunkK &c, const float ro_w[3],
+                                  const float rd[3], const float inv[3], float t_limit) {
     Hit h{false, 0.0f, 0u, 0u, 0u, 0u};
     const float vs = c.vs;
+    // x / vs and x * vs; with kUnit (vs == 1.0f) both are x exactly.
+    auto dvs = [&](float x, float v) { return kUnit ? x : x / v; };
+    auto mvs = [&](float x, float v) { return kUnit ? x : x * v; };
     float ro[3], bmin[3], bmax[3], tmin3[3], tmax3[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        ro[i] = div_vs(ro_w[i] - c.origin[i], vs);
-        bmin[i] = div_vs(c.bmin[i] - c.origin[i], vs);
-        bmax[i] = div_vs(c.bmax[i] - c.origin[i], vs);
+        ro[i] = dvs(ro_w[i] - c.origin[i], vs);
+        bmin[i] = dvs(c.bmin[i] - c.origin[i], vs);
+        bmax[i] = dvs(c.bmax[i] - c.origin[i], vs);
         float t0 = (bmin[i] - ro[i]) * inv[i];
         float t1 = (bmax[i] - ro[i]) * inv[i];
         tmin3[i] = fminf(t0, t1);
@@ -330,7 +353,7 @@ __device__ inline Hit march_chunk(const Pools &p, const FrameK &f, const Spec &s
     }
     const float t_enter = fmaxf(fmaxf(tmin3[0], tmin3[1]), tmin3[2]);
     const float t_exit = fminf(fminf(tmax3[0], tmax3[1]), tmax3[2]);
-    const float t_end = fminf(t_exit, div_vs(t_limit, vs));
+    const float t_end = fminf(t_exit, dvs(t_limit, vs));
     float t = fmaxf(t_enter, 0.0f);
     if (t >= t_end) return h;
     uint32_t axis;
@@ -383,7 +406,7 @@ __device__ inline Hit march_chunk(const Pools &p, const FrameK &f, const Spec &s
     const bool root_full = (c.root[2] & FULL_BIT) != 0u;
     if (root_full && !(s.bits & SPEC_FULL_MARCH) && !noncaster_aware && !water_interface) {
         h.hit = true;
-        h.t = mul_vs(t, vs);
+        h.t = mvs(t, vs);
         h.normal = axis * 2u + (pd[axis] ? 0u : 1u);
         h.voxel = (uint32_t)cell[0] | ((uint32_t)cell[1] << 6) | ((uint32_t)cell[2] << 12);
         h.micro = micro_entry((uint32_t)cell[0], (uint32_t)cell[1], (uint32_t)cell[2], ro, rd, t, axis, pd, t > 0.0f);
@@ -408,7 +431,6 @@ __device__ inline Hit march_chunk(const Pools &p, const FrameK &f, const Spec &s
             if (l1b != l1_cached) {
                 l1_cached = l1b;
                 if (root_full) l1 = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, FULL_BIT, (l1b * 64u) | 0xF0000000u);
-                else if (kLdsL1) l1 = l1s[mask_below(root_mask, l1b)];
                 else l1 = ld_inner(p, root_ptr + mask_below(root_mask, l1b));
                 leaf_cached = 0xFFFFFFFFu;
             }
@@ -475,8 +497,8 @@ __device__ inline Hit march_chunk(const Pools &p, const FrameK &f, const Spec &s
                 if (axis == 0) o[0] = sgn;
                 else if (axis == 1) o[1] = sgn;
                 else o[2] = sgn;
-                float tw = mul_vs(t, vs);
-                float off = mul_vs(0.002f, vs);
+                float tw = mvs(t, vs);
+                float off = mvs(0.002f, vs);
                 float b0 = ro_w[0] + rd[0] * tw, b1 = ro_w[1] + rd[1] * tw, b2 = ro_w[2] + rd[2] * tw;
                 if (world_block_at(p, f, b0 + o[0] * off, b1 + o[1] * off, b2 + o[2] * off) == WATER_ID) {
                     solid = false;
@@ -504,7 +526,7 @@ __device__ inline Hit march_chunk(const Pools &p, const FrameK &f, const Spec &s
                 MicroHit q = cutout_march(s.leaf_fill, wx, wy, wz, ux, uy, uz, ro, rd, inv, t, axis, pd, t > 0.0f);
                 if (q.hit) {
                     h.hit = true;
-                    h.t = mul_vs(q.t, vs);
+                    h.t = mvs(q.t, vs);
                     h.normal = q.normal;
                     h.voxel = ux | (uy << 6) | (uz << 12);
                     h.micro = q.micro;
@@ -517,7 +539,7 @@ __device__ inline Hit march_chunk(const Pools &p, const FrameK &f, const Spec &s
         }
         if (solid) {
             h.hit = true;
-            h.t = mul_vs(t, vs);
+            h.t = mvs(t, vs);
             h.normal = axis * 2u + (pd[axis] ? 0u : 1u);
             h.voxel = ux | (uy << 6) | (uz << 12);
             h.micro = micro_entry(ux, uy, uz, ro, rd, t, axis, pd, t > 0.0f);
@@ -546,6 +568,17 @@ __device__ inline Hit march_chunk(const Pools &p, const FrameK &f, const Spec &s
     }
     h.iters = iters;
     return h;
+}
+
+// march_chunk dispatches on the chunk's voxel size: LOD-0 chunks (vs == 1.0f) take the
+// variant with no division or multiplication by vs, which is bit-identical because
+// x / 1.0f == x * 1.0f == x in IEEE f32. Without the split the compiler evaluates the
+// correctly rounded divisions of the general case and then selects x.
+template <bool skip_water, bool skip_glass, bool sun_ray>
+__device__ __forceinline__ Hit march_chunk(const Pools &p, const FrameK &f, const Spec &s, const ChunkK &c,
+                                           const float ro_w[3], const float rd[3], const float inv[3], float t_limit) {
+    if (c.vs == 1.0f) return march_chunk_vs<skip_water, skip_glass, sun_ray, true>(p, f, s, c, ro_w, rd, inv, t_limit);
+    return march_chunk_vs<skip_water, skip_glass, sun_ray, false>(p, f, s, c, ro_w, rd, inv, t_limit);
 }
 
 // depth_key / make_key (common.wgsl:1320-1330)
@@ -624,7 +657,9 @@ __device__ __forceinline__ float hit_t(const FrameK &f, const ChunkK &c, uint32_
     if (axis == 0) plane = bx + (float)(mx + add) * ms;
     else if (axis == 1) plane = by + (float)(my + add) * ms;
     else plane = bz + (float)(mz + add) * ms;
-    return (plane - f.cam[axis]) / rd[axis];
+    const float cam_a = axis == 0 ? f.cam[0] : (axis == 1 ? f.cam[1] : f.cam[2]);
+    const float rd_a = axis == 0 ? rd[0] : (axis == 1 ? rd[1] : rd[2]);
+    return (plane - cam_a) / rd_a;
 }
 
 // f32 -> f16 bits, IEEE round-to-nearest-even (the R16F store of primary_shadow).

@@ -148,23 +148,20 @@ struct vx_serve {
 
 namespace {
 
-template <int O, int V>
-void launch_march(uint32_t mode, uint32_t chunk, uint32_t wps, int n_cu, uint32_t tiles, const BatchArgs &a,
-                  hipStream_t st) {
-    if (mode == VX_BATCH_FLAT)
-        k_batch_flat<O, V><<<tiles, 64, 0, st>>>(a);
-    else
-        k_batch_persistent<O, V><<<(unsigned)n_cu * 4u * wps, 64, 0, st>>>(a, chunk);
-}
-
 int n_cus(int dev) {
     int n = 0;
     if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
     return n;
 }
 
+uint32_t req_class(const Req *r) {
+    return view_class((r->vk.f.flags & VX_FLAG_HIZ) != 0u, (r->vk.f.flags & VX_FLAG_UNDERWATER) != 0u);
+}
+
 // Enqueues one batch (the lane's admitted requests) on the lane stream.
 int launch(vx_serve *s, Lane &L) {
+    // Class-major layout: each view class is one tile range marched by its own kernel.
+    std::stable_sort(L.reqs.begin(), L.reqs.end(), [](const Req *x, const Req *y) { return req_class(x) < req_class(y); });
     const uint32_t n = (uint32_t)L.reqs.size();
     size_t recs = 0;
     uint32_t tiles = 0;
@@ -194,6 +191,7 @@ int launch(vx_serve *s, Lane &L) {
     uint32_t *grids = (uint32_t *)(stg + o_grids);
     uint32_t t = 0;
     size_t c = 0;
+    uint32_t cb[N_CLASSES + 1] = {0, 0, 0, 0, 0};
     for (uint32_t v = 0; v < n; v++) {
         const Req *r = L.reqs[v];
         vk[v] = r->vk;
@@ -207,11 +205,13 @@ int launch(vx_serve *s, Lane &L) {
         std::memcpy(grids + (size_t)v * VX_GRID_LEN, r->grid.data(), VX_GRID_LEN * 4);
         c += r->chunks.size();
         t += r->tiles;
+        for (uint32_t k = req_class(r) + 1; k <= N_CLASSES; k++) cb[k] = t;
     }
     toff[n] = t;
+    for (uint32_t k = 0; k < BH_LEN; k++) hdr[k] = 0;
     hdr[BH_TILES] = tiles;
     hdr[BH_VIEWS] = n;
-    hdr[2] = hdr[3] = 0;
+    for (uint32_t k = 0; k <= N_CLASSES; k++) hdr[BH_CB + k] = cb[k];
     hipStream_t st = L.st;
     char *din = (char *)L.in.p;
     HIPX(hipMemcpyAsync(din, stg, total, hipMemcpyHostToDevice, st));
@@ -234,12 +234,8 @@ int launch(vx_serve *s, Lane &L) {
     a.order = (const uint32_t *)L.order.p;
     a.cost = (uint32_t *)L.cost.p;
     const vx_batch_tuning &tn = s->d.tuning;
-    const int ncu = n_cus(s->ctx->device);
-    switch (tn.occupancy * 16 + tn.variant) {
-        case 4 * 16 + 5: launch_march<4, 5>(tn.mode, tn.queue_chunk, tn.waves_per_simd, ncu, tiles, a, st); break;
-        case 6 * 16 + 4: launch_march<6, 4>(tn.mode, tn.queue_chunk, tn.waves_per_simd, ncu, tiles, a, st); break;
-        default: launch_march<4, 4>(tn.mode, tn.queue_chunk, tn.waves_per_simd, ncu, tiles, a, st); break;
-    }
+    launch_batch_classes(tn.mode, tn.occupancy, tn.variant, tn.queue_chunk, tn.waves_per_simd, n_cus(s->ctx->device), cb,
+                         a, st);
     k_stat_sum<<<1, 64, 0, st>>>((const uint32_t *)L.counters.p, STAT_SHADOW, L.stat_dev);
     VX_LAUNCHED();
     HIPX(hipEventRecord(L.done, st));
@@ -348,9 +344,7 @@ int vx_serve_create(vx_ctx *c, const vx_serve_desc *d, vx_serve **out) {
     if (d->lanes < 1 || d->lanes > 8 || d->max_views < 1 || d->max_views > BATCH_MAX_VIEWS) return VX_E_ARG;
     if (d->hbm_budget < ((uint64_t)1 << 20)) return VX_E_ARG;
     const vx_batch_tuning &t = d->tuning;
-    if (t.mode == VX_BATCH_PLAIN || t.mode > VX_BATCH_PLAIN) return VX_E_ARG;
-    if (!((t.occupancy == 4 && (t.variant == 4 || t.variant == 5)) || (t.occupancy == 6 && t.variant == 4)))
-        return VX_E_ARG;
+    if (!batch_form_ok(t.mode, t.occupancy, t.variant)) return VX_E_ARG;
     if (t.mode == VX_BATCH_PERSISTENT &&
         (t.queue_chunk < 1 || t.queue_chunk > 64 || t.waves_per_simd < 1 || t.waves_per_simd > t.occupancy))
         return VX_E_ARG;

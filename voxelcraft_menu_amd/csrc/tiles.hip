@@ -211,70 +211,68 @@ VX_TILES_INST(4, 5)
 VX_TILES_INST(6, 0)
 VX_TILES_INST(6, 4)
 
-// ---- Batched march: many views, one launch -------------------------------------------
+// ---- Batched march: many views, one launch per view class ------------------------------
 // A queue entry names (view, tile). The view's frame, spec and buffer offsets are read
 // from its descriptor (wave-uniform, so scalar loads), and the tile runs the same body
 // as the single-view kernel, so a pixel's key, shadow word and Hi-Z word are the same
-// bits whichever way the views are launched.
-template <int kVar>
+// bits whichever way the views are launched. Views are laid out class by class (Hi-Z
+// on/off x under water or not) and each class range is marched by a kernel specialised
+// for it: one body per kernel keeps the register count at the single-view kernel's.
+template <int kVar, bool kHiz, bool kUnderwater>
 __device__ __forceinline__ void batch_tile(const BatchArgs &a, uint32_t entry, uint32_t n_views, uint64_t *defmask,
                                            uint64_t *selmask) {
     entry = (uint32_t)__builtin_amdgcn_readfirstlane((int)entry);  // uniform: descriptor reads are scalar
     const uint32_t v = entry >> ENTRY_VIEW_SHIFT, tile = entry & ENTRY_TILE_MASK;
     if (v >= n_views) return;
-    // References into the descriptor, not copies: the march indexes frame arrays with
-    // run-time indices, and a local copy would be placed in per-lane scratch memory.
+    // The descriptor is read through the constant address space with a uniform index:
+    // scalar loads, and the frame fields live in SGPRs (the march indexes no frame array
+    // with a run-time index, so the copy never needs addressable per-lane memory).
     const ViewK *vk = a.views + v;
-    const FrameK &f = vk->f;
-    const Spec &s = vk->s;
+    const FrameK f = ld_const(&vk->f);
+    const Spec s = ld_const(&vk->s);
     if (tile >= f.tiles[0] * f.tiles[1]) return;
+    const_as<ViewK> *vc = as_const(vk);
     Pools p = a.p;
-    p.chunks = a.p.chunks + vk->chunk_off;
-    p.grid = a.p.grid + vk->grid_off;
-    p.n_chunks = vk->n_chunks;
-    const size_t px0 = (size_t)vk->px_off_lo | ((size_t)vk->px_off_hi << 32);
-    const uint32_t toff = vk->tile_off;
-    uint32_t *hiz = a.hiz + toff;
-    unsigned long long *vis = a.vis + px0;
-    uint16_t *shadow = a.shadow + px0;
+    p.chunks = a.p.chunks + vc->chunk_off;
+    p.grid = a.p.grid + vc->grid_off;
+    p.n_chunks = vc->n_chunks;
+    const size_t px0 = (size_t)vc->px_off_lo | ((size_t)vc->px_off_hi << 32);
+    const uint32_t toff = vc->tile_off;
     const uint64_t t0 = wall_clock64();
-    const bool hz = (f.flags & VX_FLAG_HIZ) != 0u, uw = (f.flags & VX_FLAG_UNDERWATER) != 0u;
-    if (uw) {
-        if (hz) march_tile_body<true, true, kVar>(p, f, s, tile, hiz, vis, shadow, a.counters, defmask, selmask);
-        else march_tile_body<false, true, kVar>(p, f, s, tile, hiz, vis, shadow, a.counters, defmask, selmask);
-    } else {
-        if (hz) march_tile_body<true, false, kVar>(p, f, s, tile, hiz, vis, shadow, a.counters, defmask, selmask);
-        else march_tile_body<false, false, kVar>(p, f, s, tile, hiz, vis, shadow, a.counters, defmask, selmask);
-    }
+    march_tile_body<kHiz, kUnderwater, kVar>(p, f, s, tile, a.hiz + toff, a.vis + px0, a.shadow + px0, a.counters,
+                                             defmask, selmask);
     if (lane_id() == 0) {
         uint64_t dt = wall_clock64() - t0;
         a.cost[toff + tile] = dt > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)dt;
     }
 }
 
-template <int kOcc, int kVar>
+template <int kOcc, int kVar, bool kHiz, bool kUnderwater>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kOcc))) void k_batch_flat(BatchArgs a) {
-    __shared__ uint64_t defmask[128];
+    __shared__ uint64_t defmask[kHiz ? 128 : 1];
     __shared__ uint64_t selmask[(kVar & VAR_SPLIT) ? 128 : 1];
-    const uint32_t n = a.hdr[BH_TILES];
-    if (blockIdx.x >= n) return;
+    constexpr uint32_t cls = view_class(kHiz, kUnderwater);
+    const uint32_t k = a.hdr[BH_CB + cls] + blockIdx.x;
+    if (k >= a.hdr[BH_CB + cls + 1]) return;
     if (blockIdx.x < kPrioTiles) __builtin_amdgcn_s_setprio(3);
-    batch_tile<kVar>(a, a.order[blockIdx.x], a.hdr[BH_VIEWS], defmask, selmask);
+    batch_tile<kVar, kHiz, kUnderwater>(a, a.order[k], a.hdr[BH_VIEWS], defmask, selmask);
 }
 
 // Persistent form: the grid is sized to the resident wave slots; every wave takes
-// `chunk` consecutive queue entries per atomic on the head word until the queue is
-// exhausted (the head only grows, so every wave reaches the exit). The queue is sorted
-// most expensive first, so small chunks keep the heavy head of the queue spread over
-// many waves.
-template <int kOcc, int kVar>
+// `chunk` consecutive entries of its class range per atomic on the class's head word
+// until the range is exhausted (the head only grows, so every wave reaches the exit).
+// The range is sorted most expensive first, so small chunks keep the heavy head of the
+// queue spread over many waves.
+template <int kOcc, int kVar, bool kHiz, bool kUnderwater>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kOcc))) void k_batch_persistent(BatchArgs a,
                                                                                                    uint32_t chunk) {
-    __shared__ uint64_t defmask[128];
+    __shared__ uint64_t defmask[kHiz ? 128 : 1];
     __shared__ uint64_t selmask[(kVar & VAR_SPLIT) ? 128 : 1];
-    const uint32_t n = a.hdr[BH_TILES];
+    constexpr uint32_t cls = view_class(kHiz, kUnderwater);
+    const uint32_t begin = a.hdr[BH_CB + cls], end = a.hdr[BH_CB + cls + 1];
     const uint32_t nv = a.hdr[BH_VIEWS];
-    uint32_t *head = a.counters + CNT_CTL + CTL_HEAD;
+    const uint32_t n = end > begin ? end - begin : 0u;
+    uint32_t *head = a.counters + CNT_CTL + CTL_HEAD + cls * CTL_HEAD_STRIDE;
     const uint32_t grab = chunk ? chunk : 1u;
     for (;;) {
         uint32_t base = 0;
@@ -283,17 +281,61 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kOcc))) void
         if (base >= n) break;
         if (base < kPrioTiles) __builtin_amdgcn_s_setprio(3);
         else __builtin_amdgcn_s_setprio(0);
-        const uint32_t end = min(base + grab, n);
-        for (uint32_t k = base; k < end; k++) batch_tile<kVar>(a, a.order[k], nv, defmask, selmask);
+        const uint32_t stop = min(base + grab, n);
+        for (uint32_t k = base; k < stop; k++)
+            batch_tile<kVar, kHiz, kUnderwater>(a, a.order[begin + k], nv, defmask, selmask);
     }
 }
 
-#define VX_BATCH_INST(O, V)                                        \
-    template __global__ void k_batch_flat<O, V>(BatchArgs);        \
-    template __global__ void k_batch_persistent<O, V>(BatchArgs, uint32_t);
+#define VX_BATCH_INST1(O, V, H, U)                                               \
+    template __global__ void k_batch_flat<O, V, H, U>(BatchArgs);                \
+    template __global__ void k_batch_persistent<O, V, H, U>(BatchArgs, uint32_t);
+#define VX_BATCH_INST(O, V)              \
+    VX_BATCH_INST1(O, V, true, false)   \
+    VX_BATCH_INST1(O, V, false, false)  \
+    VX_BATCH_INST1(O, V, true, true)    \
+    VX_BATCH_INST1(O, V, false, true)
 VX_BATCH_INST(4, 4)
 VX_BATCH_INST(4, 5)
 VX_BATCH_INST(6, 4)
+VX_BATCH_INST(6, 5)
+
+bool batch_form_ok(uint32_t mode, uint32_t occ, uint32_t var) {
+    if (mode != VX_BATCH_FLAT && mode != VX_BATCH_PERSISTENT) return false;
+    return (occ == 4 || occ == 6) && (var == 4 || var == 5);
+}
+
+template <int O, int V, bool H, bool U>
+static void launch_class(uint32_t mode, uint32_t chunk, uint32_t wps, int n_cu, uint32_t n, const BatchArgs &a,
+                         hipStream_t st) {
+    if (mode == VX_BATCH_FLAT) k_batch_flat<O, V, H, U><<<n, 64, 0, st>>>(a);
+    else k_batch_persistent<O, V, H, U><<<(unsigned)n_cu * 4u * wps, 64, 0, st>>>(a, chunk);
+}
+
+template <int O, int V>
+static void launch_classes_ov(uint32_t mode, uint32_t chunk, uint32_t wps, int n_cu, const uint32_t cb[N_CLASSES + 1],
+                              const BatchArgs &a, hipStream_t st) {
+    for (uint32_t c = 0; c < N_CLASSES; c++) {
+        const uint32_t n = cb[c + 1] - cb[c];
+        if (!n) continue;
+        switch (c) {
+            case 0: launch_class<O, V, true, false>(mode, chunk, wps, n_cu, n, a, st); break;
+            case 1: launch_class<O, V, false, false>(mode, chunk, wps, n_cu, n, a, st); break;
+            case 2: launch_class<O, V, true, true>(mode, chunk, wps, n_cu, n, a, st); break;
+            default: launch_class<O, V, false, true>(mode, chunk, wps, n_cu, n, a, st); break;
+        }
+    }
+}
+
+void launch_batch_classes(uint32_t mode, uint32_t occ, uint32_t var, uint32_t chunk, uint32_t wps, int n_cu,
+                          const uint32_t cb[N_CLASSES + 1], const BatchArgs &a, hipStream_t st) {
+    switch (occ * 16 + var) {
+        case 4 * 16 + 5: launch_classes_ov<4, 5>(mode, chunk, wps, n_cu, cb, a, st); break;
+        case 6 * 16 + 4: launch_classes_ov<6, 4>(mode, chunk, wps, n_cu, cb, a, st); break;
+        case 6 * 16 + 5: launch_classes_ov<6, 5>(mode, chunk, wps, n_cu, cb, a, st); break;
+        default: launch_classes_ov<4, 4>(mode, chunk, wps, n_cu, cb, a, st); break;
+    }
+}
 
 // Launch order for the next frame: tiles grouped into 32 buckets by the log2 of their
 // measured cycles, most expensive bucket first (longest-processing-time-first list
@@ -316,48 +358,65 @@ __device__ __forceinline__ uint32_t tile_entry(uint32_t i, const uint32_t *toff,
     return (lo << ENTRY_VIEW_SHIFT) | (i - toff[lo]);
 }
 
+// Order bin of global tile i: its class (from the class boundaries, classes are laid out
+// one after another) and its cost bucket, most expensive first inside the class.
+struct ClassBounds {
+    uint32_t b1, b2, b3;
+    __device__ explicit ClassBounds(const uint32_t *cb)
+        : b1(cb ? cb[1] : 0xFFFFFFFFu), b2(cb ? cb[2] : 0xFFFFFFFFu), b3(cb ? cb[3] : 0xFFFFFFFFu) {}
+    __device__ uint32_t bin(uint32_t i, uint32_t cost) const {
+        const uint32_t cls = (uint32_t)(i >= b1) + (uint32_t)(i >= b2) + (uint32_t)(i >= b3);
+        return cls * COST_BUCKETS + (COST_BUCKETS - 1u - cost_bucket(cost));
+    }
+};
+
 __global__ __launch_bounds__(256) void k_cost_hist(const uint32_t *__restrict__ cost, const uint32_t *__restrict__ n_dev,
-                                                   uint32_t n, uint32_t *__restrict__ ctl) {
-    __shared__ uint32_t hist[COST_BUCKETS];
+                                                   uint32_t n, const uint32_t *__restrict__ cb,
+                                                   uint32_t *__restrict__ ctl) {
+    __shared__ uint32_t hist[ORDER_BINS];
     if (n_dev) n = *n_dev;
+    const ClassBounds bounds(cb);
     const uint32_t t = threadIdx.x;
-    if (t < COST_BUCKETS) hist[t] = 0;
+    for (uint32_t k = t; k < ORDER_BINS; k += blockDim.x) hist[k] = 0;
     __syncthreads();
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + t; i < n; i += stride) atomicAdd(&hist[cost_bucket(cost[i])], 1u);
+    for (uint32_t i = blockIdx.x * blockDim.x + t; i < n; i += stride) atomicAdd(&hist[bounds.bin(i, cost[i])], 1u);
     __syncthreads();
-    if (t < COST_BUCKETS && hist[t]) atomicAdd(&ctl[CTL_HIST + t], hist[t]);
+    for (uint32_t k = t; k < ORDER_BINS; k += blockDim.x)
+        if (hist[k]) atomicAdd(&ctl[CTL_HIST + k], hist[k]);
 }
 
 __global__ __launch_bounds__(256) void k_cost_scatter(const uint32_t *__restrict__ cost,
                                                       const uint32_t *__restrict__ n_dev, uint32_t n,
-                                                      uint32_t *__restrict__ ctl, const uint32_t *__restrict__ tile_off,
+                                                      const uint32_t *__restrict__ cb, uint32_t *__restrict__ ctl,
+                                                      const uint32_t *__restrict__ tile_off,
                                                       const uint32_t *__restrict__ n_views_dev,
                                                       uint32_t *__restrict__ order) {
-    __shared__ uint32_t cnt[COST_BUCKETS];
-    __shared__ uint32_t base[COST_BUCKETS];
+    __shared__ uint32_t cnt[ORDER_BINS];
+    __shared__ uint32_t base[ORDER_BINS];
     __shared__ uint32_t toff[BATCH_MAX_VIEWS + 1];
     if (n_dev) n = *n_dev;
+    const ClassBounds bounds(cb);
     const uint32_t nv = n_views_dev ? min(*n_views_dev, BATCH_MAX_VIEWS) : 0u;
     const uint32_t t = threadIdx.x;
-    if (t < COST_BUCKETS) cnt[t] = 0;
+    for (uint32_t k = t; k < ORDER_BINS; k += blockDim.x) cnt[k] = 0;
     if (nv)
         for (uint32_t v = t; v <= nv; v += blockDim.x) toff[v] = tile_off[v];
     __syncthreads();
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + t; i < n; i += stride) atomicAdd(&cnt[cost_bucket(cost[i])], 1u);
+    for (uint32_t i = blockIdx.x * blockDim.x + t; i < n; i += stride) atomicAdd(&cnt[bounds.bin(i, cost[i])], 1u);
     __syncthreads();
-    if (t < COST_BUCKETS) {
+    for (uint32_t k = t; k < ORDER_BINS; k += blockDim.x) {
         uint32_t start = 0;
-        for (uint32_t b = t + 1u; b < COST_BUCKETS; b++) start += ctl[CTL_HIST + b];
-        const uint32_t c = cnt[t];
-        base[t] = c ? start + atomicAdd(&ctl[CTL_CURSOR + t], c) : 0u;
-        cnt[t] = 0;
+        for (uint32_t q = 0; q < k; q++) start += ctl[CTL_HIST + q];
+        const uint32_t c = cnt[k];
+        base[k] = c ? start + atomicAdd(&ctl[CTL_CURSOR + k], c) : 0u;
+        cnt[k] = 0;
     }
     __syncthreads();
     for (uint32_t i = blockIdx.x * blockDim.x + t; i < n; i += stride) {
-        const uint32_t b = cost_bucket(cost[i]);
-        order[base[b] + atomicAdd(&cnt[b], 1u)] = tile_entry(i, toff, nv);
+        const uint32_t k = bounds.bin(i, cost[i]);
+        order[base[k] + atomicAdd(&cnt[k], 1u)] = tile_entry(i, toff, nv);
     }
 }
 

@@ -65,8 +65,8 @@ int fill_u64(hipStream_t st, void *p, uint64_t v, size_t n) {
 
 void launch_order(hipStream_t st, const uint32_t *cost, uint32_t n, uint32_t *ctl, uint32_t *order) {
     unsigned nb = std::max(1u, std::min<unsigned>(ORDER_BLOCKS, (n + 255u) / 256u));
-    k_cost_hist<<<nb, 256, 0, st>>>(cost, nullptr, n, ctl);
-    k_cost_scatter<<<nb, 256, 0, st>>>(cost, nullptr, n, ctl, nullptr, nullptr, order);
+    k_cost_hist<<<nb, 256, 0, st>>>(cost, nullptr, n, nullptr, ctl);
+    k_cost_scatter<<<nb, 256, 0, st>>>(cost, nullptr, n, nullptr, ctl, nullptr, nullptr, order);
 }
 
 }  // namespace vxh
