@@ -165,7 +165,7 @@ typedef struct vx_hit {
 
 /* Kernel variants chosen by the autotuner (no counterpart in the reference). */
 typedef struct vx_tuning {
-    uint32_t occupancy;       /* tile march built for 4 or 6 waves per SIMD */
+    uint32_t occupancy;       /* tile march built for 4 or 6 waves per SIMD (6: variants 0, 4, 5) */
     uint32_t variant;         /* bit 0: classify the whole chunk list before marching;
                                  bit 2: trace the sun-shadow ray in the tile's wave (fused) */
     uint32_t ordered;         /* 1: launch tiles most expensive first (costs of the previous frame) */

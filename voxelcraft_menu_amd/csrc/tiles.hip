@@ -210,6 +210,7 @@ VX_TILES_INST(4, 4)
 VX_TILES_INST(4, 5)
 VX_TILES_INST(6, 0)
 VX_TILES_INST(6, 4)
+VX_TILES_INST(6, 5)
 
 // ---- Batched march: many views, one launch per view class ------------------------------
 // A queue entry names (view, tile). The view's frame, spec and buffer offsets are read

@@ -125,7 +125,7 @@ static void launch_tiles_ov(bool hz, bool uw, unsigned nb, hipStream_t st, const
 // Valid (occupancy, variant) pairs compiled into the library.
 bool tuning_ok(uint32_t occ, uint32_t var) {
     if (occ == 4) return var == 0 || var == 1 || var == 4 || var == 5;
-    if (occ == 6) return var == 0 || var == 4;
+    if (occ == 6) return var == 0 || var == 4 || var == 5;
     return false;
 }
 
@@ -137,6 +137,7 @@ void launch_tiles(uint32_t occ, uint32_t var, bool hz, bool uw, unsigned nb, hip
         case 4 * 16 + 5: launch_tiles_ov<4, 5>(hz, uw, nb, st, a); break;
         case 6 * 16 + 0: launch_tiles_ov<6, 0>(hz, uw, nb, st, a); break;
         case 6 * 16 + 4: launch_tiles_ov<6, 4>(hz, uw, nb, st, a); break;
+        case 6 * 16 + 5: launch_tiles_ov<6, 5>(hz, uw, nb, st, a); break;
         default: launch_tiles_ov<4, 0>(hz, uw, nb, st, a); break;
     }
 }
