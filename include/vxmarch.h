@@ -21,6 +21,7 @@
  *   vx_read_decoded           shadow_guide + hit_t decode    (src/render/shaders/shadow.wgsl:8-30,
  *                                                            src/render/shaders/common.wgsl:1300-1318)
  *   vx_pair_demand            Renderer::pair_demand          (src/render/mod.rs:1948-1975)
+ *   vx_march_stats / read_heat Renderer::march_stats           (src/render/mod.rs:1977-2045)
  *   vx_timings                Readback per-pass timestamps   (src/render/timer.rs:150-176)
  *   vx_batch_*                no counterpart: many views per launch (the reference renders
  *                             one view per render() call, src/headless.rs:680-837)
@@ -74,7 +75,12 @@ enum {
 #define VX_FLAG_WATER_FAR 2097152u
 #define VX_FLAG_WATER_DARK 4194304u
 /* spec_hi bits (src/render/mod.rs:220-270). */
+#define VX_FLAG_WATER_REFLECT 16u
+#define VX_FLAG_WATER_REFRACT 32u
+#define VX_FLAG_WAVES 512u
 #define VX_FLAG_HI_FULL_MARCH 2u
+#define VX_FLAG_HI_GLASS_REFLECT 128u
+#define VX_FLAG_HI_WATER_LOOK 4096u
 #define VX_FLAG_HI_EMITTER_WORLD 4u
 #define VX_FLAG_HI_SNELL_BEND 256u
 #define VX_FLAG_HI_WIND_SWAY 32768u
@@ -197,6 +203,23 @@ int vx_timings_get(vx_ctx *ctx, vx_timings *out);
  * most recent 256 at most; *frames receives the count). */
 int vx_timings_reset(vx_ctx *ctx);
 int vx_timings_sum(vx_ctx *ctx, vx_timings *out, uint32_t *frames);
+/* FLAG_HEATMAP (the reference's iteration heatmap, march.wgsl:51-53): a frame whose flags
+ * hold VX_FLAG_HEATMAP runs the reference's pair decomposition and counts, per pixel, the
+ * march_chunk iterations of every pair marched there (sun shadows are off in such frames,
+ * as in shadow_active). vx_read_heat copies that counter (W*H u32); vx_march_stats_get
+ * summarises it as Renderer::march_stats does (src/render/mod.rs:1977-2045). Both
+ * return VX_E_ARG unless the last frame was a heatmap frame. */
+typedef struct vx_march_stats {
+    uint64_t pixels, iters, touched, capped;  /* capped: pixels above MAX_ITERS (512) */
+    uint32_t worst, p50, p90, p99;
+    double top_1, top_10;                     /* % of all iterations in the top 1 % / 10 % pixels */
+    uint32_t chunks, pad;                     /* visible chunks of the frame */
+    vx_pair_demand pairs;
+} vx_march_stats;
+int vx_read_heat(vx_ctx *ctx, uint32_t *out, size_t n);
+int vx_march_stats_get(vx_ctx *ctx, vx_march_stats *out);
+/* The same summary of a caller-held counter (no device work). */
+int vx_march_stats_compute(const uint32_t *heat, size_t n, uint32_t chunks, vx_march_stats *out);
 /* Device pointers of the current outputs (for callers that keep data on the GPU). */
 int vx_device_outputs(vx_ctx *ctx, uint64_t **vis, uint16_t **shadow);
 
@@ -257,6 +280,26 @@ int vx_batch_demand_get(vx_batch *b, vx_pair_demand *out);
 int vx_batch_timings_reset(vx_batch *b);
 int vx_batch_timings_sum(vx_batch *b, float *total_ms, float *march_ms, uint32_t *frames);
 int vx_batch_device_outputs(vx_batch *b, uint32_t view, uint64_t **vis, uint16_t **shadow);
+
+/* Secondary bounce (the reflect / refract / transmit legs of resolve.wgsl's shade_water
+ * and shade_glass, src/render/shaders/resolve.wgsl:1029-1058, :1209-1220, :1261-1310, each
+ * traced with trace_world, common.wgsl:1205-1253). With it on, every vx_batch_render also
+ * gathers the legs of the pixels whose primary hit is water or (glass specialisation)
+ * glass into compact queues and traces them. Water normals are the flat face normal, so
+ * views whose pipeline key enables waves (VX_FLAG_WAVES in spec and frame flags) or the
+ * water-look ripples return VX_E_UNSUPPORTED. Two records per pixel: [0] refract (water)
+ * or transmit (glass), [1] reflect. */
+typedef struct vx_sec_hit {
+    float t;            /* distance along the leg to its hit, or the leg's range */
+    uint32_t voxel;     /* x | y << 6 | z << 12 in the hit chunk */
+    uint32_t info;      /* chunk index | normal << 13 | VX_SEC_TRACED | VX_SEC_HIT (chunk 0x1FFF: miss) */
+    uint32_t material;  /* block id at the hit */
+} vx_sec_hit;
+#define VX_SEC_TRACED (1u << 30)
+#define VX_SEC_HIT (1u << 31)
+int vx_batch_set_secondary(vx_batch *b, uint32_t on);
+int vx_batch_read_secondary(vx_batch *b, uint32_t view, vx_sec_hit *out, size_t n);
+int vx_batch_secondary_count(vx_batch *b, uint64_t *water_legs, uint64_t *glass_legs, uint32_t *overflow);
 
 /* ---- Serving: continuous batching of view requests --------------------------------
  * No counterpart in the reference (its request loop renders and waits one view at a

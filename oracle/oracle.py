@@ -23,12 +23,13 @@ class Pools(C.Structure):
 
 class Spec(C.Structure):
     _fields_ = [("leaf_cutout", C.c_uint32), ("glass", C.c_uint32), ("full_march", C.c_uint32),
-                ("emitter_gather", C.c_uint32), ("water_repair", C.c_uint32), ("leaf_fill", C.c_float)]
+                ("emitter_gather", C.c_uint32), ("water_repair", C.c_uint32), ("leaf_fill", C.c_float),
+                ("glass_reflect", C.c_uint32)]
 
 
 _STAT_FIELDS = ["pairs_select", "pairs_deferred", "pairs_recover", "rays_primary", "rays_shadow",
                 "chunk_marches", "iters", "l1_loads", "leaf_loads", "attr_loads", "brick_words",
-                "grid_loads", "chunk_loads", "node_bytes", "capped", "oob"]
+                "grid_loads", "chunk_loads", "node_bytes", "capped", "oob", "rays_secondary"]
 
 
 class Stats(C.Structure):
@@ -43,6 +44,7 @@ class Hit(C.Structure):
                 ("voxel", C.c_uint32 * 3), ("micro", C.c_uint32 * 3), ("iters", C.c_uint32)]
 
 
+SEC = np.dtype([("t", "<f4"), ("voxel", "<u4"), ("info", "<u4"), ("material", "<u4")])
 DECODED = np.dtype([("voxel", "<i4", 3), ("face", "<u4"), ("material", "<u4"), ("ci", "<u4"), ("t", "<f4")])
 
 _lib = None
@@ -65,6 +67,12 @@ def lib():
         L.vxo_render.restype = C.c_int
         L.vxo_render.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, C.POINTER(Spec), V, V, V,
                                  C.POINTER(Stats), C.c_int]
+        L.vxo_render_heat.restype = C.c_int
+        L.vxo_render_heat.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, C.POINTER(Spec), V, V, V, V,
+                                      C.POINTER(Stats), C.c_int]
+        L.vxo_secondary.restype = C.c_int
+        L.vxo_secondary.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, C.POINTER(Spec), V, V,
+                                    C.POINTER(Stats), C.c_int]
         L.vxo_primary_shadow.restype = C.c_int
         L.vxo_primary_shadow.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, C.POINTER(Spec), V, V,
                                          C.POINTER(Stats), C.c_int]
@@ -106,6 +114,7 @@ def spec_from(flags: int, spec_hi: int) -> Spec:
         emitter_gather=1 if spec_hi & 4 else 0,
         water_repair=0 if spec_hi & 2097152 else 1,
         leaf_fill=0.62 if flags & 524288 else 0.72,
+        glass_reflect=1 if spec_hi & 128 else 0,
     )
 
 
@@ -140,6 +149,39 @@ def render(scene: Scene, frame_bytes: bytes, spec: Spec, hiz: np.ndarray, shadow
     if rc != 0:
         raise RuntimeError("oracle render rejected its arguments")
     return vis, sh, hz, st.as_dict()
+
+
+def render_heat(scene: Scene, frame_bytes: bytes, spec: Spec, hiz: np.ndarray, shadow: bool = True,
+                threads: int = 0):
+    """render() plus the FLAG_HEATMAP counter (march.wgsl:51-53): per pixel, the sum of
+    march_chunk iterations over every marched pair. Returns (vis, shadow, hiz_out, heat, stats)."""
+    f = np.frombuffer(frame_bytes, dtype=np.uint8).copy()
+    res = np.frombuffer(frame_bytes[80:88], dtype=np.uint32)
+    w, h = int(res[0]), int(res[1])
+    vis = np.zeros(w * h, dtype=np.uint64)
+    heat = np.zeros(w * h, dtype=np.uint32)
+    sh = np.zeros(w * h, dtype=np.uint16) if shadow else None
+    hz = np.ascontiguousarray(hiz, dtype=np.uint32).copy()
+    st = Stats()
+    rc = lib().vxo_render_heat(C.byref(scene.pools), _p(scene.chunks), scene.n_chunks, _p(scene.grid), _p(f),
+                               C.byref(spec), _p(hz), _p(vis), _p(sh), _p(heat), C.byref(st), threads)
+    if rc != 0:
+        raise RuntimeError("oracle render rejected its arguments")
+    return vis, sh, hz, heat, st.as_dict()
+
+
+def secondary(scene: Scene, frame_bytes: bytes, spec: Spec, vis: np.ndarray, threads: int = 0):
+    """Secondary legs of every pixel: (W*H, 2) array of SEC records ([:, 0] refract or
+    transmit, [:, 1] reflect), and the stats of the pass."""
+    f = np.frombuffer(frame_bytes, dtype=np.uint8).copy()
+    v = np.ascontiguousarray(vis, dtype=np.uint64)
+    out = np.zeros((v.size, 2), dtype=SEC)
+    st = Stats()
+    rc = lib().vxo_secondary(C.byref(scene.pools), _p(scene.chunks), scene.n_chunks, _p(scene.grid), _p(f),
+                             C.byref(spec), _p(v), _p(out), C.byref(st), threads)
+    if rc != 0:
+        raise RuntimeError("oracle secondary rejected its arguments")
+    return out, st.as_dict()
 
 
 def primary_shadow(scene: Scene, frame_bytes: bytes, spec: Spec, vis: np.ndarray, threads: int = 0):

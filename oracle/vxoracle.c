@@ -604,6 +604,172 @@ static float shadow_ray_t(ctx *c, const float ro[3], const float rd[3], float ma
     return -1.0f;
 }
 
+/* ---- secondary legs ---------------------------------------------------------------- */
+#define WATER_IOR 1.333f           /* common.wgsl:1256 */
+#define SURFACE_EPS 0.02f          /* common.wgsl:1258 */
+#define WATER_REFRACT_NEAR 96.0f   /* resolve.wgsl:873-880, :886 */
+#define WATER_REFRACT_FAR 192.0f
+#define WATER_REFLECT_NEAR 512.0f
+#define WATER_REFLECT_FAR 1024.0f
+#define WATER_REFRACT_DIST 96.0f
+#define WATER_REFLECT_DIST 768.0f
+#define GLASS_TRANSMIT_DIST 192.0f
+#define FLAG_WATER_REFLECT 16u     /* common.wgsl:146-147 */
+#define FLAG_WATER_REFRACT 32u
+
+typedef struct world_hit {
+    int hit;
+    float t;
+    uint32_t normal, voxel, ci;
+} world_hit;
+
+static float wgsl_sign(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
+
+/* trace_world (common.wgsl:1205-1253): grid DDA over chunk cells (at most 16) from ro,
+ * marching each resident chunk with skip_glass; the first hit ends the walk. */
+static world_hit trace_world(ctx *c, const float ro[3], const float rd[3], float max_dist, int skip_water) {
+    const vxo_frame *f = c->f;
+    world_hit out = {0, max_dist, 0u, 0u, NO_CHUNK};
+    float dir[3], inv[3];
+    int pos_dir[3];
+    for (int k = 0; k < 3; k++) {
+        int tiny = fabsf(rd[k]) < 1e-6f;
+        dir[k] = tiny ? 1e-6f * wgsl_sign(rd[k] + 1e-9f) : rd[k];
+        inv[k] = 1.0f / dir[k];
+        pos_dir[k] = dir[k] > 0.0f;
+    }
+    int32_t cell[3];
+    for (int k = 0; k < 3; k++) cell[k] = (int32_t)floorf(ro[k] / 64.0f);
+    for (uint32_t i = 0; i < 16u; i++) {
+        int32_t rel[3];
+        for (int k = 0; k < 3; k++) rel[k] = cell[k] - f->grid_min[k];
+        for (int k = 0; k < 3; k++)
+            if (rel[k] < 0 || rel[k] >= (int32_t)f->grid_dim[k]) return out;
+        uint32_t u[3] = {(uint32_t)rel[0], (uint32_t)rel[1], (uint32_t)rel[2]};
+        uint32_t ci = grid_at(c, u[0] + u[1] * f->grid_dim[0] + u[2] * f->grid_dim[0] * f->grid_dim[1]);
+        float tn[3];
+        for (int k = 0; k < 3; k++) {
+            float bound = (float)(pos_dir[k] ? cell[k] + 1 : cell[k]) * 64.0f;
+            tn[k] = (bound - ro[k]) * inv[k];
+        }
+        float t_exit = tn[0];
+        uint32_t axis = 0;
+        if (tn[1] < t_exit) { t_exit = tn[1]; axis = 1; }
+        if (tn[2] < t_exit) { t_exit = tn[2]; axis = 2; }
+        if (ci != NO_CHUNK) {
+            vxo_hit h;
+            march_chunk(c, ci, ro, rd, fminf(t_exit, max_dist), skip_water, 0, 1, 0, &h);
+            if (h.hit) {
+                out.hit = 1;
+                out.t = h.t;
+                out.normal = h.normal;
+                out.voxel = h.voxel[0] | (h.voxel[1] << 6) | (h.voxel[2] << 12);
+                out.ci = ci;
+                return out;
+            }
+        }
+        if (t_exit >= max_dist) return out;
+        cell[axis] += pos_dir[axis] ? 1 : -1;
+    }
+    return out;
+}
+
+/* WGSL builtins, operation order fixed: dot = (x*x' + y*y') + z*z'. */
+static float dot3o(const float a[3], const float b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+/* refract(I, N, eta) = k < 0 ? 0 : eta*I - (eta*dot(N,I) + sqrt(k))*N,
+ * k = 1 - eta*eta*(1 - dot(N,I)*dot(N,I)) */
+static void wgsl_refract(const float i[3], const float n[3], float eta, float out[3]) {
+    float d = dot3o(n, i);
+    float k = 1.0f - eta * eta * (1.0f - d * d);
+    if (k < 0.0f) { out[0] = out[1] = out[2] = 0.0f; return; }
+    float s = eta * d + sqrtf(k);
+    for (int q = 0; q < 3; q++) out[q] = eta * i[q] - s * n[q];
+}
+
+/* reflect(I, N) = I - 2*dot(N,I)*N */
+static void wgsl_reflect(const float i[3], const float n[3], float out[3]) {
+    float d2 = 2.0f * dot3o(n, i);
+    for (int q = 0; q < 3; q++) out[q] = i[q] - d2 * n[q];
+}
+
+/* smoothstep(e0, e1, x) = t*t*(3 - 2t), t = clamp((x - e0)/(e1 - e0), 0, 1) */
+static float wgsl_smoothstep(float e0, float e1, float x) {
+    float t = fminf(fmaxf((x - e0) / (e1 - e0), 0.0f), 1.0f);
+    return t * t * (3.0f - 2.0f * t);
+}
+
+static void store_leg(ctx *c, const world_hit *w, vxo_sec *o) {
+    o->t = w->t;
+    o->voxel = w->hit ? w->voxel : 0u;
+    o->info = (w->hit ? (w->ci & 0x1FFFu) : 0x1FFFu) | ((w->hit ? w->normal : 0u) << 13) | (1u << 30) |
+              (w->hit ? (1u << 31) : 0u);
+    o->material = 0u;
+    if (w->hit) {
+        uint32_t v[3] = {w->voxel & 63u, (w->voxel >> 6) & 63u, (w->voxel >> 12) & 63u};
+        o->material = block_at(c, w->ci, v);
+    }
+    c->st.rays_secondary++;
+}
+
+static float hit_t(ctx *c, uint32_t ci, const uint32_t v[3], const uint32_t micro[3], uint32_t nid,
+                   const float rd[3]);
+
+/* The legs of one pixel: shade_water (resolve.wgsl:1209-1220 with water_ctx's flat normal,
+ * :1078-1167: no waves, no water-look ripples) and shade_glass (:1261-1310). */
+static void secondary_pixel(ctx *c, const uint64_t *vis, uint32_t px, uint32_t py, vxo_sec out[2]) {
+    const vxo_frame *f = c->f;
+    memset(out, 0, 2 * sizeof(vxo_sec));
+    uint64_t key = vis[(size_t)py * f->res[0] + px];
+    if (key == 0) return;
+    uint32_t ci = (uint32_t)(key >> 24u) & 0x1FFFu;
+    uint32_t low = (uint32_t)(key & 0xFFFFFFu);
+    uint32_t v[3] = {(low >> 16u) & VOXEL_MASK, (low >> 8u) & VOXEL_MASK, low & VOXEL_MASK};
+    uint32_t micro[3] = {(low >> 22u) & 3u, (low >> 14u) & 3u, (low >> 6u) & 3u};
+    uint32_t nid = (uint32_t)(key >> 37u) & 7u;
+    uint32_t id = block_at(c, ci, v);
+    int glass = c->s->glass && id == GLASS_ID;
+    if (id != WATER_ID && !glass) return;
+    float rd[3], n[3], hit[3];
+    ray_dir(f, (float)px + 0.5f, (float)py + 0.5f, rd);
+    float t = hit_t(c, ci, v, micro, nid, rd);
+    for (int k = 0; k < 3; k++) hit[k] = f->cam_pos[k] + rd[k] * t;
+    normal_of(nid, n);
+    if (id == WATER_ID) {
+        float refract_detail = 1.0f - wgsl_smoothstep(WATER_REFRACT_NEAR, WATER_REFRACT_FAR, t);
+        float reflect_detail = 1.0f - wgsl_smoothstep(WATER_REFLECT_NEAR, WATER_REFLECT_FAR, t);
+        if ((f->flags & FLAG_WATER_REFRACT) != 0u && refract_detail > 0.0f) {
+            float rdir[3], ro2[3];
+            wgsl_refract(rd, n, 1.0f / WATER_IOR, rdir);
+            for (int k = 0; k < 3; k++) ro2[k] = hit[k] + rdir[k] * SURFACE_EPS;
+            world_hit w = trace_world(c, ro2, rdir, WATER_REFRACT_DIST, 1);
+            store_leg(c, &w, &out[0]);
+        }
+        if ((f->flags & FLAG_WATER_REFLECT) != 0u && reflect_detail > 0.0f) {
+            float m[3], ro3[3];
+            for (int k = 0; k < 3; k++) ro3[k] = hit[k] + n[k] * SURFACE_EPS;
+            wgsl_reflect(rd, n, m);
+            world_hit w = trace_world(c, ro3, m, WATER_REFLECT_DIST, 1);
+            store_leg(c, &w, &out[1]);
+        }
+    } else {
+        if (dot3o(n, rd) > 0.0f)
+            for (int k = 0; k < 3; k++) n[k] = -n[k];
+        float dist = fmaxf(GLASS_TRANSMIT_DIST, f->far_);
+        float ro2[3];
+        for (int k = 0; k < 3; k++) ro2[k] = hit[k] + rd[k] * SURFACE_EPS;
+        world_hit w = trace_world(c, ro2, rd, dist, 0);
+        store_leg(c, &w, &out[0]);
+        if (c->s->glass_reflect) {
+            float m[3], o[3];
+            for (int k = 0; k < 3; k++) o[k] = hit[k] + n[k] * SURFACE_EPS;
+            wgsl_reflect(rd, n, m);
+            world_hit w2 = trace_world(c, o, m, dist, 0);
+            store_leg(c, &w2, &out[1]);
+        }
+    }
+}
+
 /* hit_t (common.wgsl:1300-1318) */
 static float hit_t(ctx *c, uint32_t ci, const uint32_t v[3], const uint32_t micro[3], uint32_t nid,
                    const float rd[3]) {
@@ -669,6 +835,7 @@ typedef struct job {
     uint32_t *hiz;
     uint64_t *vis;
     uint16_t *shadow;
+    uint32_t *heat;        /* FLAG_HEATMAP frames: marched iterations per pixel, or NULL */
     int do_march;
     int next;              /* atomic work counter (rows of tiles or rows of pixels) */
     int nthreads;
@@ -681,8 +848,10 @@ static void ctx_init(ctx *c, const job *j) {
     c->grid = j->grid; c->f = j->f; c->s = j->s;
 }
 
-/* March every pixel of one tile against chunk ci (march.wgsl:1-55, Snell bend off). */
-static void march_pair(ctx *c, uint32_t tile, uint32_t ci, uint64_t *tilevis) {
+/* March every pixel of one tile against chunk ci (march.wgsl:1-55, Snell bend off).
+ * tileheat, when non-NULL, receives each pixel's h.iters (the FLAG_HEATMAP atomicAdd
+ * into dbg, march.wgsl:51-53). */
+static void march_pair(ctx *c, uint32_t tile, uint32_t ci, uint64_t *tilevis, uint32_t *tileheat) {
     const vxo_frame *f = c->f;
     uint32_t tx = tile % f->tiles[0], ty = tile / f->tiles[0];
     int underwater = (f->flags & FLAG_UNDERWATER) != 0u;
@@ -700,6 +869,7 @@ static void march_pair(ctx *c, uint32_t tile, uint32_t ci, uint64_t *tilevis) {
                 uint64_t k = vxo_make_key(f, h.t, h.normal, ci, h.voxel, h.micro);
                 if (k > tilevis[ly * 8 + lx]) tilevis[ly * 8 + lx] = k;
             }
+            if (tileheat) tileheat[ly * 8 + lx] += h.iters;
         }
     }
 }
@@ -764,6 +934,8 @@ static void *frame_worker(void *arg) {
     uint32_t *sel = (uint32_t *)malloc(sizeof(uint32_t) * (f->chunk_count + 1));
     uint32_t *def = (uint32_t *)malloc(sizeof(uint32_t) * (f->chunk_count + 1));
     uint64_t tv[64];
+    uint32_t th[64];
+    uint32_t *heat = (j->heat && (f->flags & FLAG_HEATMAP) != 0u) ? th : NULL;
     for (;;) {
         int row = __atomic_fetch_add(&j->next, 1, __ATOMIC_SEQ_CST);
         if ((uint32_t)row >= f->tiles[1]) break;
@@ -777,7 +949,8 @@ static void *frame_worker(void *arg) {
             c.st.pairs_select += nsel;
             c.st.pairs_deferred += ndef;
             memset(tv, 0, sizeof tv);
-            for (uint32_t k = 0; k < nsel; k++) march_pair(&c, tile, sel[k], tv);
+            memset(th, 0, sizeof th);
+            for (uint32_t k = 0; k < nsel; k++) march_pair(&c, tile, sel[k], tv, heat);
             if (use_hiz) {
                 /* build_hiz (march.wgsl:57-76): min of the 24-bit depth, 0 off-frame. */
                 uint32_t mn = 0xFFFFFFFFu;
@@ -795,7 +968,7 @@ static void *frame_worker(void *arg) {
                     float dmin = box_distance(f->cam_pos, ch->aabb_min, ch->aabb_max);
                     if (vxo_depth_key(f, dmin) >= mn) {
                         c.st.pairs_recover++;
-                        march_pair(&c, tile, def[k], tv);
+                        march_pair(&c, tile, def[k], tv, heat);
                     }
                 }
             } else {
@@ -804,7 +977,10 @@ static void *frame_worker(void *arg) {
             for (uint32_t ly = 0; ly < 8; ly++)
                 for (uint32_t lx = 0; lx < 8; lx++) {
                     uint32_t px = gx * 8u + lx, py = gy * 8u + ly;
-                    if (px < f->res[0] && py < f->res[1]) j->vis[(size_t)py * f->res[0] + px] = tv[ly * 8 + lx];
+                    if (px < f->res[0] && py < f->res[1]) {
+                        j->vis[(size_t)py * f->res[0] + px] = tv[ly * 8 + lx];
+                        if (j->heat) j->heat[(size_t)py * f->res[0] + px] = th[ly * 8 + lx];
+                    }
                 }
         }
     }
@@ -920,9 +1096,60 @@ int vxo_primary_shadow(const vxo_pools *p, const vxo_chunk *chunks, uint32_t n_c
     return 0;
 }
 
+typedef struct secjob {
+    job base;
+    const uint64_t *vis_in;
+    vxo_sec *out;
+} secjob;
+
+static void *secondary_worker(void *arg) {
+    secjob *sj = (secjob *)arg;
+    job *j = &sj->base;
+    int tid = __atomic_fetch_add(&j->nthreads, 1, __ATOMIC_SEQ_CST);
+    ctx c;
+    ctx_init(&c, j);
+    const vxo_frame *f = j->f;
+    for (;;) {
+        int row = __atomic_fetch_add(&j->next, 1, __ATOMIC_SEQ_CST);
+        if ((uint32_t)row >= f->res[1]) break;
+        for (uint32_t px = 0; px < f->res[0]; px++)
+            secondary_pixel(&c, sj->vis_in, px, (uint32_t)row, &sj->out[2 * ((size_t)row * f->res[0] + px)]);
+    }
+    j->per_thread[tid] = c.st;
+    return NULL;
+}
+
+int vxo_secondary(const vxo_pools *p, const vxo_chunk *chunks, uint32_t n_chunks, const uint32_t *grid,
+                  const vxo_frame *f, const vxo_spec *s, const uint64_t *vis, vxo_sec *out,
+                  vxo_stats *stats, int threads) {
+    if (!p || !chunks || !grid || !f || !s || !vis || !out) return -1;
+    int n = threads > 0 ? threads : vxo_hardware_threads();
+    secjob sj;
+    memset(&sj, 0, sizeof sj);
+    sj.base.p = p; sj.base.chunks = chunks; sj.base.n_chunks = n_chunks; sj.base.grid = grid;
+    sj.base.f = f; sj.base.s = s;
+    sj.base.per_thread = (vxo_stats *)calloc((size_t)n, sizeof(vxo_stats));
+    sj.vis_in = vis;
+    sj.out = out;
+    run_threads(secondary_worker, &sj, n);
+    if (stats) {
+        memset(stats, 0, sizeof *stats);
+        sum_stats(stats, sj.base.per_thread, n);
+        finish_bytes(stats);
+    }
+    free(sj.base.per_thread);
+    return 0;
+}
+
 int vxo_render(const vxo_pools *p, const vxo_chunk *chunks, uint32_t n_chunks, const uint32_t *grid,
                const vxo_frame *f, const vxo_spec *s, uint32_t *hiz, uint64_t *vis, uint16_t *shadow,
                vxo_stats *stats, int threads) {
+    return vxo_render_heat(p, chunks, n_chunks, grid, f, s, hiz, vis, shadow, NULL, stats, threads);
+}
+
+int vxo_render_heat(const vxo_pools *p, const vxo_chunk *chunks, uint32_t n_chunks, const uint32_t *grid,
+                    const vxo_frame *f, const vxo_spec *s, uint32_t *hiz, uint64_t *vis, uint16_t *shadow,
+                    uint32_t *heat, vxo_stats *stats, int threads) {
     if (!p || !chunks || !grid || !f || !s || !hiz || !vis) return -1;
     if (f->chunk_count > n_chunks) return -1;
     int n = threads > 0 ? threads : vxo_hardware_threads();
@@ -930,7 +1157,7 @@ int vxo_render(const vxo_pools *p, const vxo_chunk *chunks, uint32_t n_chunks, c
     job j;
     memset(&j, 0, sizeof j);
     j.p = p; j.chunks = chunks; j.n_chunks = n_chunks; j.grid = grid; j.f = f; j.s = s;
-    j.hiz = hiz; j.vis = vis;
+    j.hiz = hiz; j.vis = vis; j.heat = heat;
     j.per_thread = (vxo_stats *)calloc((size_t)n, sizeof(vxo_stats));
     run_threads(frame_worker, &j, n);
     if (stats) sum_stats(stats, j.per_thread, n);

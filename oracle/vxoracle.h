@@ -67,6 +67,7 @@ typedef struct vxo_spec {
     uint32_t emitter_gather;  /* SPEC_EMITTER_GATHER */
     uint32_t water_repair;    /* SPEC_WATER_REPAIR */
     float leaf_fill;          /* SPEC_LEAF_FILL */
+    uint32_t glass_reflect;   /* SPEC_GLASS_REFLECT */
 } vxo_spec;
 
 /* Algorithmic traffic and work of one frame, counted exactly by the restatement. */
@@ -82,6 +83,7 @@ typedef struct vxo_stats {
     uint64_t node_bytes;       /* 96*chunk + 16*l1 + 8*leaf + 4*(attr+brick+grid) */
     uint64_t capped;           /* marches stopped by MAX_ITERS */
     uint64_t oob;              /* pool reads outside the supplied arrays */
+    uint64_t rays_secondary;   /* secondary legs traced (trace_world calls) */
 } vxo_stats;
 
 /* Single-function restatements exposed for pinning tests. */
@@ -128,6 +130,30 @@ int vxo_render(const vxo_pools *p, const vxo_chunk *chunks, uint32_t n_chunks,
                const uint32_t *grid, const vxo_frame *f, const vxo_spec *s,
                uint32_t *hiz, uint64_t *vis, uint16_t *shadow, vxo_stats *stats,
                int threads);
+
+/* vxo_render with the FLAG_HEATMAP counter: heat[W*H] receives, for a frame whose flags
+ * hold FLAG_HEATMAP, the sum of march_chunk iterations over every pair marched at each
+ * pixel (march.wgsl:51-53), zero elsewhere; heat may be NULL. */
+int vxo_render_heat(const vxo_pools *p, const vxo_chunk *chunks, uint32_t n_chunks,
+                    const uint32_t *grid, const vxo_frame *f, const vxo_spec *s,
+                    uint32_t *hiz, uint64_t *vis, uint16_t *shadow, uint32_t *heat,
+                    vxo_stats *stats, int threads);
+
+/* One secondary leg: trace_world's result (common.wgsl:1205-1253) plus the block id at its
+ * hit. info = ci | normal << 13 | traced << 30 | hit << 31 (ci = 0x1FFF on a miss). */
+typedef struct vxo_sec {
+    float t;            /* hit distance along the leg, or its max_dist */
+    uint32_t voxel;     /* x | y << 6 | z << 12 */
+    uint32_t info;
+    uint32_t material;  /* block_at at the hit, 0 otherwise */
+} vxo_sec;
+
+/* Secondary bounce of every pixel of a visibility buffer (the legs of resolve.wgsl's
+ * shade_water and shade_glass with flat water normals): out[2*pix] is the refract (water)
+ * or transmit (glass) leg, out[2*pix+1] the reflect leg. */
+int vxo_secondary(const vxo_pools *p, const vxo_chunk *chunks, uint32_t n_chunks, const uint32_t *grid,
+                  const vxo_frame *f, const vxo_spec *s, const uint64_t *vis, vxo_sec *out,
+                  vxo_stats *stats, int threads);
 
 /* Shadow pass alone, from a given visibility buffer. */
 int vxo_primary_shadow(const vxo_pools *p, const vxo_chunk *chunks, uint32_t n_chunks,

@@ -84,9 +84,11 @@ def test_struct_sizes_match_the_c_header(tmp_path):
     names = {"vx_gpu_frame": A.Frame, "vx_gpu_chunk": A.Chunk, "vx_edit": A.Edit, "vx_range": A.Range,
              "vx_ctx_desc": A.CtxDesc, "vx_tuning": A.Tuning, "vx_pair_demand": A.PairDemand,
              "vx_timings": A.Timings, "vx_hit": A.Hit, "vx_view": A.ViewDesc, "vx_batch_tuning": A.BatchTuning,
-             "vx_serve_desc": A.ServeDesc, "vx_serve_result": A.ServeResult, "vx_serve_stats": A.ServeStats}
+             "vx_serve_desc": A.ServeDesc, "vx_serve_result": A.ServeResult, "vx_serve_stats": A.ServeStats,
+             "vx_march_stats": A.MarchStats}
+    names_np = {"vx_sec_hit": A.SEC_DTYPE}
     src = tmp_path / "sz.c"
-    body = "".join(f'    printf("{n} %zu\\n", sizeof({n}));\n' for n in names)
+    body = "".join(f'    printf("{n} %zu\\n", sizeof({n}));\n' for n in list(names) + list(names_np))
     src.write_text(f'#include <stdio.h>\n#include "vxmarch.h"\nint main(void) {{\n{body}    return 0;\n}}\n')
     exe = tmp_path / "sz"
     inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
@@ -95,3 +97,5 @@ def test_struct_sizes_match_the_c_header(tmp_path):
     got = dict(zip(out[0::2], (int(x) for x in out[1::2])))
     for n, cls in names.items():
         assert got[n] == C.sizeof(cls), (n, got[n], C.sizeof(cls))
+    for n, dt in names_np.items():
+        assert got[n] == dt.itemsize, (n, got[n], dt.itemsize)

@@ -8,11 +8,12 @@ PERSISTENT queue, each with and without hipGraph replay) must agree with each ot
 import numpy as np
 import pytest
 
+from oracle import oracle as O
 from tests import helpers as H
 from voxelcraft_menu_amd import _abi as A
 from voxelcraft_menu_amd import workloads as W
 from voxelcraft_menu_amd.renderer import Batch, Renderer
-from voxelcraft_menu_amd.world import build_view
+from voxelcraft_menu_amd.world import World, build_view
 
 pytestmark = pytest.mark.gpu
 
@@ -182,5 +183,72 @@ def test_config_batch_64_views_all_forms_identical():
         ref = H.oracle_frames(world, views[i], s, hi, frames=2)
         assert np.array_equal(results[keys[0]][i][0], ref[1][0]), f"view {i} vs oracle"
         assert np.array_equal(results[keys[0]][i][1], ref[1][1]), f"view {i} vs oracle"
+    b.close()
+    r.close()
+
+
+def _sec_world():
+    from tests.test_secondary_oracle import _dense
+    world = World()
+    world.insert_dense((0, 0, 0), _dense())
+    return world
+
+
+@pytest.mark.parametrize("form", [dict(mode=A.BATCH_FLAT, graph=0), dict(mode=A.BATCH_PERSISTENT, graph=1),
+                                  dict(mode=A.BATCH_PLAIN, graph=0)], ids=_form_id)
+def test_secondary_legs_match_oracle(form):
+    """Secondary legs (water refract/reflect, glass transmit/reflect) of every pixel equal
+    the oracle's records bit for bit: on the constructed water/glass chunk and on views of
+    the 1024x1024x256 world with flat water."""
+    flags = A.FLAG_WATER_REFLECT | A.FLAG_WATER_REFRACT | A.FLAG_GLASS | A.FLAG_SHADOWS | A.FLAG_HIZ
+    cases = []
+    w1 = _sec_world()
+    cases.append((w1, [H.camera((30.5, 47.3, 6.5), 90.0, -25.0, 96, 64, flags)],
+                  A.FLAG_HI_FULL_MARCH | A.FLAG_HI_GLASS_REFLECT))
+    wl = W.WORKLOADS[1]
+    w2 = W.make_world(wl)
+    cams = W.random_cameras(wl, 4, seed=91, resolutions=[(320, 180), (200, 120)],
+                            flags=W.DEFAULT_FLAGS & ~A.FLAG_WAVES)
+    cases.append((w2, cams, W.DEFAULT_SPEC_HI))
+    for world, cams, spec_hi in cases:
+        views = [build_view(world, c) for c in cams]
+        specs = [W.spec_words(c.flags, world, spec_hi) for c in cams]
+        r = Renderer(64, 64)
+        r.sync_world(world)
+        b = Batch(r, 8)
+        b.set_tuning(**form)
+        b.set_views(views, specs)
+        b.set_secondary(True)
+        b.render()
+        b.sync()
+        total = 0
+        for i, (v, (s, hi)) in enumerate(zip(views, specs)):
+            sc = O.Scene(world.pool(0), world.pool(1), world.pool(2), v.chunks, v.grid)
+            sp = O.spec_from(s, hi)
+            vis = b.read_vis(i)
+            ref_vis = O.render(sc, v.frame_bytes, sp, np.zeros(v.tiles[0] * v.tiles[1], np.uint32), shadow=False)[0]
+            assert np.array_equal(vis, ref_vis)
+            legs, st = O.secondary(sc, v.frame_bytes, sp, ref_vis)
+            got = b.read_secondary(i)
+            diff = got.view(np.uint32) != legs.view(np.uint32)
+            assert not diff.any(), f"view {i}: {diff.any(axis=-1).sum()} leg records differ"
+            total += st["rays_secondary"]
+        cnt = b.secondary_count()
+        assert cnt["water_legs"] + cnt["glass_legs"] == total and not cnt["overflow"]
+        assert total > 100
+        b.close()
+        r.close()
+
+
+def test_secondary_refuses_wavy_water(small_world):
+    wl = W.WORKLOADS[0]
+    cam = W.random_cameras(wl, 1, seed=3, resolutions=[(64, 48)], flags=W.DEFAULT_FLAGS | A.FLAG_WAVES)[0]
+    v = build_view(small_world, cam)
+    r = Renderer(64, 48)
+    r.sync_world(small_world)
+    b = Batch(r, 2)
+    b.set_views([v], [W.spec_words(cam.flags, small_world)])
+    with pytest.raises(RuntimeError):
+        b.set_secondary(True)
     b.close()
     r.close()

@@ -184,3 +184,55 @@ def test_tiles_and_pairs_paths_agree_and_are_deterministic(big_world):
         r.close()
     for (a, b) in zip(outs[0], outs[1]):
         assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_heatmap_and_march_stats_match_oracle(small_world, path):
+    """FLAG_HEATMAP frames: per-pixel iteration counts (every marched pair, march.wgsl:51-53)
+    and the march_stats summary equal the oracle's, over two frames with Hi-Z carried;
+    a normal frame afterwards is exact again and has no heat to read."""
+    wl = W.WORKLOADS[0]
+    flags = W.DEFAULT_FLAGS | A.FLAG_HEATMAP
+    cams = [W.default_camera(wl, flags=flags)] + W.random_cameras(wl, 2, seed=77, resolutions=[(200, 120)],
+                                                                  flags=flags)
+    r = Renderer(cams[0].width, cams[0].height, path=path)
+    r.sync_world(small_world)
+    for ci, cam in enumerate(cams):
+        if (cam.width, cam.height) != r.size:
+            r.resize(cam.width, cam.height)
+        v = build_view(small_world, cam)
+        r.set_view(v)
+        spec, hi = W.spec_words(cam.flags, small_world)
+        sc = O.Scene(small_world.pool(0), small_world.pool(1), small_world.pool(2), v.chunks, v.grid)
+        sp = O.spec_from(spec, hi)
+        hz = r.read_hiz()
+        for f in range(2):
+            vis_o, sh_o, hz, heat_o, st = O.render_heat(sc, v.frame_bytes, sp, hz)
+            r.render(v.frame, spec, hi)
+            r.sync()
+            tag = f"cam{ci} frame{f} path{path}"
+            assert np.array_equal(r.read_vis(), vis_o), tag
+            assert np.array_equal(r.read_shadow(), sh_o) and not sh_o.any(), tag  # shadows off in heatmap frames
+            heat = r.read_heat()
+            assert np.array_equal(heat, heat_o), f"{tag}: {(heat != heat_o).sum()} heat words differ"
+            assert heat.sum() > 0
+            m = r.march_stats()
+            ref = A.march_stats_of(heat_o, v.n_visible)
+            for k in ("pixels", "iters", "touched", "capped", "worst", "p50", "p90", "p99", "top_1", "top_10",
+                      "chunks"):
+                assert m[k] == ref[k], (tag, k, m[k], ref[k])
+            assert (m["pairs"]["select"], m["pairs"]["deferred"], m["pairs"]["recover"]) == (
+                st["pairs_select"], st["pairs_deferred"], st["pairs_recover"]), tag
+    # back to a normal frame
+    cam = cams[-1]
+    cam.flags = W.DEFAULT_FLAGS
+    v = build_view(small_world, cam)
+    r.set_view(v)
+    spec, hi = W.spec_words(cam.flags, small_world)
+    ref = H.oracle_frames(small_world, v, spec, hi, frames=1, hiz=r.read_hiz())[0]
+    r.render(v.frame, spec, hi)
+    r.sync()
+    assert np.array_equal(r.read_vis(), ref[0]) and np.array_equal(r.read_shadow(), ref[1])
+    with pytest.raises(RuntimeError):
+        r.read_heat()
+    r.close()

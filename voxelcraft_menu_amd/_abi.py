@@ -36,6 +36,7 @@ FLAG_WATER_REFRACT = 32
 FLAG_UNDERWATER = 64
 FLAG_TINT = 128
 FLAG_FOLIAGE = 256
+FLAG_WAVES = 512
 FLAG_DISTANT_SHADOWS = 32768
 FLAG_LEAF_CUTOUT = 65536
 FLAG_GLASS = 131072
@@ -47,6 +48,8 @@ FLAG_SNELL = 8388608
 FLAG_HI_SKY_SPECULAR = 1
 FLAG_HI_FULL_MARCH = 2
 FLAG_HI_EMITTER_WORLD = 4
+FLAG_HI_GLASS_REFLECT = 128
+FLAG_HI_WATER_LOOK = 4096
 FLAG_HI_SNELL_BEND = 256
 FLAG_HI_WIND_SWAY = 32768
 FLAG_HI_SHADOW_PASS = 524288
@@ -197,6 +200,13 @@ class PairDemand(C.Structure):
                 ("cap", C.c_uint32), ("truncated", C.c_uint32), ("shadow_rays", C.c_uint32)]
 
 
+class MarchStats(C.Structure):
+    _fields_ = [("pixels", C.c_uint64), ("iters", C.c_uint64), ("touched", C.c_uint64), ("capped", C.c_uint64),
+                ("worst", C.c_uint32), ("p50", C.c_uint32), ("p90", C.c_uint32), ("p99", C.c_uint32),
+                ("top_1", C.c_double), ("top_10", C.c_double), ("chunks", C.c_uint32), ("pad", C.c_uint32),
+                ("pairs", PairDemand)]
+
+
 class Timings(C.Structure):
     _fields_ = [(n, C.c_float) for n in ("tile_select", "march", "hiz", "recover", "march2", "shadow", "total")]
 
@@ -275,6 +285,12 @@ _sig_march = {
     "vx_batch_timings_reset": (C.c_int, [P]),
     "vx_batch_timings_sum": (C.c_int, [P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_uint32)]),
     "vx_batch_device_outputs": (C.c_int, [P, C.c_uint32, P, P]),
+    "vx_read_heat": (C.c_int, [P, P, C.c_size_t]),
+    "vx_march_stats_get": (C.c_int, [P, C.POINTER(MarchStats)]),
+    "vx_march_stats_compute": (C.c_int, [P, C.c_size_t, C.c_uint32, C.POINTER(MarchStats)]),
+    "vx_batch_set_secondary": (C.c_int, [P, C.c_uint32]),
+    "vx_batch_read_secondary": (C.c_int, [P, C.c_uint32, P, C.c_size_t]),
+    "vx_batch_secondary_count": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
     "vx_serve_clock": (C.c_double, []),
     "vx_serve_create": (C.c_int, [P, C.POINTER(ServeDesc), C.POINTER(P)]),
     "vx_serve_destroy": (C.c_int, [P]),
@@ -338,3 +354,23 @@ def check(rc, what="call"):
 
 def ptr(a: np.ndarray):
     return C.c_void_p(a.ctypes.data)
+
+
+def march_stats_dict(m: "MarchStats") -> dict:
+    out = {k: getattr(m, k) for k, _ in MarchStats._fields_ if k not in ("pairs", "pad")}
+    out["pairs"] = {k: int(getattr(m.pairs, k)) for k, _ in PairDemand._fields_}
+    return out
+
+
+def march_stats_of(heat: np.ndarray, chunks: int = 0) -> dict:
+    """Renderer::march_stats' summary of a host heat array (vx_march_stats_compute)."""
+    a = np.ascontiguousarray(heat, dtype=np.uint32)
+    m = MarchStats()
+    check(march().vx_march_stats_compute(ptr(a) if a.size else None, a.size, chunks, C.byref(m)),
+          "vx_march_stats_compute")
+    return march_stats_dict(m)
+
+
+# vx_sec_hit: one secondary leg (t, voxel, info = ci | normal << 13 | traced << 30 | hit << 31, block id)
+SEC_DTYPE = np.dtype([("t", "<f4"), ("voxel", "<u4"), ("info", "<u4"), ("material", "<u4")])
+SEC_TRACED, SEC_HIT = 1 << 30, 1 << 31

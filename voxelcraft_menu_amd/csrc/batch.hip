@@ -47,6 +47,10 @@ struct vx_batch {
     std::vector<uint32_t> slot_of; // caller's view index -> slot
     uint32_t cb[N_CLASSES + 1] = {0, 0, 0, 0, 0};  // class boundaries of the tile range
     DevBuf views, tile_off, hdr, chunks, grids, vis, shadow, hiz, cost, order, counters;
+    // Secondary bounce: two 16-byte leg records per pixel, two leg queues of sec_cap entries.
+    bool sec_on = false;
+    DevBuf sec_out, sec_queue;
+    uint32_t sec_cap = 0;
     vx_batch_tuning tn{};
     // Graph of the current submission (captured lazily, dropped when it would go stale).
     hipGraph_t graph = nullptr;
@@ -102,6 +106,38 @@ BatchArgs batch_args(const vx_batch *b) {
     return a;
 }
 
+// The secondary bounce over the frame's keys: gather the legs into the two queues, then
+// trace each queue (leg counts are read on the device, so a replay follows the frame).
+int enqueue_secondary(vx_batch *b, hipStream_t st) {
+    BatchArgs a = batch_args(b);
+    SecArgs sa;
+    sa.out = (uint4 *)b->sec_out.p;
+    sa.queue = (unsigned long long *)b->sec_queue.p;
+    sa.cap = b->sec_cap;
+    k_sec_gather<<<std::max(b->n_tiles, 1u), 64, 0, st>>>(a, sa);
+    const unsigned nb = (unsigned)b->n_cu * 16u;
+    k_sec_trace<true><<<nb, 64, 0, st>>>(a, sa);
+    k_sec_trace<false><<<nb, 64, 0, st>>>(a, sa);
+    LAUNCHED();
+    return VX_OK;
+}
+
+// Sizes the secondary buffers for the current views; legs need flat water normals.
+int secondary_buffers(vx_batch *b) {
+    for (const ViewK &k : b->vk)
+        if (k.s.bits & SPEC_WAVY_WATER) return VX_E_UNSUPPORTED;
+    if (b->n_px * 2 > 0xFFFFFFFFull) return VX_E_OVERFLOW;
+    const uint32_t cap = (uint32_t)std::max<uint64_t>(b->n_px * 2, 64);
+    const void *o0 = b->sec_out.p, *q0 = b->sec_queue.p;
+    int rc;
+    if ((rc = ensure(b->sec_out, (size_t)std::max<uint64_t>(b->n_px, 1) * 2 * 16)) ||
+        (rc = ensure(b->sec_queue, (size_t)cap * 2 * 8)))
+        return rc;
+    if (o0 != b->sec_out.p || q0 != b->sec_queue.p || cap != b->sec_cap) drop_graph(b);
+    b->sec_cap = cap;
+    return VX_OK;
+}
+
 // The whole submission of one batch frame, on the context stream (also what a graph
 // captures). PLAIN reads every view's geometry from the host copies, so its graph is
 // tied to the view set; the other forms read counts from the device header.
@@ -139,6 +175,14 @@ int enqueue(vx_batch *b, hipStream_t st, hipEvent_t ev_march) {
                 k_shadow<<<((nt + 3) / 4 + 7u) & ~7u, 256, 0, st>>>(ta.p, k.f, k.s, ta.vis, ta.shadow, cnt);
         }
         LAUNCHED();
+        if (b->sec_on) {
+            // The gather walks packed (view, tile) entries; PLAIN's order slices hold
+            // per-view tile indices, so lay the packed order over them first.
+            k_iota_views<<<ORDER_BLOCKS, 256, 0, st>>>((uint32_t *)b->order.p, (const uint32_t *)b->hdr.p + BH_TILES,
+                                                       0, (const uint32_t *)b->tile_off.p,
+                                                       (const uint32_t *)b->hdr.p + BH_VIEWS);
+            return enqueue_secondary(b, st);
+        }
         return VX_OK;
     }
     const uint32_t *hdr = (const uint32_t *)b->hdr.p;
@@ -157,6 +201,7 @@ int enqueue(vx_batch *b, hipStream_t st, hipEvent_t ev_march) {
     launch_batch_classes(b->tn.mode, b->tn.occupancy, b->tn.variant, b->tn.queue_chunk, b->tn.waves_per_simd, b->n_cu,
                          b->cb, batch_args(b), st);
     LAUNCHED();
+    if (b->sec_on) return enqueue_secondary(b, st);
     return VX_OK;
 }
 
@@ -250,8 +295,8 @@ int vx_batch_destroy(vx_batch *b) {
     (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
     drop_graph(b);
-    DevBuf *bufs[] = {&b->views, &b->tile_off, &b->hdr,  &b->chunks, &b->grids,   &b->vis,
-                      &b->shadow, &b->hiz,     &b->cost, &b->order,  &b->counters};
+    DevBuf *bufs[] = {&b->views,  &b->tile_off, &b->hdr,   &b->chunks,  &b->grids,   &b->vis,      &b->shadow,
+                      &b->hiz,    &b->cost,     &b->order, &b->counters, &b->sec_out, &b->sec_queue};
     for (DevBuf *d : bufs) vxh::release(*d);
     for (auto &set : b->ring)
         for (auto &e : set)
@@ -356,6 +401,11 @@ int vx_batch_set_views(vx_batch *b, const vx_view *views, uint32_t n) {
     b->n_tiles = b->toff[n];
     b->n_px = b->poff[n];
     b->rendered = false;
+    if (b->sec_on && (rc = secondary_buffers(b))) {
+        b->sec_on = false;
+        drop_graph(b);
+        return rc;
+    }
     return VX_OK;
 }
 
@@ -534,6 +584,37 @@ int vx_batch_autotune(vx_batch *b, uint32_t reps, vx_batch_tuning *best, float *
     if ((rc = vx_batch_set_tuning(b, &best_c))) return rc;
     if (best) *best = best_c;
     if (best_ms) *best_ms = best_t;
+    return VX_OK;
+}
+
+int vx_batch_set_secondary(vx_batch *b, uint32_t on) {
+    if (!b || on > 1) return VX_E_ARG;
+    HIPX(hipSetDevice(b->ctx->device));
+    HIPX(hipStreamSynchronize(b->ctx->stream));
+    if (on && b->n_views) {
+        int rc = secondary_buffers(b);
+        if (rc) return rc;
+    }
+    if ((bool)on != b->sec_on) drop_graph(b);
+    b->sec_on = on != 0;
+    return VX_OK;
+}
+
+int vx_batch_read_secondary(vx_batch *b, uint32_t view, vx_sec_hit *out, size_t n) {
+    if (!b || !out || view >= b->n_views || !b->sec_on || !b->rendered) return VX_E_ARG;
+    const uint32_t v = b->slot_of[view];
+    if (n != 2 * (b->poff[v + 1] - b->poff[v])) return VX_E_ARG;
+    return batch_read(b, b->sec_out, b->poff[v] * 2 * 16, out, n * 16);
+}
+
+int vx_batch_secondary_count(vx_batch *b, uint64_t *water_legs, uint64_t *glass_legs, uint32_t *overflow) {
+    if (!b) return VX_E_ARG;
+    uint32_t w[CNT_TOTAL - CNT_SEC];
+    int rc = batch_read(b, b->counters, CNT_SEC * 4, w, sizeof w);
+    if (rc) return rc;
+    if (water_legs) *water_legs = w[SEC_WATER * SEC_LINE];
+    if (glass_legs) *glass_legs = w[SEC_GLASS * SEC_LINE];
+    if (overflow) *overflow = w[SEC_OVERFLOW * SEC_LINE];
     return VX_OK;
 }
 

@@ -67,6 +67,8 @@ struct vx_ctx {
     bool pool_synced[3] = {false, false, false};
     vxh::DevBuf chunks, grid, vis, shadow, hiz, pairs, counters;
     vxh::DevBuf cost, order;  // per-tile cycles of the last frame, and the launch order
+    vxh::DevBuf heat;         // FLAG_HEATMAP: marched iterations per pixel
+    bool heat_valid = false;  // the last frame was a heatmap frame
     bool have_cost = false;
     uint32_t n_chunks = 0;
     // Ring of per-frame event sets: [0] start, [1..5] pass boundaries, [6] end.

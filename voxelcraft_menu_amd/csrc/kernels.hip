@@ -83,7 +83,7 @@ __global__ void k_finalize(FrameK f, uint32_t *counters) {
 // march (march.wgsl:1-55): one wave64 per pair, persistent over the pair list.
 __global__ __launch_bounds__(256) void k_march_pairs(Pools p, FrameK f, Spec s, const uint32_t *__restrict__ pairs,
                                                      const uint32_t *__restrict__ counters,
-                                                     unsigned long long *__restrict__ vis) {
+                                                     unsigned long long *__restrict__ vis, uint32_t *__restrict__ heat) {
     uint32_t n = ld_counter(counters, 0);
     if (n > f.max_pairs) n = f.max_pairs;
     uint32_t waves = gridDim.x * (blockDim.x >> 6);
@@ -103,6 +103,8 @@ __global__ __launch_bounds__(256) void k_march_pairs(Pools p, FrameK f, Spec s, 
         Hit h = underwater ? march_chunk<true, false, false>(p, f, s, c, f.cam, rd, inv, f.far_)
                            : march_chunk<false, false, false>(p, f, s, c, f.cam, rd, inv, f.far_);
         if (h.hit) atomicMax(&vis[(size_t)py * f.res[0] + px], (unsigned long long)make_key(f.far_, h, ci));
+        // FLAG_HEATMAP frames: every marched pair adds its iterations (march.wgsl:51-53).
+        if (heat) atomicAdd(&heat[(size_t)py * f.res[0] + px], h.iters);
     }
 }
 

@@ -44,7 +44,11 @@ __host__ __device__ constexpr uint32_t view_class(bool hiz, bool underwater) {
 constexpr uint32_t CNT_CTL = CNT_LEN;
 constexpr uint32_t CTL_HIST = 0, CTL_CURSOR = ORDER_BINS, CTL_HEAD = 2 * ORDER_BINS, CTL_HEAD_STRIDE = 32,
                    CTL_LEN = 2 * ORDER_BINS + N_CLASSES * CTL_HEAD_STRIDE;
-constexpr uint32_t CNT_TOTAL = CNT_LEN + CTL_LEN;
+// Secondary-leg queues (two: water legs, glass legs) and their overflow flag, each word
+// on its own line, after the control block.
+constexpr uint32_t CNT_SEC = CNT_CTL + CTL_LEN, SEC_LINE = 32, SEC_WATER = 0, SEC_GLASS = 1, SEC_OVERFLOW = 2,
+                   SEC_LEN = 3 * SEC_LINE;
+constexpr uint32_t CNT_TOTAL = CNT_LEN + CTL_LEN + SEC_LEN;
 
 __device__ __forceinline__ void stat_add(uint32_t *counters, uint32_t slot, uint32_t k, uint32_t v) {
     if (v) atomicAdd(&counters[CNT_STAT_BASE + (slot & (CNT_STAT_SLOTS - 1)) * CNT_STAT_STRIDE + k], v);
@@ -56,7 +60,8 @@ __global__ void k_tile_select(Pools p, FrameK f, const uint32_t *__restrict__ hi
                               uint32_t *__restrict__ counters);
 __global__ void k_finalize(FrameK f, uint32_t *counters);
 __global__ void k_march_pairs(Pools p, FrameK f, Spec s, const uint32_t *__restrict__ pairs,
-                              const uint32_t *__restrict__ counters, unsigned long long *__restrict__ vis);
+                              const uint32_t *__restrict__ counters, unsigned long long *__restrict__ vis,
+                              uint32_t *__restrict__ heat);
 __global__ void k_build_hiz(FrameK f, const unsigned long long *__restrict__ vis, uint32_t *__restrict__ hiz);
 __global__ void k_recover_select(Pools p, FrameK f, const uint32_t *__restrict__ hiz, uint32_t *__restrict__ pairs,
                                  uint32_t *__restrict__ counters);
@@ -135,5 +140,19 @@ __global__ void k_shadow(Pools p, FrameK f, Spec s, const unsigned long long *__
                          uint16_t *__restrict__ out, uint32_t *__restrict__ counters);
 __global__ void k_decode(Pools p, FrameK f, const unsigned long long *__restrict__ vis,
                          const uint16_t *__restrict__ shadow, vx_hit *__restrict__ out);
+
+// Secondary bounce of a batch (one leg record per pixel and leg, 16 bytes):
+//   k_sec_gather: one wave per tile (queue entry of `order`), lane = pixel: writes empty
+//     records for the legs the pixel does not trace and appends the others to the water /
+//     glass queues (entries view << 32 | pixel << 1 | leg);
+//   k_sec_trace<kWater>: one leg per lane from its queue, grouped by view, trace_world.
+struct SecArgs {
+    uint4 *__restrict__ out;                  // 2 records per pixel of the batch
+    unsigned long long *__restrict__ queue;   // queue q at q * cap
+    uint32_t cap;                             // entries per queue
+};
+__global__ void k_sec_gather(BatchArgs a, SecArgs s);
+template <bool kWater>
+__global__ void k_sec_trace(BatchArgs a, SecArgs s);
 
 }  // namespace vxd

@@ -26,7 +26,9 @@ constexpr uint32_t NORMAL_CROSS_A = 6u, NORMAL_CROSS_B = 7u;
 
 // Pipeline specialisation bits (the WGSL override constants the march reads).
 constexpr uint32_t SPEC_CUTOUT = 1u, SPEC_GLASS = 2u, SPEC_FULL_MARCH = 4u, SPEC_EMITTER = 8u,
-                   SPEC_WATER_REPAIR = 16u;
+                   SPEC_WATER_REPAIR = 16u, SPEC_GLASS_REFLECT = 32u,
+                   // frames whose water normals are not the flat face normal (wave field, ripples)
+                   SPEC_WAVY_WATER = 64u;
 
 struct Spec {
     uint32_t bits;
@@ -798,6 +800,182 @@ __device__ __forceinline__ uint32_t classify(const FrameK &f, const TileCull &tc
     if (dmin >= tc.reach) return 0u;
     if (use_hiz && depth_key(f.far_, dmin) < tile_far) return 2u;
     return 1u;
+}
+
+// ---- Secondary legs (resolve.wgsl shade_water / shade_glass, flat water normals) -------
+constexpr float WATER_IOR = 1.333f, SURFACE_EPS = 0.02f;               // common.wgsl:1256-1258
+constexpr float WATER_REFRACT_NEAR = 96.0f, WATER_REFRACT_FAR = 192.0f;  // resolve.wgsl:873-886
+constexpr float WATER_REFLECT_NEAR = 512.0f, WATER_REFLECT_FAR = 1024.0f;
+constexpr float WATER_REFRACT_DIST = 96.0f, WATER_REFLECT_DIST = 768.0f, GLASS_TRANSMIT_DIST = 192.0f;
+constexpr uint32_t FLAG_WATER_REFLECT = 16u, FLAG_WATER_REFRACT = 32u;  // common.wgsl:146-147
+
+struct WorldHit {
+    bool hit;
+    float t;
+    uint32_t normal, voxel, ci;
+};
+
+__device__ __forceinline__ float wgsl_sign(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
+
+// trace_world (common.wgsl:1205-1253): chunk-grid DDA over at most 16 cells from ro,
+// marching each resident chunk with skip_glass; the first hit ends the walk. The chunk
+// march takes 1 / rd of the leg itself (march_chunk's own `let inv = 1.0 / rd`); the grid
+// walk steps with the tiny-component guard of trace_world.
+template <bool kSkipWater>
+__device__ inline WorldHit trace_world(const Pools &p, const FrameK &f, const Spec &s, const float ro[3],
+                                       const float rd[3], float max_dist) {
+    WorldHit out{false, max_dist, 0u, 0u, NO_CHUNK};
+    float inv[3], inv_rd[3];
+    bool pos_dir[3];
+    int32_t cell[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const bool tiny = fabsf(rd[k]) < 1e-6f;
+        const float dir = tiny ? 1e-6f * wgsl_sign(rd[k] + 1e-9f) : rd[k];
+        inv[k] = 1.0f / dir;
+        inv_rd[k] = 1.0f / rd[k];
+        pos_dir[k] = dir > 0.0f;
+        cell[k] = (int32_t)floorf(ro[k] / 64.0f);
+    }
+    for (uint32_t i = 0; i < 16u; i++) {
+        bool ok;
+        const uint32_t gi = grid_index(f, cell[0], cell[1], cell[2], ok);
+        if (!ok) return out;
+        const uint32_t ci = ld_grid(p, f, gi);
+        float tn[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const float bound = (float)(pos_dir[k] ? cell[k] + 1 : cell[k]) * 64.0f;
+            tn[k] = (bound - ro[k]) * inv[k];
+        }
+        float t_exit = tn[0];
+        uint32_t axis = 0;
+        if (tn[1] < t_exit) { t_exit = tn[1]; axis = 1; }
+        if (tn[2] < t_exit) { t_exit = tn[2]; axis = 2; }
+        if (ci != NO_CHUNK && ci < p.n_chunks) {
+            const ChunkK c = ld_chunk(p, ci);
+            const Hit h = march_chunk<kSkipWater, true, false>(p, f, s, c, ro, rd, inv_rd, fminf(t_exit, max_dist));
+            if (h.hit) {
+                out.hit = true;
+                out.t = h.t;
+                out.normal = h.normal;
+                out.voxel = h.voxel;
+                out.ci = ci;
+                return out;
+            }
+        }
+        if (t_exit >= max_dist) return out;
+        const int32_t step = pos_dir[axis] ? 1 : -1;
+        if (axis == 0) cell[0] += step;
+        else if (axis == 1) cell[1] += step;
+        else cell[2] += step;
+    }
+    return out;
+}
+
+// WGSL builtins with a fixed operation order: dot = (x*x' + y*y') + z*z'.
+__device__ __forceinline__ float dot3o(const float a[3], const float b[3]) {
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+// refract(I, N, eta): k = 1 - eta*eta*(1 - dot(N,I)^2); k < 0 ? 0 : eta*I - (eta*dot(N,I) + sqrt(k))*N
+__device__ __forceinline__ void wgsl_refract(const float i[3], const float n[3], float eta, float out[3]) {
+    const float d = dot3o(n, i);
+    const float k = 1.0f - eta * eta * (1.0f - d * d);
+    if (k < 0.0f) {
+        out[0] = out[1] = out[2] = 0.0f;
+        return;
+    }
+    const float sc = eta * d + sqrtf(k);
+#pragma unroll
+    for (int q = 0; q < 3; q++) out[q] = eta * i[q] - sc * n[q];
+}
+// reflect(I, N) = I - 2*dot(N,I)*N
+__device__ __forceinline__ void wgsl_reflect(const float i[3], const float n[3], float out[3]) {
+    const float d2 = 2.0f * dot3o(n, i);
+#pragma unroll
+    for (int q = 0; q < 3; q++) out[q] = i[q] - d2 * n[q];
+}
+// smoothstep(e0, e1, x) = t*t*(3 - 2t), t = clamp((x - e0) / (e1 - e0), 0, 1)
+__device__ __forceinline__ float wgsl_smoothstep(float e0, float e1, float x) {
+    const float t = fminf(fmaxf((x - e0) / (e1 - e0), 0.0f), 1.0f);
+    return t * t * (3.0f - 2.0f * t);
+}
+
+// One secondary leg to trace: origin, direction, range, and which leg of the pixel.
+struct SecLeg {
+    float ro[3], rd[3], max_dist;
+    bool water;  // water legs skip water voxels (trace_world(..., true))
+};
+
+// The legs of one pixel's primary key (shade_water with water_ctx's flat normal,
+// resolve.wgsl:1078-1167, :1209-1220; shade_glass, :1261-1310). Returns the number of
+// legs set: legs[0] = refract (water) / transmit (glass) when want[0], legs[1] = reflect.
+__device__ inline void secondary_legs(const Pools &p, const FrameK &f, const Spec &s, uint64_t key, uint32_t px,
+                                      uint32_t py, SecLeg legs[2], bool want[2]) {
+    want[0] = want[1] = false;
+    if (key == 0ull) return;
+    const uint32_t ci = (uint32_t)(key >> 24) & 0x1FFFu;
+    if (ci >= p.n_chunks) return;
+    const uint32_t low = (uint32_t)(key & 0xFFFFFFull);
+    const uint32_t vx = (low >> 16) & 63u, vy = (low >> 8) & 63u, vz = low & 63u;
+    const ChunkK c = ld_chunk(p, ci);
+    const uint32_t id = block_at(p, c, vx, vy, vz);
+    const bool glass = (s.bits & SPEC_GLASS) && id == GLASS_ID;
+    if (id != WATER_ID && !glass) return;
+    const uint32_t nid = (uint32_t)(key >> 37) & 7u;
+    const uint32_t mx = (low >> 22) & 3u, my = (low >> 14) & 3u, mz = (low >> 6) & 3u;
+    float rd[3], n[3], hit[3];
+    ray_dir(f, (float)px + 0.5f, (float)py + 0.5f, rd[0], rd[1], rd[2]);
+    const float t = hit_t(f, c, vx, vy, vz, mx, my, mz, nid, rd);
+#pragma unroll
+    for (int k = 0; k < 3; k++) hit[k] = f.cam[k] + rd[k] * t;
+    normal_of(nid, n);
+    if (id == WATER_ID) {
+        const float refract_detail = 1.0f - wgsl_smoothstep(WATER_REFRACT_NEAR, WATER_REFRACT_FAR, t);
+        const float reflect_detail = 1.0f - wgsl_smoothstep(WATER_REFLECT_NEAR, WATER_REFLECT_FAR, t);
+        if ((f.flags & FLAG_WATER_REFRACT) && refract_detail > 0.0f) {
+            SecLeg &l = legs[0];
+            wgsl_refract(rd, n, 1.0f / WATER_IOR, l.rd);
+#pragma unroll
+            for (int k = 0; k < 3; k++) l.ro[k] = hit[k] + l.rd[k] * SURFACE_EPS;
+            l.max_dist = WATER_REFRACT_DIST;
+            l.water = true;
+            want[0] = true;
+        }
+        if ((f.flags & FLAG_WATER_REFLECT) && reflect_detail > 0.0f) {
+            SecLeg &l = legs[1];
+#pragma unroll
+            for (int k = 0; k < 3; k++) l.ro[k] = hit[k] + n[k] * SURFACE_EPS;
+            wgsl_reflect(rd, n, l.rd);
+            l.max_dist = WATER_REFLECT_DIST;
+            l.water = true;
+            want[1] = true;
+        }
+    } else {
+        if (dot3o(n, rd) > 0.0f) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) n[k] = -n[k];
+        }
+        const float dist = fmaxf(GLASS_TRANSMIT_DIST, f.far_);
+        SecLeg &l = legs[0];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            l.ro[k] = hit[k] + rd[k] * SURFACE_EPS;
+            l.rd[k] = rd[k];
+        }
+        l.max_dist = dist;
+        l.water = false;
+        want[0] = true;
+        if (s.bits & SPEC_GLASS_REFLECT) {
+            SecLeg &m = legs[1];
+#pragma unroll
+            for (int k = 0; k < 3; k++) m.ro[k] = hit[k] + n[k] * SURFACE_EPS;
+            wgsl_reflect(rd, n, m.rd);
+            m.max_dist = dist;
+            m.water = false;
+            want[1] = true;
+        }
+    }
 }
 
 }  // namespace vxd

@@ -1,0 +1,126 @@
+// secondary.hip -- the secondary bounce of a batch: one leg per water/glass hit and leg
+// kind, gathered into compact queues and traced one leg per lane.
+//
+// Legs follow resolve.wgsl's shade_water (refract to 96, reflect to 768, with the flat
+// face normal of water_ctx) and shade_glass (transmit, and reflect under
+// SPEC_GLASS_REFLECT, to max(192, far)), each traced with trace_world
+// (common.wgsl:1205-1253). Only a minority of pixels hit water or glass, so the tile
+// waves of the gather pass append the legs they own to two queues (water legs skip water
+// voxels, glass legs do not: one trace kernel specialisation each) and the trace pass
+// runs full waves of legs.
+#include "kernels.hpp"
+
+namespace vxd {
+
+// Output record of a leg (vx_sec_hit): t, voxel, info = ci | normal << 13 | traced << 30
+// | hit << 31 (ci = 0x1FFF on a miss), block id at the hit.
+__device__ __forceinline__ uint4 leg_record(const Pools &p, const WorldHit &w) {
+    uint32_t material = 0u;
+    if (w.hit) {
+        const ChunkK c = ld_chunk(p, w.ci);
+        material = block_at(p, c, w.voxel & 63u, (w.voxel >> 6) & 63u, (w.voxel >> 12) & 63u);
+    }
+    const uint32_t info = (w.hit ? (w.ci & 0x1FFFu) : 0x1FFFu) | ((w.hit ? w.normal : 0u) << 13) | (1u << 30) |
+                          (w.hit ? (1u << 31) : 0u);
+    return make_uint4(__float_as_uint(w.t), w.hit ? w.voxel : 0u, info, material);
+}
+
+// Appends this lane's entry to queue q when `want` (one atomic per wave and queue).
+__device__ __forceinline__ void queue_push(const SecArgs &s, uint32_t *counters, uint32_t q, bool want,
+                                           unsigned long long entry) {
+    const uint64_t m = ballot(want);
+    if (!m) return;
+    const uint32_t lane = lane_id();
+    const uint32_t first = (uint32_t)__ffsll((long long)m) - 1u;
+    uint32_t base = 0;
+    if (lane == first) base = atomicAdd(&counters[CNT_SEC + q * SEC_LINE], (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, (int)first);
+    if (want) {
+        const uint32_t idx = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        if (idx < s.cap) s.queue[(size_t)q * s.cap + idx] = entry;
+        else counters[CNT_SEC + SEC_OVERFLOW * SEC_LINE] = 1u;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_sec_gather(BatchArgs a, SecArgs s) {
+    const uint32_t n = a.hdr[BH_TILES];
+    if (blockIdx.x >= n) return;
+    const uint32_t entry = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.order[blockIdx.x]);
+    const uint32_t v = entry >> ENTRY_VIEW_SHIFT, tile = entry & ENTRY_TILE_MASK;
+    if (v >= a.hdr[BH_VIEWS]) return;
+    const ViewK *vk = a.views + v;
+    const FrameK f = ld_const(&vk->f);
+    const Spec sp = ld_const(&vk->s);
+    if (tile >= f.tiles[0] * f.tiles[1]) return;
+    const_as<ViewK> *vc = as_const(vk);
+    Pools p = a.p;
+    p.chunks = a.p.chunks + vc->chunk_off;
+    p.grid = a.p.grid + vc->grid_off;
+    p.n_chunks = vc->n_chunks;
+    const size_t px0 = (size_t)vc->px_off_lo | ((size_t)vc->px_off_hi << 32);
+    const uint32_t lane = lane_id();
+    const uint32_t px = (tile % f.tiles[0]) * 8u + (lane & 7u), py = (tile / f.tiles[0]) * 8u + (lane >> 3);
+    const bool in_frame = px < f.res[0] && py < f.res[1];
+    const uint32_t pix = in_frame ? py * f.res[0] + px : 0u;
+    SecLeg legs[2];
+    bool want[2] = {false, false};
+    if (in_frame) secondary_legs(p, f, sp, a.vis[px0 + pix], px, py, legs, want);
+    if (in_frame) {
+        if (!want[0]) s.out[2 * (px0 + pix)] = make_uint4(0u, 0u, 0u, 0u);
+        if (!want[1]) s.out[2 * (px0 + pix) + 1] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    const unsigned long long e = ((unsigned long long)v << 32) | ((unsigned long long)pix << 1);
+    for (uint32_t leg = 0; leg < 2; leg++) {
+        const bool w = want[leg] && legs[leg].water, g = want[leg] && !legs[leg].water;
+        queue_push(s, a.counters, SEC_WATER, w, e | leg);
+        queue_push(s, a.counters, SEC_GLASS, g, e | leg);
+    }
+}
+
+// Legs of one queue, one per lane. The lanes of a wave are served view by view (the view
+// of the first unserved lane, made uniform), so frame and chunk data stay scalar.
+template <bool kWater>
+__global__ __launch_bounds__(64) void k_sec_trace(BatchArgs a, SecArgs s) {
+    const uint32_t q = kWater ? SEC_WATER : SEC_GLASS;
+    const uint32_t n = min(ld_counter(a.counters, CNT_SEC + q * SEC_LINE), s.cap);
+    const uint32_t nv = a.hdr[BH_VIEWS];
+    const uint32_t lane = lane_id();
+    for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
+        const uint32_t k = base + lane;
+        const bool mine = k < n;
+        const unsigned long long e = mine ? s.queue[(size_t)q * s.cap + k] : 0ull;
+        const uint32_t my_view = mine ? (uint32_t)(e >> 32) : 0xFFFFFFFFu;
+        bool todo = mine && my_view < nv;
+        while (ballot(todo) != 0ull) {
+            const uint32_t v = (uint32_t)__builtin_amdgcn_readfirstlane(
+                (int)__shfl((int)my_view, __ffsll((long long)ballot(todo)) - 1));
+            const bool here = todo && my_view == v;
+            const ViewK *vk = a.views + v;
+            const FrameK f = ld_const(&vk->f);
+            const Spec sp = ld_const(&vk->s);
+            const_as<ViewK> *vc = as_const(vk);
+            Pools p = a.p;
+            p.chunks = a.p.chunks + vc->chunk_off;
+            p.grid = a.p.grid + vc->grid_off;
+            p.n_chunks = vc->n_chunks;
+            const size_t px0 = (size_t)vc->px_off_lo | ((size_t)vc->px_off_hi << 32);
+            if (here) {
+                const uint32_t pix = (uint32_t)(e >> 1) & 0x7FFFFFFFu, leg = (uint32_t)e & 1u;
+                const uint32_t px = pix % f.res[0], py = pix / f.res[0];
+                SecLeg legs[2];
+                bool want[2];
+                secondary_legs(p, f, sp, a.vis[px0 + pix], px, py, legs, want);
+                const SecLeg &l = legs[leg];
+                WorldHit w{false, 0.0f, 0u, 0u, NO_CHUNK};
+                if (want[leg]) w = trace_world<kWater>(p, f, sp, l.ro, l.rd, l.max_dist);
+                s.out[2 * (px0 + pix) + leg] = leg_record(p, w);
+            }
+            todo = todo && !here;
+        }
+    }
+}
+
+template __global__ void k_sec_trace<true>(BatchArgs, SecArgs);
+template __global__ void k_sec_trace<false>(BatchArgs, SecArgs);
+
+}  // namespace vxd

@@ -230,8 +230,8 @@ int vx_ctx_destroy(vx_ctx *c) {
     if (c->n_batches > 0) return VX_E_BUSY;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    DevBuf *bufs[] = {&c->pool[0], &c->pool[1], &c->pool[2], &c->chunks, &c->grid, &c->vis,   &c->shadow,
-                      &c->hiz,     &c->pairs,   &c->counters, &c->cost,  &c->order};
+    DevBuf *bufs[] = {&c->pool[0], &c->pool[1], &c->pool[2], &c->chunks, &c->grid,  &c->vis,  &c->shadow,
+                      &c->hiz,     &c->pairs,   &c->counters, &c->cost,  &c->order, &c->heat};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto &set : c->ring)
@@ -393,6 +393,9 @@ int make_spec(uint32_t spec, uint32_t spec_hi, uint32_t frame_flags, Spec &s) {
     if (spec_hi & VX_FLAG_HI_FULL_MARCH) s.bits |= SPEC_FULL_MARCH;
     if (spec_hi & VX_FLAG_HI_EMITTER_WORLD) s.bits |= SPEC_EMITTER;
     if (!(spec_hi & VX_FLAG_HI_LEGACY_WATER)) s.bits |= SPEC_WATER_REPAIR;
+    if (spec_hi & VX_FLAG_HI_GLASS_REFLECT) s.bits |= SPEC_GLASS_REFLECT;
+    if (((spec & VX_FLAG_WAVES) && (frame_flags & VX_FLAG_WAVES)) || (spec_hi & VX_FLAG_HI_WATER_LOOK))
+        s.bits |= SPEC_WAVY_WATER;
     s.leaf_fill = (spec & VX_FLAG_LEAF_THIN) ? 0.62f : 0.72f;
     return VX_OK;
 }
@@ -419,11 +422,20 @@ int vx_render(vx_ctx *c, const vx_gpu_frame *g, uint32_t spec, uint32_t spec_hi)
     if (g->res[0] != c->w || g->res[1] != c->h) return VX_E_ARG;
     if (g->tiles[0] != c->tx || g->tiles[1] != c->ty) return VX_E_ARG;
     if (int rc0 = vxh::check_frame(g, c->n_chunks)) return rc0;
-    if (c->path == VX_PATH_PAIRS && g->max_pairs != c->max_pairs) return VX_E_ARG;
+    // A FLAG_HEATMAP frame counts the iterations of every (tile, chunk) pair the reference
+    // marches, so it runs the reference's pair decomposition whatever the context's path.
+    const bool heatmap = (g->flags & VX_FLAG_HEATMAP) != 0u;
+    const bool use_pairs = c->path == VX_PATH_PAIRS || heatmap;
+    if (use_pairs && g->max_pairs != c->max_pairs) return VX_E_ARG;
     Spec s;
     int rc = make_spec(spec, spec_hi, g->flags, s);
     if (rc) return rc;
     HIPX(hipSetDevice(c->device));
+    if (heatmap) {
+        if ((rc = ensure(c->pairs, (size_t)c->max_pairs * 2 * 4)) || (rc = ensure(c->heat, (size_t)c->w * c->h * 4)))
+            return rc;
+        if ((rc = fill_u32(c, c->heat.p, 0, (size_t)c->w * c->h))) return rc;
+    }
     FrameK f = make_frame(g);
     Pools p = make_pools(c);
     const bool hiz_on = (f.flags & VX_FLAG_HIZ) != 0u;
@@ -438,7 +450,8 @@ int vx_render(vx_ctx *c, const vx_gpu_frame *g, uint32_t spec, uint32_t spec_hi)
     if (c->ring_count < (uint32_t)vx_ctx::RING) c->ring_count++;
     HIPX(hipEventRecord(c->ev[0], st));
     if ((rc = fill_u32(c, counters, 0, CNT_TOTAL))) return rc;
-    if (c->path == VX_PATH_PAIRS) {
+    uint32_t *heat = heatmap ? (uint32_t *)c->heat.p : nullptr;
+    if (use_pairs) {
         // clear vis (+ hiz when Hi-Z is off), tile_select, finalize, march,
         // [build_hiz, recover_select, finalize_recover, march] (src/render/mod.rs:1716-1795)
         if ((rc = fill_u64(c, vis, 0, (size_t)c->w * c->h))) return rc;
@@ -450,7 +463,7 @@ int vx_render(vx_ctx *c, const vx_gpu_frame *g, uint32_t spec, uint32_t spec_hi)
         k_finalize<<<1, 1, 0, st>>>(f, counters);
         LAUNCHED();
         HIPX(hipEventRecord(c->ev[1], st));
-        k_march_pairs<<<2048, 256, 0, st>>>(p, f, s, pairs, counters, vis);
+        k_march_pairs<<<2048, 256, 0, st>>>(p, f, s, pairs, counters, vis, heat);
         LAUNCHED();
         HIPX(hipEventRecord(c->ev[2], st));
         if (hiz_on) {
@@ -462,7 +475,7 @@ int vx_render(vx_ctx *c, const vx_gpu_frame *g, uint32_t spec, uint32_t spec_hi)
             k_finalize_recover<<<1, 1, 0, st>>>(f, counters);
             LAUNCHED();
             HIPX(hipEventRecord(c->ev[4], st));
-            k_march_pairs<<<2048, 256, 0, st>>>(p, f, s, pairs, counters, vis);
+            k_march_pairs<<<2048, 256, 0, st>>>(p, f, s, pairs, counters, vis, heat);
             LAUNCHED();
         } else {
             HIPX(hipEventRecord(c->ev[3], st));
@@ -486,7 +499,7 @@ int vx_render(vx_ctx *c, const vx_gpu_frame *g, uint32_t spec, uint32_t spec_hi)
         HIPX(hipEventRecord(c->ev[4], st));
         HIPX(hipEventRecord(c->ev[5], st));
     }
-    if (c->path == VX_PATH_PAIRS || !(c->variant & VAR_FUSED_SHADOW)) {
+    if (use_pairs || !(c->variant & VAR_FUSED_SHADOW)) {
         k_shadow<<<pad8((ntiles + 3) / 4), 256, 0, st>>>(p, f, s, vis, (uint16_t *)c->shadow.p, counters);
         LAUNCHED();
     }
@@ -495,6 +508,7 @@ int vx_render(vx_ctx *c, const vx_gpu_frame *g, uint32_t spec, uint32_t spec_hi)
     c->frame = f;
     c->spec = s;
     c->have_frame = true;
+    c->heat_valid = heatmap;
     return VX_OK;
 }
 
@@ -527,7 +541,7 @@ int vx_read_hiz(vx_ctx *c, uint32_t *out, size_t n) {
 }
 
 int vx_read_pairs(vx_ctx *c, uint32_t *out, size_t n, uint32_t *counters16) {
-    if (!c || (!out && n) || c->path != VX_PATH_PAIRS || n > (size_t)c->max_pairs * 2) return VX_E_ARG;
+    if (!c || (!out && n) || !c->pairs.p || n > (size_t)c->max_pairs * 2) return VX_E_ARG;
     int rc = VX_OK;
     if (n && (rc = read_dev(c, c->pairs, out, n * 4))) return rc;
     if (counters16) rc = read_dev(c, c->counters, counters16, 16 * 4);
@@ -639,6 +653,54 @@ int vx_timings_sum(vx_ctx *c, vx_timings *t, uint32_t *frames) {
     }
     if (frames) *frames = c->ring_count;
     return VX_OK;
+}
+
+int vx_read_heat(vx_ctx *c, uint32_t *out, size_t n) {
+    if (!c || !out || n != (size_t)c->w * c->h || !c->heat_valid) return VX_E_ARG;
+    return read_dev(c, c->heat, out, n * 4);
+}
+
+// Renderer::march_stats (src/render/mod.rs:1977-2045), on a host copy of the counter.
+int vx_march_stats_compute(const uint32_t *heat, size_t n, uint32_t chunks, vx_march_stats *out) {
+    if (!out || (!heat && n)) return VX_E_ARG;
+    std::memset(out, 0, sizeof *out);
+    std::vector<uint32_t> hot(heat, heat + n);
+    uint64_t total = 0;
+    for (uint32_t i : hot) {
+        total += i;
+        out->worst = std::max(out->worst, i);
+        out->touched += i > 0u ? 1u : 0u;
+        out->capped += i > MAX_ITERS ? 1u : 0u;
+    }
+    std::sort(hot.begin(), hot.end(), [](uint32_t a, uint32_t b) { return a > b; });
+    auto share = [&](size_t k) -> double {
+        k = std::min(k, hot.size());
+        uint64_t s = 0;
+        for (size_t i = 0; i < k; i++) s += hot[i];
+        return total == 0 ? 0.0 : 100.0 * (double)s / (double)total;
+    };
+    auto pct = [&](double p) -> uint32_t {
+        if (hot.empty()) return 0u;
+        return hot[(size_t)(((100.0 - p) / 100.0) * (double)hot.size()) % hot.size()];
+    };
+    out->pixels = n;
+    out->iters = total;
+    out->top_1 = share(hot.size() / 100);
+    out->top_10 = share(hot.size() / 10);
+    out->p50 = pct(50.0);
+    out->p90 = pct(90.0);
+    out->p99 = pct(99.0);
+    out->chunks = chunks;
+    return VX_OK;
+}
+
+int vx_march_stats_get(vx_ctx *c, vx_march_stats *out) {
+    if (!c || !out || !c->heat_valid) return VX_E_ARG;
+    std::vector<uint32_t> heat((size_t)c->w * c->h);
+    int rc = read_dev(c, c->heat, heat.data(), heat.size() * 4);
+    if (rc) return rc;
+    if ((rc = vx_march_stats_compute(heat.data(), heat.size(), c->frame.chunk_count, out))) return rc;
+    return vx_pair_demand_get(c, &out->pairs);
 }
 
 int vx_device_outputs(vx_ctx *c, uint64_t **vis, uint16_t **shadow) {

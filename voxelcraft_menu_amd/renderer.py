@@ -125,6 +125,19 @@ class Renderer:
         A.check(self._L.vx_read_decoded(self._c, A.ptr(out), w * h), "vx_read_decoded")
         return out
 
+    def read_heat(self) -> np.ndarray:
+        """Marched iterations per pixel of the last FLAG_HEATMAP frame."""
+        w, h = self.size
+        out = np.zeros(w * h, dtype=np.uint32)
+        A.check(self._L.vx_read_heat(self._c, A.ptr(out), out.size), "vx_read_heat")
+        return out
+
+    def march_stats(self) -> dict:
+        """Renderer::march_stats (src/render/mod.rs:1977-2045) of the last heatmap frame."""
+        m = A.MarchStats()
+        A.check(self._L.vx_march_stats_get(self._c, C.byref(m)), "vx_march_stats_get")
+        return A.march_stats_dict(m)
+
     def pair_demand(self) -> dict:
         d = A.PairDemand()
         A.check(self._L.vx_pair_demand_get(self._c, C.byref(d)), "vx_pair_demand_get")
@@ -238,6 +251,23 @@ class Batch:
         out = np.zeros(((w + 7) // 8) * ((h + 7) // 8), dtype=np.uint32)
         A.check(self._L.vx_batch_read_hiz(self._b, i, A.ptr(out), out.size), "vx_batch_read_hiz")
         return out
+
+    def set_secondary(self, on: bool = True):
+        """Trace the secondary legs of water / glass hits with every frame."""
+        A.check(self._L.vx_batch_set_secondary(self._b, 1 if on else 0), "vx_batch_set_secondary")
+
+    def read_secondary(self, i: int) -> np.ndarray:
+        """(W*H, 2) leg records of view i: [:, 0] refract / transmit, [:, 1] reflect."""
+        w, h = self._size(i)
+        out = np.zeros((w * h, 2), dtype=A.SEC_DTYPE)
+        A.check(self._L.vx_batch_read_secondary(self._b, i, A.ptr(out), out.size), "vx_batch_read_secondary")
+        return out
+
+    def secondary_count(self) -> dict:
+        wl, gl, ov = C.c_uint64(), C.c_uint64(), C.c_uint32()
+        A.check(self._L.vx_batch_secondary_count(self._b, C.byref(wl), C.byref(gl), C.byref(ov)),
+                "vx_batch_secondary_count")
+        return {"water_legs": int(wl.value), "glass_legs": int(gl.value), "overflow": int(ov.value)}
 
     def demand(self) -> dict:
         d = A.PairDemand()
