@@ -10,7 +10,8 @@ batched form (the launch form picked by the autotuner, replayed from a hipGraph)
 are reported beside it as `plain`, with the ratio. Every view keeps its Hi-Z words
 between steps, so steps after the first run in the Hi-Z steady state.
 
-The same line carries `serving`: p50/p99 request latency and Grays/s of an open-loop
+The same line carries `incoherent` (config 3: 32 views at 1280x720 of the 2048x2048x512
+world with one secondary bounce per water / glass hit) and `serving`: p50/p99 request latency and Grays/s of an open-loop
 Poisson stream of view requests (1000/s by default, the four resolutions) mixed with
 edit batches of 1000 voxel ops (2/s) on the device-resident 2048x2048x512 world,
 through the continuous-batching server.
@@ -223,6 +224,8 @@ def run_batch(args, out):
 
     if not args.no_serving:
         out["serving"] = run_serving(args)
+    if not args.no_incoherent:
+        out["incoherent"] = run_incoherent(args)
 
     if args.no_cpu_baseline:
         out["cpu_baseline"] = None
@@ -262,6 +265,57 @@ def run_serving(args):
     r.close()
     res["workload"] = wl.name
     _log(f"serving: p50 {res['p50_ms']} ms p99 {res['p99_ms']} ms at {args.serve_qps} qps")
+    return res
+
+
+def run_incoherent(args):
+    """The incoherent batch: the 2048x2048x512 world (caves, sea level 96, 1 % glass), 32
+    seeded views at 1280x720, primary + sun shadow + one secondary bounce per water / glass
+    hit (water refract and reflect legs, glass transmit leg, each traced with trace_world
+    through compact leg queues), flat water normals (waves off). A step is one frame of
+    all 32 views; rays = primary + shadow + secondary legs."""
+    from voxelcraft_menu_amd import _abi as A
+    from voxelcraft_menu_amd import workloads as W
+    from voxelcraft_menu_amd.renderer import Batch, Renderer
+    from voxelcraft_menu_amd.world import build_view
+
+    wl = W.WORKLOADS[3]
+    world = W.make_world(wl)
+    flags = W.DEFAULT_FLAGS & ~A.FLAG_WAVES
+    cams = W.random_cameras(wl, wl.views, seed=3, resolutions=[(wl.width, wl.height)], flags=flags)
+    views = [build_view(world, c) for c in cams]
+    specs = [W.spec_words(c.flags, world) for c in cams]
+    px = sum(int(v.frame.res[0]) * int(v.frame.res[1]) for v in views)
+    r = Renderer(64, 64)
+    r.sync_world(world)
+    b = Batch(r, len(views))
+    b.set_views(views, specs)
+    best, _ = b.autotune(2)
+    b.set_secondary(False)
+    for _ in range(2):
+        b.render()
+    prim_dt, _, _ = _time_batch(b, args.steps)
+    b.set_secondary(True)
+    for _ in range(2):
+        b.render()
+    b.sync()
+    d = b.demand()
+    sec = b.secondary_count()
+    if sec["overflow"]:
+        _fail("secondary leg queue overflowed")
+    dt, _, _ = _time_batch(b, args.steps)
+    b.close()
+    r.close()
+    rays = px + d["shadow_rays"] + sec["water_legs"] + sec["glass_legs"]
+    res = {
+        "workload": wl.name, "views": len(views), "resolution": [wl.width, wl.height], "camera_seed": 3,
+        "value": round(rays * args.steps / dt / 1e9, 4), "unit": "Grays/s", "ms_per_step": round(dt / args.steps * 1e3, 4),
+        "ms_per_step_without_secondary": round(prim_dt / args.steps * 1e3, 4),
+        "primary_rays": px, "shadow_rays": d["shadow_rays"], "secondary_rays": sec["water_legs"] + sec["glass_legs"],
+        "water_legs": sec["water_legs"], "glass_legs": sec["glass_legs"], "tuning": best,
+        "water": "flat normals (waves off)",
+    }
+    _log(f"incoherent: {res['ms_per_step']} ms/step ({res['ms_per_step_without_secondary']} without the bounce)")
     return res
 
 
@@ -351,6 +405,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle work for cpu_baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving stream (workload 2 only)")
+    ap.add_argument("--no-incoherent", action="store_true", help="skip the incoherent batch (workload 2 only)")
     ap.add_argument("--serve-qps", type=float, default=1000.0, help="offered view requests per second")
     ap.add_argument("--serve-seconds", type=float, default=4.0, help="seconds of request arrivals")
     ap.add_argument("--serve-edit-rate", type=float, default=2.0, help="edit batches (1000 ops) per second")
