@@ -306,13 +306,14 @@ def run_incoherent(args):
     dt, _, _ = _time_batch(b, args.steps)
     b.close()
     r.close()
-    rays = px + d["shadow_rays"] + sec["water_legs"] + sec["glass_legs"]
+    rays = px + d["shadow_rays"] + sec["water_legs"] + sec["glass_legs"] + sec["sun_rays"]
     res = {
         "workload": wl.name, "views": len(views), "resolution": [wl.width, wl.height], "camera_seed": 3,
         "value": round(rays * args.steps / dt / 1e9, 4), "unit": "Grays/s", "ms_per_step": round(dt / args.steps * 1e3, 4),
         "ms_per_step_without_secondary": round(prim_dt / args.steps * 1e3, 4),
         "primary_rays": px, "shadow_rays": d["shadow_rays"], "secondary_rays": sec["water_legs"] + sec["glass_legs"],
-        "water_legs": sec["water_legs"], "glass_legs": sec["glass_legs"], "tuning": best,
+        "water_legs": sec["water_legs"], "glass_legs": sec["glass_legs"], "secondary_sun_rays": sec["sun_rays"],
+        "tuning": best,
         "water": "flat normals (waves off)",
     }
     _log(f"incoherent: {res['ms_per_step']} ms/step ({res['ms_per_step_without_secondary']} without the bounce)")

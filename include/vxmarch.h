@@ -74,10 +74,11 @@ enum {
 #define VX_FLAG_LEAF_THIN 524288u
 #define VX_FLAG_WATER_FAR 2097152u
 #define VX_FLAG_WATER_DARK 4194304u
-/* spec_hi bits (src/render/mod.rs:220-270). */
 #define VX_FLAG_WATER_REFLECT 16u
 #define VX_FLAG_WATER_REFRACT 32u
 #define VX_FLAG_WAVES 512u
+#define VX_FLAG_WATER_SHADOW_CUT 262144u
+/* spec_hi bits (src/render/mod.rs:220-270). */
 #define VX_FLAG_HI_FULL_MARCH 2u
 #define VX_FLAG_HI_GLASS_REFLECT 128u
 #define VX_FLAG_HI_WATER_LOOK 4096u
@@ -288,18 +289,29 @@ int vx_batch_device_outputs(vx_batch *b, uint32_t view, uint64_t **vis, uint16_t
  * glass into compact queues and traces them. Water normals are the flat face normal, so
  * views whose pipeline key enables waves (VX_FLAG_WAVES in spec and frame flags) or the
  * water-look ripples return VX_E_UNSUPPORTED. Two records per pixel: [0] refract (water)
- * or transmit (glass), [1] reflect. */
+ * or transmit (glass), [1] reflect. At each hit that shade_hit would shade, the sun test
+ * it makes runs too (resolve.wgsl:741-745: dot(n, sun) > 0, daylight > 0.01,
+ * VX_FLAG_SHADOWS, then shadow_ray with the sea-surface jump, common.wgsl:1098-1148, over
+ * 16 / 48 blocks for the refract leg with / without VX_FLAG_WATER_SHADOW_CUT, the frame's
+ * shadow_dist for the water reflect leg, 48 for glass legs). A glass leg that ends on
+ * water is shade_water's in the reference, a further bounce that is not traced here. */
 typedef struct vx_sec_hit {
     float t;            /* distance along the leg to its hit, or the leg's range */
     uint32_t voxel;     /* x | y << 6 | z << 12 in the hit chunk */
-    uint32_t info;      /* chunk index | normal << 13 | VX_SEC_TRACED | VX_SEC_HIT (chunk 0x1FFF: miss) */
+    uint32_t info;      /* chunk index | normal << 13 | VX_SEC_SUN_* | VX_SEC_TRACED | VX_SEC_HIT
+                           (chunk 0x1FFF: miss) */
     uint32_t material;  /* block id at the hit */
 } vx_sec_hit;
+#define VX_SEC_SUN_BLOCKED (1u << 28) /* the sun ray at the hit was blocked */
+#define VX_SEC_SUN_TESTED (1u << 29)  /* a sun ray ran at the hit */
 #define VX_SEC_TRACED (1u << 30)
 #define VX_SEC_HIT (1u << 31)
 int vx_batch_set_secondary(vx_batch *b, uint32_t on);
 int vx_batch_read_secondary(vx_batch *b, uint32_t view, vx_sec_hit *out, size_t n);
-int vx_batch_secondary_count(vx_batch *b, uint64_t *water_legs, uint64_t *glass_legs, uint32_t *overflow);
+/* Legs queued (water, glass) and sun rays run at their hits in the last frame; overflow
+ * is set when a queue ran out of room (never with the sizes the batch allocates). */
+int vx_batch_secondary_count(vx_batch *b, uint64_t *water_legs, uint64_t *glass_legs, uint64_t *sun_rays,
+                             uint32_t *overflow);
 
 /* ---- Serving: continuous batching of view requests --------------------------------
  * No counterpart in the reference (its request loop renders and waits one view at a

@@ -29,7 +29,7 @@ class Spec(C.Structure):
 
 _STAT_FIELDS = ["pairs_select", "pairs_deferred", "pairs_recover", "rays_primary", "rays_shadow",
                 "chunk_marches", "iters", "l1_loads", "leaf_loads", "attr_loads", "brick_words",
-                "grid_loads", "chunk_loads", "node_bytes", "capped", "oob", "rays_secondary"]
+                "grid_loads", "chunk_loads", "node_bytes", "capped", "oob", "rays_secondary", "rays_secondary_sun"]
 
 
 class Stats(C.Structure):

@@ -604,6 +604,24 @@ static float shadow_ray_t(ctx *c, const float ro[3], const float rd[3], float ma
     return -1.0f;
 }
 
+/* shadow_ray (common.wgsl:1098-1148): a ray that starts under the sea level and rises
+ * jumps to the sea surface first (SPEC_WATER_REPAIR), then walks the chunk grid as
+ * shadow_ray_t does with the origin and range it has now; any hit blocks. shadow_ray_t's
+ * hit distance is march_chunk's max(t_enter, 0) * voxel_size, never negative, so
+ * "shadow_ray_t >= 0" is the walk's hit flag. */
+static int shadow_ray(ctx *c, const float ro[3], const float rd[3], float max_dist) {
+    const vxo_frame *f = c->f;
+    float r_ro[3] = {ro[0], ro[1], ro[2]};
+    float r_dist = max_dist;
+    if (c->s->water_repair && rd[1] > 1e-4f && ro[1] < f->sea_level) {
+        float t_surf = (f->sea_level - ro[1]) / rd[1];
+        if (t_surf >= r_dist) return 0;
+        for (int k = 0; k < 3; k++) r_ro[k] = ro[k] + rd[k] * t_surf;
+        r_dist = r_dist - t_surf;
+    }
+    return shadow_ray_t(c, r_ro, rd, r_dist) >= 0.0f;
+}
+
 /* ---- secondary legs ---------------------------------------------------------------- */
 #define WATER_IOR 1.333f           /* common.wgsl:1256 */
 #define SURFACE_EPS 0.02f          /* common.wgsl:1258 */
@@ -616,6 +634,15 @@ static float shadow_ray_t(ctx *c, const float ro[3], const float rd[3], float ma
 #define GLASS_TRANSMIT_DIST 192.0f
 #define FLAG_WATER_REFLECT 16u     /* common.wgsl:146-147 */
 #define FLAG_WATER_REFRACT 32u
+#define FLAG_WATER_SHADOW_CUT 262144u  /* src/render/mod.rs:198 */
+/* Sun-ray ranges of shade_hit at a leg's hit: SPEC_WATER_SHADOW_DIST for the refract leg
+ * (src/render/mod.rs:325-327, :2427-2432), frame.shadow_dist for the water reflect leg,
+ * GLASS_SHADOW_DIST for the glass legs (resolve.wgsl:888). */
+#define WATER_SHADOW_DIST 16.0f
+#define WATER_SHADOW_DIST_PRE41 48.0f
+#define GLASS_SHADOW_DIST 48.0f
+#define SEC_SUN_TESTED (1u << 29)
+#define SEC_SUN_BLOCKED (1u << 28)
 
 typedef struct world_hit {
     int hit;
@@ -699,16 +726,40 @@ static float wgsl_smoothstep(float e0, float e1, float x) {
     return t * t * (3.0f - 2.0f * t);
 }
 
-static void store_leg(ctx *c, const world_hit *w, vxo_sec *o) {
+/* The sun test shade_hit makes at a leg's hit (shade_hit_compact, resolve.wgsl:587-595
+ * for n and vs, :741-745; shade_hit_legacy :507-514 is the same): n = normal_of(normal),
+ * turned toward the ray for cross quads; only when dot(n, sun) > 0, daylight > 0.01 and
+ * FLAG_SHADOWS does shadow_ray run, from hit + n * (0.02 * vs). A glass leg that ends on
+ * water is shaded by shade_water instead (resolve.wgsl:1284-1289, :1305-1309), whose own
+ * legs are not traced here: no sun test. */
+static uint32_t leg_sun(ctx *c, const float ro[3], const float rd[3], const world_hit *w, uint32_t material,
+                        int water_leg, float shadow_dist) {
+    const vxo_frame *f = c->f;
+    if (!w->hit || (!water_leg && material == WATER_ID)) return 0u;
+    float n[3], hit[3], o[3];
+    normal_of(w->normal, n);
+    if (w->normal >= NORMAL_CROSS_A && dot3o(n, rd) > 0.0f)
+        for (int k = 0; k < 3; k++) n[k] = -n[k];
+    float ndl = dot3o(n, f->sun_dir);
+    if (!(ndl > 0.0f && f->daylight > 0.01f) || (f->flags & FLAG_SHADOWS) == 0u) return 0u;
+    float vs = chunk_at(c, w->ci)->voxel_size;
+    for (int k = 0; k < 3; k++) hit[k] = ro[k] + rd[k] * w->t;
+    for (int k = 0; k < 3; k++) o[k] = hit[k] + n[k] * (0.02f * vs);
+    c->st.rays_secondary_sun++;
+    return SEC_SUN_TESTED | (shadow_ray(c, o, f->sun_dir, shadow_dist) ? SEC_SUN_BLOCKED : 0u);
+}
+
+static void store_leg(ctx *c, const float ro[3], const float rd[3], const world_hit *w, int water_leg,
+                      float shadow_dist, vxo_sec *o) {
     o->t = w->t;
     o->voxel = w->hit ? w->voxel : 0u;
-    o->info = (w->hit ? (w->ci & 0x1FFFu) : 0x1FFFu) | ((w->hit ? w->normal : 0u) << 13) | (1u << 30) |
-              (w->hit ? (1u << 31) : 0u);
     o->material = 0u;
     if (w->hit) {
         uint32_t v[3] = {w->voxel & 63u, (w->voxel >> 6) & 63u, (w->voxel >> 12) & 63u};
         o->material = block_at(c, w->ci, v);
     }
+    o->info = (w->hit ? (w->ci & 0x1FFFu) : 0x1FFFu) | ((w->hit ? w->normal : 0u) << 13) | (1u << 30) |
+              (w->hit ? (1u << 31) : 0u) | leg_sun(c, ro, rd, w, o->material, water_leg, shadow_dist);
     c->st.rays_secondary++;
 }
 
@@ -743,14 +794,15 @@ static void secondary_pixel(ctx *c, const uint64_t *vis, uint32_t px, uint32_t p
             wgsl_refract(rd, n, 1.0f / WATER_IOR, rdir);
             for (int k = 0; k < 3; k++) ro2[k] = hit[k] + rdir[k] * SURFACE_EPS;
             world_hit w = trace_world(c, ro2, rdir, WATER_REFRACT_DIST, 1);
-            store_leg(c, &w, &out[0]);
+            float sd = (f->flags & FLAG_WATER_SHADOW_CUT) ? WATER_SHADOW_DIST : WATER_SHADOW_DIST_PRE41;
+            store_leg(c, ro2, rdir, &w, 1, sd, &out[0]);
         }
         if ((f->flags & FLAG_WATER_REFLECT) != 0u && reflect_detail > 0.0f) {
             float m[3], ro3[3];
             for (int k = 0; k < 3; k++) ro3[k] = hit[k] + n[k] * SURFACE_EPS;
             wgsl_reflect(rd, n, m);
             world_hit w = trace_world(c, ro3, m, WATER_REFLECT_DIST, 1);
-            store_leg(c, &w, &out[1]);
+            store_leg(c, ro3, m, &w, 1, f->shadow_dist, &out[1]);
         }
     } else {
         if (dot3o(n, rd) > 0.0f)
@@ -759,13 +811,13 @@ static void secondary_pixel(ctx *c, const uint64_t *vis, uint32_t px, uint32_t p
         float ro2[3];
         for (int k = 0; k < 3; k++) ro2[k] = hit[k] + rd[k] * SURFACE_EPS;
         world_hit w = trace_world(c, ro2, rd, dist, 0);
-        store_leg(c, &w, &out[0]);
+        store_leg(c, ro2, rd, &w, 0, GLASS_SHADOW_DIST, &out[0]);
         if (c->s->glass_reflect) {
             float m[3], o[3];
             for (int k = 0; k < 3; k++) o[k] = hit[k] + n[k] * SURFACE_EPS;
             wgsl_reflect(rd, n, m);
             world_hit w2 = trace_world(c, o, m, dist, 0);
-            store_leg(c, &w2, &out[1]);
+            store_leg(c, o, m, &w2, 0, GLASS_SHADOW_DIST, &out[1]);
         }
     }
 }

@@ -84,6 +84,7 @@ typedef struct vxo_stats {
     uint64_t capped;           /* marches stopped by MAX_ITERS */
     uint64_t oob;              /* pool reads outside the supplied arrays */
     uint64_t rays_secondary;   /* secondary legs traced (trace_world calls) */
+    uint64_t rays_secondary_sun; /* shadow_ray calls at secondary hits */
 } vxo_stats;
 
 /* Single-function restatements exposed for pinning tests. */
@@ -144,7 +145,7 @@ int vxo_render_heat(const vxo_pools *p, const vxo_chunk *chunks, uint32_t n_chun
 typedef struct vxo_sec {
     float t;            /* hit distance along the leg, or its max_dist */
     uint32_t voxel;     /* x | y << 6 | z << 12 */
-    uint32_t info;
+    uint32_t info;      /* ci | normal << 13 | sun blocked << 28 | sun tested << 29 | traced << 30 | hit << 31 */
     uint32_t material;  /* block_at at the hit, 0 otherwise */
 } vxo_sec;
 

@@ -197,13 +197,15 @@ def _sec_world():
 @pytest.mark.parametrize("form", [dict(mode=A.BATCH_FLAT, graph=0), dict(mode=A.BATCH_PERSISTENT, graph=1),
                                   dict(mode=A.BATCH_PLAIN, graph=0)], ids=_form_id)
 def test_secondary_legs_match_oracle(form):
-    """Secondary legs (water refract/reflect, glass transmit/reflect) of every pixel equal
-    the oracle's records bit for bit: on the constructed water/glass chunk and on views of
-    the 1024x1024x256 world with flat water."""
+    """Secondary legs (water refract/reflect, glass transmit/reflect) of every pixel, with
+    the sun test at each hit, equal the oracle's records bit for bit: on the constructed
+    water/glass chunk (sea level at its water surface, both water sun-ray ranges) and on
+    views of the 1024x1024x256 world with flat water."""
     flags = A.FLAG_WATER_REFLECT | A.FLAG_WATER_REFRACT | A.FLAG_GLASS | A.FLAG_SHADOWS | A.FLAG_HIZ
     cases = []
     w1 = _sec_world()
-    cases.append((w1, [H.camera((30.5, 47.3, 6.5), 90.0, -25.0, 96, 64, flags)],
+    cases.append((w1, [H.camera((30.5, 47.3, 6.5), 90.0, -25.0, 96, 64, flags, sea=32.0),
+                       H.camera((30.5, 47.3, 6.5), 90.0, -25.0, 96, 64, flags | A.FLAG_WATER_SHADOW_CUT, sea=32.0)],
                   A.FLAG_HI_FULL_MARCH | A.FLAG_HI_GLASS_REFLECT))
     wl = W.WORKLOADS[1]
     w2 = W.make_world(wl)
@@ -221,7 +223,7 @@ def test_secondary_legs_match_oracle(form):
         b.set_secondary(True)
         b.render()
         b.sync()
-        total = 0
+        total = sun = 0
         for i, (v, (s, hi)) in enumerate(zip(views, specs)):
             sc = O.Scene(world.pool(0), world.pool(1), world.pool(2), v.chunks, v.grid)
             sp = O.spec_from(s, hi)
@@ -233,9 +235,11 @@ def test_secondary_legs_match_oracle(form):
             diff = got.view(np.uint32) != legs.view(np.uint32)
             assert not diff.any(), f"view {i}: {diff.any(axis=-1).sum()} leg records differ"
             total += st["rays_secondary"]
+            sun += st["rays_secondary_sun"]
         cnt = b.secondary_count()
         assert cnt["water_legs"] + cnt["glass_legs"] == total and not cnt["overflow"]
-        assert total > 100
+        assert cnt["sun_rays"] == sun
+        assert total > 100 and sun > 10
         b.close()
         r.close()
 

@@ -290,7 +290,8 @@ _sig_march = {
     "vx_march_stats_compute": (C.c_int, [P, C.c_size_t, C.c_uint32, C.POINTER(MarchStats)]),
     "vx_batch_set_secondary": (C.c_int, [P, C.c_uint32]),
     "vx_batch_read_secondary": (C.c_int, [P, C.c_uint32, P, C.c_size_t]),
-    "vx_batch_secondary_count": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
+    "vx_batch_secondary_count": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                           C.POINTER(C.c_uint32)]),
     "vx_serve_clock": (C.c_double, []),
     "vx_serve_create": (C.c_int, [P, C.POINTER(ServeDesc), C.POINTER(P)]),
     "vx_serve_destroy": (C.c_int, [P]),
@@ -373,4 +374,4 @@ def march_stats_of(heat: np.ndarray, chunks: int = 0) -> dict:
 
 # vx_sec_hit: one secondary leg (t, voxel, info = ci | normal << 13 | traced << 30 | hit << 31, block id)
 SEC_DTYPE = np.dtype([("t", "<f4"), ("voxel", "<u4"), ("info", "<u4"), ("material", "<u4")])
-SEC_TRACED, SEC_HIT = 1 << 30, 1 << 31
+SEC_SUN_BLOCKED, SEC_SUN_TESTED, SEC_TRACED, SEC_HIT = 1 << 28, 1 << 29, 1 << 30, 1 << 31

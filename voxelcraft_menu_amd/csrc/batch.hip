@@ -607,13 +607,15 @@ int vx_batch_read_secondary(vx_batch *b, uint32_t view, vx_sec_hit *out, size_t 
     return batch_read(b, b->sec_out, b->poff[v] * 2 * 16, out, n * 16);
 }
 
-int vx_batch_secondary_count(vx_batch *b, uint64_t *water_legs, uint64_t *glass_legs, uint32_t *overflow) {
+int vx_batch_secondary_count(vx_batch *b, uint64_t *water_legs, uint64_t *glass_legs, uint64_t *sun_rays,
+                             uint32_t *overflow) {
     if (!b) return VX_E_ARG;
     uint32_t w[CNT_TOTAL - CNT_SEC];
     int rc = batch_read(b, b->counters, CNT_SEC * 4, w, sizeof w);
     if (rc) return rc;
     if (water_legs) *water_legs = w[SEC_WATER * SEC_LINE];
     if (glass_legs) *glass_legs = w[SEC_GLASS * SEC_LINE];
+    if (sun_rays) *sun_rays = w[SEC_SUN * SEC_LINE];
     if (overflow) *overflow = w[SEC_OVERFLOW * SEC_LINE];
     return VX_OK;
 }

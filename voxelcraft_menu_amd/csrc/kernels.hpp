@@ -47,7 +47,7 @@ constexpr uint32_t CTL_HIST = 0, CTL_CURSOR = ORDER_BINS, CTL_HEAD = 2 * ORDER_B
 // Secondary-leg queues (two: water legs, glass legs) and their overflow flag, each word
 // on its own line, after the control block.
 constexpr uint32_t CNT_SEC = CNT_CTL + CTL_LEN, SEC_LINE = 32, SEC_WATER = 0, SEC_GLASS = 1, SEC_OVERFLOW = 2,
-                   SEC_LEN = 3 * SEC_LINE;
+                   SEC_SUN = 3, SEC_LEN = 4 * SEC_LINE;
 constexpr uint32_t CNT_TOTAL = CNT_LEN + CTL_LEN + SEC_LEN;
 
 __device__ __forceinline__ void stat_add(uint32_t *counters, uint32_t slot, uint32_t k, uint32_t v) {

@@ -644,6 +644,24 @@ __device__ inline float shadow_ray_t(const Pools &p, const FrameK &f, const Spec
     return -1.0f;
 }
 
+// shadow_ray (common.wgsl:1098-1148): a rising ray that starts under the sea level jumps to
+// the sea surface first (SPEC_WATER_REPAIR), then walks the chunk grid as shadow_ray_t
+// does from there; any hit blocks. shadow_ray_t's hit distance is march_chunk's
+// max(t_enter, 0) * voxel_size, never negative, so ">= 0" is the walk's hit flag.
+__device__ inline bool shadow_ray(const Pools &p, const FrameK &f, const Spec &s, const float ro[3],
+                                  const float rd[3], float max_dist) {
+    float r_ro[3] = {ro[0], ro[1], ro[2]};
+    float r_dist = max_dist;
+    if ((s.bits & SPEC_WATER_REPAIR) && rd[1] > 1e-4f && ro[1] < f.sea_level) {
+        const float t_surf = (f.sea_level - ro[1]) / rd[1];
+        if (t_surf >= r_dist) return false;
+#pragma unroll
+        for (int k = 0; k < 3; k++) r_ro[k] = ro[k] + rd[k] * t_surf;
+        r_dist = r_dist - t_surf;
+    }
+    return shadow_ray_t(p, f, s, r_ro, rd, r_dist) >= 0.0f;
+}
+
 // hit_t (common.wgsl:1300-1318)
 __device__ __forceinline__ float hit_t(const FrameK &f, const ChunkK &c, uint32_t vx, uint32_t vy, uint32_t vz,
                                        uint32_t mx, uint32_t my, uint32_t mz, uint32_t nid, const float rd[3]) {
@@ -808,6 +826,12 @@ constexpr float WATER_REFRACT_NEAR = 96.0f, WATER_REFRACT_FAR = 192.0f;  // reso
 constexpr float WATER_REFLECT_NEAR = 512.0f, WATER_REFLECT_FAR = 1024.0f;
 constexpr float WATER_REFRACT_DIST = 96.0f, WATER_REFLECT_DIST = 768.0f, GLASS_TRANSMIT_DIST = 192.0f;
 constexpr uint32_t FLAG_WATER_REFLECT = 16u, FLAG_WATER_REFRACT = 32u;  // common.wgsl:146-147
+constexpr uint32_t FLAG_WATER_SHADOW_CUT = 262144u;                     // src/render/mod.rs:198
+// Sun-ray range of shade_hit at a leg's hit: SPEC_WATER_SHADOW_DIST (src/render/mod.rs:
+// 325-327, :2427-2432) for the refract leg, frame.shadow_dist for the water reflect leg,
+// GLASS_SHADOW_DIST (resolve.wgsl:888) for the glass legs.
+constexpr float WATER_SHADOW_DIST = 16.0f, WATER_SHADOW_DIST_PRE41 = 48.0f, GLASS_SHADOW_DIST = 48.0f;
+constexpr uint32_t SEC_SUN_TESTED = 1u << 29, SEC_SUN_BLOCKED = 1u << 28;
 
 struct WorldHit {
     bool hit;
@@ -904,8 +928,33 @@ __device__ __forceinline__ float wgsl_smoothstep(float e0, float e1, float x) {
 // One secondary leg to trace: origin, direction, range, and which leg of the pixel.
 struct SecLeg {
     float ro[3], rd[3], max_dist;
-    bool water;  // water legs skip water voxels (trace_world(..., true))
+    float sun_dist;  // range of the sun ray at the leg's hit
+    bool water;      // water legs skip water voxels (trace_world(..., true))
 };
+
+// The sun test shade_hit makes at a leg's hit (shade_hit_compact, resolve.wgsl:587-595 for
+// n and vs, :741-745; shade_hit_legacy :507-514): n = normal_of(normal), turned toward the
+// ray for cross quads; only when dot(n, sun) > 0, daylight > 0.01 and FLAG_SHADOWS does
+// shadow_ray run, from hit + n * (0.02 * vs). A glass leg that ends on water is shaded by
+// shade_water (resolve.wgsl:1284-1289, :1305-1309), whose own legs are not traced: no test.
+__device__ inline uint32_t leg_sun(const Pools &p, const FrameK &f, const Spec &s, const SecLeg &l,
+                                   const WorldHit &w, const ChunkK &c, uint32_t material) {
+    if (!w.hit || (!l.water && material == WATER_ID)) return 0u;
+    float n[3], o[3];
+    normal_of(w.normal, n);
+    if (w.normal >= NORMAL_CROSS_A && dot3o(n, l.rd) > 0.0f) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) n[k] = -n[k];
+    }
+    const float ndl = dot3o(n, f.sun);
+    if (!(ndl > 0.0f && f.daylight > 0.01f) || !(f.flags & VX_FLAG_SHADOWS)) return 0u;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const float hit = l.ro[k] + l.rd[k] * w.t;
+        o[k] = hit + n[k] * (0.02f * c.vs);
+    }
+    return SEC_SUN_TESTED | (shadow_ray(p, f, s, o, f.sun, l.sun_dist) ? SEC_SUN_BLOCKED : 0u);
+}
 
 // The legs of one pixel's primary key (shade_water with water_ctx's flat normal,
 // resolve.wgsl:1078-1167, :1209-1220; shade_glass, :1261-1310). Returns the number of
@@ -939,6 +988,7 @@ __device__ inline void secondary_legs(const Pools &p, const FrameK &f, const Spe
 #pragma unroll
             for (int k = 0; k < 3; k++) l.ro[k] = hit[k] + l.rd[k] * SURFACE_EPS;
             l.max_dist = WATER_REFRACT_DIST;
+            l.sun_dist = (f.flags & FLAG_WATER_SHADOW_CUT) ? WATER_SHADOW_DIST : WATER_SHADOW_DIST_PRE41;
             l.water = true;
             want[0] = true;
         }
@@ -948,6 +998,7 @@ __device__ inline void secondary_legs(const Pools &p, const FrameK &f, const Spe
             for (int k = 0; k < 3; k++) l.ro[k] = hit[k] + n[k] * SURFACE_EPS;
             wgsl_reflect(rd, n, l.rd);
             l.max_dist = WATER_REFLECT_DIST;
+            l.sun_dist = f.shadow_dist;
             l.water = true;
             want[1] = true;
         }
@@ -964,6 +1015,7 @@ __device__ inline void secondary_legs(const Pools &p, const FrameK &f, const Spe
             l.rd[k] = rd[k];
         }
         l.max_dist = dist;
+        l.sun_dist = GLASS_SHADOW_DIST;
         l.water = false;
         want[0] = true;
         if (s.bits & SPEC_GLASS_REFLECT) {
@@ -972,6 +1024,7 @@ __device__ inline void secondary_legs(const Pools &p, const FrameK &f, const Spe
             for (int k = 0; k < 3; k++) m.ro[k] = hit[k] + n[k] * SURFACE_EPS;
             wgsl_reflect(rd, n, m.rd);
             m.max_dist = dist;
+            m.sun_dist = GLASS_SHADOW_DIST;
             m.water = false;
             want[1] = true;
         }

@@ -12,16 +12,19 @@
 
 namespace vxd {
 
-// Output record of a leg (vx_sec_hit): t, voxel, info = ci | normal << 13 | traced << 30
-// | hit << 31 (ci = 0x1FFF on a miss), block id at the hit.
-__device__ __forceinline__ uint4 leg_record(const Pools &p, const WorldHit &w) {
-    uint32_t material = 0u;
+// Output record of a leg (vx_sec_hit): t, voxel, info = ci | normal << 13 | sun blocked
+// << 28 | sun tested << 29 | traced << 30 | hit << 31 (ci = 0x1FFF on a miss), block id
+// at the hit. The sun test at the hit (leg_sun) runs here, in the lane that traced it.
+__device__ __forceinline__ uint4 leg_record(const Pools &p, const FrameK &f, const Spec &s, const SecLeg &l,
+                                            const WorldHit &w) {
+    uint32_t material = 0u, sun = 0u;
     if (w.hit) {
         const ChunkK c = ld_chunk(p, w.ci);
         material = block_at(p, c, w.voxel & 63u, (w.voxel >> 6) & 63u, (w.voxel >> 12) & 63u);
+        sun = leg_sun(p, f, s, l, w, c, material);
     }
     const uint32_t info = (w.hit ? (w.ci & 0x1FFFu) : 0x1FFFu) | ((w.hit ? w.normal : 0u) << 13) | (1u << 30) |
-                          (w.hit ? (1u << 31) : 0u);
+                          (w.hit ? (1u << 31) : 0u) | sun;
     return make_uint4(__float_as_uint(w.t), w.hit ? w.voxel : 0u, info, material);
 }
 
@@ -78,13 +81,15 @@ __global__ __launch_bounds__(64) void k_sec_gather(BatchArgs a, SecArgs s) {
 }
 
 // Legs of one queue, one per lane. The lanes of a wave are served view by view (the view
-// of the first unserved lane, made uniform), so frame and chunk data stay scalar.
+// of the first unserved lane, made uniform), so frame and chunk data stay scalar. The sun
+// rays run at the hits are counted per wave and added once at the end.
 template <bool kWater>
 __global__ __launch_bounds__(64) void k_sec_trace(BatchArgs a, SecArgs s) {
     const uint32_t q = kWater ? SEC_WATER : SEC_GLASS;
     const uint32_t n = min(ld_counter(a.counters, CNT_SEC + q * SEC_LINE), s.cap);
     const uint32_t nv = a.hdr[BH_VIEWS];
     const uint32_t lane = lane_id();
+    uint32_t sun_rays = 0;  // wave-uniform
     for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
         const uint32_t k = base + lane;
         const bool mine = k < n;
@@ -104,6 +109,7 @@ __global__ __launch_bounds__(64) void k_sec_trace(BatchArgs a, SecArgs s) {
             p.grid = a.p.grid + vc->grid_off;
             p.n_chunks = vc->n_chunks;
             const size_t px0 = (size_t)vc->px_off_lo | ((size_t)vc->px_off_hi << 32);
+            uint32_t info = 0;
             if (here) {
                 const uint32_t pix = (uint32_t)(e >> 1) & 0x7FFFFFFFu, leg = (uint32_t)e & 1u;
                 const uint32_t px = pix % f.res[0], py = pix / f.res[0];
@@ -113,11 +119,15 @@ __global__ __launch_bounds__(64) void k_sec_trace(BatchArgs a, SecArgs s) {
                 const SecLeg &l = legs[leg];
                 WorldHit w{false, 0.0f, 0u, 0u, NO_CHUNK};
                 if (want[leg]) w = trace_world<kWater>(p, f, sp, l.ro, l.rd, l.max_dist);
-                s.out[2 * (px0 + pix) + leg] = leg_record(p, w);
+                const uint4 rec = leg_record(p, f, sp, l, w);
+                s.out[2 * (px0 + pix) + leg] = rec;
+                info = rec.z;
             }
+            sun_rays += (uint32_t)__popcll(ballot((info & SEC_SUN_TESTED) != 0u));
             todo = todo && !here;
         }
     }
+    if (lane == 0u && sun_rays) atomicAdd(&a.counters[CNT_SEC + SEC_SUN * SEC_LINE], sun_rays);
 }
 
 template __global__ void k_sec_trace<true>(BatchArgs, SecArgs);

@@ -393,6 +393,7 @@ __global__ __launch_bounds__(256) void k_cost_scatter(const uint32_t *__restrict
                                                       const uint32_t *__restrict__ tile_off,
                                                       const uint32_t *__restrict__ n_views_dev,
                                                       uint32_t *__restrict__ order) {
+    static_assert(ORDER_BINS == 128, "the bin scan below gives each lane of one wave two bins");
     __shared__ uint32_t cnt[ORDER_BINS];
     __shared__ uint32_t base[ORDER_BINS];
     __shared__ uint32_t toff[BATCH_MAX_VIEWS + 1];
@@ -403,15 +404,25 @@ __global__ __launch_bounds__(256) void k_cost_scatter(const uint32_t *__restrict
     for (uint32_t k = t; k < ORDER_BINS; k += blockDim.x) cnt[k] = 0;
     if (nv)
         for (uint32_t v = t; v <= nv; v += blockDim.x) toff[v] = tile_off[v];
+    // Bin starts: exclusive prefix sum of the grid-wide histogram, by the first wave (two
+    // bins per lane, then a shuffle scan over the lanes' pair sums).
+    if (t < 64u) {
+        const uint32_t h0 = ctl[CTL_HIST + 2u * t], h1 = ctl[CTL_HIST + 2u * t + 1u];
+        uint32_t s = h0 + h1;
+        for (uint32_t d = 1; d < 64u; d <<= 1) {
+            const uint32_t o = (uint32_t)__shfl_up((int)s, d);
+            if (t >= d) s += o;
+        }
+        base[2u * t] = s - h0 - h1;
+        base[2u * t + 1u] = s - h1;
+    }
     __syncthreads();
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + t; i < n; i += stride) atomicAdd(&cnt[bounds.bin(i, cost[i])], 1u);
     __syncthreads();
     for (uint32_t k = t; k < ORDER_BINS; k += blockDim.x) {
-        uint32_t start = 0;
-        for (uint32_t q = 0; q < k; q++) start += ctl[CTL_HIST + q];
         const uint32_t c = cnt[k];
-        base[k] = c ? start + atomicAdd(&ctl[CTL_CURSOR + k], c) : 0u;
+        base[k] = c ? base[k] + atomicAdd(&ctl[CTL_CURSOR + k], c) : 0u;
         cnt[k] = 0;
     }
     __syncthreads();

@@ -264,10 +264,11 @@ class Batch:
         return out
 
     def secondary_count(self) -> dict:
-        wl, gl, ov = C.c_uint64(), C.c_uint64(), C.c_uint32()
-        A.check(self._L.vx_batch_secondary_count(self._b, C.byref(wl), C.byref(gl), C.byref(ov)),
+        wl, gl, sr, ov = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_uint32()
+        A.check(self._L.vx_batch_secondary_count(self._b, C.byref(wl), C.byref(gl), C.byref(sr), C.byref(ov)),
                 "vx_batch_secondary_count")
-        return {"water_legs": int(wl.value), "glass_legs": int(gl.value), "overflow": int(ov.value)}
+        return {"water_legs": int(wl.value), "glass_legs": int(gl.value), "sun_rays": int(sr.value),
+                "overflow": int(ov.value)}
 
     def demand(self) -> dict:
         d = A.PairDemand()
