@@ -183,6 +183,8 @@ int vx_set_tuning(vx_ctx *ctx, const vx_tuning *t);
 int vx_get_tuning(vx_ctx *ctx, vx_tuning *t);
 int vx_ctx_destroy(vx_ctx *ctx);
 int vx_resize(vx_ctx *ctx, uint32_t width, uint32_t height);
+/* VX_E_OVERFLOW for a pool above 4 GiB - 16 bytes (kernels address pools with 32-bit
+ * byte offsets). */
 int vx_pool_sync(vx_ctx *ctx, int pool, const void *data, size_t bytes, const vx_range *dirty, size_t n_dirty);
 int vx_set_chunks(vx_ctx *ctx, const vx_gpu_chunk *chunks, uint32_t n_total, const uint32_t *grid);
 /* spec = frame flags & SPEC_MASK, spec_hi = spec_hi word (the pipeline key of

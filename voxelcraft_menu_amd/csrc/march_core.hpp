@@ -56,14 +56,28 @@ struct FrameK {
     uint32_t grid_dim[3];
 };
 
+// Pool reads go through buffer resources whose size is the pool's length: an index past
+// the end reads zeros (the restatement's reading of robust buffer access) by the
+// hardware's range check instead of a compare and branch per fetch. Pools stay below
+// 4 GiB - 16 bytes (vx_pool_sync refuses more), and the index is clamped first so that
+// index * element size cannot wrap back into range.
+constexpr uint32_t BUF_DWORD3 = 0x00020000u;  // gfx9 buffer descriptor word 3: 32-bit data format
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t pool_rsrc(const void *base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, (int)BUF_DWORD3);
+}
 __device__ __forceinline__ uint4 ld_inner(const Pools &p, uint32_t i) {
-    return i < p.n_inners ? p.inners[i] : make_uint4(0, 0, 0, 0);
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(pool_rsrc(p.inners, p.n_inners * 16u),
+                                                         min(i, 0x0FFFFFFFu) << 4, 0, 0);
+    return make_uint4(v[0], v[1], v[2], v[3]);
 }
 __device__ __forceinline__ uint2 ld_leaf(const Pools &p, uint32_t i) {
-    return i < p.n_leaves ? p.leaves[i] : make_uint2(0, 0);
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(pool_rsrc(p.leaves, p.n_leaves * 8u),
+                                                        min(i, 0x1FFFFFFFu) << 3, 0, 0);
+    return make_uint2(v[0], v[1]);
 }
 __device__ __forceinline__ uint32_t ld_brick(const Pools &p, uint32_t i) {
-    return i < p.n_bricks ? p.bricks[i] : 0u;
+    return __builtin_amdgcn_raw_buffer_load_b32(pool_rsrc(p.bricks, p.n_bricks * 4u), min(i, 0x3FFFFFFFu) << 2,
+                                                0, 0);
 }
 
 // The chunk fields the march reads from a GpuChunk record (96 B).
@@ -363,6 +377,11 @@ __device__ __forceinline__ Hit march_chunk_vs(const Pools &p, const FrameK &f, c
     else if (tmin3[1] >= tmin3[2]) axis = 1;
     else axis = 2;
     const bool pd[3] = {rd[0] > 0.0f, rd[1] > 0.0f, rd[2] > 0.0f};
+    // All ones along + axes. Kept as per-lane integers in VGPRs (the empty asm hides that
+    // they are booleans): as lane masks they compete for the scalar registers.
+    int32_t pdm[3] = {pd[0] ? -1 : 0, pd[1] ? -1 : 0, pd[2] ? -1 : 0};
+#pragma unroll
+    for (int i = 0; i < 3; i++) asm volatile("" : "+v"(pdm[i]));
     int32_t cell[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) cell[i] = (int32_t)floorf(ro[i] + rd[i] * t);
@@ -548,25 +567,31 @@ __device__ __forceinline__ Hit march_chunk_vs(const Pools &p, const FrameK &f, c
             h.iters = iters;
             return h;
         }
-        int32_t bmin_i[3], bmax_i[3];
+        // Step to the exit face of the skip box (common.wgsl:888-907), written with
+        // selects only: box = cell - (cell & (skip - 1)) = cell & -skip; the exit bound
+        // is box + skip along + axes and box along - axes; the first axis wins ties
+        // (x <= y <= z); the new cell is floor(ro + rd * t) with the stepped axis forced
+        // to the next box, box + skip or box - 1.
+        int32_t box[3];
         float tn[3];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            bmin_i[i] = cell[i] - (cell[i] & (skip - 1));
-            bmax_i[i] = bmin_i[i] + skip;
-            float bound = pd[i] ? (float)bmax_i[i] : (float)bmin_i[i];
-            tn[i] = (bound - ro[i]) * inv[i];
+            box[i] = cell[i] & -skip;
+            tn[i] = ((float)(box[i] + (skip & pdm[i])) - ro[i]) * inv[i];
         }
-        if (tn[0] <= tn[1] && tn[0] <= tn[2]) { t = tn[0]; axis = 0; }
-        else if (tn[1] <= tn[2]) { t = tn[1]; axis = 1; }
-        else { t = tn[2]; axis = 2; }
+        const bool s0 = (tn[0] <= tn[1]) & (tn[0] <= tn[2]);
+        const bool s1 = !s0 & (tn[1] <= tn[2]);
+        const bool s2 = !s0 & !s1;
+        t = s0 ? tn[0] : (s1 ? tn[1] : tn[2]);
         if (t >= t_end) break;
+        axis = s0 ? 0u : (s1 ? 1u : 2u);
+        const bool sel[3] = {s0, s1, s2};
 #pragma unroll
-        for (int i = 0; i < 3; i++) cell[i] = (int32_t)floorf(ro[i] + rd[i] * t);
-        int32_t forced = pd[axis] ? bmax_i[axis] : bmin_i[axis] - 1;
-        if (axis == 0) cell[0] = forced;
-        else if (axis == 1) cell[1] = forced;
-        else cell[2] = forced;
+        for (int i = 0; i < 3; i++) {
+            const int32_t next = box[i] + ((skip + 1) & pdm[i]) - 1;
+            const int32_t fl = (int32_t)floorf(ro[i] + rd[i] * t);
+            cell[i] = sel[i] ? next : fl;
+        }
     }
     h.iters = iters;
     return h;

@@ -255,6 +255,7 @@ int vx_pool_sync(vx_ctx *c, int pool, const void *data, size_t bytes, const vx_r
     if (!c || pool < 0 || pool > 2 || (!data && bytes) || (!dirty && n_dirty)) return VX_E_ARG;
     static const size_t elem[3] = {16, 8, 4};
     if (bytes % elem[pool]) return VX_E_ARG;
+    if (bytes > 0xFFFFFFF0ull) return VX_E_OVERFLOW;  // pools are read through 32-bit buffer ranges
     HIPX(hipSetDevice(c->device));
     DevBuf &b = c->pool[pool];
     const void *p0 = b.p;
