@@ -229,7 +229,7 @@ __device__ __forceinline__ void batch_tile(const BatchArgs &a, uint32_t entry, u
     // scalar loads, and the frame fields live in SGPRs (the march indexes no frame array
     // with a run-time index, so the copy never needs addressable per-lane memory).
     const ViewK *vk = a.views + v;
-    const FrameK f = ld_const(&vk->f);
+    const FrameK &f = *(const FrameK *)as_const(&vk->f);
     const Spec s = ld_const(&vk->s);
     if (tile >= f.tiles[0] * f.tiles[1]) return;
     const_as<ViewK> *vc = as_const(vk);
@@ -248,8 +248,16 @@ __device__ __forceinline__ void batch_tile(const BatchArgs &a, uint32_t entry, u
     }
 }
 
+// The launch arguments are read in place in the kernel-argument segment (the by-value
+// parameter is the segment's first field and is not read directly), so each field is
+// loaded where it is used instead of being held in scalar registers from the start.
+__device__ __forceinline__ const BatchArgs &kernel_batch_args() {
+    return *(const BatchArgs *)__builtin_amdgcn_kernarg_segment_ptr();
+}
+
 template <int kOcc, int kVar, bool kHiz, bool kUnderwater>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kOcc))) void k_batch_flat(BatchArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kOcc))) void k_batch_flat(BatchArgs) {
+    const BatchArgs &a = kernel_batch_args();
     __shared__ uint64_t defmask[kHiz ? 128 : 1];
     __shared__ uint64_t selmask[(kVar & VAR_SPLIT) ? 128 : 1];
     constexpr uint32_t cls = view_class(kHiz, kUnderwater);
@@ -265,8 +273,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kOcc))) void
 // The range is sorted most expensive first, so small chunks keep the heavy head of the
 // queue spread over many waves.
 template <int kOcc, int kVar, bool kHiz, bool kUnderwater>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kOcc))) void k_batch_persistent(BatchArgs a,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kOcc))) void k_batch_persistent(BatchArgs,
                                                                                                    uint32_t chunk) {
+    const BatchArgs &a = kernel_batch_args();
     __shared__ uint64_t defmask[kHiz ? 128 : 1];
     __shared__ uint64_t selmask[(kVar & VAR_SPLIT) ? 128 : 1];
     constexpr uint32_t cls = view_class(kHiz, kUnderwater);
