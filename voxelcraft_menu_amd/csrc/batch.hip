@@ -51,6 +51,7 @@ struct vx_batch {
     bool sec_on = false;
     DevBuf sec_out, sec_queue;
     uint32_t sec_cap = 0;
+    uint64_t vk_pool_gen = 0;  // pool generation the device descriptors' pool fields were written for
     vx_batch_tuning tn{};
     // Graph of the current submission (captured lazily, dropped when it would go stale).
     hipGraph_t graph = nullptr;
@@ -87,6 +88,24 @@ void drop_graph(vx_batch *b) {
     b->exec = nullptr;
     b->graph = nullptr;
     b->graph_valid = false;
+}
+
+// Pool fields of a view descriptor: the context's pools with the view's chunk list and grid.
+void view_pools(const vx_batch *b, ViewK &k) {
+    k.p = vxh::make_pools(b->ctx);
+    k.p.chunks = (const vx_gpu_chunk *)b->chunks.p + k.chunk_off;
+    k.p.grid = (const uint32_t *)b->grids.p + k.grid_off;
+    k.p.n_chunks = k.n_chunks;
+}
+
+// Rewrites the descriptors on the device when a pool moved or grew since they were written.
+int refresh_view_pools(vx_batch *b, hipStream_t st) {
+    if (b->vk_pool_gen == b->ctx->pool_gen || b->n_views == 0) return VX_OK;
+    for (ViewK &k : b->vk) view_pools(b, k);
+    HIPX(hipMemcpyAsync(b->views.p, b->vk.data(), (size_t)b->n_views * sizeof(ViewK), hipMemcpyHostToDevice, st));
+    HIPX(hipStreamSynchronize(st));
+    b->vk_pool_gen = b->ctx->pool_gen;
+    return VX_OK;
 }
 
 BatchArgs batch_args(const vx_batch *b) {
@@ -360,6 +379,7 @@ int vx_batch_set_views(vx_batch *b, const vx_view *views, uint32_t n) {
         b->n_views = 0;
         return rc;
     }
+    for (ViewK &k : vk) view_pools(b, k);  // after view_buffers: the chunk and grid buffers are placed
     std::vector<vx_gpu_chunk> cat(std::max<size_t>(recs, 1));
     std::vector<uint32_t> grids((size_t)n * VX_GRID_LEN);
     for (uint32_t v = 0; v < n; v++) {
@@ -393,6 +413,7 @@ int vx_batch_set_views(vx_batch *b, const vx_view *views, uint32_t n) {
     if (moved || b->tn.mode == VX_BATCH_PLAIN || (b->tn.mode == VX_BATCH_FLAT && !same_ranges) || !same_classes)
         drop_graph(b);
     b->vk.swap(vk);
+    b->vk_pool_gen = c->pool_gen;
     b->toff.swap(toff);
     b->poff.swap(poff);
     b->slot_of.swap(slot_of);
@@ -432,7 +453,8 @@ int vx_batch_render(vx_batch *b) {
     b->ring_next = (b->ring_next + 1) % vx_batch::RING;
     if (b->ring_count < (uint32_t)vx_batch::RING) b->ring_count++;
     HIPX(hipEventRecord(ev[0], st));
-    int rc = VX_OK;
+    int rc = refresh_view_pools(b, st);
+    if (rc) return rc;
     if (b->tn.graph) {
         if ((!b->graph_valid || b->graph_pool_gen != c->pool_gen) && (rc = capture(b))) return rc;
         HIPX(hipEventRecord(ev[1], st));

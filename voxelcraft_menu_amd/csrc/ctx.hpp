@@ -39,17 +39,7 @@ int make_spec(uint32_t spec, uint32_t spec_hi, uint32_t frame_flags, vxd::Spec &
 int check_frame(const vx_gpu_frame *g, uint32_t n_chunks);
 
 // Launch of the tile-per-wave march for one view (one workgroup per tile, in `order`).
-struct TileArgs {
-    vxd::Pools p;
-    vxd::FrameK f;
-    vxd::Spec s;
-    uint32_t *hiz;
-    unsigned long long *vis;
-    uint16_t *shadow;
-    uint32_t *cnt;
-    const uint32_t *order;
-    uint32_t *cost;
-};
+using vxd::TileArgs;
 bool tuning_ok(uint32_t occ, uint32_t var);
 void launch_tiles(uint32_t occ, uint32_t var, bool hz, bool uw, unsigned nb, hipStream_t st, const TileArgs &a);
 // Cost-bucket launch order of n tiles into `order` (entries are tile indices); `ctl`

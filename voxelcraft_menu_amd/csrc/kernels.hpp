@@ -72,11 +72,21 @@ __global__ void k_finalize_recover(FrameK f, uint32_t *counters);
 //   VAR_FUSED_SHADOW: trace each pixel's sun ray in the tile's wave right after its key.
 constexpr int VAR_SPLIT = 1, VAR_FUSED_SHADOW = 4;
 
+// Arguments of the single-view tile march (one frame of one view).
+struct TileArgs {
+    Pools p;
+    FrameK f;
+    Spec s;
+    uint32_t *hiz;
+    unsigned long long *vis;
+    uint16_t *shadow;
+    uint32_t *cnt;
+    const uint32_t *order;
+    uint32_t *cost;
+};
+
 template <int kOcc, int kVar, bool kUseHiz, bool kUnderwater>
-__global__ void k_march_tiles(Pools p, FrameK f, Spec s, uint32_t *__restrict__ hiz,
-                              unsigned long long *__restrict__ vis, uint16_t *__restrict__ shadow,
-                              uint32_t *__restrict__ counters, const uint32_t *__restrict__ order,
-                              uint32_t *__restrict__ cost);
+__global__ void k_march_tiles(TileArgs a);
 
 // Launch ordering (class, then most expensive cost bucket first). Entries are tile
 // indices, or with a view table (n_views > 0) packed view << ENTRY_VIEW_SHIFT |
@@ -105,6 +115,10 @@ struct ViewK {
     uint32_t grid_off;   // first word of the view's 32x8x32 chunk grid
     uint32_t tile_off;   // first Hi-Z / cost word of the view
     uint32_t px_off_lo, px_off_hi;  // first pixel of the view in vis / shadow
+    // The tree pools with the view's chunk list and grid (chunks = batch list + chunk_off,
+    // grid = batch grids + grid_off, n_chunks): kernels read it in place, field by field.
+    // The host rewrites it when a pool moves or grows.
+    Pools p;
 };
 
 // Header words of a batch in device memory (read by graph-captured kernels, so a replay

@@ -727,45 +727,51 @@ __device__ __forceinline__ void normal_of(uint32_t id, float n[3]) {
     n[2] = axis == 2 ? s : 0.0f;
 }
 
+// shadow_guide (shadow.wgsl:14-30): whether the pixel's key gets a sun ray, and its
+// origin hit + n * 0.02 * vs, with hit = cam + rd * hit_t.
+__device__ inline bool shadow_origin(const Pools &p, const FrameK &f, const Spec &s, uint64_t key, uint32_t px,
+                                     uint32_t py, float o[3]) {
+    const bool active = (f.flags & VX_FLAG_SHADOWS) && !(f.flags & VX_FLAG_HEATMAP) && f.daylight > 0.01f;
+    if (!active || key == 0ull) return false;
+    uint32_t ci = (uint32_t)(key >> 24) & 0x1FFFu;
+    uint32_t low = (uint32_t)(key & 0xFFFFFFull);
+    uint32_t vx = (low >> 16) & 63u, vy = (low >> 8) & 63u, vz = low & 63u;
+    if (ci >= p.n_chunks) return false;
+    ChunkK c = ld_chunk(p, ci);
+    uint32_t id = block_at(p, c, vx, vy, vz);
+    if (id == WATER_ID || ((s.bits & SPEC_GLASS) && id == GLASS_ID)) return false;
+    uint32_t nid = (uint32_t)(key >> 37) & 7u;
+    uint32_t mx = (low >> 22) & 3u, my = (low >> 14) & 3u, mz = (low >> 6) & 3u;
+    float rd[3], n[3];
+    ray_dir(f, (float)px + 0.5f, (float)py + 0.5f, rd[0], rd[1], rd[2]);
+    normal_of(nid, n);
+    if (nid >= NORMAL_CROSS_A && (n[0] * rd[0] + n[1] * rd[1] + n[2] * rd[2]) > 0.0f) {
+        n[0] = -n[0]; n[1] = -n[1]; n[2] = -n[2];
+    }
+    if (!((n[0] * f.sun[0] + n[1] * f.sun[1] + n[2] * f.sun[2]) > 0.0f)) return false;
+    float t = hit_t(f, c, vx, vy, vz, mx, my, mz, nid, rd);
+    if (!(t >= 0.0f)) return false;
+    float e = 0.02f * c.vs;
+#pragma unroll
+    for (int i = 0; i < 3; i++) o[i] = (f.cam[i] + rd[i] * t) + n[i] * e;
+    return true;
+}
+
+// primary_shadow's stored value (shadow.wgsl:34-48): R16F of clamp(t, 0.001, 65504) for
+// a blocked sun ray, 0 otherwise.
+__device__ __forceinline__ uint16_t shadow_word(float blocker) {
+    float distance = 0.0f;
+    if (blocker >= 0.0f) distance = fminf(fmaxf(blocker, 0.001f), 65504.0f);
+    return f32_to_f16_bits(distance);
+}
+
 // primary_shadow for one pixel (shadow.wgsl:8-48). Returns the R16F bits and, through
 // traced, whether a sun ray was cast (the pixel's shadow_guide was valid).
 __device__ inline uint16_t shadow_pixel(const Pools &p, const FrameK &f, const Spec &s, uint64_t key, uint32_t px,
                                         uint32_t py, bool &traced) {
-    traced = false;
-    float distance = 0.0f;
-    const bool active = (f.flags & VX_FLAG_SHADOWS) && !(f.flags & VX_FLAG_HEATMAP) && f.daylight > 0.01f;
-    if (active && key != 0ull) {
-        uint32_t ci = (uint32_t)(key >> 24) & 0x1FFFu;
-        uint32_t low = (uint32_t)(key & 0xFFFFFFull);
-        uint32_t vx = (low >> 16) & 63u, vy = (low >> 8) & 63u, vz = low & 63u;
-        if (ci < p.n_chunks) {
-            ChunkK c = ld_chunk(p, ci);
-            uint32_t id = block_at(p, c, vx, vy, vz);
-            if (!(id == WATER_ID || ((s.bits & SPEC_GLASS) && id == GLASS_ID))) {
-                uint32_t nid = (uint32_t)(key >> 37) & 7u;
-                uint32_t mx = (low >> 22) & 3u, my = (low >> 14) & 3u, mz = (low >> 6) & 3u;
-                float rd[3], n[3];
-                ray_dir(f, (float)px + 0.5f, (float)py + 0.5f, rd[0], rd[1], rd[2]);
-                normal_of(nid, n);
-                if (nid >= NORMAL_CROSS_A && (n[0] * rd[0] + n[1] * rd[1] + n[2] * rd[2]) > 0.0f) {
-                    n[0] = -n[0]; n[1] = -n[1]; n[2] = -n[2];
-                }
-                if ((n[0] * f.sun[0] + n[1] * f.sun[1] + n[2] * f.sun[2]) > 0.0f) {
-                    float t = hit_t(f, c, vx, vy, vz, mx, my, mz, nid, rd);
-                    if (t >= 0.0f) {
-                        float e = 0.02f * c.vs;
-                        float o[3];
-#pragma unroll
-                        for (int i = 0; i < 3; i++) o[i] = (f.cam[i] + rd[i] * t) + n[i] * e;
-                        traced = true;
-                        float blocker = shadow_ray_t(p, f, s, o, f.sun, f.shadow_dist);
-                        if (blocker >= 0.0f) distance = fminf(fmaxf(blocker, 0.001f), 65504.0f);
-                    }
-                }
-            }
-        }
-    }
-    return f32_to_f16_bits(distance);
+    float o[3];
+    traced = shadow_origin(p, f, s, key, px, py, o);
+    return shadow_word(traced ? shadow_ray_t(p, f, s, o, f.sun, f.shadow_dist) : -1.0f);
 }
 
 // box_outside / box_distance (tile_select.wgsl:1-9)

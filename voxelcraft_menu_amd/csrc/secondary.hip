@@ -52,14 +52,11 @@ __global__ __launch_bounds__(64) void k_sec_gather(BatchArgs a, SecArgs s) {
     const uint32_t v = entry >> ENTRY_VIEW_SHIFT, tile = entry & ENTRY_TILE_MASK;
     if (v >= a.hdr[BH_VIEWS]) return;
     const ViewK *vk = a.views + v;
-    const FrameK f = ld_const(&vk->f);
+    const FrameK &f = *(const FrameK *)as_const(&vk->f);
     const Spec sp = ld_const(&vk->s);
     if (tile >= f.tiles[0] * f.tiles[1]) return;
     const_as<ViewK> *vc = as_const(vk);
-    Pools p = a.p;
-    p.chunks = a.p.chunks + vc->chunk_off;
-    p.grid = a.p.grid + vc->grid_off;
-    p.n_chunks = vc->n_chunks;
+    const Pools &p = *(const Pools *)as_const(&vk->p);
     const size_t px0 = (size_t)vc->px_off_lo | ((size_t)vc->px_off_hi << 32);
     const uint32_t lane = lane_id();
     const uint32_t px = (tile % f.tiles[0]) * 8u + (lane & 7u), py = (tile / f.tiles[0]) * 8u + (lane >> 3);
@@ -101,13 +98,10 @@ __global__ __launch_bounds__(64) void k_sec_trace(BatchArgs a, SecArgs s) {
                 (int)__shfl((int)my_view, __ffsll((long long)ballot(todo)) - 1));
             const bool here = todo && my_view == v;
             const ViewK *vk = a.views + v;
-            const FrameK f = ld_const(&vk->f);
+            const FrameK &f = *(const FrameK *)as_const(&vk->f);
             const Spec sp = ld_const(&vk->s);
             const_as<ViewK> *vc = as_const(vk);
-            Pools p = a.p;
-            p.chunks = a.p.chunks + vc->chunk_off;
-            p.grid = a.p.grid + vc->grid_off;
-            p.n_chunks = vc->n_chunks;
+            const Pools &p = *(const Pools *)as_const(&vk->p);
             const size_t px0 = (size_t)vc->px_off_lo | ((size_t)vc->px_off_hi << 32);
             uint32_t info = 0;
             if (here) {

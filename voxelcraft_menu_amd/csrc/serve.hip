@@ -197,6 +197,10 @@ int launch(vx_serve *s, Lane &L) {
         vk[v] = r->vk;
         vk[v].chunk_off = (uint32_t)c;
         vk[v].grid_off = v * VX_GRID_LEN;
+        vk[v].p = vxh::make_pools(s->ctx);
+        vk[v].p.chunks = (const vx_gpu_chunk *)((const char *)L.in.p + o_chunks) + c;
+        vk[v].p.grid = (const uint32_t *)((const char *)L.in.p + o_grids) + (size_t)v * VX_GRID_LEN;
+        vk[v].p.n_chunks = (uint32_t)r->chunks.size();
         vk[v].tile_off = t;
         vk[v].px_off_lo = (uint32_t)r->px_off;
         vk[v].px_off_hi = (uint32_t)(r->px_off >> 32);

@@ -173,32 +173,37 @@ __device__ __forceinline__ void march_tile_body(const Pools &p, const FrameK &f,
 
 constexpr uint32_t kPrioTiles = 256;
 
+// Launch arguments are read in place in the kernel-argument segment (the by-value
+// parameter is the segment's first field and is not read directly), so each field is
+// loaded where it is used instead of being held in scalar registers from the start.
+template <typename T>
+__device__ __forceinline__ const T &kernel_args() {
+    return *(const T *)__builtin_amdgcn_kernarg_segment_ptr();
+}
+
 template <int kOcc, int kVar, bool kUseHiz, bool kUnderwater>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kOcc))) void k_march_tiles(
-    Pools p, FrameK f, Spec s, uint32_t *__restrict__ hiz, unsigned long long *__restrict__ vis,
-    uint16_t *__restrict__ shadow, uint32_t *__restrict__ counters, const uint32_t *__restrict__ order,
-    uint32_t *__restrict__ cost) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kOcc))) void k_march_tiles(TileArgs) {
+    const TileArgs &a = kernel_args<TileArgs>();
     __shared__ uint64_t defmask[kUseHiz ? 128 : 1];
     __shared__ uint64_t selmask[(kVar & VAR_SPLIT) ? 128 : 1];
+    const FrameK &f = a.f;
     const uint32_t ntiles = f.tiles[0] * f.tiles[1];
     if (blockIdx.x >= ntiles) return;
-    const uint32_t tile = order[blockIdx.x];
+    const uint32_t tile = a.order[blockIdx.x];
     if (tile >= ntiles) return;
     // The order list starts with last frame's most expensive tiles: they are the
     // frame's critical path, so their waves get instruction-issue priority.
     if (blockIdx.x < kPrioTiles) __builtin_amdgcn_s_setprio(3);
     const uint64_t t0 = wall_clock64();
-    march_tile_body<kUseHiz, kUnderwater, kVar>(p, f, s, tile, hiz, vis, shadow, counters, defmask, selmask);
+    const Spec s = a.s;
+    march_tile_body<kUseHiz, kUnderwater, kVar>(a.p, f, s, tile, a.hiz, a.vis, a.shadow, a.cnt, defmask, selmask);
     if (lane_id() == 0) {
         uint64_t dt = wall_clock64() - t0;
-        cost[tile] = dt > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)dt;
+        a.cost[tile] = dt > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)dt;
     }
 }
 
-#define VX_TILES_INST1(O, V, H, U)                                                                     \
-    template __global__ void k_march_tiles<O, V, H, U>(Pools, FrameK, Spec, uint32_t *,                \
-                                                       unsigned long long *, uint16_t *, uint32_t *, \
-                                                       const uint32_t *, uint32_t *);
+#define VX_TILES_INST1(O, V, H, U) template __global__ void k_march_tiles<O, V, H, U>(TileArgs);
 #define VX_TILES_INST(O, V)          \
     VX_TILES_INST1(O, V, false, false) \
     VX_TILES_INST1(O, V, true, false)  \
@@ -233,10 +238,7 @@ __device__ __forceinline__ void batch_tile(const BatchArgs &a, uint32_t entry, u
     const Spec s = ld_const(&vk->s);
     if (tile >= f.tiles[0] * f.tiles[1]) return;
     const_as<ViewK> *vc = as_const(vk);
-    Pools p = a.p;
-    p.chunks = a.p.chunks + vc->chunk_off;
-    p.grid = a.p.grid + vc->grid_off;
-    p.n_chunks = vc->n_chunks;
+    const Pools &p = *(const Pools *)as_const(&vk->p);
     const size_t px0 = (size_t)vc->px_off_lo | ((size_t)vc->px_off_hi << 32);
     const uint32_t toff = vc->tile_off;
     const uint64_t t0 = wall_clock64();
@@ -248,16 +250,10 @@ __device__ __forceinline__ void batch_tile(const BatchArgs &a, uint32_t entry, u
     }
 }
 
-// The launch arguments are read in place in the kernel-argument segment (the by-value
-// parameter is the segment's first field and is not read directly), so each field is
-// loaded where it is used instead of being held in scalar registers from the start.
-__device__ __forceinline__ const BatchArgs &kernel_batch_args() {
-    return *(const BatchArgs *)__builtin_amdgcn_kernarg_segment_ptr();
-}
 
 template <int kOcc, int kVar, bool kHiz, bool kUnderwater>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kOcc))) void k_batch_flat(BatchArgs) {
-    const BatchArgs &a = kernel_batch_args();
+    const BatchArgs &a = kernel_args<BatchArgs>();
     __shared__ uint64_t defmask[kHiz ? 128 : 1];
     __shared__ uint64_t selmask[(kVar & VAR_SPLIT) ? 128 : 1];
     constexpr uint32_t cls = view_class(kHiz, kUnderwater);
@@ -275,7 +271,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kOcc))) void
 template <int kOcc, int kVar, bool kHiz, bool kUnderwater>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kOcc))) void k_batch_persistent(BatchArgs,
                                                                                                    uint32_t chunk) {
-    const BatchArgs &a = kernel_batch_args();
+    const BatchArgs &a = kernel_args<BatchArgs>();
     __shared__ uint64_t defmask[kHiz ? 128 : 1];
     __shared__ uint64_t selmask[(kVar & VAR_SPLIT) ? 128 : 1];
     constexpr uint32_t cls = view_class(kHiz, kUnderwater);

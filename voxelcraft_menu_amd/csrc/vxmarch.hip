@@ -113,13 +113,13 @@ namespace vxh {
 template <int O, int V>
 static void launch_tiles_ov(bool hz, bool uw, unsigned nb, hipStream_t st, const TileArgs &a) {
     if (hz && !uw)
-        k_march_tiles<O, V, true, false><<<nb, 64, 0, st>>>(a.p, a.f, a.s, a.hiz, a.vis, a.shadow, a.cnt, a.order, a.cost);
+        k_march_tiles<O, V, true, false><<<nb, 64, 0, st>>>(a);
     else if (!hz && !uw)
-        k_march_tiles<O, V, false, false><<<nb, 64, 0, st>>>(a.p, a.f, a.s, a.hiz, a.vis, a.shadow, a.cnt, a.order, a.cost);
+        k_march_tiles<O, V, false, false><<<nb, 64, 0, st>>>(a);
     else if (hz)
-        k_march_tiles<O, V, true, true><<<nb, 64, 0, st>>>(a.p, a.f, a.s, a.hiz, a.vis, a.shadow, a.cnt, a.order, a.cost);
+        k_march_tiles<O, V, true, true><<<nb, 64, 0, st>>>(a);
     else
-        k_march_tiles<O, V, false, true><<<nb, 64, 0, st>>>(a.p, a.f, a.s, a.hiz, a.vis, a.shadow, a.cnt, a.order, a.cost);
+        k_march_tiles<O, V, false, true><<<nb, 64, 0, st>>>(a);
 }
 
 // Valid (occupancy, variant) pairs compiled into the library.
