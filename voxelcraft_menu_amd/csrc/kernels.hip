@@ -100,8 +100,8 @@ __global__ __launch_bounds__(256) void k_march_pairs(Pools p, FrameK f, Spec s, 
         float rd[3], inv[3];
         ray_dir(f, (float)px + 0.5f, (float)py + 0.5f, rd[0], rd[1], rd[2]);
         for (int i = 0; i < 3; i++) inv[i] = 1.0f / rd[i];
-        Hit h = underwater ? march_chunk<true, false, false>(p, f, s, c, f.cam, rd, inv, f.far_)
-                           : march_chunk<false, false, false>(p, f, s, c, f.cam, rd, inv, f.far_);
+        Hit h = underwater ? march_chunk<true, false, false>(p, f, s, c, ci, f.cam, rd, inv, f.far_)
+                           : march_chunk<false, false, false>(p, f, s, c, ci, f.cam, rd, inv, f.far_);
         if (h.hit) atomicMax(&vis[(size_t)py * f.res[0] + px], (unsigned long long)make_key(f.far_, h, ci));
         // FLAG_HEATMAP frames: every marched pair adds its iterations (march.wgsl:51-53).
         if (heat) atomicAdd(&heat[(size_t)py * f.res[0] + px], h.iters);

@@ -209,6 +209,55 @@ __device__ inline uint32_t world_block_at(const Pools &p, const FrameK &f, float
     return block_at(p, c, (uint32_t)vx, (uint32_t)vy, (uint32_t)vz);
 }
 
+// world_block_at for a point beside the cell a march is on (the water-interface probe of
+// march_chunk, common.wgsl:800-813): the grid cell is looked up exactly as world_block_at
+// does; when it names the chunk being marched (ci), the point's voxel is read from that
+// chunk with the march's L1 node, leaf and attribute word reused where the voxel shares
+// them (same L1 cell, same leaf), instead of reloading the record and the node chain.
+// The nodes are the same words world_block_at would load, so the block id is the same.
+template <bool kUnit>
+__device__ inline uint32_t world_block_near(const Pools &p, const FrameK &f, const ChunkK &c, uint32_t ci, float px,
+                                            float py, float pz, uint32_t l1_cached, const uint4 &l1,
+                                            uint32_t leaf_cached, uint64_t leaf, uint32_t attr) {
+    bool ok;
+    const uint32_t gi =
+        grid_index(f, (int32_t)floorf(px / 64.0f), (int32_t)floorf(py / 64.0f), (int32_t)floorf(pz / 64.0f), ok);
+    if (!ok) return 0u;
+    const uint32_t ci2 = ld_grid(p, f, gi);
+    if (ci2 == NO_CHUNK || ci2 >= p.n_chunks) return 0u;
+    if (ci2 != ci) return world_block_at(p, f, px, py, pz);
+    auto dvs = [&](float x) { return kUnit ? x : x / c.vs; };
+    const int32_t vx = (int32_t)floorf(dvs(px - c.origin[0]));
+    const int32_t vy = (int32_t)floorf(dvs(py - c.origin[1]));
+    const int32_t vz = (int32_t)floorf(dvs(pz - c.origin[2]));
+    if (vx < 0 || vy < 0 || vz < 0 || vx >= 64 || vy >= 64 || vz >= 64) return 0u;
+    const uint32_t ux = (uint32_t)vx, uy = (uint32_t)vy, uz = (uint32_t)vz;
+    const uint64_t root_mask = mk64(c.root[0], c.root[1]);
+    const uint32_t l1b = cell_bit(ux >> 4, uy >> 4, uz >> 4);
+    if (!mask_has(root_mask, l1b)) return 0u;  // voxel_occupied
+    const bool root_full = (c.root[2] & FULL_BIT) != 0u;
+    const bool same_l1 = l1b == l1_cached;
+    uint4 n1;
+    if (same_l1) n1 = l1;
+    else if (root_full) n1 = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, FULL_BIT, l1b * 64u);
+    else n1 = ld_inner(p, (c.root[2] & PTR_MASK) + mask_below(root_mask, l1b));
+    const uint64_t l1m = mk64(n1.x, n1.y);
+    const uint32_t lfb = cell_bit((ux >> 2) & 3u, (uy >> 2) & 3u, (uz >> 2) & 3u);
+    const bool same_leaf = same_l1 && lfb == leaf_cached;
+    if (!root_full && !(n1.z & FULL_BIT)) {
+        if (!mask_has(l1m, lfb)) return 0u;
+        uint64_t lm = leaf;
+        if (!same_leaf) {
+            const uint2 lf = ld_leaf(p, (n1.z & PTR_MASK) + mask_below(l1m, lfb));
+            lm = mk64(lf.x, lf.y);
+        }
+        if (!mask_has(lm, cell_bit(ux & 3u, uy & 3u, uz & 3u))) return 0u;
+    }
+    if (c.attr_flags & ATTR_UNIFORM) return c.attr_base;  // block_at
+    const uint32_t entry = same_leaf ? attr : ld_brick(p, c.attr_base + (n1.w & 0x0FFFFFFFu) + mask_below(l1m, lfb));
+    return leaf_block(p, entry, ux, uy, uz);
+}
+
 // ray_dir (common.wgsl:358-368): normalize = v / sqrt(dot(v, v)).
 __device__ __forceinline__ void ray_dir(const FrameK &f, float px, float py, float &dx, float &dy, float &dz) {
     float ndc_x = ((px + f.jitter[0]) / (float)f.res[0]) * 2.0f - 1.0f;
@@ -349,8 +398,9 @@ struct Hit {
 // rays use <false,false,false> (or <true,false,false> under water), sun rays
 // <true,true,true>, so each kernel carries only the branches its rays can take.
 template <bool skip_water, bool skip_glass, bool sun_ray, bool kUnit>
-__device__ __forceinline__ Hit march_chunk_vs(const Pools &p, const FrameK &f, const Spec &s, const ChunkK &c, const float ro_w[3],
-                                  const float rd[3], const float inv[3], float t_limit) {
+__device__ __forceinline__ Hit march_chunk_vs(const Pools &p, const FrameK &f, const Spec &s, const ChunkK &c,
+                                              uint32_t ci, const float ro_w[3], const float rd[3], const float inv[3],
+                                              float t_limit) {
     Hit h{false, 0.0f, 0u, 0u, 0u, 0u};
     const float vs = c.vs;
     // x / vs and x * vs; with kUnit (vs == 1.0f) both are x exactly.
@@ -521,7 +571,8 @@ __device__ __forceinline__ Hit march_chunk_vs(const Pools &p, const FrameK &f, c
                 float tw = mvs(t, vs);
                 float off = mvs(0.002f, vs);
                 float b0 = ro_w[0] + rd[0] * tw, b1 = ro_w[1] + rd[1] * tw, b2 = ro_w[2] + rd[2] * tw;
-                if (world_block_at(p, f, b0 + o[0] * off, b1 + o[1] * off, b2 + o[2] * off) == WATER_ID) {
+                if (world_block_near<kUnit>(p, f, c, ci, b0 + o[0] * off, b1 + o[1] * off, b2 + o[2] * off, l1_cached,
+                                            l1, leaf_cached, leaf, attr) == WATER_ID) {
                     solid = false;
                     skip = 1;
                 }
@@ -602,10 +653,11 @@ __device__ __forceinline__ Hit march_chunk_vs(const Pools &p, const FrameK &f, c
 // x / 1.0f == x * 1.0f == x in IEEE f32. Without the split the compiler evaluates the
 // correctly rounded divisions of the general case and then selects x.
 template <bool skip_water, bool skip_glass, bool sun_ray>
-__device__ __forceinline__ Hit march_chunk(const Pools &p, const FrameK &f, const Spec &s, const ChunkK &c,
+__device__ __forceinline__ Hit march_chunk(const Pools &p, const FrameK &f, const Spec &s, const ChunkK &c, uint32_t ci,
                                            const float ro_w[3], const float rd[3], const float inv[3], float t_limit) {
-    if (c.vs == 1.0f) return march_chunk_vs<skip_water, skip_glass, sun_ray, true>(p, f, s, c, ro_w, rd, inv, t_limit);
-    return march_chunk_vs<skip_water, skip_glass, sun_ray, false>(p, f, s, c, ro_w, rd, inv, t_limit);
+    if (c.vs == 1.0f)
+        return march_chunk_vs<skip_water, skip_glass, sun_ray, true>(p, f, s, c, ci, ro_w, rd, inv, t_limit);
+    return march_chunk_vs<skip_water, skip_glass, sun_ray, false>(p, f, s, c, ci, ro_w, rd, inv, t_limit);
 }
 
 // depth_key / make_key (common.wgsl:1320-1330)
@@ -645,7 +697,7 @@ __device__ inline float shadow_ray_t(const Pools &p, const FrameK &f, const Spec
             last = ci;
             if (ci < p.n_chunks) {
                 ChunkK c = ld_chunk(p, ci);
-                Hit m = march_chunk<true, true, true>(p, f, s, c, ro, rd, inv, max_dist);
+                Hit m = march_chunk<true, true, true>(p, f, s, c, ci, ro, rd, inv, max_dist);
                 if (m.hit) return m.t;
             }
         }
@@ -909,7 +961,7 @@ __device__ inline WorldHit trace_world(const Pools &p, const FrameK &f, const Sp
         if (tn[2] < t_exit) { t_exit = tn[2]; axis = 2; }
         if (ci != NO_CHUNK && ci < p.n_chunks) {
             const ChunkK c = ld_chunk(p, ci);
-            const Hit h = march_chunk<kSkipWater, true, false>(p, f, s, c, ro, rd, inv_rd, fminf(t_exit, max_dist));
+            const Hit h = march_chunk<kSkipWater, true, false>(p, f, s, c, ci, ro, rd, inv_rd, fminf(t_exit, max_dist));
             if (h.hit) {
                 out.hit = true;
                 out.t = h.t;

@@ -58,7 +58,7 @@ __device__ __forceinline__ void march_tile_body(const Pools &p, const FrameK &f,
         if (!ballot(need)) return;
         ChunkK c = ld_chunk(p, cj);
         if (need && !ray_skips(f, c.fade, px, py)) {
-            Hit h = march_chunk<kUnderwater, false, false>(p, f, s, c, f.cam, rd, inv, f.far_);
+            Hit h = march_chunk<kUnderwater, false, false>(p, f, s, c, cj, f.cam, rd, inv, f.far_);
             if (h.hit) {
                 uint64_t k = make_key(f.far_, h, cj);
                 best = k > best ? k : best;
