@@ -132,6 +132,7 @@ int enqueue_secondary(vx_batch *b, hipStream_t st) {
     SecArgs sa;
     sa.out = (uint4 *)b->sec_out.p;
     sa.queue = (unsigned long long *)b->sec_queue.p;
+    sa.tile_off = (const uint32_t *)b->tile_off.p;
     sa.cap = b->sec_cap;
     k_sec_gather<<<std::max(b->n_tiles, 1u), 64, 0, st>>>(a, sa);
     const unsigned nb = (unsigned)b->n_cu * 16u;
@@ -194,14 +195,7 @@ int enqueue(vx_batch *b, hipStream_t st, hipEvent_t ev_march) {
                 k_shadow<<<((nt + 3) / 4 + 7u) & ~7u, 256, 0, st>>>(ta.p, k.f, k.s, ta.vis, ta.shadow, cnt);
         }
         LAUNCHED();
-        if (b->sec_on) {
-            // The gather walks packed (view, tile) entries; PLAIN's order slices hold
-            // per-view tile indices, so lay the packed order over them first.
-            k_iota_views<<<ORDER_BLOCKS, 256, 0, st>>>((uint32_t *)b->order.p, (const uint32_t *)b->hdr.p + BH_TILES,
-                                                       0, (const uint32_t *)b->tile_off.p,
-                                                       (const uint32_t *)b->hdr.p + BH_VIEWS);
-            return enqueue_secondary(b, st);
-        }
+        if (b->sec_on) return enqueue_secondary(b, st);
         return VX_OK;
     }
     const uint32_t *hdr = (const uint32_t *)b->hdr.p;

@@ -163,6 +163,7 @@ __global__ void k_decode(Pools p, FrameK f, const unsigned long long *__restrict
 struct SecArgs {
     uint4 *__restrict__ out;                  // 2 records per pixel of the batch
     unsigned long long *__restrict__ queue;   // queue q at q * cap
+    const uint32_t *__restrict__ tile_off;    // n_views + 1 tile offsets (batch slot order)
     uint32_t cap;                             // entries per queue
 };
 __global__ void k_sec_gather(BatchArgs a, SecArgs s);

@@ -45,12 +45,19 @@ __device__ __forceinline__ void queue_push(const SecArgs &s, uint32_t *counters,
     }
 }
 
+// Tiles are gathered in view order (block i is global tile i, its view found by binary
+// search of the tile offsets), so each queue holds the legs view after view and the trace
+// waves rarely span more than one view.
 __global__ __launch_bounds__(64) void k_sec_gather(BatchArgs a, SecArgs s) {
-    const uint32_t n = a.hdr[BH_TILES];
-    if (blockIdx.x >= n) return;
-    const uint32_t entry = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.order[blockIdx.x]);
-    const uint32_t v = entry >> ENTRY_VIEW_SHIFT, tile = entry & ENTRY_TILE_MASK;
-    if (v >= a.hdr[BH_VIEWS]) return;
+    const uint32_t n = a.hdr[BH_TILES], nv = a.hdr[BH_VIEWS];
+    if (blockIdx.x >= n || nv == 0u) return;
+    uint32_t lo = 0, hi = nv;  // tile_off[lo] <= blockIdx.x < tile_off[hi]
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s.tile_off[mid] <= blockIdx.x) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t v = lo, tile = blockIdx.x - s.tile_off[lo];
     const ViewK *vk = a.views + v;
     const FrameK &f = *(const FrameK *)as_const(&vk->f);
     const Spec sp = ld_const(&vk->s);
