@@ -172,7 +172,9 @@ __global__ __launch_bounds__(256) void k_shadow(Pools p, FrameK f, Spec s, const
     bool traced = false;
     if (px < f.res[0] && py < f.res[1]) {
         size_t i = (size_t)py * f.res[0] + px;
-        out[i] = shadow_pixel(p, f, s, vis[i], px, py, traced);
+        float rd[3];
+        ray_dir(f, (float)px + 0.5f, (float)py + 0.5f, rd[0], rd[1], rd[2]);
+        out[i] = shadow_pixel(p, f, s, vis[i], rd, traced);
     }
     uint64_t t = ballot(traced);
     if (lane == 0) stat_add(counters, tile, STAT_SHADOW, (uint32_t)__popcll(t));

@@ -49,6 +49,7 @@ struct FrameK {
     float cam[3], fwd[3], right[3], up[3];
     float tan_half_fov, aspect, far_, daylight;
     float sun[3], shadow_dist, sea_level;
+    float inv_sun[3];  // 1.0 / sun per component (IEEE division on the host): the sun ray's inv
     float jitter[2];
     uint32_t res[2], tiles[2];
     uint32_t chunk_count, camera_chunk, flags, max_pairs, frame_index;
@@ -675,15 +676,15 @@ __device__ __forceinline__ uint64_t make_key(float far_, const Hit &h, uint32_t 
     return ((uint64_t)d << 40) | ((uint64_t)h.normal << 37) | ((uint64_t)ci << 24) | (uint64_t)b;
 }
 
-// shadow_ray_t (common.wgsl:1156-1195): chunk-grid DDA over at most 24 cells.
+// shadow_ray_t (common.wgsl:1156-1195): chunk-grid DDA over at most 24 cells along the
+// sun direction; its `inv = 1.0 / rd` is the frame's inv_sun.
 __device__ inline float shadow_ray_t(const Pools &p, const FrameK &f, const Spec &s, const float ro[3],
-                                     const float rd[3], float max_dist) {
-    float inv[3];
+                                     float max_dist) {
+    const float *rd = f.sun, *inv = f.inv_sun;
     bool pd[3];
     int32_t cell[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        inv[i] = 1.0f / rd[i];
         pd[i] = rd[i] > 0.0f;
         cell[i] = (int32_t)floorf(ro[i] / 64.0f);
     }
@@ -725,8 +726,8 @@ __device__ inline float shadow_ray_t(const Pools &p, const FrameK &f, const Spec
 // the sea surface first (SPEC_WATER_REPAIR), then walks the chunk grid as shadow_ray_t
 // does from there; any hit blocks. shadow_ray_t's hit distance is march_chunk's
 // max(t_enter, 0) * voxel_size, never negative, so ">= 0" is the walk's hit flag.
-__device__ inline bool shadow_ray(const Pools &p, const FrameK &f, const Spec &s, const float ro[3],
-                                  const float rd[3], float max_dist) {
+__device__ inline bool shadow_ray(const Pools &p, const FrameK &f, const Spec &s, const float ro[3], float max_dist) {
+    const float *rd = f.sun;
     float r_ro[3] = {ro[0], ro[1], ro[2]};
     float r_dist = max_dist;
     if ((s.bits & SPEC_WATER_REPAIR) && rd[1] > 1e-4f && ro[1] < f.sea_level) {
@@ -736,7 +737,7 @@ __device__ inline bool shadow_ray(const Pools &p, const FrameK &f, const Spec &s
         for (int k = 0; k < 3; k++) r_ro[k] = ro[k] + rd[k] * t_surf;
         r_dist = r_dist - t_surf;
     }
-    return shadow_ray_t(p, f, s, r_ro, rd, r_dist) >= 0.0f;
+    return shadow_ray_t(p, f, s, r_ro, r_dist) >= 0.0f;
 }
 
 // hit_t (common.wgsl:1300-1318)
@@ -781,8 +782,9 @@ __device__ __forceinline__ void normal_of(uint32_t id, float n[3]) {
 
 // shadow_guide (shadow.wgsl:14-30): whether the pixel's key gets a sun ray, and its
 // origin hit + n * 0.02 * vs, with hit = cam + rd * hit_t.
-__device__ inline bool shadow_origin(const Pools &p, const FrameK &f, const Spec &s, uint64_t key, uint32_t px,
-                                     uint32_t py, float o[3]) {
+// rd is the pixel's ray_dir(px + 0.5, py + 0.5), which the caller has already computed.
+__device__ inline bool shadow_origin(const Pools &p, const FrameK &f, const Spec &s, uint64_t key, const float rd[3],
+                                     float o[3]) {
     const bool active = (f.flags & VX_FLAG_SHADOWS) && !(f.flags & VX_FLAG_HEATMAP) && f.daylight > 0.01f;
     if (!active || key == 0ull) return false;
     uint32_t ci = (uint32_t)(key >> 24) & 0x1FFFu;
@@ -794,8 +796,7 @@ __device__ inline bool shadow_origin(const Pools &p, const FrameK &f, const Spec
     if (id == WATER_ID || ((s.bits & SPEC_GLASS) && id == GLASS_ID)) return false;
     uint32_t nid = (uint32_t)(key >> 37) & 7u;
     uint32_t mx = (low >> 22) & 3u, my = (low >> 14) & 3u, mz = (low >> 6) & 3u;
-    float rd[3], n[3];
-    ray_dir(f, (float)px + 0.5f, (float)py + 0.5f, rd[0], rd[1], rd[2]);
+    float n[3];
     normal_of(nid, n);
     if (nid >= NORMAL_CROSS_A && (n[0] * rd[0] + n[1] * rd[1] + n[2] * rd[2]) > 0.0f) {
         n[0] = -n[0]; n[1] = -n[1]; n[2] = -n[2];
@@ -819,11 +820,11 @@ __device__ __forceinline__ uint16_t shadow_word(float blocker) {
 
 // primary_shadow for one pixel (shadow.wgsl:8-48). Returns the R16F bits and, through
 // traced, whether a sun ray was cast (the pixel's shadow_guide was valid).
-__device__ inline uint16_t shadow_pixel(const Pools &p, const FrameK &f, const Spec &s, uint64_t key, uint32_t px,
-                                        uint32_t py, bool &traced) {
+__device__ inline uint16_t shadow_pixel(const Pools &p, const FrameK &f, const Spec &s, uint64_t key, const float rd[3],
+                                        bool &traced) {
     float o[3];
-    traced = shadow_origin(p, f, s, key, px, py, o);
-    return shadow_word(traced ? shadow_ray_t(p, f, s, o, f.sun, f.shadow_dist) : -1.0f);
+    traced = shadow_origin(p, f, s, key, rd, o);
+    return shadow_word(traced ? shadow_ray_t(p, f, s, o, f.shadow_dist) : -1.0f);
 }
 
 // box_outside / box_distance (tile_select.wgsl:1-9)
@@ -858,6 +859,54 @@ __device__ inline TileCull tile_cull_setup(const FrameK &f, uint32_t gx, uint32_
     ray_dir(f, x1, y1, d[2][0], d[2][1], d[2][2]);
     ray_dir(f, x0, y1, d[3][0], d[3][1], d[3][2]);
     ray_dir(f, (x0 + x1) * 0.5f, (y0 + y1) * 0.5f, ce[0], ce[1], ce[2]);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const float *a = d[k];
+        const float *b = d[(k + 1) & 3];
+        float cx = a[1] * b[2] - a[2] * b[1];
+        float cy = a[2] * b[0] - a[0] * b[2];
+        float cz = a[0] * b[1] - a[1] * b[0];
+        bool keep = (cx * ce[0] + cy * ce[1] + cz * ce[2]) >= 0.0f;
+        tc.n[k][0] = keep ? cx : -cx;
+        tc.n[k][1] = keep ? cy : -cy;
+        tc.n[k][2] = keep ? cz : -cz;
+    }
+    tc.reach = f.far_;
+    const uint32_t uw = VX_FLAG_UNDERWATER | VX_FLAG_WATER_FAR;
+    if ((f.flags & uw) == uw) {
+        float up = fmaxf(fmaxf(d[0][1], d[1][1]), fmaxf(d[2][1], d[3][1]));
+        float submerged = fmaxf(f.sea_level - f.cam[1], 0.0f);
+        float escape = 1e30f;
+        if (up > 0.0f) escape = submerged / up;
+        if (escape >= 128.0f) {
+            tc.reach = fminf(f.far_, 128.0f);
+            float dep_far = submerged - up * 128.0f;
+            if ((f.flags & VX_FLAG_WATER_DARK) && fminf(submerged, dep_far) >= 15.0f) tc.reach = fminf(tc.reach, 64.0f);
+        }
+    }
+    return tc;
+}
+
+// tile_cull_setup with the five ray_dir evaluations spread over lanes 0-4 of the wave
+// (corner k on lane k, the centre on lane 4) and broadcast: the same function on the same
+// arguments, so the same planes, for one ray_dir's worth of instructions instead of five.
+// Needs the whole wave active.
+__device__ inline TileCull tile_cull_setup_wave(const FrameK &f, uint32_t gx, uint32_t gy, uint32_t lane) {
+    TileCull tc;
+    const float x0 = (float)(gx * 8u), y0 = (float)(gy * 8u);
+    const float x1 = x0 + 8.0f, y1 = y0 + 8.0f;
+    // corners in the order (x0,y0) (x1,y0) (x1,y1) (x0,y1), then the centre
+    const float qx = lane == 4u ? (x0 + x1) * 0.5f : ((lane == 1u || lane == 2u) ? x1 : x0);
+    const float qy = lane == 4u ? (y0 + y1) * 0.5f : ((lane == 2u || lane == 3u) ? y1 : y0);
+    float q[3];
+    ray_dir(f, qx, qy, q[0], q[1], q[2]);
+    float d[4][3], ce[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) d[k][i] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q[i]), k));
+        ce[i] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q[i]), 4));
+    }
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const float *a = d[k];
@@ -1036,7 +1085,7 @@ __device__ inline uint32_t leg_sun(const Pools &p, const FrameK &f, const Spec &
         const float hit = l.ro[k] + l.rd[k] * w.t;
         o[k] = hit + n[k] * (0.02f * c.vs);
     }
-    return SEC_SUN_TESTED | (shadow_ray(p, f, s, o, f.sun, l.sun_dist) ? SEC_SUN_BLOCKED : 0u);
+    return SEC_SUN_TESTED | (shadow_ray(p, f, s, o, l.sun_dist) ? SEC_SUN_BLOCKED : 0u);
 }
 
 // The legs of one pixel's primary key (shade_water with water_ctx's flat normal,

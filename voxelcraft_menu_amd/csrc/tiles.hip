@@ -45,7 +45,7 @@ __device__ __forceinline__ void march_tile_body(const Pools &p, const FrameK &f,
     ray_dir(f, (float)px + 0.5f, (float)py + 0.5f, rd[0], rd[1], rd[2]);
 #pragma unroll
     for (int i = 0; i < 3; i++) inv[i] = 1.0f / rd[i];
-    TileCull tc = tile_cull_setup(f, gx, gy);
+    TileCull tc = tile_cull_setup_wave(f, gx, gy, lane);
     const uint32_t tile_far = kUseHiz ? hiz[tile] : 0u;
     uint64_t best = 0;
     uint32_t n_sel = 0, n_def = 0, n_rec = 0;
@@ -160,7 +160,7 @@ __device__ __forceinline__ void march_tile_body(const Pools &p, const FrameK &f,
     if constexpr ((kVar & VAR_FUSED_SHADOW) != 0) {
         // primary_shadow reads only its own pixel's key: trace it here from the register.
         bool traced = false;
-        if (in_frame) shadow[pix] = shadow_pixel(p, f, s, best, px, py, traced);
+        if (in_frame) shadow[pix] = shadow_pixel(p, f, s, best, rd, traced);
         n_shadow = (uint32_t)__popcll(ballot(traced));
     }
     if (lane == 0) {

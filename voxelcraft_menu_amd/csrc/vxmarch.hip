@@ -362,6 +362,7 @@ FrameK make_frame(const vx_gpu_frame *g) {
     std::memcpy(f.right, g->cam_right, 12);
     std::memcpy(f.up, g->cam_up, 12);
     std::memcpy(f.sun, g->sun_dir, 12);
+    for (int i = 0; i < 3; i++) f.inv_sun[i] = 1.0f / g->sun_dir[i];  // IEEE division, as the kernels' 1.0 / rd
     f.tan_half_fov = g->tan_half_fov;
     f.aspect = g->aspect;
     f.far_ = g->far_;
