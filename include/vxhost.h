@@ -16,6 +16,8 @@
  *                            (src/math.rs:69-117, src/render/mod.rs:82-138,1447-1564)
  *   vxw_pack_frame        <- the GpuFrame packing in Renderer::render
  *                            (src/render/mod.rs:1612-1710)
+ *   vxw_journal_parse     <- EditJournal::load (src/journal.rs:391-475)
+ *   vxw_journal_write     <- EditJournal::save (src/journal.rs:477-508)
  *   vxw_generate          <- a seeded terrain generator with the reference's
  *                            structure (heightfield, sea, caves, trees); it is
  *                            fixture support and does not reproduce worldgen.rs bits.
@@ -38,7 +40,8 @@ extern "C" {
 enum {
     VXW_OK = 0,
     VXW_E_ARG = -3,
-    VXW_E_MISSING = -7
+    VXW_E_MISSING = -7,
+    VXW_E_JOURNAL = -9
 };
 
 typedef struct vxw_world vxw_world;
@@ -142,6 +145,40 @@ int vxw_build_view(const vxw_world *w, const vxw_camera *cam, vx_gpu_chunk *chun
 int vxw_pack_frame(const vxw_camera *cam, const vxw_view *view, uint32_t max_pairs, vx_gpu_frame *out);
 /* render::pairs_for (src/render/mod.rs:27-30). */
 uint32_t vxw_pairs_for(uint32_t tiles_x, uint32_t tiles_y);
+
+/* ---- Edit journal files (VXJ1, src/journal.rs:13-139) -------------------------------
+ * Header (24 B) | PlayerState (32 B, flags & 1) | BarState (24 B, flags & 4) | count
+ * journal records (vx_edit, 16 B). flags & 2 marks an append log: records past the
+ * header's count are recovered, a torn trailing record dropped. */
+typedef struct vxw_player {
+    float pos[3];
+    float yaw, pitch;
+    uint32_t hotbar, fly, pad;
+} vxw_player;
+
+typedef struct vxw_journal_info {
+    uint32_t version, flags;
+    int32_t seed, sea_level;
+    uint32_t count;              /* edits in the file (after append recovery) */
+    uint32_t recovered;          /* of which past the header's count (append log) */
+    uint32_t torn_bytes;         /* trailing bytes of a half-written record, dropped */
+    uint32_t has_player, has_bar;
+    vxw_player player;
+    uint16_t bar[10];
+} vxw_journal_info;
+
+/* Checks the file as EditJournal::load does (magic, version 1..=3, seed and sea level,
+ * player and bar blocks, record count) and copies its records into edits (cap >= count;
+ * edits may be NULL to learn the count). VXW_E_JOURNAL with the reason in err. Records
+ * come in file order: applied in order (vxw_apply_edits) the last edit of a voxel wins,
+ * as in the journal's edit map. */
+int vxw_journal_parse(const void *bytes, size_t n, int32_t seed, int32_t sea_level, vxw_journal_info *info,
+                      vx_edit *edits, size_t cap, char *err, size_t err_cap);
+/* Writes the journal EditJournal::save writes: the last edit of each voxel, sorted by
+ * (x, z, y), with the player / bar blocks when given (NULL: absent). *written = bytes
+ * needed; out may be NULL to learn it. */
+int vxw_journal_write(const vx_edit *edits, size_t n, int32_t seed, int32_t sea_level, const vxw_player *player,
+                      const uint16_t *bar, void *out, size_t cap, size_t *written);
 
 #ifdef __cplusplus
 }

@@ -113,6 +113,17 @@ assert C.sizeof(Edit) == 16
 EDIT_DTYPE = np.dtype([("x", "<i4"), ("y", "<i4"), ("z", "<i4"), ("id", "<u2"), ("pad", "<u2")])
 
 
+class Player(C.Structure):  # vxw_player (journal PlayerState, src/journal.rs:36-45)
+    _fields_ = [("pos", C.c_float * 3), ("yaw", C.c_float), ("pitch", C.c_float), ("hotbar", C.c_uint32),
+                ("fly", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class JournalInfo(C.Structure):  # vxw_journal_info
+    _fields_ = [("version", C.c_uint32), ("flags", C.c_uint32), ("seed", C.c_int32), ("sea_level", C.c_int32),
+                ("count", C.c_uint32), ("recovered", C.c_uint32), ("torn_bytes", C.c_uint32),
+                ("has_player", C.c_uint32), ("has_bar", C.c_uint32), ("player", Player), ("bar", C.c_uint16 * 10)]
+
+
 class ChunkInfo(C.Structure):
     _fields_ = [
         ("key", C.c_int32 * 3), ("root", C.c_uint32 * 4), ("attr_uniform", C.c_uint32),
@@ -246,6 +257,10 @@ _sig_host = {
     "vxw_build_view": (C.c_int, [P, C.POINTER(Camera), P, P, C.POINTER(View)]),
     "vxw_pack_frame": (C.c_int, [C.POINTER(Camera), C.POINTER(View), C.c_uint32, C.POINTER(Frame)]),
     "vxw_pairs_for": (C.c_uint32, [C.c_uint32, C.c_uint32]),
+    "vxw_journal_parse": (C.c_int, [P, C.c_size_t, C.c_int32, C.c_int32, C.POINTER(JournalInfo), P, C.c_size_t,
+                                    C.c_char_p, C.c_size_t]),
+    "vxw_journal_write": (C.c_int, [P, C.c_size_t, C.c_int32, C.c_int32, C.POINTER(Player), P, P, C.c_size_t,
+                                    C.POINTER(C.c_size_t)]),
 }
 
 _sig_march = {

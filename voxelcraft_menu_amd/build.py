@@ -19,7 +19,7 @@ ROOT = os.path.dirname(HERE)
 INC = os.path.join(ROOT, "include")
 CSRC = os.path.join(HERE, "csrc")
 
-HOST_SOURCES = ["host/world.cpp", "host/worldgen.cpp", "host/view.cpp", "host/capi.cpp"]
+HOST_SOURCES = ["host/world.cpp", "host/worldgen.cpp", "host/view.cpp", "host/capi.cpp", "host/journal.cpp"]
 HIP_SOURCES = ["kernels.hip", "tiles.hip", "secondary.hip", "vxmarch.hip", "batch.hip", "serve.hip"]
 HIP_EXTRA = []  # further .hip translation units are appended by later modules
 
