@@ -226,6 +226,8 @@ def run_batch(args, out):
         out["serving"] = run_serving(args)
     if not args.no_incoherent:
         out["incoherent"] = run_incoherent(args)
+    if not args.no_build:
+        out["chunk_build"] = run_chunk_build(args)
 
     if args.no_cpu_baseline:
         out["cpu_baseline"] = None
@@ -265,6 +267,68 @@ def run_serving(args):
     r.close()
     res["workload"] = wl.name
     _log(f"serving: p50 {res['p50_ms']} ms p99 {res['p99_ms']} ms at {args.serve_qps} qps")
+    return res
+
+
+def run_chunk_build(args):
+    """build_local of the 1024 chunks of the 1024x1024x256 world: on the device (dense
+    chunks resident in HBM, vx_build_local_device, records written to HBM) and on one host
+    core (the host build_local into the same flat records, a sample of chunks). The
+    device records are checked against the host's on the sample."""
+    import ctypes as C
+
+    from voxelcraft_menu_amd import _abi as A
+    from voxelcraft_menu_amd import workloads as W
+    from voxelcraft_menu_amd.renderer import Renderer
+    from voxelcraft_menu_amd.world import build_local_flat
+
+    wl = W.WORKLOADS[1]
+    world = W.make_world(wl)
+    keys = [tuple(int(v) for v in k) for k in world.keys()]
+    dense = np.stack([world.extract_dense(k) for k in keys])
+    n = dense.shape[0]
+    r = Renderer(64, 64)
+    L = A.march()
+    d_dense, d_out = C.c_void_p(), C.c_void_p()
+    A.check(L.vx_dev_alloc(r.handle, dense.nbytes, C.byref(d_dense)), "vx_dev_alloc")
+    A.check(L.vx_dev_alloc(r.handle, n * A.LOCAL_TREE_MAX, C.byref(d_out)), "vx_dev_alloc")
+    A.check(L.vx_dev_upload(r.handle, d_dense, A.ptr(dense), dense.nbytes), "vx_dev_upload")
+
+    def device_build():
+        A.check(L.vx_build_local_device(r.handle, d_dense, n, d_out, A.LOCAL_TREE_MAX), "vx_build_local_device")
+
+    for _ in range(2):
+        device_build()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        device_build()
+    dt = (time.perf_counter() - t0) / reps
+    sample = list(range(0, n, max(1, n // 64)))
+    host = []
+    t0 = time.perf_counter()
+    for i in sample:
+        host.append(build_local_flat(dense[i]))
+    host_dt = (time.perf_counter() - t0) / len(sample)
+    out_h = np.zeros(n * A.LOCAL_TREE_MAX, dtype=np.uint8)
+    A.check(L.vx_dev_download(r.handle, A.ptr(out_h), d_out, out_h.nbytes), "vx_dev_download")
+    A.check(L.vx_dev_free(r.handle, d_dense), "vx_dev_free")
+    A.check(L.vx_dev_free(r.handle, d_out), "vx_dev_free")
+    for i, want in zip(sample, host):
+        got = out_h[i * A.LOCAL_TREE_MAX:i * A.LOCAL_TREE_MAX + len(want)].tobytes()
+        if got != want:
+            _fail(f"device chunk build differs from the host build for chunk {keys[i]}")
+    rec_bytes = int(sum(int(out_h[i * A.LOCAL_TREE_MAX + 60:i * A.LOCAL_TREE_MAX + 64].view(np.uint32)[0])
+                        for i in range(n)))
+    r.close()
+    res = {
+        "workload": "chunks-of-1024x1024x256", "chunks": n, "device_ms": round(dt * 1e3, 3),
+        "device_chunks_per_s": round(n / dt, 1), "host_ms_per_chunk_1core": round(host_dt * 1e3, 3),
+        "host_chunks_per_s_1core": round(1.0 / host_dt, 1), "device_over_host_1core": round(host_dt * n / dt, 1),
+        "dense_GBps": round(n * 64 ** 3 * 2 / dt / 1e9, 1), "record_bytes": rec_bytes,
+        "checked_against_host": len(sample),
+    }
+    _log(f"chunk build: {n} chunks in {res['device_ms']} ms on the device; host {res['host_ms_per_chunk_1core']} ms/chunk")
     return res
 
 
@@ -407,6 +471,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving stream (workload 2 only)")
     ap.add_argument("--no-incoherent", action="store_true", help="skip the incoherent batch (workload 2 only)")
+    ap.add_argument("--no-build", action="store_true", help="skip the device chunk build (workload 2 only)")
     ap.add_argument("--serve-qps", type=float, default=1000.0, help="offered view requests per second")
     ap.add_argument("--serve-seconds", type=float, default=4.0, help="seconds of request arrivals")
     ap.add_argument("--serve-edit-rate", type=float, default=2.0, help="edit batches (1000 ops) per second")

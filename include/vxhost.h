@@ -16,6 +16,9 @@
  *                            (src/math.rs:69-117, src/render/mod.rs:82-138,1447-1564)
  *   vxw_pack_frame        <- the GpuFrame packing in Renderer::render
  *                            (src/render/mod.rs:1612-1710)
+ *   vxw_build_local_flat / vxw_insert_local_flat
+ *                         <- build_local / insert_local with a flat LocalTree record
+ *                            (the device build's output format, vx_local_tree)
  *   vxw_journal_parse     <- EditJournal::load (src/journal.rs:391-475)
  *   vxw_journal_write     <- EditJournal::save (src/journal.rs:477-508)
  *   vxw_generate          <- a seeded terrain generator with the reference's
@@ -145,6 +148,13 @@ int vxw_build_view(const vxw_world *w, const vxw_camera *cam, vx_gpu_chunk *chun
 int vxw_pack_frame(const vxw_camera *cam, const vxw_view *view, uint32_t max_pairs, vx_gpu_frame *out);
 /* render::pairs_for (src/render/mod.rs:27-30). */
 uint32_t vxw_pairs_for(uint32_t tiles_x, uint32_t tiles_y);
+
+/* build_local (src/voxel/tree.rs:89-204) of one dense chunk as a flat vx_local_tree record
+ * (the format vx_build_local writes); *written = record bytes (out may be NULL to learn
+ * the size, at most VX_LOCAL_TREE_MAX). */
+int vxw_build_local_flat(const uint16_t *dense, void *out, size_t cap, size_t *written);
+/* insert_local (src/voxel/world.rs:63-153) of a flat vx_local_tree record. */
+int vxw_insert_local_flat(vxw_world *w, int32_t cx, int32_t cy, int32_t cz, const void *rec, size_t bytes);
 
 /* ---- Edit journal files (VXJ1, src/journal.rs:13-139) -------------------------------
  * Header (24 B) | PlayerState (32 B, flags & 1) | BarState (24 B, flags & 4) | count

@@ -365,6 +365,50 @@ int vx_serve_stats_get(vx_serve *s, vx_serve_stats *out);
 const char *vx_error_string(int code);
 int vx_device_name(char *buf, size_t cap);
 
+/* ---- Device-side chunk build -------------------------------------------------------
+ * build_local (src/voxel/tree.rs:89-204) for many chunks at once on the GPU: dense 64^3
+ * block ids (x | z << 6 | y << 12) in, each chunk's LocalTree out as one flat record
+ * (vx_local_tree header, then n_l1 vx_local_l1, n_leaves u64 leaf masks, n_attrs u32
+ * attribute entries, n_palette u32 palette words; `bytes` is the whole record, a
+ * multiple of 8). The records are what the host's build_local gives for the same dense
+ * chunk (vxw_build_local_flat) and go into a world with vxw_insert_local_flat, which
+ * interns them as insert_local does (src/voxel/world.rs:63-153). No counterpart call in
+ * the reference, whose chunk builds run on CPU threads (src/stream.rs:119-146). */
+typedef struct vx_local_tree {
+    uint64_t root_mask, dry_mask;
+    uint32_t n_l1, n_leaves, n_attrs, n_palette;
+    uint32_t uniform;          /* the block id when flags & VX_LT_UNIFORM */
+    uint32_t flags;            /* VX_LT_* */
+    uint32_t emitters, solid_count, leaf_count;
+    uint8_t aabb_min[3], aabb_max[3], pad[2];
+    uint32_t bytes;
+} vx_local_tree;
+typedef struct vx_local_l1 {
+    uint64_t mask;
+    uint32_t leaf_start, leaf_prefix;
+    uint32_t full, water_line;
+} vx_local_l1;
+#define VX_LT_UNIFORM 1u
+#define VX_LT_WATER 2u
+#define VX_LT_FOLIAGE 4u
+#define VX_LT_CUTOUT 8u
+#define VX_LT_GLASS 16u
+#define VX_LT_EMITTER 32u
+/* Largest record: 64 L1 nodes, 4096 leaves, 4096 attribute entries, 49 palette words per leaf. */
+#define VX_LOCAL_TREE_MAX (64u + 64u * 24u + 4096u * 8u + 4096u * 4u + 4096u * 49u * 4u)
+/* dense: n chunks of 64^3 u16 in host memory; out: n slots of slot_bytes (at least
+ * VX_LOCAL_TREE_MAX) in host memory, record i at out + i * slot_bytes. */
+int vx_build_local(vx_ctx *ctx, const uint16_t *dense, uint32_t n, void *out, size_t slot_bytes);
+/* The same on device memory (dense and out on the context's device, 8-byte aligned),
+ * without copies: for callers that keep chunks on the GPU, and to time the build alone. */
+int vx_build_local_device(vx_ctx *ctx, const uint16_t *dense, uint32_t n, void *out, size_t slot_bytes);
+/* Device memory on the context's device for the *_device calls (no counterpart in the
+ * reference): allocate, free, and synchronous copies to and from host memory. */
+int vx_dev_alloc(vx_ctx *ctx, size_t bytes, void **out);
+int vx_dev_free(vx_ctx *ctx, void *p);
+int vx_dev_upload(vx_ctx *ctx, void *dst, const void *src, size_t bytes);
+int vx_dev_download(vx_ctx *ctx, void *dst, const void *src, size_t bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -113,6 +113,17 @@ assert C.sizeof(Edit) == 16
 EDIT_DTYPE = np.dtype([("x", "<i4"), ("y", "<i4"), ("z", "<i4"), ("id", "<u2"), ("pad", "<u2")])
 
 
+LOCAL_TREE_MAX = 64 + 64 * 24 + 4096 * 8 + 4096 * 4 + 4096 * 49 * 4  # VX_LOCAL_TREE_MAX
+
+
+class LocalTree(C.Structure):  # vx_local_tree (flat LocalTree record header)
+    _fields_ = [("root_mask", C.c_uint64), ("dry_mask", C.c_uint64), ("n_l1", C.c_uint32), ("n_leaves", C.c_uint32),
+                ("n_attrs", C.c_uint32), ("n_palette", C.c_uint32), ("uniform", C.c_uint32), ("flags", C.c_uint32),
+                ("emitters", C.c_uint32), ("solid_count", C.c_uint32), ("leaf_count", C.c_uint32),
+                ("aabb_min", C.c_uint8 * 3), ("aabb_max", C.c_uint8 * 3), ("pad", C.c_uint8 * 2),
+                ("bytes", C.c_uint32)]
+
+
 class Player(C.Structure):  # vxw_player (journal PlayerState, src/journal.rs:36-45)
     _fields_ = [("pos", C.c_float * 3), ("yaw", C.c_float), ("pitch", C.c_float), ("hotbar", C.c_uint32),
                 ("fly", C.c_uint32), ("pad", C.c_uint32)]
@@ -257,6 +268,8 @@ _sig_host = {
     "vxw_build_view": (C.c_int, [P, C.POINTER(Camera), P, P, C.POINTER(View)]),
     "vxw_pack_frame": (C.c_int, [C.POINTER(Camera), C.POINTER(View), C.c_uint32, C.POINTER(Frame)]),
     "vxw_pairs_for": (C.c_uint32, [C.c_uint32, C.c_uint32]),
+    "vxw_build_local_flat": (C.c_int, [P, P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "vxw_insert_local_flat": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, P, C.c_size_t]),
     "vxw_journal_parse": (C.c_int, [P, C.c_size_t, C.c_int32, C.c_int32, C.POINTER(JournalInfo), P, C.c_size_t,
                                     C.c_char_p, C.c_size_t]),
     "vxw_journal_write": (C.c_int, [P, C.c_size_t, C.c_int32, C.c_int32, C.POINTER(Player), P, P, C.c_size_t,
@@ -320,6 +333,12 @@ _sig_march = {
     "vx_serve_stats_get": (C.c_int, [P, C.POINTER(ServeStats)]),
     "vx_error_string": (C.c_char_p, [C.c_int]),
     "vx_device_name": (C.c_int, [C.c_char_p, C.c_size_t]),
+    "vx_build_local": (C.c_int, [P, P, C.c_uint32, P, C.c_size_t]),
+    "vx_build_local_device": (C.c_int, [P, P, C.c_uint32, P, C.c_size_t]),
+    "vx_dev_alloc": (C.c_int, [P, C.c_size_t, C.POINTER(P)]),
+    "vx_dev_free": (C.c_int, [P, P]),
+    "vx_dev_upload": (C.c_int, [P, P, P, C.c_size_t]),
+    "vx_dev_download": (C.c_int, [P, P, P, C.c_size_t]),
 }
 
 _host = None

@@ -77,6 +77,9 @@ struct vx_ctx {
     // Bumped whenever a pool's device address or length changes: captured graphs hold
     // both in their kernel arguments and are re-captured when it moves.
     uint64_t pool_gen = 0;
+    // Device chunk build (vx_build_local): dense input staging, per-leaf and per-L1
+    // scratch, record output staging; grown on demand, kept for the next call.
+    vxh::DevBuf build_dense, build_leaf, build_words, build_l1, build_out;
 };
 
 namespace vxh {

@@ -48,6 +48,20 @@ class Renderer:
     def set_tuning(self, occupancy: int = 4, variant: int = 0, ordered: int = 1):
         A.check(self._L.vx_set_tuning(self._c, C.byref(A.Tuning(occupancy, variant, ordered))), "vx_set_tuning")
 
+    def build_local(self, dense: np.ndarray) -> list[bytes]:
+        """build_local of n dense chunks ((n, 64^3) u16) on the device: n flat LocalTree
+        records, each equal to world.build_local_flat of the same chunk."""
+        d = np.ascontiguousarray(dense, dtype=np.uint16).reshape(-1, 64 ** 3)
+        n = d.shape[0]
+        out = np.zeros(n * A.LOCAL_TREE_MAX, dtype=np.uint8)
+        A.check(self._L.vx_build_local(self._c, A.ptr(d), n, A.ptr(out), A.LOCAL_TREE_MAX), "vx_build_local")
+        recs = []
+        for i in range(n):
+            slot = out[i * A.LOCAL_TREE_MAX:(i + 1) * A.LOCAL_TREE_MAX]
+            nbytes = int(slot[60:64].view(np.uint32)[0])
+            recs.append(slot[:nbytes].tobytes())
+        return recs
+
     def tuning(self) -> dict:
         t = A.Tuning()
         A.check(self._L.vx_get_tuning(self._c, C.byref(t)), "vx_get_tuning")

@@ -50,6 +50,11 @@ class World:
             raise ValueError("a chunk is 64^3 ids")
         A.check(self._L.vxw_insert_dense(self._h, *key, A.ptr(d)), "insert_dense")
 
+    def insert_local_flat(self, key, record: bytes):
+        """insert_local of a flat LocalTree record (build_local_flat, Renderer.build_local)."""
+        buf = C.create_string_buffer(bytes(record), len(record))
+        A.check(self._L.vxw_insert_local_flat(self._h, *key, buf, len(record)), "insert_local_flat")
+
     def remove_chunk(self, key) -> bool:
         return self._L.vxw_remove_chunk(self._h, *key) == 0
 
@@ -219,3 +224,17 @@ def build_view(world: World, cam: CameraParams, max_pairs: int = 0) -> ViewData:
     n = int(v.n_total)
     return ViewData(chunks[:max(n, 1)].copy() if n else np.zeros((1, 96), np.uint8), grid, fr,
                     int(v.n_visible), n, int(v.camera_chunk), cam.width, cam.height)
+
+
+def build_local_flat(dense: np.ndarray) -> bytes:
+    """build_local (src/voxel/tree.rs:89-204) of one dense chunk on the host, as the flat
+    vx_local_tree record the device build writes."""
+    L = A.host()
+    d = np.ascontiguousarray(dense, dtype=np.uint16).reshape(-1)
+    if d.size != 64 ** 3:
+        raise ValueError("a chunk is 64^3 ids")
+    n = C.c_size_t()
+    A.check(L.vxw_build_local_flat(A.ptr(d), None, 0, C.byref(n)), "build_local_flat")
+    out = C.create_string_buffer(n.value)
+    A.check(L.vxw_build_local_flat(A.ptr(d), out, n.value, C.byref(n)), "build_local_flat")
+    return out.raw[:n.value]
