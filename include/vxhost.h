@@ -14,6 +14,8 @@
  *                            (src/voxel/geometry.rs:74-221, src/voxel/attributes.rs:6-88)
  *   vxw_build_view        <- Frustum::from_camera + Renderer::build_chunk_list
  *                            (src/math.rs:69-117, src/render/mod.rs:82-138,1447-1564)
+ *   vxw_planner_*         <- lod::Planner (src/lod.rs:20-257)
+ *   vxw_build_view_items  <- Renderer::build_chunk_list over a LOD draw list
  *   vxw_pack_frame        <- the GpuFrame packing in Renderer::render
  *                            (src/render/mod.rs:1612-1710)
  *   vxw_build_local_flat / vxw_insert_local_flat
@@ -60,6 +62,8 @@ typedef struct vxw_chunk_info {
     uint32_t solid_count, leaf_count;
     uint32_t has_water, has_foliage, has_cutout, has_glass, has_emitter, emitters;
     uint64_t dry_mask;
+    uint32_t lod;                /* ChunkKey::lod: voxels of 2^lod blocks */
+    uint32_t pad;
 } vxw_chunk_info;
 
 typedef struct vxw_pool_stats {
@@ -80,10 +84,28 @@ int vxw_set_block(vxw_world *w, int32_t x, int32_t y, int32_t z, uint16_t id);
 int64_t vxw_apply_edits(vxw_world *w, const vx_edit *edits, size_t n);
 int vxw_extract_dense(const vxw_world *w, int32_t cx, int32_t cy, int32_t cz, uint16_t *out);
 int vxw_chunk_info_get(const vxw_world *w, int32_t cx, int32_t cy, int32_t cz, vxw_chunk_info *out);
+/* Resident chunks of every LOD. */
 size_t vxw_chunk_count(const vxw_world *w);
-/* Resident chunk keys in insertion order (the render set order); returns the count. */
+/* Resident LOD-0 chunk keys in insertion order (the render set order); returns their count. */
 size_t vxw_chunk_keys(const vxw_world *w, int32_t *keys_xyz, size_t cap);
 uint32_t vxw_emitter_chunks(const vxw_world *w);
+
+/* ---- chunks of any LOD (ChunkKey, src/voxel/chunk.rs:6-60) ----
+ * A key {pos, lod} is the chunk of 64^3 voxels of 2^lod blocks whose origin is
+ * pos * (64 << lod). Block edits (vxw_set_block, vxw_apply_edits) and vxw_get_block
+ * address LOD 0; a coarse chunk is inserted whole. */
+#define VXW_MAX_LOD 12u
+typedef struct vxw_key {
+    int32_t pos[3];
+    uint32_t lod;
+} vxw_key;
+
+int vxw_insert_dense_key(vxw_world *w, const vxw_key *key, const uint16_t *dense);
+int vxw_remove_chunk_key(vxw_world *w, const vxw_key *key);
+int vxw_extract_dense_key(const vxw_world *w, const vxw_key *key, uint16_t *out);
+int vxw_chunk_info_key(const vxw_world *w, const vxw_key *key, vxw_chunk_info *out);
+/* Resident chunk keys of every LOD in insertion order; returns the count. */
+size_t vxw_chunk_keys_all(const vxw_world *w, vxw_key *out, size_t cap);
 
 /* Pools: element sizes 16 (inners), 8 (leaves), 4 (bricks). */
 const void *vxw_pool_data(const vxw_world *w, int pool);
@@ -111,6 +133,10 @@ int vxw_generate(vxw_world *w, const vxw_gen_params *p);
 int32_t vxw_height(const vxw_gen_params *p, int32_t x, int32_t z);
 /* Generates one chunk's dense grid (x | z<<6 | y<<12) without inserting it. */
 int vxw_generate_dense(const vxw_gen_params *p, int32_t cx, int32_t cy, int32_t cz, uint16_t *out);
+/* The chunk of any LOD: LOD 0 as vxw_generate_dense; a coarse key gets a stand-in with
+ * the structure of WorldGen::generate_coarse (src/worldgen.rs:535-562): one height sample
+ * per coarse column, each voxel the block at the top of its span. */
+int vxw_generate_dense_key(const vxw_gen_params *p, const vxw_key *key, uint16_t *out);
 
 /* ---- per-view setup ---- */
 typedef struct vxw_camera {
@@ -148,6 +174,53 @@ int vxw_build_view(const vxw_world *w, const vxw_camera *cam, vx_gpu_chunk *chun
 int vxw_pack_frame(const vxw_camera *cam, const vxw_view *view, uint32_t max_pairs, vx_gpu_frame *out);
 /* render::pairs_for (src/render/mod.rs:27-30). */
 uint32_t vxw_pairs_for(uint32_t tiles_x, uint32_t tiles_y);
+
+/* ---- LOD planning (src/lod.rs:20-257) ----
+ * vxw_planner_plan <- Planner::plan (src/lod.rs:108-152): the draw list of a camera
+ * (chunks of several LODs; a chunk changing LOD draws beside its children with fade -f
+ * against their f) and the keys it wants resident. Residency comes from the world
+ * (w != NULL: absent, empty = no solid voxel, solid), else from fn(user, key), else
+ * every key is solid. A planner keeps the set of split keys between calls (the
+ * hysteresis of the no-cross-fade mode). */
+enum { VXW_FADE_SMOOTHSTEP = 0, VXW_FADE_SMOOTHERSTEP = 1 };
+enum { VXW_ABSENT = 0, VXW_EMPTY = 1, VXW_SOLID = 2 };
+
+typedef struct vxw_lod_config {      /* LodConfig (src/lod.rs:20-53) */
+    uint32_t max_lod;                /* 4 */
+    float factor;                    /* 2.0: a key splits inside size * factor of its centre */
+    float view_distance;             /* 2600 */
+    float fade_band;                 /* 1.25: the fade runs to size * factor * fade_band */
+    uint32_t fade_schedule;          /* VXW_FADE_* */
+    uint32_t cover_depth;            /* 2 */
+    uint32_t cross_fade;             /* 1 */
+    uint32_t shadow_share;           /* 1: the chunk grid prefers the larger share */
+    uint32_t offscreen_shadows;      /* 1: off-screen grid chunks cast shadows */
+} vxw_lod_config;
+
+typedef struct vxw_render_item {     /* RenderItem (src/lod.rs:55-81) */
+    vxw_key key;
+    float fade;                      /* 1 solid; f in (0,1) a fading child; -f its parent */
+} vxw_render_item;
+
+typedef int (*vxw_residency_fn)(void *user, const vxw_key *key);
+typedef struct vxw_planner vxw_planner;
+
+void vxw_lod_config_default(vxw_lod_config *cfg);
+vxw_planner *vxw_planner_new(void);
+void vxw_planner_free(vxw_planner *p);
+int vxw_planner_plan(vxw_planner *p, const float cam[3], const vxw_lod_config *cfg, const vxw_world *w,
+                     vxw_residency_fn fn, void *user);
+/* The last plan's draw list / wanted keys; return the count (out may be NULL). */
+size_t vxw_planner_render(const vxw_planner *p, vxw_render_item *out, size_t cap);
+size_t vxw_planner_desired(const vxw_planner *p, vxw_key *out, size_t cap);
+
+/* Renderer::build_chunk_list (src/render/mod.rs:1447-1564) over a draw list: items not
+ * resident or without solid voxels are dropped; GpuChunk voxel_size 2^lod, origin and
+ * fade from the item; the grid is filled in (share, -lod) order when cfg->shadow_share
+ * (cfg NULL: vxw_lod_config_default). vxw_build_view is this over every LOD-0 chunk in
+ * insertion order with fade 1. */
+int vxw_build_view_items(const vxw_world *w, const vxw_camera *cam, const vxw_render_item *items, size_t n,
+                         const vxw_lod_config *cfg, vx_gpu_chunk *chunks, uint32_t *grid, vxw_view *out);
 
 /* build_local (src/voxel/tree.rs:89-204) of one dense chunk as a flat vx_local_tree record
  * (the format vx_build_local writes); *written = record bytes (out may be NULL to learn

@@ -143,8 +143,31 @@ class ChunkInfo(C.Structure):
         ("solid_count", C.c_uint32), ("leaf_count", C.c_uint32),
         ("has_water", C.c_uint32), ("has_foliage", C.c_uint32), ("has_cutout", C.c_uint32),
         ("has_glass", C.c_uint32), ("has_emitter", C.c_uint32), ("emitters", C.c_uint32),
-        ("dry_mask", C.c_uint64),
+        ("dry_mask", C.c_uint64), ("lod", C.c_uint32), ("pad", C.c_uint32),
     ]
+
+
+class Key(C.Structure):  # vxw_key: ChunkKey {pos, lod}
+    _fields_ = [("pos", C.c_int32 * 3), ("lod", C.c_uint32)]
+
+
+class RenderItem(C.Structure):  # vxw_render_item
+    _fields_ = [("key", Key), ("fade", C.c_float)]
+
+
+RENDER_ITEM_DTYPE = np.dtype([("pos", "<i4", 3), ("lod", "<u4"), ("fade", "<f4")])
+KEY_DTYPE = np.dtype([("pos", "<i4", 3), ("lod", "<u4")])
+
+
+class LodConfig(C.Structure):  # vxw_lod_config
+    _fields_ = [("max_lod", C.c_uint32), ("factor", C.c_float), ("view_distance", C.c_float),
+                ("fade_band", C.c_float), ("fade_schedule", C.c_uint32), ("cover_depth", C.c_uint32),
+                ("cross_fade", C.c_uint32), ("shadow_share", C.c_uint32), ("offscreen_shadows", C.c_uint32)]
+
+
+RESIDENCY_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(Key))
+VXW_ABSENT, VXW_EMPTY, VXW_SOLID = 0, 1, 2
+VXW_MAX_LOD = 12
 
 
 class PoolStats(C.Structure):
@@ -270,6 +293,20 @@ _sig_host = {
     "vxw_pairs_for": (C.c_uint32, [C.c_uint32, C.c_uint32]),
     "vxw_build_local_flat": (C.c_int, [P, P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "vxw_insert_local_flat": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, P, C.c_size_t]),
+    "vxw_insert_dense_key": (C.c_int, [P, C.POINTER(Key), P]),
+    "vxw_remove_chunk_key": (C.c_int, [P, C.POINTER(Key)]),
+    "vxw_extract_dense_key": (C.c_int, [P, C.POINTER(Key), P]),
+    "vxw_chunk_info_key": (C.c_int, [P, C.POINTER(Key), C.POINTER(ChunkInfo)]),
+    "vxw_chunk_keys_all": (C.c_size_t, [P, P, C.c_size_t]),
+    "vxw_generate_dense_key": (C.c_int, [C.POINTER(GenParams), C.POINTER(Key), P]),
+    "vxw_lod_config_default": (None, [C.POINTER(LodConfig)]),
+    "vxw_planner_new": (P, []),
+    "vxw_planner_free": (None, [P]),
+    "vxw_planner_plan": (C.c_int, [P, P, C.POINTER(LodConfig), P, RESIDENCY_FN, P]),
+    "vxw_planner_render": (C.c_size_t, [P, P, C.c_size_t]),
+    "vxw_planner_desired": (C.c_size_t, [P, P, C.c_size_t]),
+    "vxw_build_view_items": (C.c_int, [P, C.POINTER(Camera), P, C.c_size_t, C.POINTER(LodConfig), P, P,
+                                       C.POINTER(View)]),
     "vxw_journal_parse": (C.c_int, [P, C.c_size_t, C.c_int32, C.c_int32, C.POINTER(JournalInfo), P, C.c_size_t,
                                     C.c_char_p, C.c_size_t]),
     "vxw_journal_write": (C.c_int, [P, C.c_size_t, C.c_int32, C.c_int32, C.POINTER(Player), P, P, C.c_size_t,

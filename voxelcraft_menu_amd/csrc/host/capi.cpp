@@ -45,13 +45,14 @@ int vxw_extract_dense(const vxw_world *w, int32_t cx, int32_t cy, int32_t cz, ui
     return w->w.extract_dense(Key{cx, cy, cz}, out) ? VXW_OK : VXW_E_MISSING;
 }
 
-int vxw_chunk_info_get(const vxw_world *w, int32_t cx, int32_t cy, int32_t cz, vxw_chunk_info *o) {
+static int chunk_info(const vxw_world *w, const Key &k, vxw_chunk_info *o) {
     if (!w || !o) return VXW_E_ARG;
-    auto it = w->w.chunks.find(Key{cx, cy, cz});
+    auto it = w->w.chunks.find(k);
     if (it == w->w.chunks.end()) return VXW_E_MISSING;
     const ChunkRecord &r = it->second;
     std::memset(o, 0, sizeof *o);
-    o->key[0] = cx; o->key[1] = cy; o->key[2] = cz;
+    o->key[0] = k.x; o->key[1] = k.y; o->key[2] = k.z;
+    o->lod = (uint32_t)k.lod;
     o->root[0] = (uint32_t)r.root.mask;
     o->root[1] = (uint32_t)(r.root.mask >> 32);
     o->root[2] = r.root.run_ptr;
@@ -75,16 +76,60 @@ int vxw_chunk_info_get(const vxw_world *w, int32_t cx, int32_t cy, int32_t cz, v
     return VXW_OK;
 }
 
+int vxw_chunk_info_get(const vxw_world *w, int32_t cx, int32_t cy, int32_t cz, vxw_chunk_info *o) {
+    return chunk_info(w, Key{cx, cy, cz}, o);
+}
+
 size_t vxw_chunk_count(const vxw_world *w) { return w ? w->w.chunks.size() : 0; }
 
 size_t vxw_chunk_keys(const vxw_world *w, int32_t *keys, size_t cap) {
     if (!w) return 0;
+    size_t n = 0;
+    for (const Key &k : w->w.order) {
+        if (k.lod != 0) continue;
+        if (keys && n < cap) {
+            keys[3 * n] = k.x;
+            keys[3 * n + 1] = k.y;
+            keys[3 * n + 2] = k.z;
+        }
+        n++;
+    }
+    return n;
+}
+
+// ---- chunks of any LOD ----
+
+static bool key_ok(const vxw_key *k) { return k && k->lod <= VXW_MAX_LOD; }
+static Key key_of(const vxw_key *k) { return Key{k->pos[0], k->pos[1], k->pos[2], (int32_t)k->lod}; }
+
+int vxw_insert_dense_key(vxw_world *w, const vxw_key *k, const uint16_t *dense) {
+    if (!w || !dense || !key_ok(k)) return VXW_E_ARG;
+    w->w.insert_local(key_of(k), build_local(dense));
+    return VXW_OK;
+}
+
+int vxw_remove_chunk_key(vxw_world *w, const vxw_key *k) {
+    if (!w || !key_ok(k)) return VXW_E_ARG;
+    return w->w.remove_chunk(key_of(k)) ? VXW_OK : VXW_E_MISSING;
+}
+
+int vxw_extract_dense_key(const vxw_world *w, const vxw_key *k, uint16_t *out) {
+    if (!w || !out || !key_ok(k)) return VXW_E_ARG;
+    return w->w.extract_dense(key_of(k), out) ? VXW_OK : VXW_E_MISSING;
+}
+
+int vxw_chunk_info_key(const vxw_world *w, const vxw_key *k, vxw_chunk_info *o) {
+    if (!key_ok(k)) return VXW_E_ARG;
+    return chunk_info(w, key_of(k), o);
+}
+
+size_t vxw_chunk_keys_all(const vxw_world *w, vxw_key *out, size_t cap) {
+    if (!w) return 0;
     size_t n = w->w.order.size();
-    if (keys)
+    if (out)
         for (size_t i = 0; i < n && i < cap; i++) {
-            keys[3 * i] = w->w.order[i].x;
-            keys[3 * i + 1] = w->w.order[i].y;
-            keys[3 * i + 2] = w->w.order[i].z;
+            const Key &k = w->w.order[i];
+            out[i] = vxw_key{{k.x, k.y, k.z}, (uint32_t)k.lod};
         }
     return n;
 }

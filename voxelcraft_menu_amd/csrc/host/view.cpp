@@ -66,10 +66,141 @@ struct Frustum {
 };
 
 Aabb world_aabb(const ChunkRecord &rec) {
-    // ChunkRecord::world_aabb (src/voxel/chunk.rs:100-110), LOD 0: voxel size 1.
-    V3 o{(float)(rec.key.x * 64), (float)(rec.key.y * 64), (float)(rec.key.z * 64)};
-    return Aabb{add(o, V3{(float)rec.aabb_min[0], (float)rec.aabb_min[1], (float)rec.aabb_min[2]}),
-                add(o, V3{(float)rec.aabb_max[0], (float)rec.aabb_max[1], (float)rec.aabb_max[2]})};
+    // ChunkRecord::world_aabb (src/voxel/chunk.rs:100-110): origin + local bounds * voxel size.
+    const Key &k = rec.key;
+    const float vs = (float)k.voxel_size();
+    V3 o{(float)(k.x * k.size()), (float)(k.y * k.size()), (float)(k.z * k.size())};
+    return Aabb{add(o, V3{(float)rec.aabb_min[0] * vs, (float)rec.aabb_min[1] * vs, (float)rec.aabb_min[2] * vs}),
+                add(o, V3{(float)rec.aabb_max[0] * vs, (float)rec.aabb_max[1] * vs, (float)rec.aabb_max[2] * vs})};
+}
+
+struct DrawItem {
+    Key key;
+    float fade;
+    float share() const { return fade >= 0.0f ? fade : 1.0f + fade; }
+};
+
+// f32::total_cmp's order as a signed integer.
+int32_t total_order(float f) {
+    int32_t b;
+    std::memcpy(&b, &f, 4);
+    return b ^ (int32_t)((uint32_t)(b >> 31) >> 1);
+}
+
+// partition_for_grid + build_chunk_list (src/render/mod.rs:82-138, :1447-1564).
+int build_view(const World &w, const vxw_camera *cam, const std::vector<DrawItem> &items, bool shadow_share,
+               bool offscreen_shadows, vx_gpu_chunk *chunks, uint32_t *grid, vxw_view *out) {
+    V3 pos{cam->pos[0], cam->pos[1], cam->pos[2]};
+    V3 fwd{cam->fwd[0], cam->fwd[1], cam->fwd[2]};
+    V3 right{cam->right[0], cam->right[1], cam->right[2]};
+    V3 up{cam->up[0], cam->up[1], cam->up[2]};
+    float tan_half = std::tan(cam->fov_y * 0.5f);
+    float aspect = (float)cam->width / (float)cam->height;
+    Frustum fr(pos, fwd, right, up, tan_half, aspect, cam->far_);
+    // grid_origin (src/render/mod.rs:88-91)
+    const int32_t gmin[3] = {(int32_t)std::floor(pos.x / 64.0f) - (int32_t)VX_GRID_X / 2, 0,
+                             (int32_t)std::floor(pos.z / 64.0f) - (int32_t)VX_GRID_Z / 2};
+    const int32_t gdim[3] = {(int32_t)VX_GRID_X, (int32_t)VX_GRID_Y, (int32_t)VX_GRID_Z};
+    // grid_span (src/render/mod.rs:93-108): the grid cells a key covers, [lo, hi).
+    auto grid_span = [&](const Key &k, int32_t lo[3], int32_t hi[3]) {
+        const int32_t span = 1 << k.lod, p[3] = {k.x, k.y, k.z};
+        bool any = true;
+        for (int i = 0; i < 3; i++) {
+            const int32_t base = p[i] * span - gmin[i];
+            lo[i] = std::max(base, 0);
+            hi[i] = std::min(base + span, gdim[i]);
+            any = any && lo[i] < hi[i];
+        }
+        return any;
+    };
+    struct Item {
+        float d;
+        DrawItem it;
+        const ChunkRecord *rec;
+    };
+    std::vector<Item> visible, offscreen;
+    for (const DrawItem &it : items) {
+        auto f = w.chunks.find(it.key);
+        if (f == w.chunks.end() || f->second.solid_count == 0) continue;
+        const ChunkRecord &rec = f->second;
+        Aabb a = world_aabb(rec);
+        int32_t lo[3], hi[3];
+        if (fr.contains(a)) visible.push_back({a.distance_to(pos), it, &rec});
+        else if (offscreen_shadows && grid_span(it.key, lo, hi)) offscreen.push_back({a.distance_to(pos), it, &rec});
+    }
+    auto by_d = [](const Item &a, const Item &b) { return a.d < b.d; };
+    std::stable_sort(visible.begin(), visible.end(), by_d);
+    if (visible.size() > VX_MAX_CHUNKS) visible.resize(VX_MAX_CHUNKS);
+    std::stable_sort(offscreen.begin(), offscreen.end(), by_d);
+    if (offscreen.size() > VX_MAX_CHUNKS - visible.size()) offscreen.resize(VX_MAX_CHUNKS - visible.size());
+
+    uint32_t camera_chunk = VX_NO_CHUNK;
+    uint32_t idx = 0;
+    std::vector<DrawItem> grid_set;
+    grid_set.reserve(visible.size() + offscreen.size());
+    auto emit = [&](const Item &it) {
+        const ChunkRecord &rec = *it.rec;
+        const Key &k = rec.key;
+        Aabb a = world_aabb(rec);
+        if (idx < visible.size() && camera_chunk == VX_NO_CHUNK && pos.x >= a.min.x && pos.y >= a.min.y &&
+            pos.z >= a.min.z && pos.x <= a.max.x && pos.y <= a.max.y && pos.z <= a.max.z)
+            camera_chunk = idx;
+        vx_gpu_chunk &g = chunks[idx];
+        std::memset(&g, 0, sizeof g);
+        g.aabb_min[0] = a.min.x; g.aabb_min[1] = a.min.y; g.aabb_min[2] = a.min.z;
+        g.voxel_size = (float)k.voxel_size();
+        g.aabb_max[0] = a.max.x; g.aabb_max[1] = a.max.y; g.aabb_max[2] = a.max.z;
+        g.lod = (uint32_t)k.lod;
+        g.origin[0] = (float)(k.x * k.size()); g.origin[1] = (float)(k.y * k.size());
+        g.origin[2] = (float)(k.z * k.size());
+        g.fade = it.it.fade;
+        g.root[0] = (uint32_t)rec.root.mask;
+        g.root[1] = (uint32_t)(rec.root.mask >> 32);
+        g.root[2] = rec.root.run_ptr;
+        g.root[3] = rec.root.leaf_prefix;
+        uint32_t flags = rec.attr_uniform ? 1u : 0u;
+        if (rec.has_water) flags |= 2u;
+        if (rec.has_foliage) flags |= 4u;
+        if (rec.has_cutout) flags |= 8u;
+        if (rec.has_emitter) flags |= 32u;
+        if (rec.has_glass) flags |= 16u;
+        g.attr_base = rec.attr_base;
+        g.attr_flags = flags;
+        g.light_base = 0xF000u;  // LightRef::Uniform(0xF000): lighting is not on this path
+        g.light_flags = 1u;
+        g.dry_mask[0] = (uint32_t)rec.dry_mask;
+        g.dry_mask[1] = (uint32_t)(rec.dry_mask >> 32);
+        grid_set.push_back(it.it);
+        idx++;
+    };
+    for (const auto &it : visible) emit(it);
+    for (const auto &it : offscreen) emit(it);
+
+    // Grid fill (mod.rs:1528-1555): a stable sort by (share, -lod), later items
+    // overwriting earlier ones, so the largest share and then the finest LOD wins a cell.
+    std::fill(grid, grid + VX_GRID_LEN, VX_NO_CHUNK);
+    std::vector<uint32_t> order(grid_set.size());
+    for (uint32_t i = 0; i < order.size(); i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        const float sa = shadow_share ? grid_set[a].share() : 1.0f;
+        const float sb = shadow_share ? grid_set[b].share() : 1.0f;
+        const int32_t ta = total_order(sa), tb = total_order(sb);
+        if (ta != tb) return ta < tb;
+        return grid_set[b].key.lod < grid_set[a].key.lod;
+    });
+    for (uint32_t i : order) {
+        int32_t lo[3], hi[3];
+        if (!grid_span(grid_set[i].key, lo, hi)) continue;
+        for (int32_t z = lo[2]; z < hi[2]; z++)
+            for (int32_t y = lo[1]; y < hi[1]; y++)
+                for (int32_t x = lo[0]; x < hi[0]; x++)
+                    grid[(uint32_t)x + (uint32_t)y * VX_GRID_X + (uint32_t)z * VX_GRID_X * VX_GRID_Y] = i;
+    }
+    out->n_visible = (uint32_t)visible.size();
+    out->n_total = idx;
+    out->camera_chunk = camera_chunk;
+    std::memcpy(out->grid_min, gmin, sizeof gmin);
+    return VXW_OK;
 }
 
 }  // namespace
@@ -98,97 +229,37 @@ extern "C" uint32_t vxw_pairs_for(uint32_t tx, uint32_t ty) {
     return (uint32_t)want;
 }
 
-// partition_for_grid + build_chunk_list (src/render/mod.rs:82-138, :1447-1564), LOD 0,
-// fade 1.0, offscreen_shadows and shadow_share on (src/lod.rs:40-52).
+// build_chunk_list over the LOD-0 chunks in insertion order, fade 1.0, with
+// offscreen_shadows and shadow_share on (src/lod.rs:40-52).
 extern "C" int vxw_build_view(const vxw_world *wh, const vxw_camera *cam, vx_gpu_chunk *chunks, uint32_t *grid,
                               vxw_view *out) {
     if (!wh || !cam || !chunks || !grid || !out) return VXW_E_ARG;
     if (cam->width == 0 || cam->height == 0) return VXW_E_ARG;
-    const World &w = wh->w;
-    V3 pos{cam->pos[0], cam->pos[1], cam->pos[2]};
-    V3 fwd{cam->fwd[0], cam->fwd[1], cam->fwd[2]};
-    V3 right{cam->right[0], cam->right[1], cam->right[2]};
-    V3 up{cam->up[0], cam->up[1], cam->up[2]};
-    float tan_half = std::tan(cam->fov_y * 0.5f);
-    float aspect = (float)cam->width / (float)cam->height;
-    Frustum fr(pos, fwd, right, up, tan_half, aspect, cam->far_);
-    // grid_origin (src/render/mod.rs:88-91)
-    int32_t gmin[3] = {(int32_t)std::floor(pos.x / 64.0f) - (int32_t)VX_GRID_X / 2, 0,
-                       (int32_t)std::floor(pos.z / 64.0f) - (int32_t)VX_GRID_Z / 2};
-    auto grid_cell = [&](const Key &k, int32_t c[3]) {
-        c[0] = k.x - gmin[0];
-        c[1] = k.y - gmin[1];
-        c[2] = k.z - gmin[2];
-        return c[0] >= 0 && c[1] >= 0 && c[2] >= 0 && c[0] < (int32_t)VX_GRID_X && c[1] < (int32_t)VX_GRID_Y &&
-               c[2] < (int32_t)VX_GRID_Z;
-    };
-    struct Item {
-        float d;
-        const ChunkRecord *rec;
-    };
-    std::vector<Item> visible, offscreen;
-    for (const Key &k : w.order) {
-        const ChunkRecord &rec = w.chunks.at(k);
-        if (rec.solid_count == 0) continue;
-        Aabb a = world_aabb(rec);
-        int32_t c[3];
-        if (fr.contains(a)) visible.push_back({a.distance_to(pos), &rec});
-        else if (grid_cell(k, c)) offscreen.push_back({a.distance_to(pos), &rec});
-    }
-    auto by_d = [](const Item &a, const Item &b) { return a.d < b.d; };
-    std::stable_sort(visible.begin(), visible.end(), by_d);
-    if (visible.size() > VX_MAX_CHUNKS) visible.resize(VX_MAX_CHUNKS);
-    std::stable_sort(offscreen.begin(), offscreen.end(), by_d);
-    if (offscreen.size() > VX_MAX_CHUNKS - visible.size()) offscreen.resize(VX_MAX_CHUNKS - visible.size());
+    std::vector<DrawItem> items;
+    items.reserve(wh->w.order.size());
+    for (const Key &k : wh->w.order)
+        if (k.lod == 0) items.push_back(DrawItem{k, 1.0f});
+    return build_view(wh->w, cam, items, true, true, chunks, grid, out);
+}
 
-    uint32_t camera_chunk = VX_NO_CHUNK;
-    uint32_t idx = 0;
-    std::fill(grid, grid + VX_GRID_LEN, VX_NO_CHUNK);
-    auto emit = [&](const Item &it) {
-        const ChunkRecord &rec = *it.rec;
-        Aabb a = world_aabb(rec);
-        if (idx < visible.size() && camera_chunk == VX_NO_CHUNK && pos.x >= a.min.x && pos.y >= a.min.y &&
-            pos.z >= a.min.z && pos.x <= a.max.x && pos.y <= a.max.y && pos.z <= a.max.z)
-            camera_chunk = idx;
-        vx_gpu_chunk &g = chunks[idx];
-        std::memset(&g, 0, sizeof g);
-        g.aabb_min[0] = a.min.x; g.aabb_min[1] = a.min.y; g.aabb_min[2] = a.min.z;
-        g.voxel_size = 1.0f;
-        g.aabb_max[0] = a.max.x; g.aabb_max[1] = a.max.y; g.aabb_max[2] = a.max.z;
-        g.lod = 0;
-        g.origin[0] = (float)(rec.key.x * 64); g.origin[1] = (float)(rec.key.y * 64);
-        g.origin[2] = (float)(rec.key.z * 64);
-        g.fade = 1.0f;
-        g.root[0] = (uint32_t)rec.root.mask;
-        g.root[1] = (uint32_t)(rec.root.mask >> 32);
-        g.root[2] = rec.root.run_ptr;
-        g.root[3] = rec.root.leaf_prefix;
-        uint32_t flags = rec.attr_uniform ? 1u : 0u;
-        if (rec.has_water) flags |= 2u;
-        if (rec.has_foliage) flags |= 4u;
-        if (rec.has_cutout) flags |= 8u;
-        if (rec.has_emitter) flags |= 32u;
-        if (rec.has_glass) flags |= 16u;
-        g.attr_base = rec.attr_base;
-        g.attr_flags = flags;
-        g.light_base = 0xF000u;  // LightRef::Uniform(0xF000): lighting is not on this path
-        g.light_flags = 1u;
-        g.dry_mask[0] = (uint32_t)rec.dry_mask;
-        g.dry_mask[1] = (uint32_t)(rec.dry_mask >> 32);
-        // Grid fill: every item has share 1.0 and LOD 0, so the sort by (share, -lod)
-        // is the identity and a later item overwrites an earlier one (mod.rs:1531-1555).
-        int32_t c[3];
-        if (grid_cell(rec.key, c))
-            grid[(uint32_t)c[0] + (uint32_t)c[1] * VX_GRID_X + (uint32_t)c[2] * VX_GRID_X * VX_GRID_Y] = idx;
-        idx++;
-    };
-    for (const auto &it : visible) emit(it);
-    for (const auto &it : offscreen) emit(it);
-    out->n_visible = (uint32_t)visible.size();
-    out->n_total = idx;
-    out->camera_chunk = camera_chunk;
-    std::memcpy(out->grid_min, gmin, sizeof gmin);
-    return VXW_OK;
+extern "C" int vxw_build_view_items(const vxw_world *wh, const vxw_camera *cam, const vxw_render_item *items,
+                                    size_t n, const vxw_lod_config *cfg, vx_gpu_chunk *chunks, uint32_t *grid,
+                                    vxw_view *out) {
+    if (!wh || !cam || !chunks || !grid || !out || (!items && n)) return VXW_E_ARG;
+    if (cam->width == 0 || cam->height == 0) return VXW_E_ARG;
+    vxw_lod_config def;
+    if (!cfg) {
+        vxw_lod_config_default(&def);
+        cfg = &def;
+    }
+    std::vector<DrawItem> v;
+    v.reserve(n);
+    for (size_t i = 0; i < n; i++) {
+        const vxw_render_item &it = items[i];
+        if (it.key.lod > VXW_MAX_LOD) return VXW_E_ARG;
+        v.push_back(DrawItem{Key{it.key.pos[0], it.key.pos[1], it.key.pos[2], (int32_t)it.key.lod}, it.fade});
+    }
+    return build_view(wh->w, cam, v, cfg->shadow_share != 0, cfg->offscreen_shadows != 0, chunks, grid, out);
 }
 
 // GpuFrame packing (src/render/mod.rs:1612-1710); shading-only fields stay zero.

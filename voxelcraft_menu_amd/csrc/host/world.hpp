@@ -150,15 +150,23 @@ private:
     std::vector<std::pair<uint32_t, uint32_t>> dirty_;
 };
 
+// ChunkKey (src/voxel/chunk.rs:6-60): a chunk of 64^3 voxels of size 2^lod at
+// position pos, covering [pos * (64 << lod), (pos + 1) * (64 << lod)) in blocks.
 struct Key {
     int32_t x, y, z;
-    bool operator==(const Key &o) const { return x == o.x && y == o.y && z == o.z; }
+    int32_t lod = 0;
+    bool operator==(const Key &o) const { return x == o.x && y == o.y && z == o.z && lod == o.lod; }
+    int32_t size() const { return 64 << lod; }
+    int32_t voxel_size() const { return 1 << lod; }
+    Key parent() const { return Key{x >> 1, y >> 1, z >> 1, lod + 1}; }
+    Key child(int32_t dx, int32_t dy, int32_t dz) const { return Key{x * 2 + dx, y * 2 + dy, z * 2 + dz, lod - 1}; }
 };
 struct KeyHash {
     size_t operator()(const Key &k) const {
         uint64_t h = (uint64_t)(uint32_t)k.x * 0x9E3779B97F4A7C15ull;
         h ^= (uint64_t)(uint32_t)k.y * 0xC2B2AE3D27D4EB4Full + (h << 6);
         h ^= (uint64_t)(uint32_t)k.z * 0x165667B19E3779F9ull + (h >> 2);
+        h ^= (uint64_t)(uint32_t)k.lod * 0xD6E8FEB86659FD93ull + (h << 3);
         return (size_t)h;
     }
 };

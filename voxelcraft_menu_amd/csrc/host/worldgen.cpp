@@ -218,6 +218,35 @@ struct Gen {
                 }
         }
     }
+
+    // A coarse stand-in (lod > 0) with generate_coarse's structure (src/worldgen.rs:
+    // 535-562): one height sample per coarse column, each cell given the block at the
+    // top of its span below the column's ceiling (the terrain or the sea surface). The
+    // surface cell takes the LOD-0 column's glass / leaf pick; caves and trees are
+    // left out of stand-ins.
+    void coarse(const Key &k, BlockId *out) const {
+        std::fill(out, out + VOL, AIR);
+        const int32_t s = k.voxel_size(), span = k.size();
+        const int32_t ox = k.x * span, oy = k.y * span, oz = k.z * span;
+        for (int z = 0; z < 64; z++)
+            for (int x = 0; x < 64; x++) {
+                const int32_t wx = ox + x * s, wz = oz + z * s;
+                const int32_t h = height(wx, wz);
+                const int32_t ceiling = std::max(h, p.sea_level);
+                const float r = unit01(hash3i(wx, 77, wz, p.seed));
+                for (int y = 0; y < 64; y++) {
+                    const int32_t wy = oy + y * s;
+                    if (wy >= ceiling) break;
+                    const int32_t top = std::min(wy + s - 1, ceiling - 1);
+                    BlockId b = column_block(top, h);
+                    if (top == h - 1 && b != AIR && b != WATER && h > p.sea_level) {
+                        if (r < p.glass_fraction) b = GLASS;
+                        else if (r < p.glass_fraction + p.leaf_extra) b = LEAVES;
+                    }
+                    out[dense_index((uint32_t)x, (uint32_t)y, (uint32_t)z)] = b;
+                }
+            }
+    }
 };
 
 }  // namespace vxh
@@ -236,6 +265,14 @@ extern "C" int vxw_generate_dense(const vxw_gen_params *p, int32_t cx, int32_t c
     if (!p || !out) return VXW_E_ARG;
     Gen g(*p);
     g.chunk(cx, cy, cz, out);
+    return VXW_OK;
+}
+
+extern "C" int vxw_generate_dense_key(const vxw_gen_params *p, const vxw_key *k, uint16_t *out) {
+    if (!p || !k || !out || k->lod > VXW_MAX_LOD) return VXW_E_ARG;
+    Gen g(*p);
+    if (k->lod == 0) g.chunk(k->pos[0], k->pos[1], k->pos[2], out);
+    else g.coarse(Key{k->pos[0], k->pos[1], k->pos[2], (int32_t)k->lod}, out);
     return VXW_OK;
 }
 

@@ -44,19 +44,28 @@ class World:
         return self._h
 
     # ---- store ----
-    def insert_dense(self, key, dense: np.ndarray):
+    # A key is a chunk position (x, y, z); lod > 0 names the chunk of 64^3 voxels of
+    # 2^lod blocks at that position (ChunkKey, src/voxel/chunk.rs:6-60).
+    def insert_dense(self, key, dense: np.ndarray, lod: int = 0):
         d = np.ascontiguousarray(dense, dtype=np.uint16).reshape(-1)
         if d.size != 64 ** 3:
             raise ValueError("a chunk is 64^3 ids")
-        A.check(self._L.vxw_insert_dense(self._h, *key, A.ptr(d)), "insert_dense")
+        if lod:
+            A.check(self._L.vxw_insert_dense_key(self._h, C.byref(chunk_key(key, lod)), A.ptr(d)), "insert_dense")
+        else:
+            A.check(self._L.vxw_insert_dense(self._h, *key, A.ptr(d)), "insert_dense")
 
     def insert_local_flat(self, key, record: bytes):
         """insert_local of a flat LocalTree record (build_local_flat, Renderer.build_local)."""
         buf = C.create_string_buffer(bytes(record), len(record))
         A.check(self._L.vxw_insert_local_flat(self._h, *key, buf, len(record)), "insert_local_flat")
 
-    def remove_chunk(self, key) -> bool:
-        return self._L.vxw_remove_chunk(self._h, *key) == 0
+    def remove_chunk(self, key, lod: int = 0) -> bool:
+        return self._L.vxw_remove_chunk_key(self._h, C.byref(chunk_key(key, lod))) == 0
+
+    def contains(self, key, lod: int = 0) -> bool:
+        info = A.ChunkInfo()
+        return self._L.vxw_chunk_info_key(self._h, C.byref(chunk_key(key, lod)), C.byref(info)) == 0
 
     def get_block(self, x, y, z) -> int:
         return int(self._L.vxw_get_block(self._h, x, y, z))
@@ -68,20 +77,29 @@ class World:
         e = np.ascontiguousarray(edits, dtype=A.EDIT_DTYPE)
         return int(self._L.vxw_apply_edits(self._h, A.ptr(e), e.size))
 
-    def extract_dense(self, key) -> np.ndarray:
+    def extract_dense(self, key, lod: int = 0) -> np.ndarray:
         out = np.zeros(64 ** 3, dtype=np.uint16)
-        A.check(self._L.vxw_extract_dense(self._h, *key, A.ptr(out)), "extract_dense")
+        A.check(self._L.vxw_extract_dense_key(self._h, C.byref(chunk_key(key, lod)), A.ptr(out)), "extract_dense")
         return out
 
-    def chunk_info(self, key) -> A.ChunkInfo:
+    def chunk_info(self, key, lod: int = 0) -> A.ChunkInfo:
         info = A.ChunkInfo()
-        A.check(self._L.vxw_chunk_info_get(self._h, *key, C.byref(info)), "chunk_info")
+        A.check(self._L.vxw_chunk_info_key(self._h, C.byref(chunk_key(key, lod)), C.byref(info)), "chunk_info")
         return info
 
     def keys(self) -> np.ndarray:
+        """LOD-0 chunk positions in insertion order, (n, 3) int32."""
         n = self._L.vxw_chunk_keys(self._h, None, 0)
         out = np.zeros((n, 3), dtype=np.int32)
         self._L.vxw_chunk_keys(self._h, A.ptr(out), n)
+        return out
+
+    def keys_all(self) -> np.ndarray:
+        """Chunk keys of every LOD in insertion order (KEY_DTYPE records)."""
+        n = self._L.vxw_chunk_keys_all(self._h, None, 0)
+        out = np.zeros(n, dtype=A.KEY_DTYPE)
+        if n:
+            self._L.vxw_chunk_keys_all(self._h, A.ptr(out), n)
         return out
 
     def __len__(self):
@@ -141,10 +159,18 @@ class GenParams:
     def height(self, x: int, z: int) -> int:
         return int(A.host().vxw_height(C.byref(self.to_c()), x, z))
 
-    def dense(self, key) -> np.ndarray:
+    def dense(self, key, lod: int = 0) -> np.ndarray:
+        """The chunk at key; a coarse key gets a stand-in sampled one column per voxel."""
         out = np.zeros(64 ** 3, dtype=np.uint16)
-        A.check(A.host().vxw_generate_dense(C.byref(self.to_c()), *key, A.ptr(out)), "generate_dense")
+        A.check(A.host().vxw_generate_dense_key(C.byref(self.to_c()), C.byref(chunk_key(key, lod)), A.ptr(out)),
+                "generate_dense")
         return out
+
+
+def chunk_key(key, lod: int = 0) -> A.Key:
+    if not 0 <= lod <= A.VXW_MAX_LOD:
+        raise ValueError(f"lod {lod} outside 0..{A.VXW_MAX_LOD}")
+    return A.Key((C.c_int32 * 3)(*(int(v) for v in key)), lod)
 
 
 def camera_basis(yaw: float, pitch: float):
@@ -219,6 +245,116 @@ def build_view(world: World, cam: CameraParams, max_pairs: int = 0) -> ViewData:
     grid = np.zeros(A.VX_GRID_LEN, dtype=np.uint32)
     v = A.View()
     A.check(L.vxw_build_view(world.handle, C.byref(c), A.ptr(chunks), A.ptr(grid), C.byref(v)), "build_view")
+    fr = A.Frame()
+    A.check(L.vxw_pack_frame(C.byref(c), C.byref(v), max_pairs, C.byref(fr)), "pack_frame")
+    n = int(v.n_total)
+    return ViewData(chunks[:max(n, 1)].copy() if n else np.zeros((1, 96), np.uint8), grid, fr,
+                    int(v.n_visible), n, int(v.camera_chunk), cam.width, cam.height)
+
+
+@dataclass
+class LodConfig:
+    """LodConfig (src/lod.rs:20-53) with its defaults."""
+    max_lod: int = 4
+    factor: float = 2.0
+    view_distance: float = 2600.0
+    fade_band: float = 1.25
+    smootherstep: bool = False
+    cover_depth: int = 2
+    cross_fade: bool = True
+    shadow_share: bool = True
+    offscreen_shadows: bool = True
+
+    def to_c(self) -> A.LodConfig:
+        return A.LodConfig(self.max_lod, self.factor, self.view_distance, self.fade_band, int(self.smootherstep),
+                           self.cover_depth, int(self.cross_fade), int(self.shadow_share),
+                           int(self.offscreen_shadows))
+
+
+class Planner:
+    """lod::Planner (src/lod.rs:100-152): a camera's draw list over chunks of several LODs.
+
+    ``plan`` returns (render, desired): RENDER_ITEM_DTYPE records (fade 1 solid, f a
+    fading child, -f its parent) and the KEY_DTYPE keys the plan wants resident.
+    Residency is read from ``world``, else from ``status(pos, lod) -> VXW_*``, else
+    every key is solid. The planner keeps its split set between calls.
+    """
+
+    def __init__(self):
+        self._L = A.host()
+        self._h = self._L.vxw_planner_new()
+        if not self._h:
+            raise MemoryError("vxw_planner_new")
+
+    def close(self):
+        if self._h:
+            self._L.vxw_planner_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def plan(self, cam, cfg: LodConfig | None = None, world: World | None = None, status=None):
+        cfg = cfg or LodConfig()
+        cam_a = (C.c_float * 3)(*cam)
+        fn = A.RESIDENCY_FN()
+        if world is None and status is not None:
+            fn = A.RESIDENCY_FN(lambda _u, k: int(status(tuple(k.contents.pos), int(k.contents.lod))))
+        A.check(self._L.vxw_planner_plan(self._h, cam_a, C.byref(cfg.to_c()), world.handle if world else None,
+                                         fn, None), "planner_plan")
+        n = self._L.vxw_planner_render(self._h, None, 0)
+        render = np.zeros(n, dtype=A.RENDER_ITEM_DTYPE)
+        if n:
+            self._L.vxw_planner_render(self._h, A.ptr(render), n)
+        m = self._L.vxw_planner_desired(self._h, None, 0)
+        desired = np.zeros(m, dtype=A.KEY_DTYPE)
+        if m:
+            self._L.vxw_planner_desired(self._h, A.ptr(desired), m)
+        return render, desired
+
+
+def settle(world: World, gen: GenParams, planner: Planner, cam, cfg: LodConfig | None = None,
+           max_rounds: int = 64):
+    """Stream chunks until the plan at ``cam`` is resident: each round plans, then
+    generates and inserts every wanted key (and its ancestors up to max_lod) that is not
+    resident, coarse levels first (the request order of ChunkManager::update,
+    src/stream.rs:252-313, without its worker threads, budget or retention). Returns the
+    settled draw list."""
+    cfg = cfg or LodConfig()
+    for _ in range(max_rounds):
+        render, desired = planner.plan(cam, cfg, world=world)
+        missing = {}
+        for k in desired:
+            pos, lod = tuple(int(v) for v in k["pos"]), int(k["lod"])
+            while True:
+                if not world.contains(pos, lod) and (pos, lod) not in missing:
+                    missing[(pos, lod)] = None
+                if lod >= cfg.max_lod:
+                    break
+                pos, lod = tuple(v >> 1 for v in pos), lod + 1
+        if not missing:
+            return render
+        for pos, lod in sorted(missing, key=lambda pk: -pk[1]):
+            world.insert_dense(pos, gen.dense(pos, lod), lod)
+    raise RuntimeError("the plan never settled")
+
+
+def build_view_items(world: World, cam: CameraParams, items: np.ndarray, cfg: LodConfig | None = None,
+                     max_pairs: int = 0) -> ViewData:
+    """Renderer::build_chunk_list over a LOD draw list (RENDER_ITEM_DTYPE records) + the
+    GpuFrame packing."""
+    L = A.host()
+    c = cam.to_c()
+    it = np.ascontiguousarray(items, dtype=A.RENDER_ITEM_DTYPE)
+    chunks = np.zeros((A.VX_MAX_CHUNKS, 96), dtype=np.uint8)
+    grid = np.zeros(A.VX_GRID_LEN, dtype=np.uint32)
+    v = A.View()
+    A.check(L.vxw_build_view_items(world.handle, C.byref(c), A.ptr(it) if it.size else None, it.size,
+                                   C.byref((cfg or LodConfig()).to_c()), A.ptr(chunks), A.ptr(grid), C.byref(v)),
+            "build_view_items")
     fr = A.Frame()
     A.check(L.vxw_pack_frame(C.byref(c), C.byref(v), max_pairs, C.byref(fr)), "pack_frame")
     n = int(v.n_total)
