@@ -5,11 +5,15 @@
 //                     the leaf's occupancy mask, its palette encoding (encode_leaf,
 //                     src/voxel/attributes.rs:90-166) into a fixed 49-word slot, and the
 //                     L1 summary (mask, full, dry, water line, flags, counts, bounds) by
-//                     wave reductions;
-//   k_local_assemble  one wave per chunk, one lane per L1 cell: prefix sums over the L1
-//                     cells give leaf_start, leaf_prefix and the palette offsets, and the
-//                     wave writes the chunk's flat vx_local_tree record (header, L1 nodes,
-//                     kept leaf masks, attribute entries, palette words).
+//                     wave reductions. A block is the 4 L1 cells of one row along x: a
+//                     row of the dense chunk is 128 bytes, a quarter per L1 cell, so the
+//                     block reads each line once, on one CU (one wave per L1 cell as
+//                     separate blocks lands the four quarters on different XCDs, whose L2s
+//                     each fetch the whole line);
+//   k_local_assemble  per chunk, one lane per L1 cell: prefix sums over the L1 cells give
+//                     leaf_start, leaf_prefix and the palette offsets, and the chunk's flat
+//                     vx_local_tree record is written (header, L1 nodes, kept leaf masks,
+//                     attribute entries, palette words); four waves split the L1 cells.
 // The order of everything written follows build_local's loops (L1 cells, then leaves,
 // in cell-bit order), so the record is the host build's, word for word.
 #include <hip/hip_runtime.h>
@@ -70,81 +74,199 @@ __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t lane) {
     return s - v;
 }
 
-__global__ __launch_bounds__(64) void k_local_leaves(const uint16_t *__restrict__ dense, LeafInfo *__restrict__ leaves,
-                                                     uint32_t *__restrict__ words, L1Info *__restrict__ l1s) {
-    __shared__ uint16_t ids[64 * 64];  // [voxel bit][lane]
-    __shared__ uint8_t idx[64 * 64];   // palette index, [voxel bit][lane]
-    const uint32_t chunk = blockIdx.x >> 6, l1 = blockIdx.x & 63u, lane = threadIdx.x;
+// Bit spreading for the index words: the low 16 / 8 / 4 bits of x to every 2nd / 4th /
+// 8th bit.
+__device__ __forceinline__ uint32_t spread_2(uint64_t v) {
+    uint32_t x = (uint32_t)v & 0xFFFFu;
+    x = (x | (x << 8)) & 0x00FF00FFu;
+    x = (x | (x << 4)) & 0x0F0F0F0Fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    return (x | (x << 1)) & 0x55555555u;
+}
+__device__ __forceinline__ uint32_t spread_4(uint64_t v) {
+    uint32_t x = (uint32_t)v & 0xFFu;
+    x = (x | (x << 12)) & 0x000F000Fu;
+    x = (x | (x << 6)) & 0x03030303u;
+    return (x | (x << 3)) & 0x11111111u;
+}
+__device__ __forceinline__ uint32_t spread_8(uint64_t v) {
+    uint32_t x = (uint32_t)v & 0xFu;
+    x = (x | (x << 14)) & 0x00030003u;
+    return (x | (x << 7)) & 0x01010101u;
+}
+
+constexpr uint32_t LEAVES_BLOCK = 256;  // 4 waves: the L1 cells l1x = 0..3 of one row
+
+__global__ __launch_bounds__(LEAVES_BLOCK) void k_local_leaves(const uint16_t *__restrict__ dense,
+                                                               LeafInfo *__restrict__ leaves,
+                                                               uint32_t *__restrict__ words,
+                                                               L1Info *__restrict__ l1s) {
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t chunk = blockIdx.x >> 4, l1 = wave | ((blockIdx.x & 15u) << 2);
     const uint32_t l1x = l1 & 3u, l1y = (l1 >> 2) & 3u, l1z = l1 >> 4;
     const uint32_t lx = lane & 3u, ly = (lane >> 2) & 3u, lz = lane >> 4;
     const uint32_t bx = l1x * 16u + lx * 4u, by = l1y * 16u + ly * 4u, bz = l1z * 16u + lz * 4u;
     const uint16_t *src = dense + (size_t)chunk * VOL;
-    uint64_t m = 0;
-    bool water = false, dry = false, foliage = false, cutout = false, glass = false;
-    uint32_t maxy = 0, emitters = 0;
+    uint2 rows[16];  // all 16 row loads in flight before any is used
+#pragma unroll
+    for (uint32_t r = 0; r < 16u; r++)
+        rows[r] = *(const uint2 *)(src + (bx | ((bz + (r >> 2)) << 6) | ((by + (r & 3u)) << 12)));
+    // One branch-free pass in registers, voxel-bit order (x fastest), masks kept as 32-bit
+    // halves: occupancy, dry (non-water) and emitter voxels, one bit per block-table row
+    // present (ids past the table count as its last row) and the largest id.
+    uint32_t m_h[2] = {0u, 0u}, dry_h[2] = {0u, 0u}, em_h[2] = {0u, 0u};
+    uint32_t idbits = 0, idmax = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < 16u; r++) {
+        const uint32_t w4[4] = {rows[r].x & 0xFFFFu, rows[r].x >> 16, rows[r].y & 0xFFFFu, rows[r].y >> 16};
+#pragma unroll
+        for (uint32_t vx = 0; vx < 4u; vx++) {
+            const uint32_t id = w4[vx], vb = vx | (r << 2), h = vb >> 5, sh = vb & 31u;  // r = vz * 4 + vy
+            const uint32_t cid = min(id, 25u);
+            m_h[h] |= min(id, 1u) << sh;
+            dry_h[h] |= min(id ^ (uint32_t)B_WATER, 1u) << sh;
+            em_h[h] |= ((EMITTER_IDS >> cid) & 1u) << sh;
+            idbits |= 1u << cid;
+            idmax = max(idmax, id);
+        }
+        // accumulate row by row: left free, the compiler extracts all 64 ids first and
+        // sums them as trees, which holds some 200 registers
+        asm volatile("" : "+v"(m_h[0]), "+v"(m_h[1]), "+v"(dry_h[0]), "+v"(dry_h[1]), "+v"(em_h[0]), "+v"(em_h[1]),
+                     "+v"(idbits), "+v"(idmax));
+    }
+    const uint64_t m = ((uint64_t)m_h[1] << 32) | m_h[0];
+    const uint64_t dry_m = (((uint64_t)dry_h[1] << 32) | dry_h[0]) & m;
+    const uint64_t em_m = ((uint64_t)em_h[1] << 32) | em_h[0];
+    idbits &= ~1u;  // air
+    // Several table rows means several ids. A leaf whose ids are all at or past the last
+    // row is checked voxel by voxel against its largest id.
+    bool mixed = (idbits & (idbits - 1u)) != 0u;
+    const bool past_only = idbits == (1u << 25);
+    const uint32_t pal0 = past_only ? idmax : (idbits ? (uint32_t)__builtin_ctz(idbits) : 0u);
+    // The later passes re-extract ids from the row registers: keeping the 64 extracted ids
+    // of the pass above alive across them would cost 64 registers.
+#pragma unroll
+    for (uint32_t r = 0; r < 16u; r++) asm volatile("" : "+v"(rows[r].x), "+v"(rows[r].y));
+    if (past_only) {
+#pragma unroll
+        for (uint32_t r = 0; r < 16u; r++) {
+            const uint32_t w4[4] = {rows[r].x & 0xFFFFu, rows[r].x >> 16, rows[r].y & 0xFFFFu, rows[r].y >> 16};
+#pragma unroll
+            for (uint32_t vx = 0; vx < 4u; vx++) mixed = mixed || (w4[vx] != 0u && w4[vx] != idmax);
+        }
+    }
+    const bool water = (idbits >> B_WATER) & 1u, glass = (idbits >> B_GLASS) & 1u;
+    const bool foliage = (idbits & FOLIAGE_IDS) != 0u, cutout = (idbits & CUTOUT_IDS) != 0u;
+    const bool dry = dry_m != 0ull;
+    const uint32_t emitters = (uint32_t)__popcll(em_m);
+    // Bounds and water line from the masks (bit x | y << 2 | z << 4): fold to 4-bit
+    // occupancies per axis.
+    auto axis_bits = [](uint64_t v, int axis) {
+        if (axis == 0) {  // x: fold z (16-bit groups), then y (4-bit groups)
+            uint32_t t = (uint32_t)(v | (v >> 32));
+            t |= t >> 16;
+            t |= t >> 8;
+            t |= t >> 4;
+            return t & 0xFu;
+        }
+        if (axis == 1) {  // y: fold z, then x within each 4-bit group
+            uint32_t t = (uint32_t)(v | (v >> 32));
+            t |= t >> 16;
+            t = (t | (t >> 1) | (t >> 2) | (t >> 3)) & 0x1111u;
+            return (t & 1u) | ((t >> 3) & 2u) | ((t >> 6) & 4u) | ((t >> 9) & 8u);
+        }
+        // z: fold x and y within each 16-bit group
+        uint64_t t = v | (v >> 1) | (v >> 2) | (v >> 3);
+        t |= (t >> 4) | (t >> 8) | (t >> 12);
+        t &= 0x0001000100010001ull;
+        return (uint32_t)((t & 1u) | ((t >> 15) & 2u) | ((t >> 30) & 4u) | ((t >> 45) & 8u));
+    };
     uint32_t mn[3] = {64u, 64u, 64u}, mx[3] = {0u, 0u, 0u};
-    for (uint32_t vz = 0; vz < 4u; vz++)
-        for (uint32_t vy = 0; vy < 4u; vy++) {
-            const uint2 row = *(const uint2 *)(src + (bx | ((bz + vz) << 6) | ((by + vy) << 12)));
-            const uint32_t r[4] = {row.x & 0xFFFFu, row.x >> 16, row.y & 0xFFFFu, row.y >> 16};
+    if (m) {
+        const uint32_t base[3] = {bx, by, bz};
 #pragma unroll
-            for (uint32_t vx = 0; vx < 4u; vx++) {
-                const uint32_t id = r[vx], vb = vx | (vy << 2) | (vz << 4);
-                ids[vb * 64u + lane] = (uint16_t)id;
-                if (id == B_AIR) continue;
-                m |= 1ull << vb;
-                water |= id == B_WATER;
-                if (id != B_WATER) {
-                    dry = true;
-                    maxy = max(maxy, ly * 4u + vy);
-                }
-                foliage |= id_flag(FOLIAGE_IDS, id);
-                cutout |= id_flag(CUTOUT_IDS, id);
-                glass |= id == B_GLASS;
-                emitters += id_flag(EMITTER_IDS, id) ? 1u : 0u;
-                const uint32_t p[3] = {bx + vx, by + vy, bz + vz};
-#pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    mn[k] = min(mn[k], p[k]);
-                    mx[k] = max(mx[k], p[k] + 1u);
-                }
-            }
+        for (int k = 0; k < 3; k++) {
+            const uint32_t ab = axis_bits(m, k);
+            mn[k] = base[k] + (uint32_t)__builtin_ctz(ab);
+            mx[k] = base[k] + 32u - (uint32_t)__builtin_clz(ab);
         }
-    // encode_leaf: palette in order of first appearance (voxel-bit order), then indices.
-    uint32_t n = 0, pal0 = 0, pending = 0;
+    }
+    const uint32_t maxy = dry ? ly * 4u + 31u - (uint32_t)__builtin_clz(axis_bits(dry_m, 1)) : 0u;
+    uint32_t n = m ? 1u : 0u;
     uint32_t *wslot = words + ((size_t)chunk * 4096u + l1 * 64u + lane) * LEAF_WORDS;
-    uint64_t done = ~m;
-    while (done != ~0ull) {
-        const uint32_t first = (uint32_t)__ffsll((long long)~done) - 1u;
-        const uint32_t id = ids[first * 64u + lane];
-        for (uint32_t v = first; v < 64u; v++) {
-            if (!((done >> v) & 1ull) && ids[v * 64u + lane] == id) {
-                done |= 1ull << v;
-                idx[v * 64u + lane] = (uint8_t)n;
+    // encode_leaf for a leaf of several ids: the palette in order of first appearance
+    // (voxel-bit order), each entry's voxels found by one compare pass over the row
+    // registers; index bit b of every voxel is kept as a 64-bit plane (n <= 64 needs 6).
+    uint64_t planes[6] = {0ull, 0ull, 0ull, 0ull, 0ull, 0ull};
+    if (mixed) {
+        n = 0;
+        uint32_t pending = 0;
+        uint64_t todo = m;
+        while (todo) {
+#pragma unroll
+            for (uint32_t r = 0; r < 16u; r++) asm volatile("" : "+v"(rows[r].x), "+v"(rows[r].y));
+            const uint32_t first = (uint32_t)__ffsll((long long)todo) - 1u;
+            const uint32_t fr = first >> 2;
+            uint2 w = rows[0];
+#pragma unroll
+            for (uint32_t r = 1; r < 16u; r++) w = fr == r ? rows[r] : w;
+            const uint32_t half = (first & 2u) ? w.y : w.x;
+            const uint32_t id = (first & 1u) ? half >> 16 : half & 0xFFFFu;
+            uint32_t eq_lo = 0, eq_hi = 0;
+#pragma unroll
+            for (uint32_t r = 0; r < 16u; r++) {
+                const uint32_t w4[4] = {rows[r].x & 0xFFFFu, rows[r].x >> 16, rows[r].y & 0xFFFFu, rows[r].y >> 16};
+#pragma unroll
+                for (uint32_t vx = 0; vx < 4u; vx++) {
+                    const uint32_t vb = vx | (r << 2);  // r = vz * 4 + vy
+                    if (vb < 32u) eq_lo |= w4[vx] == id ? 1u << vb : 0u;
+                    else eq_hi |= w4[vx] == id ? 1u << (vb - 32u) : 0u;
+                }
+                asm volatile("" : "+v"(eq_lo), "+v"(eq_hi));  // row by row, as in the pass above
             }
+            const uint64_t eq = (((uint64_t)eq_hi << 32) | eq_lo) & todo;
+            todo &= ~eq;
+#pragma unroll
+            for (uint32_t b2 = 0; b2 < 6u; b2++) planes[b2] |= ((n >> b2) & 1u) ? eq : 0ull;
+            // palette words: pairs lo | hi << 16 from word 1 on
+            if (n & 1u) wslot[1u + n / 2u] = pending | (id << 16);
+            else pending = id;
+            n++;
         }
-        if (n == 0) pal0 = id;
-        // palette words: pairs lo | hi << 16 from word 1 on
-        if (n & 1u) wslot[1u + n / 2u] = pending | (id << 16);
-        else pending = id;
-        n++;
+        if (n & 1u) wslot[1u + n / 2u] = pending;  // odd count: last pair's high half is 0
     }
     uint32_t nwords = 0, entry = 0;
     if (n <= 1u) {
         entry = UNIFORM_BIT | pal0;
     } else {
-        if (n & 1u) wslot[1u + n / 2u] = pending;  // odd count: last pair's high half is 0
+        // index words: voxel v's index at bit (v % (32 / bits)) * bits of word v / (32 / bits)
         const uint32_t bits = n <= 2u ? 1u : (n <= 4u ? 2u : (n <= 16u ? 4u : 8u));
         wslot[0] = n | (bits << 8);
         const uint32_t base = 1u + (n + 1u) / 2u, iw = 64u * bits / 32u;
-        for (uint32_t k = 0; k < iw; k++) {
-            uint32_t w = 0;
-            const uint32_t per = 32u / bits;  // indices per word
-            for (uint32_t j = 0; j < per; j++) {
-                const uint32_t v = k * per + j;
-                if ((m >> v) & 1ull) w |= (uint32_t)idx[v * 64u + lane] << (j * bits);
+        uint32_t *iwp = wslot + base;
+        if (bits == 1u) {
+            iwp[0] = (uint32_t)planes[0];
+            iwp[1] = (uint32_t)(planes[0] >> 32);
+        } else if (bits == 2u) {
+#pragma unroll
+            for (uint32_t k = 0; k < 4u; k++)
+                iwp[k] = spread_2(planes[0] >> (16u * k)) | (spread_2(planes[1] >> (16u * k)) << 1);
+        } else if (bits == 4u) {
+#pragma unroll
+            for (uint32_t k = 0; k < 8u; k++) {
+                uint32_t wd = 0;
+#pragma unroll
+                for (uint32_t b2 = 0; b2 < 4u; b2++) wd |= spread_4(planes[b2] >> (8u * k)) << b2;
+                iwp[k] = wd;
             }
-            wslot[base + k] = w;
+        } else {
+#pragma unroll
+            for (uint32_t k = 0; k < 16u; k++) {
+                uint32_t wd = 0;
+#pragma unroll
+                for (uint32_t b2 = 0; b2 < 6u; b2++) wd |= spread_8(planes[b2] >> (4u * k)) << b2;
+                iwp[k] = wd;
+            }
         }
         nwords = base + iw;
     }
@@ -165,8 +287,8 @@ __global__ __launch_bounds__(64) void k_local_leaves(const uint16_t *__restrict_
     if (l1mask) {
         const uint32_t lead = (uint32_t)__ffsll((long long)l1mask) - 1u;
         uni_id = (uint32_t)__shfl((int)pal0, (int)lead);
-        const bool mixed = ballot(present && (n != 1u || pal0 != uni_id)) != 0ull;
-        uni_state = mixed ? 2u : 1u;
+        const bool l1_mixed = ballot(present && (mixed || pal0 != uni_id)) != 0ull;
+        uni_state = l1_mixed ? 2u : 1u;
     }
     uint32_t cmin[3], cmax[3];
 #pragma unroll
@@ -199,11 +321,15 @@ __global__ __launch_bounds__(64) void k_local_leaves(const uint16_t *__restrict_
     }
 }
 
-__global__ __launch_bounds__(64) void k_local_assemble(const LeafInfo *__restrict__ leaves,
-                                                       const uint32_t *__restrict__ words,
-                                                       const L1Info *__restrict__ l1s, uint8_t *__restrict__ out,
-                                                       size_t slot_bytes) {
-    const uint32_t chunk = blockIdx.x, lane = threadIdx.x;
+constexpr uint32_t ASSEMBLE_WAVES = 8, ASSEMBLE_BLOCK = 64u * ASSEMBLE_WAVES;
+
+__global__ __launch_bounds__(ASSEMBLE_BLOCK) void k_local_assemble(const LeafInfo *__restrict__ leaves,
+                                                                   const uint32_t *__restrict__ words,
+                                                                   const L1Info *__restrict__ l1s,
+                                                                   uint8_t *__restrict__ out, size_t slot_bytes) {
+    // Every wave computes the chunk's scans (64 L1 summaries); wave 0 writes the header
+    // and the L1 nodes, and the waves take the L1 cells' leaves in turn (c % 8 == wave).
+    const uint32_t chunk = blockIdx.x, wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const L1Info s = l1s[(size_t)chunk * 64u + lane];
     const bool present = s.mask != 0ull;
     const uint32_t cnt = (uint32_t)__popcll(s.mask);
@@ -240,7 +366,7 @@ __global__ __launch_bounds__(64) void k_local_assemble(const LeafInfo *__restric
     const size_t bytes = (o_pal + (size_t)n_pal * 4u + 7u) & ~(size_t)7;
     uint8_t *rec = out + (size_t)chunk * slot_bytes;
     const uint64_t dry_mask = ballot(present && s.dry);  // all lanes: a ballot under lane == 0 sees one lane
-    if (lane == 0) {
+    if (wave == 0 && lane == 0) {
         vx_local_tree h{};
         h.root_mask = pres;
         h.dry_mask = dry_mask;
@@ -263,12 +389,12 @@ __global__ __launch_bounds__(64) void k_local_assemble(const LeafInfo *__restric
         *(vx_local_tree *)rec = h;
         if (bytes > o_pal + (size_t)n_pal * 4u) *(uint32_t *)(rec + bytes - 4u) = 0u;  // tail padding
     }
-    if (present) {
+    if (wave == 0 && present) {
         vx_local_l1 r{s.mask, leaf_start, leaf_prefix, s.full, s.water_line};
         *(vx_local_l1 *)(rec + o_l1 + (size_t)rank * 24u) = r;
     }
     // leaves, attribute entries and palette words, L1 cell by L1 cell (lanes = leaves)
-    uint64_t todo = pres;
+    uint64_t todo = pres & (0x0101010101010101ull << wave);
     while (todo) {
         const uint32_t c = (uint32_t)__ffsll((long long)todo) - 1u;
         todo &= todo - 1ull;
@@ -299,7 +425,12 @@ using vxh::ensure;
 
 namespace {
 
-constexpr uint32_t GROUP = 256;  // chunks per device pass
+// Chunks per device pass. The pass's scratch is 800 KB of palette slots and 100 KB of
+// leaf and L1 summaries per chunk (0.9 GB at 1024 chunks).
+constexpr uint32_t GROUP = 1024;
+// Chunks per pass of the host-buffer entry point, which also stages the dense chunks and
+// the records in device memory (512 KB + 834 KB per chunk).
+constexpr uint32_t HOST_GROUP = 256;
 
 int build_group(vx_ctx *c, const uint16_t *d_dense, uint32_t n, uint8_t *d_out, size_t slot_bytes) {
     int rc;
@@ -307,10 +438,11 @@ int build_group(vx_ctx *c, const uint16_t *d_dense, uint32_t n, uint8_t *d_out, 
         (rc = ensure(c->build_words, (size_t)n * 4096u * LEAF_WORDS * 4u)) ||
         (rc = ensure(c->build_l1, (size_t)n * 64u * sizeof(L1Info))))
         return rc;
-    k_local_leaves<<<n * 64u, 64, 0, c->stream>>>(d_dense, (LeafInfo *)c->build_leaf.p, (uint32_t *)c->build_words.p,
-                                                  (L1Info *)c->build_l1.p);
-    k_local_assemble<<<n, 64, 0, c->stream>>>((const LeafInfo *)c->build_leaf.p, (const uint32_t *)c->build_words.p,
-                                              (const L1Info *)c->build_l1.p, d_out, slot_bytes);
+    k_local_leaves<<<n * 16u, LEAVES_BLOCK, 0, c->stream>>>(d_dense, (LeafInfo *)c->build_leaf.p,
+                                                             (uint32_t *)c->build_words.p, (L1Info *)c->build_l1.p);
+    k_local_assemble<<<n, ASSEMBLE_BLOCK, 0, c->stream>>>((const LeafInfo *)c->build_leaf.p,
+                                                          (const uint32_t *)c->build_words.p,
+                                                          (const L1Info *)c->build_l1.p, d_out, slot_bytes);
     VX_LAUNCHED();
     return VX_OK;
 }
@@ -372,8 +504,8 @@ int vx_build_local(vx_ctx *c, const uint16_t *dense, uint32_t n, void *out, size
     if (!c || (n && (!dense || !out)) || slot_bytes < VX_LOCAL_TREE_MAX || slot_bytes % 8u) return VX_E_ARG;
     VX_HIPX(hipSetDevice(c->device));
     hipStream_t st = c->stream;
-    for (uint32_t b = 0; b < n; b += GROUP) {
-        const uint32_t g = std::min(GROUP, n - b);
+    for (uint32_t b = 0; b < n; b += HOST_GROUP) {
+        const uint32_t g = std::min(HOST_GROUP, n - b);
         int rc;
         if ((rc = ensure(c->build_dense, (size_t)g * VOL * 2u)) ||
             (rc = ensure(c->build_out, (size_t)g * VX_LOCAL_TREE_MAX)))
