@@ -220,14 +220,14 @@ BlockId local_block(const LocalTree &t, uint32_t x, uint32_t y, uint32_t z) {
 
 // ---- RunPool (src/voxel/geometry.rs:74-221) ----
 template <class T> uint64_t RunPool<T>::hash_run(const T *run, uint32_t len) {
-    // FNV-1a over the run bytes; only lookups depend on it, never offsets.
-    const unsigned char *b = reinterpret_cast<const unsigned char *>(run);
-    uint64_t h = 1469598103934665603ull ^ len;
-    for (size_t i = 0; i < sizeof(T) * len; i++) {
-        h ^= b[i];
-        h *= 1099511628211ull;
-    }
-    return h;
+    // A multiply-rotate hash over the run's 64-bit words (the element types are 8 and 16
+    // bytes); only lookups depend on it, never offsets.
+    static_assert(sizeof(T) % 8 == 0, "runs hash as 64-bit words");
+    const uint64_t *w = reinterpret_cast<const uint64_t *>(run);
+    const size_t n = sizeof(T) / 8 * len;
+    uint64_t h = 0x9E3779B97F4A7C15ull ^ len;
+    for (size_t i = 0; i < n; i++) h = (((h << 5) | (h >> 59)) ^ w[i]) * 0x517CC1B727220A95ull;
+    return h ^ (h >> 29);
 }
 
 template <class T> uint32_t RunPool<T>::alloc(uint32_t len) {
