@@ -1,6 +1,7 @@
 """GPU parity over LOD worlds: a world streamed by the LOD planner (chunks of LOD 0-2,
-cross-fading pairs drawn with the per-pixel dither) rendered by the single-view paths
-and by view batches, against the oracle, bit for bit.
+cross-fading pairs drawn with the per-pixel dither) rendered by the single-view paths,
+by view batches (with and without the secondary bounce) and by the request server,
+against the oracle, bit for bit.
 
 The views hit coarse chunks (voxel size 2 and 4) and fading chunks of both signs, so
 the march's coarse body, the dither and the grid's (share, -lod) fill all reach the
@@ -99,4 +100,74 @@ def test_lod_batch_matches_oracle(lod_views, form):
             assert np.array_equal(gs, sh), f"{tag}: {(gs != sh).sum()} shadow words differ"
             assert np.array_equal(gh, hz), f"{tag}: {(gh != hz).sum()} hiz words differ"
     b.close()
+    r.close()
+
+
+@pytest.mark.parametrize("form", [dict(mode=A.BATCH_FLAT, graph=1, occupancy=6, variant=5),
+                                  dict(mode=A.BATCH_PERSISTENT, graph=0)], ids=["flat65", "persistent"])
+def test_lod_secondary_legs_match_oracle(form):
+    """The secondary bounce over the LOD world: legs leave from hits on chunks of every
+    LOD and their grid walk meets coarse and fading chunks; every leg record and sun
+    test equals the oracle's."""
+    from oracle import oracle as O
+
+    w = World()
+    render = settle(w, GEN, Planner(), EYE, CFG)
+    flags = W.DEFAULT_FLAGS & ~A.FLAG_WAVES
+    cams = [H.camera(EYE, yaw, pitch, 200, 120, flags) for yaw, pitch in ((200.0, -30.0), (300.0, -45.0),
+                                                                             (0.0, -35.0))]
+    views = [build_view_items(w, c, render, CFG) for c in cams]
+    specs = [W.spec_words(c.flags, w, W.DEFAULT_SPEC_HI) for c in cams]
+    r = Renderer(64, 64)
+    r.sync_world(w)
+    b = Batch(r, 4)
+    b.set_tuning(**form)
+    b.set_views(views, specs)
+    b.set_secondary(True)
+    b.render()
+    b.sync()
+    total = sun = 0
+    for i, (v, (s, hi)) in enumerate(zip(views, specs)):
+        print(f"lod secondary view {i}", flush=True)
+        sc = O.Scene(w.pool(0), w.pool(1), w.pool(2), v.chunks, v.grid)
+        sp = O.spec_from(s, hi)
+        ref_vis = O.render(sc, v.frame_bytes, sp, np.zeros(v.tiles[0] * v.tiles[1], np.uint32), shadow=False)[0]
+        assert np.array_equal(b.read_vis(i), ref_vis), f"view {i} vis"
+        legs, st = O.secondary(sc, v.frame_bytes, sp, ref_vis)
+        diff = b.read_secondary(i).view(np.uint32) != legs.view(np.uint32)
+        assert not diff.any(), f"view {i}: {diff.any(axis=-1).sum()} leg records differ"
+        total += st["rays_secondary"]
+        sun += st["rays_secondary_sun"]
+    cnt = b.secondary_count()
+    assert cnt["water_legs"] + cnt["glass_legs"] == total and cnt["sun_rays"] == sun and not cnt["overflow"]
+    assert total > 1000 and sun > 500
+    b.close()
+    r.close()
+
+
+def test_lod_views_served_match_oracle(lod_views):
+    """The request server renders LOD views as the oracle's first frame of each view."""
+    from voxelcraft_menu_amd.renderer import Server
+
+    w, views, specs, refs = lod_views
+    r = Renderer(64, 64)
+    r.sync_world(w)
+    srv = Server(r, lanes=2, max_views=4, hbm_budget=256 << 20, mode=A.BATCH_FLAT)
+    for i, (v, (s, hi)) in enumerate(zip(views, specs)):
+        srv.submit(i, v, s, hi)
+    done = {}
+    polls = 0
+    while len(done) < len(views):
+        for res in srv.poll(wait_ms=50):
+            done[res["id"]] = res
+        polls += 1
+        assert polls < 2000, f"only {len(done)} of {len(views)} requests completed"
+    for i, v in enumerate(views):
+        assert done[i]["status"] == 0
+        wd, ht = int(v.frame.res[0]), int(v.frame.res[1])
+        vis, sh, _, _ = refs[i][0]  # first frame: Hi-Z words start at zero
+        assert np.array_equal(srv.read_vis(i, wd, ht), vis), f"served view {i} vis"
+        assert np.array_equal(srv.read_shadow(i, wd, ht), sh), f"served view {i} shadow"
+        srv.release(i)
+    srv.close()
     r.close()
