@@ -16,6 +16,7 @@
  *                            (src/math.rs:69-117, src/render/mod.rs:82-138,1447-1564)
  *   vxw_planner_*         <- lod::Planner (src/lod.rs:20-257)
  *   vxw_build_view_items  <- Renderer::build_chunk_list over a LOD draw list
+ *   vxw_snapshot_*        <- (no counterpart) chunk records for view setup during edits
  *   vxw_pack_frame        <- the GpuFrame packing in Renderer::render
  *                            (src/render/mod.rs:1612-1710)
  *   vxw_build_local_flat / vxw_insert_local_flat
@@ -170,6 +171,18 @@ typedef struct vxw_view {
  * chunks must hold VX_MAX_CHUNKS entries, grid VX_GRID_LEN. */
 int vxw_build_view(const vxw_world *w, const vxw_camera *cam, vx_gpu_chunk *chunks, uint32_t *grid,
                    vxw_view *out);
+/* View snapshots (no counterpart in the reference, whose one owner thread edits and
+ * renders in turn): a copy of the LOD-0 chunk records vxw_build_view reads, so that a
+ * server can set up views from the world as it was while an edit batch is applied to
+ * it. vxw_build_view_snapshot gives what vxw_build_view gave at vxw_snapshot_take.
+ * A snapshot is read-only between takes and may be read from several threads. */
+typedef struct vxw_snapshot vxw_snapshot;
+vxw_snapshot *vxw_snapshot_new(void);
+void vxw_snapshot_free(vxw_snapshot *s);
+int vxw_snapshot_take(vxw_snapshot *s, const vxw_world *w);
+uint32_t vxw_snapshot_emitter_chunks(const vxw_snapshot *s);
+int vxw_build_view_snapshot(const vxw_snapshot *s, const vxw_camera *cam, vx_gpu_chunk *chunks, uint32_t *grid,
+                            vxw_view *out);
 /* Packs the GpuFrame record for a camera and view (max_pairs 0 = pairs_for(tiles)). */
 int vxw_pack_frame(const vxw_camera *cam, const vxw_view *view, uint32_t max_pairs, vx_gpu_frame *out);
 /* render::pairs_for (src/render/mod.rs:27-30). */

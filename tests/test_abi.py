@@ -99,3 +99,25 @@ def test_struct_sizes_match_the_c_header(tmp_path):
         assert got[n] == C.sizeof(cls), (n, got[n], C.sizeof(cls))
     for n, dt in names_np.items():
         assert got[n] == dt.itemsize, (n, got[n], dt.itemsize)
+
+
+def test_host_struct_sizes_match_the_c_header(tmp_path):
+    """ctypes / numpy mirrors against the C compiler's view of include/vxhost.h."""
+    import subprocess
+
+    names = {"vxw_chunk_info": A.ChunkInfo, "vxw_key": A.Key, "vxw_render_item": A.RenderItem,
+             "vxw_lod_config": A.LodConfig, "vxw_camera": A.Camera, "vxw_view": A.View,
+             "vxw_gen_params": A.GenParams, "vxw_player": A.Player, "vxw_journal_info": A.JournalInfo}
+    names_np = {"vxw_render_item": A.RENDER_ITEM_DTYPE, "vxw_key": A.KEY_DTYPE}
+    src = tmp_path / "szh.c"
+    body = "".join(f'    printf("{n} %zu\\n", sizeof({n}));\n' for n in names)
+    src.write_text(f'#include <stdio.h>\n#include "vxhost.h"\nint main(void) {{\n{body}    return 0;\n}}\n')
+    exe = tmp_path / "szh"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.run(["gcc", "-std=c99", f"-I{inc}", str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    got = dict(zip(out[0::2], (int(x) for x in out[1::2])))
+    for n, cls in names.items():
+        assert got[n] == C.sizeof(cls), (n, got[n], C.sizeof(cls))
+    for n, dt in names_np.items():
+        assert got[n] == dt.itemsize, (n, got[n], dt.itemsize)

@@ -307,6 +307,11 @@ _sig_host = {
     "vxw_planner_desired": (C.c_size_t, [P, P, C.c_size_t]),
     "vxw_build_view_items": (C.c_int, [P, C.POINTER(Camera), P, C.c_size_t, C.POINTER(LodConfig), P, P,
                                        C.POINTER(View)]),
+    "vxw_snapshot_new": (P, []),
+    "vxw_snapshot_free": (None, [P]),
+    "vxw_snapshot_take": (C.c_int, [P, P]),
+    "vxw_snapshot_emitter_chunks": (C.c_uint32, [P]),
+    "vxw_build_view_snapshot": (C.c_int, [P, C.POINTER(Camera), P, P, C.POINTER(View)]),
     "vxw_journal_parse": (C.c_int, [P, C.c_size_t, C.c_int32, C.c_int32, C.POINTER(JournalInfo), P, C.c_size_t,
                                     C.c_char_p, C.c_size_t]),
     "vxw_journal_write": (C.c_int, [P, C.c_size_t, C.c_int32, C.c_int32, C.POINTER(Player), P, P, C.c_size_t,

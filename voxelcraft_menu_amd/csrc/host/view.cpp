@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <new>
 #include <vector>
 
 #include "vxhost.h"
@@ -65,7 +66,38 @@ struct Frustum {
     }
 };
 
-Aabb world_aabb(const ChunkRecord &rec) {
+// The fields of a ChunkRecord the chunk list reads (src/voxel/chunk.rs:73-111).
+struct ViewRec {
+    Key key;
+    Inner root;
+    uint32_t attr_base, attr_flags;  // GpuChunk attr_base / attr_flags (src/render/mod.rs:1485-1504)
+    int32_t aabb_min[3], aabb_max[3];
+    uint32_t solid_count;
+    uint64_t dry_mask;
+};
+
+ViewRec view_rec(const ChunkRecord &r) {
+    ViewRec v;
+    v.key = r.key;
+    v.root = r.root;
+    v.attr_base = r.attr_base;
+    uint32_t flags = r.attr_uniform ? 1u : 0u;
+    if (r.has_water) flags |= 2u;
+    if (r.has_foliage) flags |= 4u;
+    if (r.has_cutout) flags |= 8u;
+    if (r.has_emitter) flags |= 32u;
+    if (r.has_glass) flags |= 16u;
+    v.attr_flags = flags;
+    for (int i = 0; i < 3; i++) {
+        v.aabb_min[i] = r.aabb_min[i];
+        v.aabb_max[i] = r.aabb_max[i];
+    }
+    v.solid_count = r.solid_count;
+    v.dry_mask = r.dry_mask;
+    return v;
+}
+
+Aabb world_aabb(const ViewRec &rec) {
     // ChunkRecord::world_aabb (src/voxel/chunk.rs:100-110): origin + local bounds * voxel size.
     const Key &k = rec.key;
     const float vs = (float)k.voxel_size();
@@ -75,7 +107,7 @@ Aabb world_aabb(const ChunkRecord &rec) {
 }
 
 struct DrawItem {
-    Key key;
+    const ViewRec *rec;
     float fade;
     float share() const { return fade >= 0.0f ? fade : 1.0f + fade; }
 };
@@ -88,8 +120,9 @@ int32_t total_order(float f) {
 }
 
 // partition_for_grid + build_chunk_list (src/render/mod.rs:82-138, :1447-1564).
-int build_view(const World &w, const vxw_camera *cam, const std::vector<DrawItem> &items, bool shadow_share,
-               bool offscreen_shadows, vx_gpu_chunk *chunks, uint32_t *grid, vxw_view *out) {
+// items: resident chunks with their fades, in draw-list order.
+int build_view(const vxw_camera *cam, const std::vector<DrawItem> &items, bool shadow_share, bool offscreen_shadows,
+               vx_gpu_chunk *chunks, uint32_t *grid, vxw_view *out) {
     V3 pos{cam->pos[0], cam->pos[1], cam->pos[2]};
     V3 fwd{cam->fwd[0], cam->fwd[1], cam->fwd[2]};
     V3 right{cam->right[0], cam->right[1], cam->right[2]};
@@ -116,17 +149,16 @@ int build_view(const World &w, const vxw_camera *cam, const std::vector<DrawItem
     struct Item {
         float d;
         DrawItem it;
-        const ChunkRecord *rec;
+        const ViewRec *rec;
     };
     std::vector<Item> visible, offscreen;
     for (const DrawItem &it : items) {
-        auto f = w.chunks.find(it.key);
-        if (f == w.chunks.end() || f->second.solid_count == 0) continue;
-        const ChunkRecord &rec = f->second;
+        if (it.rec->solid_count == 0) continue;
+        const ViewRec &rec = *it.rec;
         Aabb a = world_aabb(rec);
         int32_t lo[3], hi[3];
         if (fr.contains(a)) visible.push_back({a.distance_to(pos), it, &rec});
-        else if (offscreen_shadows && grid_span(it.key, lo, hi)) offscreen.push_back({a.distance_to(pos), it, &rec});
+        else if (offscreen_shadows && grid_span(rec.key, lo, hi)) offscreen.push_back({a.distance_to(pos), it, &rec});
     }
     auto by_d = [](const Item &a, const Item &b) { return a.d < b.d; };
     std::stable_sort(visible.begin(), visible.end(), by_d);
@@ -139,7 +171,7 @@ int build_view(const World &w, const vxw_camera *cam, const std::vector<DrawItem
     std::vector<DrawItem> grid_set;
     grid_set.reserve(visible.size() + offscreen.size());
     auto emit = [&](const Item &it) {
-        const ChunkRecord &rec = *it.rec;
+        const ViewRec &rec = *it.rec;
         const Key &k = rec.key;
         Aabb a = world_aabb(rec);
         if (idx < visible.size() && camera_chunk == VX_NO_CHUNK && pos.x >= a.min.x && pos.y >= a.min.y &&
@@ -158,14 +190,8 @@ int build_view(const World &w, const vxw_camera *cam, const std::vector<DrawItem
         g.root[1] = (uint32_t)(rec.root.mask >> 32);
         g.root[2] = rec.root.run_ptr;
         g.root[3] = rec.root.leaf_prefix;
-        uint32_t flags = rec.attr_uniform ? 1u : 0u;
-        if (rec.has_water) flags |= 2u;
-        if (rec.has_foliage) flags |= 4u;
-        if (rec.has_cutout) flags |= 8u;
-        if (rec.has_emitter) flags |= 32u;
-        if (rec.has_glass) flags |= 16u;
         g.attr_base = rec.attr_base;
-        g.attr_flags = flags;
+        g.attr_flags = rec.attr_flags;
         g.light_base = 0xF000u;  // LightRef::Uniform(0xF000): lighting is not on this path
         g.light_flags = 1u;
         g.dry_mask[0] = (uint32_t)rec.dry_mask;
@@ -186,11 +212,11 @@ int build_view(const World &w, const vxw_camera *cam, const std::vector<DrawItem
         const float sb = shadow_share ? grid_set[b].share() : 1.0f;
         const int32_t ta = total_order(sa), tb = total_order(sb);
         if (ta != tb) return ta < tb;
-        return grid_set[b].key.lod < grid_set[a].key.lod;
+        return grid_set[b].rec->key.lod < grid_set[a].rec->key.lod;
     });
     for (uint32_t i : order) {
         int32_t lo[3], hi[3];
-        if (!grid_span(grid_set[i].key, lo, hi)) continue;
+        if (!grid_span(grid_set[i].rec->key, lo, hi)) continue;
         for (int32_t z = lo[2]; z < hi[2]; z++)
             for (int32_t y = lo[1]; y < hi[1]; y++)
                 for (int32_t x = lo[0]; x < hi[0]; x++)
@@ -229,17 +255,34 @@ extern "C" uint32_t vxw_pairs_for(uint32_t tx, uint32_t ty) {
     return (uint32_t)want;
 }
 
+namespace {
+
+// The LOD-0 chunks in insertion order: the render set of vxw_build_view.
+std::vector<ViewRec> lod0_records(const World &w) {
+    std::vector<ViewRec> recs;
+    recs.reserve(w.order.size());
+    for (const Key &k : w.order)
+        if (k.lod == 0) recs.push_back(view_rec(w.chunks.at(k)));
+    return recs;
+}
+
+int build_view_all(const std::vector<ViewRec> &recs, const vxw_camera *cam, vx_gpu_chunk *chunks, uint32_t *grid,
+                   vxw_view *out) {
+    std::vector<DrawItem> items;
+    items.reserve(recs.size());
+    for (const ViewRec &r : recs) items.push_back(DrawItem{&r, 1.0f});
+    return build_view(cam, items, true, true, chunks, grid, out);
+}
+
+}  // namespace
+
 // build_chunk_list over the LOD-0 chunks in insertion order, fade 1.0, with
 // offscreen_shadows and shadow_share on (src/lod.rs:40-52).
 extern "C" int vxw_build_view(const vxw_world *wh, const vxw_camera *cam, vx_gpu_chunk *chunks, uint32_t *grid,
                               vxw_view *out) {
     if (!wh || !cam || !chunks || !grid || !out) return VXW_E_ARG;
     if (cam->width == 0 || cam->height == 0) return VXW_E_ARG;
-    std::vector<DrawItem> items;
-    items.reserve(wh->w.order.size());
-    for (const Key &k : wh->w.order)
-        if (k.lod == 0) items.push_back(DrawItem{k, 1.0f});
-    return build_view(wh->w, cam, items, true, true, chunks, grid, out);
+    return build_view_all(lod0_records(wh->w), cam, chunks, grid, out);
 }
 
 extern "C" int vxw_build_view_items(const vxw_world *wh, const vxw_camera *cam, const vxw_render_item *items,
@@ -252,14 +295,48 @@ extern "C" int vxw_build_view_items(const vxw_world *wh, const vxw_camera *cam, 
         vxw_lod_config_default(&def);
         cfg = &def;
     }
-    std::vector<DrawItem> v;
-    v.reserve(n);
+    std::vector<ViewRec> recs;
+    std::vector<float> fades;
+    recs.reserve(n);
+    fades.reserve(n);
     for (size_t i = 0; i < n; i++) {
         const vxw_render_item &it = items[i];
         if (it.key.lod > VXW_MAX_LOD) return VXW_E_ARG;
-        v.push_back(DrawItem{Key{it.key.pos[0], it.key.pos[1], it.key.pos[2], (int32_t)it.key.lod}, it.fade});
+        auto f = wh->w.chunks.find(Key{it.key.pos[0], it.key.pos[1], it.key.pos[2], (int32_t)it.key.lod});
+        if (f == wh->w.chunks.end()) continue;  // not resident: build_chunk_list skips it
+        recs.push_back(view_rec(f->second));
+        fades.push_back(it.fade);
     }
-    return build_view(wh->w, cam, v, cfg->shadow_share != 0, cfg->offscreen_shadows != 0, chunks, grid, out);
+    std::vector<DrawItem> v;
+    v.reserve(recs.size());
+    for (size_t i = 0; i < recs.size(); i++) v.push_back(DrawItem{&recs[i], fades[i]});
+    return build_view(cam, v, cfg->shadow_share != 0, cfg->offscreen_shadows != 0, chunks, grid, out);
+}
+
+// A copy of the LOD-0 chunk records a view's chunk list reads, taken between edits, so
+// that views can be set up while the world is being edited.
+struct vxw_snapshot {
+    std::vector<ViewRec> recs;
+    uint32_t emitter_chunks = 0;
+};
+
+extern "C" vxw_snapshot *vxw_snapshot_new(void) { return new (std::nothrow) vxw_snapshot(); }
+extern "C" void vxw_snapshot_free(vxw_snapshot *s) { delete s; }
+
+extern "C" int vxw_snapshot_take(vxw_snapshot *s, const vxw_world *wh) {
+    if (!s || !wh) return VXW_E_ARG;
+    s->recs = lod0_records(wh->w);
+    s->emitter_chunks = wh->w.emitter_chunks;
+    return VXW_OK;
+}
+
+extern "C" uint32_t vxw_snapshot_emitter_chunks(const vxw_snapshot *s) { return s ? s->emitter_chunks : 0u; }
+
+extern "C" int vxw_build_view_snapshot(const vxw_snapshot *s, const vxw_camera *cam, vx_gpu_chunk *chunks,
+                                       uint32_t *grid, vxw_view *out) {
+    if (!s || !cam || !chunks || !grid || !out) return VXW_E_ARG;
+    if (cam->width == 0 || cam->height == 0) return VXW_E_ARG;
+    return build_view_all(s->recs, cam, chunks, grid, out);
 }
 
 // GpuFrame packing (src/render/mod.rs:1612-1710); shading-only fields stay zero.

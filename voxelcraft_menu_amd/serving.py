@@ -5,13 +5,16 @@ the chunk list, render, wait. Here requests arrive on a seeded Poisson clock whe
 not earlier ones have finished; each is stamped with its scheduled arrival time, so
 latency = completion - arrival includes every host and queueing delay.
 
-  view request  host chunk list (build_view, under a shared lock on the host world),
-                then Server.submit; the native server batches it with whatever else is
-                queued and launches it on a free lane.
-  edit batch    1000 random voxel set/clear ops: applied to the host world under the
-                exclusive lock (the GPU keeps rendering submitted views meanwhile), then
-                barrier -> dirty-range pool upload -> resume, so views submitted before
-                the edit see the old world and views after it the new one.
+  view request  host chunk list from the current snapshot of the world's chunk records
+                (ViewSnapshot.build_view, under a shared lock), then Server.submit; the
+                native server batches it with whatever else is queued and launches it
+                on a free lane.
+  edit batch    1000 random voxel set/clear ops, on their own thread: applied to the
+                host world while views keep being set up from the snapshot and rendered
+                from the device pools, both still the world before the edit; a new
+                snapshot is taken; then, under the exclusive lock, barrier -> dirty-range
+                pool upload -> snapshot swap -> resume. Views submitted before the swap
+                see the old world, views after it the new one.
 
 A poller thread collects completions and releases them (returning their HBM budget).
 """
@@ -26,7 +29,7 @@ import numpy as np
 from . import _abi as A
 from . import workloads as W
 from .renderer import Renderer, Server
-from .world import World, build_view
+from .world import ViewSnapshot, World
 
 EDIT_IDS = [0, 1, 2, 5, 8, 11]  # air, stone, dirt, planks, glass, glowstone
 
@@ -91,6 +94,7 @@ def run(world: World, renderer: Renderer, wl: W.Workload, qps: float, duration_s
 
     srv = Server(renderer, lanes=lanes, max_views=max_views, hbm_budget=hbm_budget, mode=mode)
     lock = _RWLock()
+    snaps = [ViewSnapshot(world), ViewSnapshot()]  # [current, spare]
     lat = {}
     px = {}
     batches = {}
@@ -114,8 +118,9 @@ def run(world: World, renderer: Renderer, wl: W.Workload, qps: float, duration_s
         lock.acquire_read()
         try:
             cam = cams[i]
-            v = build_view(world, cam)
-            spec, hi = W.spec_words(cam.flags, world)
+            snap = snaps[0]
+            v = snap.build_view(cam)
+            spec, hi = W.spec_words(cam.flags, snap)
             srv.submit(i, v, spec, hi, t_arr)
             px[i] = cam.width * cam.height
             with sub_lock:
@@ -123,21 +128,28 @@ def run(world: World, renderer: Renderer, wl: W.Workload, qps: float, duration_s
         finally:
             lock.release_read()
 
+    edit_ms = []
+
     def apply_edit(e: np.ndarray):
+        te = srv.clock()
+        world.apply_edits(e)  # views meanwhile come from snaps[0] and the device pools
+        snaps[1].take(world)
         lock.acquire_write()
         try:
-            world.apply_edits(e)
             srv.barrier()
             renderer.sync_world(world)
+            snaps[0], snaps[1] = snaps[1], snaps[0]
             srv.resume()
         finally:
             lock.release_write()
+        edit_ms.append((srv.clock() - te) * 1e3)
 
     pt = threading.Thread(target=poller, daemon=True)
     pt.start()
     futs = []
-    edit_ms = []
-    with ThreadPoolExecutor(max_workers=workers) as pool:
+    edit_futs = []
+    # edits in order on one thread of their own, so the arrival loop never waits on one
+    with ThreadPoolExecutor(max_workers=workers) as pool, ThreadPoolExecutor(max_workers=1) as editor:
         t0 = srv.clock() + 0.05
         for t, kind, i in events:
             target = t0 + t
@@ -149,16 +161,16 @@ def run(world: World, renderer: Renderer, wl: W.Workload, qps: float, duration_s
             if kind == 0:
                 futs.append(pool.submit(prep_submit, i, target))
             else:
-                te = srv.clock()
-                apply_edit(edits[i])
-                edit_ms.append((srv.clock() - te) * 1e3)
-        for f in futs:
+                edit_futs.append(editor.submit(apply_edit, edits[i]))
+        for f in futs + edit_futs:
             f.result()
     done_evt.set()
     pt.join()
     t_end = max(v[2] for v in lat.values())
     st = srv.stats()
     srv.close()
+    for sn in snaps:
+        sn.close()
     if errors:
         raise RuntimeError(f"{len(errors)} requests failed, first: {errors[0]}")
     l = np.array([v[0] for v in lat.values()]) * 1e3

@@ -362,6 +362,51 @@ def build_view_items(world: World, cam: CameraParams, items: np.ndarray, cfg: Lo
                     int(v.n_visible), n, int(v.camera_chunk), cam.width, cam.height)
 
 
+class ViewSnapshot:
+    """The LOD-0 chunk records view setup reads, copied from a world (vxw_snapshot_*):
+    ``build_view`` gives what ``world.build_view`` gave when ``take`` ran, while the world
+    itself may be under edit. Read-only between takes; safe to read from many threads."""
+
+    def __init__(self, world: World | None = None):
+        self._L = A.host()
+        self._h = self._L.vxw_snapshot_new()
+        if not self._h:
+            raise MemoryError("vxw_snapshot_new")
+        if world is not None:
+            self.take(world)
+
+    def close(self):
+        if self._h:
+            self._L.vxw_snapshot_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def take(self, world: World):
+        A.check(self._L.vxw_snapshot_take(self._h, world.handle), "snapshot_take")
+
+    def any_emitter_resident(self) -> bool:
+        return self._L.vxw_snapshot_emitter_chunks(self._h) != 0
+
+    def build_view(self, cam: CameraParams, max_pairs: int = 0) -> ViewData:
+        L = A.host()
+        c = cam.to_c()
+        chunks = np.zeros((A.VX_MAX_CHUNKS, 96), dtype=np.uint8)
+        grid = np.zeros(A.VX_GRID_LEN, dtype=np.uint32)
+        v = A.View()
+        A.check(L.vxw_build_view_snapshot(self._h, C.byref(c), A.ptr(chunks), A.ptr(grid), C.byref(v)),
+                "build_view_snapshot")
+        fr = A.Frame()
+        A.check(L.vxw_pack_frame(C.byref(c), C.byref(v), max_pairs, C.byref(fr)), "pack_frame")
+        n = int(v.n_total)
+        return ViewData(chunks[:max(n, 1)].copy() if n else np.zeros((1, 96), np.uint8), grid, fr,
+                        int(v.n_visible), n, int(v.camera_chunk), cam.width, cam.height)
+
+
 def build_local_flat(dense: np.ndarray) -> bytes:
     """build_local (src/voxel/tree.rs:89-204) of one dense chunk on the host, as the flat
     vx_local_tree record the device build writes."""
