@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import threading
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -216,6 +217,18 @@ class CameraParams:
                         self.flags, self.frame_index)
 
 
+_tls = threading.local()
+
+
+def _chunk_scratch() -> np.ndarray:
+    """A per-thread buffer of VX_MAX_CHUNKS chunk records for the builders to write into;
+    callers copy out the entries the builder returns."""
+    buf = getattr(_tls, "chunks", None)
+    if buf is None:
+        buf = _tls.chunks = np.empty((A.VX_MAX_CHUNKS, 96), dtype=np.uint8)
+    return buf
+
+
 @dataclass
 class ViewData:
     """Everything one view uploads: chunk list, grid and frame record."""
@@ -241,14 +254,14 @@ def build_view(world: World, cam: CameraParams, max_pairs: int = 0) -> ViewData:
     """Renderer::build_chunk_list + the GpuFrame packing of Renderer::render."""
     L = A.host()
     c = cam.to_c()
-    chunks = np.zeros((A.VX_MAX_CHUNKS, 96), dtype=np.uint8)
-    grid = np.zeros(A.VX_GRID_LEN, dtype=np.uint32)
+    chunks = _chunk_scratch()
+    grid = np.empty(A.VX_GRID_LEN, dtype=np.uint32)  # ... and every grid cell
     v = A.View()
     A.check(L.vxw_build_view(world.handle, C.byref(c), A.ptr(chunks), A.ptr(grid), C.byref(v)), "build_view")
     fr = A.Frame()
     A.check(L.vxw_pack_frame(C.byref(c), C.byref(v), max_pairs, C.byref(fr)), "pack_frame")
     n = int(v.n_total)
-    return ViewData(chunks[:max(n, 1)].copy() if n else np.zeros((1, 96), np.uint8), grid, fr,
+    return ViewData(chunks[:n].copy() if n else np.zeros((1, 96), np.uint8), grid, fr,
                     int(v.n_visible), n, int(v.camera_chunk), cam.width, cam.height)
 
 
@@ -349,8 +362,8 @@ def build_view_items(world: World, cam: CameraParams, items: np.ndarray, cfg: Lo
     L = A.host()
     c = cam.to_c()
     it = np.ascontiguousarray(items, dtype=A.RENDER_ITEM_DTYPE)
-    chunks = np.zeros((A.VX_MAX_CHUNKS, 96), dtype=np.uint8)
-    grid = np.zeros(A.VX_GRID_LEN, dtype=np.uint32)
+    chunks = _chunk_scratch()
+    grid = np.empty(A.VX_GRID_LEN, dtype=np.uint32)  # ... and every grid cell
     v = A.View()
     A.check(L.vxw_build_view_items(world.handle, C.byref(c), A.ptr(it) if it.size else None, it.size,
                                    C.byref((cfg or LodConfig()).to_c()), A.ptr(chunks), A.ptr(grid), C.byref(v)),
@@ -358,7 +371,7 @@ def build_view_items(world: World, cam: CameraParams, items: np.ndarray, cfg: Lo
     fr = A.Frame()
     A.check(L.vxw_pack_frame(C.byref(c), C.byref(v), max_pairs, C.byref(fr)), "pack_frame")
     n = int(v.n_total)
-    return ViewData(chunks[:max(n, 1)].copy() if n else np.zeros((1, 96), np.uint8), grid, fr,
+    return ViewData(chunks[:n].copy() if n else np.zeros((1, 96), np.uint8), grid, fr,
                     int(v.n_visible), n, int(v.camera_chunk), cam.width, cam.height)
 
 
@@ -395,15 +408,15 @@ class ViewSnapshot:
     def build_view(self, cam: CameraParams, max_pairs: int = 0) -> ViewData:
         L = A.host()
         c = cam.to_c()
-        chunks = np.zeros((A.VX_MAX_CHUNKS, 96), dtype=np.uint8)
-        grid = np.zeros(A.VX_GRID_LEN, dtype=np.uint32)
+        chunks = _chunk_scratch()
+        grid = np.empty(A.VX_GRID_LEN, dtype=np.uint32)  # ... and every grid cell
         v = A.View()
         A.check(L.vxw_build_view_snapshot(self._h, C.byref(c), A.ptr(chunks), A.ptr(grid), C.byref(v)),
                 "build_view_snapshot")
         fr = A.Frame()
         A.check(L.vxw_pack_frame(C.byref(c), C.byref(v), max_pairs, C.byref(fr)), "pack_frame")
         n = int(v.n_total)
-        return ViewData(chunks[:max(n, 1)].copy() if n else np.zeros((1, 96), np.uint8), grid, fr,
+        return ViewData(chunks[:n].copy() if n else np.zeros((1, 96), np.uint8), grid, fr,
                         int(v.n_visible), n, int(v.camera_chunk), cam.width, cam.height)
 
 
