@@ -186,6 +186,17 @@ int vx_resize(vx_ctx *ctx, uint32_t width, uint32_t height);
 /* VX_E_OVERFLOW for a pool above 4 GiB - 16 bytes (kernels address pools with 32-bit
  * byte offsets). */
 int vx_pool_sync(vx_ctx *ctx, int pool, const void *data, size_t bytes, const vx_range *dirty, size_t n_dirty);
+/* A pool mirror's state (GpuMirror's cap and uploaded_bytes, src/render/pools.rs:1-9):
+ * device capacity, bytes mirrored, bytes uploaded so far, and the context's pool
+ * generation (bumped whenever a pool's device buffer or length changes). */
+typedef struct vx_pool_mirror {
+    uint64_t cap, len, uploaded_bytes;
+    uint32_t generation, pad;
+} vx_pool_mirror;
+int vx_pool_mirror_get(vx_ctx *ctx, int pool, vx_pool_mirror *out);
+/* Copies the first `bytes` of a pool's device mirror to the host (bytes <= its length);
+ * no counterpart in the reference beyond a buffer readback. */
+int vx_pool_read(vx_ctx *ctx, int pool, void *out, size_t bytes);
 int vx_set_chunks(vx_ctx *ctx, const vx_gpu_chunk *chunks, uint32_t n_total, const uint32_t *grid);
 /* spec = frame flags & SPEC_MASK, spec_hi = spec_hi word (the pipeline key of
  * Renderer::render, src/render/mod.rs:1597). Foliage tufts and the Snell bend need

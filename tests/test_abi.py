@@ -85,7 +85,7 @@ def test_struct_sizes_match_the_c_header(tmp_path):
              "vx_ctx_desc": A.CtxDesc, "vx_tuning": A.Tuning, "vx_pair_demand": A.PairDemand,
              "vx_timings": A.Timings, "vx_hit": A.Hit, "vx_view": A.ViewDesc, "vx_batch_tuning": A.BatchTuning,
              "vx_serve_desc": A.ServeDesc, "vx_serve_result": A.ServeResult, "vx_serve_stats": A.ServeStats,
-             "vx_march_stats": A.MarchStats}
+             "vx_march_stats": A.MarchStats, "vx_pool_mirror": A.PoolMirror}
     names_np = {"vx_sec_hit": A.SEC_DTYPE}
     src = tmp_path / "sz.c"
     body = "".join(f'    printf("{n} %zu\\n", sizeof({n}));\n' for n in list(names) + list(names_np))

@@ -170,6 +170,11 @@ VXW_ABSENT, VXW_EMPTY, VXW_SOLID = 0, 1, 2
 VXW_MAX_LOD = 12
 
 
+class PoolMirror(C.Structure):  # vx_pool_mirror
+    _fields_ = [("cap", C.c_uint64), ("len", C.c_uint64), ("uploaded_bytes", C.c_uint64),
+                ("generation", C.c_uint32), ("pad", C.c_uint32)]
+
+
 class PoolStats(C.Structure):
     _fields_ = [("live_elems", C.c_uint64), ("referenced_elems", C.c_uint64),
                 ("live_runs", C.c_uint64), ("len", C.c_uint64)]
@@ -326,6 +331,8 @@ _sig_march = {
     "vx_resize": (C.c_int, [P, C.c_uint32, C.c_uint32]),
     "vx_pool_sync": (C.c_int, [P, C.c_int, P, C.c_size_t, P, C.c_size_t]),
     "vx_set_chunks": (C.c_int, [P, P, C.c_uint32, P]),
+    "vx_pool_mirror_get": (C.c_int, [P, C.c_int, C.POINTER(PoolMirror)]),
+    "vx_pool_read": (C.c_int, [P, C.c_int, P, C.c_size_t]),
     "vx_render": (C.c_int, [P, C.POINTER(Frame), C.c_uint32, C.c_uint32]),
     "vx_sync": (C.c_int, [P]),
     "vx_read_vis": (C.c_int, [P, P, C.c_size_t]),

@@ -55,6 +55,7 @@ struct vx_ctx {
     vxh::DevBuf pool[3];
     size_t pool_len[3] = {0, 0, 0};
     bool pool_synced[3] = {false, false, false};
+    uint64_t pool_uploaded[3] = {0, 0, 0};  // GpuMirror::uploaded_bytes per pool
     vxh::DevBuf chunks, grid, vis, shadow, hiz, pairs, counters;
     vxh::DevBuf cost, order;  // per-tile cycles of the last frame, and the launch order
     vxh::DevBuf heat;         // FLAG_HEATMAP: marched iterations per pixel

@@ -275,6 +275,7 @@ int vx_pool_sync(vx_ctx *c, int pool, const void *data, size_t bytes, const vx_r
         // A new context has no mirror contents yet: its first sync is a full upload
         // whatever the dirty list says (another mirror may have drained it already).
         if (bytes) HIPX(hipMemcpyAsync(b.p, data, bytes, hipMemcpyHostToDevice, c->stream));
+        c->pool_uploaded[pool] += bytes;
         c->pool_len[pool] = bytes;
         c->pool_synced[pool] = true;
         HIPX(hipStreamSynchronize(c->stream));
@@ -291,6 +292,7 @@ int vx_pool_sync(vx_ctx *c, int pool, const void *data, size_t bytes, const vx_r
         HIPX(hipMalloc(&b.p, cap));
         b.cap = cap;
         HIPX(hipMemcpyAsync(b.p, data, bytes, hipMemcpyHostToDevice, c->stream));
+        c->pool_uploaded[pool] += bytes;
         c->pool_len[pool] = bytes;
         HIPX(hipStreamSynchronize(c->stream));
         return VX_OK;
@@ -310,14 +312,36 @@ int vx_pool_sync(vx_ctx *c, int pool, const void *data, size_t bytes, const vx_r
     }
     if (merged.size() > 512) {
         HIPX(hipMemcpyAsync(b.p, data, bytes, hipMemcpyHostToDevice, c->stream));
+        c->pool_uploaded[pool] += bytes;
     } else {
         for (auto &r : merged) {
             uint64_t s = r.first & ~3ull, e = std::min<uint64_t>((r.second + 3) & ~3ull, bytes);
-            if (e > s)
+            if (e > s) {
                 HIPX(hipMemcpyAsync((char *)b.p + s, (const char *)data + s, e - s, hipMemcpyHostToDevice,
                                     c->stream));
+                c->pool_uploaded[pool] += e - s;
+            }
         }
     }
+    HIPX(hipStreamSynchronize(c->stream));
+    return VX_OK;
+}
+
+int vx_pool_mirror_get(vx_ctx *c, int pool, vx_pool_mirror *o) {
+    if (!c || !o || pool < 0 || pool > 2) return VX_E_ARG;
+    std::memset(o, 0, sizeof *o);
+    o->cap = c->pool[pool].cap;
+    o->len = c->pool_len[pool];
+    o->uploaded_bytes = c->pool_uploaded[pool];
+    o->generation = c->pool_gen;
+    return VX_OK;
+}
+
+int vx_pool_read(vx_ctx *c, int pool, void *out, size_t bytes) {
+    if (!c || pool < 0 || pool > 2 || (!out && bytes) || bytes > c->pool_len[pool]) return VX_E_ARG;
+    if (!bytes) return VX_OK;
+    HIPX(hipSetDevice(c->device));
+    HIPX(hipMemcpyAsync(out, c->pool[pool].p, bytes, hipMemcpyDeviceToHost, c->stream));
     HIPX(hipStreamSynchronize(c->stream));
     return VX_OK;
 }
