@@ -7,6 +7,19 @@
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it. The
  * product library never links or calls it.
  *
+ * What pins it (tests/): the reference's literal tests (key packing, pair capacity,
+ * tree round trips, ray_dir against view_proj, the journal file); float64 brute force
+ * (the single-chunk march at voxel sizes 1-8, whole frames over multi-chunk worlds: first
+ * hit, face, distance and the primary sun bit; leaf cutouts at fill 1 and 0; the
+ * secondary legs and their sun tests, including the sea-surface jump); closed forms
+ * (the heatmap counts on full chunks, the fade dither's exclusivity); invariants (Hi-Z
+ * on = off, thread decomposition).
+ * Parity unpinned (restated, checked only against the GPU): the water-interface probe
+ * and water-line skip seen from under water, the non-caster rule of sun rays through
+ * emitter blocks, and leaf cutouts at fills strictly between 0 and 1 (the hash
+ * threshold). Foliage tufts and the Snell bend are not restated; the library refuses
+ * them.
+ *
  * Every struct here is an independent restatement of the reference layouts:
  *   vxo_frame  <- GpuFrame  (src/render/mod.rs:398-466, 352 bytes)
  *   vxo_chunk  <- GpuChunk  (src/render/mod.rs:468-489, 96 bytes)
