@@ -14,11 +14,13 @@
  * secondary legs and their sun tests, including the sea-surface jump); closed forms
  * (the heatmap counts on full chunks, the fade dither's exclusivity); invariants (Hi-Z
  * on = off, thread decomposition).
+ * Also pinned on whole frames: water surfaces, glass and glowstone hits from above the
+ * water, rays under water skipping water (with the water-line skip), and sun rays
+ * passing water, glass and (emitter worlds) glowstone.
  * Parity unpinned (restated, checked only against the GPU): the water-interface probe
- * and water-line skip seen from under water, the non-caster rule of sun rays through
- * emitter blocks, and leaf cutouts at fills strictly between 0 and 1 (the hash
- * threshold). Foliage tufts and the Snell bend are not restated; the library refuses
- * them.
+ * (a water voxel entered from a water cell, which rays from air never meet first) and
+ * leaf cutouts at fills strictly between 0 and 1 (the hash threshold). Foliage tufts and
+ * the Snell bend are not restated; the library refuses them.
  *
  * Every struct here is an independent restatement of the reference layouts:
  *   vxo_frame  <- GpuFrame  (src/render/mod.rs:398-466, 352 bytes)
