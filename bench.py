@@ -228,6 +228,8 @@ def run_batch(args, out):
         out["incoherent"] = run_incoherent(args)
     if not args.no_build:
         out["chunk_build"] = run_chunk_build(args)
+    if not args.no_view_setup:
+        out["view_setup"] = run_view_setup(args)
 
     if args.no_cpu_baseline:
         out["cpu_baseline"] = None
@@ -268,6 +270,56 @@ def run_serving(args):
     res["workload"] = wl.name
     _log(f"serving: p50 {res['p50_ms']} ms p99 {res['p99_ms']} ms at {args.serve_qps} qps")
     return res
+
+
+def run_view_setup(args):
+    """Setting up the 64 views of the batch workload on the 8192-chunk world: chunk lists
+    and grids built on the host (build_view per view, then vx_batch_set_views) against
+    built on the device from the chunk table (vx_batch_set_cameras). The device lists are
+    checked against the host's for every view."""
+    from voxelcraft_menu_amd import workloads as W
+    from voxelcraft_menu_amd.renderer import Batch, Renderer
+    from voxelcraft_menu_amd.world import ViewSnapshot, build_view
+
+    wl = W.WORKLOADS[4]
+    world = W.make_world(wl)
+    cams = W.random_cameras(wl, 64, seed=2)
+    specs = [W.spec_words(c.flags, world) for c in cams]
+    snap = ViewSnapshot(world)
+    table = snap.chunk_table()
+    r = Renderer(64, 64)
+    r.sync_world(world)
+    b = Batch(r, 64)
+    b.set_chunk_table(table)
+
+    def host_setup():
+        b.set_views([snap.build_view(c) for c in cams], specs)
+
+    def device_setup():
+        b.set_cameras(cams, specs)
+
+    res = {}
+    for name, fn in (("host", host_setup), ("device", device_setup)):
+        fn()
+        reps = 5
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        res[name] = (time.perf_counter() - t0) / reps
+    b.set_cameras(cams, specs)
+    for i, c in enumerate(cams):
+        v = build_view(world, c)
+        ch, grid = b.read_chunks(i)
+        if b.view_counts(i) != (v.n_total, v.n_visible, v.camera_chunk) or not np.array_equal(
+                ch, v.chunks[:v.n_total]) or not np.array_equal(grid, v.grid):
+            _fail(f"device chunk list of view {i} differs from the host's")
+    b.close()
+    r.close()
+    out = {"workload": "64-views-of-2048x2048x512", "views": 64, "table_chunks": int(len(table)),
+           "host_ms": round(res["host"] * 1e3, 3), "device_ms": round(res["device"] * 1e3, 3),
+           "host_over_device": round(res["host"] / res["device"], 1), "checked_against_host": 64}
+    _log(f"view setup: 64 views in {out['host_ms']} ms with host lists, {out['device_ms']} ms built on the device")
+    return out
 
 
 def run_chunk_build(args):
@@ -472,6 +524,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving stream (workload 2 only)")
     ap.add_argument("--no-incoherent", action="store_true", help="skip the incoherent batch (workload 2 only)")
     ap.add_argument("--no-build", action="store_true", help="skip the device chunk build (workload 2 only)")
+    ap.add_argument("--no-view-setup", action="store_true", help="skip the view-setup comparison (workload 2 only)")
     ap.add_argument("--serve-qps", type=float, default=1000.0, help="offered view requests per second")
     ap.add_argument("--serve-seconds", type=float, default=4.0, help="seconds of request arrivals")
     ap.add_argument("--serve-edit-rate", type=float, default=2.0, help="edit batches (1000 ops) per second")

@@ -183,6 +183,16 @@ int vxw_snapshot_take(vxw_snapshot *s, const vxw_world *w);
 uint32_t vxw_snapshot_emitter_chunks(const vxw_snapshot *s);
 int vxw_build_view_snapshot(const vxw_snapshot *s, const vxw_camera *cam, vx_gpu_chunk *chunks, uint32_t *grid,
                             vxw_view *out);
+/* The chunk table the device builds views from (vx_batch_set_chunk_table): the GpuChunk
+ * record (fade 1) of every snapshot chunk with solid voxels, in the order vxw_build_view
+ * takes them; returns the count (out may be NULL). */
+size_t vxw_snapshot_chunks(const vxw_snapshot *s, vx_gpu_chunk *out, size_t cap);
+/* Frustum::from_camera (src/math.rs:69-117) of a camera: six planes (nx, ny, nz, d); a
+ * point p is outside plane i when nx*px + ny*py + nz*pz + d < 0. */
+int vxw_frustum_planes(const vxw_camera *cam, float planes[24]);
+/* The GpuFrame of a camera whose chunk list the device builds (vx_batch_set_cameras):
+ * vxw_pack_frame with the camera's grid origin, no chunks counted yet. */
+int vxw_pack_camera_frame(const vxw_camera *cam, uint32_t max_pairs, vx_gpu_frame *out);
 /* Packs the GpuFrame record for a camera and view (max_pairs 0 = pairs_for(tiles)). */
 int vxw_pack_frame(const vxw_camera *cam, const vxw_view *view, uint32_t max_pairs, vx_gpu_frame *out);
 /* render::pairs_for (src/render/mod.rs:27-30). */

@@ -275,6 +275,27 @@ int vx_batch_destroy(vx_batch *b);
 /* Replaces the view set (1..max_views views). Targets, Hi-Z and tile costs start at 0,
  * as for a new Renderer. */
 int vx_batch_set_views(vx_batch *b, const vx_view *views, uint32_t n);
+
+/* Chunk lists built on the device (no counterpart in the reference, which builds each
+ * view's list on the CPU: partition_for_grid + build_chunk_list, src/render/mod.rs:82-138,
+ * :1447-1564). vx_batch_set_chunk_table uploads the LOD-0 GpuChunk records of a render
+ * set (fade 1, in render-set order, at most VX_CHUNK_TABLE_MAX; e.g. vxw_snapshot_chunks);
+ * call it again after edits, before the next vx_batch_set_cameras. vx_batch_set_cameras
+ * then sets views from cameras alone: each view's frame as vxw_pack_frame gives it (its
+ * chunk_count and camera_chunk are ignored) and its frustum planes (vxw_frustum_planes);
+ * the device culls, sorts and packs the view's chunk list and grid, which are the bytes
+ * vxw_build_view gives for that camera. vx_batch_view_counts / vx_batch_read_chunks return
+ * what was built. */
+#define VX_CHUNK_TABLE_MAX 8192u
+typedef struct vx_camera_view {
+    const vx_gpu_frame *frame;
+    uint32_t spec, spec_hi;
+    float planes[24];           /* (nx, ny, nz, d) x 6 */
+} vx_camera_view;
+int vx_batch_set_chunk_table(vx_batch *b, const vx_gpu_chunk *chunks, uint32_t n);
+int vx_batch_set_cameras(vx_batch *b, const vx_camera_view *views, uint32_t n);
+int vx_batch_view_counts(vx_batch *b, uint32_t view, uint32_t *n_total, uint32_t *n_visible, uint32_t *camera_chunk);
+int vx_batch_read_chunks(vx_batch *b, uint32_t view, vx_gpu_chunk *chunks, size_t cap, uint32_t *grid);
 int vx_batch_set_tuning(vx_batch *b, const vx_batch_tuning *t);
 int vx_batch_get_tuning(vx_batch *b, vx_batch_tuning *t);
 /* Enqueues one frame of every view on the context stream (asynchronous). */

@@ -83,7 +83,7 @@ def test_struct_sizes_match_the_c_header(tmp_path):
 
     names = {"vx_gpu_frame": A.Frame, "vx_gpu_chunk": A.Chunk, "vx_edit": A.Edit, "vx_range": A.Range,
              "vx_ctx_desc": A.CtxDesc, "vx_tuning": A.Tuning, "vx_pair_demand": A.PairDemand,
-             "vx_timings": A.Timings, "vx_hit": A.Hit, "vx_view": A.ViewDesc, "vx_batch_tuning": A.BatchTuning,
+             "vx_timings": A.Timings, "vx_hit": A.Hit, "vx_view": A.ViewDesc, "vx_camera_view": A.CameraView, "vx_batch_tuning": A.BatchTuning,
              "vx_serve_desc": A.ServeDesc, "vx_serve_result": A.ServeResult, "vx_serve_stats": A.ServeStats,
              "vx_march_stats": A.MarchStats, "vx_pool_mirror": A.PoolMirror}
     names_np = {"vx_sec_hit": A.SEC_DTYPE}

@@ -18,6 +18,7 @@ MARCH_LIB = os.path.join(HERE, "libvxmarch.so")
 VX_OK = 0
 VX_TILE = 8
 VX_MAX_CHUNKS = 8191
+VX_CHUNK_TABLE_MAX = 8192
 VX_MAX_PAIRS = 1 << 20
 VX_MAX_TILES = 1 << 19
 VX_GRID = (32, 8, 32)
@@ -221,6 +222,11 @@ class ViewDesc(C.Structure):
                 ("grid", C.c_void_p), ("spec", C.c_uint32), ("spec_hi", C.c_uint32)]
 
 
+class CameraView(C.Structure):
+    """vx_camera_view: a batch view given by its camera (chunk list built on the device)."""
+    _fields_ = [("frame", C.c_void_p), ("spec", C.c_uint32), ("spec_hi", C.c_uint32), ("planes", C.c_float * 24)]
+
+
 class BatchTuning(C.Structure):
     _fields_ = [("mode", C.c_uint32), ("occupancy", C.c_uint32), ("variant", C.c_uint32),
                 ("ordered", C.c_uint32), ("queue_chunk", C.c_uint32), ("waves_per_simd", C.c_uint32),
@@ -317,6 +323,9 @@ _sig_host = {
     "vxw_snapshot_take": (C.c_int, [P, P]),
     "vxw_snapshot_emitter_chunks": (C.c_uint32, [P]),
     "vxw_build_view_snapshot": (C.c_int, [P, C.POINTER(Camera), P, P, C.POINTER(View)]),
+    "vxw_snapshot_chunks": (C.c_size_t, [P, P, C.c_size_t]),
+    "vxw_frustum_planes": (C.c_int, [C.POINTER(Camera), P]),
+    "vxw_pack_camera_frame": (C.c_int, [C.POINTER(Camera), C.c_uint32, C.POINTER(Frame)]),
     "vxw_journal_parse": (C.c_int, [P, C.c_size_t, C.c_int32, C.c_int32, C.POINTER(JournalInfo), P, C.c_size_t,
                                     C.c_char_p, C.c_size_t]),
     "vxw_journal_write": (C.c_int, [P, C.c_size_t, C.c_int32, C.c_int32, C.POINTER(Player), P, P, C.c_size_t,
@@ -349,6 +358,11 @@ _sig_march = {
     "vx_batch_create": (C.c_int, [P, C.c_uint32, C.POINTER(P)]),
     "vx_batch_destroy": (C.c_int, [P]),
     "vx_batch_set_views": (C.c_int, [P, P, C.c_uint32]),
+    "vx_batch_set_chunk_table": (C.c_int, [P, P, C.c_uint32]),
+    "vx_batch_set_cameras": (C.c_int, [P, P, C.c_uint32]),
+    "vx_batch_view_counts": (C.c_int, [P, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                       C.POINTER(C.c_uint32)]),
+    "vx_batch_read_chunks": (C.c_int, [P, C.c_uint32, P, C.c_size_t, P]),
     "vx_batch_set_tuning": (C.c_int, [P, C.POINTER(BatchTuning)]),
     "vx_batch_get_tuning": (C.c_int, [P, C.POINTER(BatchTuning)]),
     "vx_batch_render": (C.c_int, [P]),

@@ -50,6 +50,8 @@ struct vx_batch {
     // Secondary bounce: two 16-byte leg records per pixel, two leg queues of sec_cap entries.
     bool sec_on = false;
     DevBuf sec_out, sec_queue;
+    DevBuf table, cams, list_counts;  // device chunk lists: chunk table, per-view cameras, counts
+    uint32_t n_table = 0;
     uint32_t sec_cap = 0;
     uint64_t vk_pool_gen = 0;  // pool generation the device descriptors' pool fields were written for
     vx_batch_tuning tn{};
@@ -264,6 +266,251 @@ int view_buffers(vx_batch *b, uint32_t n_views, uint32_t n_tiles, uint64_t n_px,
     return VX_OK;
 }
 
+
+}  // namespace
+
+namespace vxd {
+
+// Per-view chunk lists on the device: partition_for_grid + build_chunk_list
+// (src/render/mod.rs:110-138, :1447-1564) over a table of LOD-0 GpuChunk records (fade 1)
+// in render-set order, for a camera given by its frustum planes, position and grid origin.
+struct ListCam {
+    float planes[24];  // Frustum::from_camera: (nx, ny, nz, d) x 6
+    float pos[3];
+    int32_t gmin[3];
+    uint32_t pad[2];
+};
+static_assert(sizeof(ListCam) == 128, "ListCam layout");
+constexpr uint32_t LIST_BLOCK = 1024, LIST_KEYS = 8192;
+
+// One block per view. Each table record is classified (visible: inside the six planes;
+// off-screen caster: outside but on the 32x8x32 grid; else dropped) and keyed by
+// (class, f32 bits of its AABB distance, table index): an ascending sort of the keys is
+// the two stable distance sorts, visible first. The block then copies the records in that
+// order, fills the grid (one cell per LOD-0 chunk) and finds the camera chunk. counts
+// gets (records, visible, camera chunk) per view. The f32 arithmetic is the host's, in
+// its order, so the lists are the host builder's bytes.
+__global__ __launch_bounds__(LIST_BLOCK) void k_view_lists(const vx_gpu_chunk *__restrict__ table, uint32_t n_table,
+                                                           const ListCam *__restrict__ cams,
+                                                           vx_gpu_chunk *__restrict__ out_chunks,
+                                                           uint32_t *__restrict__ grids, uint32_t *__restrict__ counts) {
+    __shared__ unsigned long long keys[LIST_KEYS];
+    __shared__ uint32_t s_vis_end, s_off_end, s_cam;
+    const uint32_t v = blockIdx.x, tid = threadIdx.x;
+    const ListCam &cam = cams[v];
+    uint32_t n2 = 1;
+    while (n2 < n_table) n2 <<= 1;
+    if (tid == 0) {
+        s_vis_end = 0;
+        s_off_end = 0;
+        s_cam = VX_NO_CHUNK;
+    }
+    for (uint32_t i = tid; i < n2; i += LIST_BLOCK) {
+        unsigned long long key = ~0ull;
+        if (i < n_table) {
+            const vx_gpu_chunk &c = table[i];
+            const float mn[3] = {c.aabb_min[0], c.aabb_min[1], c.aabb_min[2]};
+            const float mx[3] = {c.aabb_max[0], c.aabb_max[1], c.aabb_max[2]};
+            bool inside = true;
+            for (int p = 0; p < 6; p++) {
+                const float *pl = cam.planes + 4 * p;
+                const float q0 = pl[0] > 0.0f ? mx[0] : mn[0], q1 = pl[1] > 0.0f ? mx[1] : mn[1],
+                            q2 = pl[2] > 0.0f ? mx[2] : mn[2];
+                const float dd = pl[0] * q0 + pl[1] * q1 + pl[2] * q2;
+                if (dd + pl[3] < 0.0f) inside = false;
+            }
+            float g[3];
+            for (int k = 0; k < 3; k++) g[k] = fmaxf(fmaxf(mn[k] - cam.pos[k], cam.pos[k] - mx[k]), 0.0f);
+            const float dist = sqrtf(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+            const unsigned long long dbits = (unsigned long long)__float_as_uint(dist) << 32;
+            if (inside) {
+                key = dbits | i;
+            } else {
+                // grid_span (src/render/mod.rs:93-108) of the record's key
+                const int32_t span = 1 << c.lod, size = 64 << c.lod;
+                bool on_grid = true;
+                const int32_t dims[3] = {(int32_t)VX_GRID_X, (int32_t)VX_GRID_Y, (int32_t)VX_GRID_Z};
+                for (int k = 0; k < 3; k++) {
+                    const int32_t base = ((int32_t)c.origin[k] / size) * span - cam.gmin[k];
+                    on_grid = on_grid && max(base, 0) < min(base + span, dims[k]);
+                }
+                if (on_grid) key = (1ull << 63) | dbits | i;
+            }
+        }
+        keys[i] = key;
+    }
+    __syncthreads();
+    // bitonic sort, ascending
+    for (uint32_t k = 2; k <= n2; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = tid; i < n2; i += LIST_BLOCK) {
+                const uint32_t l = i ^ j;
+                if (l > i) {
+                    const unsigned long long a = keys[i], b = keys[l];
+                    if ((a > b) == ((i & k) == 0)) {
+                        keys[i] = b;
+                        keys[l] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    // class boundaries: visible keys [0, vis_end), casters [vis_end, off_end)
+    for (uint32_t i = tid; i < n2; i += LIST_BLOCK) {
+        const unsigned long long k0 = keys[i];
+        const uint32_t c0 = k0 == ~0ull ? 2u : (uint32_t)(k0 >> 63);
+        const unsigned long long k1 = i + 1 < n2 ? keys[i + 1] : ~0ull;
+        const uint32_t c1 = k1 == ~0ull ? 2u : (uint32_t)(k1 >> 63);
+        if (c0 == 0u && c1 != 0u) s_vis_end = i + 1;
+        if (c0 != 2u && c1 == 2u) s_off_end = i + 1;
+    }
+    uint32_t *grid = grids + (size_t)v * VX_GRID_LEN;
+    for (uint32_t j = tid; j < VX_GRID_LEN; j += LIST_BLOCK) grid[j] = VX_NO_CHUNK;
+    __syncthreads();
+    const uint32_t vis_all = s_vis_end, off_all = s_off_end > vis_all ? s_off_end - vis_all : 0u;
+    const uint32_t nv = min(vis_all, (uint32_t)VX_MAX_CHUNKS);
+    const uint32_t no = min(off_all, (uint32_t)VX_MAX_CHUNKS - nv);
+    vx_gpu_chunk *out = out_chunks + (size_t)v * n_table;
+    for (uint32_t r = tid; r < nv + no; r += LIST_BLOCK) {
+        const uint32_t at = r < nv ? r : vis_all + (r - nv);
+        const uint32_t idx = (uint32_t)keys[at];
+        const vx_gpu_chunk &c = table[idx];
+        const uint4 *src = (const uint4 *)&c;
+        uint4 *dst = (uint4 *)&out[r];
+#pragma unroll
+        for (int w = 0; w < 6; w++) dst[w] = src[w];
+        const int32_t span = 1 << c.lod, size = 64 << c.lod;
+        int32_t cell[3];
+        bool on_grid = true;
+        const int32_t dims[3] = {(int32_t)VX_GRID_X, (int32_t)VX_GRID_Y, (int32_t)VX_GRID_Z};
+        for (int k = 0; k < 3; k++) {
+            cell[k] = ((int32_t)c.origin[k] / size) * span - cam.gmin[k];
+            on_grid = on_grid && cell[k] >= 0 && cell[k] < dims[k];
+        }
+        if (on_grid) grid[(uint32_t)cell[0] + (uint32_t)cell[1] * VX_GRID_X + (uint32_t)cell[2] * VX_GRID_X * VX_GRID_Y] = r;
+        if (r < nv && cam.pos[0] >= c.aabb_min[0] && cam.pos[1] >= c.aabb_min[1] && cam.pos[2] >= c.aabb_min[2] &&
+            cam.pos[0] <= c.aabb_max[0] && cam.pos[1] <= c.aabb_max[1] && cam.pos[2] <= c.aabb_max[2])
+            atomicMin(&s_cam, r);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        counts[3 * v] = nv + no;
+        counts[3 * v + 1] = nv;
+        counts[3 * v + 2] = s_cam;
+    }
+}
+
+}  // namespace vxd
+
+namespace {
+
+// Slots, descriptors and offsets of a view set (the first half of vx_batch_set_views).
+struct ViewPlan {
+    std::vector<ViewK> vk;
+    std::vector<uint32_t> toff, slot_of, view_at;
+    std::vector<uint64_t> poff;
+    uint32_t cb[N_CLASSES + 1] = {0, 0, 0, 0, 0};
+    size_t recs = 0;
+};
+
+template <class FrameOf, class SpecOf, class SpecHiOf, class TotalOf>
+int plan_views(const vx_batch *b, uint32_t n, FrameOf frame_of, SpecOf spec_of, SpecHiOf spec_hi_of, TotalOf total_of,
+               ViewPlan &pl) {
+    pl.vk.assign(n, ViewK{});
+    pl.toff.assign(n + 1, 0);
+    pl.poff.assign(n + 1, 0);
+    pl.slot_of.assign(n, 0);
+    pl.view_at.assign(n, 0);
+    {
+        // Slots: the views of class 0, then class 1, ... (caller order inside a class).
+        uint32_t next = 0;
+        for (uint32_t c = 0; c < N_CLASSES; c++)
+            for (uint32_t v = 0; v < n; v++) {
+                const uint32_t fl = frame_of(v)->flags;
+                if (view_class((fl & VX_FLAG_HIZ) != 0u, (fl & VX_FLAG_UNDERWATER) != 0u) != c) continue;
+                pl.slot_of[v] = next;
+                pl.view_at[next++] = v;
+            }
+    }
+    for (uint32_t v = 0; v < n; v++) {
+        const uint32_t src = pl.view_at[v];
+        const vx_gpu_frame *fr = frame_of(src);
+        const uint32_t total = total_of(src);
+        int rc = vxh::check_frame(fr, total);
+        if (rc) return rc;
+        const uint32_t cls = view_class((fr->flags & VX_FLAG_HIZ) != 0u, (fr->flags & VX_FLAG_UNDERWATER) != 0u);
+        ViewK &k = pl.vk[v];
+        std::memset(&k, 0, sizeof k);
+        k.f = vxh::make_frame(fr);
+        if ((rc = vxh::make_spec(spec_of(src), spec_hi_of(src), fr->flags, k.s))) return rc;
+        k.chunk_off = (uint32_t)pl.recs;
+        k.n_chunks = total;
+        k.grid_off = v * VX_GRID_LEN;
+        k.tile_off = pl.toff[v];
+        k.px_off_lo = (uint32_t)pl.poff[v];
+        k.px_off_hi = (uint32_t)(pl.poff[v] >> 32);
+        const uint64_t nt = (uint64_t)k.f.tiles[0] * k.f.tiles[1];
+        if (pl.toff[v] + nt > 0xFFFFFFFFull) return VX_E_OVERFLOW;
+        pl.toff[v + 1] = pl.toff[v] + (uint32_t)nt;
+        pl.poff[v + 1] = pl.poff[v] + (uint64_t)k.f.res[0] * k.f.res[1];
+        pl.recs += total;
+        for (uint32_t c = cls + 1; c <= N_CLASSES; c++) pl.cb[c] = pl.toff[v + 1];
+    }
+    if (pl.recs > 0xFFFFFFFFull) return VX_E_OVERFLOW;
+    return VX_OK;
+}
+
+// The second half: descriptors, offsets and header to the device, targets cleared, the
+// captured graph kept only when it still fits the view set.
+int install_views(vx_batch *b, uint32_t n, ViewPlan &pl, bool moved) {
+    vx_ctx *c = b->ctx;
+    hipStream_t st = c->stream;
+    int rc;
+    for (ViewK &k : pl.vk) view_pools(b, k);  // after view_buffers: the chunk and grid buffers are placed
+    const std::vector<uint32_t> &toff = pl.toff;
+    const std::vector<uint64_t> &poff = pl.poff;
+    const uint32_t *cb = pl.cb;
+    uint32_t hdr[BH_LEN] = {};
+    hdr[BH_TILES] = toff[n];
+    hdr[BH_VIEWS] = n;
+    for (uint32_t k = 0; k <= N_CLASSES; k++) hdr[BH_CB + k] = cb[k];
+    HIPX(hipMemcpyAsync(b->views.p, pl.vk.data(), (size_t)n * sizeof(ViewK), hipMemcpyHostToDevice, st));
+    HIPX(hipMemcpyAsync(b->tile_off.p, toff.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice, st));
+    HIPX(hipMemcpyAsync(b->hdr.p, hdr, sizeof hdr, hipMemcpyHostToDevice, st));
+    // New views start as a new Renderer would: empty targets, Hi-Z 0; costs 0.
+    if ((rc = vxh::fill_u64(st, b->vis.p, 0, poff[n])) || (rc = vxh::fill_u32(st, b->shadow.p, 0, (poff[n] + 1) / 2)) ||
+        (rc = vxh::fill_u32(st, b->hiz.p, 0, toff[n])) || (rc = vxh::fill_u32(st, b->cost.p, 0, toff[n])))
+        return rc;
+    HIPX(hipStreamSynchronize(st));
+    // A captured graph stays valid for another view set only if it reads every count from
+    // the header and launches the same class kernels: not PLAIN (per-view launches), not
+    // FLAT with different class ranges (grid sizes), not PERSISTENT with a different set
+    // of non-empty classes.
+    bool same_classes = true, same_ranges = true;
+    for (uint32_t c = 0; c < N_CLASSES; c++) {
+        same_classes &= ((cb[c + 1] - cb[c]) != 0u) == ((b->graph_cb[c + 1] - b->graph_cb[c]) != 0u);
+        same_ranges &= (cb[c + 1] - cb[c]) == (b->graph_cb[c + 1] - b->graph_cb[c]);
+    }
+    if (moved || b->tn.mode == VX_BATCH_PLAIN || (b->tn.mode == VX_BATCH_FLAT && !same_ranges) || !same_classes)
+        drop_graph(b);
+    b->vk.swap(pl.vk);
+    b->vk_pool_gen = c->pool_gen;
+    b->toff.swap(pl.toff);
+    b->poff.swap(pl.poff);
+    b->slot_of.swap(pl.slot_of);
+    std::memcpy(b->cb, pl.cb, sizeof b->cb);
+    b->n_views = n;
+    b->n_tiles = b->toff[n];
+    b->n_px = b->poff[n];
+    b->rendered = false;
+    if (b->sec_on && (rc = secondary_buffers(b))) {
+        b->sec_on = false;
+        drop_graph(b);
+        return rc;
+    }
+    return VX_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -308,8 +555,9 @@ int vx_batch_destroy(vx_batch *b) {
     (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
     drop_graph(b);
-    DevBuf *bufs[] = {&b->views,  &b->tile_off, &b->hdr,   &b->chunks,  &b->grids,   &b->vis,      &b->shadow,
-                      &b->hiz,    &b->cost,     &b->order, &b->counters, &b->sec_out, &b->sec_queue};
+    DevBuf *bufs[] = {&b->views,   &b->tile_off, &b->hdr,       &b->chunks, &b->grids, &b->vis,
+                      &b->shadow,  &b->hiz,      &b->cost,      &b->order,  &b->counters, &b->sec_out,
+                      &b->sec_queue, &b->table,  &b->cams,      &b->list_counts};
     for (DevBuf *d : bufs) vxh::release(*d);
     for (auto &set : b->ring)
         for (auto &e : set)
@@ -321,106 +569,132 @@ int vx_batch_destroy(vx_batch *b) {
 
 int vx_batch_set_views(vx_batch *b, const vx_view *views, uint32_t n) {
     if (!b || !views || n == 0 || n > b->max_views) return VX_E_ARG;
-    std::vector<ViewK> vk(n);
-    std::vector<uint32_t> toff(n + 1, 0);
-    std::vector<uint64_t> poff(n + 1, 0);
-    std::vector<uint32_t> slot_of(n), view_at(n);
-    uint32_t cb[N_CLASSES + 1] = {0, 0, 0, 0, 0};
-    {
-        // Slots: the views of class 0, then class 1, ... (caller order inside a class).
-        uint32_t next = 0;
-        for (uint32_t c = 0; c < N_CLASSES; c++)
-            for (uint32_t v = 0; v < n; v++) {
-                if (!views[v].frame) return VX_E_ARG;
-                const uint32_t fl = views[v].frame->flags;
-                if (view_class((fl & VX_FLAG_HIZ) != 0u, (fl & VX_FLAG_UNDERWATER) != 0u) != c) continue;
-                slot_of[v] = next;
-                view_at[next++] = v;
-            }
-    }
-    size_t recs = 0;
-    for (uint32_t v = 0; v < n; v++) {
-        const vx_view &w = views[view_at[v]];
-        if (!w.frame || !w.grid || (!w.chunks && w.n_total) || w.n_total > VX_MAX_CHUNKS) return VX_E_ARG;
-        int rc = vxh::check_frame(w.frame, w.n_total);
-        if (rc) return rc;
-        const uint32_t cls = view_class((w.frame->flags & VX_FLAG_HIZ) != 0u, (w.frame->flags & VX_FLAG_UNDERWATER) != 0u);
-        ViewK &k = vk[v];
-        std::memset(&k, 0, sizeof k);
-        k.f = vxh::make_frame(w.frame);
-        if ((rc = vxh::make_spec(w.spec, w.spec_hi, w.frame->flags, k.s))) return rc;
-        k.chunk_off = (uint32_t)recs;
-        k.n_chunks = w.n_total;
-        k.grid_off = v * VX_GRID_LEN;
-        k.tile_off = toff[v];
-        k.px_off_lo = (uint32_t)poff[v];
-        k.px_off_hi = (uint32_t)(poff[v] >> 32);
-        const uint64_t nt = (uint64_t)k.f.tiles[0] * k.f.tiles[1];
-        if (toff[v] + nt > 0xFFFFFFFFull) return VX_E_OVERFLOW;
-        toff[v + 1] = toff[v] + (uint32_t)nt;
-        poff[v + 1] = poff[v] + (uint64_t)k.f.res[0] * k.f.res[1];
-        recs += w.n_total;
-        for (uint32_t c = cls + 1; c <= N_CLASSES; c++) cb[c] = toff[v + 1];
-    }
+    for (uint32_t v = 0; v < n; v++)
+        if (!views[v].frame || !views[v].grid || (!views[v].chunks && views[v].n_total) ||
+            views[v].n_total > VX_MAX_CHUNKS)
+            return VX_E_ARG;
+    ViewPlan pl;
+    int rc = plan_views(
+        b, n, [&](uint32_t v) { return views[v].frame; }, [&](uint32_t v) { return views[v].spec; },
+        [&](uint32_t v) { return views[v].spec_hi; }, [&](uint32_t v) { return views[v].n_total; }, pl);
+    if (rc) return rc;
     vx_ctx *c = b->ctx;
     HIPX(hipSetDevice(c->device));
     hipStream_t st = c->stream;
     HIPX(hipStreamSynchronize(st));
     bool moved = false;
-    int rc = view_buffers(b, n, toff[n], poff[n], recs, moved);
-    if (rc) {
+    if ((rc = view_buffers(b, n, pl.toff[n], pl.poff[n], pl.recs, moved))) {
         drop_graph(b);
         b->n_views = 0;
         return rc;
     }
-    for (ViewK &k : vk) view_pools(b, k);  // after view_buffers: the chunk and grid buffers are placed
-    std::vector<vx_gpu_chunk> cat(std::max<size_t>(recs, 1));
+    std::vector<vx_gpu_chunk> cat(std::max<size_t>(pl.recs, 1));
     std::vector<uint32_t> grids((size_t)n * VX_GRID_LEN);
     for (uint32_t v = 0; v < n; v++) {
-        const vx_view &w = views[view_at[v]];
-        if (w.n_total) std::memcpy(&cat[vk[v].chunk_off], w.chunks, (size_t)w.n_total * sizeof(vx_gpu_chunk));
+        const vx_view &w = views[pl.view_at[v]];
+        if (w.n_total) std::memcpy(&cat[pl.vk[v].chunk_off], w.chunks, (size_t)w.n_total * sizeof(vx_gpu_chunk));
         std::memcpy(&grids[(size_t)v * VX_GRID_LEN], w.grid, VX_GRID_LEN * 4);
     }
-    uint32_t hdr[BH_LEN] = {};
-    hdr[BH_TILES] = toff[n];
-    hdr[BH_VIEWS] = n;
-    for (uint32_t c = 0; c <= N_CLASSES; c++) hdr[BH_CB + c] = cb[c];
-    HIPX(hipMemcpyAsync(b->views.p, vk.data(), (size_t)n * sizeof(ViewK), hipMemcpyHostToDevice, st));
-    HIPX(hipMemcpyAsync(b->tile_off.p, toff.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice, st));
-    HIPX(hipMemcpyAsync(b->hdr.p, hdr, sizeof hdr, hipMemcpyHostToDevice, st));
-    if (recs) HIPX(hipMemcpyAsync(b->chunks.p, cat.data(), recs * sizeof(vx_gpu_chunk), hipMemcpyHostToDevice, st));
+    if (pl.recs)
+        HIPX(hipMemcpyAsync(b->chunks.p, cat.data(), pl.recs * sizeof(vx_gpu_chunk), hipMemcpyHostToDevice, st));
     HIPX(hipMemcpyAsync(b->grids.p, grids.data(), grids.size() * 4, hipMemcpyHostToDevice, st));
-    // New views start as a new Renderer would: empty targets, Hi-Z 0; costs 0.
-    if ((rc = vxh::fill_u64(st, b->vis.p, 0, poff[n])) || (rc = vxh::fill_u32(st, b->shadow.p, 0, (poff[n] + 1) / 2)) ||
-        (rc = vxh::fill_u32(st, b->hiz.p, 0, toff[n])) || (rc = vxh::fill_u32(st, b->cost.p, 0, toff[n])))
-        return rc;
+    return install_views(b, n, pl, moved);
+}
+
+int vx_batch_set_chunk_table(vx_batch *b, const vx_gpu_chunk *chunks, uint32_t n) {
+    if (!b || (!chunks && n) || n > VX_CHUNK_TABLE_MAX) return VX_E_ARG;
+    for (uint32_t i = 0; i < n; i++)  // LOD-0 records of a render set: one grid cell each, share 1
+        if (chunks[i].lod != 0u || chunks[i].fade != 1.0f || chunks[i].voxel_size != 1.0f) return VX_E_ARG;
+    vx_ctx *c = b->ctx;
+    HIPX(hipSetDevice(c->device));
+    HIPX(hipStreamSynchronize(c->stream));
+    int rc;
+    if ((rc = ensure(b->table, (size_t)std::max(n, 1u) * sizeof(vx_gpu_chunk)))) return rc;
+    if (n) HIPX(hipMemcpyAsync(b->table.p, chunks, (size_t)n * sizeof(vx_gpu_chunk), hipMemcpyHostToDevice, c->stream));
+    HIPX(hipStreamSynchronize(c->stream));
+    b->n_table = n;
+    return VX_OK;
+}
+
+int vx_batch_set_cameras(vx_batch *b, const vx_camera_view *views, uint32_t n) {
+    if (!b || !views || n == 0 || n > b->max_views) return VX_E_ARG;
+    for (uint32_t v = 0; v < n; v++)
+        if (!views[v].frame) return VX_E_ARG;
+    const uint32_t nt = b->n_table;
+    // The lists are built on the device: every view gets room for the whole table, and its
+    // frame's chunk_count / camera_chunk are the device's.
+    std::vector<vx_gpu_frame> frames(n);
+    for (uint32_t v = 0; v < n; v++) {
+        frames[v] = *views[v].frame;
+        frames[v].chunk_count = 0;
+        frames[v].camera_chunk = VX_NO_CHUNK;
+    }
+    ViewPlan pl;
+    int rc = plan_views(
+        b, n, [&](uint32_t v) { return &frames[v]; }, [&](uint32_t v) { return views[v].spec; },
+        [&](uint32_t v) { return views[v].spec_hi; }, [&](uint32_t) { return nt; }, pl);
+    if (rc) return rc;
+    vx_ctx *c = b->ctx;
+    HIPX(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
     HIPX(hipStreamSynchronize(st));
-    // A captured graph stays valid for another view set only if it reads every count from
-    // the header and launches the same class kernels: not PLAIN (per-view launches), not
-    // FLAT with different class ranges (grid sizes), not PERSISTENT with a different set
-    // of non-empty classes.
-    bool same_classes = true, same_ranges = true;
-    for (uint32_t c = 0; c < N_CLASSES; c++) {
-        same_classes &= ((cb[c + 1] - cb[c]) != 0u) == ((b->graph_cb[c + 1] - b->graph_cb[c]) != 0u);
-        same_ranges &= (cb[c + 1] - cb[c]) == (b->graph_cb[c + 1] - b->graph_cb[c]);
-    }
-    if (moved || b->tn.mode == VX_BATCH_PLAIN || (b->tn.mode == VX_BATCH_FLAT && !same_ranges) || !same_classes)
+    bool moved = false;
+    if ((rc = view_buffers(b, n, pl.toff[n], pl.poff[n], pl.recs, moved)) ||
+        (rc = ensure(b->cams, (size_t)n * sizeof(ListCam))) || (rc = ensure(b->list_counts, (size_t)n * 3 * 4))) {
         drop_graph(b);
-    b->vk.swap(vk);
-    b->vk_pool_gen = c->pool_gen;
-    b->toff.swap(toff);
-    b->poff.swap(poff);
-    b->slot_of.swap(slot_of);
-    std::memcpy(b->cb, cb, sizeof cb);
-    b->n_views = n;
-    b->n_tiles = b->toff[n];
-    b->n_px = b->poff[n];
-    b->rendered = false;
-    if (b->sec_on && (rc = secondary_buffers(b))) {
-        b->sec_on = false;
-        drop_graph(b);
+        b->n_views = 0;
         return rc;
     }
+    std::vector<ListCam> cams(n);
+    for (uint32_t slot = 0; slot < n; slot++) {
+        const vx_camera_view &w = views[pl.view_at[slot]];
+        ListCam &lc = cams[slot];
+        std::memset(&lc, 0, sizeof lc);
+        std::memcpy(lc.planes, w.planes, sizeof lc.planes);
+        for (int i = 0; i < 3; i++) {
+            lc.pos[i] = w.frame->cam_pos[i];
+            lc.gmin[i] = w.frame->grid_min[i];
+        }
+    }
+    HIPX(hipMemcpyAsync(b->cams.p, cams.data(), (size_t)n * sizeof(ListCam), hipMemcpyHostToDevice, st));
+    k_view_lists<<<n, LIST_BLOCK, 0, st>>>((const vx_gpu_chunk *)b->table.p, nt, (const ListCam *)b->cams.p,
+                                          (vx_gpu_chunk *)b->chunks.p, (uint32_t *)b->grids.p,
+                                          (uint32_t *)b->list_counts.p);
+    LAUNCHED();
+    std::vector<uint32_t> counts((size_t)n * 3);
+    HIPX(hipMemcpyAsync(counts.data(), b->list_counts.p, counts.size() * 4, hipMemcpyDeviceToHost, st));
+    HIPX(hipStreamSynchronize(st));
+    for (uint32_t slot = 0; slot < n; slot++) {
+        ViewK &k = pl.vk[slot];
+        k.n_chunks = counts[3 * slot];
+        k.f.chunk_count = counts[3 * slot + 1];
+        k.f.camera_chunk = counts[3 * slot + 2];
+    }
+    return install_views(b, n, pl, moved);
+}
+
+int vx_batch_view_counts(vx_batch *b, uint32_t view, uint32_t *n_total, uint32_t *n_visible, uint32_t *camera_chunk) {
+    if (!b || view >= b->n_views) return VX_E_ARG;
+    const ViewK &k = b->vk[b->slot_of[view]];
+    if (n_total) *n_total = k.n_chunks;
+    if (n_visible) *n_visible = k.f.chunk_count;
+    if (camera_chunk) *camera_chunk = k.f.camera_chunk;
+    return VX_OK;
+}
+
+int vx_batch_read_chunks(vx_batch *b, uint32_t view, vx_gpu_chunk *chunks, size_t cap, uint32_t *grid) {
+    if (!b || view >= b->n_views) return VX_E_ARG;
+    const ViewK &k = b->vk[b->slot_of[view]];
+    if ((!chunks && k.n_chunks) || cap < k.n_chunks) return VX_E_ARG;
+    vx_ctx *c = b->ctx;
+    HIPX(hipSetDevice(c->device));
+    if (k.n_chunks)
+        HIPX(hipMemcpyAsync(chunks, (const vx_gpu_chunk *)b->chunks.p + k.chunk_off,
+                            (size_t)k.n_chunks * sizeof(vx_gpu_chunk), hipMemcpyDeviceToHost, c->stream));
+    if (grid)
+        HIPX(hipMemcpyAsync(grid, (const uint32_t *)b->grids.p + k.grid_off, VX_GRID_LEN * 4, hipMemcpyDeviceToHost,
+                            c->stream));
+    HIPX(hipStreamSynchronize(c->stream));
     return VX_OK;
 }
 

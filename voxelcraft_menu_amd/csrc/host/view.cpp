@@ -106,6 +106,41 @@ Aabb world_aabb(const ViewRec &rec) {
                 add(o, V3{(float)rec.aabb_max[0] * vs, (float)rec.aabb_max[1] * vs, (float)rec.aabb_max[2] * vs})};
 }
 
+// The 96-byte GpuChunk of a record (build_chunk_list, src/render/mod.rs:1506-1520).
+void gpu_chunk_of(const ViewRec &rec, float fade, vx_gpu_chunk &g) {
+    const Key &k = rec.key;
+    const Aabb a = world_aabb(rec);
+    std::memset(&g, 0, sizeof g);
+    g.aabb_min[0] = a.min.x; g.aabb_min[1] = a.min.y; g.aabb_min[2] = a.min.z;
+    g.voxel_size = (float)k.voxel_size();
+    g.aabb_max[0] = a.max.x; g.aabb_max[1] = a.max.y; g.aabb_max[2] = a.max.z;
+    g.lod = (uint32_t)k.lod;
+    g.origin[0] = (float)(k.x * k.size()); g.origin[1] = (float)(k.y * k.size());
+    g.origin[2] = (float)(k.z * k.size());
+    g.fade = fade;
+    g.root[0] = (uint32_t)rec.root.mask;
+    g.root[1] = (uint32_t)(rec.root.mask >> 32);
+    g.root[2] = rec.root.run_ptr;
+    g.root[3] = rec.root.leaf_prefix;
+    g.attr_base = rec.attr_base;
+    g.attr_flags = rec.attr_flags;
+    g.light_base = 0xF000u;  // LightRef::Uniform(0xF000): lighting is not on this path
+    g.light_flags = 1u;
+    g.dry_mask[0] = (uint32_t)rec.dry_mask;
+    g.dry_mask[1] = (uint32_t)(rec.dry_mask >> 32);
+}
+
+// Frustum::from_camera for a view's camera (src/math.rs:69-117).
+Frustum frustum_of(const vxw_camera *cam) {
+    V3 pos{cam->pos[0], cam->pos[1], cam->pos[2]};
+    V3 fwd{cam->fwd[0], cam->fwd[1], cam->fwd[2]};
+    V3 right{cam->right[0], cam->right[1], cam->right[2]};
+    V3 up{cam->up[0], cam->up[1], cam->up[2]};
+    float tan_half = std::tan(cam->fov_y * 0.5f);
+    float aspect = (float)cam->width / (float)cam->height;
+    return Frustum(pos, fwd, right, up, tan_half, aspect, cam->far_);
+}
+
 struct DrawItem {
     const ViewRec *rec;
     float fade;
@@ -124,12 +159,7 @@ int32_t total_order(float f) {
 int build_view(const vxw_camera *cam, const std::vector<DrawItem> &items, bool shadow_share, bool offscreen_shadows,
                vx_gpu_chunk *chunks, uint32_t *grid, vxw_view *out) {
     V3 pos{cam->pos[0], cam->pos[1], cam->pos[2]};
-    V3 fwd{cam->fwd[0], cam->fwd[1], cam->fwd[2]};
-    V3 right{cam->right[0], cam->right[1], cam->right[2]};
-    V3 up{cam->up[0], cam->up[1], cam->up[2]};
-    float tan_half = std::tan(cam->fov_y * 0.5f);
-    float aspect = (float)cam->width / (float)cam->height;
-    Frustum fr(pos, fwd, right, up, tan_half, aspect, cam->far_);
+    const Frustum fr = frustum_of(cam);
     // grid_origin (src/render/mod.rs:88-91)
     const int32_t gmin[3] = {(int32_t)std::floor(pos.x / 64.0f) - (int32_t)VX_GRID_X / 2, 0,
                              (int32_t)std::floor(pos.z / 64.0f) - (int32_t)VX_GRID_Z / 2};
@@ -172,30 +202,11 @@ int build_view(const vxw_camera *cam, const std::vector<DrawItem> &items, bool s
     grid_set.reserve(visible.size() + offscreen.size());
     auto emit = [&](const Item &it) {
         const ViewRec &rec = *it.rec;
-        const Key &k = rec.key;
         Aabb a = world_aabb(rec);
         if (idx < visible.size() && camera_chunk == VX_NO_CHUNK && pos.x >= a.min.x && pos.y >= a.min.y &&
             pos.z >= a.min.z && pos.x <= a.max.x && pos.y <= a.max.y && pos.z <= a.max.z)
             camera_chunk = idx;
-        vx_gpu_chunk &g = chunks[idx];
-        std::memset(&g, 0, sizeof g);
-        g.aabb_min[0] = a.min.x; g.aabb_min[1] = a.min.y; g.aabb_min[2] = a.min.z;
-        g.voxel_size = (float)k.voxel_size();
-        g.aabb_max[0] = a.max.x; g.aabb_max[1] = a.max.y; g.aabb_max[2] = a.max.z;
-        g.lod = (uint32_t)k.lod;
-        g.origin[0] = (float)(k.x * k.size()); g.origin[1] = (float)(k.y * k.size());
-        g.origin[2] = (float)(k.z * k.size());
-        g.fade = it.it.fade;
-        g.root[0] = (uint32_t)rec.root.mask;
-        g.root[1] = (uint32_t)(rec.root.mask >> 32);
-        g.root[2] = rec.root.run_ptr;
-        g.root[3] = rec.root.leaf_prefix;
-        g.attr_base = rec.attr_base;
-        g.attr_flags = rec.attr_flags;
-        g.light_base = 0xF000u;  // LightRef::Uniform(0xF000): lighting is not on this path
-        g.light_flags = 1u;
-        g.dry_mask[0] = (uint32_t)rec.dry_mask;
-        g.dry_mask[1] = (uint32_t)(rec.dry_mask >> 32);
+        gpu_chunk_of(rec, it.it.fade, chunks[idx]);
         grid_set.push_back(it.it);
         idx++;
     };
@@ -331,6 +342,42 @@ extern "C" int vxw_snapshot_take(vxw_snapshot *s, const vxw_world *wh) {
 }
 
 extern "C" uint32_t vxw_snapshot_emitter_chunks(const vxw_snapshot *s) { return s ? s->emitter_chunks : 0u; }
+
+extern "C" size_t vxw_snapshot_chunks(const vxw_snapshot *s, vx_gpu_chunk *out, size_t cap) {
+    if (!s) return 0;
+    size_t n = 0;
+    for (const ViewRec &r : s->recs) {
+        if (r.solid_count == 0) continue;
+        if (out && n < cap) gpu_chunk_of(r, 1.0f, out[n]);
+        n++;
+    }
+    return n;
+}
+
+extern "C" int vxw_pack_camera_frame(const vxw_camera *cam, uint32_t max_pairs, vx_gpu_frame *f) {
+    if (!cam || !f || cam->width == 0 || cam->height == 0) return VXW_E_ARG;
+    vxw_view v{};
+    v.n_visible = 0;
+    v.n_total = 0;
+    v.camera_chunk = VX_NO_CHUNK;
+    // grid_origin (src/render/mod.rs:88-91)
+    v.grid_min[0] = (int32_t)std::floor(cam->pos[0] / 64.0f) - (int32_t)VX_GRID_X / 2;
+    v.grid_min[1] = 0;
+    v.grid_min[2] = (int32_t)std::floor(cam->pos[2] / 64.0f) - (int32_t)VX_GRID_Z / 2;
+    return vxw_pack_frame(cam, &v, max_pairs, f);
+}
+
+extern "C" int vxw_frustum_planes(const vxw_camera *cam, float planes[24]) {
+    if (!cam || !planes || cam->width == 0 || cam->height == 0) return VXW_E_ARG;
+    const Frustum fr = frustum_of(cam);
+    for (int i = 0; i < 6; i++) {
+        planes[4 * i] = fr.n[i].x;
+        planes[4 * i + 1] = fr.n[i].y;
+        planes[4 * i + 2] = fr.n[i].z;
+        planes[4 * i + 3] = fr.d[i];
+    }
+    return VXW_OK;
+}
 
 extern "C" int vxw_build_view_snapshot(const vxw_snapshot *s, const vxw_camera *cam, vx_gpu_chunk *chunks,
                                        uint32_t *grid, vxw_view *out) {

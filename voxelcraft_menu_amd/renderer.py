@@ -218,6 +218,47 @@ class Batch:
         self.views = list(views)
         self._keep = keep
 
+    def set_chunk_table(self, table: np.ndarray):
+        """Upload the LOD-0 chunk records (ViewSnapshot.chunk_table) set_cameras builds from."""
+        t = np.ascontiguousarray(table, dtype=np.uint8).reshape(-1, 96)
+        A.check(self._L.vx_batch_set_chunk_table(self._b, A.ptr(t) if len(t) else None, len(t)),
+                "vx_batch_set_chunk_table")
+
+    def set_cameras(self, cams: list, specs: list[tuple[int, int]]):
+        """Views from cameras alone: their chunk lists and grids are built on the device
+        from the chunk table, as build_view builds them on the host."""
+        from .world import camera_frame
+
+        descs = (A.CameraView * len(cams))()
+        frames = []
+        for i, (cam, (spec, spec_hi)) in enumerate(zip(cams, specs)):
+            fr, planes = camera_frame(cam)
+            frames.append(fr)
+            descs[i] = A.CameraView(C.addressof(fr), spec, spec_hi, (C.c_float * 24)(*planes))
+        A.check(self._L.vx_batch_set_cameras(self._b, descs, len(cams)), "vx_batch_set_cameras")
+        views = []
+        for i, (cam, fr) in enumerate(zip(cams, frames)):
+            nt, nv, cc = self.view_counts(i)
+            fr.chunk_count, fr.camera_chunk = nv, cc
+            views.append(ViewData(np.zeros((1, 96), np.uint8), np.zeros(0, np.uint32), fr, nv, nt, cc,
+                                  cam.width, cam.height))
+        self.views = views
+        self._keep = frames
+
+    def view_counts(self, i: int) -> tuple[int, int, int]:
+        """(records, visible records, camera chunk) of view i's list."""
+        nt, nv, cc = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        A.check(self._L.vx_batch_view_counts(self._b, i, C.byref(nt), C.byref(nv), C.byref(cc)), "vx_batch_view_counts")
+        return int(nt.value), int(nv.value), int(cc.value)
+
+    def read_chunks(self, i: int) -> tuple[np.ndarray, np.ndarray]:
+        """View i's chunk list (n, 96) and grid as the batch holds them on the device."""
+        nt = self.view_counts(i)[0]
+        ch = np.zeros((max(nt, 1), 96), np.uint8)
+        grid = np.zeros(A.VX_GRID_LEN, np.uint32)
+        A.check(self._L.vx_batch_read_chunks(self._b, i, A.ptr(ch), nt, A.ptr(grid)), "vx_batch_read_chunks")
+        return ch[:nt], grid
+
     def set_tuning(self, mode: int = A.BATCH_PERSISTENT, occupancy: int = 4, variant: int = 4, ordered: int = 1,
                    queue_chunk: int = 1, waves_per_simd: int | None = None, graph: int = 1):
         wps = occupancy if waves_per_simd is None else waves_per_simd

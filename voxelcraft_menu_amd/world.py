@@ -405,6 +405,14 @@ class ViewSnapshot:
     def any_emitter_resident(self) -> bool:
         return self._L.vxw_snapshot_emitter_chunks(self._h) != 0
 
+    def chunk_table(self) -> np.ndarray:
+        """The (n, 96) GpuChunk records the device builds view chunk lists from."""
+        n = self._L.vxw_snapshot_chunks(self._h, None, 0)
+        out = np.zeros((max(n, 1), 96), np.uint8)
+        if n:
+            self._L.vxw_snapshot_chunks(self._h, A.ptr(out), n)
+        return out[:n]
+
     def build_view(self, cam: CameraParams, max_pairs: int = 0) -> ViewData:
         L = A.host()
         c = cam.to_c()
@@ -418,6 +426,17 @@ class ViewSnapshot:
         n = int(v.n_total)
         return ViewData(chunks[:n].copy() if n else np.zeros((1, 96), np.uint8), grid, fr,
                         int(v.n_visible), n, int(v.camera_chunk), cam.width, cam.height)
+
+
+def camera_frame(cam: CameraParams, max_pairs: int = 0):
+    """(GpuFrame, frustum planes) of a camera whose chunk list the device builds."""
+    L = A.host()
+    c = cam.to_c()
+    fr = A.Frame()
+    A.check(L.vxw_pack_camera_frame(C.byref(c), max_pairs, C.byref(fr)), "pack_camera_frame")
+    planes = np.zeros(24, np.float32)
+    A.check(L.vxw_frustum_planes(C.byref(c), A.ptr(planes)), "frustum_planes")
+    return fr, planes
 
 
 def build_local_flat(dense: np.ndarray) -> bytes:
