@@ -263,11 +263,13 @@ def run_serving(args):
     world = W.make_world(wl)
     r = Renderer(64, 64)
     r.sync_world(world)
-    S.run(world, r, wl, qps=200, duration_s=1.0, edit_rate=0, seed=99)  # warm-up stream
+    dev = not args.serve_host_lists
+    S.run(world, r, wl, qps=200, duration_s=1.0, edit_rate=0, seed=99, device_lists=dev)  # warm-up stream
     res = S.run(world, r, wl, qps=args.serve_qps, duration_s=args.serve_seconds, edit_rate=args.serve_edit_rate,
-                seed=4)
+                seed=4, device_lists=dev)
     r.close()
     res["workload"] = wl.name
+    res["chunk_lists"] = "device" if dev else "host"
     _log(f"serving: p50 {res['p50_ms']} ms p99 {res['p99_ms']} ms at {args.serve_qps} qps")
     return res
 
@@ -528,6 +530,8 @@ def main():
     ap.add_argument("--serve-qps", type=float, default=1000.0, help="offered view requests per second")
     ap.add_argument("--serve-seconds", type=float, default=4.0, help="seconds of request arrivals")
     ap.add_argument("--serve-edit-rate", type=float, default=2.0, help="edit batches (1000 ops) per second")
+    ap.add_argument("--serve-host-lists", action="store_true",
+                    help="serving requests carry host-built chunk lists instead of cameras")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one 256 MiB device-to-device copy (scale reference for PMC byte counters)")
     args = ap.parse_args()

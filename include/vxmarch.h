@@ -385,6 +385,14 @@ int vx_serve_destroy(vx_serve *s);
 /* t_arrival: when the request arrived (vx_serve_clock), 0 = now. VX_E_OVERFLOW if its
  * footprint exceeds the whole budget. */
 int vx_serve_submit(vx_serve *s, uint64_t id, const vx_view *view, double t_arrival);
+/* Camera requests (chunk lists built on the device, as vx_batch_set_cameras builds them):
+ * vx_serve_set_chunk_table uploads the LOD-0 chunk table, only while no lane runs and no
+ * camera request is pending (between vx_serve_barrier and vx_serve_resume, or before the
+ * first one; VX_E_BUSY otherwise). vx_serve_submit_camera queues a view given by its frame
+ * (vxw_pack_camera_frame) and frustum planes; its results are those of vx_serve_submit
+ * with the host-built list for that camera. */
+int vx_serve_set_chunk_table(vx_serve *s, const vx_gpu_chunk *chunks, uint32_t n);
+int vx_serve_submit_camera(vx_serve *s, uint64_t id, const vx_camera_view *view, double t_arrival);
 /* Completed requests (up to cap), waiting up to wait_ms for the first. */
 int vx_serve_poll(vx_serve *s, vx_serve_result *out, uint32_t cap, uint32_t *n, int wait_ms);
 int vx_serve_read_vis(vx_serve *s, uint64_t id, uint64_t *out, size_t n);

@@ -160,3 +160,58 @@ def test_barrier_orders_edits_between_requests():
     assert changed > 0, "the edits changed no pixel"
     srv.close()
     r.close()
+
+
+def test_camera_requests_equal_host_list_requests(small_world):
+    """Requests given by their camera (lists built on the device from the chunk table)
+    give the bits of the same views submitted with host-built lists, mixed in one stream,
+    and follow an edit batch once the table is re-uploaded under the barrier."""
+    from voxelcraft_menu_amd import serving as S
+    from voxelcraft_menu_amd.world import ViewSnapshot
+
+    wl = W.WORKLOADS[0]
+    cams = W.random_cameras(wl, 10, seed=61, resolutions=[(333, 197), (64, 64), (120, 250)])
+    r = Renderer(64, 64)
+    r.sync_world(small_world)
+    srv = Server(r, lanes=3, max_views=8, hbm_budget=256 << 20, mode=A.BATCH_FLAT)
+    snap = ViewSnapshot(small_world)
+    srv.set_chunk_table(snap.chunk_table())
+
+    def run_round(tag):
+        ids = []
+        for i, c in enumerate(cams):  # each view twice: by camera and with its host list
+            s, hi = W.spec_words(c.flags, small_world)
+            srv.submit_camera(2 * i, c, s, hi)
+            srv.submit(2 * i + 1, build_view(small_world, c), s, hi)
+            ids += [2 * i, 2 * i + 1]
+        res = _drain(srv, ids)
+        for i, c in enumerate(cams):
+            v = build_view(small_world, c)
+            assert res[2 * i]["status"] == 0
+            a, b = _read(srv, 2 * i, v), _read(srv, 2 * i + 1, v)
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), f"{tag} view {i}"
+            if i < 3:
+                spec, hi = W.spec_words(c.flags, small_world)
+                ref = H.oracle_frames(small_world, v, spec, hi, frames=1)[0]
+                assert np.array_equal(a[0], ref[0]), f"{tag} view {i} vs oracle"
+        for rid in ids:
+            srv.release(rid)
+
+    run_round("before edits")
+    with pytest.raises(RuntimeError):  # a camera request pending: the table cannot change
+        srv.barrier()
+        s, hi = W.spec_words(cams[0].flags, small_world)
+        srv.submit_camera(900, cams[0], s, hi)
+        srv.set_chunk_table(snap.chunk_table())
+    srv.resume()
+    _drain(srv, [900])
+    srv.release(900)
+    small_world.apply_edits(S.random_edits(np.random.default_rng(8), wl, 3000))
+    srv.barrier()
+    r.sync_world(small_world)
+    snap.take(small_world)
+    srv.set_chunk_table(snap.chunk_table())
+    srv.resume()
+    run_round("after edits")
+    srv.close()
+    r.close()

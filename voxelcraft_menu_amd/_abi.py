@@ -387,6 +387,8 @@ _sig_march = {
     "vx_serve_create": (C.c_int, [P, C.POINTER(ServeDesc), C.POINTER(P)]),
     "vx_serve_destroy": (C.c_int, [P]),
     "vx_serve_submit": (C.c_int, [P, C.c_uint64, C.POINTER(ViewDesc), C.c_double]),
+    "vx_serve_set_chunk_table": (C.c_int, [P, P, C.c_uint32]),
+    "vx_serve_submit_camera": (C.c_int, [P, C.c_uint64, C.POINTER(CameraView), C.c_double]),
     "vx_serve_poll": (C.c_int, [P, C.POINTER(ServeResult), C.c_uint32, C.POINTER(C.c_uint32), C.c_int]),
     "vx_serve_read_vis": (C.c_int, [P, C.c_uint64, P, C.c_size_t]),
     "vx_serve_read_shadow": (C.c_int, [P, C.c_uint64, P, C.c_size_t]),

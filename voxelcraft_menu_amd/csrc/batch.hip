@@ -271,18 +271,6 @@ int view_buffers(vx_batch *b, uint32_t n_views, uint32_t n_tiles, uint64_t n_px,
 
 namespace vxd {
 
-// Per-view chunk lists on the device: partition_for_grid + build_chunk_list
-// (src/render/mod.rs:110-138, :1447-1564) over a table of LOD-0 GpuChunk records (fade 1)
-// in render-set order, for a camera given by its frustum planes, position and grid origin.
-struct ListCam {
-    float planes[24];  // Frustum::from_camera: (nx, ny, nz, d) x 6
-    float pos[3];
-    int32_t gmin[3];
-    uint32_t pad[2];
-};
-static_assert(sizeof(ListCam) == 128, "ListCam layout");
-constexpr uint32_t LIST_BLOCK = 1024, LIST_KEYS = 8192;
-
 // One block per view. Each table record is classified (visible: inside the six planes;
 // off-screen caster: outside but on the 32x8x32 grid; else dropped) and keyed by
 // (class, f32 bits of its AABB distance, table index): an ascending sort of the keys is
@@ -293,11 +281,13 @@ constexpr uint32_t LIST_BLOCK = 1024, LIST_KEYS = 8192;
 __global__ __launch_bounds__(LIST_BLOCK) void k_view_lists(const vx_gpu_chunk *__restrict__ table, uint32_t n_table,
                                                            const ListCam *__restrict__ cams,
                                                            vx_gpu_chunk *__restrict__ out_chunks,
-                                                           uint32_t *__restrict__ grids, uint32_t *__restrict__ counts) {
+                                                           uint32_t *__restrict__ grids, uint32_t *__restrict__ counts,
+                                                           ViewK *__restrict__ patch) {
     __shared__ unsigned long long keys[LIST_KEYS];
     __shared__ uint32_t s_vis_end, s_off_end, s_cam;
-    const uint32_t v = blockIdx.x, tid = threadIdx.x;
-    const ListCam &cam = cams[v];
+    const uint32_t tid = threadIdx.x;
+    const ListCam &cam = cams[blockIdx.x];
+    const uint32_t v = cam.view;
     uint32_t n2 = 1;
     while (n2 < n_table) n2 <<= 1;
     if (tid == 0) {
@@ -370,7 +360,7 @@ __global__ __launch_bounds__(LIST_BLOCK) void k_view_lists(const vx_gpu_chunk *_
     const uint32_t vis_all = s_vis_end, off_all = s_off_end > vis_all ? s_off_end - vis_all : 0u;
     const uint32_t nv = min(vis_all, (uint32_t)VX_MAX_CHUNKS);
     const uint32_t no = min(off_all, (uint32_t)VX_MAX_CHUNKS - nv);
-    vx_gpu_chunk *out = out_chunks + (size_t)v * n_table;
+    vx_gpu_chunk *out = out_chunks + cam.chunk_off;
     for (uint32_t r = tid; r < nv + no; r += LIST_BLOCK) {
         const uint32_t at = r < nv ? r : vis_all + (r - nv);
         const uint32_t idx = (uint32_t)keys[at];
@@ -394,9 +384,17 @@ __global__ __launch_bounds__(LIST_BLOCK) void k_view_lists(const vx_gpu_chunk *_
     }
     __syncthreads();
     if (tid == 0) {
-        counts[3 * v] = nv + no;
-        counts[3 * v + 1] = nv;
-        counts[3 * v + 2] = s_cam;
+        if (counts) {
+            counts[3 * v] = nv + no;
+            counts[3 * v + 1] = nv;
+            counts[3 * v + 2] = s_cam;
+        }
+        if (patch) {  // the view's descriptor, read by the march launched after this kernel
+            patch[v].n_chunks = nv + no;
+            patch[v].p.n_chunks = nv + no;
+            patch[v].f.chunk_count = nv;
+            patch[v].f.camera_chunk = s_cam;
+        }
     }
 }
 
@@ -655,11 +653,13 @@ int vx_batch_set_cameras(vx_batch *b, const vx_camera_view *views, uint32_t n) {
             lc.pos[i] = w.frame->cam_pos[i];
             lc.gmin[i] = w.frame->grid_min[i];
         }
+        lc.chunk_off = pl.vk[slot].chunk_off;
+        lc.view = slot;
     }
     HIPX(hipMemcpyAsync(b->cams.p, cams.data(), (size_t)n * sizeof(ListCam), hipMemcpyHostToDevice, st));
     k_view_lists<<<n, LIST_BLOCK, 0, st>>>((const vx_gpu_chunk *)b->table.p, nt, (const ListCam *)b->cams.p,
                                           (vx_gpu_chunk *)b->chunks.p, (uint32_t *)b->grids.p,
-                                          (uint32_t *)b->list_counts.p);
+                                          (uint32_t *)b->list_counts.p, nullptr);
     LAUNCHED();
     std::vector<uint32_t> counts((size_t)n * 3);
     HIPX(hipMemcpyAsync(counts.data(), b->list_counts.p, counts.size() * 4, hipMemcpyDeviceToHost, st));

@@ -121,6 +121,23 @@ struct ViewK {
     Pools p;
 };
 
+// Per-view chunk lists on the device: partition_for_grid + build_chunk_list
+// (src/render/mod.rs:110-138, :1447-1564) over a table of LOD-0 GpuChunk records (fade 1)
+// in render-set order, for a camera given by its frustum planes, position and grid origin.
+struct ListCam {
+    float planes[24];  // Frustum::from_camera: (nx, ny, nz, d) x 6
+    float pos[3];
+    int32_t gmin[3];
+    uint32_t chunk_off;  // first record of the view's list in the output
+    uint32_t view;       // the view's slot: its grid, counts and descriptor
+};
+static_assert(sizeof(ListCam) == 128, "ListCam layout");
+constexpr uint32_t LIST_BLOCK = 1024, LIST_KEYS = 8192;
+
+__global__ void k_view_lists(const vx_gpu_chunk *__restrict__ table, uint32_t n_table, const ListCam *__restrict__ cams,
+                             vx_gpu_chunk *__restrict__ out_chunks, uint32_t *__restrict__ grids,
+                             uint32_t *__restrict__ counts, ViewK *__restrict__ patch);
+
 // Header words of a batch in device memory (read by graph-captured kernels, so a replay
 // follows the current view set): tile and view counts, class boundaries cb[0..4].
 constexpr uint32_t BH_TILES = 0, BH_VIEWS = 1, BH_CB = 4, BH_LEN = 12;

@@ -91,6 +91,9 @@ struct Req {
     ViewK vk{};
     std::vector<vx_gpu_chunk> chunks;
     std::vector<uint32_t> grid;
+    bool cam = false;  // chunk list and grid built on the device from the chunk table
+    ListCam lc{};
+    uint32_t list_recs = 0;  // room of its list: the table size at submit
     uint32_t w = 0, h = 0, tiles = 0;
     uint64_t units = 0, px_off = 0, bytes = 0;
     double t_arrival = 0, t_submit = 0, t_launch = 0, t_done = 0;
@@ -144,6 +147,8 @@ struct vx_serve {
     std::thread worker;
     hipStream_t io = nullptr;
     vx_serve_stats stats{};
+    DevBuf table;  // LOD-0 chunk records camera requests build their lists from
+    uint32_t n_table = 0;
 };
 
 namespace {
@@ -163,14 +168,19 @@ int launch(vx_serve *s, Lane &L) {
     // Class-major layout: each view class is one tile range marched by its own kernel.
     std::stable_sort(L.reqs.begin(), L.reqs.end(), [](const Req *x, const Req *y) { return req_class(x) < req_class(y); });
     const uint32_t n = (uint32_t)L.reqs.size();
-    size_t recs = 0;
-    uint32_t tiles = 0;
+    size_t recs = 0, host_recs = 0;
+    uint32_t tiles = 0, n_cam = 0;
     for (Req *r : L.reqs) {
-        recs += r->chunks.size();
+        host_recs += r->chunks.size();
+        recs += r->cam ? r->list_recs : r->chunks.size();
         tiles += r->tiles;
+        n_cam += r->cam ? 1u : 0u;
     }
+    // [descriptors | tile offsets | header | list cameras | chunk lists: the uploaded ones,
+    // then room for the device-built ones | grids]
     const size_t o_views = 0, o_toff = align256(n * sizeof(ViewK)), o_hdr = align256(o_toff + (n + 1) * 4),
-                 o_chunks = align256(o_hdr + BH_LEN * 4), o_grids = align256(o_chunks + recs * sizeof(vx_gpu_chunk)),
+                 o_cams = align256(o_hdr + BH_LEN * 4), o_chunks = align256(o_cams + (size_t)n_cam * sizeof(ListCam)),
+                 o_grids = align256(o_chunks + recs * sizeof(vx_gpu_chunk)),
                  total = o_grids + (size_t)n * VX_GRID_LEN * 4;
     int rc;
     if ((rc = ensure(L.in, total)) || (rc = ensure(L.hiz, (size_t)tiles * 4)) ||
@@ -189,25 +199,35 @@ int launch(vx_serve *s, Lane &L) {
     uint32_t *hdr = (uint32_t *)(stg + o_hdr);
     vx_gpu_chunk *ch = (vx_gpu_chunk *)(stg + o_chunks);
     uint32_t *grids = (uint32_t *)(stg + o_grids);
-    uint32_t t = 0;
-    size_t c = 0;
+    ListCam *cams = (ListCam *)(stg + o_cams);
+    uint32_t t = 0, k_cam = 0;
+    size_t c = 0, c_cam = host_recs;
     uint32_t cb[N_CLASSES + 1] = {0, 0, 0, 0, 0};
     for (uint32_t v = 0; v < n; v++) {
         const Req *r = L.reqs[v];
+        const size_t off = r->cam ? c_cam : c;
         vk[v] = r->vk;
-        vk[v].chunk_off = (uint32_t)c;
+        vk[v].chunk_off = (uint32_t)off;
         vk[v].grid_off = v * VX_GRID_LEN;
         vk[v].p = vxh::make_pools(s->ctx);
-        vk[v].p.chunks = (const vx_gpu_chunk *)((const char *)L.in.p + o_chunks) + c;
+        vk[v].p.chunks = (const vx_gpu_chunk *)((const char *)L.in.p + o_chunks) + off;
         vk[v].p.grid = (const uint32_t *)((const char *)L.in.p + o_grids) + (size_t)v * VX_GRID_LEN;
-        vk[v].p.n_chunks = (uint32_t)r->chunks.size();
+        vk[v].p.n_chunks = r->cam ? 0u : (uint32_t)r->chunks.size();  // device-built: set by the list kernel
         vk[v].tile_off = t;
         vk[v].px_off_lo = (uint32_t)r->px_off;
         vk[v].px_off_hi = (uint32_t)(r->px_off >> 32);
         toff[v] = t;
-        if (!r->chunks.empty()) std::memcpy(ch + c, r->chunks.data(), r->chunks.size() * sizeof(vx_gpu_chunk));
-        std::memcpy(grids + (size_t)v * VX_GRID_LEN, r->grid.data(), VX_GRID_LEN * 4);
-        c += r->chunks.size();
+        if (r->cam) {
+            cams[k_cam] = r->lc;
+            cams[k_cam].chunk_off = (uint32_t)off;
+            cams[k_cam].view = v;
+            k_cam++;
+            c_cam += r->list_recs;
+        } else {
+            if (!r->chunks.empty()) std::memcpy(ch + c, r->chunks.data(), r->chunks.size() * sizeof(vx_gpu_chunk));
+            std::memcpy(grids + (size_t)v * VX_GRID_LEN, r->grid.data(), VX_GRID_LEN * 4);
+            c += r->chunks.size();
+        }
         t += r->tiles;
         for (uint32_t k = req_class(r) + 1; k <= N_CLASSES; k++) cb[k] = t;
     }
@@ -218,7 +238,17 @@ int launch(vx_serve *s, Lane &L) {
     for (uint32_t k = 0; k <= N_CLASSES; k++) hdr[BH_CB + k] = cb[k];
     hipStream_t st = L.st;
     char *din = (char *)L.in.p;
-    HIPX(hipMemcpyAsync(din, stg, total, hipMemcpyHostToDevice, st));
+    // everything up to the uploaded lists, then the grids (device-built lists and grids are
+    // written by the list kernel)
+    const size_t head = o_chunks + host_recs * sizeof(vx_gpu_chunk);
+    HIPX(hipMemcpyAsync(din, stg, head, hipMemcpyHostToDevice, st));
+    HIPX(hipMemcpyAsync(din + o_grids, stg + o_grids, total - o_grids, hipMemcpyHostToDevice, st));
+    if (n_cam) {
+        k_view_lists<<<n_cam, LIST_BLOCK, 0, st>>>((const vx_gpu_chunk *)s->table.p, s->n_table,
+                                                   (const ListCam *)(din + o_cams), (vx_gpu_chunk *)(din + o_chunks),
+                                                   (uint32_t *)(din + o_grids), nullptr, (ViewK *)(din + o_views));
+        VX_LAUNCHED();
+    }
     if ((rc = vxh::fill_u32(st, L.counters.p, 0, CNT_TOTAL)) || (rc = vxh::fill_u32(st, L.hiz.p, 0, tiles)))
         return rc;
     const uint32_t *hdr_d = (const uint32_t *)(din + o_hdr);
@@ -418,6 +448,7 @@ int vx_serve_destroy(vx_serve *s) {
     if (s->io) (void)hipStreamDestroy(s->io);
     vxh::release(s->vis);
     vxh::release(s->shadow);
+    vxh::release(s->table);
     if (started) s->ctx->n_batches--;
     delete s;
     return VX_OK;
@@ -453,6 +484,82 @@ int vx_serve_submit(vx_serve *s, uint64_t id, const vx_view *v, double t_arrival
         if (r->bytes > s->d.hbm_budget) {
             delete r;
             return VX_E_OVERFLOW;
+        }
+        if (s->live.count(id)) {
+            delete r;
+            return VX_E_ARG;
+        }
+        r->seq = s->seq++;
+        s->live[id] = r;
+        s->pending.push_back(r);
+        s->stats.submitted++;
+    }
+    s->cv_work.notify_all();
+    return VX_OK;
+}
+
+int vx_serve_set_chunk_table(vx_serve *s, const vx_gpu_chunk *chunks, uint32_t n) {
+    if (!s || (!chunks && n) || n > VX_CHUNK_TABLE_MAX) return VX_E_ARG;
+    for (uint32_t i = 0; i < n; i++)
+        if (chunks[i].lod != 0u || chunks[i].fade != 1.0f || chunks[i].voxel_size != 1.0f) return VX_E_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    // Only between vx_serve_barrier and vx_serve_resume, or before any camera request: no
+    // launch reads the table meanwhile.
+    for (const Lane &L : s->lanes)
+        if (L.busy) return VX_E_BUSY;
+    for (const Req *r : s->pending)  // their list room is the table size they were submitted with
+        if (r->cam) return VX_E_BUSY;
+    HIPX(hipSetDevice(s->ctx->device));
+    int rc;
+    if ((rc = ensure(s->table, (size_t)std::max(n, 1u) * sizeof(vx_gpu_chunk)))) return rc;
+    if (n) HIPX(hipMemcpyAsync(s->table.p, chunks, (size_t)n * sizeof(vx_gpu_chunk), hipMemcpyHostToDevice, s->io));
+    HIPX(hipStreamSynchronize(s->io));
+    s->n_table = n;
+    return VX_OK;
+}
+
+int vx_serve_submit_camera(vx_serve *s, uint64_t id, const vx_camera_view *v, double t_arrival) {
+    if (!s || !v || !v->frame) return VX_E_ARG;
+    vx_gpu_frame fr = *v->frame;
+    fr.chunk_count = 0;
+    fr.camera_chunk = VX_NO_CHUNK;
+    uint32_t nt;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        nt = s->n_table;
+    }
+    int rc = vxh::check_frame(&fr, nt);
+    if (rc) return rc;
+    Req *r = new (std::nothrow) Req();
+    if (!r) return VX_E_OOM;
+    r->id = id;
+    r->vk.f = vxh::make_frame(&fr);
+    if ((rc = vxh::make_spec(v->spec, v->spec_hi, fr.flags, r->vk.s))) {
+        delete r;
+        return rc;
+    }
+    r->cam = true;
+    r->list_recs = nt;
+    std::memcpy(r->lc.planes, v->planes, sizeof r->lc.planes);
+    for (int i = 0; i < 3; i++) {
+        r->lc.pos[i] = fr.cam_pos[i];
+        r->lc.gmin[i] = fr.grid_min[i];
+    }
+    r->w = fr.res[0];
+    r->h = fr.res[1];
+    r->tiles = fr.tiles[0] * fr.tiles[1];
+    const uint64_t px = (uint64_t)r->w * r->h;
+    r->units = (px + PX_UNIT - 1) / PX_UNIT;
+    r->bytes = r->units * PX_UNIT * 10 + (uint64_t)nt * sizeof(vx_gpu_chunk) + VX_GRID_LEN * 4 +
+               (uint64_t)r->tiles * 12 + sizeof(ViewK) + sizeof(ListCam);
+    r->t_submit = now_s();
+    r->t_arrival = t_arrival > 0 ? t_arrival : r->t_submit;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (r->bytes > s->d.hbm_budget || s->n_table != nt) {
+            const int code = r->bytes > s->d.hbm_budget ? VX_E_OVERFLOW : VX_E_BUSY;
+            delete r;
+            return code;
         }
         if (s->live.count(id)) {
             delete r;

@@ -389,6 +389,21 @@ class Server:
                        view.grid.ctypes.data, spec, spec_hi)
         A.check(self._L.vx_serve_submit(self._s, rid, C.byref(d), t_arrival), "vx_serve_submit")
 
+    def set_chunk_table(self, table: np.ndarray):
+        """The LOD-0 chunk records camera requests build their lists from; only while no
+        lane runs and no camera request is pending (between barrier and resume)."""
+        t = np.ascontiguousarray(table, dtype=np.uint8).reshape(-1, 96)
+        A.check(self._L.vx_serve_set_chunk_table(self._s, A.ptr(t) if len(t) else None, len(t)),
+                "vx_serve_set_chunk_table")
+
+    def submit_camera(self, rid: int, cam, spec: int, spec_hi: int, t_arrival: float = 0.0):
+        """A request given by its camera: the device builds its chunk list and grid."""
+        from .world import camera_frame
+
+        fr, planes = camera_frame(cam)
+        d = A.CameraView(C.addressof(fr), spec, spec_hi, (C.c_float * 24)(*planes))
+        A.check(self._L.vx_serve_submit_camera(self._s, rid, C.byref(d), t_arrival), "vx_serve_submit_camera")
+
     def poll(self, wait_ms: int = 0) -> list[dict]:
         n = C.c_uint32()
         A.check(self._L.vx_serve_poll(self._s, self._res, len(self._res), C.byref(n), wait_ms), "vx_serve_poll")

@@ -5,15 +5,17 @@ the chunk list, render, wait. Here requests arrive on a seeded Poisson clock whe
 not earlier ones have finished; each is stamped with its scheduled arrival time, so
 latency = completion - arrival includes every host and queueing delay.
 
-  view request  host chunk list from the current snapshot of the world's chunk records
-                (ViewSnapshot.build_view, under a shared lock), then Server.submit; the
-                native server batches it with whatever else is queued and launches it
-                on a free lane.
+  view request  by default the camera alone (Server.submit_camera, under a shared lock):
+                the server builds its chunk list and grid on the device from the chunk
+                table of the current snapshot; with device_lists off, a host chunk list
+                from the snapshot (ViewSnapshot.build_view) and Server.submit. The native
+                server batches it with whatever else is queued and launches it on a free
+                lane.
   edit batch    1000 random voxel set/clear ops, on their own thread: applied to the
                 host world while views keep being set up from the snapshot and rendered
                 from the device pools, both still the world before the edit; a new
                 snapshot is taken; then, under the exclusive lock, barrier -> dirty-range
-                pool upload -> snapshot swap -> resume. Views submitted before the swap
+                pool upload -> snapshot swap (and its chunk table to the server) -> resume. Views submitted before the swap
                 see the old world, views after it the new one.
 
 A poller thread collects completions and releases them (returning their HBM budget).
@@ -78,7 +80,7 @@ def random_edits(rng, wl: W.Workload, n: int = 1000) -> np.ndarray:
 
 def run(world: World, renderer: Renderer, wl: W.Workload, qps: float, duration_s: float, edit_rate: float,
         seed: int = 4, workers: int = 6, lanes: int = 3, max_views: int = 32, hbm_budget: int = 8 << 30,
-        mode: int = A.BATCH_FLAT, log=None) -> dict:
+        mode: int = A.BATCH_FLAT, device_lists: bool = True, log=None) -> dict:
     """Runs the stream for duration_s seconds of arrivals and returns latency and
     throughput figures of the completed view requests."""
     rng = np.random.default_rng(seed)
@@ -95,6 +97,8 @@ def run(world: World, renderer: Renderer, wl: W.Workload, qps: float, duration_s
     srv = Server(renderer, lanes=lanes, max_views=max_views, hbm_budget=hbm_budget, mode=mode)
     lock = _RWLock()
     snaps = [ViewSnapshot(world), ViewSnapshot()]  # [current, spare]
+    if device_lists:
+        srv.set_chunk_table(snaps[0].chunk_table())
     lat = {}
     px = {}
     batches = {}
@@ -119,9 +123,11 @@ def run(world: World, renderer: Renderer, wl: W.Workload, qps: float, duration_s
         try:
             cam = cams[i]
             snap = snaps[0]
-            v = snap.build_view(cam)
             spec, hi = W.spec_words(cam.flags, snap)
-            srv.submit(i, v, spec, hi, t_arr)
+            if device_lists:
+                srv.submit_camera(i, cam, spec, hi, t_arr)
+            else:
+                srv.submit(i, snap.build_view(cam), spec, hi, t_arr)
             px[i] = cam.width * cam.height
             with sub_lock:
                 submitted[0] += 1
@@ -139,6 +145,8 @@ def run(world: World, renderer: Renderer, wl: W.Workload, qps: float, duration_s
             srv.barrier()
             renderer.sync_world(world)
             snaps[0], snaps[1] = snaps[1], snaps[0]
+            if device_lists:
+                srv.set_chunk_table(snaps[0].chunk_table())
             srv.resume()
         finally:
             lock.release_write()
