@@ -221,21 +221,24 @@ def test_secondary_legs_match_oracle(form):
         b.set_tuning(**form)
         b.set_views(views, specs)
         b.set_secondary(True)
-        b.render()
-        b.sync()
-        total = sun = 0
+        refs = []
         for i, (v, (s, hi)) in enumerate(zip(views, specs)):
             sc = O.Scene(world.pool(0), world.pool(1), world.pool(2), v.chunks, v.grid)
             sp = O.spec_from(s, hi)
-            vis = b.read_vis(i)
             ref_vis = O.render(sc, v.frame_bytes, sp, np.zeros(v.tiles[0] * v.tiles[1], np.uint32), shadow=False)[0]
-            assert np.array_equal(vis, ref_vis)
-            legs, st = O.secondary(sc, v.frame_bytes, sp, ref_vis)
-            got = b.read_secondary(i)
-            diff = got.view(np.uint32) != legs.view(np.uint32)
-            assert not diff.any(), f"view {i}: {diff.any(axis=-1).sum()} leg records differ"
-            total += st["rays_secondary"]
-            sun += st["rays_secondary_sun"]
+            refs.append((ref_vis,) + O.secondary(sc, v.frame_bytes, sp, ref_vis))
+        for frame in range(2):  # the second frame clears the first one's records, then writes its own
+            b.render()
+            b.sync()
+            total = sun = 0
+            for i in range(len(views)):
+                ref_vis, legs, st = refs[i]
+                assert np.array_equal(b.read_vis(i), ref_vis)
+                got = b.read_secondary(i)
+                diff = got.view(np.uint32) != legs.view(np.uint32)
+                assert not diff.any(), f"frame {frame} view {i}: {diff.any(axis=-1).sum()} leg records differ"
+                total += st["rays_secondary"]
+                sun += st["rays_secondary_sun"]
         cnt = b.secondary_count()
         assert cnt["water_legs"] + cnt["glass_legs"] == total and not cnt["overflow"]
         assert cnt["sun_rays"] == sun

@@ -157,6 +157,12 @@ int secondary_buffers(vx_batch *b) {
         return rc;
     if (o0 != b->sec_out.p || q0 != b->sec_queue.p || cap != b->sec_cap) drop_graph(b);
     b->sec_cap = cap;
+    // zero records and leg counters: the next frame's clear pass has nothing to undo
+    hipStream_t st = b->ctx->stream;
+    if ((rc = vxh::fill_u32(st, b->sec_out.p, 0, std::max<uint64_t>(b->n_px, 1) * 2 * 4)) ||
+        (rc = vxh::fill_u32(st, (uint32_t *)b->counters.p + CNT_SEC, 0, SEC_LEN)))
+        return rc;
+    HIPX(hipStreamSynchronize(st));
     return VX_OK;
 }
 
@@ -166,6 +172,15 @@ int secondary_buffers(vx_batch *b) {
 int enqueue(vx_batch *b, hipStream_t st, hipEvent_t ev_march) {
     int rc;
     uint32_t *cnt = (uint32_t *)b->counters.p;
+    if (b->sec_on) {  // the previous frame's leg records, from its queues, before the counters reset
+        SecArgs sa;
+        sa.out = (uint4 *)b->sec_out.p;
+        sa.queue = (unsigned long long *)b->sec_queue.p;
+        sa.tile_off = (const uint32_t *)b->tile_off.p;
+        sa.cap = b->sec_cap;
+        k_sec_clear<<<(unsigned)b->n_cu * 4u, 256, 0, st>>>(batch_args(b), sa);
+        LAUNCHED();
+    }
     if ((rc = vxh::fill_u32(st, cnt, 0, counter_words(b->n_views)))) return rc;
     if (b->tn.mode == VX_BATCH_PLAIN) {
         vxh::TileArgs ta;

@@ -172,10 +172,12 @@ __global__ void k_shadow(Pools p, FrameK f, Spec s, const unsigned long long *__
 __global__ void k_decode(Pools p, FrameK f, const unsigned long long *__restrict__ vis,
                          const uint16_t *__restrict__ shadow, vx_hit *__restrict__ out);
 
-// Secondary bounce of a batch (one leg record per pixel and leg, 16 bytes):
-//   k_sec_gather: one wave per tile (queue entry of `order`), lane = pixel: writes empty
-//     records for the legs the pixel does not trace and appends the others to the water /
-//     glass queues (entries view << 32 | pixel << 1 | leg);
+// Secondary bounce of a batch (one leg record per pixel and leg, 16 bytes; a record is
+// zero when the pixel has no such leg):
+//   k_sec_clear: before a frame resets the counters, zeroes the records of the previous
+//     frame's queue entries, so that only pixels with legs are ever written;
+//   k_sec_gather: one wave per tile (queue entry of `order`), lane = pixel: appends the
+//     pixel's legs to the water / glass queues (entries view << 32 | pixel << 1 | leg);
 //   k_sec_trace<kWater>: one leg per lane from its queue, grouped by view, trace_world.
 struct SecArgs {
     uint4 *__restrict__ out;                  // 2 records per pixel of the batch
@@ -183,6 +185,7 @@ struct SecArgs {
     const uint32_t *__restrict__ tile_off;    // n_views + 1 tile offsets (batch slot order)
     uint32_t cap;                             // entries per queue
 };
+__global__ void k_sec_clear(BatchArgs a, SecArgs s);
 __global__ void k_sec_gather(BatchArgs a, SecArgs s);
 template <bool kWater>
 __global__ void k_sec_trace(BatchArgs a, SecArgs s);

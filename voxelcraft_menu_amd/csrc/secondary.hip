@@ -72,15 +72,32 @@ __global__ __launch_bounds__(64) void k_sec_gather(BatchArgs a, SecArgs s) {
     SecLeg legs[2];
     bool want[2] = {false, false};
     if (in_frame) secondary_legs(p, f, sp, a.vis[px0 + pix], px, py, legs, want);
-    if (in_frame) {
-        if (!want[0]) s.out[2 * (px0 + pix)] = make_uint4(0u, 0u, 0u, 0u);
-        if (!want[1]) s.out[2 * (px0 + pix) + 1] = make_uint4(0u, 0u, 0u, 0u);
-    }
     const unsigned long long e = ((unsigned long long)v << 32) | ((unsigned long long)pix << 1);
     for (uint32_t leg = 0; leg < 2; leg++) {
         const bool w = want[leg] && legs[leg].water, g = want[leg] && !legs[leg].water;
         queue_push(s, a.counters, SEC_WATER, w, e | leg);
         queue_push(s, a.counters, SEC_GLASS, g, e | leg);
+    }
+}
+
+// The previous frame's leg records back to zero: every record written is a queue entry's,
+// so clearing the entries of both queues leaves the whole record array zero. Runs before
+// the frame resets the queue counters (a new view set or bounce setting starts with zero
+// records and counters).
+__global__ __launch_bounds__(256) void k_sec_clear(BatchArgs a, SecArgs s) {
+    const uint32_t nv = a.hdr[BH_VIEWS];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t q = SEC_WATER; q <= SEC_GLASS; q++) {
+        const uint32_t n = min(ld_counter(a.counters, CNT_SEC + q * SEC_LINE), s.cap);
+        for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+            const unsigned long long e = s.queue[(size_t)q * s.cap + k];
+            const uint32_t v = (uint32_t)(e >> 32);
+            if (v >= nv) continue;
+            const uint32_t pix = (uint32_t)(e >> 1) & 0x7FFFFFFFu, leg = (uint32_t)e & 1u;
+            const ViewK *vk = a.views + v;
+            const size_t px0 = (size_t)vk->px_off_lo | ((size_t)vk->px_off_hi << 32);
+            s.out[2 * (px0 + pix) + leg] = make_uint4(0u, 0u, 0u, 0u);
+        }
     }
 }
 
