@@ -260,7 +260,9 @@ enum { VX_BATCH_PERSISTENT = 0, VX_BATCH_FLAT = 1, VX_BATCH_PLAIN = 2 };
 typedef struct vx_batch_tuning {
     uint32_t mode;            /* VX_BATCH_* */
     uint32_t occupancy;       /* waves per SIMD the march is built for: 4 or 6 */
-    uint32_t variant;         /* vx_tuning.variant bits; batched forms need the fused shadow (4, or 5 at occupancy 4) */
+    uint32_t variant;         /* vx_tuning.variant bits; batched forms need the fused shadow (4 or 5); they may add
+                                 bit 3 (8): classify only the chunks on the candidate list of the tile's 64x64-pixel
+                                 screen bin (built with the view set), not the view's whole list */
     uint32_t ordered;         /* 1: tiles most expensive first (their cost in the previous frame) */
     uint32_t queue_chunk;     /* PERSISTENT: tiles taken per queue grab, 1..64 */
     uint32_t waves_per_simd;  /* PERSISTENT: grid = CUs * 4 * waves_per_simd, 1..occupancy */

@@ -28,6 +28,10 @@ FORMS = [
     dict(mode=A.BATCH_PERSISTENT, graph=1, variant=5),
     dict(mode=A.BATCH_PERSISTENT, graph=1, queue_chunk=4, variant=5),
     dict(mode=A.BATCH_PERSISTENT, graph=1, ordered=0, occupancy=6, queue_chunk=8),
+    # screen-bin candidate lists (variant bit 8)
+    dict(mode=A.BATCH_FLAT, graph=1, occupancy=6, variant=13),
+    dict(mode=A.BATCH_FLAT, graph=0, variant=12),
+    dict(mode=A.BATCH_PERSISTENT, graph=1, queue_chunk=2, variant=13),
 ]
 
 
@@ -164,7 +168,8 @@ def test_config_batch_64_views_all_forms_identical():
     b.set_views(views, specs)
     results = {}
     for form in (dict(mode=A.BATCH_PLAIN, graph=0), dict(mode=A.BATCH_FLAT, graph=1),
-                 dict(mode=A.BATCH_PERSISTENT, graph=1, queue_chunk=2)):
+                 dict(mode=A.BATCH_PERSISTENT, graph=1, queue_chunk=2),
+                 dict(mode=A.BATCH_FLAT, graph=1, occupancy=6, variant=13)):
         print(f"64-view form {_form_id(form)}", flush=True)
         b.set_tuning(**form)
         b.reset_hiz()
@@ -183,6 +188,73 @@ def test_config_batch_64_views_all_forms_identical():
         ref = H.oracle_frames(world, views[i], s, hi, frames=2)
         assert np.array_equal(results[keys[0]][i][0], ref[1][0]), f"view {i} vs oracle"
         assert np.array_equal(results[keys[0]][i][1], ref[1][1]), f"view {i} vs oracle"
+    b.close()
+    r.close()
+
+
+def test_screen_bins_change_no_bit():
+    """Screen-bin candidate lists only drop chunks that every test of the bin's tiles
+    rejects: keys, shadow words, Hi-Z words and the select / deferred / recover counts
+    equal the whole-list forms, for cameras at the edges of the bin model (looking
+    along an axis, straight up and down, wide and narrow fields of view, jitter, odd
+    sizes with partial tiles, inside the terrain, under water), and the oracle for a
+    sample of them."""
+    wl = W.WORKLOADS[0]
+    world = W.make_world(wl)
+    flags = W.DEFAULT_FLAGS
+    from voxelcraft_menu_amd.world import CameraParams
+    import math
+    cams = W.random_cameras(wl, 6, seed=97, resolutions=[(333, 197), (130, 250), (64, 40)])
+
+    def cam(pos, yaw, pitch, w, h, fl=flags, **kw):
+        return CameraParams(pos=pos, yaw=math.radians(yaw), pitch=math.radians(pitch), width=w, height=h,
+                            flags=fl, **kw)
+
+    cams += [
+        cam((128.5, 140.0, 128.5), 0.0, 0.0, 200, 120),              # along +x, horizon row on tile rows
+        cam((128.0, 140.0, 128.0), 90.0, 0.0, 128, 128),             # along +z, on chunk corners
+        cam((100.5, 200.0, 90.5), 10.0, 89.9, 96, 64),               # straight up
+        cam((100.5, 200.0, 90.5), 30.0, -89.9, 96, 64),              # straight down
+        cam((60.5, 150.0, 200.5), 45.0, -20.0, 160, 90, fov_deg=150.0),
+        cam((60.5, 150.0, 200.5), 200.0, -10.0, 160, 90, fov_deg=8.0),
+        cam((30.5, 135.0, 30.5), 300.0, -15.0, 150, 101, jitter=(0.37, -0.42)),
+        cam((120.5, 60.0, 130.5), 120.0, 5.0, 72, 40),               # inside the terrain
+    ]
+    h = wl.gen.height(200, 40)
+    cams.append(H.camera((200.5, min(h, 127) - 0.5, 40.5), 120.0, 10.0, 96, 72, flags | A.FLAG_UNDERWATER))
+    views = [build_view(world, c) for c in cams]
+    specs = [W.spec_words(c.flags, world) for c in cams]
+    r = _renderer(world, views)
+    b = Batch(r, len(views))
+    b.set_views(views, specs)
+    out = {}
+    for var in (5, 13, 4, 12):
+        for mode in (A.BATCH_FLAT, A.BATCH_PERSISTENT):
+            form = dict(mode=mode, graph=1, occupancy=6, variant=var)
+            print(f"bins form {_form_id(form)}", flush=True)
+            b.set_tuning(**form)
+            b.reset_hiz()
+            got = []
+            for f in range(2):
+                b.render()
+                b.sync()
+                got.append(([(b.read_vis(i), b.read_shadow(i), b.read_hiz(i)) for i in range(len(views))],
+                            b.demand()))
+            out[_form_id(form)] = got
+    keys = list(out)
+    for k in keys[1:]:
+        for f in range(2):
+            assert out[k][f][1] == out[keys[0]][f][1], f"{k} frame {f}: demand {out[k][f][1]}"
+            for i in range(len(views)):
+                for j, name in enumerate(("vis", "shadow", "hiz")):
+                    assert np.array_equal(out[k][f][0][i][j], out[keys[0]][f][0][i][j]), f"{k} frame {f} view {i} {name}"
+    for i in (6, 8, 10, 12):
+        s, hi = specs[i]
+        ref = H.oracle_frames(world, views[i], s, hi, frames=2)
+        got = out["mode1-graph1-occupancy6-variant13"]
+        for f in range(2):
+            for j, name in enumerate(("vis", "shadow", "hiz")):
+                assert np.array_equal(got[f][0][i][j], ref[f][j]), f"view {i} frame {f} {name} vs oracle"
     b.close()
     r.close()
 

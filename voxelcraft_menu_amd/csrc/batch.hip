@@ -51,6 +51,9 @@ struct vx_batch {
     bool sec_on = false;
     DevBuf sec_out, sec_queue;
     DevBuf table, cams, list_counts;  // device chunk lists: chunk table, per-view cameras, counts
+    DevBuf bin_cnt, bin_list;         // screen-bin candidate lists (k_bin_chunks), when bins_ok
+    uint32_t n_bins = 0;
+    bool bins_ok = false;
     uint32_t n_table = 0;
     uint32_t sec_cap = 0;
     uint64_t vk_pool_gen = 0;  // pool generation the device descriptors' pool fields were written for
@@ -69,6 +72,9 @@ struct vx_batch {
 };
 
 namespace {
+
+// Largest screen-bin list allocation; a view set that needs more keeps whole lists.
+constexpr uint64_t BIN_LIST_MAX_BYTES = 1ull << 30;
 
 // Per-view control blocks of the PLAIN form follow the shared counters.
 size_t counter_words(uint32_t n_views) { return (size_t)CNT_TOTAL + (size_t)n_views * CTL_LEN; }
@@ -124,6 +130,10 @@ BatchArgs batch_args(const vx_batch *b) {
     a.counters = (uint32_t *)b->counters.p;
     a.order = (const uint32_t *)b->order.p;
     a.cost = (uint32_t *)b->cost.p;
+    if ((b->tn.variant & VAR_BINS) && b->bins_ok && b->tn.mode != VX_BATCH_PLAIN) {
+        a.bin_cnt = (const uint32_t *)b->bin_cnt.p;
+        a.bin_list = (const uint16_t *)b->bin_list.p;
+    }
     return a;
 }
 
@@ -261,8 +271,9 @@ int capture(vx_batch *b) {
 }
 
 int view_buffers(vx_batch *b, uint32_t n_views, uint32_t n_tiles, uint64_t n_px, size_t n_chunk_recs, bool &moved) {
-    const void *before[11] = {b->views.p, b->tile_off.p, b->hdr.p,  b->chunks.p, b->grids.p,   b->vis.p,
-                              b->shadow.p, b->hiz.p,     b->cost.p, b->order.p,  b->counters.p};
+    const void *before[13] = {b->views.p, b->tile_off.p, b->hdr.p,     b->chunks.p,  b->grids.p,
+                              b->vis.p,   b->shadow.p,   b->hiz.p,     b->cost.p,    b->order.p,
+                              b->counters.p, b->bin_cnt.p, b->bin_list.p};
     int rc;
     if ((rc = ensure(b->views, (size_t)std::max(n_views, 1u) * sizeof(ViewK))) ||
         (rc = ensure(b->tile_off, ((size_t)n_views + 1) * 4)) || (rc = ensure(b->hdr, BH_LEN * 4)) ||
@@ -275,8 +286,9 @@ int view_buffers(vx_batch *b, uint32_t n_views, uint32_t n_tiles, uint64_t n_px,
         (rc = ensure(b->order, (size_t)std::max(n_tiles, 1u) * 4)) ||
         (rc = ensure(b->counters, counter_words(n_views) * 4)))
         return rc;
-    const void *after[11] = {b->views.p, b->tile_off.p, b->hdr.p,  b->chunks.p, b->grids.p,   b->vis.p,
-                             b->shadow.p, b->hiz.p,     b->cost.p, b->order.p,  b->counters.p};
+    const void *after[13] = {b->views.p, b->tile_off.p, b->hdr.p,     b->chunks.p,  b->grids.p,
+                             b->vis.p,   b->shadow.p,   b->hiz.p,     b->cost.p,    b->order.p,
+                             b->counters.p, b->bin_cnt.p, b->bin_list.p};
     moved = std::memcmp(before, after, sizeof before) != 0;
     return VX_OK;
 }
@@ -424,6 +436,7 @@ struct ViewPlan {
     std::vector<uint64_t> poff;
     uint32_t cb[N_CLASSES + 1] = {0, 0, 0, 0, 0};
     size_t recs = 0;
+    uint64_t bins = 0, bin_entries = 0;  // screen bins of the set and their list entries
 };
 
 template <class FrameOf, class SpecOf, class SpecHiOf, class TotalOf>
@@ -462,6 +475,12 @@ int plan_views(const vx_batch *b, uint32_t n, FrameOf frame_of, SpecOf spec_of, 
         k.tile_off = pl.toff[v];
         k.px_off_lo = (uint32_t)pl.poff[v];
         k.px_off_hi = (uint32_t)(pl.poff[v] >> 32);
+        const uint64_t nbv = (uint64_t)bins_x(k.f.tiles[0]) * bins_y(k.f.tiles[1]);
+        k.bin_off = (uint32_t)std::min<uint64_t>(pl.bins, 0xFFFFFFFFull);
+        k.bin_stride = total;
+        k.binl_off = (uint32_t)std::min<uint64_t>(pl.bin_entries, 0xFFFFFFFFull);
+        pl.bins += nbv;
+        pl.bin_entries += nbv * total;
         const uint64_t nt = (uint64_t)k.f.tiles[0] * k.f.tiles[1];
         if (pl.toff[v] + nt > 0xFFFFFFFFull) return VX_E_OVERFLOW;
         pl.toff[v + 1] = pl.toff[v] + (uint32_t)nt;
@@ -483,13 +502,37 @@ int install_views(vx_batch *b, uint32_t n, ViewPlan &pl, bool moved) {
     const std::vector<uint32_t> &toff = pl.toff;
     const std::vector<uint64_t> &poff = pl.poff;
     const uint32_t *cb = pl.cb;
+    // Screen-bin lists go with the view set: they depend only on the views' frames and
+    // chunk lists. A set whose lists would not fit keeps whole lists (bins off).
+    const bool bins_ok = pl.bins > 0 && pl.bins <= 0xFFFFFFFFull && pl.bin_entries <= 0xFFFFFFFFull &&
+                         pl.bin_entries * 2 <= BIN_LIST_MAX_BYTES;
+    if (bins_ok) {
+        const void *c0 = b->bin_cnt.p, *l0 = b->bin_list.p;
+        if ((rc = ensure(b->bin_cnt, (size_t)pl.bins * 4)) ||
+            (rc = ensure(b->bin_list, (size_t)std::max<uint64_t>(pl.bin_entries, 1) * 2))) {
+            drop_graph(b);
+            b->n_views = 0;
+            b->bins_ok = false;
+            return rc;
+        }
+        moved |= c0 != b->bin_cnt.p || l0 != b->bin_list.p;
+    }
+    moved |= bins_ok != b->bins_ok;  // a captured graph holds the bin pointers or not
     uint32_t hdr[BH_LEN] = {};
     hdr[BH_TILES] = toff[n];
     hdr[BH_VIEWS] = n;
+    hdr[BH_BINS] = bins_ok ? (uint32_t)pl.bins : 0u;
     for (uint32_t k = 0; k <= N_CLASSES; k++) hdr[BH_CB + k] = cb[k];
     HIPX(hipMemcpyAsync(b->views.p, pl.vk.data(), (size_t)n * sizeof(ViewK), hipMemcpyHostToDevice, st));
     HIPX(hipMemcpyAsync(b->tile_off.p, toff.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice, st));
     HIPX(hipMemcpyAsync(b->hdr.p, hdr, sizeof hdr, hipMemcpyHostToDevice, st));
+    if (bins_ok) {
+        k_bin_chunks<<<(unsigned)b->n_cu * 4u, 256, 0, st>>>(batch_args(b), (uint16_t *)b->bin_list.p,
+                                                             (uint32_t *)b->bin_cnt.p);
+        LAUNCHED();
+    }
+    b->bins_ok = bins_ok;
+    b->n_bins = bins_ok ? (uint32_t)pl.bins : 0u;
     // New views start as a new Renderer would: empty targets, Hi-Z 0; costs 0.
     if ((rc = vxh::fill_u64(st, b->vis.p, 0, poff[n])) || (rc = vxh::fill_u32(st, b->shadow.p, 0, (poff[n] + 1) / 2)) ||
         (rc = vxh::fill_u32(st, b->hiz.p, 0, toff[n])) || (rc = vxh::fill_u32(st, b->cost.p, 0, toff[n])))
@@ -570,7 +613,7 @@ int vx_batch_destroy(vx_batch *b) {
     drop_graph(b);
     DevBuf *bufs[] = {&b->views,   &b->tile_off, &b->hdr,       &b->chunks, &b->grids, &b->vis,
                       &b->shadow,  &b->hiz,      &b->cost,      &b->order,  &b->counters, &b->sec_out,
-                      &b->sec_queue, &b->table,  &b->cams,      &b->list_counts};
+                      &b->sec_queue, &b->table,  &b->cams,      &b->list_counts, &b->bin_cnt, &b->bin_list};
     for (DevBuf *d : bufs) vxh::release(*d);
     for (auto &set : b->ring)
         for (auto &e : set)
@@ -857,7 +900,7 @@ int vx_batch_autotune(vx_batch *b, uint32_t reps, vx_batch_tuning *best, float *
         cand.push_back(t);
     };
     for (uint32_t occ : {4u, 6u})
-        for (uint32_t var : {4u, 5u}) {
+        for (uint32_t var : {4u, 5u, 12u, 13u}) {
             add(VX_BATCH_FLAT, occ, var, 1, occ);
             add(VX_BATCH_PERSISTENT, occ, var, 4, occ);
         }

@@ -70,7 +70,9 @@ __global__ void k_finalize_recover(FrameK f, uint32_t *counters);
 //   VAR_SPLIT: classify the whole chunk list into LDS bitmasks before marching; otherwise
 //              each 64-chunk block is classified and marched in turn.
 //   VAR_FUSED_SHADOW: trace each pixel's sun ray in the tile's wave right after its key.
-constexpr int VAR_SPLIT = 1, VAR_FUSED_SHADOW = 4;
+//   VAR_BINS (batched forms): classify only the chunks on the candidate list of the
+//              tile's screen bin (k_bin_chunks) instead of the view's whole list.
+constexpr int VAR_SPLIT = 1, VAR_FUSED_SHADOW = 4, VAR_BINS = 8;
 
 // Arguments of the single-view tile march (one frame of one view).
 struct TileArgs {
@@ -115,6 +117,9 @@ struct ViewK {
     uint32_t grid_off;   // first word of the view's 32x8x32 chunk grid
     uint32_t tile_off;   // first Hi-Z / cost word of the view
     uint32_t px_off_lo, px_off_hi;  // first pixel of the view in vis / shadow
+    uint32_t bin_off;     // first screen bin of the view (bins of BIN_TILES x BIN_TILES tiles)
+    uint32_t bin_stride;  // list entries per bin: the records reserved for the view's list
+    uint32_t binl_off;    // first list entry of the view's bins
     // The tree pools with the view's chunk list and grid (chunks = batch list + chunk_off,
     // grid = batch grids + grid_off, n_chunks): kernels read it in place, field by field.
     // The host rewrites it when a pool moves or grows.
@@ -140,7 +145,8 @@ __global__ void k_view_lists(const vx_gpu_chunk *__restrict__ table, uint32_t n_
 
 // Header words of a batch in device memory (read by graph-captured kernels, so a replay
 // follows the current view set): tile and view counts, class boundaries cb[0..4].
-constexpr uint32_t BH_TILES = 0, BH_VIEWS = 1, BH_CB = 4, BH_LEN = 12;
+// BH_BINS: screen bins of all views.
+constexpr uint32_t BH_TILES = 0, BH_VIEWS = 1, BH_BINS = 2, BH_CB = 4, BH_LEN = 12;
 
 struct BatchArgs {
     Pools p;                  // tree pools; chunks/grid are the batch's concatenated arrays
@@ -152,7 +158,25 @@ struct BatchArgs {
     uint32_t *__restrict__ counters;
     const uint32_t *__restrict__ order;
     uint32_t *__restrict__ cost;
+    // Screen-bin candidate lists (k_bin_chunks): per bin a count and, at binl_off +
+    // bin * bin_stride of its view, the chunk indices in list order. bin_cnt null: every
+    // tile classifies its view's whole list.
+    const uint32_t *__restrict__ bin_cnt = nullptr;
+    const uint16_t *__restrict__ bin_list = nullptr;
 };
+
+// Screen bins: a bin is BIN_TILES x BIN_TILES tiles (64 x 64 pixels), in rows of
+// ceil(tiles_x / BIN_TILES) bins.
+constexpr uint32_t BIN_TILES = 8;
+__host__ __device__ constexpr uint32_t bins_x(uint32_t tiles_x) { return (tiles_x + BIN_TILES - 1) / BIN_TILES; }
+__host__ __device__ constexpr uint32_t bins_y(uint32_t tiles_y) { return (tiles_y + BIN_TILES - 1) / BIN_TILES; }
+
+// k_bin_chunks: one wave per screen bin of the batch (a grid-stride loop over BH_BINS),
+// writes the bin's candidate list: every chunk of the view's visible list except those
+// that lie wholly in front of the camera and wholly outside one of the bin's four side
+// planes, each moved outwards by a margin. Such a chunk is outside the matching side
+// plane of every tile in the bin, so tile_select's tests reject it for each of them.
+__global__ void k_bin_chunks(BatchArgs a, uint16_t *__restrict__ bin_list, uint32_t *__restrict__ bin_cnt);
 
 // Batched tile march over the queue range of one view class (kHiz, kUnderwater; the
 // range is the header's cb[c]..cb[c+1]): FLAT = one workgroup per entry; PERSISTENT =

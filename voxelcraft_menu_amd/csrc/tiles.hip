@@ -36,7 +36,8 @@ template <bool kUseHiz, bool kUnderwater, int kVar>
 __device__ __forceinline__ void march_tile_body(const Pools &p, const FrameK &f, const Spec &s, uint32_t tile,
                                                 uint32_t *__restrict__ hiz, unsigned long long *__restrict__ vis,
                                                 uint16_t *__restrict__ shadow, uint32_t *__restrict__ counters,
-                                                uint64_t *defmask, uint64_t *selmask) {
+                                                uint64_t *defmask, uint64_t *selmask,
+                                                const uint32_t *bin_n = nullptr, const uint16_t *bin_l = nullptr) {
     const uint32_t lane = lane_id();
     const uint32_t gx = tile % f.tiles[0], gy = tile / f.tiles[0];
     const uint32_t px = gx * 8u + (lane & 7u), py = gy * 8u + (lane >> 3);
@@ -49,7 +50,11 @@ __device__ __forceinline__ void march_tile_body(const Pools &p, const FrameK &f,
     const uint32_t tile_far = kUseHiz ? hiz[tile] : 0u;
     uint64_t best = 0;
     uint32_t n_sel = 0, n_def = 0, n_rec = 0;
-    const uint32_t nwords = (f.chunk_count + 63u) >> 6;
+    // The items classified: the view's visible list, or the tile's bin list (the same
+    // chunks in the same order, less some that every test of this tile rejects).
+    const uint32_t n_items = bin_l ? *bin_n : min(f.chunk_count, p.n_chunks);
+    auto chunk_at = [&](uint32_t pos) -> uint32_t { return bin_l ? (uint32_t)bin_l[pos] : pos; };
+    const uint32_t nwords = (n_items + 63u) >> 6;
 
     auto march_one = [&](uint32_t cj, uint32_t pk) {
         // pk: prune key of chunk cj; a lane whose key already beats every hit the chunk
@@ -69,10 +74,11 @@ __device__ __forceinline__ void march_tile_body(const Pools &p, const FrameK &f,
     if constexpr ((kVar & VAR_SPLIT) != 0) {
         // Classify the whole chunk list first: the box loads of all words are independent.
         for (uint32_t w = 0; w < nwords; w++) {
-            uint32_t ci = (w << 6) + lane;
+            const uint32_t pos = (w << 6) + lane;
             uint32_t cls = 0;
             float dmin = 0.0f;
-            if (ci < f.chunk_count && ci < p.n_chunks) {
+            if (pos < n_items) {
+                const uint32_t ci = chunk_at(pos);
                 ChunkK c = ld_chunk(p, ci);
                 cls = classify(f, tc, c.bmin, c.bmax, ci, tile_far, kUseHiz, dmin);
             }
@@ -91,21 +97,22 @@ __device__ __forceinline__ void march_tile_body(const Pools &p, const FrameK &f,
             if (!sel) continue;
             uint32_t pk = 0;
             if ((sel >> lane) & 1ull) {
-                ChunkK c = ld_chunk(p, (w << 6) + lane);
+                ChunkK c = ld_chunk(p, chunk_at((w << 6) + lane));
                 pk = prune_key(f.far_, box_distance(f.cam, c.bmin, c.bmax));
             }
             while (sel) {
                 uint32_t j = (uint32_t)__ffsll((long long)sel) - 1u;
                 sel &= sel - 1ull;
-                march_one((w << 6) + j, (uint32_t)__shfl((int)pk, (int)j));
+                march_one(chunk_at((w << 6) + j), (uint32_t)__shfl((int)pk, (int)j));
             }
         }
     } else {
         for (uint32_t w = 0; w < nwords; w++) {
-            uint32_t ci = (w << 6) + lane;
+            const uint32_t pos = (w << 6) + lane;
             uint32_t cls = 0;
             float dmin = 0.0f;
-            if (ci < f.chunk_count && ci < p.n_chunks) {
+            if (pos < n_items) {
+                const uint32_t ci = chunk_at(pos);
                 ChunkK c = ld_chunk(p, ci);
                 cls = classify(f, tc, c.bmin, c.bmax, ci, tile_far, kUseHiz, dmin);
             }
@@ -120,7 +127,7 @@ __device__ __forceinline__ void march_tile_body(const Pools &p, const FrameK &f,
             while (sel) {
                 uint32_t j = (uint32_t)__ffsll((long long)sel) - 1u;
                 sel &= sel - 1ull;
-                march_one((w << 6) + j, (uint32_t)__shfl((int)pk, (int)j));
+                march_one(chunk_at((w << 6) + j), (uint32_t)__shfl((int)pk, (int)j));
             }
         }
     }
@@ -134,11 +141,10 @@ __device__ __forceinline__ void march_tile_body(const Pools &p, const FrameK &f,
         for (uint32_t w = 0; w < nwords; w++) {
             uint64_t def = defmask[w];
             if (!def) continue;
-            uint32_t ci = (w << 6) + lane;
             bool rec = false;
             uint32_t pk = 0;
             if ((def >> lane) & 1ull) {
-                ChunkK c = ld_chunk(p, ci);
+                ChunkK c = ld_chunk(p, chunk_at((w << 6) + lane));
                 float dmin = box_distance(f.cam, c.bmin, c.bmax);
                 rec = depth_key(f.far_, dmin) >= d;
                 pk = prune_key(f.far_, dmin);
@@ -148,7 +154,7 @@ __device__ __forceinline__ void march_tile_body(const Pools &p, const FrameK &f,
             while (r) {
                 uint32_t j = (uint32_t)__ffsll((long long)r) - 1u;
                 r &= r - 1ull;
-                march_one((w << 6) + j, (uint32_t)__shfl((int)pk, (int)j));
+                march_one(chunk_at((w << 6) + j), (uint32_t)__shfl((int)pk, (int)j));
             }
         }
     } else {
@@ -241,15 +247,153 @@ __device__ __forceinline__ void batch_tile(const BatchArgs &a, uint32_t entry, u
     const Pools &p = *(const Pools *)as_const(&vk->p);
     const size_t px0 = (size_t)vc->px_off_lo | ((size_t)vc->px_off_hi << 32);
     const uint32_t toff = vc->tile_off;
+    const uint32_t *bin_n = nullptr;
+    const uint16_t *bin_l = nullptr;
+    if (a.bin_cnt) {
+        const uint32_t tx = f.tiles[0];
+        const uint32_t bin = (tile / tx / BIN_TILES) * bins_x(tx) + (tile % tx) / BIN_TILES;
+        bin_n = a.bin_cnt + vc->bin_off + bin;
+        bin_l = a.bin_list + vc->binl_off + (size_t)bin * vc->bin_stride;
+    }
     const uint64_t t0 = wall_clock64();
     march_tile_body<kHiz, kUnderwater, kVar>(p, f, s, tile, a.hiz + toff, a.vis + px0, a.shadow + px0, a.counters,
-                                             defmask, selmask);
+                                             defmask, selmask, bin_n, bin_l);
     if (lane_id() == 0) {
         uint64_t dt = wall_clock64() - t0;
         a.cost[toff + tile] = dt > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)dt;
     }
 }
 
+
+// ---- Screen bins ------------------------------------------------------------------------
+// ray_dir's pinhole model: pixel input (p, q) looks along fwd + right * sx(p) + up * sy(q),
+// sx(p) = ((p + jitter_x) / W * 2 - 1) * tan * aspect, sy(q) = (1 - (q + jitter_y) / H * 2) * tan.
+// A tile's side planes (tile_cull_setup) are the planes through the camera that hold the
+// rays of its edges. A bin's side plane, moved BIN_MARGIN pixels outwards, has every
+// tile's matching plane at least that far inside it, so a box wholly in front of the
+// camera (positive along fwd in that model) and wholly outside the moved plane is at
+// least BIN_MARGIN pixels outside each tile's frustum: no pixel ray of those tiles (at
+// pixel centres) meets it, and tile_select's plane test rejects it. A box is counted
+// outside only when its largest signed distance is below -BIN_REL times a bound of the
+// distances over the box, so that boxes that reach near the camera stay candidates.
+constexpr float BIN_MARGIN = 2.0f, BIN_REL = 1e-3f;
+
+__device__ __forceinline__ void cross3(const float a[3], const float b[3], float o[3]) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+__device__ __forceinline__ float dot3(const float a[3], const float b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// Largest value of n . (x - cam) over the box and a bound of |n . (x - cam)| over it.
+__device__ __forceinline__ void box_plane_range(const float n[3], const float lo[3], const float hi[3], float &mx,
+                                                float &bound) {
+    mx = 0.0f;
+    bound = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const float a = n[i] * lo[i], b = n[i] * hi[i];
+        mx += fmaxf(a, b);
+        bound += fmaxf(fabsf(a), fabsf(b));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bin_chunks(BatchArgs a, uint16_t *__restrict__ bin_list,
+                                                    uint32_t *__restrict__ bin_cnt) {
+    const uint32_t nb = a.hdr[BH_BINS], nv = a.hdr[BH_VIEWS];
+    const uint32_t lane = lane_id();
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    uint32_t g = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+    for (; g < nb; g += waves) {
+        // The view of bin g: the last one whose first bin is <= g (bin_off grows with the slot).
+        uint32_t lo = 0, hi = nv;
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (as_const(a.views + mid)->bin_off <= g) lo = mid;
+            else hi = mid;
+        }
+        const ViewK *vk = a.views + lo;
+        const_as<ViewK> *vc = as_const(vk);
+        const FrameK &f = *(const FrameK *)as_const(&vk->f);
+        const Pools &p = *(const Pools *)as_const(&vk->p);
+        const uint32_t b = g - vc->bin_off;
+        const uint32_t nbx = bins_x(f.tiles[0]);
+        const uint32_t bx = b % nbx, by = b / nbx;
+        const uint32_t n = min(f.chunk_count, p.n_chunks);
+        uint16_t *out = bin_list + vc->binl_off + (size_t)b * vc->bin_stride;
+        // Views whose camera the model above does not cover well keep whole lists.
+        const float th = f.tan_half_fov, ta = f.tan_half_fov * f.aspect;
+        float rxu[3];
+        cross3(f.right, f.up, rxu);
+        const float det = dot3(f.fwd, rxu);
+        const float nf = dot3(f.fwd, f.fwd), nr = dot3(f.right, f.right), nu = dot3(f.up, f.up);
+        const bool usable = th > 0.0f && th < 8.0f && ta > 0.0f && ta < 8.0f && f.res[0] <= 16384u &&
+                            f.res[1] <= 16384u && fabsf(f.jitter[0]) <= 4.0f && fabsf(f.jitter[1]) <= 4.0f &&
+                            nf >= 0.25f && nf <= 4.0f && nr >= 0.25f && nr <= 4.0f && nu >= 0.25f && nu <= 4.0f &&
+                            fabsf(det) >= 0.5f;
+        float pl[5][3];  // left, right, top, bottom, front
+        {
+            const float W = (float)f.res[0], H = (float)f.res[1];
+            constexpr float span = (float)(BIN_TILES * 8u);
+            const float px0 = (float)(bx * BIN_TILES * 8u) - BIN_MARGIN, px1 = px0 + span + 2.0f * BIN_MARGIN;
+            const float py0 = (float)(by * BIN_TILES * 8u) - BIN_MARGIN, py1 = py0 + span + 2.0f * BIN_MARGIN;
+            auto sx = [&](float q) { return ((q + f.jitter[0]) / W * 2.0f - 1.0f) * ta; };
+            auto sy = [&](float q) { return (1.0f - (q + f.jitter[1]) / H * 2.0f) * th; };
+            const float cx = sx(px0 + 0.5f * span + BIN_MARGIN), cy = sy(py0 + 0.5f * span + BIN_MARGIN);
+            float ce[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++) ce[i] = f.fwd[i] + f.right[i] * cx + f.up[i] * cy;
+            const float s4[4] = {sx(px0), sx(px1), sy(py0), sy(py1)};
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                float d[3];
+#pragma unroll
+                for (int i = 0; i < 3; i++) d[i] = f.fwd[i] + (k < 2 ? f.right[i] : f.up[i]) * s4[k];
+                if (k < 2) cross3(d, f.up, pl[k]);
+                else cross3(f.right, d, pl[k]);
+                if (dot3(pl[k], ce) < 0.0f)
+#pragma unroll
+                    for (int i = 0; i < 3; i++) pl[k][i] = -pl[k][i];
+            }
+            const float sg = det < 0.0f ? -1.0f : 1.0f;
+#pragma unroll
+            for (int i = 0; i < 3; i++) pl[4][i] = rxu[i] * sg;
+        }
+        uint32_t count = 0;
+        for (uint32_t w = 0; w < n; w += 64u) {
+            const uint32_t ci = w + lane;
+            bool keep = false;
+            if (ci < n) {
+                keep = true;
+                if (usable && ci != f.camera_chunk) {
+                    ChunkK c = ld_chunk(p, ci);
+                    float lo3[3], hi3[3];
+#pragma unroll
+                    for (int i = 0; i < 3; i++) {
+                        lo3[i] = c.bmin[i] - f.cam[i];
+                        hi3[i] = c.bmax[i] - f.cam[i];
+                    }
+                    // in front: the smallest front distance over the box is positive
+                    float neg[3] = {-pl[4][0], -pl[4][1], -pl[4][2]};
+                    float mx, bound;
+                    box_plane_range(neg, lo3, hi3, mx, bound);
+                    const bool front = mx < -BIN_REL * bound;
+                    bool out_side = false;
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        box_plane_range(pl[k], lo3, hi3, mx, bound);
+                        out_side |= mx < -BIN_REL * bound;
+                    }
+                    keep = !(front && out_side);
+                }
+            }
+            const uint64_t m = ballot(keep);
+            if (keep) out[count + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (uint16_t)ci;
+            count += (uint32_t)__popcll(m);
+        }
+        if (lane == 0) bin_cnt[g] = count;
+    }
+}
 
 template <int kOcc, int kVar, bool kHiz, bool kUnderwater>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kOcc))) void k_batch_flat(BatchArgs) {
@@ -308,7 +452,8 @@ VX_BATCH_INST(6, 5)
 
 bool batch_form_ok(uint32_t mode, uint32_t occ, uint32_t var) {
     if (mode != VX_BATCH_FLAT && mode != VX_BATCH_PERSISTENT) return false;
-    return (occ == 4 || occ == 6) && (var == 4 || var == 5);
+    const uint32_t v = var & ~(uint32_t)VAR_BINS;  // bins: a launch argument, not a kernel form
+    return (occ == 4 || occ == 6) && (v == 4 || v == 5);
 }
 
 template <int O, int V, bool H, bool U>
@@ -335,7 +480,7 @@ static void launch_classes_ov(uint32_t mode, uint32_t chunk, uint32_t wps, int n
 
 void launch_batch_classes(uint32_t mode, uint32_t occ, uint32_t var, uint32_t chunk, uint32_t wps, int n_cu,
                           const uint32_t cb[N_CLASSES + 1], const BatchArgs &a, hipStream_t st) {
-    switch (occ * 16 + var) {
+    switch (occ * 16 + (var & ~(uint32_t)VAR_BINS)) {
         case 4 * 16 + 5: launch_classes_ov<4, 5>(mode, chunk, wps, n_cu, cb, a, st); break;
         case 6 * 16 + 4: launch_classes_ov<6, 4>(mode, chunk, wps, n_cu, cb, a, st); break;
         case 6 * 16 + 5: launch_classes_ov<6, 5>(mode, chunk, wps, n_cu, cb, a, st); break;
