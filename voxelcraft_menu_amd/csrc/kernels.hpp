@@ -174,8 +174,10 @@ __host__ __device__ constexpr uint32_t bins_y(uint32_t tiles_y) { return (tiles_
 // k_bin_chunks: one wave per screen bin of the batch (a grid-stride loop over BH_BINS),
 // writes the bin's candidate list: every chunk of the view's visible list except those
 // that lie wholly in front of the camera and wholly outside one of the bin's four side
-// planes, each moved outwards by a margin. Such a chunk is outside the matching side
-// plane of every tile in the bin, so tile_select's tests reject it for each of them.
+// planes, each moved outwards by a margin. Such a chunk lies at least the margin outside
+// the matching side plane of every tile in the bin: no pixel ray of those tiles meets it,
+// and tile_select's f32 plane test rejects it unless that test's own rounding is larger
+// than the margin (tile edges spanning far less than a milliradian).
 __global__ void k_bin_chunks(BatchArgs a, uint16_t *__restrict__ bin_list, uint32_t *__restrict__ bin_cnt);
 
 // Batched tile march over the queue range of one view class (kHiz, kUnderwater; the

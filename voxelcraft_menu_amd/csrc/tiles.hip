@@ -273,7 +273,8 @@ __device__ __forceinline__ void batch_tile(const BatchArgs &a, uint32_t entry, u
 // tile's matching plane at least that far inside it, so a box wholly in front of the
 // camera (positive along fwd in that model) and wholly outside the moved plane is at
 // least BIN_MARGIN pixels outside each tile's frustum: no pixel ray of those tiles (at
-// pixel centres) meets it, and tile_select's plane test rejects it. A box is counted
+// pixel centres) meets it, so no key changes, and tile_select's plane test rejects it
+// unless that test's own f32 rounding is larger than the margin. A box is counted
 // outside only when its largest signed distance is below -BIN_REL times a bound of the
 // distances over the box, so that boxes that reach near the camera stay candidates.
 constexpr float BIN_MARGIN = 2.0f, BIN_REL = 1e-3f;
