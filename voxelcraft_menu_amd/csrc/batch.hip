@@ -271,9 +271,8 @@ int capture(vx_batch *b) {
 }
 
 int view_buffers(vx_batch *b, uint32_t n_views, uint32_t n_tiles, uint64_t n_px, size_t n_chunk_recs, bool &moved) {
-    const void *before[13] = {b->views.p, b->tile_off.p, b->hdr.p,     b->chunks.p,  b->grids.p,
-                              b->vis.p,   b->shadow.p,   b->hiz.p,     b->cost.p,    b->order.p,
-                              b->counters.p, b->bin_cnt.p, b->bin_list.p};
+    const void *before[11] = {b->views.p, b->tile_off.p, b->hdr.p,  b->chunks.p, b->grids.p,   b->vis.p,
+                              b->shadow.p, b->hiz.p,     b->cost.p, b->order.p,  b->counters.p};
     int rc;
     if ((rc = ensure(b->views, (size_t)std::max(n_views, 1u) * sizeof(ViewK))) ||
         (rc = ensure(b->tile_off, ((size_t)n_views + 1) * 4)) || (rc = ensure(b->hdr, BH_LEN * 4)) ||
@@ -286,9 +285,8 @@ int view_buffers(vx_batch *b, uint32_t n_views, uint32_t n_tiles, uint64_t n_px,
         (rc = ensure(b->order, (size_t)std::max(n_tiles, 1u) * 4)) ||
         (rc = ensure(b->counters, counter_words(n_views) * 4)))
         return rc;
-    const void *after[13] = {b->views.p, b->tile_off.p, b->hdr.p,     b->chunks.p,  b->grids.p,
-                             b->vis.p,   b->shadow.p,   b->hiz.p,     b->cost.p,    b->order.p,
-                             b->counters.p, b->bin_cnt.p, b->bin_list.p};
+    const void *after[11] = {b->views.p, b->tile_off.p, b->hdr.p,  b->chunks.p, b->grids.p,   b->vis.p,
+                             b->shadow.p, b->hiz.p,     b->cost.p, b->order.p,  b->counters.p};
     moved = std::memcmp(before, after, sizeof before) != 0;
     return VX_OK;
 }
