@@ -5,6 +5,7 @@ cpu_baseline leg of bench.py as the checker. The product never imports it.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 import subprocess
@@ -57,48 +58,67 @@ def build(force: bool = False) -> str:
     return LIB
 
 
+def _bind(L):
+    V = C.c_void_p
+    L.vxo_render.restype = C.c_int
+    L.vxo_render.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, C.POINTER(Spec), V, V, V,
+                             C.POINTER(Stats), C.c_int]
+    L.vxo_render_heat.restype = C.c_int
+    L.vxo_render_heat.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, C.POINTER(Spec), V, V, V, V,
+                                  C.POINTER(Stats), C.c_int]
+    L.vxo_secondary.restype = C.c_int
+    L.vxo_secondary.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, C.POINTER(Spec), V, V,
+                                C.POINTER(Stats), C.c_int]
+    L.vxo_primary_shadow.restype = C.c_int
+    L.vxo_primary_shadow.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, C.POINTER(Spec), V, V,
+                                     C.POINTER(Stats), C.c_int]
+    L.vxo_decode.restype = C.c_int
+    L.vxo_decode.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, V, C.c_int]
+    L.vxo_ray_dir.restype = None
+    L.vxo_ray_dir.argtypes = [V, C.c_float, C.c_float, V]
+    L.vxo_depth_key.restype = C.c_uint32
+    L.vxo_depth_key.argtypes = [V, C.c_float]
+    L.vxo_make_key.restype = C.c_uint64
+    L.vxo_make_key.argtypes = [V, C.c_float, C.c_uint32, C.c_uint32, V, V]
+    L.vxo_hash_u32.restype = C.c_uint32
+    L.vxo_hash_u32.argtypes = [C.c_uint32]
+    L.vxo_f32_to_f16.restype = C.c_uint16
+    L.vxo_f32_to_f16.argtypes = [C.c_float]
+    L.vxo_block_at.restype = C.c_uint32
+    L.vxo_block_at.argtypes = [C.POINTER(Pools), V, C.c_uint32, V]
+    L.vxo_march_chunk.restype = None
+    L.vxo_march_chunk.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, C.POINTER(Spec), C.c_uint32,
+                                  V, V, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(Hit)]
+    L.vxo_shadow_ray_t.restype = C.c_float
+    L.vxo_shadow_ray_t.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, C.POINTER(Spec), V, V,
+                                   C.c_float]
+    L.vxo_hardware_threads.restype = C.c_int
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        L = C.CDLL(LIB)
-        V = C.c_void_p
-        L.vxo_render.restype = C.c_int
-        L.vxo_render.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, C.POINTER(Spec), V, V, V,
-                                 C.POINTER(Stats), C.c_int]
-        L.vxo_render_heat.restype = C.c_int
-        L.vxo_render_heat.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, C.POINTER(Spec), V, V, V, V,
-                                      C.POINTER(Stats), C.c_int]
-        L.vxo_secondary.restype = C.c_int
-        L.vxo_secondary.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, C.POINTER(Spec), V, V,
-                                    C.POINTER(Stats), C.c_int]
-        L.vxo_primary_shadow.restype = C.c_int
-        L.vxo_primary_shadow.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, C.POINTER(Spec), V, V,
-                                         C.POINTER(Stats), C.c_int]
-        L.vxo_decode.restype = C.c_int
-        L.vxo_decode.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, V, C.c_int]
-        L.vxo_ray_dir.restype = None
-        L.vxo_ray_dir.argtypes = [V, C.c_float, C.c_float, V]
-        L.vxo_depth_key.restype = C.c_uint32
-        L.vxo_depth_key.argtypes = [V, C.c_float]
-        L.vxo_make_key.restype = C.c_uint64
-        L.vxo_make_key.argtypes = [V, C.c_float, C.c_uint32, C.c_uint32, V, V]
-        L.vxo_hash_u32.restype = C.c_uint32
-        L.vxo_hash_u32.argtypes = [C.c_uint32]
-        L.vxo_f32_to_f16.restype = C.c_uint16
-        L.vxo_f32_to_f16.argtypes = [C.c_float]
-        L.vxo_block_at.restype = C.c_uint32
-        L.vxo_block_at.argtypes = [C.POINTER(Pools), V, C.c_uint32, V]
-        L.vxo_march_chunk.restype = None
-        L.vxo_march_chunk.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, C.POINTER(Spec), C.c_uint32,
-                                      V, V, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(Hit)]
-        L.vxo_shadow_ray_t.restype = C.c_float
-        L.vxo_shadow_ray_t.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, C.POINTER(Spec), V, V,
-                                       C.c_float]
-        L.vxo_hardware_threads.restype = C.c_int
-        _lib = L
+        _lib = _bind(C.CDLL(LIB))
     return _lib
+
+
+@contextlib.contextmanager
+def planted(n: int, tmpdir: str):
+    """Every oracle call inside the block runs a copy compiled with planted mistake n
+    (VXO_PLANT in vxoracle.c), built into tmpdir: for tests showing that a check rejects
+    a wrong restatement."""
+    global _lib
+    out = os.path.join(str(tmpdir), f"libvxoracle_plant{n}.so")
+    subprocess.run(["make", "-C", HERE, "-s", "-B", f"PLANT={n}", f"OUT={out}"], check=True)
+    saved = lib()
+    _lib = _bind(C.CDLL(out))
+    try:
+        yield _lib
+    finally:
+        _lib = saved
 
 
 def _p(a):

@@ -53,6 +53,20 @@
 #define MICRO_SHIFT 6u
 #define VOXEL_MASK 63u
 
+/* Planted mistakes, for the tests that show their float64 references can tell a wrong
+ * restatement from a right one (tests/test_oracle_cutout_water_pins.py compiles copies
+ * of this file with -DVXO_PLANT=n into a temporary directory). 0, the only value any
+ * shipped build uses, plants nothing.
+ *   1: leaf_solid compares with <= instead of <
+ *   2: leaf_solid hashes the voxel without its micro offset scaled (q = wv + m)
+ *   3: micro_entry snaps to the far micro layer instead of the entry one
+ *   4: cutout_march steps to the near bound of the micro cell instead of the far one
+ *   5: the water-interface probe looks inward (probe = boundary - outward * off)
+ *   6: the water-interface probe is dropped (a water voxel always stops the ray) */
+#ifndef VXO_PLANT
+#define VXO_PLANT 0
+#endif
+
 /* ---- per-thread context: pools + counters ---- */
 typedef struct ctx {
     const vxo_pools *p;
@@ -223,9 +237,17 @@ uint32_t vxo_hash_u32(uint32_t x) {
 /* leaf_solid (common.wgsl:514-518) */
 static int leaf_solid(const vxo_spec *s, const int32_t wv[3], const uint32_t m[3]) {
     uint32_t q[3];
+#if VXO_PLANT == 2
+    for (int i = 0; i < 3; i++) q[i] = (uint32_t)wv[i] + m[i];
+#else
     for (int i = 0; i < 3; i++) q[i] = (uint32_t)wv[i] * MICRO + m[i];
+#endif
     uint32_t h = vxo_hash_u32((q[0] * 73856093u) ^ (q[1] * 19349663u) ^ (q[2] * 83492791u));
+#if VXO_PLANT == 1
+    return (float)(h >> 8u) * (1.0f / 16777216.0f) <= s->leaf_fill;
+#else
     return (float)(h >> 8u) * (1.0f / 16777216.0f) < s->leaf_fill;
+#endif
 }
 
 static int32_t clampi(int32_t v, int32_t lo, int32_t hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -238,7 +260,11 @@ static void micro_entry(const uint32_t uc[3], const float ro[3], const float rd[
         float p = (ro[i] + rd[i] * t - (float)uc[i]) * (float)MICRO;
         m[i] = clampi((int32_t)floorf(p), 0, (int32_t)MICRO - 1);
     }
+#if VXO_PLANT == 3
+    if (snap) m[axis] = pos_dir[axis] ? (int32_t)MICRO - 1 : 0;
+#else
     if (snap) m[axis] = pos_dir[axis] ? 0 : (int32_t)MICRO - 1;
+#endif
     for (int i = 0; i < 3; i++) out[i] = (uint32_t)m[i];
 }
 
@@ -272,7 +298,11 @@ static micro_hit cutout_march(const vxo_spec *s, const int32_t wv[3], const uint
         }
         float tn[3];
         for (int k = 0; k < 3; k++) {
+#if VXO_PLANT == 4
+            float bound = (float)(pos_dir[k] ? m[k] : m[k] + 1);
+#else
             float bound = (float)(pos_dir[k] ? m[k] + 1 : m[k]);
+#endif
             tn[k] = (base[k] + bound * ms - ro[k]) * inv[k];
         }
         if (tn[0] <= tn[1] && tn[0] <= tn[2]) { t = tn[0]; axis = 0; }
@@ -507,8 +537,16 @@ static void march_chunk(ctx *c, uint32_t ci, const float ro_w[3], const float rd
                 float tw = t * vs;
                 float off = 0.002f * vs;
                 for (int i = 0; i < 3; i++) boundary[i] = ro_w[i] + rd[i] * tw;
+#if VXO_PLANT == 5
+                for (int i = 0; i < 3; i++) probe[i] = boundary[i] - outward[i] * off;
+#else
                 for (int i = 0; i < 3; i++) probe[i] = boundary[i] + outward[i] * off;
+#endif
+#if VXO_PLANT != 6
                 if (world_block_at(c, probe) == WATER_ID) { solid = 0; skip = 1; }
+#else
+                (void)probe;
+#endif
             }
         }
         if (solid && water_aware) {

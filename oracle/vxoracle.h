@@ -17,10 +17,14 @@
  * Also pinned on whole frames: water surfaces, glass and glowstone hits from above the
  * water, rays under water skipping water (with the water-line skip), and sun rays
  * passing water, glass and (emitter worlds) glowstone.
- * Parity unpinned (restated, checked only against the GPU): the water-interface probe
- * (a water voxel entered from a water cell, which rays from air never meet first) and
- * leaf cutouts at fills strictly between 0 and 1 (the hash threshold). Foliage tufts and
- * the Snell bend are not restated; the library refuses them.
+ * Leaf cutouts at the fills the flags give (0.62, 0.72): a float64 DDA at micro
+ * resolution with an integer form of the hash threshold, on whole frames (keys, micro
+ * cells, sun bits), on random rays into a chunk at negative coordinates, and on micro
+ * cells whose hash lies exactly on the threshold (found through the hash's inverse). The
+ * water-interface probe: rays crossing from a neighbouring chunk's water, against a
+ * float64 DDA whose stop rule is "water only when entered from a non-water cell".
+ * Planted mistakes (VXO_PLANT in vxoracle.c) in either are caught by those checks.
+ * Foliage tufts and the Snell bend are not restated; the library refuses them.
  *
  * Every struct here is an independent restatement of the reference layouts:
  *   vxo_frame  <- GpuFrame  (src/render/mod.rs:398-466, 352 bytes)
