@@ -237,6 +237,32 @@ int vx_march_stats_compute(const uint32_t *heat, size_t n, uint32_t chunks, vx_m
 /* Device pointers of the current outputs (for callers that keep data on the GPU). */
 int vx_device_outputs(vx_ctx *ctx, uint64_t **vis, uint16_t **shadow);
 
+/* ---- Rendering without allocation ---------------------------------------------------
+ * No counterpart in the reference. For callers that cannot allocate device memory at
+ * call time (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): the tile march of vx_render (VX_PATH_TILES, tiles launched in raster
+ * order) on device buffers the caller holds, on the calling thread's current HIP device
+ * and the given stream (a hipStream_t, or NULL for the default stream). Inputs: the
+ * frame (host memory, copied into the launch), the three pools, the chunk list and the
+ * grid in the reference layouts; hiz (tiles words) is read as the previous frame's Hi-Z
+ * and overwritten with this frame's; vis (W*H keys) and shadow (W*H R16F words) are
+ * written whole. The output bits are vx_render's for the same inputs. The working memory
+ * (counters, launch order, tile costs) is a fixed device array inside the library, one
+ * per GPU, held until the process ends: at most VX_NOALLOC_MAX_TILES tiles per call
+ * (VX_E_OVERFLOW beyond). The call allocates nothing and returns without waiting. Calls
+ * that overlap on one GPU share that array and race: serialise them (one stream, or
+ * stream order). FLAG_HEATMAP frames return VX_E_UNSUPPORTED. */
+#define VX_NOALLOC_MAX_TILES (1u << 17)  /* 131072 tiles: 3840x2160 is 129600 */
+typedef struct vx_device_view {
+    const void *inners; uint64_t inners_bytes;  /* uint4 Inner nodes */
+    const void *leaves; uint64_t leaves_bytes;  /* uint2 leaf masks */
+    const void *bricks; uint64_t bricks_bytes;  /* u32 attribute words */
+    const vx_gpu_chunk *chunks; uint32_t n_chunks;
+    const uint32_t *grid;                       /* VX_GRID_LEN words */
+} vx_device_view;
+int vx_render_noalloc(const vx_gpu_frame *frame, uint32_t spec, uint32_t spec_hi, const vx_device_view *in,
+                      uint32_t *hiz, uint64_t *vis, uint16_t *shadow, void *stream);
+
 /* ---- View batches -----------------------------------------------------------------
  * No counterpart in the reference, which renders one view per Renderer::render call and
  * waits on it (src/headless.rs:680-837, :419-497). A batch renders many views of the

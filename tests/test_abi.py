@@ -121,3 +121,23 @@ def test_host_struct_sizes_match_the_c_header(tmp_path):
         assert got[n] == C.sizeof(cls), (n, got[n], C.sizeof(cls))
     for n, dt in names_np.items():
         assert got[n] == dt.itemsize, (n, got[n], dt.itemsize)
+
+
+def test_noalloc_checks_its_arguments_and_tile_limit_on_the_host():
+    """vx_render_noalloc refuses missing buffers and frames beyond VX_NOALLOC_MAX_TILES
+    before it touches a device (the pointers below are never dereferenced)."""
+    L = A.march()
+    assert C.sizeof(A.DeviceView) == 72 and A.DeviceView.grid.offset == 64
+    dv = A.DeviceView(inners=8, inners_bytes=16, leaves=8, leaves_bytes=8, bricks=8, bricks_bytes=4,
+                      chunks=8, n_chunks=1, grid=8)
+    f = A.Frame()
+    f.res[0], f.res[1], f.tiles[0], f.tiles[1] = 4096, 4096, 512, 512
+    f.grid_dim[0], f.grid_dim[1], f.grid_dim[2] = 32, 8, 32
+    buf = C.c_void_p(8)
+    assert 512 * 512 > A.NOALLOC_MAX_TILES
+    assert L.vx_render_noalloc(C.byref(f), 0, 0, C.byref(dv), buf, buf, buf, None) == -4  # VX_E_OVERFLOW
+    assert L.vx_render_noalloc(C.byref(f), 0, 0, C.byref(dv), None, buf, buf, None) == -3
+    assert L.vx_render_noalloc(C.byref(f), 0, 0, None, buf, buf, buf, None) == -3
+    f.flags = A.FLAG_HEATMAP
+    f.res[0], f.res[1], f.tiles[0], f.tiles[1] = 64, 64, 8, 8
+    assert L.vx_render_noalloc(C.byref(f), 0, 0, C.byref(dv), buf, buf, buf, None) == -5

@@ -404,6 +404,7 @@ _sig_march = {
     "vx_dev_free": (C.c_int, [P, P]),
     "vx_dev_upload": (C.c_int, [P, P, P, C.c_size_t]),
     "vx_dev_download": (C.c_int, [P, P, P, C.c_size_t]),
+    "vx_render_noalloc": (C.c_int, [P, C.c_uint32, C.c_uint32, P, P, P, P, P]),
 }
 
 _host = None
@@ -470,6 +471,17 @@ def march_stats_of(heat: np.ndarray, chunks: int = 0) -> dict:
           "vx_march_stats_compute")
     return march_stats_dict(m)
 
+
+class DeviceView(C.Structure):
+    """vx_device_view: device-resident inputs of vx_render_noalloc."""
+    _fields_ = [("inners", C.c_void_p), ("inners_bytes", C.c_uint64),
+                ("leaves", C.c_void_p), ("leaves_bytes", C.c_uint64),
+                ("bricks", C.c_void_p), ("bricks_bytes", C.c_uint64),
+                ("chunks", C.c_void_p), ("n_chunks", C.c_uint32),
+                ("grid", C.c_void_p)]
+
+
+NOALLOC_MAX_TILES = 1 << 17
 
 # vx_sec_hit: one secondary leg (t, voxel, info = ci | normal << 13 | traced << 30 | hit << 31, block id)
 SEC_DTYPE = np.dtype([("t", "<f4"), ("voxel", "<u4"), ("info", "<u4"), ("material", "<u4")])

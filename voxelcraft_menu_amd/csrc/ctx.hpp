@@ -45,6 +45,9 @@ void launch_tiles(uint32_t occ, uint32_t var, bool hz, bool uw, unsigned nb, hip
 // Cost-bucket launch order of n tiles into `order` (entries are tile indices); `ctl`
 // is the CTL_LEN-word control block, cleared beforehand.
 void launch_order(hipStream_t st, const uint32_t *cost, uint32_t n, uint32_t *ctl, uint32_t *order);
+// Looks up the device address of vx_render_noalloc's working array on `device` (which
+// must be current) and keeps it for later calls.
+int noalloc_setup(int device);
 
 }  // namespace vxh
 

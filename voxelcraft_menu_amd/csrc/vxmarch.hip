@@ -192,6 +192,7 @@ int vx_ctx_create(const vx_ctx_desc *d, vx_ctx **out) {
             c->device = d->device;
         } else if (hipGetDevice(&c->device) != hipSuccess) { rc = VX_E_HIP; break; }
         if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { rc = VX_E_HIP; break; }
+        if ((rc = vxh::noalloc_setup(c->device))) break;
         for (auto &set : c->ring)
             for (auto &e : set)
                 if (rc == VX_OK && hipEventCreate(&e) != hipSuccess) rc = VX_E_HIP;
