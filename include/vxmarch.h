@@ -288,7 +288,9 @@ typedef struct vx_batch_tuning {
     uint32_t occupancy;       /* waves per SIMD the march is built for: 4 or 6 */
     uint32_t variant;         /* vx_tuning.variant bits; batched forms need the fused shadow (4 or 5); they may add
                                  bit 3 (8): classify only the chunks on the candidate list of the tile's 64x64-pixel
-                                 screen bin (built with the view set), not the view's whole list */
+                                 screen bin (built at the first render of the view set in such a form; a set
+                                 whose lists would exceed 1 GiB, or VX_BIN_LIST_MAX_BYTES from the environment,
+                                 keeps whole lists), not the view's whole list */
     uint32_t ordered;         /* 1: tiles most expensive first (their cost in the previous frame) */
     uint32_t queue_chunk;     /* PERSISTENT: tiles taken per queue grab, 1..64 */
     uint32_t waves_per_simd;  /* PERSISTENT: grid = CUs * 4 * waves_per_simd, 1..occupancy */
@@ -390,7 +392,8 @@ typedef struct vx_serve_desc {
     uint32_t max_views;        /* views per batch, 1..1024 */
     uint64_t hbm_budget;       /* device bytes for in-flight request state: outputs
                                   (10 B/pixel) + inputs (chunk list, grid, tile words) */
-    vx_batch_tuning tuning;    /* launch form of every batch (FLAT or PERSISTENT) */
+    vx_batch_tuning tuning;    /* launch form of every batch (FLAT or PERSISTENT; variant 4 or 5,
+                                  without the screen-bin bit 8: VX_E_ARG) */
 } vx_serve_desc;
 
 typedef struct vx_serve_result {

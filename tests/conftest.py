@@ -21,6 +21,7 @@ def _ensure_built():
         B.build_host()
     if not os.path.exists(_abi.MARCH_LIB):
         B.build_march()
+    # a library that does not match its sources is refused at load (_abi.check_fresh)
     oracle_lib = os.path.join(ROOT, "oracle", "libvxoracle.so")
     if not os.path.exists(oracle_lib):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "-s"], check=True)

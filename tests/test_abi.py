@@ -141,3 +141,12 @@ def test_noalloc_checks_its_arguments_and_tile_limit_on_the_host():
     f.flags = A.FLAG_HEATMAP
     f.res[0], f.res[1], f.tiles[0], f.tiles[1] = 64, 64, 8, 8
     assert L.vx_render_noalloc(C.byref(f), 0, 0, C.byref(dv), buf, buf, buf, None) == -5
+
+
+def test_a_library_built_from_other_sources_is_refused(monkeypatch):
+    from voxelcraft_menu_amd import build as B
+    A.check_fresh(A.MARCH_LIB, "march")
+    A.check_fresh(A.HOST_LIB, "host")
+    monkeypatch.setattr(B, "source_digest", lambda kind: "0" * 64)
+    with pytest.raises(RuntimeError, match="does not match its sources"):
+        A.check_fresh(A.MARCH_LIB, "march")

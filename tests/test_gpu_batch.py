@@ -183,7 +183,8 @@ def test_config_batch_64_views_all_forms_identical():
             for j, name in enumerate(("vis", "shadow", "hiz")):
                 assert np.array_equal(results[k][i][j], results[keys[0]][i][j]), f"{k} view {i} {name}"
     small = [i for i, v in enumerate(views) if int(v.frame.res[0]) <= 640][:4]
-    for i in small:
+    large = [next(i for i, v in enumerate(views) if int(v.frame.res[0]) == w) for w in (1280, 1920)]
+    for i in small + large:
         s, hi = specs[i]
         ref = H.oracle_frames(world, views[i], s, hi, frames=2)
         assert np.array_equal(results[keys[0]][i][0], ref[1][0]), f"view {i} vs oracle"
@@ -330,4 +331,63 @@ def test_secondary_refuses_wavy_water(small_world):
     with pytest.raises(RuntimeError):
         b.set_secondary(True)
     b.close()
+    r.close()
+
+
+def test_screen_bins_follow_view_set_changes(small_world, monkeypatch):
+    """The bin lists are rebuilt for each view set: a second set with the same class
+    ranges replays the captured graph over rebuilt lists; a set whose lists exceed the
+    size limit (lowered here through VX_BIN_LIST_MAX_BYTES) drops to whole lists and the
+    graph with it; a set from cameras (lists built on the device) uses bins too. Every
+    frame equals the whole-list form (variant 5) on the same set."""
+    from voxelcraft_menu_amd.world import ViewSnapshot
+
+    wl = W.WORKLOADS[0]
+    sets = [W.random_cameras(wl, 6, seed=s, resolutions=[(200, 120), (96, 64)]) for s in (301, 302)]
+    for c, a in zip(sets[1], sets[0]):
+        c.width, c.height = a.width, a.height  # same sizes: the same FLAT class ranges
+    views = [[build_view(small_world, c) for c in cams] for cams in sets]
+    specs = [[W.spec_words(c.flags, small_world) for c in cams] for cams in sets]
+    r = _renderer(small_world, views[0])
+    ref_b = Batch(r, 8)
+    ref_b.set_tuning(mode=A.BATCH_FLAT, graph=0, occupancy=6, variant=5)
+    b = Batch(r, 8)
+    b.set_tuning(mode=A.BATCH_FLAT, graph=1, occupancy=6, variant=13)
+
+    def frames(batch, k):
+        batch.reset_hiz()
+        batch.render()
+        batch.render()
+        batch.sync()
+        return [(batch.read_vis(i), batch.read_shadow(i), batch.read_hiz(i)) for i in range(len(views[k]))], \
+            batch.demand()
+
+    refs = []
+    for k in range(2):
+        ref_b.set_views(views[k], specs[k])
+        refs.append(frames(ref_b, k))
+
+    def check(k, tag):
+        got = frames(b, k)
+        assert got[1] == refs[k][1], f"{tag}: demand {got[1]} != {refs[k][1]}"
+        for i in range(len(views[k])):
+            for j, name in enumerate(("vis", "shadow", "hiz")):
+                assert np.array_equal(got[0][i][j], refs[k][0][i][j]), f"{tag}: view {i} {name}"
+
+    b.set_views(views[0], specs[0])
+    check(0, "first set")
+    b.set_views(views[1], specs[1])
+    check(1, "second set, graph kept")
+    monkeypatch.setenv("VX_BIN_LIST_MAX_BYTES", "64")
+    b.set_views(views[0], specs[0])
+    check(0, "lists over the limit: whole lists")
+    monkeypatch.delenv("VX_BIN_LIST_MAX_BYTES")
+    b.set_views(views[1], specs[1])
+    check(1, "bins again")
+    snap = ViewSnapshot(small_world)
+    b.set_chunk_table(snap.chunk_table())
+    b.set_cameras(sets[0], specs[0])
+    check(0, "device-built lists with bins")
+    b.close()
+    ref_b.close()
     r.close()

@@ -198,10 +198,10 @@ def test_camera_requests_equal_host_list_requests(small_world):
             srv.release(rid)
 
     run_round("before edits")
+    srv.barrier()
+    s, hi = W.spec_words(cams[0].flags, small_world)
+    srv.submit_camera(900, cams[0], s, hi)
     with pytest.raises(RuntimeError):  # a camera request pending: the table cannot change
-        srv.barrier()
-        s, hi = W.spec_words(cams[0].flags, small_world)
-        srv.submit_camera(900, cams[0], s, hi)
         srv.set_chunk_table(snap.chunk_table())
     srv.resume()
     _drain(srv, [900])

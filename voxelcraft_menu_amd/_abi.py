@@ -419,22 +419,36 @@ def _bind(lib, sigs):
     return lib
 
 
+def check_fresh(lib: str, kind: str):
+    """Refuses a library that is missing or was built from other sources than the ones
+    beside it (the digest the build wrote next to it differs from the current one)."""
+    from . import build as B
+    if not os.path.exists(lib):
+        raise RuntimeError(f"{lib} is not built; run voxelcraft_menu_amd.build.build()")
+    try:
+        with open(lib + ".srchash") as fh:
+            stamp = fh.read().strip()
+    except OSError:
+        stamp = ""
+    if stamp != B.source_digest(kind):
+        raise RuntimeError(f"{lib} does not match its sources (stale or foreign build); "
+                           f"rebuild it with voxelcraft_menu_amd.build.build()")
+
+
 def host():
     """The host world-store library (CPU only)."""
     global _host
     if _host is None:
-        if not os.path.exists(HOST_LIB):
-            raise RuntimeError(f"{HOST_LIB} is not built; run voxelcraft_menu_amd.build.build()")
+        check_fresh(HOST_LIB, "host")
         _host = _bind(C.CDLL(HOST_LIB), _sig_host)
     return _host
 
 
 def march():
-    """The HIP ray-march library. Raises if it is not built: there is no CPU fallback."""
+    """The HIP ray-march library. Raises if it is not built or stale: there is no CPU fallback."""
     global _march
     if _march is None:
-        if not os.path.exists(MARCH_LIB):
-            raise RuntimeError(f"{MARCH_LIB} is not built; run voxelcraft_menu_amd.build.build()")
+        check_fresh(MARCH_LIB, "march")
         _march = _bind(C.CDLL(MARCH_LIB), _sig_march)
     return _march
 

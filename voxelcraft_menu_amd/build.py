@@ -32,6 +32,33 @@ HIPCC_FLAGS = [
 ]
 
 
+def source_digest(kind: str) -> str:
+    """Digest of what a library is built from: its sources, every header they can include
+    and the compile flags. The build writes it beside the library (<lib>.srchash); the
+    loader refuses a library whose digest differs from the current sources."""
+    import glob
+    import hashlib
+
+    if kind == "host":
+        files = [os.path.join(CSRC, s) for s in HOST_SOURCES] + glob.glob(os.path.join(CSRC, "host", "*.hpp"))
+        flags = "g++ -std=c++17 -O2"
+    else:
+        files = [os.path.join(CSRC, s) for s in HIP_SOURCES + HIP_EXTRA] + glob.glob(os.path.join(CSRC, "*.hpp"))
+        flags = " ".join(HIPCC_FLAGS)
+    files += glob.glob(os.path.join(INC, "*.h"))
+    h = hashlib.sha256(flags.encode())
+    for f in sorted(set(files)):
+        h.update(os.path.relpath(f, ROOT).encode())
+        with open(f, "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
+
+
+def _stamp(lib: str, kind: str):
+    with open(lib + ".srchash", "w") as fh:
+        fh.write(source_digest(kind) + "\n")
+
+
 def _hipcc() -> str:
     for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
         if c and os.path.exists(c):
@@ -54,6 +81,7 @@ def build_host(verbose: bool = False) -> str:
     cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-Wextra", f"-I{INC}"]
     cmd += [os.path.join(CSRC, s) for s in HOST_SOURCES] + ["-lpthread", "-o", out]
     _run(cmd, verbose)
+    _stamp(out, "host")
     return out
 
 
@@ -64,6 +92,7 @@ def build_march(verbose: bool = False) -> str:
     cmd = [_hipcc()] + HIPCC_FLAGS + [f"-I{INC}", f"-I{CSRC}"] + srcs + ["-pthread", "-o", tmp]
     _run(cmd, verbose)
     os.replace(tmp, out)
+    _stamp(out, "march")
     return out
 
 

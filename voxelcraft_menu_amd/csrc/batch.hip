@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -51,9 +52,12 @@ struct vx_batch {
     bool sec_on = false;
     DevBuf sec_out, sec_queue;
     DevBuf table, cams, list_counts;  // device chunk lists: chunk table, per-view cameras, counts
-    DevBuf bin_cnt, bin_list;         // screen-bin candidate lists (k_bin_chunks), when bins_ok
+    // Screen-bin candidate lists (k_bin_chunks) of the installed view set: bins_fit when the
+    // set's lists fit bin_list_max_bytes(); built by the first render of a form that uses them.
+    DevBuf bin_cnt, bin_list;
     uint32_t n_bins = 0;
-    bool bins_ok = false;
+    uint64_t bin_entries = 0;
+    bool bins_fit = false, bins_built = false;
     uint32_t n_table = 0;
     uint32_t sec_cap = 0;
     uint64_t vk_pool_gen = 0;  // pool generation the device descriptors' pool fields were written for
@@ -65,6 +69,7 @@ struct vx_batch {
     uint32_t graph_cb[N_CLASSES + 1] = {0, 0, 0, 0, 0};
     uint64_t graph_pool_gen = 0;
     bool graph_valid = false;
+    bool graph_bins = false;  // the captured launches classify by the bin lists
     static constexpr int RING = 256;
     hipEvent_t ring[RING][4] = {};  // [0] call, [1] submission start, [2] march kernel start, [3] end
     uint32_t ring_next = 0, ring_count = 0;
@@ -74,7 +79,15 @@ struct vx_batch {
 namespace {
 
 // Largest screen-bin list allocation; a view set that needs more keeps whole lists.
+// VX_BIN_LIST_MAX_BYTES in the environment lowers it (read when a view set is installed).
 constexpr uint64_t BIN_LIST_MAX_BYTES = 1ull << 30;
+uint64_t bin_list_max_bytes() {
+    if (const char *e = std::getenv("VX_BIN_LIST_MAX_BYTES")) {
+        const unsigned long long v = std::strtoull(e, nullptr, 10);
+        if (v < BIN_LIST_MAX_BYTES) return v;
+    }
+    return BIN_LIST_MAX_BYTES;
+}
 
 // Per-view control blocks of the PLAIN form follow the shared counters.
 size_t counter_words(uint32_t n_views) { return (size_t)CNT_TOTAL + (size_t)n_views * CTL_LEN; }
@@ -116,6 +129,10 @@ int refresh_view_pools(vx_batch *b, hipStream_t st) {
     return VX_OK;
 }
 
+// The tuning asks for bin lists (a batched form, not PLAIN, with VAR_BINS).
+bool wants_bins(const vx_batch *b) { return (b->tn.variant & VAR_BINS) && b->tn.mode != VX_BATCH_PLAIN; }
+bool uses_bins(const vx_batch *b) { return wants_bins(b) && b->bins_fit && b->bins_built; }
+
 BatchArgs batch_args(const vx_batch *b) {
     BatchArgs a;
     a.p = vxh::make_pools(b->ctx);
@@ -130,7 +147,7 @@ BatchArgs batch_args(const vx_batch *b) {
     a.counters = (uint32_t *)b->counters.p;
     a.order = (const uint32_t *)b->order.p;
     a.cost = (uint32_t *)b->cost.p;
-    if ((b->tn.variant & VAR_BINS) && b->bins_ok && b->tn.mode != VX_BATCH_PLAIN) {
+    if (uses_bins(b)) {
         a.bin_cnt = (const uint32_t *)b->bin_cnt.p;
         a.bin_list = (const uint16_t *)b->bin_list.p;
     }
@@ -245,6 +262,26 @@ int enqueue(vx_batch *b, hipStream_t st, hipEvent_t ev_march) {
     return VX_OK;
 }
 
+// Builds the installed view set's screen-bin lists, the first time a form that classifies
+// by bins renders the set. If their buffers cannot be had, the set keeps whole lists (the
+// same bits). A captured graph whose bin pointers no longer hold is dropped.
+int build_bins(vx_batch *b, hipStream_t st) {
+    if (!wants_bins(b) || !b->bins_fit || b->bins_built) return VX_OK;
+    const void *c0 = b->bin_cnt.p, *l0 = b->bin_list.p;
+    if (ensure(b->bin_cnt, (size_t)b->n_bins * 4) ||
+        ensure(b->bin_list, (size_t)std::max<uint64_t>(b->bin_entries, 1) * 2)) {
+        (void)hipGetLastError();
+        b->bins_fit = false;
+        return VX_OK;
+    }
+    if (c0 != b->bin_cnt.p || l0 != b->bin_list.p) drop_graph(b);
+    k_bin_chunks<<<(unsigned)b->n_cu * 4u, 256, 0, st>>>(batch_args(b), (uint16_t *)b->bin_list.p,
+                                                         (uint32_t *)b->bin_cnt.p);
+    LAUNCHED();
+    b->bins_built = true;
+    return VX_OK;
+}
+
 int capture(vx_batch *b) {
     drop_graph(b);
     hipStream_t st = b->ctx->stream;
@@ -266,6 +303,7 @@ int capture(vx_batch *b) {
     b->graph_tiles = b->n_tiles;
     std::memcpy(b->graph_cb, b->cb, sizeof b->cb);
     b->graph_pool_gen = b->ctx->pool_gen;
+    b->graph_bins = uses_bins(b);
     b->graph_valid = true;
     return VX_OK;
 }
@@ -501,36 +539,22 @@ int install_views(vx_batch *b, uint32_t n, ViewPlan &pl, bool moved) {
     const std::vector<uint64_t> &poff = pl.poff;
     const uint32_t *cb = pl.cb;
     // Screen-bin lists go with the view set: they depend only on the views' frames and
-    // chunk lists. A set whose lists would not fit keeps whole lists (bins off).
-    const bool bins_ok = pl.bins > 0 && pl.bins <= 0xFFFFFFFFull && pl.bin_entries <= 0xFFFFFFFFull &&
-                         pl.bin_entries * 2 <= BIN_LIST_MAX_BYTES;
-    if (bins_ok) {
-        const void *c0 = b->bin_cnt.p, *l0 = b->bin_list.p;
-        if ((rc = ensure(b->bin_cnt, (size_t)pl.bins * 4)) ||
-            (rc = ensure(b->bin_list, (size_t)std::max<uint64_t>(pl.bin_entries, 1) * 2))) {
-            drop_graph(b);
-            b->n_views = 0;
-            b->bins_ok = false;
-            return rc;
-        }
-        moved |= c0 != b->bin_cnt.p || l0 != b->bin_list.p;
-    }
-    moved |= bins_ok != b->bins_ok;  // a captured graph holds the bin pointers or not
+    // chunk lists. A set whose lists would not fit keeps whole lists (bins off). They are
+    // built by the first render of a form that classifies by bins (build_bins).
+    const bool fit = pl.bins > 0 && pl.bins <= 0xFFFFFFFFull && pl.bin_entries <= 0xFFFFFFFFull &&
+                     pl.bin_entries * 2 <= bin_list_max_bytes();
     uint32_t hdr[BH_LEN] = {};
     hdr[BH_TILES] = toff[n];
     hdr[BH_VIEWS] = n;
-    hdr[BH_BINS] = bins_ok ? (uint32_t)pl.bins : 0u;
+    hdr[BH_BINS] = fit ? (uint32_t)pl.bins : 0u;
     for (uint32_t k = 0; k <= N_CLASSES; k++) hdr[BH_CB + k] = cb[k];
     HIPX(hipMemcpyAsync(b->views.p, pl.vk.data(), (size_t)n * sizeof(ViewK), hipMemcpyHostToDevice, st));
     HIPX(hipMemcpyAsync(b->tile_off.p, toff.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice, st));
     HIPX(hipMemcpyAsync(b->hdr.p, hdr, sizeof hdr, hipMemcpyHostToDevice, st));
-    if (bins_ok) {
-        k_bin_chunks<<<(unsigned)b->n_cu * 4u, 256, 0, st>>>(batch_args(b), (uint16_t *)b->bin_list.p,
-                                                             (uint32_t *)b->bin_cnt.p);
-        LAUNCHED();
-    }
-    b->bins_ok = bins_ok;
-    b->n_bins = bins_ok ? (uint32_t)pl.bins : 0u;
+    b->bins_fit = fit;
+    b->bins_built = false;
+    b->n_bins = fit ? (uint32_t)pl.bins : 0u;
+    b->bin_entries = fit ? pl.bin_entries : 0u;
     // New views start as a new Renderer would: empty targets, Hi-Z 0; costs 0.
     if ((rc = vxh::fill_u64(st, b->vis.p, 0, poff[n])) || (rc = vxh::fill_u32(st, b->shadow.p, 0, (poff[n] + 1) / 2)) ||
         (rc = vxh::fill_u32(st, b->hiz.p, 0, toff[n])) || (rc = vxh::fill_u32(st, b->cost.p, 0, toff[n])))
@@ -779,6 +803,8 @@ int vx_batch_render(vx_batch *b) {
     HIPX(hipEventRecord(ev[0], st));
     int rc = refresh_view_pools(b, st);
     if (rc) return rc;
+    if ((rc = build_bins(b, st))) return rc;
+    if (b->graph_valid && b->graph_bins != uses_bins(b)) drop_graph(b);
     if (b->tn.graph) {
         if ((!b->graph_valid || b->graph_pool_gen != c->pool_gen) && (rc = capture(b))) return rc;
         HIPX(hipEventRecord(ev[1], st));

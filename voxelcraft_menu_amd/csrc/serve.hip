@@ -378,7 +378,8 @@ int vx_serve_create(vx_ctx *c, const vx_serve_desc *d, vx_serve **out) {
     if (d->lanes < 1 || d->lanes > 8 || d->max_views < 1 || d->max_views > BATCH_MAX_VIEWS) return VX_E_ARG;
     if (d->hbm_budget < ((uint64_t)1 << 20)) return VX_E_ARG;
     const vx_batch_tuning &t = d->tuning;
-    if (!batch_form_ok(t.mode, t.occupancy, t.variant)) return VX_E_ARG;
+    // Screen bins (VAR_BINS) need per-set candidate lists the server does not build.
+    if ((t.variant & VAR_BINS) || !batch_form_ok(t.mode, t.occupancy, t.variant)) return VX_E_ARG;
     if (t.mode == VX_BATCH_PERSISTENT &&
         (t.queue_chunk < 1 || t.queue_chunk > 64 || t.waves_per_simd < 1 || t.waves_per_simd > t.occupancy))
         return VX_E_ARG;
