@@ -37,6 +37,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# MI355X published FP32 vector peak, 157.3 TFLOP/s: 78.65 T lane FMAs/s, i.e. two wave64
+# VALU instructions per 4 cycles per SIMD (1024 SIMDs at 2.4 GHz, dual issue).
+VALU_PEAK_TLANEOPS = 157.3 / 2.0
 METRIC = "Grays/s primary+shadow, batched vs per-view plain; p50/p99 view latency at QPS"
 
 
@@ -49,19 +52,45 @@ def _log(msg: str):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def traffic_from_profiles(kernel: str, workload: str):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
+def profile_figures(kernel: str, workload: str) -> dict:
+    """The `name: value` lines of the committed rocprofv3 --pmc summary of a kernel
+    (per-launch counters: traffic_bytes_per_launch, valu_insts_per_launch, ...), or {}."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_pmc.md")
+    out = {}
     if not os.path.exists(path):
-        return None
+        return out
     with open(path) as fh:
         for line in fh:
-            if line.startswith("traffic_bytes_per_launch:"):
-                try:
-                    return float(line.split(":", 1)[1].strip())
-                except ValueError:
-                    return None
-    return None
+            if ":" in line and not line.startswith((" ", "|", "#")):
+                k, v = line.split(":", 1)
+                if k.replace("_", "").isalnum():
+                    out[k.strip()] = v.strip()
+    return out
+
+
+def traffic_from_profiles(kernel: str, workload: str):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
+    try:
+        return float(profile_figures(kernel, workload)["traffic_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
+def valu_probe():
+    """Measured VALU issue ceiling: wave64 VALU instructions per second of a grid that fills
+    every SIMD with independent 32-bit VALU chains (libvxcalib.so), f32 FMA and integer."""
+    import ctypes as C
+
+    from voxelcraft_menu_amd import _abi as A
+    L = A.calib()
+    blocks, iters = 2048, 40000  # 8192 waves of 64 lanes: eight per SIMD on 256 CUs
+    res = {}
+    for kind, name in ((0, "fma_f32"), (1, "int32")):
+        ms, per_wave = C.c_double(), C.c_double()
+        if L.vxc_valu_probe(kind, blocks, iters, 3, C.byref(ms), C.byref(per_wave)) != 0:
+            return None
+        res[name] = blocks * 4 * per_wave.value / (ms.value * 1e-3) / 1e12  # T wave64 instructions / s
+    return {k: round(v * 64.0, 2) for k, v in res.items()}  # T lane-ops / s
 
 
 def _threads():
@@ -200,6 +229,9 @@ def run_batch(args, out):
         },
     })
 
+    if not args.no_moving:
+        out["moving_camera"] = run_moving(args, world, cams, specs, tuned, px)
+
     # Single 1920x1080 view (the plain-at-scale case), for reference.
     wl1 = W.WORKLOADS[1]
     cam1 = W.default_camera(wl1)
@@ -241,13 +273,91 @@ def run_batch(args, out):
     # accounting of the primary and shadow rays of every view, plus per pixel the 8-byte
     # key and 2-byte R16F shadow word, plus the 4-byte Hi-Z word per tile.
     alg = prim["node_bytes"] + sh["node_bytes"] + 10 * px + 4 * tiles
-    achieved = alg / (march_ms * 1e-3) / 1e9
-    out["roofline"] = {
-        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(kname, "workload2"),
-        "kernel": kname, "kernel_ms": round(march_ms, 4), "alg_bytes_per_launch": int(alg),
-        "alg_bytes_per_pixel": round(alg / px, 2),
+    out["roofline"] = roofline_valu(kname, "workload2", march_ms, alg, px)
+
+
+def roofline_valu(kname: str, workload: str, kernel_ms: float, alg: float, px: int) -> dict:
+    """The march kernel is VALU-issue bound (integer mask logic and f32 stepping, no dense
+    contraction; the ~20 MB tree is served from L2 and the Infinity Cache): its roofline is
+    VALU lane-operations per second against the FP32 vector peak. The VALU instruction
+    count per launch is the committed rocprofv3 SQ_INSTS_VALU figure for this workload,
+    valid only for the library build it was measured on (its source digest is recorded
+    with it); the kernel time is measured here with HIP events."""
+    from voxelcraft_menu_amd import build as B
+
+    prof = profile_figures(kname, workload)
+    kernel_s = kernel_ms * 1e-3
+    valu = float(prof["valu_insts_per_launch"]) if "valu_insts_per_launch" in prof else None
+    same_build = prof.get("library_digest") == B.source_digest("march")
+    achieved = valu * 64.0 / kernel_s / 1e12 if valu and same_build else None
+    traffic = float(prof["traffic_bytes_per_launch"]) if "traffic_bytes_per_launch" in prof else None
+    probe = valu_probe()
+    return {
+        "bound": "valu", "achieved": round(achieved, 2) if achieved else None, "peak": VALU_PEAK_TLANEOPS,
+        "unit": "T VALU lane-ops/s (wave64 VALU instructions x 64)",
+        "frac": round(achieved / VALU_PEAK_TLANEOPS, 4) if achieved else None,
+        "traffic": traffic, "kernel": kname, "kernel_ms": round(kernel_ms, 4),
+        "valu_insts_per_launch": valu, "profile_matches_build": same_build,
+        "peak_source": "MI355X FP32 vector 157.3 TFLOP/s = 78.65 T lane FMAs/s (2 wave64 VALU issues / 4 cycles / SIMD)",
+        "measured_valu_ceiling": probe,
+        "lane_utilization": float(prof["lane_utilization"]) if "lane_utilization" in prof else None,
+        "hbm_GBps": round(traffic / kernel_s / 1e9, 1) if traffic else None,
+        "hbm_frac": round(traffic / kernel_s / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
+        "cache_served_node_bytes_per_launch": int(alg),
+        "cache_served_node_bytes_note": "the oracle's count for the reference's unpruned pair decomposition; "
+                                        "served from L2 / Infinity Cache, not an HBM figure",
+        "cache_served_bytes_per_pixel": round(alg / px, 2),
     }
+
+
+def run_moving(args, world, cams, specs, tuned, px):
+    """The same 64 views with every camera turning by 0.01 rad of yaw per step, as the
+    reference's bench_frames turns its camera each frame (src/headless.rs:718-720). A step
+    rebuilds every view's chunk list and grid on the device for its moved camera
+    (vx_batch_move_cameras: Hi-Z words and tile costs carried from the previous frame,
+    screen bins rebuilt) and renders the batch in the tuned form."""
+    import dataclasses
+
+    from voxelcraft_menu_amd.renderer import Batch, Renderer
+    from voxelcraft_menu_amd.world import ViewSnapshot
+
+    r = Renderer(256, 144)
+    r.sync_world(world)
+    b = Batch(r, len(cams))
+    b.set_chunk_table(ViewSnapshot(world).chunk_table())
+    b.set_tuning(**tuned)
+    b.set_cameras(cams, specs)
+
+    def turned(k):
+        return [dataclasses.replace(c, yaw=c.yaw + 0.01 * k) for c in cams]
+
+    k = 0
+    for _ in range(max(args.warmup, 2)):
+        k += 1
+        b.set_cameras(turned(k), specs, moved=True)
+        b.render()
+    b.sync()
+    rays = rec = dfr = 0
+    b.timings_reset()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        k += 1
+        b.set_cameras(turned(k), specs, moved=True)
+        b.render()
+        d = b.demand()
+        rays += px + d["shadow_rays"]
+        rec += d["recover"]
+        dfr += d["deferred"]
+    dt = time.perf_counter() - t0
+    tot, _, n = b.timings_sum()
+    b.close()
+    r.close()
+    res = {"value": round(rays / dt / 1e9, 4), "unit": "Grays/s", "ms_per_step": round(dt / args.steps * 1e3, 4),
+           "render_ms_per_step": round(tot / max(n, 1), 4), "yaw_per_step_rad": 0.01,
+           "deferred_pairs_per_step": dfr // args.steps, "recovered_pairs_per_step": rec // args.steps,
+           "step": "device chunk lists for the moved cameras + batched render (Hi-Z carried)"}
+    _log(f"moving camera: {res['ms_per_step']} ms/step ({res['render_ms_per_step']} ms render)")
+    return res
 
 
 def run_serving(args):
@@ -527,6 +637,7 @@ def main():
     ap.add_argument("--no-incoherent", action="store_true", help="skip the incoherent batch (workload 2 only)")
     ap.add_argument("--no-build", action="store_true", help="skip the device chunk build (workload 2 only)")
     ap.add_argument("--no-view-setup", action="store_true", help="skip the view-setup comparison (workload 2 only)")
+    ap.add_argument("--no-moving", action="store_true", help="skip the turning-camera steps (workload 2 only)")
     ap.add_argument("--serve-qps", type=float, default=1000.0, help="offered view requests per second")
     ap.add_argument("--serve-seconds", type=float, default=4.0, help="seconds of request arrivals")
     ap.add_argument("--serve-edit-rate", type=float, default=2.0, help="edit batches (1000 ops) per second")

@@ -324,6 +324,12 @@ typedef struct vx_camera_view {
 } vx_camera_view;
 int vx_batch_set_chunk_table(vx_batch *b, const vx_gpu_chunk *chunks, uint32_t n);
 int vx_batch_set_cameras(vx_batch *b, const vx_camera_view *views, uint32_t n);
+/* The same views one frame later, their cameras moved: as vx_batch_set_cameras, but each
+ * view keeps its Hi-Z words and tile costs, as the reference's Renderer keeps its Hi-Z
+ * buffer while the camera moves from frame to frame (src/render/mod.rs:1716-1795, driven
+ * by src/headless.rs:709-733). The views must be the installed ones in count, order, size
+ * and class (Hi-Z on/off, camera under water or not); VX_E_ARG otherwise. */
+int vx_batch_move_cameras(vx_batch *b, const vx_camera_view *views, uint32_t n);
 int vx_batch_view_counts(vx_batch *b, uint32_t view, uint32_t *n_total, uint32_t *n_visible, uint32_t *camera_chunk);
 int vx_batch_read_chunks(vx_batch *b, uint32_t view, vx_gpu_chunk *chunks, size_t cap, uint32_t *grid);
 int vx_batch_set_tuning(vx_batch *b, const vx_batch_tuning *t);

@@ -13,6 +13,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 HOST_LIB = os.path.join(HERE, "libvxhost.so")
 MARCH_LIB = os.path.join(HERE, "libvxmarch.so")
+CALIB_LIB = os.path.join(HERE, "libvxcalib.so")
 
 # ---- constants (include/vxmarch.h; src/render/mod.rs:20-37, :144-270) ----
 VX_OK = 0
@@ -360,6 +361,7 @@ _sig_march = {
     "vx_batch_set_views": (C.c_int, [P, P, C.c_uint32]),
     "vx_batch_set_chunk_table": (C.c_int, [P, P, C.c_uint32]),
     "vx_batch_set_cameras": (C.c_int, [P, P, C.c_uint32]),
+    "vx_batch_move_cameras": (C.c_int, [P, P, C.c_uint32]),
     "vx_batch_view_counts": (C.c_int, [P, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                        C.POINTER(C.c_uint32)]),
     "vx_batch_read_chunks": (C.c_int, [P, C.c_uint32, P, C.c_size_t, P]),
@@ -451,6 +453,22 @@ def march():
         check_fresh(MARCH_LIB, "march")
         _march = _bind(C.CDLL(MARCH_LIB), _sig_march)
     return _march
+
+
+_calib = None
+
+
+def calib():
+    """The VALU issue-rate probe (measurement tooling for the benchmark's roofline)."""
+    global _calib
+    if _calib is None:
+        check_fresh(CALIB_LIB, "calib")
+        L = C.CDLL(CALIB_LIB)
+        L.vxc_valu_probe.restype = C.c_int
+        L.vxc_valu_probe.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double)]
+        _calib = L
+    return _calib
 
 
 def host_symbols():

@@ -42,6 +42,9 @@ def source_digest(kind: str) -> str:
     if kind == "host":
         files = [os.path.join(CSRC, s) for s in HOST_SOURCES] + glob.glob(os.path.join(CSRC, "host", "*.hpp"))
         flags = "g++ -std=c++17 -O2"
+    elif kind == "calib":
+        return hashlib.sha256(" ".join(HIPCC_FLAGS).encode() +
+                              open(os.path.join(CSRC, "calib.hip"), "rb").read()).hexdigest()
     else:
         files = [os.path.join(CSRC, s) for s in HIP_SOURCES + HIP_EXTRA] + glob.glob(os.path.join(CSRC, "*.hpp"))
         flags = " ".join(HIPCC_FLAGS)
@@ -96,8 +99,17 @@ def build_march(verbose: bool = False) -> str:
     return out
 
 
+def build_calib(verbose: bool = False) -> str:
+    """libvxcalib.so: the VALU issue-rate probe the benchmark's roofline is checked with."""
+    out = os.path.join(HERE, "libvxcalib.so")
+    cmd = [_hipcc()] + HIPCC_FLAGS + [os.path.join(CSRC, "calib.hip"), "-o", out]
+    _run(cmd, verbose)
+    _stamp(out, "calib")
+    return out
+
+
 def build(verbose: bool = False):
-    return build_host(verbose), build_march(verbose)
+    return build_host(verbose), build_march(verbose), build_calib(verbose)
 
 
 if __name__ == "__main__":

@@ -530,7 +530,7 @@ int plan_views(const vx_batch *b, uint32_t n, FrameOf frame_of, SpecOf spec_of, 
 
 // The second half: descriptors, offsets and header to the device, targets cleared, the
 // captured graph kept only when it still fits the view set.
-int install_views(vx_batch *b, uint32_t n, ViewPlan &pl, bool moved) {
+int install_views(vx_batch *b, uint32_t n, ViewPlan &pl, bool moved, bool keep_targets = false) {
     vx_ctx *c = b->ctx;
     hipStream_t st = c->stream;
     int rc;
@@ -555,9 +555,11 @@ int install_views(vx_batch *b, uint32_t n, ViewPlan &pl, bool moved) {
     b->bins_built = false;
     b->n_bins = fit ? (uint32_t)pl.bins : 0u;
     b->bin_entries = fit ? pl.bin_entries : 0u;
-    // New views start as a new Renderer would: empty targets, Hi-Z 0; costs 0.
-    if ((rc = vxh::fill_u64(st, b->vis.p, 0, poff[n])) || (rc = vxh::fill_u32(st, b->shadow.p, 0, (poff[n] + 1) / 2)) ||
-        (rc = vxh::fill_u32(st, b->hiz.p, 0, toff[n])) || (rc = vxh::fill_u32(st, b->cost.p, 0, toff[n])))
+    // New views start as a new Renderer would: empty targets, Hi-Z 0; costs 0. Moved
+    // cameras keep them (the same views, one frame later).
+    if (!keep_targets &&
+        ((rc = vxh::fill_u64(st, b->vis.p, 0, poff[n])) || (rc = vxh::fill_u32(st, b->shadow.p, 0, (poff[n] + 1) / 2)) ||
+         (rc = vxh::fill_u32(st, b->hiz.p, 0, toff[n])) || (rc = vxh::fill_u32(st, b->cost.p, 0, toff[n]))))
         return rc;
     HIPX(hipStreamSynchronize(st));
     // A captured graph stays valid for another view set only if it reads every count from
@@ -694,7 +696,7 @@ int vx_batch_set_chunk_table(vx_batch *b, const vx_gpu_chunk *chunks, uint32_t n
     return VX_OK;
 }
 
-int vx_batch_set_cameras(vx_batch *b, const vx_camera_view *views, uint32_t n) {
+static int set_cameras(vx_batch *b, const vx_camera_view *views, uint32_t n, bool keep_targets) {
     if (!b || !views || n == 0 || n > b->max_views) return VX_E_ARG;
     for (uint32_t v = 0; v < n; v++)
         if (!views[v].frame) return VX_E_ARG;
@@ -712,6 +714,11 @@ int vx_batch_set_cameras(vx_batch *b, const vx_camera_view *views, uint32_t n) {
         b, n, [&](uint32_t v) { return &frames[v]; }, [&](uint32_t v) { return views[v].spec; },
         [&](uint32_t v) { return views[v].spec_hi; }, [&](uint32_t) { return nt; }, pl);
     if (rc) return rc;
+    // Moved cameras: the views must sit where they sat (same count, sizes and classes),
+    // so every pixel, tile and Hi-Z word keeps its place.
+    if (keep_targets && (n != b->n_views || pl.toff != b->toff || pl.poff != b->poff || pl.slot_of != b->slot_of ||
+                         std::memcmp(pl.cb, b->cb, sizeof b->cb) != 0))
+        return VX_E_ARG;
     vx_ctx *c = b->ctx;
     HIPX(hipSetDevice(c->device));
     hipStream_t st = c->stream;
@@ -750,7 +757,15 @@ int vx_batch_set_cameras(vx_batch *b, const vx_camera_view *views, uint32_t n) {
         k.f.chunk_count = counts[3 * slot + 1];
         k.f.camera_chunk = counts[3 * slot + 2];
     }
-    return install_views(b, n, pl, moved);
+    return install_views(b, n, pl, moved, keep_targets);
+}
+
+int vx_batch_set_cameras(vx_batch *b, const vx_camera_view *views, uint32_t n) {
+    return set_cameras(b, views, n, false);
+}
+
+int vx_batch_move_cameras(vx_batch *b, const vx_camera_view *views, uint32_t n) {
+    return set_cameras(b, views, n, true);
 }
 
 int vx_batch_view_counts(vx_batch *b, uint32_t view, uint32_t *n_total, uint32_t *n_visible, uint32_t *camera_chunk) {

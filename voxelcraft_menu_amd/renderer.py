@@ -224,9 +224,10 @@ class Batch:
         A.check(self._L.vx_batch_set_chunk_table(self._b, A.ptr(t) if len(t) else None, len(t)),
                 "vx_batch_set_chunk_table")
 
-    def set_cameras(self, cams: list, specs: list[tuple[int, int]]):
+    def set_cameras(self, cams: list, specs: list[tuple[int, int]], moved: bool = False):
         """Views from cameras alone: their chunk lists and grids are built on the device
-        from the chunk table, as build_view builds them on the host."""
+        from the chunk table, as build_view builds them on the host. moved: the installed
+        views one frame later (vx_batch_move_cameras), keeping their Hi-Z words."""
         from .world import camera_frame
 
         descs = (A.CameraView * len(cams))()
@@ -235,7 +236,8 @@ class Batch:
             fr, planes = camera_frame(cam)
             frames.append(fr)
             descs[i] = A.CameraView(C.addressof(fr), spec, spec_hi, (C.c_float * 24)(*planes))
-        A.check(self._L.vx_batch_set_cameras(self._b, descs, len(cams)), "vx_batch_set_cameras")
+        fn = self._L.vx_batch_move_cameras if moved else self._L.vx_batch_set_cameras
+        A.check(fn(self._b, descs, len(cams)), "vx_batch_move_cameras" if moved else "vx_batch_set_cameras")
         views = []
         for i, (cam, fr) in enumerate(zip(cams, frames)):
             nt, nv, cc = self.view_counts(i)
