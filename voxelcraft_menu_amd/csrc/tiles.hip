@@ -137,6 +137,7 @@ __device__ __forceinline__ void march_tile_body(const Pools &p, const FrameK &f,
         uint32_t d = in_frame ? (uint32_t)(best >> 40) : 0u;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) d = min(d, (uint32_t)__shfl_xor((int)d, o));
+        d = (uint32_t)__builtin_amdgcn_readfirstlane((int)d);  // every lane holds the minimum
         if (lane == 0) hiz[tile] = d;
         for (uint32_t w = 0; w < nwords; w++) {
             uint64_t def = defmask[w];
@@ -160,13 +161,19 @@ __device__ __forceinline__ void march_tile_body(const Pools &p, const FrameK &f,
     } else {
         if (lane == 0) hiz[tile] = 0u;
     }
-    const size_t pix = (size_t)py * f.res[0] + px;
-    if (in_frame) vis[pix] = best;
+    // The lane's pixel again, from the lane id and the tile (both free to recompute), so
+    // the coordinates need not be held through the march. Every lane is active here.
+    uint32_t lane2;
+    asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(lane2));
+    const uint32_t px2 = gx * 8u + (lane2 & 7u), py2 = gy * 8u + (lane2 >> 3);
+    const bool in_frame2 = px2 < f.res[0] && py2 < f.res[1];
+    const size_t pix = (size_t)py2 * f.res[0] + px2;
+    if (in_frame2) vis[pix] = best;
     uint32_t n_shadow = 0;
     if constexpr ((kVar & VAR_FUSED_SHADOW) != 0) {
         // primary_shadow reads only its own pixel's key: trace it here from the register.
         bool traced = false;
-        if (in_frame) shadow[pix] = shadow_pixel(p, f, s, best, rd, traced);
+        if (in_frame2) shadow[pix] = shadow_pixel(p, f, s, best, rd, traced);
         n_shadow = (uint32_t)__popcll(ballot(traced));
     }
     if (lane == 0) {
