@@ -11,7 +11,7 @@ are reported beside it as `plain`, with the ratio. Every view keeps its Hi-Z wor
 between steps, so steps after the first run in the Hi-Z steady state.
 
 The same line carries `incoherent` (config 3: 32 views at 1280x720 of the 2048x2048x512
-world with one secondary bounce per water / glass hit) and `serving`: p50/p99 request latency and Grays/s of an open-loop
+world, waves on, with one secondary bounce per water / glass hit) and `serving`: p50/p99 request latency and Grays/s of an open-loop
 Poisson stream of view requests (1000/s by default, the four resolutions) mixed with
 edit batches of 1000 voxel ops (2/s) on the device-resident 2048x2048x512 world,
 through the continuous-batching server.
@@ -499,18 +499,21 @@ def run_chunk_build(args):
 def run_incoherent(args):
     """The incoherent batch: the 2048x2048x512 world (caves, sea level 96, 1 % glass), 32
     seeded views at 1280x720, primary + sun shadow + one secondary bounce per water / glass
-    hit (water refract and reflect legs, glass transmit leg, each traced with trace_world
-    through compact leg queues), flat water normals (waves off). A step is one frame of
-    all 32 views; rays = primary + shadow + secondary legs."""
-    from voxelcraft_menu_amd import _abi as A
+    hit (water refract and reflect legs, glass transmit leg and the water legs behind it,
+    each traced with trace_world through compact leg queues), waves on with the reference's
+    defaults (amp 0.12, scale 32, speed 0.32, reflect slope 0; aniso, shoal and fill on) at
+    animation time 2.5. A step is one frame of all 32 views; rays = primary + shadow +
+    secondary legs + sun rays at their hits (the shoal probes under waved reflect legs
+    are not counted)."""
     from voxelcraft_menu_amd import workloads as W
     from voxelcraft_menu_amd.renderer import Batch, Renderer
     from voxelcraft_menu_amd.world import build_view
 
     wl = W.WORKLOADS[3]
     world = W.make_world(wl)
-    flags = W.DEFAULT_FLAGS & ~A.FLAG_WAVES
-    cams = W.random_cameras(wl, wl.views, seed=3, resolutions=[(wl.width, wl.height)], flags=flags)
+    cams = W.random_cameras(wl, wl.views, seed=3, resolutions=[(wl.width, wl.height)])
+    for c in cams:
+        c.anim_time = 2.5
     views = [build_view(world, c) for c in cams]
     specs = [W.spec_words(c.flags, world) for c in cams]
     px = sum(int(v.frame.res[0]) * int(v.frame.res[1]) for v in views)
@@ -542,7 +545,7 @@ def run_incoherent(args):
         "primary_rays": px, "shadow_rays": d["shadow_rays"], "secondary_rays": sec["water_legs"] + sec["glass_legs"],
         "water_legs": sec["water_legs"], "glass_legs": sec["glass_legs"], "secondary_sun_rays": sec["sun_rays"],
         "tuning": best,
-        "water": "flat normals (waves off)",
+        "water": "waves on (reference defaults, time 2.5)",
     }
     _log(f"incoherent: {res['ms_per_step']} ms/step ({res['ms_per_step_without_secondary']} without the bounce)")
     return res

@@ -153,6 +153,7 @@ typedef struct vxw_camera {
     float jitter[2];
     uint32_t flags;     /* FLAG_* word (src/render/mod.rs:144-270) */
     uint32_t frame_index;
+    float waves[4];     /* wave amp, scale, speed, reflect slope (config.rs defaults 0.12, 32, 0.32, 0) */
 } vxw_camera;
 
 /* Player::look_dir + scene::camera_basis (src/player.rs:122-126, src/scene.rs:334-339). */

@@ -77,6 +77,9 @@ enum {
 #define VX_FLAG_WATER_REFLECT 16u
 #define VX_FLAG_WATER_REFRACT 32u
 #define VX_FLAG_WAVES 512u
+#define VX_FLAG_WAVE_ANISO 1024u
+#define VX_FLAG_WAVE_SHOAL 2048u
+#define VX_FLAG_WAVE_FILL 4096u
 #define VX_FLAG_WATER_SHADOW_CUT 262144u
 /* spec_hi bits (src/render/mod.rs:220-270). */
 #define VX_FLAG_HI_FULL_MARCH 2u
@@ -103,7 +106,9 @@ typedef struct vx_gpu_frame {
     uint32_t grid_dim[3];  float water_absorb;
     float jitter[2];       float taa_feedback;
     uint32_t frame_index;
-    float shading[24];     /* cloud_cover .. zenith_deep (offsets 192-287), unused here */
+    float shading[12];     /* cloud_cover .. tint_freq_h (offsets 192-239), unused here */
+    float wave_amp, wave_scale, wave_speed, wave_reflect_slope;  /* offsets 240-255: the wave field */
+    float shading_tail[8]; /* shaft_origin .. zenith_deep (offsets 256-287), unused here */
     float prev_view_proj[16];
 } vx_gpu_frame;
 
@@ -356,15 +361,21 @@ int vx_batch_device_outputs(vx_batch *b, uint32_t view, uint64_t **vis, uint16_t
  * and shade_glass, src/render/shaders/resolve.wgsl:1029-1058, :1209-1220, :1261-1310, each
  * traced with trace_world, common.wgsl:1205-1253). With it on, every vx_batch_render also
  * gathers the legs of the pixels whose primary hit is water or (glass specialisation)
- * glass into compact queues and traces them. Water normals are the flat face normal, so
- * views whose pipeline key enables waves (VX_FLAG_WAVES in spec and frame flags) or the
- * water-look ripples return VX_E_UNSUPPORTED. Two records per pixel: [0] refract (water)
- * or transmit (glass), [1] reflect. At each hit that shade_hit would shade, the sun test
- * it makes runs too (resolve.wgsl:741-745: dot(n, sun) > 0, daylight > 0.01,
- * VX_FLAG_SHADOWS, then shadow_ray with the sea-surface jump, common.wgsl:1098-1148, over
- * 16 / 48 blocks for the refract leg with / without VX_FLAG_WATER_SHADOW_CUT, the frame's
- * shadow_dist for the water reflect leg, 48 for glass legs). A glass leg that ends on
- * water is shade_water's in the reference, a further bounce that is not traced here. */
+ * glass into compact queues and traces them. The water reflect leg follows water_ctx's
+ * mirror_t (resolve.wgsl:1078-1167): on top faces of views whose pipeline key and frame
+ * flags enable waves (VX_FLAG_WAVES, with VX_FLAG_WAVE_ANISO / _SHOAL / _FILL) it reflects
+ * about the wave normal at wave_reflect_slope, from the frame's time and wave_amp /
+ * wave_scale / wave_speed; views with the water-look ripples (VX_FLAG_HI_WATER_LOOK)
+ * return VX_E_UNSUPPORTED. Four records per pixel: [0] refract (water) or transmit
+ * (glass), [1] reflect, and where the glass transmit leg ends on water, shade_water's legs
+ * from that point (resolve.wgsl:1281-1289): [2] refract, [3] reflect. At each hit that
+ * shade_hit would shade, the sun test it makes runs too (resolve.wgsl:741-745:
+ * dot(n, sun) > 0, daylight > 0.01, VX_FLAG_SHADOWS, then shadow_ray with the sea-surface
+ * jump, common.wgsl:1098-1148, over 16 / 48 blocks for the refract leg with / without
+ * VX_FLAG_WATER_SHADOW_CUT, the frame's shadow_dist for the water reflect leg, 48 for
+ * glass legs); a glass leg that ends on water is shaded by shade_water, with no sun test
+ * of its own. The glass reflect leg (VX_FLAG_HI_GLASS_REFLECT) is not continued past
+ * water. */
 typedef struct vx_sec_hit {
     float t;            /* distance along the leg to its hit, or the leg's range */
     uint32_t voxel;     /* x | y << 6 | z << 12 in the hit chunk */
@@ -378,7 +389,8 @@ typedef struct vx_sec_hit {
 #define VX_SEC_HIT (1u << 31)
 int vx_batch_set_secondary(vx_batch *b, uint32_t on);
 int vx_batch_read_secondary(vx_batch *b, uint32_t view, vx_sec_hit *out, size_t n);
-/* Legs queued (water, glass) and sun rays run at their hits in the last frame; overflow
+/* Legs queued (water legs, those behind glass included; glass legs) and sun rays run at
+ * their hits in the last frame; overflow
  * is set when a queue ran out of room (never with the sizes the batch allocates). */
 int vx_batch_secondary_count(vx_batch *b, uint64_t *water_legs, uint64_t *glass_legs, uint64_t *sun_rays,
                              uint32_t *overflow);

@@ -25,7 +25,8 @@ class Pools(C.Structure):
 class Spec(C.Structure):
     _fields_ = [("leaf_cutout", C.c_uint32), ("glass", C.c_uint32), ("full_march", C.c_uint32),
                 ("emitter_gather", C.c_uint32), ("water_repair", C.c_uint32), ("leaf_fill", C.c_float),
-                ("glass_reflect", C.c_uint32)]
+                ("glass_reflect", C.c_uint32), ("waves", C.c_uint32), ("wave_aniso", C.c_uint32),
+                ("wave_shoal", C.c_uint32), ("wave_fill", C.c_uint32)]
 
 
 _STAT_FIELDS = ["pairs_select", "pairs_deferred", "pairs_recover", "rays_primary", "rays_shadow",
@@ -93,6 +94,16 @@ def _bind(L):
     L.vxo_shadow_ray_t.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, C.POINTER(Spec), V, V,
                                    C.c_float]
     L.vxo_hardware_threads.restype = C.c_int
+    L.vxo_wave_cos.restype = C.c_float
+    L.vxo_wave_cos.argtypes = [C.c_float]
+    L.vxo_wave_exp.restype = C.c_float
+    L.vxo_wave_exp.argtypes = [C.c_float]
+    L.vxo_water_reflect_dir.restype = None
+    L.vxo_water_reflect_dir.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, C.POINTER(Spec), C.c_uint32, V, V,
+                                        C.c_float, V]
+    L.vxo_wave_field.restype = None
+    L.vxo_wave_field.argtypes = [V, C.POINTER(Spec), C.c_float, C.c_float, C.c_float, V, C.c_float, C.c_int,
+                                 C.c_float, V]
     return L
 
 
@@ -135,6 +146,10 @@ def spec_from(flags: int, spec_hi: int) -> Spec:
         water_repair=0 if spec_hi & 2097152 else 1,
         leaf_fill=0.62 if flags & 524288 else 0.72,
         glass_reflect=1 if spec_hi & 128 else 0,
+        waves=1 if flags & 512 else 0,
+        wave_aniso=1 if flags & 1024 else 0,
+        wave_shoal=1 if flags & 2048 else 0,
+        wave_fill=1 if flags & 4096 else 0,
     )
 
 
@@ -191,11 +206,12 @@ def render_heat(scene: Scene, frame_bytes: bytes, spec: Spec, hiz: np.ndarray, s
 
 
 def secondary(scene: Scene, frame_bytes: bytes, spec: Spec, vis: np.ndarray, threads: int = 0):
-    """Secondary legs of every pixel: (W*H, 2) array of SEC records ([:, 0] refract or
-    transmit, [:, 1] reflect), and the stats of the pass."""
+    """Secondary legs of every pixel: (W*H, 4) array of SEC records ([:, 0] refract or
+    transmit, [:, 1] reflect, [:, 2:4] the refract and reflect legs from where a glass
+    transmit leg lands on water), and the stats of the pass."""
     f = np.frombuffer(frame_bytes, dtype=np.uint8).copy()
     v = np.ascontiguousarray(vis, dtype=np.uint64)
-    out = np.zeros((v.size, 2), dtype=SEC)
+    out = np.zeros((v.size, 4), dtype=SEC)
     st = Stats()
     rc = lib().vxo_secondary(C.byref(scene.pools), _p(scene.chunks), scene.n_chunks, _p(scene.grid), _p(f),
                              C.byref(spec), _p(v), _p(out), C.byref(st), threads)

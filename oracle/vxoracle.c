@@ -66,6 +66,10 @@
 #ifndef VXO_PLANT
 #define VXO_PLANT 0
 #endif
+/*   7: wave_octave adds the phase with the wrong sign
+ *   8: wave_octave swaps the components of its direction in the dot product
+ *   9: the anisotropic fade takes exp(-u) instead of exp(-u * u)
+ *  10: the reflect direction skips the optical-roughness blend toward the face normal */
 
 /* ---- per-thread context: pools + counters ---- */
 typedef struct ctx {
@@ -764,12 +768,196 @@ static float wgsl_smoothstep(float e0, float e1, float x) {
     return t * t * (3.0f - 2.0f * t);
 }
 
+/* ---- waved water (water_ctx, resolve.wgsl:1078-1167) ------------------------------------ */
+#define NORMAL_PY 3u               /* common.wgsl:295 */
+#define FLAG_WAVES 512u            /* src/render/mod.rs:180-186 */
+#define FLAG_WAVE_ANISO 1024u
+#define FLAG_WAVE_SHOAL 2048u
+#define FLAG_WAVE_FILL 4096u
+#define WAVE_TAU 6.28318531f       /* resolve.wgsl:902-912 */
+#define WAVE_GRAZE_MIN 1e-3f
+#define SHOAL_DEPTH 6.0f
+#define SHOAL_MAX 6.0f
+#define WAVE_FILTER_K 1.0f
+
+/* cos and exp: WGSL leaves their accuracy to the implementation, so the reference GPU's
+ * own rounding is not pinned. This restatement and the HIP kernels use the same f32 steps:
+ *   cos: k = round-to-even(x * 2/pi); r = ((x - k*A) - k*B) - k*C with A + B + C = pi/2
+ *        (Cody and Waite's reduction; A and B have 12 significant bits, so k*A and k*B are
+ *        exact for |k| < 4096); then the Taylor polynomial of cos (degree 8) or sin (degree
+ *        9) on |r| <= pi/4, chosen and signed by k mod 4, evaluated by Horner's rule;
+ *   exp: 0 for x < -86; k = round-to-even(x / ln 2), r = (x - k*L1) - k*L2 (L1 of 12 bits),
+ *        the degree-7 Taylor polynomial of exp(r), times 2^k.
+ * tests/test_oracle_wave_pins.py bounds both against float64. */
+float vxo_wave_cos(float x) {
+    const float kf = rintf(x * 0x1.45f306p-1f);                      /* 2/pi */
+    const float r = ((x - kf * 0x1.922p0f) - kf * -0x1.2aep-18f) - kf * -0x1.de973ep-31f;
+    const float r2 = r * r;
+    float cs = 0x1.a01a02p-16f;                                       /* 1/8! */
+    cs = cs * r2 - 0x1.6c16c2p-10f;                                   /* 1/6! */
+    cs = cs * r2 + 0x1.555556p-5f;                                    /* 1/4! */
+    cs = cs * r2 - 0.5f;
+    cs = cs * r2 + 1.0f;
+    float sn = 0x1.71de3ap-19f;                                       /* 1/9! */
+    sn = sn * r2 - 0x1.a01a02p-13f;                                   /* 1/7! */
+    sn = sn * r2 + 0x1.111112p-7f;                                    /* 1/5! */
+    sn = sn * r2 - 0x1.555556p-3f;                                    /* 1/3! */
+    sn = sn * r2 + 1.0f;
+    sn = sn * r;
+    switch ((int32_t)kf & 3) {
+        case 0: return cs;
+        case 1: return -sn;
+        case 2: return -cs;
+        default: return sn;
+    }
+}
+
+float vxo_wave_exp(float x) {
+    if (x < -86.0f) return 0.0f;
+    const float kf = rintf(x * 0x1.715476p0f);                        /* 1/ln 2 */
+    const float r = (x - kf * 0x1.62ep-1f) - kf * 0x1.0bfbe8p-15f;
+    float e = 0x1.a01a02p-13f;                                        /* 1/7! */
+    e = e * r + 0x1.6c16c2p-10f;
+    e = e * r + 0x1.111112p-7f;
+    e = e * r + 0x1.555556p-5f;
+    e = e * r + 0x1.555556p-3f;
+    e = e * r + 0.5f;
+    e = e * r + 1.0f;
+    e = e * r + 1.0f;
+    return ldexpf(e, (int)kf);
+}
+
+/* normalize(v) = v / sqrt(dot(v, v)); inverseSqrt(x) = 1 / sqrt(x) */
+static void wgsl_normalize(float v[3]) {
+    const float len = sqrtf(dot3o(v, v));
+    for (int k = 0; k < 3; k++) v[k] = v[k] / len;
+}
+
+/* wave_octave (resolve.wgsl:928-944) */
+static void wave_octave(const float p[2], const float dir[2], float lambda, float slope, float phase, float px,
+                        const float vh[2], float kk, int aniso, float out[3]) {
+    float fade;
+    if (aniso) {
+        const float d = dir[0] * vh[0] + dir[1] * vh[1];
+        const float ext = px * sqrtf(1.0f + kk * d * d);
+        const float u = ext / lambda;
+#if VXO_PLANT == 9
+        fade = vxo_wave_exp(-WAVE_FILTER_K * u);
+#else
+        fade = vxo_wave_exp(-WAVE_FILTER_K * u * u);
+#endif
+    } else {
+        fade = wgsl_smoothstep(2.0f, 4.0f, lambda / fmaxf(px, 1e-4f));
+    }
+#if VXO_PLANT == 7
+    const float g = slope * fade * vxo_wave_cos(WAVE_TAU / lambda * (p[0] * dir[0] + p[1] * dir[1]) - phase);
+#elif VXO_PLANT == 8
+    const float g = slope * fade * vxo_wave_cos(WAVE_TAU / lambda * (p[0] * dir[1] + p[1] * dir[0]) + phase);
+#else
+    const float g = slope * fade * vxo_wave_cos(WAVE_TAU / lambda * (p[0] * dir[0] + p[1] * dir[1]) + phase);
+#endif
+    out[0] = dir[0] * g;
+    out[1] = dir[1] * g;
+    out[2] = 0.5f * slope * slope * (1.0f - fade * fade);
+}
+
+/* wave_field (resolve.wgsl:946-967): the octaves summed left to right */
+static const float WAVE_D[4][2] = {{0.913089f, 0.407760f}, {-0.388685f, 0.921371f},
+                                   {-0.980382f, -0.197108f}, {0.477328f, -0.878725f}};
+static const float WAVE_D_DIV[4] = {0.0f, 2.17f, 4.71f, 10.20f};
+static const float WAVE_D_AMP[4] = {0.42f, 0.27f, 0.19f, 0.12f};
+static const float WAVE_D_PH[4] = {1.0000f, 1.4731f, 2.1703f, 3.1937f};
+static const float WAVE_F[8][2] = {{-0.438371f, 0.898794f}, {-0.927184f, 0.374607f}, {-0.190809f, 0.981627f},
+                                   {-0.669131f, 0.743145f}, {0.121869f, 0.992546f}, {0.694658f, 0.719340f},
+                                   {0.970296f, 0.241922f}, {0.438371f, 0.898794f}};
+static const float WAVE_F_DIV[8] = {0.0f, 1.3934f, 1.9417f, 2.7056f, 3.7700f, 5.2533f, 7.3201f, 10.2000f};
+static const float WAVE_F_AMP[8] = {0.3086f, 0.2584f, 0.2163f, 0.1810f, 0.1515f, 0.1269f, 0.1062f, 0.0889f};
+static const float WAVE_F_PH[8] = {1.0000f, 1.1804f, 1.3934f, 1.6449f, 1.9417f, 2.2920f, 2.7056f, 3.1937f};
+
+void vxo_wave_field(const vxo_frame *f, const vxo_spec *s, float px, float pz, float px_world,
+                    const float vh[2], float kk, int aniso, float shoal, float out[3]) {
+    const float p[2] = {px, pz};
+    const float l0 = f->wave_scale;
+    const float a = f->wave_amp * shoal;
+    const float ph = f->time * f->wave_speed * WAVE_TAU;
+    const int fill = s->wave_fill && (f->flags & FLAG_WAVE_FILL) != 0u;
+    const int n = fill ? 8 : 4;
+    out[0] = out[1] = out[2] = 0.0f;
+    for (int i = 0; i < n; i++) {
+        const float *dir = fill ? WAVE_F[i] : WAVE_D[i];
+        const float div = fill ? WAVE_F_DIV[i] : WAVE_D_DIV[i];
+        const float lambda = i == 0 ? l0 : l0 / div;  /* the first octave is l0 itself */
+        const float slope = a * (fill ? WAVE_F_AMP[i] : WAVE_D_AMP[i]);
+        const float phase = ph * (fill ? WAVE_F_PH[i] : WAVE_D_PH[i]);
+        float o[3];
+        wave_octave(p, dir, lambda, slope, phase, px_world, vh, kk, aniso, o);
+        if (i == 0) { out[0] = o[0]; out[1] = o[1]; out[2] = o[2]; }
+        else { out[0] = out[0] + o[0]; out[1] = out[1] + o[1]; out[2] = out[2] + o[2]; }
+    }
+}
+
+/* water_ctx's reflect-leg direction mirror_t (resolve.wgsl:1078-1167): about the face
+ * normal, or on waved top faces about the wave normal at the reflect slope, blended back
+ * toward the face normal by the optical roughness of the unresolved wave variance. The
+ * water-look ripples (SPEC_WATER_LOOK) are not restated; callers keep them off. */
+static void water_mirror_t(ctx *c, uint32_t nid, const float hit[3], const float rd[3], float t, float out[3]) {
+    const vxo_frame *f = c->f;
+    const vxo_spec *s = c->s;
+    float n[3], nt[3];
+    normal_of(nid, n);
+    for (int k = 0; k < 3; k++) nt[k] = n[k];
+    float unresolved = 0.0f;
+    const int waved = s->waves && (f->flags & FLAG_WAVES) != 0u && nid == NORMAL_PY;
+    if (waved) {
+        const float px_world = fmaxf(t, 1e-3f) * 2.0f * f->tan_half_fov / (float)f->res[1];
+        float kk = 0.0f, vh[2] = {0.0f, 0.0f};
+        const int aniso = s->wave_aniso && (f->flags & FLAG_WAVE_ANISO) != 0u;
+        if (aniso) {
+            const float k = 1.0f / fmaxf(fabsf(rd[1]), WAVE_GRAZE_MIN);
+            kk = k * k - 1.0f;
+            const float is = 1.0f / sqrtf(fmaxf(rd[0] * rd[0] + rd[2] * rd[2], 1e-12f));
+            vh[0] = rd[0] * is;
+            vh[1] = rd[2] * is;
+        }
+        float shoal = 1.0f;
+        if (s->wave_shoal && (f->flags & FLAG_WAVE_SHOAL) != 0u) {
+            const float dro[3] = {floorf(hit[0]) + 0.5f, hit[1] - SURFACE_EPS, floorf(hit[2]) + 0.5f};
+            const float sdir[3] = {0.001f, -0.999999f, 0.001f};
+            world_hit dh = trace_world(c, dro, sdir, SHOAL_MAX, 1);
+            const float depth = dh.hit ? dh.t : SHOAL_MAX;
+            shoal = wgsl_smoothstep(0.0f, SHOAL_DEPTH, depth);
+        }
+        float w[3];
+        vxo_wave_field(f, s, hit[0], hit[2], px_world, vh, kk, aniso, shoal, w);
+        unresolved = fmaxf(w[2], 0.0f);
+        const float len = sqrtf(w[0] * w[0] + w[1] * w[1]);
+        const float m = fminf(1.0f, f->wave_reflect_slope / fmaxf(len, 1e-6f));
+        nt[0] = -(w[0] * m);
+        nt[1] = 1.0f;
+        nt[2] = -(w[1] * m);
+        wgsl_normalize(nt);
+    }
+#if VXO_PLANT == 10
+    const float rough = 0.0f;
+    (void)unresolved;
+#else
+    const float rough = fminf(fmaxf(1.0f - vxo_wave_exp(-8.0f * unresolved), 0.0f), 1.0f);
+#endif
+    for (int k = 0; k < 3; k++) nt[k] = nt[k] * (1.0f - rough) + n[k] * rough;
+    wgsl_normalize(nt);
+    wgsl_reflect(rd, nt, out);
+    if (waved) {
+        out[1] = fmaxf(out[1], 0.0f);
+        wgsl_normalize(out);
+    }
+}
+
 /* The sun test shade_hit makes at a leg's hit (shade_hit_compact, resolve.wgsl:587-595
  * for n and vs, :741-745; shade_hit_legacy :507-514 is the same): n = normal_of(normal),
  * turned toward the ray for cross quads; only when dot(n, sun) > 0, daylight > 0.01 and
  * FLAG_SHADOWS does shadow_ray run, from hit + n * (0.02 * vs). A glass leg that ends on
- * water is shaded by shade_water instead (resolve.wgsl:1284-1289, :1305-1309), whose own
- * legs are not traced here: no sun test. */
+ * water is shaded by shade_water instead (resolve.wgsl:1284-1289, :1305-1309): no sun test
+ * at that hit (the transmit leg's water legs are records 2 and 3, tested at their own). */
 static uint32_t leg_sun(ctx *c, const float ro[3], const float rd[3], const world_hit *w, uint32_t material,
                         int water_leg, float shadow_dist) {
     const vxo_frame *f = c->f;
@@ -804,11 +992,40 @@ static void store_leg(ctx *c, const float ro[3], const float rd[3], const world_
 static float hit_t(ctx *c, uint32_t ci, const uint32_t v[3], const uint32_t micro[3], uint32_t nid,
                    const float rd[3]);
 
-/* The legs of one pixel: shade_water (resolve.wgsl:1209-1220 with water_ctx's flat normal,
- * :1078-1167: no waves, no water-look ripples) and shade_glass (:1261-1310). */
-static void secondary_pixel(ctx *c, const uint64_t *vis, uint32_t px, uint32_t py, vxo_sec out[2]) {
+/* shade_water's two legs from a water hit (resolve.wgsl:1209-1220): the refract leg about
+ * the face normal (water_refract_leg, :1029-1042) into r0, the reflect leg along water_ctx's
+ * mirror_t (water_reflect_leg, :1044-1058) into r1. */
+static void water_legs(ctx *c, uint32_t nid, const float hit[3], const float rd[3], float t, vxo_sec *r0,
+                       vxo_sec *r1) {
     const vxo_frame *f = c->f;
-    memset(out, 0, 2 * sizeof(vxo_sec));
+    float n[3];
+    normal_of(nid, n);
+    float refract_detail = 1.0f - wgsl_smoothstep(WATER_REFRACT_NEAR, WATER_REFRACT_FAR, t);
+    float reflect_detail = 1.0f - wgsl_smoothstep(WATER_REFLECT_NEAR, WATER_REFLECT_FAR, t);
+    if ((f->flags & FLAG_WATER_REFRACT) != 0u && refract_detail > 0.0f) {
+        float rdir[3], ro2[3];
+        wgsl_refract(rd, n, 1.0f / WATER_IOR, rdir);
+        for (int k = 0; k < 3; k++) ro2[k] = hit[k] + rdir[k] * SURFACE_EPS;
+        world_hit w = trace_world(c, ro2, rdir, WATER_REFRACT_DIST, 1);
+        float sd = (f->flags & FLAG_WATER_SHADOW_CUT) ? WATER_SHADOW_DIST : WATER_SHADOW_DIST_PRE41;
+        store_leg(c, ro2, rdir, &w, 1, sd, r0);
+    }
+    if ((f->flags & FLAG_WATER_REFLECT) != 0u && reflect_detail > 0.0f) {
+        float m[3], ro3[3];
+        water_mirror_t(c, nid, hit, rd, t, m);
+        for (int k = 0; k < 3; k++) ro3[k] = hit[k] + n[k] * SURFACE_EPS;
+        world_hit w = trace_world(c, ro3, m, WATER_REFLECT_DIST, 1);
+        store_leg(c, ro3, m, &w, 1, f->shadow_dist, r1);
+    }
+}
+
+/* The legs of one pixel: shade_water (resolve.wgsl:1209-1220) and shade_glass (:1261-1310),
+ * whose transmit leg, when it ends on water, is shaded by shade_water at that point with
+ * t + the leg's distance (:1284-1289): its two legs go to out[2] and out[3]. The glass
+ * reflect leg (SPEC_GLASS_REFLECT) ending on water is not continued. */
+static void secondary_pixel(ctx *c, const uint64_t *vis, uint32_t px, uint32_t py, vxo_sec out[4]) {
+    const vxo_frame *f = c->f;
+    memset(out, 0, 4 * sizeof(vxo_sec));
     uint64_t key = vis[(size_t)py * f->res[0] + px];
     if (key == 0) return;
     uint32_t ci = (uint32_t)(key >> 24u) & 0x1FFFu;
@@ -825,23 +1042,7 @@ static void secondary_pixel(ctx *c, const uint64_t *vis, uint32_t px, uint32_t p
     for (int k = 0; k < 3; k++) hit[k] = f->cam_pos[k] + rd[k] * t;
     normal_of(nid, n);
     if (id == WATER_ID) {
-        float refract_detail = 1.0f - wgsl_smoothstep(WATER_REFRACT_NEAR, WATER_REFRACT_FAR, t);
-        float reflect_detail = 1.0f - wgsl_smoothstep(WATER_REFLECT_NEAR, WATER_REFLECT_FAR, t);
-        if ((f->flags & FLAG_WATER_REFRACT) != 0u && refract_detail > 0.0f) {
-            float rdir[3], ro2[3];
-            wgsl_refract(rd, n, 1.0f / WATER_IOR, rdir);
-            for (int k = 0; k < 3; k++) ro2[k] = hit[k] + rdir[k] * SURFACE_EPS;
-            world_hit w = trace_world(c, ro2, rdir, WATER_REFRACT_DIST, 1);
-            float sd = (f->flags & FLAG_WATER_SHADOW_CUT) ? WATER_SHADOW_DIST : WATER_SHADOW_DIST_PRE41;
-            store_leg(c, ro2, rdir, &w, 1, sd, &out[0]);
-        }
-        if ((f->flags & FLAG_WATER_REFLECT) != 0u && reflect_detail > 0.0f) {
-            float m[3], ro3[3];
-            for (int k = 0; k < 3; k++) ro3[k] = hit[k] + n[k] * SURFACE_EPS;
-            wgsl_reflect(rd, n, m);
-            world_hit w = trace_world(c, ro3, m, WATER_REFLECT_DIST, 1);
-            store_leg(c, ro3, m, &w, 1, f->shadow_dist, &out[1]);
-        }
+        water_legs(c, nid, hit, rd, t, &out[0], &out[1]);
     } else {
         if (dot3o(n, rd) > 0.0f)
             for (int k = 0; k < 3; k++) n[k] = -n[k];
@@ -850,6 +1051,11 @@ static void secondary_pixel(ctx *c, const uint64_t *vis, uint32_t px, uint32_t p
         for (int k = 0; k < 3; k++) ro2[k] = hit[k] + rd[k] * SURFACE_EPS;
         world_hit w = trace_world(c, ro2, rd, dist, 0);
         store_leg(c, ro2, rd, &w, 0, GLASS_SHADOW_DIST, &out[0]);
+        if (w.hit && out[0].material == WATER_ID) {
+            float p2[3];
+            for (int k = 0; k < 3; k++) p2[k] = ro2[k] + rd[k] * w.t;
+            water_legs(c, w.normal, p2, rd, t + w.t, &out[2], &out[3]);
+        }
         if (c->s->glass_reflect) {
             float m[3], o[3];
             for (int k = 0; k < 3; k++) o[k] = hit[k] + n[k] * SURFACE_EPS;
@@ -1203,7 +1409,7 @@ static void *secondary_worker(void *arg) {
         int row = __atomic_fetch_add(&j->next, 1, __ATOMIC_SEQ_CST);
         if ((uint32_t)row >= f->res[1]) break;
         for (uint32_t px = 0; px < f->res[0]; px++)
-            secondary_pixel(&c, sj->vis_in, px, (uint32_t)row, &sj->out[2 * ((size_t)row * f->res[0] + px)]);
+            secondary_pixel(&c, sj->vis_in, px, (uint32_t)row, &sj->out[4 * ((size_t)row * f->res[0] + px)]);
     }
     j->per_thread[tid] = c.st;
     return NULL;
@@ -1316,6 +1522,17 @@ void vxo_march_chunk(const vxo_pools *p, const vxo_chunk *chunks, uint32_t n_chu
     ctx c;
     ctx_init(&c, &j);
     march_chunk(&c, ci, ro_w, rd, t_limit, skip_water, skip_foliage, skip_glass, sun_ray, out);
+}
+
+void vxo_water_reflect_dir(const vxo_pools *p, const vxo_chunk *chunks, uint32_t n_chunks, const uint32_t *grid,
+                           const vxo_frame *f, const vxo_spec *s, uint32_t nid, const float hit[3],
+                           const float rd[3], float t, float out[3]) {
+    job j;
+    memset(&j, 0, sizeof j);
+    j.p = p; j.chunks = chunks; j.n_chunks = n_chunks; j.grid = grid; j.f = f; j.s = s;
+    ctx c;
+    ctx_init(&c, &j);
+    water_mirror_t(&c, nid, hit, rd, t, out);
 }
 
 float vxo_shadow_ray_t(const vxo_pools *p, const vxo_chunk *chunks, uint32_t n_chunks, const uint32_t *grid,

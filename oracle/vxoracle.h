@@ -24,7 +24,14 @@
  * water-interface probe: rays crossing from a neighbouring chunk's water, against a
  * float64 DDA whose stop rule is "water only when entered from a non-water cell".
  * Planted mistakes (VXO_PLANT in vxoracle.c) in either are caught by those checks.
- * Foliage tufts and the Snell bend are not restated; the library refuses them.
+ * The wave field and the waved water reflect direction: cos and exp bounded against
+ * float64; the field as the gradient of the float64 sum of its octaves' heights, and
+ * curl-free; the unresolved variance in closed form (isotropic ends, anisotropic fade);
+ * the reflect direction against a float64 construction over deep and shallow water
+ * (shoal); planted mistakes in them are caught. The water legs behind glass: the same
+ * float64 leg construction from the point the transmit leg meets the water.
+ * Foliage tufts, the water-look ripples and the Snell bend are not restated; the library
+ * refuses them.
  *
  * Every struct here is an independent restatement of the reference layouts:
  *   vxo_frame  <- GpuFrame  (src/render/mod.rs:398-466, 352 bytes)
@@ -58,7 +65,9 @@ typedef struct vxo_frame {
     uint32_t grid_dim[3];  float water_absorb;
     float jitter[2];       float taa_feedback;
     uint32_t frame_index;
-    float rest[40];        /* cloud_cover .. prev_view_proj: not read by these passes */
+    float shading[12];     /* cloud_cover .. tint_freq_h: not read by these passes */
+    float wave_amp, wave_scale, wave_speed, wave_reflect_slope;  /* offsets 240-255 */
+    float rest[24];        /* shaft_origin .. prev_view_proj: not read */
 } vxo_frame;
 
 typedef struct vxo_chunk {
@@ -87,6 +96,7 @@ typedef struct vxo_spec {
     uint32_t water_repair;    /* SPEC_WATER_REPAIR */
     float leaf_fill;          /* SPEC_LEAF_FILL */
     uint32_t glass_reflect;   /* SPEC_GLASS_REFLECT */
+    uint32_t waves, wave_aniso, wave_shoal, wave_fill;  /* SPEC_WAVES, SPEC_WAVE_ANISO/SHOAL/FILL */
 } vxo_spec;
 
 /* Algorithmic traffic and work of one frame, counted exactly by the restatement. */
@@ -131,6 +141,19 @@ void vxo_march_chunk(const vxo_pools *p, const vxo_chunk *chunks, uint32_t n_chu
                      int skip_water, int skip_foliage, int skip_glass, int sun_ray,
                      vxo_hit *out);
 
+/* cos and exp as the wave field evaluates them (the same f32 steps as the HIP kernels;
+ * WGSL leaves their accuracy to the implementation). */
+float vxo_wave_cos(float x);
+float vxo_wave_exp(float x);
+/* wave_field (resolve.wgsl:946-967): x, y = slope gradient, z = unresolved variance. */
+void vxo_wave_field(const vxo_frame *f, const vxo_spec *s, float px, float pz, float px_world,
+                    const float vh[2], float kk, int aniso, float shoal, float out[3]);
+
+/* water_ctx's reflect-leg direction mirror_t at a water hit (resolve.wgsl:1078-1167). */
+void vxo_water_reflect_dir(const vxo_pools *p, const vxo_chunk *chunks, uint32_t n_chunks, const uint32_t *grid,
+                           const vxo_frame *f, const vxo_spec *s, uint32_t nid, const float hit[3],
+                           const float rd[3], float t, float out[3]);
+
 float vxo_shadow_ray_t(const vxo_pools *p, const vxo_chunk *chunks, uint32_t n_chunks,
                        const uint32_t *grid, const vxo_frame *f, const vxo_spec *s,
                        const float ro[3], const float rd[3], float max_dist);
@@ -169,8 +192,10 @@ typedef struct vxo_sec {
 } vxo_sec;
 
 /* Secondary bounce of every pixel of a visibility buffer (the legs of resolve.wgsl's
- * shade_water and shade_glass with flat water normals): out[2*pix] is the refract (water)
- * or transmit (glass) leg, out[2*pix+1] the reflect leg. */
+ * shade_water and shade_glass): out[4*pix] is the refract (water) or transmit (glass) leg,
+ * out[4*pix+1] the reflect leg (about water_ctx's wave normal on waved water), and, when a
+ * glass transmit leg ends on water, out[4*pix+2] and out[4*pix+3] are the refract and
+ * reflect legs shade_water traces from that point. */
 int vxo_secondary(const vxo_pools *p, const vxo_chunk *chunks, uint32_t n_chunks, const uint32_t *grid,
                   const vxo_frame *f, const vxo_spec *s, const uint64_t *vis, vxo_sec *out,
                   vxo_stats *stats, int threads);

@@ -270,20 +270,25 @@ def _sec_world():
 @pytest.mark.parametrize("form", [dict(mode=A.BATCH_FLAT, graph=0), dict(mode=A.BATCH_PERSISTENT, graph=1),
                                   dict(mode=A.BATCH_PLAIN, graph=0)], ids=_form_id)
 def test_secondary_legs_match_oracle(form):
-    """Secondary legs (water refract/reflect, glass transmit/reflect) of every pixel, with
-    the sun test at each hit, equal the oracle's records bit for bit: on the constructed
-    water/glass chunk (sea level at its water surface, both water sun-ray ranges) and on
-    views of the 1024x1024x256 world with flat water."""
+    """Secondary legs (water refract/reflect, glass transmit/reflect, and the water legs
+    behind glass) of every pixel, with the sun test at each hit, equal the oracle's records
+    bit for bit: on the constructed water/glass chunk (sea level at its water surface, both
+    water sun-ray ranges, flat and waved water) and on views of the 1024x1024x256 world
+    with waves on (reflect slope 0 and 0.35, two animation times)."""
     flags = A.FLAG_WATER_REFLECT | A.FLAG_WATER_REFRACT | A.FLAG_GLASS | A.FLAG_SHADOWS | A.FLAG_HIZ
     cases = []
     w1 = _sec_world()
+    waved = H.camera((30.5, 47.3, 6.5), 90.0, -25.0, 96, 64, flags | 512 | 1024 | 2048 | 4096, sea=32.0)
+    waved.anim_time, waved.waves = 1.7, (0.12, 32.0, 0.32, 0.35)
     cases.append((w1, [H.camera((30.5, 47.3, 6.5), 90.0, -25.0, 96, 64, flags, sea=32.0),
-                       H.camera((30.5, 47.3, 6.5), 90.0, -25.0, 96, 64, flags | A.FLAG_WATER_SHADOW_CUT, sea=32.0)],
+                       H.camera((30.5, 47.3, 6.5), 90.0, -25.0, 96, 64, flags | A.FLAG_WATER_SHADOW_CUT, sea=32.0),
+                       waved],
                   A.FLAG_HI_FULL_MARCH | A.FLAG_HI_GLASS_REFLECT))
     wl = W.WORKLOADS[1]
     w2 = W.make_world(wl)
-    cams = W.random_cameras(wl, 4, seed=91, resolutions=[(320, 180), (200, 120)],
-                            flags=W.DEFAULT_FLAGS & ~A.FLAG_WAVES)
+    cams = W.random_cameras(wl, 4, seed=91, resolutions=[(320, 180), (200, 120)])
+    for i, c in enumerate(cams):
+        c.anim_time, c.waves = 0.5 + i, (0.12, 32.0, 0.32, 0.35 if i % 2 else 0.0)
     cases.append((w2, cams, W.DEFAULT_SPEC_HI))
     for world, cams, spec_hi in cases:
         views = [build_view(world, c) for c in cams]
@@ -320,14 +325,15 @@ def test_secondary_legs_match_oracle(form):
         r.close()
 
 
-def test_secondary_refuses_wavy_water(small_world):
+def test_secondary_refuses_water_look(small_world):
+    """The water-look ripples are not carried into the legs: such views are refused."""
     wl = W.WORKLOADS[0]
-    cam = W.random_cameras(wl, 1, seed=3, resolutions=[(64, 48)], flags=W.DEFAULT_FLAGS | A.FLAG_WAVES)[0]
+    cam = W.random_cameras(wl, 1, seed=3, resolutions=[(64, 48)])[0]
     v = build_view(small_world, cam)
     r = Renderer(64, 48)
     r.sync_world(small_world)
     b = Batch(r, 2)
-    b.set_views([v], [W.spec_words(cam.flags, small_world)])
+    b.set_views([v], [W.spec_words(cam.flags, small_world, W.DEFAULT_SPEC_HI | A.FLAG_HI_WATER_LOOK)])
     with pytest.raises(RuntimeError):
         b.set_secondary(True)
     b.close()

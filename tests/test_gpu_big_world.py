@@ -1,7 +1,8 @@
 """GPU parity on the 2048x2048x512 world (caves, sea level 96, 1 % glass): the incoherent
 batch and the request server at their benchmark shapes, against the oracle.
 
-- Incoherent batch: the 32 seeded 1280x720 views, secondary bounce on. Every view's keys,
+- Incoherent batch: the 32 seeded 1280x720 views, secondary bounce on, waves on (every
+  other view with reflect slope 0.35). Every view's keys,
   R16F shadow words, Hi-Z words and secondary leg records (with their sun tests) equal
   the oracle's over two frames (Hi-Z carried), the select / deferred / recover / shadow
   counts and the leg counts equal the oracle's totals, and three launch forms give the
@@ -36,8 +37,9 @@ def _digest(*arrays):
 def test_incoherent_batch_matches_oracle_on_the_2048_world():
     wl = W.WORKLOADS[3]
     world = W.make_world(wl)
-    flags = W.DEFAULT_FLAGS & ~A.FLAG_WAVES  # flat water normals: the legs the library traces
-    cams = W.random_cameras(wl, wl.views, seed=3, resolutions=[(wl.width, wl.height)], flags=flags)
+    cams = W.random_cameras(wl, wl.views, seed=3, resolutions=[(wl.width, wl.height)])
+    for i, c in enumerate(cams):  # waves on: every other view with the wave normal tilting the reflect leg
+        c.anim_time, c.waves = 2.5, (0.12, 32.0, 0.32, 0.35 if i % 2 else 0.0)
     views = [build_view(world, c) for c in cams]
     specs = [W.spec_words(c.flags, world) for c in cams]
     r = Renderer(64, 64)

@@ -113,9 +113,10 @@ def test_lod_secondary_legs_match_oracle(form):
 
     w = World()
     render = settle(w, GEN, Planner(), EYE, CFG)
-    flags = W.DEFAULT_FLAGS & ~A.FLAG_WAVES
-    cams = [H.camera(EYE, yaw, pitch, 200, 120, flags) for yaw, pitch in ((200.0, -30.0), (300.0, -45.0),
-                                                                             (0.0, -35.0))]
+    cams = [H.camera(EYE, yaw, pitch, 200, 120, W.DEFAULT_FLAGS) for yaw, pitch in ((200.0, -30.0), (300.0, -45.0),
+                                                                                       (0.0, -35.0))]
+    for c in cams:
+        c.anim_time, c.waves = 1.2, (0.12, 32.0, 0.32, 0.35)
     views = [build_view_items(w, c, render, CFG) for c in cams]
     specs = [W.spec_words(c.flags, w, W.DEFAULT_SPEC_HI) for c in cams]
     r = Renderer(64, 64)

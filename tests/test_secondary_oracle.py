@@ -11,7 +11,9 @@ is built in float64 from the camera and the geometry alone:
   glass transmit  the primary ray continued from 0.02 past the glass face;
   glass reflect   the mirror about the face normal turned toward the ray;
 and its first hit is found with a unit-voxel DDA over the voxels that leg kind cannot skip
-(water legs: stone; glass legs: stone and water).
+(water legs: stone; glass legs: stone and water). Where the transmit leg ends on water
+(the glass column stands in the water), shade_water's two legs from that point are built
+the same way as the water legs above, from the face the transmit leg hit.
 
 The sun test at each hit (shade_hit, resolve.wgsl:741-745, with shadow_ray,
 common.wgsl:1098-1148) is built the same way: from hit + 0.02 n when dot(n, sun) > 0, a
@@ -136,8 +138,8 @@ def test_secondary_legs_match_float64_construction(glass_reflect, shadow_cut, le
     eta = 1.0 / 1.333
     sun = np.array(f.sun_dir, np.float64)
     refract_dist = 16.0 if shadow_cut else 48.0
-    counts = {"water": 0, "glass": 0}
-    tally = {k: {"lit": 0, "blocked": 0, "ambiguous": 0} for k in ("refract", "reflect", "glass")}
+    counts = {"water": 0, "glass": 0, "glass_water": 0}
+    tally = {k: {"lit": 0, "blocked": 0, "ambiguous": 0} for k in ("refract", "reflect", "glass", "behind_glass")}
 
     def sun_of(leg, ro, rd, exp, dist, kind, tag):
         if exp is None or exp[1] is None:
@@ -147,6 +149,21 @@ def test_secondary_legs_match_float64_construction(glass_reflect, shadow_cut, le
             return
         p = np.asarray(ro, np.float64) + np.asarray(rd, np.float64) * exp[2]
         _check_sun(leg, _sun_expect(stone, p, exp[1], sun, sea, dist, not legacy_water), tag + " sun", tally[kind])
+
+    def water_pair(p, d, nid, rec_refract, rec_reflect, tag, kind_r, kind_m):
+        """shade_water's legs from water point p hit along d on face nid (flat normal)."""
+        n = np.zeros(3)
+        n[nid >> 1] = -1.0 if nid & 1 == 0 else 1.0
+        m = d - 2.0 * np.dot(n, d) * n
+        e = H.brute_dda(stone, p + n * 0.02, m)
+        _check(rec_reflect, e, tag + " reflect")
+        sun_of(rec_reflect, p + n * 0.02, m, e, float(f.shadow_dist), kind_m, tag + " reflect")
+        dd = np.dot(n, d)
+        k = 1.0 - eta * eta * (1.0 - dd * dd)
+        r = eta * d - (eta * dd + np.sqrt(k)) * n
+        e = H.brute_dda(stone, p + r * 0.02, r)
+        _check(rec_refract, e, tag + " refract")
+        sun_of(rec_refract, p + r * 0.02, r, e, refract_dist, kind_r, tag + " refract")
 
     for i in range(w * h):
         key = int(vis[i])
@@ -164,17 +181,8 @@ def test_secondary_legs_match_float64_construction(glass_reflect, shadow_cut, le
             counts["water"] += 1
             t = (32.0 - cam_pos[1]) / d[1]
             p = cam_pos + d * t
-            n = np.array([0.0, 1.0, 0.0])
-            m = d - 2.0 * np.dot(n, d) * n
-            e = H.brute_dda(stone, p + n * 0.02, m)
-            _check(legs[i][1], e, tag + " reflect")
-            sun_of(legs[i][1], p + n * 0.02, m, e, float(f.shadow_dist), "reflect", tag + " reflect")
-            dd = np.dot(n, d)
-            k = 1.0 - eta * eta * (1.0 - dd * dd)
-            r = eta * d - (eta * dd + np.sqrt(k)) * n
-            e = H.brute_dda(stone, p + r * 0.02, r)
-            _check(legs[i][0], e, tag + " refract")
-            sun_of(legs[i][0], p + r * 0.02, r, e, refract_dist, "refract", tag + " refract")
+            water_pair(p, d, 3, legs[i][0], legs[i][1], tag, "refract", "reflect")
+            assert legs[i][2]["info"] == 0 and legs[i][3]["info"] == 0, tag
         elif mat == GLASS:
             counts["glass"] += 1
             axis = nid >> 1
@@ -184,6 +192,15 @@ def test_secondary_legs_match_float64_construction(glass_reflect, shadow_cut, le
             e = H.brute_dda(stone_or_water, p + d * 0.02, d)
             _check(legs[i][0], e, tag + " transmit")
             sun_of(legs[i][0], p + d * 0.02, d, e, 48.0, "glass", tag + " transmit")
+            ev = None if e is None else e[0]
+            if ev is not None and dense[ev[0] | (ev[2] << 6) | (ev[1] << 12)] == WATER:
+                assert e[1] is not None, tag
+                counts["glass_water"] += 1
+                p2 = p + d * 0.02 + d * e[2]
+                water_pair(p2, d, e[1], legs[i][2], legs[i][3], tag + " behind glass", "behind_glass",
+                           "behind_glass")
+            else:
+                assert legs[i][2]["info"] == 0 and legs[i][3]["info"] == 0, tag
             if glass_reflect:
                 n = np.zeros(3)
                 n[axis] = -1.0 if d[axis] > 0 else 1.0
@@ -195,10 +212,12 @@ def test_secondary_legs_match_float64_construction(glass_reflect, shadow_cut, le
                 assert legs[i][1]["info"] == 0, tag
         else:
             assert legs[i]["info"].sum() == 0, tag
-    assert counts["water"] > 200 and counts["glass"] > 10, counts
-    assert st["rays_secondary"] == 2 * counts["water"] + counts["glass"] * (2 if glass_reflect else 1)
+    assert counts["water"] > 200 and counts["glass"] > 10 and counts["glass_water"] >= 5, counts
+    assert st["rays_secondary"] == (2 * counts["water"] + counts["glass"] * (2 if glass_reflect else 1)
+                                    + 2 * counts["glass_water"])
     print(tally)
     # both outcomes occur for every leg kind
+    print(counts)
     for kind in ("refract", "reflect", "glass"):
         assert tally[kind]["lit"] >= 5 and tally[kind]["blocked"] >= 5, tally
     assert sum(t["ambiguous"] for t in tally.values()) < 10, tally

@@ -79,7 +79,10 @@ class Frame(C.Structure):
         ("grid_dim", C.c_uint32 * 3), ("water_absorb", C.c_float),
         ("jitter", C.c_float * 2), ("taa_feedback", C.c_float),
         ("frame_index", C.c_uint32),
-        ("shading", C.c_float * 24),
+        ("shading", C.c_float * 12),
+        ("wave_amp", C.c_float), ("wave_scale", C.c_float), ("wave_speed", C.c_float),
+        ("wave_reflect_slope", C.c_float),
+        ("shading_tail", C.c_float * 8),
         ("prev_view_proj", C.c_float * 16),
     ]
 
@@ -197,6 +200,7 @@ class Camera(C.Structure):
         ("fov_y", C.c_float), ("width", C.c_uint32), ("height", C.c_uint32), ("far_", C.c_float),
         ("sun_dir", C.c_float * 3), ("daylight", C.c_float), ("sea_level", C.c_float), ("time", C.c_float),
         ("jitter", C.c_float * 2), ("flags", C.c_uint32), ("frame_index", C.c_uint32),
+        ("waves", C.c_float * 4),
     ]
 
 

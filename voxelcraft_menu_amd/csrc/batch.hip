@@ -48,7 +48,8 @@ struct vx_batch {
     std::vector<uint32_t> slot_of; // caller's view index -> slot
     uint32_t cb[N_CLASSES + 1] = {0, 0, 0, 0, 0};  // class boundaries of the tile range
     DevBuf views, tile_off, hdr, chunks, grids, vis, shadow, hiz, cost, order, counters;
-    // Secondary bounce: two 16-byte leg records per pixel, two leg queues of sec_cap entries.
+    // Secondary bounce: SEC_RECORDS 16-byte leg records per pixel, three leg queues of
+    // sec_cap entries (water legs, glass legs, water legs behind glass).
     bool sec_on = false;
     DevBuf sec_out, sec_queue;
     DevBuf table, cams, list_counts;  // device chunk lists: chunk table, per-view cameras, counts
@@ -154,8 +155,9 @@ BatchArgs batch_args(const vx_batch *b) {
     return a;
 }
 
-// The secondary bounce over the frame's keys: gather the legs into the two queues, then
-// trace each queue (leg counts are read on the device, so a replay follows the frame).
+// The secondary bounce over the frame's keys: gather the legs into the water and glass
+// queues, trace each, then the water legs the glass trace queued where its transmit legs
+// end on water (leg counts are read on the device, so a replay follows the frame).
 int enqueue_secondary(vx_batch *b, hipStream_t st) {
     BatchArgs a = batch_args(b);
     SecArgs sa;
@@ -167,26 +169,28 @@ int enqueue_secondary(vx_batch *b, hipStream_t st) {
     const unsigned nb = (unsigned)b->n_cu * 16u;
     k_sec_trace<true><<<nb, 64, 0, st>>>(a, sa);
     k_sec_trace<false><<<nb, 64, 0, st>>>(a, sa);
+    k_sec_trace_sub<<<nb, 64, 0, st>>>(a, sa);
     LAUNCHED();
     return VX_OK;
 }
 
-// Sizes the secondary buffers for the current views; legs need flat water normals.
+// Sizes the secondary buffers for the current views (the water-look ripples are not
+// carried into the legs: such views are refused).
 int secondary_buffers(vx_batch *b) {
     for (const ViewK &k : b->vk)
-        if (k.s.bits & SPEC_WAVY_WATER) return VX_E_UNSUPPORTED;
+        if (k.s.bits & SPEC_WATER_LOOK) return VX_E_UNSUPPORTED;
     if (b->n_px * 2 > 0xFFFFFFFFull) return VX_E_OVERFLOW;
     const uint32_t cap = (uint32_t)std::max<uint64_t>(b->n_px * 2, 64);
     const void *o0 = b->sec_out.p, *q0 = b->sec_queue.p;
     int rc;
-    if ((rc = ensure(b->sec_out, (size_t)std::max<uint64_t>(b->n_px, 1) * 2 * 16)) ||
-        (rc = ensure(b->sec_queue, (size_t)cap * 2 * 8)))
+    if ((rc = ensure(b->sec_out, (size_t)std::max<uint64_t>(b->n_px, 1) * SEC_RECORDS * 16)) ||
+        (rc = ensure(b->sec_queue, (size_t)cap * 3 * 8)))
         return rc;
     if (o0 != b->sec_out.p || q0 != b->sec_queue.p || cap != b->sec_cap) drop_graph(b);
     b->sec_cap = cap;
     // zero records and leg counters: the next frame's clear pass has nothing to undo
     hipStream_t st = b->ctx->stream;
-    if ((rc = vxh::fill_u32(st, b->sec_out.p, 0, std::max<uint64_t>(b->n_px, 1) * 2 * 4)) ||
+    if ((rc = vxh::fill_u32(st, b->sec_out.p, 0, std::max<uint64_t>(b->n_px, 1) * SEC_RECORDS * 4)) ||
         (rc = vxh::fill_u32(st, (uint32_t *)b->counters.p + CNT_SEC, 0, SEC_LEN)))
         return rc;
     HIPX(hipStreamSynchronize(st));
@@ -990,8 +994,8 @@ int vx_batch_set_secondary(vx_batch *b, uint32_t on) {
 int vx_batch_read_secondary(vx_batch *b, uint32_t view, vx_sec_hit *out, size_t n) {
     if (!b || !out || view >= b->n_views || !b->sec_on || !b->rendered) return VX_E_ARG;
     const uint32_t v = b->slot_of[view];
-    if (n != 2 * (b->poff[v + 1] - b->poff[v])) return VX_E_ARG;
-    return batch_read(b, b->sec_out, b->poff[v] * 2 * 16, out, n * 16);
+    if (n != SEC_RECORDS * (b->poff[v + 1] - b->poff[v])) return VX_E_ARG;
+    return batch_read(b, b->sec_out, b->poff[v] * SEC_RECORDS * 16, out, n * 16);
 }
 
 int vx_batch_secondary_count(vx_batch *b, uint64_t *water_legs, uint64_t *glass_legs, uint64_t *sun_rays,
@@ -1000,7 +1004,7 @@ int vx_batch_secondary_count(vx_batch *b, uint64_t *water_legs, uint64_t *glass_
     uint32_t w[CNT_TOTAL - CNT_SEC];
     int rc = batch_read(b, b->counters, CNT_SEC * 4, w, sizeof w);
     if (rc) return rc;
-    if (water_legs) *water_legs = w[SEC_WATER * SEC_LINE];
+    if (water_legs) *water_legs = (uint64_t)w[SEC_WATER * SEC_LINE] + w[SEC_SUB * SEC_LINE];
     if (glass_legs) *glass_legs = w[SEC_GLASS * SEC_LINE];
     if (sun_rays) *sun_rays = w[SEC_SUN * SEC_LINE];
     if (overflow) *overflow = w[SEC_OVERFLOW * SEC_LINE];

@@ -421,5 +421,10 @@ extern "C" int vxw_pack_frame(const vxw_camera *cam, const vxw_view *view, uint3
     f->jitter[1] = cam->jitter[1];
     f->taa_feedback = 1.0f;
     f->frame_index = cam->frame_index % 64u;
+    // the wave field's parameters, clamped as Renderer::render packs them (src/render/mod.rs:1690-1694)
+    f->wave_amp = std::max(cam->waves[0], 0.0f);
+    f->wave_scale = std::max(cam->waves[1], 1.0f);
+    f->wave_speed = cam->waves[2];
+    f->wave_reflect_slope = std::max(cam->waves[3], 0.0f);
     return VXW_OK;
 }

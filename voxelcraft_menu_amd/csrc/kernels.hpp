@@ -46,8 +46,16 @@ constexpr uint32_t CTL_HIST = 0, CTL_CURSOR = ORDER_BINS, CTL_HEAD = 2 * ORDER_B
                    CTL_LEN = 2 * ORDER_BINS + N_CLASSES * CTL_HEAD_STRIDE;
 // Secondary-leg queues (two: water legs, glass legs) and their overflow flag, each word
 // on its own line, after the control block.
+// SEC_SUB: the water legs spawned where a glass transmit leg ends on water.
 constexpr uint32_t CNT_SEC = CNT_CTL + CTL_LEN, SEC_LINE = 32, SEC_WATER = 0, SEC_GLASS = 1, SEC_OVERFLOW = 2,
-                   SEC_SUN = 3, SEC_LEN = 4 * SEC_LINE;
+                   SEC_SUN = 3, SEC_SUB = 4, SEC_LEN = 5 * SEC_LINE;
+// A pixel's leg records: [0] refract / transmit, [1] reflect, [2] [3] the refract and
+// reflect legs from where its glass transmit leg ends on water.
+constexpr uint32_t SEC_RECORDS = 4;
+// Leg-queue entry: view << 32 | pixel << 2 | record.
+__host__ __device__ constexpr unsigned long long sec_entry(uint32_t view, uint32_t pix, uint32_t rec) {
+    return ((unsigned long long)view << 32) | ((unsigned long long)pix << 2) | rec;
+}
 constexpr uint32_t CNT_TOTAL = CNT_LEN + CTL_LEN + SEC_LEN;
 
 __device__ __forceinline__ void stat_add(uint32_t *counters, uint32_t slot, uint32_t k, uint32_t v) {
@@ -206,8 +214,8 @@ __global__ void k_decode(Pools p, FrameK f, const unsigned long long *__restrict
 //     pixel's legs to the water / glass queues (entries view << 32 | pixel << 1 | leg);
 //   k_sec_trace<kWater>: one leg per lane from its queue, grouped by view, trace_world.
 struct SecArgs {
-    uint4 *__restrict__ out;                  // 2 records per pixel of the batch
-    unsigned long long *__restrict__ queue;   // queue q at q * cap
+    uint4 *__restrict__ out;                  // SEC_RECORDS records per pixel of the batch
+    unsigned long long *__restrict__ queue;   // queue q (SEC_WATER, SEC_GLASS, then the sub-leg queue) at q * cap
     const uint32_t *__restrict__ tile_off;    // n_views + 1 tile offsets (batch slot order)
     uint32_t cap;                             // entries per queue
 };
@@ -215,5 +223,8 @@ __global__ void k_sec_clear(BatchArgs a, SecArgs s);
 __global__ void k_sec_gather(BatchArgs a, SecArgs s);
 template <bool kWater>
 __global__ void k_sec_trace(BatchArgs a, SecArgs s);
+__global__ void k_sec_trace_sub(BatchArgs a, SecArgs s);
+// Queue slot of the sub-leg queue in SecArgs::queue (after the water and glass queues).
+constexpr uint32_t SEC_Q_SUB = 2;
 
 }  // namespace vxd

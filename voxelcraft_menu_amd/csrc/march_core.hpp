@@ -27,8 +27,10 @@ constexpr uint32_t NORMAL_CROSS_A = 6u, NORMAL_CROSS_B = 7u;
 // Pipeline specialisation bits (the WGSL override constants the march reads).
 constexpr uint32_t SPEC_CUTOUT = 1u, SPEC_GLASS = 2u, SPEC_FULL_MARCH = 4u, SPEC_EMITTER = 8u,
                    SPEC_WATER_REPAIR = 16u, SPEC_GLASS_REFLECT = 32u,
-                   // frames whose water normals are not the flat face normal (wave field, ripples)
-                   SPEC_WAVY_WATER = 64u;
+                   // the water-look ripples (spec_hi), whose normals are not restated
+                   SPEC_WATER_LOOK = 64u,
+                   // the wave field (SPEC_WAVES, SPEC_WAVE_ANISO, SPEC_WAVE_SHOAL, SPEC_WAVE_FILL)
+                   SPEC_WAVES = 128u, SPEC_WAVE_ANISO = 256u, SPEC_WAVE_SHOAL = 512u, SPEC_WAVE_FILL = 1024u;
 
 struct Spec {
     uint32_t bits;
@@ -55,6 +57,7 @@ struct FrameK {
     uint32_t chunk_count, camera_chunk, flags, max_pairs, frame_index;
     int32_t grid_min[3];
     uint32_t grid_dim[3];
+    float time, wave_amp, wave_scale, wave_speed, wave_reflect_slope;  // the wave field's inputs
 };
 
 // Pool reads go through buffer resources whose size is the pool's length: an index past
@@ -1057,6 +1060,171 @@ __device__ __forceinline__ float wgsl_smoothstep(float e0, float e1, float x) {
     return t * t * (3.0f - 2.0f * t);
 }
 
+// ---- waved water (water_ctx, resolve.wgsl:1078-1167) ---------------------------------------
+constexpr uint32_t NORMAL_PY = 3u;  // common.wgsl:295
+constexpr uint32_t FLAG_WAVES = 512u, FLAG_WAVE_ANISO = 1024u, FLAG_WAVE_SHOAL = 2048u,
+                   FLAG_WAVE_FILL = 4096u;  // src/render/mod.rs:180-186
+constexpr float WAVE_TAU = 6.28318531f, WAVE_GRAZE_MIN = 1e-3f, SHOAL_DEPTH = 6.0f, SHOAL_MAX = 6.0f,
+                WAVE_FILTER_K = 1.0f;  // resolve.wgsl:902-912
+
+// cos and exp of the wave field. WGSL leaves their accuracy to the implementation; these
+// are fixed f32 steps, the ones the oracle takes:
+//   cos: k = round-to-even(x * 2/pi); r = ((x - k*A) - k*B) - k*C with A + B + C = pi/2
+//        (Cody and Waite's reduction: A and B have 12 significant bits, so k*A and k*B
+//        are exact for |k| < 4096); the Taylor polynomial of cos (degree 8) or sin
+//        (degree 9) on |r| <= pi/4, chosen and signed by k mod 4, by Horner's rule;
+//   exp: 0 for x < -86; k = round-to-even(x / ln 2), r = (x - k*L1) - k*L2 (L1 of 12
+//        bits); the degree-7 Taylor polynomial of exp(r), times 2^k.
+__device__ __forceinline__ float wave_cos(float x) {
+    const float kf = rintf(x * 0x1.45f306p-1f);
+    const float r = ((x - kf * 0x1.922p0f) - kf * -0x1.2aep-18f) - kf * -0x1.de973ep-31f;
+    const float r2 = r * r;
+    float cs = 0x1.a01a02p-16f;
+    cs = cs * r2 - 0x1.6c16c2p-10f;
+    cs = cs * r2 + 0x1.555556p-5f;
+    cs = cs * r2 - 0.5f;
+    cs = cs * r2 + 1.0f;
+    float sn = 0x1.71de3ap-19f;
+    sn = sn * r2 - 0x1.a01a02p-13f;
+    sn = sn * r2 + 0x1.111112p-7f;
+    sn = sn * r2 - 0x1.555556p-3f;
+    sn = sn * r2 + 1.0f;
+    sn = sn * r;
+    const int32_t q = (int32_t)kf & 3;
+    return q == 0 ? cs : (q == 1 ? -sn : (q == 2 ? -cs : sn));
+}
+
+__device__ __forceinline__ float wave_exp(float x) {
+    if (x < -86.0f) return 0.0f;
+    const float kf = rintf(x * 0x1.715476p0f);
+    const float r = (x - kf * 0x1.62ep-1f) - kf * 0x1.0bfbe8p-15f;
+    float e = 0x1.a01a02p-13f;
+    e = e * r + 0x1.6c16c2p-10f;
+    e = e * r + 0x1.111112p-7f;
+    e = e * r + 0x1.555556p-5f;
+    e = e * r + 0x1.555556p-3f;
+    e = e * r + 0.5f;
+    e = e * r + 1.0f;
+    e = e * r + 1.0f;
+    return ldexpf(e, (int32_t)kf);
+}
+
+// normalize(v) = v / sqrt(dot(v, v)) (common conventions of the restatement)
+__device__ __forceinline__ void wgsl_normalize(float v[3]) {
+    const float len = sqrtf(dot3o(v, v));
+#pragma unroll
+    for (int k = 0; k < 3; k++) v[k] = v[k] / len;
+}
+
+// wave_octave (resolve.wgsl:928-944)
+__device__ __forceinline__ void wave_octave(const float p[2], float dx, float dz, float lambda, float slope,
+                                            float phase, float px, const float vh[2], float kk, bool aniso,
+                                            float out[3]) {
+    float fade;
+    if (aniso) {
+        const float d = dx * vh[0] + dz * vh[1];
+        const float ext = px * sqrtf(1.0f + kk * d * d);
+        const float u = ext / lambda;
+        fade = wave_exp(-WAVE_FILTER_K * u * u);
+    } else {
+        fade = wgsl_smoothstep(2.0f, 4.0f, lambda / fmaxf(px, 1e-4f));
+    }
+    const float g = slope * fade * wave_cos(WAVE_TAU / lambda * (p[0] * dx + p[1] * dz) + phase);
+    out[0] = dx * g;
+    out[1] = dz * g;
+    out[2] = 0.5f * slope * slope * (1.0f - fade * fade);
+}
+
+// wave_field (resolve.wgsl:946-967): the octaves summed left to right; the first octave's
+// wavelength is wave_scale itself.
+__device__ inline void wave_field(const FrameK &f, const Spec &s, float hx, float hz, float px_world,
+                                  const float vh[2], float kk, bool aniso, float shoal, float out[3]) {
+    constexpr float DD[4][2] = {{0.913089f, 0.407760f}, {-0.388685f, 0.921371f}, {-0.980382f, -0.197108f},
+                                {0.477328f, -0.878725f}};
+    constexpr float DDIV[4] = {0.0f, 2.17f, 4.71f, 10.20f}, DAMP[4] = {0.42f, 0.27f, 0.19f, 0.12f},
+                    DPH[4] = {1.0000f, 1.4731f, 2.1703f, 3.1937f};
+    constexpr float FD[8][2] = {{-0.438371f, 0.898794f}, {-0.927184f, 0.374607f}, {-0.190809f, 0.981627f},
+                                {-0.669131f, 0.743145f}, {0.121869f, 0.992546f},  {0.694658f, 0.719340f},
+                                {0.970296f, 0.241922f},  {0.438371f, 0.898794f}};
+    constexpr float FDIV[8] = {0.0f, 1.3934f, 1.9417f, 2.7056f, 3.7700f, 5.2533f, 7.3201f, 10.2000f},
+                    FAMP[8] = {0.3086f, 0.2584f, 0.2163f, 0.1810f, 0.1515f, 0.1269f, 0.1062f, 0.0889f},
+                    FPH[8] = {1.0000f, 1.1804f, 1.3934f, 1.6449f, 1.9417f, 2.2920f, 2.7056f, 3.1937f};
+    const float p[2] = {hx, hz};
+    const float l0 = f.wave_scale;
+    const float a = f.wave_amp * shoal;
+    const float ph = f.time * f.wave_speed * WAVE_TAU;
+    const bool fill = (s.bits & SPEC_WAVE_FILL) && (f.flags & FLAG_WAVE_FILL);
+    float o[3];
+    if (fill) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const float lambda = i == 0 ? l0 : l0 / FDIV[i];
+            wave_octave(p, FD[i][0], FD[i][1], lambda, a * FAMP[i], ph * FPH[i], px_world, vh, kk, aniso, o);
+            if (i == 0) { out[0] = o[0]; out[1] = o[1]; out[2] = o[2]; }
+            else { out[0] = out[0] + o[0]; out[1] = out[1] + o[1]; out[2] = out[2] + o[2]; }
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const float lambda = i == 0 ? l0 : l0 / DDIV[i];
+            wave_octave(p, DD[i][0], DD[i][1], lambda, a * DAMP[i], ph * DPH[i], px_world, vh, kk, aniso, o);
+            if (i == 0) { out[0] = o[0]; out[1] = o[1]; out[2] = o[2]; }
+            else { out[0] = out[0] + o[0]; out[1] = out[1] + o[1]; out[2] = out[2] + o[2]; }
+        }
+    }
+}
+
+// water_ctx's reflect-leg direction mirror_t (resolve.wgsl:1078-1167): about the face
+// normal, or on waved top faces about the wave normal at the reflect slope (the shoal
+// depth from a short trace straight down), blended back toward the face normal by the
+// optical roughness of the unresolved wave variance. Water-look ripples are refused.
+__device__ inline void water_mirror_t(const Pools &p, const FrameK &f, const Spec &s, uint32_t nid, const float hit[3],
+                                      const float rd[3], float t, float out[3]) {
+    float n[3], nt[3];
+    normal_of(nid, n);
+#pragma unroll
+    for (int k = 0; k < 3; k++) nt[k] = n[k];
+    float unresolved = 0.0f;
+    const bool waved = (s.bits & SPEC_WAVES) && (f.flags & FLAG_WAVES) && nid == NORMAL_PY;
+    if (waved) {
+        const float px_world = fmaxf(t, 1e-3f) * 2.0f * f.tan_half_fov / (float)f.res[1];
+        float kk = 0.0f, vh[2] = {0.0f, 0.0f};
+        const bool aniso = (s.bits & SPEC_WAVE_ANISO) && (f.flags & FLAG_WAVE_ANISO);
+        if (aniso) {
+            const float k = 1.0f / fmaxf(fabsf(rd[1]), WAVE_GRAZE_MIN);
+            kk = k * k - 1.0f;
+            const float is = 1.0f / sqrtf(fmaxf(rd[0] * rd[0] + rd[2] * rd[2], 1e-12f));
+            vh[0] = rd[0] * is;
+            vh[1] = rd[2] * is;
+        }
+        float shoal = 1.0f;
+        if ((s.bits & SPEC_WAVE_SHOAL) && (f.flags & FLAG_WAVE_SHOAL)) {
+            const float dro[3] = {floorf(hit[0]) + 0.5f, hit[1] - SURFACE_EPS, floorf(hit[2]) + 0.5f};
+            const float sdir[3] = {0.001f, -0.999999f, 0.001f};
+            const WorldHit dh = trace_world<true>(p, f, s, dro, sdir, SHOAL_MAX);
+            shoal = wgsl_smoothstep(0.0f, SHOAL_DEPTH, dh.hit ? dh.t : SHOAL_MAX);
+        }
+        float w[3];
+        wave_field(f, s, hit[0], hit[2], px_world, vh, kk, aniso, shoal, w);
+        unresolved = fmaxf(w[2], 0.0f);
+        const float len = sqrtf(w[0] * w[0] + w[1] * w[1]);
+        const float m = fminf(1.0f, f.wave_reflect_slope / fmaxf(len, 1e-6f));
+        nt[0] = -(w[0] * m);
+        nt[1] = 1.0f;
+        nt[2] = -(w[1] * m);
+        wgsl_normalize(nt);
+    }
+    const float rough = fminf(fmaxf(1.0f - wave_exp(-8.0f * unresolved), 0.0f), 1.0f);
+#pragma unroll
+    for (int k = 0; k < 3; k++) nt[k] = nt[k] * (1.0f - rough) + n[k] * rough;
+    wgsl_normalize(nt);
+    wgsl_reflect(rd, nt, out);
+    if (waved) {
+        out[1] = fmaxf(out[1], 0.0f);
+        wgsl_normalize(out);
+    }
+}
+
 // One secondary leg to trace: origin, direction, range, and which leg of the pixel.
 struct SecLeg {
     float ro[3], rd[3], max_dist;
@@ -1068,7 +1236,8 @@ struct SecLeg {
 // n and vs, :741-745; shade_hit_legacy :507-514): n = normal_of(normal), turned toward the
 // ray for cross quads; only when dot(n, sun) > 0, daylight > 0.01 and FLAG_SHADOWS does
 // shadow_ray run, from hit + n * (0.02 * vs). A glass leg that ends on water is shaded by
-// shade_water (resolve.wgsl:1284-1289, :1305-1309), whose own legs are not traced: no test.
+// shade_water (resolve.wgsl:1284-1289, :1305-1309): no test at that hit (the transmit
+// leg's water legs are records 2 and 3, tested at their own hits).
 __device__ inline uint32_t leg_sun(const Pools &p, const FrameK &f, const Spec &s, const SecLeg &l,
                                    const WorldHit &w, const ChunkK &c, uint32_t material) {
     if (!w.hit || (!l.water && material == WATER_ID)) return 0u;
@@ -1088,79 +1257,123 @@ __device__ inline uint32_t leg_sun(const Pools &p, const FrameK &f, const Spec &
     return SEC_SUN_TESTED | (shadow_ray(p, f, s, o, l.sun_dist) ? SEC_SUN_BLOCKED : 0u);
 }
 
-// The legs of one pixel's primary key (shade_water with water_ctx's flat normal,
-// resolve.wgsl:1078-1167, :1209-1220; shade_glass, :1261-1310). Returns the number of
-// legs set: legs[0] = refract (water) / transmit (glass) when want[0], legs[1] = reflect.
-__device__ inline void secondary_legs(const Pools &p, const FrameK &f, const Spec &s, uint64_t key, uint32_t px,
-                                      uint32_t py, SecLeg legs[2], bool want[2]) {
-    want[0] = want[1] = false;
-    if (key == 0ull) return;
+// shade_water's two legs from a water hit (resolve.wgsl:1209-1220): the refract leg about
+// the face normal (water_refract_leg, :1029-1042) in legs[0], the reflect leg along
+// water_ctx's mirror_t (water_reflect_leg, :1044-1058) in legs[1]. kDirs false: only which
+// legs exist (want) and their origins' inputs are needed, not the reflect direction.
+template <bool kDirs>
+__device__ inline void water_leg_pair(const Pools &p, const FrameK &f, const Spec &s, uint32_t nid, const float hit[3],
+                                      const float rd[3], float t, SecLeg legs[2], bool want[2]) {
+    float n[3];
+    normal_of(nid, n);
+    const float refract_detail = 1.0f - wgsl_smoothstep(WATER_REFRACT_NEAR, WATER_REFRACT_FAR, t);
+    const float reflect_detail = 1.0f - wgsl_smoothstep(WATER_REFLECT_NEAR, WATER_REFLECT_FAR, t);
+    if ((f.flags & FLAG_WATER_REFRACT) && refract_detail > 0.0f) {
+        SecLeg &l = legs[0];
+        wgsl_refract(rd, n, 1.0f / WATER_IOR, l.rd);
+#pragma unroll
+        for (int k = 0; k < 3; k++) l.ro[k] = hit[k] + l.rd[k] * SURFACE_EPS;
+        l.max_dist = WATER_REFRACT_DIST;
+        l.sun_dist = (f.flags & FLAG_WATER_SHADOW_CUT) ? WATER_SHADOW_DIST : WATER_SHADOW_DIST_PRE41;
+        l.water = true;
+        want[0] = true;
+    }
+    if ((f.flags & FLAG_WATER_REFLECT) && reflect_detail > 0.0f) {
+        SecLeg &l = legs[1];
+        if constexpr (kDirs) water_mirror_t(p, f, s, nid, hit, rd, t, l.rd);
+#pragma unroll
+        for (int k = 0; k < 3; k++) l.ro[k] = hit[k] + n[k] * SURFACE_EPS;
+        l.max_dist = WATER_REFLECT_DIST;
+        l.sun_dist = f.shadow_dist;
+        l.water = true;
+        want[1] = true;
+    }
+}
+
+// The primary hit a key names: ray direction, hit point, distance (hit_t), face and block.
+struct KeyHit {
+    float rd[3], hit[3], t;
+    uint32_t nid, id;
+    bool ok;
+};
+__device__ inline KeyHit key_hit(const Pools &p, const FrameK &f, uint64_t key, uint32_t px, uint32_t py) {
+    KeyHit h;
+    h.ok = false;
+    if (key == 0ull) return h;
     const uint32_t ci = (uint32_t)(key >> 24) & 0x1FFFu;
-    if (ci >= p.n_chunks) return;
+    if (ci >= p.n_chunks) return h;
     const uint32_t low = (uint32_t)(key & 0xFFFFFFull);
     const uint32_t vx = (low >> 16) & 63u, vy = (low >> 8) & 63u, vz = low & 63u;
     const ChunkK c = ld_chunk(p, ci);
-    const uint32_t id = block_at(p, c, vx, vy, vz);
-    const bool glass = (s.bits & SPEC_GLASS) && id == GLASS_ID;
-    if (id != WATER_ID && !glass) return;
-    const uint32_t nid = (uint32_t)(key >> 37) & 7u;
+    h.id = block_at(p, c, vx, vy, vz);
+    h.nid = (uint32_t)(key >> 37) & 7u;
     const uint32_t mx = (low >> 22) & 3u, my = (low >> 14) & 3u, mz = (low >> 6) & 3u;
-    float rd[3], n[3], hit[3];
-    ray_dir(f, (float)px + 0.5f, (float)py + 0.5f, rd[0], rd[1], rd[2]);
-    const float t = hit_t(f, c, vx, vy, vz, mx, my, mz, nid, rd);
+    ray_dir(f, (float)px + 0.5f, (float)py + 0.5f, h.rd[0], h.rd[1], h.rd[2]);
+    h.t = hit_t(f, c, vx, vy, vz, mx, my, mz, h.nid, h.rd);
 #pragma unroll
-    for (int k = 0; k < 3; k++) hit[k] = f.cam[k] + rd[k] * t;
-    normal_of(nid, n);
-    if (id == WATER_ID) {
-        const float refract_detail = 1.0f - wgsl_smoothstep(WATER_REFRACT_NEAR, WATER_REFRACT_FAR, t);
-        const float reflect_detail = 1.0f - wgsl_smoothstep(WATER_REFLECT_NEAR, WATER_REFLECT_FAR, t);
-        if ((f.flags & FLAG_WATER_REFRACT) && refract_detail > 0.0f) {
-            SecLeg &l = legs[0];
-            wgsl_refract(rd, n, 1.0f / WATER_IOR, l.rd);
-#pragma unroll
-            for (int k = 0; k < 3; k++) l.ro[k] = hit[k] + l.rd[k] * SURFACE_EPS;
-            l.max_dist = WATER_REFRACT_DIST;
-            l.sun_dist = (f.flags & FLAG_WATER_SHADOW_CUT) ? WATER_SHADOW_DIST : WATER_SHADOW_DIST_PRE41;
-            l.water = true;
-            want[0] = true;
-        }
-        if ((f.flags & FLAG_WATER_REFLECT) && reflect_detail > 0.0f) {
-            SecLeg &l = legs[1];
-#pragma unroll
-            for (int k = 0; k < 3; k++) l.ro[k] = hit[k] + n[k] * SURFACE_EPS;
-            wgsl_reflect(rd, n, l.rd);
-            l.max_dist = WATER_REFLECT_DIST;
-            l.sun_dist = f.shadow_dist;
-            l.water = true;
-            want[1] = true;
-        }
-    } else {
-        if (dot3o(n, rd) > 0.0f) {
-#pragma unroll
-            for (int k = 0; k < 3; k++) n[k] = -n[k];
-        }
-        const float dist = fmaxf(GLASS_TRANSMIT_DIST, f.far_);
-        SecLeg &l = legs[0];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            l.ro[k] = hit[k] + rd[k] * SURFACE_EPS;
-            l.rd[k] = rd[k];
-        }
-        l.max_dist = dist;
-        l.sun_dist = GLASS_SHADOW_DIST;
-        l.water = false;
-        want[0] = true;
-        if (s.bits & SPEC_GLASS_REFLECT) {
-            SecLeg &m = legs[1];
-#pragma unroll
-            for (int k = 0; k < 3; k++) m.ro[k] = hit[k] + n[k] * SURFACE_EPS;
-            wgsl_reflect(rd, n, m.rd);
-            m.max_dist = dist;
-            m.sun_dist = GLASS_SHADOW_DIST;
-            m.water = false;
-            want[1] = true;
-        }
+    for (int k = 0; k < 3; k++) h.hit[k] = f.cam[k] + h.rd[k] * h.t;
+    h.ok = true;
+    return h;
+}
+
+// The legs of one pixel's primary key (shade_water, resolve.wgsl:1078-1167, :1209-1220;
+// shade_glass, :1261-1310): legs[0] = refract (water) / transmit (glass) when want[0],
+// legs[1] = reflect. kDirs false: which legs exist, without the water reflect direction.
+template <bool kDirs>
+__device__ inline void secondary_legs(const Pools &p, const FrameK &f, const Spec &s, uint64_t key, uint32_t px,
+                                      uint32_t py, SecLeg legs[2], bool want[2]) {
+    want[0] = want[1] = false;
+    const KeyHit h = key_hit(p, f, key, px, py);
+    if (!h.ok) return;
+    const bool glass = (s.bits & SPEC_GLASS) && h.id == GLASS_ID;
+    if (h.id != WATER_ID && !glass) return;
+    if (h.id == WATER_ID) {
+        water_leg_pair<kDirs>(p, f, s, h.nid, h.hit, h.rd, h.t, legs, want);
+        return;
     }
+    float n[3];
+    normal_of(h.nid, n);
+    if (dot3o(n, h.rd) > 0.0f) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) n[k] = -n[k];
+    }
+    const float dist = fmaxf(GLASS_TRANSMIT_DIST, f.far_);
+    SecLeg &l = legs[0];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        l.ro[k] = h.hit[k] + h.rd[k] * SURFACE_EPS;
+        l.rd[k] = h.rd[k];
+    }
+    l.max_dist = dist;
+    l.sun_dist = GLASS_SHADOW_DIST;
+    l.water = false;
+    want[0] = true;
+    if (s.bits & SPEC_GLASS_REFLECT) {
+        SecLeg &m = legs[1];
+#pragma unroll
+        for (int k = 0; k < 3; k++) m.ro[k] = h.hit[k] + n[k] * SURFACE_EPS;
+        wgsl_reflect(h.rd, n, m.rd);
+        m.max_dist = dist;
+        m.sun_dist = GLASS_SHADOW_DIST;
+        m.water = false;
+        want[1] = true;
+    }
+}
+
+// The legs shade_water traces where a pixel's glass transmit leg ends on water
+// (shade_glass, resolve.wgsl:1281-1289): from p2 = the leg's origin + rd * its distance,
+// with the water face the leg hit and the distance t + the leg's; legs[0] refract, legs[1]
+// reflect (the pixel's records 2 and 3). leg_t and leg_normal come from its record 0.
+template <bool kDirs>
+__device__ inline void glass_water_legs(const Pools &p, const FrameK &f, const Spec &s, uint64_t key, uint32_t px,
+                                        uint32_t py, float leg_t, uint32_t leg_normal, SecLeg legs[2], bool want[2]) {
+    want[0] = want[1] = false;
+    const KeyHit h = key_hit(p, f, key, px, py);
+    if (!h.ok) return;
+    float p2[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) p2[k] = (h.hit[k] + h.rd[k] * SURFACE_EPS) + h.rd[k] * leg_t;
+    water_leg_pair<kDirs>(p, f, s, leg_normal, p2, h.rd, h.t + leg_t, legs, want);
 }
 
 }  // namespace vxd

@@ -406,6 +406,11 @@ FrameK make_frame(const vx_gpu_frame *g) {
     f.flags = g->flags;
     f.max_pairs = g->max_pairs;
     f.frame_index = g->frame_index;
+    f.time = g->time;
+    f.wave_amp = g->wave_amp;
+    f.wave_scale = g->wave_scale;
+    f.wave_speed = g->wave_speed;
+    f.wave_reflect_slope = g->wave_reflect_slope;
     std::memcpy(f.grid_min, g->grid_min, 12);
     std::memcpy(f.grid_dim, g->grid_dim, 12);
     return f;
@@ -422,8 +427,11 @@ int make_spec(uint32_t spec, uint32_t spec_hi, uint32_t frame_flags, Spec &s) {
     if (spec_hi & VX_FLAG_HI_EMITTER_WORLD) s.bits |= SPEC_EMITTER;
     if (!(spec_hi & VX_FLAG_HI_LEGACY_WATER)) s.bits |= SPEC_WATER_REPAIR;
     if (spec_hi & VX_FLAG_HI_GLASS_REFLECT) s.bits |= SPEC_GLASS_REFLECT;
-    if (((spec & VX_FLAG_WAVES) && (frame_flags & VX_FLAG_WAVES)) || (spec_hi & VX_FLAG_HI_WATER_LOOK))
-        s.bits |= SPEC_WAVY_WATER;
+    if (spec & VX_FLAG_WAVES) s.bits |= SPEC_WAVES;
+    if (spec & VX_FLAG_WAVE_ANISO) s.bits |= SPEC_WAVE_ANISO;
+    if (spec & VX_FLAG_WAVE_SHOAL) s.bits |= SPEC_WAVE_SHOAL;
+    if (spec & VX_FLAG_WAVE_FILL) s.bits |= SPEC_WAVE_FILL;
+    if (spec_hi & VX_FLAG_HI_WATER_LOOK) s.bits |= SPEC_WATER_LOOK;
     s.leaf_fill = (spec & VX_FLAG_LEAF_THIN) ? 0.62f : 0.72f;
     return VX_OK;
 }

@@ -314,9 +314,11 @@ class Batch:
         A.check(self._L.vx_batch_set_secondary(self._b, 1 if on else 0), "vx_batch_set_secondary")
 
     def read_secondary(self, i: int) -> np.ndarray:
-        """(W*H, 2) leg records of view i: [:, 0] refract / transmit, [:, 1] reflect."""
+        """(W*H, 4) leg records of view i: [:, 0] refract / transmit, [:, 1] reflect, and
+        where the glass transmit leg ends on water, that water's [:, 2] refract and [:, 3]
+        reflect legs."""
         w, h = self._size(i)
-        out = np.zeros((w * h, 2), dtype=A.SEC_DTYPE)
+        out = np.zeros((w * h, 4), dtype=A.SEC_DTYPE)
         A.check(self._L.vx_batch_read_secondary(self._b, i, A.ptr(out), out.size), "vx_batch_read_secondary")
         return out
 

@@ -207,14 +207,16 @@ class CameraParams:
     flags: int = 0
     jitter: tuple = (0.0, 0.0)
     frame_index: int = 0
+    anim_time: float = 0.0                  # frame.time (config.rs anim_time)
+    waves: tuple = (0.12, 32.0, 0.32, 0.0)  # wave amp, scale, speed, reflect slope (config.rs:318-321)
 
     def to_c(self) -> A.Camera:
         f, r, u = camera_basis(self.yaw, self.pitch)
         sd, dl = sun(self.time_of_day)
         return A.Camera((C.c_float * 3)(*self.pos), (C.c_float * 3)(*f), (C.c_float * 3)(*r), (C.c_float * 3)(*u),
                         math.radians(self.fov_deg), self.width, self.height, self.far,
-                        (C.c_float * 3)(*sd), dl, self.sea_level, 0.0, (C.c_float * 2)(*self.jitter),
-                        self.flags, self.frame_index)
+                        (C.c_float * 3)(*sd), dl, self.sea_level, self.anim_time, (C.c_float * 2)(*self.jitter),
+                        self.flags, self.frame_index, (C.c_float * 4)(*self.waves))
 
 
 _tls = threading.local()
