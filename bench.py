@@ -364,7 +364,11 @@ def run_serving(args):
     """The serving workload: the 2048x2048x512 world resident on the device, an open-loop
     Poisson stream of view requests (resolutions uniform over the four sizes, seeded
     poses) mixed with Poisson edit batches of 1000 random voxel set/clear ops, through
-    the continuous-batching server. Latency = completion - scheduled arrival."""
+    the continuous-batching server. Latency = completion - scheduled arrival. Camera
+    requests are submitted by the server's native replay driver. Then a rate sweep
+    (--serve-sweep): the same stream at increasing rates, up to the first whose p99
+    exceeds 5 ms, reporting the highest rate within it with its mean batch size and the
+    fraction of time a lane had a batch on the device."""
     from voxelcraft_menu_amd import serving as S
     from voxelcraft_menu_amd import workloads as W
     from voxelcraft_menu_amd.renderer import Renderer
@@ -377,10 +381,16 @@ def run_serving(args):
     S.run(world, r, wl, qps=200, duration_s=1.0, edit_rate=0, seed=99, device_lists=dev)  # warm-up stream
     res = S.run(world, r, wl, qps=args.serve_qps, duration_s=args.serve_seconds, edit_rate=args.serve_edit_rate,
                 seed=4, device_lists=dev)
-    r.close()
     res["workload"] = wl.name
     res["chunk_lists"] = "device" if dev else "host"
     _log(f"serving: p50 {res['p50_ms']} ms p99 {res['p99_ms']} ms at {args.serve_qps} qps")
+    rates = [float(x) for x in args.serve_sweep.split(",") if x.strip()] if dev else []
+    if rates:
+        sw = S.sweep(world, r, wl, rates, duration_s=args.serve_sweep_seconds, edit_rate=args.serve_edit_rate,
+                     p99_bound_ms=5.0, seed=5, log=lambda m: _log(m[:160]))
+        res["sweep"] = sw
+        _log(f"serving sweep: max {sw['max_qps_within_bound']} qps with p99 <= 5 ms")
+    r.close()
     return res
 
 
@@ -644,6 +654,9 @@ def main():
     ap.add_argument("--serve-qps", type=float, default=1000.0, help="offered view requests per second")
     ap.add_argument("--serve-seconds", type=float, default=4.0, help="seconds of request arrivals")
     ap.add_argument("--serve-edit-rate", type=float, default=2.0, help="edit batches (1000 ops) per second")
+    ap.add_argument("--serve-sweep", default="1000,2000,2500,3000,3500,4000,5000,6000,8000",
+                    help="comma-separated request rates of the sweep (empty: no sweep)")
+    ap.add_argument("--serve-sweep-seconds", type=float, default=3.0, help="seconds of arrivals per sweep rate")
     ap.add_argument("--serve-host-lists", action="store_true",
                     help="serving requests carry host-built chunk lists instead of cameras")
     ap.add_argument("--calibrate-copy", action="store_true",

@@ -436,8 +436,9 @@ int vx_serve_destroy(vx_serve *s);
 int vx_serve_submit(vx_serve *s, uint64_t id, const vx_view *view, double t_arrival);
 /* Camera requests (chunk lists built on the device, as vx_batch_set_cameras builds them):
  * vx_serve_set_chunk_table uploads the LOD-0 chunk table, only while no lane runs and no
- * camera request is pending (between vx_serve_barrier and vx_serve_resume, or before the
- * first one; VX_E_BUSY otherwise). vx_serve_submit_camera queues a view given by its frame
+ * camera request is pending ahead of a barrier (between vx_serve_barrier and
+ * vx_serve_resume, or before the first request; VX_E_BUSY otherwise); camera requests held
+ * by the barrier build their lists from the new table. vx_serve_submit_camera queues a view given by its frame
  * (vxw_pack_camera_frame) and frustum planes; its results are those of vx_serve_submit
  * with the host-built list for that camera. */
 int vx_serve_set_chunk_table(vx_serve *s, const vx_gpu_chunk *chunks, uint32_t n);
@@ -451,6 +452,27 @@ int vx_serve_release(vx_serve *s, uint64_t id);
 int vx_serve_barrier(vx_serve *s);
 int vx_serve_resume(vx_serve *s);
 int vx_serve_stats_get(vx_serve *s, vx_serve_stats *out);
+/* Native request replay (no counterpart in the reference, whose loop takes one request
+ * at a time, src/headless.rs:419-497): a driver thread submits camera requests at their
+ * scheduled arrival times (t0 + reqs[i].t on the vx_serve_clock, arrival-sorted; t0 0 =
+ * now) and a collector thread takes their completions and releases their outputs, so no
+ * caller thread sits on the arrival path. Request i gets id first_id + i. While a replay
+ * runs the caller submits and polls nothing, but may apply edit batches
+ * (vx_serve_barrier, vx_pool_sync, vx_serve_set_chunk_table, vx_serve_resume): requests
+ * arriving during the barrier are held and render the edited world. The requests and
+ * their frames are copied. One replay at a time (VX_E_BUSY). With VX_REPLAY_KEEP the
+ * outputs are not released: the caller reads and releases them after the join. */
+typedef struct vx_replay_req {
+    double t;                  /* arrival, seconds after t0 */
+    vx_camera_view view;
+} vx_replay_req;
+#define VX_REPLAY_KEEP 1u
+int vx_serve_replay_start(vx_serve *s, const vx_replay_req *reqs, uint32_t n, uint64_t first_id, double t0,
+                          uint32_t flags);
+int vx_serve_replay_progress(vx_serve *s, uint32_t *submitted, uint32_t *completed);
+/* Waits for every request of the replay: out[i] is request i's result (a submit error in
+ * its status, with t_done 0). n must be the replay's count. */
+int vx_serve_replay_join(vx_serve *s, vx_serve_result *out, uint32_t n);
 const char *vx_error_string(int code);
 int vx_device_name(char *buf, size_t cap);
 

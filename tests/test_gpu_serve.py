@@ -198,20 +198,73 @@ def test_camera_requests_equal_host_list_requests(small_world):
             srv.release(rid)
 
     run_round("before edits")
-    srv.barrier()
-    s, hi = W.spec_words(cams[0].flags, small_world)
-    srv.submit_camera(900, cams[0], s, hi)
-    with pytest.raises(RuntimeError):  # a camera request pending: the table cannot change
-        srv.set_chunk_table(snap.chunk_table())
-    srv.resume()
-    _drain(srv, [900])
-    srv.release(900)
     small_world.apply_edits(S.random_edits(np.random.default_rng(8), wl, 3000))
     srv.barrier()
+    s, hi = W.spec_words(cams[0].flags, small_world)
+    srv.submit_camera(900, cams[0], s, hi)  # held by the barrier: it must render the edited world
     r.sync_world(small_world)
     snap.take(small_world)
     srv.set_chunk_table(snap.chunk_table())
     srv.resume()
+    v = build_view(small_world, cams[0])
+    srv.submit(901, v, s, hi)
+    res = _drain(srv, [900, 901])
+    assert res[900]["status"] == 0 and res[901]["status"] == 0
+    a, b = _read(srv, 900, v), _read(srv, 901, v)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), "held camera request vs edited world"
+    srv.release(900)
+    srv.release(901)
     run_round("after edits")
+    srv.close()
+    r.close()
+
+
+def test_native_replay_with_an_edit_batch():
+    """The native replay driver submits camera requests on their schedule while an edit
+    batch is applied under the barrier: every request completes, requests launched before
+    the barrier render the world before the edit and those held by it the edited world,
+    each equal to the oracle's first frame of its view."""
+    import time
+
+    from voxelcraft_menu_amd import serving as S
+    from voxelcraft_menu_amd.world import ViewSnapshot
+
+    wl = W.WORKLOADS[0]
+    world, before = W.make_world(wl), W.make_world(wl)
+    cams = W.random_cameras(wl, 30, seed=71, resolutions=[(160, 90), (96, 64), (256, 144)])
+    r = Renderer(64, 64)
+    r.sync_world(world)
+    srv = Server(r, lanes=3, max_views=8, hbm_budget=512 << 20, mode=A.BATCH_FLAT)
+    snap = ViewSnapshot(world)
+    srv.set_chunk_table(snap.chunk_table())
+    specs = [W.spec_words(c.flags, world) for c in cams]
+    t0 = srv.replay_start(np.arange(len(cams)) * 0.002, cams, specs, first_id=100, keep=True)
+    while srv.clock() < t0 + 0.03:
+        time.sleep(0.001)
+    world.apply_edits(S.random_edits(np.random.default_rng(9), wl, 3000))
+    srv.barrier()
+    t_barrier = srv.clock()
+    r.sync_world(world)
+    snap.take(world)
+    srv.set_chunk_table(snap.chunk_table())
+    t_resume = srv.clock()
+    srv.resume()
+    res = srv.replay_join()
+    groups = {"before": 0, "after": 0}
+    for i, (c, x) in enumerate(zip(cams, res)):
+        assert x["status"] == 0 and x["id"] == 100 + i, x
+        assert x["t_arrival"] == pytest.approx(t0 + i * 0.002)
+        assert not (t_barrier < x["t_launch"] < t_resume), "a launch inside the barrier"
+        src = before if x["t_launch"] <= t_barrier else world
+        groups["before" if src is before else "after"] += 1
+        v = build_view(src, c)
+        got = _read(srv, 100 + i, v)
+        print(f"replay request {i} ({'before' if src is before else 'after'} the edit)", flush=True)
+        ref = H.oracle_frames(src, v, *specs[i], frames=1)[0]
+        assert np.array_equal(got[0], ref[0]), f"request {i}: {(got[0] != ref[0]).sum()} keys differ"
+        assert np.array_equal(got[1], ref[1]), f"request {i}: shadow words differ"
+        srv.release(100 + i)
+    assert groups["before"] > 0 and groups["after"] > 0, groups
+    snap.close()
     srv.close()
     r.close()

@@ -232,6 +232,11 @@ class CameraView(C.Structure):
     _fields_ = [("frame", C.c_void_p), ("spec", C.c_uint32), ("spec_hi", C.c_uint32), ("planes", C.c_float * 24)]
 
 
+class ReplayReq(C.Structure):
+    """vx_replay_req: a camera request of a native replay and its arrival offset."""
+    _fields_ = [("t", C.c_double), ("view", CameraView)]
+
+
 class BatchTuning(C.Structure):
     _fields_ = [("mode", C.c_uint32), ("occupancy", C.c_uint32), ("variant", C.c_uint32),
                 ("ordered", C.c_uint32), ("queue_chunk", C.c_uint32), ("waves_per_simd", C.c_uint32),
@@ -402,6 +407,9 @@ _sig_march = {
     "vx_serve_barrier": (C.c_int, [P]),
     "vx_serve_resume": (C.c_int, [P]),
     "vx_serve_stats_get": (C.c_int, [P, C.POINTER(ServeStats)]),
+    "vx_serve_replay_start": (C.c_int, [P, P, C.c_uint32, C.c_uint64, C.c_double, C.c_uint32]),
+    "vx_serve_replay_progress": (C.c_int, [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "vx_serve_replay_join": (C.c_int, [P, C.POINTER(ServeResult), C.c_uint32]),
     "vx_error_string": (C.c_char_p, [C.c_int]),
     "vx_device_name": (C.c_int, [C.c_char_p, C.c_size_t]),
     "vx_build_local": (C.c_int, [P, P, C.c_uint32, P, C.c_size_t]),

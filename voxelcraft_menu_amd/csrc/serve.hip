@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -149,6 +150,21 @@ struct vx_serve {
     vx_serve_stats stats{};
     DevBuf table;  // LOD-0 chunk records camera requests build their lists from
     uint32_t n_table = 0;
+    struct Replay *replay = nullptr;  // a running vx_serve_replay_start, if any
+};
+
+// A native request replay: the driver thread submits camera requests at their scheduled
+// arrival times, the collector thread takes completions and releases their outputs.
+struct Replay {
+    std::vector<vx_replay_req> reqs;    // copies, their frame pointers into `frames`
+    std::vector<vx_gpu_frame> frames;
+    std::vector<vx_serve_result> res;
+    uint64_t first_id = 0;
+    double t0 = 0;
+    bool keep = false;  // VX_REPLAY_KEEP: outputs stay until the caller releases them
+    std::atomic<uint32_t> submitted{0}, completed{0}, failed{0};
+    std::atomic<bool> driver_done{false}, stop{false};
+    std::thread driver, collector;
 };
 
 namespace {
@@ -356,6 +372,69 @@ void worker_main(vx_serve *s) {
     }
 }
 
+void replay_wait(Replay *R) {
+    if (R->driver.joinable()) R->driver.join();
+    if (R->collector.joinable()) R->collector.join();
+}
+
+// Sleeps to within 0.2 ms of `target`, then spins.
+void wait_until(double target) {
+    for (;;) {
+        const double left = target - now_s();
+        if (left <= 0) return;
+        if (left > 3e-4) std::this_thread::sleep_for(std::chrono::duration<double>(left - 2e-4));
+        else std::this_thread::yield();
+    }
+}
+
+void replay_driver(vx_serve *s, Replay *R) {
+    const uint32_t n = (uint32_t)R->reqs.size();
+    for (uint32_t i = 0; i < n && !R->stop; i++) {
+        const double t_arr = R->t0 + R->reqs[i].t;
+        wait_until(t_arr);
+        int rc;
+        // VX_E_BUSY: the chunk table changed between reading its size and queueing
+        while ((rc = vx_serve_submit_camera(s, R->first_id + i, &R->reqs[i].view, t_arr)) == VX_E_BUSY && !R->stop)
+            std::this_thread::yield();
+        if (rc) {
+            vx_serve_result &o = R->res[i];
+            o.id = R->first_id + i;
+            o.status = rc;
+            o.t_arrival = t_arr;
+            R->failed++;
+        } else {
+            R->submitted++;
+        }
+    }
+    R->driver_done = true;
+}
+
+void replay_collector(vx_serve *s, Replay *R) {
+    vx_serve_result buf[256];
+    const uint64_t n = R->reqs.size();
+    for (;;) {
+        uint32_t k = 0;
+        if (vx_serve_poll(s, buf, 256, &k, 2) != VX_OK) break;
+        for (uint32_t j = 0; j < k; j++) {
+            const uint64_t i = buf[j].id - R->first_id;
+            if (i < n) R->res[i] = buf[j];
+            if (!R->keep) (void)vx_serve_release(s, buf[j].id);
+            R->completed++;
+        }
+        if (R->driver_done && R->completed.load() >= R->submitted.load()) break;
+        if (R->stop && k == 0) {
+            std::lock_guard<std::mutex> lk(s->mu);
+            if (lanes_idle(s) && s->pending.empty()) break;
+        }
+    }
+}
+
+// Device footprint of a camera request whose list has room for a table of n records.
+uint64_t cam_bytes(const Req *r, uint32_t n) {
+    return r->units * PX_UNIT * 10 + (uint64_t)n * sizeof(vx_gpu_chunk) + VX_GRID_LEN * 4 + (uint64_t)r->tiles * 12 +
+           sizeof(ViewK) + sizeof(ListCam);
+}
+
 void free_req(vx_serve *s, Req *r) {
     if (r->state == RQ_DONE && r->units) {
         s->arena.release(r->px_off / PX_UNIT, r->units);
@@ -425,6 +504,12 @@ int vx_serve_create(vx_ctx *c, const vx_serve_desc *d, vx_serve **out) {
 
 int vx_serve_destroy(vx_serve *s) {
     if (!s) return VX_E_ARG;
+    if (s->replay) {
+        s->replay->stop = true;
+        replay_wait(s->replay);
+        delete s->replay;
+        s->replay = nullptr;
+    }
     {
         std::lock_guard<std::mutex> lk(s->mu);
         s->stop = true;
@@ -504,18 +589,26 @@ int vx_serve_set_chunk_table(vx_serve *s, const vx_gpu_chunk *chunks, uint32_t n
     for (uint32_t i = 0; i < n; i++)
         if (chunks[i].lod != 0u || chunks[i].fade != 1.0f || chunks[i].voxel_size != 1.0f) return VX_E_ARG;
     std::lock_guard<std::mutex> lk(s->mu);
-    // Only between vx_serve_barrier and vx_serve_resume, or before any camera request: no
-    // launch reads the table meanwhile.
+    // Only while no launch reads the table: between vx_serve_barrier and vx_serve_resume,
+    // or before any camera request. Camera requests held back by the barrier have not
+    // built their lists yet: they take the new table, with list room for its size.
     for (const Lane &L : s->lanes)
         if (L.busy) return VX_E_BUSY;
-    for (const Req *r : s->pending)  // their list room is the table size they were submitted with
-        if (r->cam) return VX_E_BUSY;
+    for (const Req *r : s->pending)
+        if (r->cam && r->seq < s->fence) return VX_E_BUSY;
+    for (const Req *r : s->pending)
+        if (r->cam && cam_bytes(r, n) > s->d.hbm_budget) return VX_E_OVERFLOW;
     HIPX(hipSetDevice(s->ctx->device));
     int rc;
     if ((rc = ensure(s->table, (size_t)std::max(n, 1u) * sizeof(vx_gpu_chunk)))) return rc;
     if (n) HIPX(hipMemcpyAsync(s->table.p, chunks, (size_t)n * sizeof(vx_gpu_chunk), hipMemcpyHostToDevice, s->io));
     HIPX(hipStreamSynchronize(s->io));
     s->n_table = n;
+    for (Req *r : s->pending) {  // not admitted yet: their bytes are not in s->used
+        if (!r->cam) continue;
+        r->list_recs = n;
+        r->bytes = cam_bytes(r, n);
+    }
     return VX_OK;
 }
 
@@ -551,8 +644,7 @@ int vx_serve_submit_camera(vx_serve *s, uint64_t id, const vx_camera_view *v, do
     r->tiles = fr.tiles[0] * fr.tiles[1];
     const uint64_t px = (uint64_t)r->w * r->h;
     r->units = (px + PX_UNIT - 1) / PX_UNIT;
-    r->bytes = r->units * PX_UNIT * 10 + (uint64_t)nt * sizeof(vx_gpu_chunk) + VX_GRID_LEN * 4 +
-               (uint64_t)r->tiles * 12 + sizeof(ViewK) + sizeof(ListCam);
+    r->bytes = cam_bytes(r, nt);
     r->t_submit = now_s();
     r->t_arrival = t_arrival > 0 ? t_arrival : r->t_submit;
     {
@@ -661,6 +753,58 @@ int vx_serve_resume(vx_serve *s) {
         s->fence = ~0ull;
     }
     s->cv_work.notify_all();
+    return VX_OK;
+}
+
+int vx_serve_replay_start(vx_serve *s, const vx_replay_req *reqs, uint32_t n, uint64_t first_id, double t0,
+                          uint32_t flags) {
+    if (!s || (!reqs && n) || (flags & ~VX_REPLAY_KEEP)) return VX_E_ARG;
+    if (s->replay) return VX_E_BUSY;
+    for (uint32_t i = 0; i < n; i++)
+        if (!reqs[i].view.frame || !(reqs[i].t >= 0.0) || (i && reqs[i].t < reqs[i - 1].t)) return VX_E_ARG;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        for (uint32_t i = 0; i < n; i++)
+            if (s->live.count(first_id + i)) return VX_E_ARG;
+    }
+    Replay *R = new (std::nothrow) Replay();
+    if (!R) return VX_E_OOM;
+    try {
+        R->reqs.assign(reqs, reqs + n);
+        R->frames.resize(n);
+        R->res.assign(n, vx_serve_result{});
+    } catch (...) {
+        delete R;
+        return VX_E_OOM;
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        R->frames[i] = *reqs[i].view.frame;
+        R->reqs[i].view.frame = &R->frames[i];
+    }
+    R->first_id = first_id;
+    R->keep = (flags & VX_REPLAY_KEEP) != 0u;
+    R->t0 = t0 > 0 ? t0 : now_s();
+    s->replay = R;
+    R->collector = std::thread(replay_collector, s, R);
+    R->driver = std::thread(replay_driver, s, R);
+    return VX_OK;
+}
+
+int vx_serve_replay_progress(vx_serve *s, uint32_t *submitted, uint32_t *completed) {
+    if (!s || !s->replay) return VX_E_ARG;
+    if (submitted) *submitted = s->replay->submitted + s->replay->failed;
+    if (completed) *completed = s->replay->completed;
+    return VX_OK;
+}
+
+int vx_serve_replay_join(vx_serve *s, vx_serve_result *out, uint32_t n) {
+    if (!s || !s->replay || (!out && n)) return VX_E_ARG;
+    Replay *R = s->replay;
+    if (n != R->reqs.size()) return VX_E_ARG;
+    replay_wait(R);
+    if (n) std::memcpy(out, R->res.data(), (size_t)n * sizeof(vx_serve_result));
+    delete R;
+    s->replay = nullptr;
     return VX_OK;
 }
 

@@ -437,6 +437,42 @@ class Server:
         A.check(self._L.vx_serve_stats_get(self._s, C.byref(st)), "vx_serve_stats_get")
         return {k: int(getattr(st, k)) for k, _ in A.ServeStats._fields_}
 
+    def replay_start(self, arrivals, cams, specs, first_id: int = 0, t0: float = 0.0, keep: bool = False) -> float:
+        """Starts a native replay: camera request i arrives at t0 + arrivals[i] (sorted,
+        seconds) with pipeline key specs[i]; a native driver thread submits them on
+        time and a native collector releases their outputs. Meanwhile submit nothing and
+        poll nothing; barrier / pool sync / set_chunk_table / resume are allowed. t0 0:
+        50 ms after the requests are prepared. keep: outputs stay until release().
+        Returns t0."""
+        from .world import camera_frame
+
+        n = len(cams)
+        reqs = (A.ReplayReq * n)()
+        frames = []
+        for i, (t, cam, (spec, hi)) in enumerate(zip(arrivals, cams, specs)):
+            fr, planes = camera_frame(cam)
+            frames.append(fr)
+            reqs[i].t = float(t)
+            reqs[i].view = A.CameraView(C.addressof(fr), spec, hi, (C.c_float * 24)(*planes))
+        if not t0:
+            t0 = self.clock() + 0.05
+        A.check(self._L.vx_serve_replay_start(self._s, reqs, n, first_id, t0, 1 if keep else 0),
+                "vx_serve_replay_start")
+        self._replay_n = n  # the library copied the requests and frames
+        return t0
+
+    def replay_progress(self) -> tuple[int, int]:
+        sub, done = C.c_uint32(), C.c_uint32()
+        A.check(self._L.vx_serve_replay_progress(self._s, C.byref(sub), C.byref(done)), "vx_serve_replay_progress")
+        return int(sub.value), int(done.value)
+
+    def replay_join(self) -> list[dict]:
+        """Waits for the replay; one result dict per request, in request order."""
+        n = self._replay_n
+        out = (A.ServeResult * max(n, 1))()
+        A.check(self._L.vx_serve_replay_join(self._s, out, n), "vx_serve_replay_join")
+        return [{k: getattr(out[i], k) for k, _ in A.ServeResult._fields_} for i in range(n)]
+
 
 def device_name() -> str:
     buf = C.create_string_buffer(256)
