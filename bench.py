@@ -251,6 +251,7 @@ def run_batch(args, out):
     dt1 = time.perf_counter() - t0
     out["single_view_1080p"] = {"value": round((wl1.width * wl1.height + sr) * args.steps / dt1 / 1e9, 4),
                                 "ms_per_step": round(dt1 / args.steps * 1e3, 4)}
+    out["resolve_1080p"] = run_resolve(r1, world, wl1, args)
     r1.close()
     r.close()
 
@@ -357,6 +358,42 @@ def run_moving(args, world, cams, specs, tuned, px):
            "deferred_pairs_per_step": dfr // args.steps, "recovered_pairs_per_step": rec // args.steps,
            "step": "device chunk lists for the moved cameras + batched render (Hi-Z carried)"}
     _log(f"moving camera: {res['ms_per_step']} ms/step ({res['render_ms_per_step']} ms render)")
+    return res
+
+
+def run_resolve(r1, world, wl1, args):
+    """Resolve lane 0 (vx_resolve) of the 1080p view: the workload's flags less what the
+    resolve does not restate (probes, biome tint, distant terrain shadows), sky tint,
+    sky specular and fog on, a synthetic 27-layer 16x16 atlas with its mip chain. The
+    march runs once; the resolve alone is timed over the steps."""
+    from voxelcraft_menu_amd import workloads as W
+    from voxelcraft_menu_amd.world import build_view
+
+    probes = 16777216 | 33554432 | 67108864 | 134217728 | 536870912 | 1073741824
+    cam = W.default_camera(wl1, flags=W.DEFAULT_FLAGS & ~(probes | 128 | 32768))
+    v = build_view(world, cam)
+    v.frame.ambient, v.frame.mip_bias, v.frame.fog_density, v.frame.fog_falloff = 0.05, 0.0, 0.002, 0.01
+    v.frame.fog_scatter, v.frame.fog_g, v.frame.fog_height = 0.3, 0.6, 80.0
+    spec, hi = W.spec_words(cam.flags, world)
+    hi = (hi | 1) & ~(64 | 8192 | 16384 | 65536)
+    rng = np.random.default_rng(0)
+    texels = sum(27 * (16 >> l) ** 2 for l in range(4))
+    faces = (np.arange(32 * 8, dtype=np.uint32) % 27) | ((np.arange(32 * 8, dtype=np.uint32) % 3) << 16)
+    r1.set_resolve_assets(rng.integers(0, 256, texels * 4, dtype=np.uint8), 16, 27, 4, faces)
+    r1.set_view(v)
+    r1.render(v.frame, spec, hi)
+    r1.resolve()
+    r1.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r1.resolve()
+    r1.sync()
+    dt = time.perf_counter() - t0
+    px = wl1.width * wl1.height
+    res = {"ms_per_frame": round(dt / args.steps * 1e3, 4), "Gpix_per_s": round(px * args.steps / dt / 1e9, 3),
+           "config": "1920x1080, workload flags less probes / biome tint / distant shadows; sky tint, "
+                     "sky specular and fog on; synthetic 16x16x27 atlas, 4 mips"}
+    _log(f"resolve 1080p: {res['ms_per_frame']} ms")
     return res
 
 

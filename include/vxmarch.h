@@ -23,6 +23,8 @@
  *   vx_pair_demand            Renderer::pair_demand          (src/render/mod.rs:1948-1975)
  *   vx_march_stats / read_heat Renderer::march_stats           (src/render/mod.rs:1977-2045)
  *   vx_timings                Readback per-pass timestamps   (src/render/timer.rs:150-176)
+ *   vx_resolve                the resolve pass, lane 0       (src/render/shaders/resolve.wgsl:1378-1466,
+ *                                                            dispatched at src/render/mod.rs:1853-1878)
  *   vx_batch_*                no counterpart: many views per launch (the reference renders
  *                             one view per render() call, src/headless.rs:680-837)
  *   vx_serve_*                no counterpart: continuous-batching request loop with
@@ -106,9 +108,12 @@ typedef struct vx_gpu_frame {
     uint32_t grid_dim[3];  float water_absorb;
     float jitter[2];       float taa_feedback;
     uint32_t frame_index;
-    float shading[12];     /* cloud_cover .. tint_freq_h (offsets 192-239), unused here */
+    float cloud_cover, cloud_height, cloud_scale, cloud_speed, godray_strength;  /* offset 192 */
+    uint32_t godray_steps; float godray_dist, cloud_shadow, tint_strength;
+    int32_t tint_seed;     float tint_freq_t, tint_freq_h;
     float wave_amp, wave_scale, wave_speed, wave_reflect_slope;  /* offsets 240-255: the wave field */
-    float shading_tail[8]; /* shaft_origin .. zenith_deep (offsets 256-287), unused here */
+    float shaft_origin[2], shaft_texel, shaft_soft, cloud_patch, cloud_relief;  /* offset 256 */
+    float haze_warm, zenith_deep;
     float prev_view_proj[16];
 } vx_gpu_frame;
 
@@ -239,6 +244,36 @@ int vx_read_heat(vx_ctx *ctx, uint32_t *out, size_t n);
 int vx_march_stats_get(vx_ctx *ctx, vx_march_stats *out);
 /* The same summary of a caller-held counter (no device work). */
 int vx_march_stats_compute(const uint32_t *heat, size_t n, uint32_t chunks, vx_march_stats *out);
+
+/* ---- Resolve, lane 0 ---------------------------------------------------------------
+ * The reference's resolve pass (src/render/shaders/resolve.wgsl:1378-1466, lane 0) over
+ * the last vx_render frame, into an RGBA32F colour target (W*H*4 floats): a FLAG_HEATMAP
+ * frame becomes heat(iterations / 300); a miss becomes the sky (sky_color without a cloud
+ * deck); an opaque hit is shaded from its key as shade_hit_compact_cached /
+ * shade_hit_legacy_cached do (resolve.wgsl:1537-2059: smooth light and AO from the 3x3
+ * neighbourhood in front of the face, the sun visibility of the shadow passes -- 0 where
+ * the shadow word holds a blocker distance, 1 elsewhere, shadow.wgsl:61-116 without soft
+ * shadows --, sky specular, the atlas texel at the footprint's mip level) and mixed with
+ * sky_base by the fog. Water and
+ * glass hits are lanes 1 and 2 of the reference: their pixels keep alpha 0 here. The
+ * frame's shading fields (fog_*, ambient, mip_bias, haze_warm, zenith_deep) are read as
+ * given. Views that need what is not restated return VX_E_UNSUPPORTED: probes, biome tint
+ * (FLAG_TINT with its spec bit), caustics, soft shadows, foliage-rich or canopy-relief albedo, distant
+ * terrain shadows, a cloud deck (cloud_cover > 0), terrain god-ray shafts
+ * (godray_strength > 0 with FLAG_TERRAIN_SHAFTS), the underwater view.
+ * vx_resolve_set_assets copies the atlas and block face table the reference's host builds
+ * (src/render/mod.rs:2159-2224, src/textures.rs; src/block.rs:96-102): RGBA8 sRGB texels,
+ * mip level 0 of every layer, then level 1, ..., level l being (size >> l)^2 texels a
+ * layer; 8 face words per block id (layer | perm class << 16). */
+typedef struct vx_resolve_assets {
+    const uint8_t *atlas;
+    uint32_t size, layers, mips;
+    const uint32_t *block_faces;
+    uint32_t n_faces;             /* words */
+} vx_resolve_assets;
+int vx_resolve_set_assets(vx_ctx *ctx, const vx_resolve_assets *a);
+int vx_resolve(vx_ctx *ctx);      /* asynchronous on the context stream */
+int vx_read_color(vx_ctx *ctx, float *rgba, size_t n);  /* n = W*H*4 */
 /* Device pointers of the current outputs (for callers that keep data on the GPU). */
 int vx_device_outputs(vx_ctx *ctx, uint64_t **vis, uint16_t **shadow);
 

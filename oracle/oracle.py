@@ -101,6 +101,15 @@ def _bind(L):
     L.vxo_water_reflect_dir.restype = None
     L.vxo_water_reflect_dir.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, C.POINTER(Spec), C.c_uint32, V, V,
                                         C.c_float, V]
+    L.vxo_log2.restype = C.c_float
+    L.vxo_log2.argtypes = [C.c_float]
+    L.vxo_exp2.restype = C.c_float
+    L.vxo_exp2.argtypes = [C.c_float]
+    L.vxo_pow.restype = C.c_float
+    L.vxo_pow.argtypes = [C.c_float, C.c_float]
+    L.vxo_resolve.restype = C.c_int
+    L.vxo_resolve.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, C.POINTER(RSpec), C.POINTER(ResolveAssets),
+                              V, V, V, V, C.POINTER(Stats), C.c_int]
     L.vxo_wave_field.restype = None
     L.vxo_wave_field.argtypes = [V, C.POINTER(Spec), C.c_float, C.c_float, C.c_float, V, C.c_float, C.c_int,
                                  C.c_float, V]
@@ -134,6 +143,49 @@ def planted(n: int, tmpdir: str):
 
 def _p(a):
     return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class RSpec(C.Structure):
+    """vxo_rspec: the resolve's override switches."""
+    _fields_ = [(k, C.c_uint32) for k in ("sky_specular", "lighting_repair", "sky_tint", "sky_cool", "sky_look",
+                                          "light_rgb", "emitter_gather", "shore_wet", "tex_variation", "glass")]
+
+
+class ResolveAssets(C.Structure):
+    _fields_ = [("atlas", C.c_void_p), ("size", C.c_uint32), ("layers", C.c_uint32), ("mips", C.c_uint32),
+                ("block_faces", C.c_void_p), ("n_faces", C.c_uint32)]
+
+
+def rspec_from(flags: int, spec_hi: int) -> RSpec:
+    """The resolve's override values for a (flags, spec_hi) pipeline key (src/render/mod.rs:
+    2355-2689)."""
+    return RSpec(sky_specular=1 if spec_hi & 1 else 0, lighting_repair=0 if spec_hi & 1048576 else 1,
+                 sky_tint=1 if flags & 2147483648 else 0, sky_cool=1 if spec_hi & 2048 else 0,
+                 sky_look=1 if spec_hi & 1024 else 0, light_rgb=1 if flags & 268435456 else 0,
+                 emitter_gather=1 if spec_hi & 4 else 0, shore_wet=1 if spec_hi & 8 else 0,
+                 tex_variation=1 if flags & 16384 else 0, glass=1 if flags & 131072 else 0)
+
+
+def resolve(scene: "Scene", frame_bytes: bytes, rspec: RSpec, atlas: np.ndarray, size: int, layers: int, mips: int,
+            block_faces: np.ndarray, vis: np.ndarray, sun: np.ndarray | None, heat: np.ndarray | None = None,
+            threads: int = 0) -> np.ndarray:
+    """Resolve lane 0 of a frame: (H, W, 4) float32 (alpha 0 where lane 0 writes nothing)."""
+    f = np.frombuffer(frame_bytes, dtype=np.uint8).copy()
+    res = np.frombuffer(frame_bytes[80:88], dtype=np.uint32)
+    w, h = int(res[0]), int(res[1])
+    at = np.ascontiguousarray(atlas, dtype=np.uint8).reshape(-1)
+    fc = np.ascontiguousarray(block_faces, dtype=np.uint32).reshape(-1)
+    a = ResolveAssets(at.ctypes.data, size, layers, mips, fc.ctypes.data, fc.size)
+    v = np.ascontiguousarray(vis, dtype=np.uint64)
+    s = None if sun is None else np.ascontiguousarray(sun, dtype=np.uint16)
+    hz = None if heat is None else np.ascontiguousarray(heat, dtype=np.uint32)
+    out = np.zeros((h, w, 4), dtype=np.float32)
+    st = Stats()
+    rc = lib().vxo_resolve(C.byref(scene.pools), _p(scene.chunks), scene.n_chunks, _p(scene.grid), _p(f),
+                           C.byref(rspec), C.byref(a), _p(v), _p(s), _p(hz), _p(out), C.byref(st), threads)
+    if rc != 0:
+        raise RuntimeError("oracle resolve rejected its arguments")
+    return out
 
 
 def spec_from(flags: int, spec_hi: int) -> Spec:

@@ -1571,3 +1571,5 @@ uint16_t vxo_f32_to_f16(float x) {
     if (rem > 0x1000u || (rem == 0x1000u && (half & 1u))) half++;
     return (uint16_t)(sign | half);
 }
+
+#include "vxresolve.inc"

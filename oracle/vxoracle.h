@@ -65,9 +65,13 @@ typedef struct vxo_frame {
     uint32_t grid_dim[3];  float water_absorb;
     float jitter[2];       float taa_feedback;
     uint32_t frame_index;
-    float shading[12];     /* cloud_cover .. tint_freq_h: not read by these passes */
+    float cloud_cover, cloud_height, cloud_scale, cloud_speed, godray_strength;  /* offset 192 */
+    uint32_t godray_steps; float godray_dist, cloud_shadow, tint_strength;
+    int32_t tint_seed;     float tint_freq_t, tint_freq_h;
     float wave_amp, wave_scale, wave_speed, wave_reflect_slope;  /* offsets 240-255 */
-    float rest[24];        /* shaft_origin .. prev_view_proj: not read */
+    float shaft_origin[2], shaft_texel, shaft_soft, cloud_patch, cloud_relief;  /* offset 256 */
+    float haze_warm, zenith_deep;
+    float prev_view_proj[16];  /* not read */
 } vxo_frame;
 
 typedef struct vxo_chunk {
@@ -219,6 +223,38 @@ int vxo_decode(const vxo_pools *p, const vxo_chunk *chunks, uint32_t n_chunks,
                const vxo_frame *f, const uint64_t *vis, vxo_decoded *out, int threads);
 
 int vxo_hardware_threads(void);
+
+/* ---- Resolve, lane 0 (resolve.wgsl:1378-1466), restated in vxresolve.inc ---- */
+/* log2 / exp2 as the library evaluates them; pow(x, y) = exp2(y * log2(x)) (WGSL). */
+float vxo_log2(float x);
+float vxo_exp2(float x);
+float vxo_pow(float x, float y);
+
+/* The override switches the resolve reads, from the pipeline key (src/render/mod.rs:
+ * 2355-2689): SPEC_SKY_SPECULAR, SPEC_LIGHTING_REPAIR, SPEC_SKY_TINT, SPEC_SKY_COOL,
+ * SPEC_SKY_LOOK, SPEC_LIGHT_RGB, SPEC_EMITTER_GATHER, SPEC_SHORE_WET,
+ * SPEC_TEX_VARIATION, SPEC_GLASS. */
+typedef struct vxo_rspec {
+    uint32_t sky_specular, lighting_repair, sky_tint, sky_cool, sky_look, light_rgb, emitter_gather,
+        shore_wet, tex_variation, glass;
+} vxo_rspec;
+
+/* The caller's atlas (RGBA8 sRGB: mip level 0 of every layer, then level 1, ...; level l
+ * is (size >> l)^2 texels a layer) and block_faces table (8 face words per block id:
+ * layer | perm class << 16, src/block.rs:96-102). */
+typedef struct vxo_resolve_assets {
+    const uint8_t *atlas;
+    uint32_t size, layers, mips;
+    const uint32_t *block_faces;
+    uint32_t n_faces;
+} vxo_resolve_assets;
+
+/* out[4 * pix]: the lane-0 colour (rgb, alpha 1), or all zero where lane 0 writes
+ * nothing (water and glass pixels). sun: the shadow pass's R16F words; heat: the
+ * iteration counter of a FLAG_HEATMAP frame (either may be NULL when not read). */
+int vxo_resolve(const vxo_pools *p, const vxo_chunk *chunks, uint32_t n_chunks, const uint32_t *grid,
+                const vxo_frame *f, const vxo_rspec *r, const vxo_resolve_assets *a, const uint64_t *vis,
+                const uint16_t *sun, const uint32_t *heat, float *out, vxo_stats *stats, int threads);
 
 #ifdef __cplusplus
 }

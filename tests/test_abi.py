@@ -34,7 +34,8 @@ def test_python_bindings_cover_the_headers():
 def test_struct_layouts_match_the_reference():
     # GpuFrame offsets asserted in src/render/mod.rs:458-466 and validation/check_compact.py:45
     assert C.sizeof(A.Frame) == 352
-    assert A.Frame.cloud_cover.offset == 192 if hasattr(A.Frame, "cloud_cover") else A.Frame.shading.offset == 192
+    assert A.Frame.cloud_cover.offset == 192 and A.Frame.wave_amp.offset == 240
+    assert A.Frame.shaft_origin.offset == 256 and A.Frame.zenith_deep.offset == 284
     assert A.Frame.prev_view_proj.offset == 288
     assert A.Frame.grid_min.offset == 144 and A.Frame.sea_level.offset == 156
     assert A.Frame.grid_dim.offset == 160 and A.Frame.jitter.offset == 176 and A.Frame.frame_index.offset == 188

@@ -79,10 +79,15 @@ class Frame(C.Structure):
         ("grid_dim", C.c_uint32 * 3), ("water_absorb", C.c_float),
         ("jitter", C.c_float * 2), ("taa_feedback", C.c_float),
         ("frame_index", C.c_uint32),
-        ("shading", C.c_float * 12),
+        ("cloud_cover", C.c_float), ("cloud_height", C.c_float), ("cloud_scale", C.c_float),
+        ("cloud_speed", C.c_float), ("godray_strength", C.c_float), ("godray_steps", C.c_uint32),
+        ("godray_dist", C.c_float), ("cloud_shadow", C.c_float), ("tint_strength", C.c_float),
+        ("tint_seed", C.c_int32), ("tint_freq_t", C.c_float), ("tint_freq_h", C.c_float),
         ("wave_amp", C.c_float), ("wave_scale", C.c_float), ("wave_speed", C.c_float),
         ("wave_reflect_slope", C.c_float),
-        ("shading_tail", C.c_float * 8),
+        ("shaft_origin", C.c_float * 2), ("shaft_texel", C.c_float), ("shaft_soft", C.c_float),
+        ("cloud_patch", C.c_float), ("cloud_relief", C.c_float), ("haze_warm", C.c_float),
+        ("zenith_deep", C.c_float),
         ("prev_view_proj", C.c_float * 16),
     ]
 
@@ -230,6 +235,13 @@ class ViewDesc(C.Structure):
 class CameraView(C.Structure):
     """vx_camera_view: a batch view given by its camera (chunk list built on the device)."""
     _fields_ = [("frame", C.c_void_p), ("spec", C.c_uint32), ("spec_hi", C.c_uint32), ("planes", C.c_float * 24)]
+
+
+class ResolveAssets(C.Structure):
+    """vx_resolve_assets: the caller's atlas (RGBA8 sRGB, mip level 0 of every layer, then
+    level 1, ...) and block face table (8 words per block id)."""
+    _fields_ = [("atlas", C.c_void_p), ("size", C.c_uint32), ("layers", C.c_uint32), ("mips", C.c_uint32),
+                ("block_faces", C.c_void_p), ("n_faces", C.c_uint32)]
 
 
 class ReplayReq(C.Structure):
@@ -388,6 +400,9 @@ _sig_march = {
     "vx_batch_timings_sum": (C.c_int, [P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_uint32)]),
     "vx_batch_device_outputs": (C.c_int, [P, C.c_uint32, P, P]),
     "vx_read_heat": (C.c_int, [P, P, C.c_size_t]),
+    "vx_resolve_set_assets": (C.c_int, [P, P]),
+    "vx_resolve": (C.c_int, [P]),
+    "vx_read_color": (C.c_int, [P, P, C.c_size_t]),
     "vx_march_stats_get": (C.c_int, [P, C.POINTER(MarchStats)]),
     "vx_march_stats_compute": (C.c_int, [P, C.c_size_t, C.c_uint32, C.POINTER(MarchStats)]),
     "vx_batch_set_secondary": (C.c_int, [P, C.c_uint32]),

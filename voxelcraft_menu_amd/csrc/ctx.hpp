@@ -84,6 +84,13 @@ struct vx_ctx {
     // Device chunk build (vx_build_local): dense input staging, per-leaf and per-L1
     // scratch, record output staging; grown on demand, kept for the next call.
     vxh::DevBuf build_dense, build_leaf, build_words, build_l1, build_out;
+    // The last vx_render's frame record and pipeline key (the resolve reads them).
+    vx_gpu_frame gframe{};
+    uint32_t spec_word = 0, spec_hi_word = 0;
+    // Resolve: the caller's atlas and face table, and the colour target.
+    vxh::DevBuf atlas, faces, color;
+    uint32_t atlas_size = 0, atlas_layers = 0, atlas_mips = 0, n_faces = 0;
+    bool color_valid = false;
 };
 
 namespace vxh {

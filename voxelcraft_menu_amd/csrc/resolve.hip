@@ -1,0 +1,668 @@
+// resolve.hip -- resolve lane 0 (resolve.wgsl:1378-1466) over the last vx_render frame:
+// the heatmap frame, sky pixels, and opaque pixels shaded from the key with the sun
+// visibility of the shadow passes, mixed with the sky by the fog transmittance
+// (vx_resolve).
+//
+// One wave per 8x8 tile, one lane per pixel, as the march. Each lane re-derives its hit
+// from the key (hit_t, block_at), gathers the 3x3 light/occlusion neighbourhood in front
+// of the face (gather_face inside the chunk, world_sample across chunks), and samples the
+// caller's atlas. The configuration and its f32 steps are the oracle's (oracle/
+// vxresolve.inc); the elementary functions are the library's own (log2, exp2, pow =
+// exp2(y log2 x), exp = wave_exp). Water and glass pixels belong to lanes 1 and 2: lane 0
+// leaves them at alpha 0.
+#include "ctx.hpp"
+
+namespace vxd {
+
+constexpr uint32_t FLAG_AO = 8u, FLAG_TEX_VARIATION = 16384u, SAND_ID = 4u, LIGHT_OPEN_SKY = 0xF000u,
+                   ATTR_HAS_FOLIAGE = 4u;
+constexpr float AO_STRENGTH = 0.25f, SKY_SPEC_GAIN = 0.5f, WET_BAND = 1.5f, WET_DARK = 0.62f, SKY_GRAD_EXP = 0.55f;
+constexpr uint32_t PERM_NONE = 0u, PERM_D4 = 2u;
+
+// Resolve switches (pipeline-key overrides, src/render/mod.rs:2355-2689).
+constexpr uint32_t RS_SKY_SPECULAR = 1u, RS_LIGHTING_REPAIR = 2u, RS_SKY_TINT = 4u, RS_SKY_COOL = 8u, RS_SKY_LOOK = 16u,
+                   RS_LIGHT_RGB = 32u, RS_EMITTER_GATHER = 64u, RS_SHORE_WET = 128u, RS_TEX_VARIATION = 256u,
+                   RS_GLASS = 512u;
+
+struct ResolveK {
+    Pools p;
+    FrameK f;
+    float fog_density, fog_falloff, ambient, fog_scatter, fog_g, fog_height, mip_bias, haze_warm, zenith_deep;
+    uint32_t bits;  // RS_*
+    const uint32_t *__restrict__ atlas;  // RGBA8 texels
+    uint32_t size, layers, mips;
+    const uint32_t *__restrict__ faces;
+    uint32_t n_faces;
+    const unsigned long long *__restrict__ vis;
+    const uint16_t *__restrict__ sun;
+    const uint32_t *__restrict__ heat;
+    float4 *__restrict__ out;
+};
+
+// ---- elementary functions (the oracle's vxo_log2 / vxo_exp2 / vxo_pow) ----
+__device__ __forceinline__ float r_log2(float x) {
+    if (x != x || x < 0.0f) return __uint_as_float(0x7FC00000u);
+    if (x == 0.0f) return -__builtin_inff();
+    if (__builtin_isinf(x)) return x;
+    uint32_t u = __float_as_uint(x);
+    int32_t e = 0;
+    if (u < 0x00800000u) {
+        x = x * 0x1p24f;
+        u = __float_as_uint(x);
+        e = -24;
+    }
+    e += (int32_t)(u >> 23) - 127;
+    float m = __uint_as_float((u & 0x007FFFFFu) | 0x3F800000u);
+    if (m > 1.41421356f) {
+        m = m * 0.5f;
+        e += 1;
+    }
+    const float s = (m - 1.0f) / (m + 1.0f);
+    const float s2 = s * s;
+    float p = 0x1.3b13b2p-4f;
+    p = p * s2 + 0x1.745d18p-4f;
+    p = p * s2 + 0x1.c71c72p-4f;
+    p = p * s2 + 0x1.24924ap-3f;
+    p = p * s2 + 0x1.99999ap-3f;
+    p = p * s2 + 0x1.555556p-2f;
+    p = p * s2 + 1.0f;
+    const float ln_m = 2.0f * s * p;
+    return (float)e + ln_m * 0x1.715476p0f;
+}
+
+__device__ __forceinline__ float r_exp2(float x) {
+    if (x != x) return x;
+    if (x < -150.0f) return 0.0f;
+    if (x > 128.0f) return __builtin_inff();
+    const float k = rintf(x);
+    const float r = (x - k) * 0x1.62e43p-1f;
+    float e = 0x1.a01a02p-13f;
+    e = e * r + 0x1.6c16c2p-10f;
+    e = e * r + 0x1.111112p-7f;
+    e = e * r + 0x1.555556p-5f;
+    e = e * r + 0x1.555556p-3f;
+    e = e * r + 0.5f;
+    e = e * r + 1.0f;
+    e = e * r + 1.0f;
+    return ldexpf(e, (int)k);
+}
+
+__device__ __forceinline__ float r_pow(float x, float y) { return r_exp2(y * r_log2(x)); }
+__device__ __forceinline__ float r_mix(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+__device__ __forceinline__ float r_clamp(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+
+__device__ __forceinline__ float srgb_decode(uint32_t u) {
+    const float c = (float)u / 255.0f;
+    if (c <= 0.04045f) return c / 12.92f;
+    return r_pow((c + 0.055f) / 1.055f, 2.4f);
+}
+
+// ---- atlas (textureSampleLevel with the atlas sampler, as the oracle restates it) ----
+__device__ __forceinline__ void atlas_texel(const ResolveK &k, uint32_t level, uint32_t layer, int32_t x, int32_t y,
+                                            float out[4]) {
+    size_t off = 0;
+    for (uint32_t l = 0; l < level; l++) off += (size_t)k.layers * (k.size >> l) * (k.size >> l);
+    const uint32_t dim = k.size >> level;
+    const uint32_t w = k.atlas[off + ((size_t)layer * dim + (uint32_t)y) * dim + (uint32_t)x];
+    out[0] = srgb_decode(w & 0xFFu);
+    out[1] = srgb_decode((w >> 8) & 0xFFu);
+    out[2] = srgb_decode((w >> 16) & 0xFFu);
+    out[3] = (float)(w >> 24) / 255.0f;
+}
+
+__device__ __forceinline__ int32_t clamp_texel(int32_t i, int32_t dim) { return i < 0 ? 0 : (i >= dim ? dim - 1 : i); }
+
+__device__ inline void atlas_bilinear(const ResolveK &k, uint32_t level, uint32_t layer, const float uv[2],
+                                      float out[4]) {
+    const int32_t dim = (int32_t)(k.size >> level);
+    const float x = uv[0] * (float)dim - 0.5f, y = uv[1] * (float)dim - 0.5f;
+    const float fx = floorf(x), fy = floorf(y);
+    const float ax = x - fx, ay = y - fy;
+    const int32_t x0 = clamp_texel((int32_t)fx, dim), x1 = clamp_texel((int32_t)fx + 1, dim);
+    const int32_t y0 = clamp_texel((int32_t)fy, dim), y1 = clamp_texel((int32_t)fy + 1, dim);
+    float t00[4], t10[4], t01[4], t11[4];
+    atlas_texel(k, level, layer, x0, y0, t00);
+    atlas_texel(k, level, layer, x1, y0, t10);
+    atlas_texel(k, level, layer, x0, y1, t01);
+    atlas_texel(k, level, layer, x1, y1, t11);
+#pragma unroll
+    for (int q = 0; q < 4; q++) out[q] = r_mix(r_mix(t00[q], t10[q], ax), r_mix(t01[q], t11[q], ax), ay);
+}
+
+__device__ inline void atlas_sample(const ResolveK &k, uint32_t layer, const float uv[2], float lod, float out[4]) {
+    if (layer >= k.layers) {
+        out[0] = out[1] = out[2] = out[3] = 0.0f;
+        return;
+    }
+    if (!(lod > 0.0f)) {
+        const int32_t dim = (int32_t)k.size;
+        const int32_t x = clamp_texel((int32_t)floorf(uv[0] * (float)dim), dim);
+        const int32_t y = clamp_texel((int32_t)floorf(uv[1] * (float)dim), dim);
+        atlas_texel(k, 0, layer, x, y, out);
+        return;
+    }
+    const float top = (float)(k.mips - 1u);
+    const float l = fminf(lod, top);
+    const float l0 = floorf(l);
+    const float fr = l - l0;
+    const uint32_t i0 = (uint32_t)l0, i1 = i0 + 1u < k.mips ? i0 + 1u : i0;
+    float c0[4], c1[4];
+    atlas_bilinear(k, i0, layer, uv, c0);
+    atlas_bilinear(k, i1, layer, uv, c1);
+#pragma unroll
+    for (int q = 0; q < 4; q++) out[q] = r_mix(c0[q], c1[q], fr);
+}
+
+// ---- sky (common.wgsl:1351-1413) ----
+__device__ __forceinline__ float hg_gain(float cos_t, float g) {
+    const float g2 = g * g;
+    const float d = 1.0f + g2 - 2.0f * g * cos_t;
+    return (1.0f - g2) / r_pow(fmaxf(d, 1e-4f), 1.5f);
+}
+
+__device__ inline void sky_base(const ResolveK &k, const float rd[3], float shaft, float c[3]) {
+    constexpr float SCATTER[3] = {1.0000f, 0.4770f, 0.1474f};
+    constexpr float NH[3] = {0.0049f, 0.0060f, 0.0134f}, DH[3] = {0.4480f, 0.6210f, 0.8900f};
+    constexpr float NZ[3] = {0.0008f, 0.0015f, 0.0039f}, DZ[3] = {0.0637f, 0.2140f, 0.8276f};
+    constexpr float CH[3] = {0.4074f, 0.6057f, 0.9150f}, CZ[3] = {0.0385f, 0.1892f, 0.8600f};
+    constexpr float SUN_C[3] = {1.0f, 0.8276f, 0.5225f}, DEEP[3] = {0.80f, 0.90f, 1.16f};
+    const bool cool = (k.bits & RS_SKY_COOL) != 0u, look = (k.bits & RS_SKY_LOOK) != 0u;
+    const float day = k.f.daylight;
+    const float up = r_clamp(rd[1], 0.0f, 1.0f);
+    const float gp = r_pow(up, SKY_GRAD_EXP);
+    const float sd = dot3o(rd, k.f.sun);
+    const float disc = wgsl_smoothstep(0.9975f, 0.9990f, sd);
+    const float lobe_k = hg_gain(sd, k.fog_g) * k.fog_scatter * k.f.daylight;
+    float band = 0.0f, toward = 0.0f, deep_t = 0.0f;
+    const bool warm = look && k.haze_warm > 0.0f, zen = look && k.zenith_deep > 0.0f;
+    if (warm) {
+        band = r_pow(1.0f - up, 3.0f);
+        toward = 0.35f + 0.65f * r_pow(fmaxf(sd, 0.0f), 4.0f);
+    }
+    if (zen) deep_t = k.zenith_deep * r_pow(up, SKY_GRAD_EXP) * day;
+    const float w = wgsl_smoothstep(0.0f, -0.35f, rd[1]);
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const float hz = r_mix(NH[q], cool ? CH[q] : DH[q], day);
+        const float zn = r_mix(NZ[q], cool ? CZ[q] : DZ[q], day);
+        float v = r_mix(hz, zn, gp);
+        v = v + SUN_C[q] * disc * fmaxf(day, 0.2f);
+        v = v + (SCATTER[q] * lobe_k) * shaft;
+        if (warm) v = v + SCATTER[q] * (band * toward * k.haze_warm * day * 0.22f);
+        if (zen) v = v * r_mix(1.0f, DEEP[q], deep_t);
+        c[q] = r_mix(v, hz * 0.26f, w);
+    }
+}
+
+// transmittance_from (common.wgsl:1332-1342)
+__device__ __forceinline__ float transmittance_from(const ResolveK &k, float y0, float s, const float rd[3]) {
+    const float lambda = k.fog_falloff;
+    const float x = r_clamp(lambda * s * rd[1], -60.0f, 60.0f);
+    float path = s;
+    if (fabsf(x) > 1e-4f) path = s * (1.0f - wave_exp(-x)) / x;
+    const float rho = k.fog_density * wave_exp(-r_clamp(lambda * (y0 - k.fog_height), -60.0f, 60.0f));
+    return wave_exp(-r_clamp(rho * path, 0.0f, 60.0f));
+}
+
+// ---- light (common.wgsl:977-1097, resolve.wgsl:1-90) ----
+__device__ __forceinline__ float light_curve(float level) {
+    const float l = level / 15.0f;
+    return l * l * (0.6f + 0.4f * l);
+}
+__device__ __forceinline__ float light_sky(uint32_t p) { return (float)(p >> 12); }
+__device__ __forceinline__ void light_block_rgb(uint32_t p, float o[3]) {
+    o[0] = (float)((p >> 8) & 0xFu);
+    o[1] = (float)((p >> 4) & 0xFu);
+    o[2] = (float)(p & 0xFu);
+}
+__device__ __forceinline__ float light_block_level(uint32_t p) {
+    float c[3];
+    light_block_rgb(p, c);
+    return fmaxf(c[0], fmaxf(c[1], c[2]));
+}
+__device__ __forceinline__ void light_curve3(const float lv[3], float o[3]) {
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const float l = lv[q] / 15.0f;
+        o[q] = l * l * (0.6f + 0.4f * l);
+    }
+}
+
+// The light fields of a chunk record (light_base, light_flags: words 18 and 19).
+__device__ __forceinline__ uint2 ld_chunk_light(const Pools &p, uint32_t ci) {
+    if (ci >= p.n_chunks) ci = 0;
+    const uint32_t *w = reinterpret_cast<const uint32_t *>(p.chunks + ci);
+    return make_uint2(w[18], w[19]);
+}
+
+__device__ inline uint32_t light_at(const Pools &p, uint2 lt, int32_t x, int32_t y, int32_t z) {
+    if (lt.y & 1u) return lt.x;
+    if (x < 0 || y < 0 || z < 0 || x > 63 || y > 63 || z > 63) return LIGHT_OPEN_SKY;
+    const uint32_t ux = (uint32_t)x, uy = (uint32_t)y, uz = (uint32_t)z;
+    const uint32_t cell = (ux >> 2) | ((uy >> 2) << 4) | ((uz >> 2) << 8);
+    const uint32_t entry = ld_brick(p, lt.x + cell);
+    if (entry & UNIFORM_BIT) return entry & 0xFFFFu;
+    const uint32_t vb = cell_bit(ux & 3u, uy & 3u, uz & 3u);
+    const uint32_t w = ld_brick(p, entry + (vb >> 1));
+    return (w >> ((vb & 1u) * 16u)) & 0xFFFFu;
+}
+
+__device__ __forceinline__ bool is_tuft(uint32_t b) { return b == 18u || b == 24u || b == 25u; }
+
+// voxel_occludes (common.wgsl:942-975)
+__device__ inline bool voxel_occludes(const Pools &p, const ChunkK &c, int32_t x, int32_t y, int32_t z) {
+    if (!voxel_occupied(p, c, x, y, z)) return false;
+    if (c.attr_flags & (ATTR_HAS_WATER | ATTR_HAS_FOLIAGE)) {
+        const uint32_t b = block_at(p, c, (uint32_t)x, (uint32_t)y, (uint32_t)z);
+        return b != WATER_ID && !is_tuft(b);
+    }
+    return true;
+}
+
+// world_sample (common.wgsl:1014-1032): (light word, occludes)
+__device__ inline uint2 world_sample(const Pools &p, const FrameK &f, const float pt[3]) {
+    bool ok;
+    const uint32_t idx = grid_index(f, (int32_t)floorf(pt[0] / 64.0f), (int32_t)floorf(pt[1] / 64.0f),
+                                    (int32_t)floorf(pt[2] / 64.0f), ok);
+    if (!ok) return make_uint2(LIGHT_OPEN_SKY, 0u);
+    const uint32_t ci = ld_grid(p, f, idx);
+    if (ci == NO_CHUNK || ci >= p.n_chunks) return make_uint2(LIGHT_OPEN_SKY, 0u);
+    const ChunkK c = ld_chunk(p, ci);
+    const int32_t x = (int32_t)floorf((pt[0] - c.origin[0]) / c.vs), y = (int32_t)floorf((pt[1] - c.origin[1]) / c.vs),
+                  z = (int32_t)floorf((pt[2] - c.origin[2]) / c.vs);
+    return make_uint2(light_at(p, ld_chunk_light(p, ci), x, y, z), voxel_occludes(p, c, x, y, z) ? 1u : 0u);
+}
+
+// sky_ambient_tint and surface_sky_fill (resolve.wgsl:1-5, :56-89)
+__device__ inline void surface_sky_fill(const ResolveK &k, const float n[3], float o[3]) {
+    constexpr float SKY_TINT[3] = {0.6038f, 0.7084f, 1.0000f};
+    constexpr float REF[3] = {0.14658824f, 0.30178431f, 0.84105882f};
+    constexpr float SCATTER[3] = {1.0000f, 0.4770f, 0.1474f};
+    constexpr float NH[3] = {0.0049f, 0.0060f, 0.0134f}, DH[3] = {0.4480f, 0.6210f, 0.8900f};
+    constexpr float NZ[3] = {0.0008f, 0.0015f, 0.0039f}, DZ[3] = {0.0637f, 0.2140f, 0.8276f};
+    if (!(k.bits & RS_SKY_TINT)) {
+#pragma unroll
+        for (int q = 0; q < 3; q++) o[q] = SKY_TINT[q];
+    } else {
+        const float day = k.f.daylight;
+        const float w = n[1] < -0.5f ? 0.0f : (n[1] > 0.5f ? 0.78431373f : 0.58578900f);
+        const float h = k.fog_scatter * day * 4.0f * fmaxf(dot3o(n, k.f.sun), 0.0f);
+        float t[3];
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const float c = r_mix(r_mix(NH[q], DH[q], day), r_mix(NZ[q], DZ[q], day), w) + SCATTER[q] * h;
+            t[q] = SKY_TINT[q] * r_pow(c / REF[q], 2.5f);
+        }
+        const float luma = 0.2126f * t[0] + 0.7152f * t[1] + 0.0722f * t[2];
+        const float s = 0.70721556f / fmaxf(luma, 1e-6f);
+#pragma unroll
+        for (int q = 0; q < 3; q++) o[q] = t[q] * s;
+    }
+    if (!(k.bits & RS_LIGHTING_REPAIR)) return;
+    const float m = 0.60f * r_mix(0.35f, 1.0f, r_clamp(n[1] * 0.5f + 0.5f, 0.0f, 1.0f));
+#pragma unroll
+    for (int q = 0; q < 3; q++) o[q] = o[q] * m;
+}
+
+__device__ __forceinline__ float schlick_ground(float cos_i) {
+    const float m = r_clamp(1.0f - cos_i, 0.0f, 1.0f);
+    const float m2 = m * m;
+    const float top = fmaxf(1.0f - 0.9f, 0.04f);
+    return 0.04f + (top - 0.04f) * m2 * m2 * m;
+}
+
+__device__ __forceinline__ uint32_t perm_key(const float b[3], uint32_t nid) {
+    const int32_t c0 = (int32_t)floorf(b[0]), c1 = (int32_t)floorf(b[1]), c2 = (int32_t)floorf(b[2]);
+    return hash_u32(((uint32_t)c0 * 0x9e3779b9u) ^ ((uint32_t)c1 * 0x85ebca6bu) ^ ((uint32_t)c2 * 0xc2b2ae35u) ^
+                    (nid * 0x27d4eb2fu));
+}
+
+// shade_hit_compact_cached / shade_hit_legacy_cached, smooth light, do_spec, faces only
+// (resolve.wgsl:1537-2059).
+__device__ inline void shade_opaque(const ResolveK &k, uint32_t ci, uint32_t vx, uint32_t vy, uint32_t vz, uint32_t id,
+                                    uint32_t nid, const float hit[3], const float rd[3], float dist, float lit_word,
+                                    float out[3]) {
+    const Pools &p = k.p;
+    const FrameK &f = k.f;
+    const ChunkK ch = ld_chunk(p, ci);
+    const uint2 clight = ld_chunk_light(p, ci);
+    const float vs = ch.vs;
+    float n[3];
+    normal_of(nid, n);
+    float spec_pre[3] = {0.0f, 0.0f, 0.0f};
+    if (k.bits & RS_SKY_SPECULAR) {
+        float m[3], sb[3];
+        wgsl_reflect(rd, n, m);
+        sky_base(k, m, 1.0f, sb);
+        const float nr[3] = {-rd[0], -rd[1], -rd[2]};
+        const float g = schlick_ground(r_clamp(dot3o(n, nr), 0.0f, 1.0f));
+#pragma unroll
+        for (int q = 0; q < 3; q++) spec_pre[q] = sb[q] * g;
+    }
+    const float grazing = fmaxf(fabsf(dot3o(n, rd)), 0.25f);
+    const float footprint = fmaxf(dist, 1e-3f) * 2.0f * f.tan_half_fov / (float)f.res[1] * 16.0f / vs / grazing;
+    const float lod = r_clamp(r_log2(fmaxf(footprint, 1e-4f)) + k.mip_bias, 0.0f, 3.0f);
+    const uint32_t axis = nid >> 1;
+    const uint32_t v[3] = {vx, vy, vz};
+    float box_min[3], local[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) box_min[q] = ch.origin[q] + (float)v[q] * vs;
+#pragma unroll
+    for (int q = 0; q < 3; q++) local[q] = r_clamp((hit[q] - box_min[q]) / vs, 0.0f, 1.0f);
+    float fa, fb;
+    int32_t ta[3] = {0, 0, 0}, tb[3] = {0, 0, 0};
+    if (axis == 0u) { fa = local[1]; fb = local[2]; ta[1] = 1; tb[2] = 1; }
+    else if (axis == 1u) { fa = local[0]; fb = local[2]; ta[0] = 1; tb[2] = 1; }
+    else { fa = local[0]; fb = local[1]; ta[0] = 1; tb[1] = 1; }
+    float sky_tint[3];
+    surface_sky_fill(k, n, sky_tint);
+    int32_t air_v[3];
+    bool local_only = true;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        air_v[q] = (int32_t)v[q] + (int32_t)n[q];
+        const int32_t span = abs(ta[q]) + abs(tb[q]);
+        if (!(air_v[q] - span >= 0 && air_v[q] + span <= 63)) local_only = false;
+    }
+    float sky_n[9], blk_n[9], open_n[9];
+    uint32_t light_w[9];
+    uint32_t any_blk = 0u;
+    const bool has_water = (ch.attr_flags & ATTR_HAS_WATER) != 0u;
+    for (uint32_t i = 0; i < 9u; i++) {
+        const int32_t da = (int32_t)(i % 3u) - 1, db = (int32_t)(i / 3u) - 1;
+        uint2 nb;
+        if (local_only) {  // gather_face (common.wgsl:1034-1097)
+            const int32_t x = air_v[0] + ta[0] * da + tb[0] * db, y = air_v[1] + ta[1] * da + tb[1] * db,
+                          z = air_v[2] + ta[2] * da + tb[2] * db;
+            bool solid = voxel_occupied(p, ch, x, y, z);
+            if (solid && has_water) solid = block_at(p, ch, (uint32_t)x, (uint32_t)y, (uint32_t)z) != WATER_ID;
+            nb = make_uint2((clight.y & 1u) ? clight.x : light_at(p, clight, x, y, z), solid ? 1u : 0u);
+        } else {
+            float pt[3];
+#pragma unroll
+            for (int q = 0; q < 3; q++)
+                pt[q] = ((box_min[q] + 0.5f * vs) + n[q] * vs) + ((float)ta[q] * vs) * ((float)(i % 3u) - 1.0f) +
+                        ((float)tb[q] * vs) * ((float)(i / 3u) - 1.0f);
+            nb = world_sample(p, f, pt);
+        }
+        light_w[i] = nb.x;
+        sky_n[i] = light_curve(light_sky(nb.x)) * f.daylight;
+        blk_n[i] = 0.0f;
+        if (k.bits & RS_LIGHT_RGB) {
+            if (k.bits & RS_EMITTER_GATHER) any_blk = any_blk | (nb.x & 0xFFFu);
+        } else {
+            blk_n[i] = light_curve(light_block_level(nb.x));
+        }
+        open_n[i] = 1.0f - (float)nb.y;
+    }
+    const bool rgb = (k.bits & RS_LIGHT_RGB) != 0u, gather = rgb && (k.bits & RS_EMITTER_GATHER) && any_blk != 0u;
+    float blk4[3] = {0.0f, 0.0f, 0.0f};
+    if (gather) {
+        float lv[3];
+        light_block_rgb(light_w[4], lv);
+        light_curve3(lv, blk4);
+    }
+    const bool ao_on = (f.flags & FLAG_AO) != 0u;
+    float amb_c[4][3], sky_c[4], ao_c[4];
+#pragma unroll
+    for (uint32_t kk = 0; kk < 4u; kk++) {
+        const uint32_t ia = (uint32_t)(4 + ((int32_t)(kk & 1u) * 2 - 1));
+        const uint32_t ib = (uint32_t)(4 + ((int32_t)(kk >> 1) * 2 - 1) * 3);
+        const uint32_t ic = ia + ib - 4u;
+        const float inv = 1.0f / (1.0f + open_n[ia] + open_n[ib] + open_n[ic]);
+        const float sk = (sky_n[4] + sky_n[ia] * open_n[ia] + sky_n[ib] * open_n[ib] + sky_n[ic] * open_n[ic]) * inv;
+        sky_c[kk] = sk;
+        if (rgb) {
+            if (gather) {
+                float ba[3], bb[3], bc[3], lv[3], bkv[3];
+                light_block_rgb(light_w[ia], lv);
+                light_curve3(lv, ba);
+                light_block_rgb(light_w[ib], lv);
+                light_curve3(lv, bb);
+                light_block_rgb(light_w[ic], lv);
+                light_curve3(lv, bc);
+#pragma unroll
+                for (int q = 0; q < 3; q++)
+                    bkv[q] = (blk4[q] + ba[q] * open_n[ia] + bb[q] * open_n[ib] + bc[q] * open_n[ic]) * inv;
+                const float bk = fmaxf(bkv[0], fmaxf(bkv[1], bkv[2]));
+                const float t = bk / fmaxf(sk + bk, 1e-4f);
+                const float m = fmaxf(sk, bk);
+#pragma unroll
+                for (int q = 0; q < 3; q++) amb_c[kk][q] = r_mix(sky_tint[q], bkv[q] / fmaxf(bk, 1e-4f), t) * m;
+            } else {
+#pragma unroll
+                for (int q = 0; q < 3; q++) amb_c[kk][q] = sky_tint[q] * sk;
+            }
+        } else {
+            constexpr float BLOCK_TINT[3] = {1.0000f, 0.6500f, 0.3200f};
+            const float bk = (blk_n[4] + blk_n[ia] * open_n[ia] + blk_n[ib] * open_n[ib] + blk_n[ic] * open_n[ic]) * inv;
+            const float t = bk / fmaxf(sk + bk, 1e-4f);
+            const float m = fmaxf(sk, bk);
+#pragma unroll
+            for (int q = 0; q < 3; q++) amb_c[kk][q] = r_mix(sky_tint[q], BLOCK_TINT[q], t) * m;
+        }
+        const float s1 = 1.0f - open_n[ia];
+        const float s2 = 1.0f - open_n[ib];
+        const float occ = (s1 > 0.5f && s2 > 0.5f) ? 3.0f : s1 + s2 + 1.0f - open_n[ic];
+        ao_c[kk] = ao_on ? 1.0f - occ * AO_STRENGTH : 1.0f;
+    }
+    const float ao = r_mix(r_mix(ao_c[0], ao_c[1], fa), r_mix(ao_c[2], ao_c[3], fa), fb);
+    const float sky_sm = r_mix(r_mix(sky_c[0], sky_c[1], fa), r_mix(sky_c[2], sky_c[3], fa), fb);
+    float direct = 0.0f;
+    const float ndl = dot3o(n, f.sun);
+    if (ndl > 0.0f && f.daylight > 0.01f) {
+        float lit = 1.0f;
+        if (f.flags & VX_FLAG_SHADOWS) lit = lit_word;
+        direct = lit * ndl * sky_sm;
+    }
+    constexpr float FACE_SHADE[8] = {0.62f, 0.62f, 0.50f, 1.00f, 0.80f, 0.80f, 0.90f, 0.90f};
+    const bool repair = (k.bits & RS_LIGHTING_REPAIR) != 0u;
+    const float shade = repair ? 1.0f : FACE_SHADE[nid];
+    const float sun_k = direct * (repair ? 0.78f : 0.52f);
+    constexpr float SUN_RGB[3] = {1.0000f, 0.8469f, 0.5647f}, FLOOR_RGB[3] = {0.85f, 0.90f, 1.00f};
+    float light[3], spec_rgb[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const float amb = r_mix(r_mix(amb_c[0][q], amb_c[1][q], fa), r_mix(amb_c[2][q], amb_c[3][q], fa), fb);
+        const float ambient_rgb = amb * 0.50f;
+        const float sun_rgb = SUN_RGB[q] * sun_k;
+        const float floor_rgb = FLOOR_RGB[q] * k.ambient;
+        spec_rgb[q] = spec_pre[q] * (sky_sm * SKY_SPEC_GAIN);
+        light[q] = repair ? ambient_rgb * ao * shade + sun_rgb + floor_rgb : ((ambient_rgb + sun_rgb) * ao + floor_rgb) * shade;
+    }
+    const uint32_t fi = id * 8u + nid;
+    const uint32_t face_word = fi < k.n_faces ? k.faces[fi] : 0u;
+    float uv[2];
+    if (axis == 0u) { uv[0] = local[2]; uv[1] = 1.0f - local[1]; }
+    else if (axis == 1u) { uv[0] = local[0]; uv[1] = local[2]; }
+    else { uv[0] = local[0]; uv[1] = 1.0f - local[1]; }
+    if (nid == 1u || nid == 4u) uv[0] = 1.0f - uv[0];
+    if ((k.bits & RS_TEX_VARIATION) && (f.flags & FLAG_TEX_VARIATION)) {  // permute_uv (resolve.wgsl:157-164)
+        const uint32_t pclass = face_word >> 16, h = perm_key(box_min, nid);
+        const bool d4 = pclass == PERM_D4;
+        float p0 = uv[0], p1 = uv[1];
+        if (d4 && (h & 1u)) { p0 = uv[1]; p1 = uv[0]; }
+        if (pclass != PERM_NONE && (h & 2u)) p0 = 1.0f - p0;
+        if (d4 && (h & 4u)) p1 = 1.0f - p1;
+        uv[0] = p0;
+        uv[1] = p1;
+    }
+    float texel[4];
+    atlas_sample(k, face_word & 0xFFFFu, uv, lod, texel);
+    float albedo[3] = {texel[0], texel[1], texel[2]};
+    if ((k.bits & RS_SHORE_WET) && id == SAND_ID) {
+        const float wet = 1.0f - wgsl_smoothstep(f.sea_level, f.sea_level + WET_BAND, hit[1]);
+        const float m = r_mix(1.0f, WET_DARK, wet);
+#pragma unroll
+        for (int q = 0; q < 3; q++) albedo[q] = albedo[q] * m;
+    }
+    if (repair && id == GLOWSTONE_ID) {
+        constexpr float GLOW[3] = {1.0f, 0.7111f, 0.3024f};
+#pragma unroll
+        for (int q = 0; q < 3; q++) out[q] = albedo[q] * (GLOW[q] * 1.5f + light[q] * 0.15f);
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < 3; q++) out[q] = albedo[q] * (light[q] * 1.0f) + spec_rgb[q];
+}
+
+// heat (resolve.wgsl:36-44)
+__device__ __forceinline__ void heat_rgb(float v, float o[3]) {
+    const float t = r_clamp(v, 0.0f, 1.0f);
+    const float c[3] = {wgsl_smoothstep(0.35f, 0.85f, t),
+                        wgsl_smoothstep(0.0f, 0.45f, t) - wgsl_smoothstep(0.65f, 1.0f, t),
+                        1.0f - wgsl_smoothstep(0.0f, 0.45f, t)};
+#pragma unroll
+    for (int q = 0; q < 3; q++) o[q] = r_pow(c[q], 2.2f);
+}
+
+__device__ __forceinline__ float f16_word(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
+
+__global__ __launch_bounds__(64) void k_resolve(ResolveK k) {
+    const FrameK &f = k.f;
+    const uint32_t tile = blockIdx.x;
+    if (tile >= f.tiles[0] * f.tiles[1]) return;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t px = (tile % f.tiles[0]) * 8u + (lane & 7u), py = (tile / f.tiles[0]) * 8u + (lane >> 3);
+    if (px >= f.res[0] || py >= f.res[1]) return;
+    const size_t pix = (size_t)py * f.res[0] + px;
+    float rd[3], color[3];
+    ray_dir(f, (float)px + 0.5f, (float)py + 0.5f, rd[0], rd[1], rd[2]);
+    if (f.flags & VX_FLAG_HEATMAP) {
+        heat_rgb((float)(k.heat ? k.heat[pix] : 0u) / 300.0f, color);
+        k.out[pix] = make_float4(color[0], color[1], color[2], 1.0f);
+        return;
+    }
+    const uint64_t key = k.vis[pix];
+    if (key == 0ull) {
+        sky_base(k, rd, 1.0f, color);
+    } else {
+        const KeyHit h = key_hit(k.p, f, key, px, py);
+        if (!h.ok || h.id == WATER_ID || ((k.bits & RS_GLASS) && h.id == GLASS_ID)) {
+            k.out[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            return;
+        }
+        const uint32_t low = (uint32_t)(key & 0xFFFFFFull);
+        const uint32_t ci = (uint32_t)(key >> 24) & 0x1FFFu;
+        // the sun visibility of shadow_horizontal / shadow_vertical without soft shadows
+        // (shadow.wgsl:61-116): clamp(1 - occ, 0, 1), occ = blocker distance > 0
+        const float d = f16_word(k.sun ? k.sun[pix] : (uint16_t)0);
+        const float occ = d > 0.0f ? 1.0f : 0.0f;
+        shade_opaque(k, ci, (low >> 16) & 63u, (low >> 8) & 63u, low & 63u, h.id, h.nid, h.hit, h.rd, h.t,
+                     r_clamp(1.0f - occ, 0.0f, 1.0f), color);
+        const float alpha = 1.0f - transmittance_from(k, f.cam[1] + 0.0f * h.rd[1], h.t - 0.0f, h.rd);
+        float sky[3];
+        sky_base(k, h.rd, 1.0f, sky);
+#pragma unroll
+        for (int q = 0; q < 3; q++) color[q] = r_mix(color[q], sky[q], alpha);
+    }
+    k.out[pix] = make_float4(color[0], color[1], color[2], 1.0f);
+}
+
+}  // namespace vxd
+
+using namespace vxd;
+
+#define HIPX VX_HIPX
+
+namespace {
+
+// The resolve switches of a pipeline key, or VX_E_UNSUPPORTED for what is not restated.
+int resolve_bits(uint32_t spec, uint32_t spec_hi, const vx_gpu_frame &g, uint32_t &bits) {
+    if (g.flags & VX_FLAG_HEATMAP) {
+        bits = 0;
+        return VX_OK;
+    }
+    constexpr uint32_t PROBES = 16777216u | 33554432u | 67108864u | 134217728u | 536870912u | 1073741824u;
+    if ((g.flags & VX_FLAG_UNDERWATER) || (spec & PROBES) || (spec & VX_FLAG_FOLIAGE)) return VX_E_UNSUPPORTED;
+    if ((spec & 128u) && (g.flags & 128u)) return VX_E_UNSUPPORTED;        // biome tint
+    if ((spec & 32768u) && (g.flags & 32768u)) return VX_E_UNSUPPORTED;    // distant terrain shadows
+    if (spec_hi & (64u | 8192u | 16384u | 65536u)) return VX_E_UNSUPPORTED;  // caustics, foliage rich, canopy relief,
+                                                                              // soft shadows
+    if (g.cloud_cover > 0.0f) return VX_E_UNSUPPORTED;                      // cloud deck
+    const bool terrain_shafts = (spec & 8192u) && (g.flags & 8192u);
+    if (g.godray_strength > 0.0f && terrain_shafts) return VX_E_UNSUPPORTED;
+    bits = 0;
+    if (spec_hi & 1u) bits |= RS_SKY_SPECULAR;
+    if (!(spec_hi & 1048576u)) bits |= RS_LIGHTING_REPAIR;
+    if (spec & 2147483648u) bits |= RS_SKY_TINT;
+    if (spec_hi & 2048u) bits |= RS_SKY_COOL;
+    if (spec_hi & 1024u) bits |= RS_SKY_LOOK;
+    if (spec & 268435456u) bits |= RS_LIGHT_RGB;
+    if (spec_hi & VX_FLAG_HI_EMITTER_WORLD) bits |= RS_EMITTER_GATHER;
+    if (spec_hi & 8u) bits |= RS_SHORE_WET;
+    if (spec & 16384u) bits |= RS_TEX_VARIATION;
+    if (spec & VX_FLAG_GLASS) bits |= RS_GLASS;
+    return VX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vx_resolve_set_assets(vx_ctx *c, const vx_resolve_assets *a) {
+    if (!c || !a || !a->atlas || !a->block_faces || a->size == 0 || a->size > 4096 || a->layers == 0 ||
+        a->layers > 65536 || a->mips == 0 || a->mips > 13 || (a->size >> (a->mips - 1u)) == 0 || a->n_faces == 0)
+        return VX_E_ARG;
+    size_t texels = 0;
+    for (uint32_t l = 0; l < a->mips; l++) texels += (size_t)a->layers * (a->size >> l) * (a->size >> l);
+    HIPX(hipSetDevice(c->device));
+    int rc;
+    if ((rc = vxh::ensure(c->atlas, texels * 4)) || (rc = vxh::ensure(c->faces, (size_t)a->n_faces * 4))) return rc;
+    HIPX(hipMemcpyAsync(c->atlas.p, a->atlas, texels * 4, hipMemcpyHostToDevice, c->stream));
+    HIPX(hipMemcpyAsync(c->faces.p, a->block_faces, (size_t)a->n_faces * 4, hipMemcpyHostToDevice, c->stream));
+    HIPX(hipStreamSynchronize(c->stream));
+    c->atlas_size = a->size;
+    c->atlas_layers = a->layers;
+    c->atlas_mips = a->mips;
+    c->n_faces = a->n_faces;
+    return VX_OK;
+}
+
+int vx_resolve(vx_ctx *c) {
+    if (!c) return VX_E_ARG;
+    if (!c->have_frame || !c->atlas_size) return VX_E_ARG;
+    const vx_gpu_frame &g = c->gframe;
+    uint32_t bits = 0;
+    int rc = resolve_bits(c->spec_word, c->spec_hi_word, g, bits);
+    if (rc) return rc;
+    HIPX(hipSetDevice(c->device));
+    if ((rc = vxh::ensure(c->color, (size_t)c->w * c->h * 16))) return rc;
+    ResolveK k;
+    k.p = vxh::make_pools(c);
+    k.f = c->frame;
+    k.fog_density = g.fog_density;
+    k.fog_falloff = g.fog_falloff;
+    k.ambient = g.ambient;
+    k.fog_scatter = g.fog_scatter;
+    k.fog_g = g.fog_g;
+    k.fog_height = g.fog_height;
+    k.mip_bias = g.mip_bias;
+    k.haze_warm = g.haze_warm;
+    k.zenith_deep = g.zenith_deep;
+    k.bits = bits;
+    k.atlas = (const uint32_t *)c->atlas.p;
+    k.size = c->atlas_size;
+    k.layers = c->atlas_layers;
+    k.mips = c->atlas_mips;
+    k.faces = (const uint32_t *)c->faces.p;
+    k.n_faces = c->n_faces;
+    k.vis = (const unsigned long long *)c->vis.p;
+    k.sun = (const uint16_t *)c->shadow.p;
+    k.heat = c->heat_valid ? (const uint32_t *)c->heat.p : nullptr;
+    k.out = (float4 *)c->color.p;
+    k_resolve<<<c->tx * c->ty, 64, 0, c->stream>>>(k);
+    VX_LAUNCHED();
+    c->color_valid = true;
+    return VX_OK;
+}
+
+int vx_read_color(vx_ctx *c, float *rgba, size_t n) {
+    if (!c || !rgba || n != (size_t)c->w * c->h * 4 || !c->color_valid) return VX_E_ARG;
+    HIPX(hipSetDevice(c->device));
+    HIPX(hipMemcpyAsync(rgba, c->color.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPX(hipStreamSynchronize(c->stream));
+    return VX_OK;
+}
+
+}  // extern "C"

@@ -233,7 +233,8 @@ int vx_ctx_destroy(vx_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     DevBuf *bufs[] = {&c->pool[0], &c->pool[1],   &c->pool[2], &c->chunks,      &c->grid,     &c->vis,
                       &c->shadow,  &c->hiz,       &c->pairs,   &c->counters,    &c->cost,     &c->order,
-                      &c->heat,    &c->build_dense, &c->build_leaf, &c->build_words, &c->build_l1, &c->build_out};
+                      &c->heat,    &c->build_dense, &c->build_leaf, &c->build_words, &c->build_l1, &c->build_out,
+                      &c->atlas,   &c->faces,     &c->color};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto &set : c->ring)
@@ -543,8 +544,12 @@ int vx_render(vx_ctx *c, const vx_gpu_frame *g, uint32_t spec, uint32_t spec_hi)
     HIPX(hipGetLastError());
     c->frame = f;
     c->spec = s;
+    c->gframe = *g;
+    c->spec_word = spec;
+    c->spec_hi_word = spec_hi;
     c->have_frame = true;
     c->heat_valid = heatmap;
+    c->color_valid = false;
     return VX_OK;
 }
 

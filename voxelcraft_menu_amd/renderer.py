@@ -139,6 +139,25 @@ class Renderer:
         A.check(self._L.vx_read_decoded(self._c, A.ptr(out), w * h), "vx_read_decoded")
         return out
 
+    def set_resolve_assets(self, atlas: np.ndarray, size: int, layers: int, mips: int, block_faces: np.ndarray):
+        """The atlas (RGBA8 sRGB bytes: mip level 0 of every layer, then level 1, ...) and
+        the block face table (8 u32 words per block id) the resolve samples."""
+        at = np.ascontiguousarray(atlas, dtype=np.uint8).reshape(-1)
+        fc = np.ascontiguousarray(block_faces, dtype=np.uint32).reshape(-1)
+        a = A.ResolveAssets(at.ctypes.data, size, layers, mips, fc.ctypes.data, fc.size)
+        A.check(self._L.vx_resolve_set_assets(self._c, C.byref(a)), "vx_resolve_set_assets")
+
+    def resolve(self):
+        """Resolve lane 0 (resolve.wgsl:1378-1466) of the last rendered frame."""
+        A.check(self._L.vx_resolve(self._c), "vx_resolve")
+
+    def read_color(self) -> np.ndarray:
+        """(H, W, 4) float32 colour of the last resolve (alpha 0: water / glass pixels)."""
+        w, h = self.size
+        out = np.zeros((h, w, 4), dtype=np.float32)
+        A.check(self._L.vx_read_color(self._c, A.ptr(out), out.size), "vx_read_color")
+        return out
+
     def read_heat(self) -> np.ndarray:
         """Marched iterations per pixel of the last FLAG_HEATMAP frame."""
         w, h = self.size
