@@ -209,8 +209,10 @@ int vx_pool_mirror_get(vx_ctx *ctx, int pool, vx_pool_mirror *out);
 int vx_pool_read(vx_ctx *ctx, int pool, void *out, size_t bytes);
 int vx_set_chunks(vx_ctx *ctx, const vx_gpu_chunk *chunks, uint32_t n_total, const uint32_t *grid);
 /* spec = frame flags & SPEC_MASK, spec_hi = spec_hi word (the pipeline key of
- * Renderer::render, src/render/mod.rs:1597). Foliage tufts and the Snell bend need
- * the texture atlas / refract and return VX_E_UNSUPPORTED. */
+ * Renderer::render, src/render/mod.rs:1597). Foliage tufts need the texture atlas and
+ * return VX_E_UNSUPPORTED. Under water with VX_FLAG_HI_SNELL_BEND, a ray that misses a
+ * chunk going up is refracted at the sea surface and marched again through that chunk
+ * (march.wgsl:23-45). */
 int vx_render(vx_ctx *ctx, const vx_gpu_frame *frame, uint32_t spec, uint32_t spec_hi);
 int vx_sync(vx_ctx *ctx);
 int vx_read_vis(vx_ctx *ctx, uint64_t *out, size_t n);

@@ -26,7 +26,7 @@ class Spec(C.Structure):
     _fields_ = [("leaf_cutout", C.c_uint32), ("glass", C.c_uint32), ("full_march", C.c_uint32),
                 ("emitter_gather", C.c_uint32), ("water_repair", C.c_uint32), ("leaf_fill", C.c_float),
                 ("glass_reflect", C.c_uint32), ("waves", C.c_uint32), ("wave_aniso", C.c_uint32),
-                ("wave_shoal", C.c_uint32), ("wave_fill", C.c_uint32)]
+                ("wave_shoal", C.c_uint32), ("wave_fill", C.c_uint32), ("snell_bend", C.c_uint32)]
 
 
 _STAT_FIELDS = ["pairs_select", "pairs_deferred", "pairs_recover", "rays_primary", "rays_shadow",
@@ -202,6 +202,7 @@ def spec_from(flags: int, spec_hi: int) -> Spec:
         wave_aniso=1 if flags & 1024 else 0,
         wave_shoal=1 if flags & 2048 else 0,
         wave_fill=1 if flags & 4096 else 0,
+        snell_bend=1 if spec_hi & 256 else 0,
     )
 
 

@@ -1147,6 +1147,39 @@ static void ctx_init(ctx *c, const job *j) {
 /* March every pixel of one tile against chunk ci (march.wgsl:1-55, Snell bend off).
  * tileheat, when non-NULL, receives each pixel's h.iters (the FLAG_HEATMAP atomicAdd
  * into dbg, march.wgsl:51-53). */
+/* The Snell bend (march.wgsl:23-45): an underwater camera's ray that misses chunk ci going
+ * up is refracted at the sea surface (from water: eta = WATER_IOR about (0, -1, 0)) and
+ * marched again through ci from SURFACE_EPS past the surface, skipping water, within
+ * far - t_surf; a hit replaces the miss with t = t_surf + its distance, iterations added. */
+static void snell_bend(ctx *c, uint32_t ci, const float rd[3], vxo_hit *h) {
+    const vxo_frame *f = c->f;
+    const float depth = f->sea_level - f->cam_pos[1];
+    if (!(depth > 0.0f)) return;
+    const float t_surf = depth / rd[1];
+    if (!(t_surf < f->far_)) return;
+    const float n[3] = {0.0f, -1.0f, 0.0f};
+    float rd2[3], ro2[3];
+#if VXO_PLANT == 16
+    wgsl_refract(rd, n, 1.0f / WATER_IOR, rd2);
+#else
+    wgsl_refract(rd, n, WATER_IOR, rd2);
+#endif
+    if (!(dot3o(rd2, rd2) > 0.0f)) return;
+    for (int k = 0; k < 3; k++) ro2[k] = (f->cam_pos[k] + rd[k] * t_surf) + rd2[k] * SURFACE_EPS;
+    vxo_hit h2;
+    march_chunk(c, ci, ro2, rd2, f->far_ - t_surf, 1, 0, 0, 0, &h2);
+    if (h2.hit) {
+        const uint32_t straight = h->iters;
+        *h = h2;
+#if VXO_PLANT == 17
+        h->t = h2.t;
+#else
+        h->t = t_surf + h2.t;
+#endif
+        h->iters = straight + h2.iters;
+    }
+}
+
 static void march_pair(ctx *c, uint32_t tile, uint32_t ci, uint64_t *tilevis, uint32_t *tileheat) {
     const vxo_frame *f = c->f;
     uint32_t tx = tile % f->tiles[0], ty = tile / f->tiles[0];
@@ -1161,6 +1194,7 @@ static void march_pair(ctx *c, uint32_t tile, uint32_t ci, uint64_t *tilevis, ui
             vxo_hit h;
             c->st.rays_primary++;
             march_chunk(c, ci, f->cam_pos, rd, f->far_, underwater, 0, 0, 0, &h);
+            if (c->s->snell_bend && underwater && !h.hit && rd[1] > 0.0f) snell_bend(c, ci, rd, &h);
             if (h.hit) {
                 uint64_t k = vxo_make_key(f, h.t, h.normal, ci, h.voxel, h.micro);
                 if (k > tilevis[ly * 8 + lx]) tilevis[ly * 8 + lx] = k;

@@ -30,8 +30,9 @@
  * the reflect direction against a float64 construction over deep and shallow water
  * (shoal); planted mistakes in them are caught. The water legs behind glass: the same
  * float64 leg construction from the point the transmit leg meets the water.
- * Foliage tufts, the water-look ripples and the Snell bend are not restated; the library
- * refuses them.
+ * The Snell bend of underwater rays: a float64 refraction at the sea surface and DDA
+ * through the chunk, with planted mistakes caught. Foliage tufts and the water-look
+ * ripples are not restated; the library refuses them.
  *
  * Every struct here is an independent restatement of the reference layouts:
  *   vxo_frame  <- GpuFrame  (src/render/mod.rs:398-466, 352 bytes)
@@ -101,6 +102,7 @@ typedef struct vxo_spec {
     float leaf_fill;          /* SPEC_LEAF_FILL */
     uint32_t glass_reflect;   /* SPEC_GLASS_REFLECT */
     uint32_t waves, wave_aniso, wave_shoal, wave_fill;  /* SPEC_WAVES, SPEC_WAVE_ANISO/SHOAL/FILL */
+    uint32_t snell_bend;       /* SPEC_SNELL_BEND (march.wgsl:23-45) */
 } vxo_spec;
 
 /* Algorithmic traffic and work of one frame, counted exactly by the restatement. */

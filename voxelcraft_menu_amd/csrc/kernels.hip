@@ -102,6 +102,7 @@ __global__ __launch_bounds__(256) void k_march_pairs(Pools p, FrameK f, Spec s, 
         for (int i = 0; i < 3; i++) inv[i] = 1.0f / rd[i];
         Hit h = underwater ? march_chunk<true, false, false>(p, f, s, c, ci, f.cam, rd, inv, f.far_)
                            : march_chunk<false, false, false>(p, f, s, c, ci, f.cam, rd, inv, f.far_);
+        if (underwater && (s.bits & SPEC_SNELL_BEND) && !h.hit && rd[1] > 0.0f) snell_bend(p, f, s, c, ci, rd, h);
         if (h.hit) atomicMax(&vis[(size_t)py * f.res[0] + px], (unsigned long long)make_key(f.far_, h, ci));
         // FLAG_HEATMAP frames: every marched pair adds its iterations (march.wgsl:51-53).
         if (heat) atomicAdd(&heat[(size_t)py * f.res[0] + px], h.iters);

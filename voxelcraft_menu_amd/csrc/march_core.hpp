@@ -30,7 +30,8 @@ constexpr uint32_t SPEC_CUTOUT = 1u, SPEC_GLASS = 2u, SPEC_FULL_MARCH = 4u, SPEC
                    // the water-look ripples (spec_hi), whose normals are not restated
                    SPEC_WATER_LOOK = 64u,
                    // the wave field (SPEC_WAVES, SPEC_WAVE_ANISO, SPEC_WAVE_SHOAL, SPEC_WAVE_FILL)
-                   SPEC_WAVES = 128u, SPEC_WAVE_ANISO = 256u, SPEC_WAVE_SHOAL = 512u, SPEC_WAVE_FILL = 1024u;
+                   SPEC_WAVES = 128u, SPEC_WAVE_ANISO = 256u, SPEC_WAVE_SHOAL = 512u, SPEC_WAVE_FILL = 1024u,
+                   SPEC_SNELL_BEND = 2048u;
 
 struct Spec {
     uint32_t bits;
@@ -1055,6 +1056,35 @@ __device__ __forceinline__ void wgsl_reflect(const float i[3], const float n[3],
     for (int q = 0; q < 3; q++) out[q] = i[q] - d2 * n[q];
 }
 // smoothstep(e0, e1, x) = t*t*(3 - 2t), t = clamp((x - e0) / (e1 - e0), 0, 1)
+// The Snell bend of march (march.wgsl:23-45): an underwater camera's ray that misses the
+// pair's chunk going up is refracted at the sea surface (from water, eta = WATER_IOR,
+// about (0, -1, 0)) and marched again through the same chunk from just past the surface,
+// skipping water, within far - t_surf; a hit there replaces the miss with t = t_surf +
+// its distance, and the iterations add up.
+__device__ inline void snell_bend(const Pools &p, const FrameK &f, const Spec &s, const ChunkK &c, uint32_t ci,
+                                  const float rd[3], Hit &h) {
+    const float depth = f.sea_level - f.cam[1];
+    if (!(depth > 0.0f)) return;
+    const float t_surf = depth / rd[1];
+    if (!(t_surf < f.far_)) return;
+    const float n[3] = {0.0f, -1.0f, 0.0f};
+    float rd2[3], ro2[3], inv2[3];
+    wgsl_refract(rd, n, WATER_IOR, rd2);
+    if (!(dot3o(rd2, rd2) > 0.0f)) return;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        ro2[k] = (f.cam[k] + rd[k] * t_surf) + rd2[k] * SURFACE_EPS;
+        inv2[k] = 1.0f / rd2[k];
+    }
+    const Hit h2 = march_chunk<true, false, false>(p, f, s, c, ci, ro2, rd2, inv2, f.far_ - t_surf);
+    if (h2.hit) {
+        const uint32_t straight = h.iters;
+        h = h2;
+        h.t = t_surf + h2.t;
+        h.iters = straight + h2.iters;
+    }
+}
+
 __device__ __forceinline__ float wgsl_smoothstep(float e0, float e1, float x) {
     const float t = fminf(fmaxf((x - e0) / (e1 - e0), 0.0f), 1.0f);
     return t * t * (3.0f - 2.0f * t);

@@ -64,6 +64,9 @@ __device__ __forceinline__ void march_tile_body(const Pools &p, const FrameK &f,
         ChunkK c = ld_chunk(p, cj);
         if (need && !ray_skips(f, c.fade, px, py)) {
             Hit h = march_chunk<kUnderwater, false, false>(p, f, s, c, cj, f.cam, rd, inv, f.far_);
+            if constexpr (kUnderwater) {
+                if ((s.bits & SPEC_SNELL_BEND) && !h.hit && rd[1] > 0.0f) snell_bend(p, f, s, c, cj, rd, h);
+            }
             if (h.hit) {
                 uint64_t k = make_key(f.far_, h, cj);
                 best = k > best ? k : best;

@@ -420,7 +420,6 @@ FrameK make_frame(const vx_gpu_frame *g) {
 // Override values for a (flags & SPEC_MASK, spec_hi) pipeline key (src/render/mod.rs:2355-2689).
 int make_spec(uint32_t spec, uint32_t spec_hi, uint32_t frame_flags, Spec &s) {
     if (spec & VX_FLAG_FOLIAGE) return VX_E_UNSUPPORTED;  // cross_quad samples the texture atlas
-    if ((spec_hi & VX_FLAG_HI_SNELL_BEND) && (frame_flags & VX_FLAG_UNDERWATER)) return VX_E_UNSUPPORTED;
     s.bits = 0;
     if (spec & VX_FLAG_LEAF_CUTOUT) s.bits |= SPEC_CUTOUT;
     if (spec & VX_FLAG_GLASS) s.bits |= SPEC_GLASS;
@@ -428,6 +427,7 @@ int make_spec(uint32_t spec, uint32_t spec_hi, uint32_t frame_flags, Spec &s) {
     if (spec_hi & VX_FLAG_HI_EMITTER_WORLD) s.bits |= SPEC_EMITTER;
     if (!(spec_hi & VX_FLAG_HI_LEGACY_WATER)) s.bits |= SPEC_WATER_REPAIR;
     if (spec_hi & VX_FLAG_HI_GLASS_REFLECT) s.bits |= SPEC_GLASS_REFLECT;
+    if (spec_hi & VX_FLAG_HI_SNELL_BEND) s.bits |= SPEC_SNELL_BEND;
     if (spec & VX_FLAG_WAVES) s.bits |= SPEC_WAVES;
     if (spec & VX_FLAG_WAVE_ANISO) s.bits |= SPEC_WAVE_ANISO;
     if (spec & VX_FLAG_WAVE_SHOAL) s.bits |= SPEC_WAVE_SHOAL;
