@@ -25,6 +25,7 @@
  *   vx_timings                Readback per-pass timestamps   (src/render/timer.rs:150-176)
  *   vx_resolve                the resolve pass, lane 0       (src/render/shaders/resolve.wgsl:1378-1466,
  *                                                            dispatched at src/render/mod.rs:1853-1878)
+ *   vx_present                the present blit               (src/render/shaders/blit.wgsl:18-93)
  *   vx_batch_*                no counterpart: many views per launch (the reference renders
  *                             one view per render() call, src/headless.rs:680-837)
  *   vx_serve_*                no counterpart: continuous-batching request loop with
@@ -276,6 +277,17 @@ typedef struct vx_resolve_assets {
 int vx_resolve_set_assets(vx_ctx *ctx, const vx_resolve_assets *a);
 int vx_resolve(vx_ctx *ctx);      /* asynchronous on the context stream */
 int vx_read_color(vx_ctx *ctx, float *rgba, size_t n);  /* n = W*H*4 */
+/* Present (src/render/shaders/blit.wgsl:18-93, src/render/blit.rs): the colour target
+ * drawn onto a surface of the same size -- max(rgb, 0), the knee tonemap or the ACES fit
+ * (tone 1), grade 1 or 2 mixed by strength below 1, the sRGB encode when the surface
+ * format is not an sRGB one (encode 1) -- stored as RGBA8 (round(clamp(c) * 255), alpha
+ * 255). vx_read_image copies it (n = W*H*4 bytes). */
+typedef struct vx_present_desc {
+    uint32_t encode, tone, grade;
+    float strength;
+} vx_present_desc;
+int vx_present(vx_ctx *ctx, const vx_present_desc *d);
+int vx_read_image(vx_ctx *ctx, uint8_t *rgba, size_t n);
 /* Device pointers of the current outputs (for callers that keep data on the GPU). */
 int vx_device_outputs(vx_ctx *ctx, uint64_t **vis, uint16_t **shadow);
 

@@ -110,6 +110,8 @@ def _bind(L):
     L.vxo_resolve.restype = C.c_int
     L.vxo_resolve.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, C.POINTER(RSpec), C.POINTER(ResolveAssets),
                               V, V, V, V, C.POINTER(Stats), C.c_int]
+    L.vxo_present.restype = None
+    L.vxo_present.argtypes = [V, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, V]
     L.vxo_wave_field.restype = None
     L.vxo_wave_field.argtypes = [V, C.POINTER(Spec), C.c_float, C.c_float, C.c_float, V, C.c_float, C.c_int,
                                  C.c_float, V]
@@ -185,6 +187,15 @@ def resolve(scene: "Scene", frame_bytes: bytes, rspec: RSpec, atlas: np.ndarray,
                            C.byref(rspec), C.byref(a), _p(v), _p(s), _p(hz), _p(out), C.byref(st), threads)
     if rc != 0:
         raise RuntimeError("oracle resolve rejected its arguments")
+    return out
+
+
+def present(rgba: np.ndarray, encode: int = 1, tone: int = 0, grade: int = 0, strength: float = 1.0) -> np.ndarray:
+    """The present blit of an (H, W, 4) float32 colour: (H, W, 4) uint8."""
+    src = np.ascontiguousarray(rgba, dtype=np.float32)
+    h, w = src.shape[:2]
+    out = np.zeros((h, w, 4), dtype=np.uint8)
+    lib().vxo_present(_p(src), w, h, encode, tone, grade, strength, _p(out))
     return out
 
 

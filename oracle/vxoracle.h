@@ -258,6 +258,11 @@ int vxo_resolve(const vxo_pools *p, const vxo_chunk *chunks, uint32_t n_chunks, 
                 const vxo_frame *f, const vxo_rspec *r, const vxo_resolve_assets *a, const uint64_t *vis,
                 const uint16_t *sun, const uint32_t *heat, float *out, vxo_stats *stats, int threads);
 
+/* Present (blit.wgsl:18-93): w*h RGBA32F colours to RGBA8 (alpha 255). tone 1: ACES fit,
+ * else the knee tonemap; grade 1 / 2 mixed by strength below 1; encode: sRGB encode. */
+void vxo_present(const float *rgba, uint32_t w, uint32_t h, uint32_t encode, uint32_t tone, uint32_t grade,
+                 float strength, uint8_t *out);
+
 #ifdef __cplusplus
 }
 #endif

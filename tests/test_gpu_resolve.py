@@ -168,3 +168,25 @@ def test_unrestated_configurations_are_refused(small_world):
         print(f"refused: {name}", flush=True)
     r.sync()
     r.close()
+
+
+@pytest.mark.parametrize("encode,tone,grade,strength", [(1, 0, 0, 1.0), (0, 1, 1, 1.0), (1, 0, 2, 0.6)])
+def test_present_matches_oracle(encode, tone, grade, strength):
+    """The present blit of a resolved frame equals the oracle's bytes."""
+    world = _pillar_world()
+    flags = A.FLAG_SHADOWS | 8 | A.FLAG_HIZ | SKY_TINT_BIT
+    cam = H.camera((32.5, 40.3, 2.5), 90.0, -24.0, 160, 96, flags, sea=10.0)
+    v = build_view(world, cam)
+    v.frame.ambient, v.frame.mip_bias, v.frame.fog_density = 0.05, -5.0, 0.003
+    atlas, faces = _atlas(5), _faces_perm()
+    r = Renderer(160, 96)
+    r.sync_world(world)
+    r.set_resolve_assets(atlas, SIZE, LAYERS, MIPS, faces)
+    color = _check(r, world, v, flags, A.FLAG_HI_FULL_MARCH | SKY_SPECULAR, atlas, faces, "present")
+    r.present(encode, tone, grade, strength)
+    r.sync()
+    got = r.read_image()
+    want = O.present(color, encode, tone, grade, strength)
+    assert np.array_equal(got, want), f"{(got != want).any(axis=-1).sum()} pixels differ"
+    assert got[..., :3].std() > 5  # an image, not a flat fill
+    r.close()

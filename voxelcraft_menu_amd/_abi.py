@@ -244,6 +244,11 @@ class ResolveAssets(C.Structure):
                 ("block_faces", C.c_void_p), ("n_faces", C.c_uint32)]
 
 
+class PresentDesc(C.Structure):
+    """vx_present_desc: the blit's encode / tone / grade switches and grade strength."""
+    _fields_ = [("encode", C.c_uint32), ("tone", C.c_uint32), ("grade", C.c_uint32), ("strength", C.c_float)]
+
+
 class ReplayReq(C.Structure):
     """vx_replay_req: a camera request of a native replay and its arrival offset."""
     _fields_ = [("t", C.c_double), ("view", CameraView)]
@@ -403,6 +408,8 @@ _sig_march = {
     "vx_resolve_set_assets": (C.c_int, [P, P]),
     "vx_resolve": (C.c_int, [P]),
     "vx_read_color": (C.c_int, [P, P, C.c_size_t]),
+    "vx_present": (C.c_int, [P, P]),
+    "vx_read_image": (C.c_int, [P, P, C.c_size_t]),
     "vx_march_stats_get": (C.c_int, [P, C.POINTER(MarchStats)]),
     "vx_march_stats_compute": (C.c_int, [P, C.c_size_t, C.c_uint32, C.POINTER(MarchStats)]),
     "vx_batch_set_secondary": (C.c_int, [P, C.c_uint32]),

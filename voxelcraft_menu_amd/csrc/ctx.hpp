@@ -88,9 +88,9 @@ struct vx_ctx {
     vx_gpu_frame gframe{};
     uint32_t spec_word = 0, spec_hi_word = 0;
     // Resolve: the caller's atlas and face table, and the colour target.
-    vxh::DevBuf atlas, faces, color;
+    vxh::DevBuf atlas, faces, color, image;
     uint32_t atlas_size = 0, atlas_layers = 0, atlas_mips = 0, n_faces = 0;
-    bool color_valid = false;
+    bool color_valid = false, image_valid = false;
 };
 
 namespace vxh {

@@ -559,6 +559,73 @@ __global__ __launch_bounds__(64) void k_resolve(ResolveK k) {
     k.out[pix] = make_float4(color[0], color[1], color[2], 1.0f);
 }
 
+// ---- present (blit.wgsl:18-93), as the oracle's vxo_present ----
+constexpr float KNEE = 0.75f;
+__device__ __forceinline__ float tone_knee(float x) {
+    const float s = (1.0f - KNEE) * (1.0f - KNEE);
+    const float shoulder = 1.0f - s / (fmaxf(x, KNEE) + (1.0f - 2.0f * KNEE));
+    return x < KNEE ? x : shoulder;
+}
+__device__ __forceinline__ float aces_fit(float x) {
+    const float a = 2.51f, b = 0.03f, c = 2.43f, d = 0.59f, e = 0.14f;
+    return r_clamp((x * (a * x + b)) / (x * (c * x + d) + e), 0.0f, 1.0f);
+}
+__device__ __forceinline__ float linear_to_srgb(float c) {
+    const float lo = c * 12.92f;
+    const float hi = 1.055f * r_pow(fmaxf(c, 0.0031308f), 1.0f / 2.4f) - 0.055f;
+    return c <= 0.0031308f ? lo : hi;
+}
+__device__ __forceinline__ uint32_t unorm8(float c) { return (uint32_t)floorf(r_clamp(c, 0.0f, 1.0f) * 255.0f + 0.5f); }
+
+__global__ __launch_bounds__(256) void k_present(const float4 *__restrict__ src, uint32_t n, uint32_t encode,
+                                                 uint32_t tone, uint32_t grade, float strength,
+                                                 uint32_t *__restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 v = src[i];
+    const float hdr[3] = {fmaxf(v.x, 0.0f), fmaxf(v.y, 0.0f), fmaxf(v.z, 0.0f)};
+    float c[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) c[k] = tone == 1u ? aces_fit(hdr[k]) : tone_knee(hdr[k]);
+    const float st = r_clamp(strength, 0.0f, 1.0f);
+    if (grade == 1u) {
+        const float before[3] = {c[0], c[1], c[2]};
+        const float luma = c[0] * 0.2126f + c[1] * 0.7152f + c[2] * 0.0722f;
+        const float bal[3] = {r_clamp(c[0] * 1.05f, 0.0f, 1.0f), r_clamp(c[1] * 1.0f, 0.0f, 1.0f),
+                              r_clamp(c[2] * 0.93f, 0.0f, 1.0f)};
+#pragma unroll
+        for (int k = 0; k < 3; k++) c[k] = r_clamp(luma + (bal[k] - luma) * 1.06f, 0.0f, 1.0f);
+        if (strength < 1.0f) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) c[k] = r_mix(before[k], c[k], st);
+        }
+    }
+    if (grade == 2u) {
+        const float before[3] = {c[0], c[1], c[2]};
+        const float luma = c[0] * 0.2126f + c[1] * 0.7152f + c[2] * 0.0722f;
+        float g[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) g[k] = r_mix(c[k], luma, 0.12f);
+#pragma unroll
+        for (int k = 0; k < 3; k++) g[k] = r_mix(g[k], g[k] * g[k] * (3.0f - 2.0f * g[k]), 0.45f);
+        const float shad = r_pow(r_clamp(1.0f - luma * 2.2f, 0.0f, 1.0f), 2.0f);
+        const float hi = r_pow(r_clamp(luma, 0.0f, 1.0f), 3.0f);
+        const float m[3] = {1.0f - 0.05f * shad + 0.03f * hi, 1.0f - 0.02f * shad + 0.02f * hi,
+                            1.0f + 0.07f * shad - 0.01f * hi};
+#pragma unroll
+        for (int k = 0; k < 3; k++) c[k] = r_clamp(g[k] * m[k], 0.0f, 1.0f);
+        if (strength < 1.0f) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) c[k] = r_mix(before[k], c[k], st);
+        }
+    }
+    if (encode != 0u) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) c[k] = linear_to_srgb(c[k]);
+    }
+    out[i] = unorm8(c[0]) | (unorm8(c[1]) << 8) | (unorm8(c[2]) << 16) | (255u << 24);
+}
+
 }  // namespace vxd
 
 using namespace vxd;
@@ -654,6 +721,28 @@ int vx_resolve(vx_ctx *c) {
     k_resolve<<<c->tx * c->ty, 64, 0, c->stream>>>(k);
     VX_LAUNCHED();
     c->color_valid = true;
+    c->image_valid = false;
+    return VX_OK;
+}
+
+int vx_present(vx_ctx *c, const vx_present_desc *d) {
+    if (!c || !d || !c->color_valid || d->tone > 1u || d->grade > 2u || d->encode > 1u) return VX_E_ARG;
+    HIPX(hipSetDevice(c->device));
+    const uint32_t n = c->w * c->h;
+    int rc;
+    if ((rc = vxh::ensure(c->image, (size_t)n * 4))) return rc;
+    k_present<<<(n + 255) / 256, 256, 0, c->stream>>>((const float4 *)c->color.p, n, d->encode, d->tone, d->grade,
+                                                      d->strength, (uint32_t *)c->image.p);
+    VX_LAUNCHED();
+    c->image_valid = true;
+    return VX_OK;
+}
+
+int vx_read_image(vx_ctx *c, uint8_t *rgba, size_t n) {
+    if (!c || !rgba || n != (size_t)c->w * c->h * 4 || !c->image_valid) return VX_E_ARG;
+    HIPX(hipSetDevice(c->device));
+    HIPX(hipMemcpyAsync(rgba, c->image.p, n, hipMemcpyDeviceToHost, c->stream));
+    HIPX(hipStreamSynchronize(c->stream));
     return VX_OK;
 }
 

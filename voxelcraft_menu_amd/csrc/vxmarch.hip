@@ -234,7 +234,7 @@ int vx_ctx_destroy(vx_ctx *c) {
     DevBuf *bufs[] = {&c->pool[0], &c->pool[1],   &c->pool[2], &c->chunks,      &c->grid,     &c->vis,
                       &c->shadow,  &c->hiz,       &c->pairs,   &c->counters,    &c->cost,     &c->order,
                       &c->heat,    &c->build_dense, &c->build_leaf, &c->build_words, &c->build_l1, &c->build_out,
-                      &c->atlas,   &c->faces,     &c->color};
+                      &c->atlas,   &c->faces,     &c->color,  &c->image};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto &set : c->ring)

@@ -158,6 +158,18 @@ class Renderer:
         A.check(self._L.vx_read_color(self._c, A.ptr(out), out.size), "vx_read_color")
         return out
 
+    def present(self, encode: int = 1, tone: int = 0, grade: int = 0, strength: float = 1.0):
+        """The present blit (blit.wgsl) of the last resolve into an RGBA8 image."""
+        d = A.PresentDesc(encode, tone, grade, strength)
+        A.check(self._L.vx_present(self._c, C.byref(d)), "vx_present")
+
+    def read_image(self) -> np.ndarray:
+        """(H, W, 4) uint8 image of the last present."""
+        w, h = self.size
+        out = np.zeros((h, w, 4), dtype=np.uint8)
+        A.check(self._L.vx_read_image(self._c, A.ptr(out), out.size), "vx_read_image")
+        return out
+
     def read_heat(self) -> np.ndarray:
         """Marched iterations per pixel of the last FLAG_HEATMAP frame."""
         w, h = self.size
