@@ -282,14 +282,14 @@ def roofline_valu(kname: str, workload: str, kernel_ms: float, alg: float, px: i
     contraction; the ~20 MB tree is served from L2 and the Infinity Cache): its roofline is
     VALU lane-operations per second against the FP32 vector peak. The VALU instruction
     count per launch is the committed rocprofv3 SQ_INSTS_VALU figure for this workload,
-    valid only for the library build it was measured on (its source digest is recorded
-    with it); the kernel time is measured here with HIP events."""
+    valid only for the kernel build it was measured on (the digest of the tile kernels'
+    sources and flags is recorded with it); the kernel time is measured here with HIP events."""
     from voxelcraft_menu_amd import build as B
 
     prof = profile_figures(kname, workload)
     kernel_s = kernel_ms * 1e-3
     valu = float(prof["valu_insts_per_launch"]) if "valu_insts_per_launch" in prof else None
-    same_build = prof.get("library_digest") == B.source_digest("march")
+    same_build = prof.get("library_digest") == B.source_digest("tile_kernels")
     achieved = valu * 64.0 / kernel_s / 1e12 if valu and same_build else None
     traffic = float(prof["traffic_bytes_per_launch"]) if "traffic_bytes_per_launch" in prof else None
     probe = valu_probe()

@@ -43,6 +43,12 @@ def source_digest(kind: str) -> str:
     if kind == "host":
         files = [os.path.join(CSRC, s) for s in HOST_SOURCES] + glob.glob(os.path.join(CSRC, "host", "*.hpp"))
         flags = "g++ -std=c++17 -O2"
+    elif kind == "tile_kernels":
+        # What the tile-march kernels (single view and batched) are compiled from: their
+        # translation unit, the headers and the flags. Counter profiles of those kernels
+        # carry this digest, so edits elsewhere in the library leave them valid.
+        files = [os.path.join(CSRC, "tiles.hip")] + glob.glob(os.path.join(CSRC, "*.hpp"))
+        flags = " ".join(HIPCC_FLAGS)
     elif kind == "calib":
         return hashlib.sha256(" ".join(HIPCC_FLAGS).encode() +
                               open(os.path.join(CSRC, "calib.hip"), "rb").read()).hexdigest()
@@ -90,11 +96,22 @@ def build_host(verbose: bool = False) -> str:
 
 
 def build_march(verbose: bool = False) -> str:
+    """Each translation unit compiles to its own object, several at once (they share no
+    device symbols: kernels are launched through their host stubs), then one link."""
+    from concurrent.futures import ThreadPoolExecutor
+
     out = os.path.join(HERE, "libvxmarch.so")
     tmp = out + ".tmp"
+    objdir = os.path.join(ROOT, "_build", "march")
+    os.makedirs(objdir, exist_ok=True)
     srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES + HIP_EXTRA]
-    cmd = [_hipcc()] + HIPCC_FLAGS + [f"-I{INC}", f"-I{CSRC}"] + srcs + ["-pthread", "-o", tmp]
-    _run(cmd, verbose)
+    objs = [os.path.join(objdir, os.path.basename(s) + ".o") for s in srcs]
+    cflags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    jobs = max(1, min(len(srcs), int(os.environ.get("MAX_JOBS", "0") or 0) or (os.cpu_count() or 1), 16))
+    with ThreadPoolExecutor(jobs) as ex:
+        list(ex.map(lambda so: _run([_hipcc()] + cflags + [f"-I{INC}", f"-I{CSRC}", "-c", so[0], "-o", so[1]], verbose),
+                    zip(srcs, objs)))
+    _run([_hipcc()] + HIPCC_FLAGS + objs + ["-pthread", "-o", tmp], verbose)
     os.replace(tmp, out)
     _stamp(out, "march")
     return out
