@@ -172,6 +172,11 @@ def run_batch(args, out):
     r = Renderer(256, 144)
     r.sync_world(world)
     b = Batch(r, len(views))
+    if args.pool_layout == "flat":
+        # Every measurement of this run on the per-chunk block layout (for counter runs).
+        table, flat, fviews = flat_views(world, views)
+        r.set_flat_pools(flat)
+        views = fviews
     b.set_views(views, specs)
     best, best_ms = b.autotune(3)
     _log(f"autotuned: {best} ({best_ms:.3f} ms)")
@@ -209,6 +214,7 @@ def run_batch(args, out):
     plain = rays_per_step * args.steps / pdt / 1e9
     _log(f"plain: {pdt / args.steps * 1e3:.3f} ms/step")
     b.set_tuning(**tuned)
+    layout_ab = None if args.no_layout_ab else run_layout_ab(args, r, b, world, views, specs, tuned)
     b.close()
 
     kname = {A.BATCH_FLAT: "k_batch_flat", A.BATCH_PERSISTENT: "k_batch_persistent"}.get(best["mode"], "k_march_tiles")
@@ -229,7 +235,9 @@ def run_batch(args, out):
         },
     })
 
-    if not args.no_moving:
+    if layout_ab:
+        out["pool_layout"] = layout_ab
+    if not args.no_moving and args.pool_layout == "reference":  # its device lists carry reference pointers
         out["moving_camera"] = run_moving(args, world, cams, specs, tuned, px)
 
     # Single 1920x1080 view (the plain-at-scale case), for reference.
@@ -275,6 +283,57 @@ def run_batch(args, out):
     # key and 2-byte R16F shadow word, plus the 4-byte Hi-Z word per tile.
     alg = prim["node_bytes"] + sh["node_bytes"] + 10 * px + 4 * tiles
     out["roofline"] = roofline_valu(kname, "workload2", march_ms, alg, px)
+
+
+def flat_views(world, views):
+    """The world's pools laid out one block per chunk (world.flat_layout) and the views
+    with their chunk records moved into it."""
+    from voxelcraft_menu_amd.world import ViewSnapshot, flat_layout, relocate_view
+
+    snap = ViewSnapshot(world)
+    table = snap.chunk_table()
+    snap.close()
+    flat, moved = flat_layout(world.pool(0), world.pool(1), world.pool(2), table)
+    return table, flat, [relocate_view(v, table, moved) for v in views]
+
+
+def run_layout_ab(args, r, b, world, views, specs, tuned) -> dict:
+    """The tuned batch on the reference pools and on the per-chunk block layout, the same
+    views and launch form, alternating in one process; keys and shadow words of a sample
+    of views are compared between the two."""
+    table, flat, fviews = flat_views(world, views)
+    sample = sorted({0, len(views) // 3, 2 * len(views) // 3, len(views) - 1})
+    ms = {"reference": [], "flat": []}
+    ref = {}
+    same = True
+    for rep in range(3):
+        for name, vs_, words in (("reference", views, None), ("flat", fviews, flat)):
+            r.set_flat_pools(words)
+            b.set_views(vs_, specs)
+            b.set_tuning(**tuned)
+            for _ in range(3):
+                b.render()
+            dt, _, _ = _time_batch(b, args.steps)
+            ms[name].append(dt / args.steps * 1e3)
+            if rep == 0:
+                for i in sample:
+                    o = (b.read_vis(i), b.read_shadow(i))
+                    if name == "reference":
+                        ref[i] = o
+                    else:
+                        same &= bool(np.array_equal(o[0], ref[i][0]) and np.array_equal(o[1], ref[i][1]))
+    r.set_flat_pools(None)
+    b.set_views(views, specs)
+    b.set_tuning(**tuned)
+    pool_bytes = sum(int(world.pool(k).nbytes) for k in range(3))
+    res = {"reference_ms": round(float(np.median(ms["reference"])), 4), "flat_ms": round(float(np.median(ms["flat"])), 4),
+           "reference_pool_bytes": pool_bytes, "flat_bytes": int(flat.nbytes), "chunks": int(table.shape[0]),
+           "identical_on_sampled_views": same, "sampled_views": sample,
+           "form": "per-chunk block (L1 nodes, leaf runs, attribute table, palettes), 128-byte aligned"}
+    _log(f"pool layout: reference {res['reference_ms']} ms, flat {res['flat_ms']} ms per step, identical {same}")
+    if not same:
+        _fail("pool layout A/B: the block layout changed keys or shadow words")
+    return res
 
 
 def roofline_valu(kname: str, workload: str, kernel_ms: float, alg: float, px: int) -> dict:
@@ -688,6 +747,9 @@ def main():
     ap.add_argument("--no-build", action="store_true", help="skip the device chunk build (workload 2 only)")
     ap.add_argument("--no-view-setup", action="store_true", help="skip the view-setup comparison (workload 2 only)")
     ap.add_argument("--no-moving", action="store_true", help="skip the turning-camera steps (workload 2 only)")
+    ap.add_argument("--no-layout-ab", action="store_true", help="skip the pool-layout A/B (workload 2 only)")
+    ap.add_argument("--pool-layout", choices=["reference", "flat"], default="reference",
+                    help="pool layout of the batched measurements (workload 2)")
     ap.add_argument("--serve-qps", type=float, default=1000.0, help="offered view requests per second")
     ap.add_argument("--serve-seconds", type=float, default=4.0, help="seconds of request arrivals")
     ap.add_argument("--serve-edit-rate", type=float, default=2.0, help="edit batches (1000 ops) per second")

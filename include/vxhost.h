@@ -22,6 +22,7 @@
  *   vxw_build_local_flat / vxw_insert_local_flat
  *                         <- build_local / insert_local with a flat LocalTree record
  *                            (the device build's output format, vx_local_tree)
+ *   vxw_flat_layout       <- (no counterpart) one pool block per chunk for vx_pool_set_flat
  *   vxw_journal_parse     <- EditJournal::load (src/journal.rs:391-475)
  *   vxw_journal_write     <- EditJournal::save (src/journal.rs:477-508)
  *   vxw_generate          <- a seeded terrain generator with the reference's
@@ -188,6 +189,19 @@ int vxw_build_view_snapshot(const vxw_snapshot *s, const vxw_camera *cam, vx_gpu
  * record (fade 1) of every snapshot chunk with solid voxels, in the order vxw_build_view
  * takes them; returns the count (out may be NULL). */
 size_t vxw_snapshot_chunks(const vxw_snapshot *s, vx_gpu_chunk *out, size_t cap);
+/* The three pools re-laid out as one contiguous, 128-byte aligned block per chunk
+ * (no counterpart in the reference, whose pools share interned runs between chunks,
+ * src/voxel/geometry.rs:12-52, attributes.rs:6-88): the chunk's L1 nodes, the leaf runs
+ * of its L1 cells, its attribute table and the palettes that table points to, with
+ * every pointer rewritten into the block. chunks_out receives the n records with their
+ * root run pointer and attr_base moved; all other fields are copied. Element counts:
+ * n_inners in 16-byte nodes, n_leaves in 8-byte leaves, n_bricks in words. flat == NULL
+ * only sizes the buffer (*words). Upload the words with vx_pool_set_flat and render
+ * with the returned records. Chunks with light tables (light_flags bit 0 clear) are
+ * refused: only uniform light (LightRef::Uniform) is carried over. */
+int vxw_flat_layout(const uint32_t *inners, size_t n_inners, const uint32_t *leaves, size_t n_leaves,
+                    const uint32_t *bricks, size_t n_bricks, const vx_gpu_chunk *chunks, uint32_t n,
+                    vx_gpu_chunk *chunks_out, uint32_t *flat, size_t cap_words, size_t *words);
 /* Frustum::from_camera (src/math.rs:69-117) of a camera: six planes (nx, ny, nz, d); a
  * point p is outside plane i when nx*px + ny*py + nz*pz + d < 0. */
 int vxw_frustum_planes(const vxw_camera *cam, float planes[24]);

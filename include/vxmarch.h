@@ -208,6 +208,12 @@ int vx_pool_mirror_get(vx_ctx *ctx, int pool, vx_pool_mirror *out);
 /* Copies the first `bytes` of a pool's device mirror to the host (bytes <= its length);
  * no counterpart in the reference beyond a buffer readback. */
 int vx_pool_read(vx_ctx *ctx, int pool, void *out, size_t bytes);
+/* No counterpart in the reference: reads every pool from one buffer laid out as one
+ * contiguous block per chunk (vxw_flat_layout in vxhost.h), for chunk records whose root
+ * run pointer and attr_base that function rewrote. bytes is a multiple of 128; data NULL
+ * with bytes 0 returns to the mirrors vx_pool_sync keeps. Batches and servers bound to
+ * the context follow (their descriptors are rewritten before the next launch). */
+int vx_pool_set_flat(vx_ctx *ctx, const void *data, size_t bytes);
 int vx_set_chunks(vx_ctx *ctx, const vx_gpu_chunk *chunks, uint32_t n_total, const uint32_t *grid);
 /* spec = frame flags & SPEC_MASK, spec_hi = spec_hi word (the pipeline key of
  * Renderer::render, src/render/mod.rs:1597). Foliage tufts need the texture atlas and

@@ -352,6 +352,8 @@ _sig_host = {
     "vxw_build_view_snapshot": (C.c_int, [P, C.POINTER(Camera), P, P, C.POINTER(View)]),
     "vxw_snapshot_chunks": (C.c_size_t, [P, P, C.c_size_t]),
     "vxw_frustum_planes": (C.c_int, [C.POINTER(Camera), P]),
+    "vxw_flat_layout": (C.c_int, [P, C.c_size_t, P, C.c_size_t, P, C.c_size_t, P, C.c_uint32, P, P, C.c_size_t,
+                                  C.POINTER(C.c_size_t)]),
     "vxw_pack_camera_frame": (C.c_int, [C.POINTER(Camera), C.c_uint32, C.POINTER(Frame)]),
     "vxw_journal_parse": (C.c_int, [P, C.c_size_t, C.c_int32, C.c_int32, C.POINTER(JournalInfo), P, C.c_size_t,
                                     C.c_char_p, C.c_size_t]),
@@ -369,6 +371,7 @@ _sig_march = {
     "vx_set_chunks": (C.c_int, [P, P, C.c_uint32, P]),
     "vx_pool_mirror_get": (C.c_int, [P, C.c_int, C.POINTER(PoolMirror)]),
     "vx_pool_read": (C.c_int, [P, C.c_int, P, C.c_size_t]),
+    "vx_pool_set_flat": (C.c_int, [P, P, C.c_size_t]),
     "vx_render": (C.c_int, [P, C.POINTER(Frame), C.c_uint32, C.c_uint32]),
     "vx_sync": (C.c_int, [P]),
     "vx_read_vis": (C.c_int, [P, P, C.c_size_t]),

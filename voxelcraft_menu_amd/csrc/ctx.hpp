@@ -81,6 +81,10 @@ struct vx_ctx {
     // Bumped whenever a pool's device address or length changes: captured graphs hold
     // both in their kernel arguments and are re-captured when it moves.
     uint64_t pool_gen = 0;
+    // vx_pool_set_flat: one buffer holding every chunk's nodes, leaves and attribute words
+    // (vxw_flat_layout); while flat_len > 0 the kernels read all three pools from it.
+    vxh::DevBuf flat;
+    size_t flat_len = 0;
     // Device chunk build (vx_build_local): dense input staging, per-leaf and per-L1
     // scratch, record output staging; grown on demand, kept for the next call.
     vxh::DevBuf build_dense, build_leaf, build_words, build_l1, build_out;

@@ -234,7 +234,7 @@ int vx_ctx_destroy(vx_ctx *c) {
     DevBuf *bufs[] = {&c->pool[0], &c->pool[1],   &c->pool[2], &c->chunks,      &c->grid,     &c->vis,
                       &c->shadow,  &c->hiz,       &c->pairs,   &c->counters,    &c->cost,     &c->order,
                       &c->heat,    &c->build_dense, &c->build_leaf, &c->build_words, &c->build_l1, &c->build_out,
-                      &c->atlas,   &c->faces,     &c->color,  &c->image};
+                      &c->atlas,   &c->faces,     &c->color,  &c->image,       &c->flat};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto &set : c->ring)
@@ -329,6 +329,25 @@ int vx_pool_sync(vx_ctx *c, int pool, const void *data, size_t bytes, const vx_r
     return VX_OK;
 }
 
+// The per-chunk block layout (vxw_flat_layout): every kernel of this context reads its
+// nodes, leaves and attribute words from these bytes until it is switched off (data NULL,
+// bytes 0); the pool mirrors are kept and serve again from then on.
+int vx_pool_set_flat(vx_ctx *c, const void *data, size_t bytes) {
+    if (!c || (!data && bytes) || bytes % 128) return VX_E_ARG;
+    if (bytes > 0xFFFFFFF0ull) return VX_E_OVERFLOW;
+    HIPX(hipSetDevice(c->device));
+    HIPX(hipStreamSynchronize(c->stream));
+    if (bytes > c->flat.cap) {
+        release(c->flat);
+        if (int rc = ensure(c->flat, bytes)) return rc;
+    }
+    if (bytes) HIPX(hipMemcpyAsync(c->flat.p, data, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPX(hipStreamSynchronize(c->stream));
+    c->flat_len = bytes;
+    c->pool_gen++;  // descriptors and captured graphs hold pool addresses
+    return VX_OK;
+}
+
 int vx_pool_mirror_get(vx_ctx *c, int pool, vx_pool_mirror *o) {
     if (!c || !o || pool < 0 || pool > 2) return VX_E_ARG;
     std::memset(o, 0, sizeof *o);
@@ -378,6 +397,15 @@ Pools make_pools(const vx_ctx *c) {
     p.n_inners = (uint32_t)(c->pool_len[0] / 16);
     p.n_leaves = (uint32_t)(c->pool_len[1] / 8);
     p.n_bricks = (uint32_t)(c->pool_len[2] / 4);
+    if (c->flat_len) {
+        // One block per chunk: the three element kinds index the same buffer.
+        p.inners = (const uint4 *)c->flat.p;
+        p.leaves = (const uint2 *)c->flat.p;
+        p.bricks = (const uint32_t *)c->flat.p;
+        p.n_inners = (uint32_t)(c->flat_len / 16);
+        p.n_leaves = (uint32_t)(c->flat_len / 8);
+        p.n_bricks = (uint32_t)(c->flat_len / 4);
+    }
     p.n_chunks = c->n_chunks;
     return p;
 }

@@ -88,6 +88,15 @@ class Renderer:
             A.check(self._L.vx_pool_sync(self._c, pool, A.ptr(a) if a.size else None, a.nbytes,
                                          A.ptr(rng) if n_elem else None, 1 if n_elem else 0), "vx_pool_sync")
 
+    def set_flat_pools(self, words: np.ndarray | None):
+        """Read every pool from one block per chunk (world.flat_layout's words), or, with
+        None, from the mirrors sync_world keeps. Chunk records must match the layout."""
+        if words is None:
+            A.check(self._L.vx_pool_set_flat(self._c, None, 0), "vx_pool_set_flat")
+            return
+        w = np.ascontiguousarray(words, dtype=np.uint32)
+        A.check(self._L.vx_pool_set_flat(self._c, A.ptr(w), w.nbytes), "vx_pool_set_flat")
+
     def set_view(self, view: ViewData):
         A.check(self._L.vx_set_chunks(self._c, A.ptr(view.chunks), view.n_total, A.ptr(view.grid)), "vx_set_chunks")
         self.view = view

@@ -453,3 +453,35 @@ def build_local_flat(dense: np.ndarray) -> bytes:
     out = C.create_string_buffer(n.value)
     A.check(L.vxw_build_local_flat(A.ptr(d), out, n.value, C.byref(n)), "build_local_flat")
     return out.raw[:n.value]
+
+
+def flat_layout(inners: np.ndarray, leaves: np.ndarray, bricks: np.ndarray, chunks: np.ndarray):
+    """The pools as one contiguous, 128-byte aligned block per chunk (vxw_flat_layout):
+    returns the block words (u32) and the (n, 96) chunk records with their root run pointer
+    and attr_base moved into them. Pool arrays are the u32 words of world.pool(i)."""
+    L = A.host()
+    ins = np.ascontiguousarray(inners, dtype=np.uint32).reshape(-1)
+    lfs = np.ascontiguousarray(leaves, dtype=np.uint32).reshape(-1)
+    brs = np.ascontiguousarray(bricks, dtype=np.uint32).reshape(-1)
+    ch = np.ascontiguousarray(chunks, dtype=np.uint8).reshape(-1, 96)
+    n = ch.shape[0]
+    out = np.zeros((max(n, 1), 96), np.uint8)
+    words = C.c_size_t()
+    args = (A.ptr(ins) if ins.size else None, ins.size // 4, A.ptr(lfs) if lfs.size else None, lfs.size // 2,
+            A.ptr(brs) if brs.size else None, brs.size, A.ptr(ch) if n else None, n, A.ptr(out))
+    A.check(L.vxw_flat_layout(*args, None, 0, C.byref(words)), "flat_layout")
+    flat = np.zeros(max(words.value, 32), np.uint32)
+    A.check(L.vxw_flat_layout(*args, A.ptr(flat), flat.size, C.byref(words)), "flat_layout")
+    return flat[:max(words.value, 32)], out[:n]
+
+
+def relocate_view(view: "ViewData", table: np.ndarray, moved: np.ndarray) -> "ViewData":
+    """The view with each chunk record replaced by its moved record from `moved` (the
+    flat_layout output for `table`), matched by origin and LOD."""
+    key = lambda rec: bytes(rec[28:44])  # lod, origin
+    index = {key(table[i]): i for i in range(table.shape[0])}
+    ch = view.chunks.copy()
+    for j in range(view.n_total):
+        ch[j] = moved[index[key(view.chunks[j])]]
+    return ViewData(ch, view.grid, view.frame, view.n_visible, view.n_total, view.camera_chunk, view.width,
+                    view.height)
