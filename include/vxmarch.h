@@ -294,6 +294,21 @@ typedef struct vx_present_desc {
 } vx_present_desc;
 int vx_present(vx_ctx *ctx, const vx_present_desc *d);
 int vx_read_image(vx_ctx *ctx, uint8_t *rgba, size_t n);
+/* TAA (src/render/shaders/taa.wgsl:1-127; the pass Renderer::render encodes after the
+ * resolve when FrameParams::taa is set, src/render/mod.rs:1610-1620, :1884-1918): each
+ * pixel's colour -- the resolve output as the reference's Rgba16Float trace target holds
+ * it, rounded to f16 -- is mixed with the previous TAA output at the pixel's
+ * reprojection through the frame's prev_view_proj (the key's hit point, or the view
+ * direction for a miss), read with history_bicubic and clipped in YCoCg to the 3x3
+ * neighbourhood's mean +- 1 sigma; the frame's taa_feedback is the weight of the colour
+ * (1 keeps the colour, as the reference's first frame does). Heatmap frames keep the
+ * colour. The output, f16 RGBA with alpha 1, is the next call's history, and vx_present
+ * draws it until the next vx_resolve. vx_read_taa copies it (n = W*H*4 halves);
+ * vx_write_taa_history replaces the history the next vx_taa reads. Histories start at
+ * zero and are remade by a size change. */
+int vx_taa(vx_ctx *ctx);
+int vx_read_taa(vx_ctx *ctx, uint16_t *rgba, size_t n);
+int vx_write_taa_history(vx_ctx *ctx, const uint16_t *rgba, size_t n);
 /* Device pointers of the current outputs (for callers that keep data on the GPU). */
 int vx_device_outputs(vx_ctx *ctx, uint64_t **vis, uint16_t **shadow);
 

@@ -110,6 +110,8 @@ def _bind(L):
     L.vxo_resolve.restype = C.c_int
     L.vxo_resolve.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, C.POINTER(RSpec), C.POINTER(ResolveAssets),
                               V, V, V, V, C.POINTER(Stats), C.c_int]
+    L.vxo_taa.restype = C.c_int
+    L.vxo_taa.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, V, V, V]
     L.vxo_present.restype = None
     L.vxo_present.argtypes = [V, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, V]
     L.vxo_wave_field.restype = None
@@ -187,6 +189,22 @@ def resolve(scene: "Scene", frame_bytes: bytes, rspec: RSpec, atlas: np.ndarray,
                            C.byref(rspec), C.byref(a), _p(v), _p(s), _p(hz), _p(out), C.byref(st), threads)
     if rc != 0:
         raise RuntimeError("oracle resolve rejected its arguments")
+    return out
+
+
+def taa(scene: "Scene", frame_bytes: bytes, vis: np.ndarray, src: np.ndarray, hist: np.ndarray) -> np.ndarray:
+    """The TAA pass (taa.wgsl): src the resolve colour (H, W, 4) f32, hist the previous
+    output (H, W, 4) f16 bits. Returns this frame's output, (H, W, 4) f16 bits."""
+    f = np.frombuffer(frame_bytes, dtype=np.uint8).copy()
+    res = np.frombuffer(frame_bytes[80:88], dtype=np.uint32)
+    w, h = int(res[0]), int(res[1])
+    s = np.ascontiguousarray(src, dtype=np.float32).reshape(h, w, 4)
+    hs = np.ascontiguousarray(hist, dtype=np.uint16).reshape(h, w, 4)
+    v = np.ascontiguousarray(vis, dtype=np.uint64).reshape(-1)
+    out = np.zeros((h, w, 4), dtype=np.uint16)
+    rc = lib().vxo_taa(C.byref(scene.pools), _p(scene.chunks), scene.n_chunks, _p(f), _p(v), _p(s), _p(hs), _p(out))
+    if rc != 0:
+        raise RuntimeError(f"vxo_taa failed ({rc})")
     return out
 
 

@@ -72,7 +72,7 @@ typedef struct vxo_frame {
     float wave_amp, wave_scale, wave_speed, wave_reflect_slope;  /* offsets 240-255 */
     float shaft_origin[2], shaft_texel, shaft_soft, cloud_patch, cloud_relief;  /* offset 256 */
     float haze_warm, zenith_deep;
-    float prev_view_proj[16];  /* not read */
+    float prev_view_proj[16];  /* read by vxo_taa only */
 } vxo_frame;
 
 typedef struct vxo_chunk {
@@ -260,6 +260,11 @@ int vxo_resolve(const vxo_pools *p, const vxo_chunk *chunks, uint32_t n_chunks, 
 
 /* Present (blit.wgsl:18-93): w*h RGBA32F colours to RGBA8 (alpha 255). tone 1: ACES fit,
  * else the knee tonemap; grade 1 / 2 mixed by strength below 1; encode: sRGB encode. */
+/* TAA (taa.wgsl:1-127) after the resolve: src is the resolve colour (W*H*4 f32, rounded
+ * to f16 as the trace output stores it), hist the previous TAA output (W*H*4 f16 bits);
+ * out receives this frame's output (W*H*4 f16 bits, alpha 1). */
+int vxo_taa(const vxo_pools *p, const vxo_chunk *chunks, uint32_t n_chunks, const vxo_frame *f, const uint64_t *vis,
+            const float *src, const uint16_t *hist, uint16_t *out);
 void vxo_present(const float *rgba, uint32_t w, uint32_t h, uint32_t encode, uint32_t tone, uint32_t grade,
                  float strength, uint8_t *out);
 

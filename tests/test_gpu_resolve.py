@@ -190,3 +190,64 @@ def test_present_matches_oracle(encode, tone, grade, strength):
     assert np.array_equal(got, want), f"{(got != want).any(axis=-1).sum()} pixels differ"
     assert got[..., :3].std() > 5  # an image, not a flat fill
     r.close()
+
+
+def test_taa_frames_match_oracle(small_world):
+    """TAA (vx_taa) over four frames of a camera turning and moving: each frame's
+    prev_view_proj is the previous camera's view_proj, the first frame's feedback is 1
+    (no history), later ones 0.1, jittered from the second frame on. Every output half
+    equals the oracle's TAA of the same resolve colour and history; the present after TAA
+    draws the TAA output."""
+    import copy
+    import math
+
+    from tests.test_taa import _vp
+    from voxelcraft_menu_amd.world import CameraParams
+
+    wl = W.WORKLOADS[0]
+    flags = BASE_FLAGS | SKY_TINT_BIT
+    atlas, faces = _atlas(3), _faces_perm()
+    r = Renderer(200, 120)
+    r.sync_world(small_world)
+    r.set_resolve_assets(atlas, SIZE, LAYERS, MIPS, faces)
+    base = W.default_camera(wl, flags=flags, width=200, height=120)
+    prev_m = None
+    hist = np.zeros((120, 200, 4), np.uint16)
+    for k in range(4):
+        cam = CameraParams(pos=(base.pos[0] + 0.7 * k, base.pos[1] - 0.3 * k, base.pos[2] + 0.2 * k),
+                           yaw=base.yaw + 0.02 * k, pitch=base.pitch, width=200, height=120, sea_level=base.sea_level,
+                           flags=flags)
+        v = build_view(small_world, cam)
+        fr = v.frame
+        fr.ambient, fr.mip_bias = 0.04, -4.0
+        if k:
+            fr.jitter[0], fr.jitter[1] = (0.25, -0.125) if k % 2 else (-0.375, 0.3125)
+        m = _vp(fr)
+        fr.taa_feedback = 1.0 if k == 0 else 0.1
+        pm = m if prev_m is None else prev_m
+        for c in range(4):
+            for rr in range(4):
+                fr.prev_view_proj[c * 4 + rr] = float(pm[rr, c])
+        prev_m = m
+        spec, hi = W.spec_words(cam.flags, small_world)
+        hi = (hi | SKY_SPECULAR) & ~(64 | 8192 | 16384 | 65536)
+        r.set_view(v)
+        r.render(fr, spec, hi)
+        r.resolve()
+        r.taa()
+        r.sync()
+        print(f"taa frame {k}", flush=True)
+        sc = O.Scene(small_world.pool(0), small_world.pool(1), small_world.pool(2), v.chunks, v.grid)
+        vis = r.read_vis()
+        want = O.taa(sc, bytes(fr), vis, r.read_color(), hist)
+        got = r.read_taa()
+        diff = got != want
+        assert not diff.any(), f"frame {k}: {diff.any(-1).sum()} pixels differ, first {np.argwhere(diff)[:3].tolist()}"
+        if k:
+            assert (got[..., :3] != O.taa(sc, bytes(copy.copy(fr)), vis, r.read_color(), np.zeros_like(hist))[..., :3]).any()
+        hist = got
+    r.present(encode=1)
+    img = r.read_image()
+    ref = O.present(hist.view(np.float16).astype(np.float32), encode=1)
+    assert np.array_equal(img, ref)
+    r.close()

@@ -372,6 +372,9 @@ _sig_march = {
     "vx_pool_mirror_get": (C.c_int, [P, C.c_int, C.POINTER(PoolMirror)]),
     "vx_pool_read": (C.c_int, [P, C.c_int, P, C.c_size_t]),
     "vx_pool_set_flat": (C.c_int, [P, P, C.c_size_t]),
+    "vx_taa": (C.c_int, [P]),
+    "vx_read_taa": (C.c_int, [P, P, C.c_size_t]),
+    "vx_write_taa_history": (C.c_int, [P, P, C.c_size_t]),
     "vx_render": (C.c_int, [P, C.POINTER(Frame), C.c_uint32, C.c_uint32]),
     "vx_sync": (C.c_int, [P]),
     "vx_read_vis": (C.c_int, [P, P, C.c_size_t]),

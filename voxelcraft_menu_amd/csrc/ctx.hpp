@@ -93,6 +93,12 @@ struct vx_ctx {
     uint32_t spec_word = 0, spec_hi_word = 0;
     // Resolve: the caller's atlas and face table, and the colour target.
     vxh::DevBuf atlas, faces, color, image;
+    // TAA: two f16 RGBA histories (the last output is hist[taa_last]) and the f32 copy of
+    // one that the present reads; taa_valid while the last pass over the colour was vx_taa.
+    vxh::DevBuf hist[2], taa_f32;
+    size_t hist_len = 0;
+    uint32_t taa_last = 0;
+    bool taa_valid = false;
     uint32_t atlas_size = 0, atlas_layers = 0, atlas_mips = 0, n_faces = 0;
     bool color_valid = false, image_valid = false;
 };

@@ -10,6 +10,8 @@
 // vxresolve.inc); the elementary functions are the library's own (log2, exp2, pow =
 // exp2(y log2 x), exp = wave_exp). Water and glass pixels belong to lanes 1 and 2: lane 0
 // leaves them at alpha 0.
+#include <cstring>
+
 #include "ctx.hpp"
 
 namespace vxd {
@@ -626,6 +628,182 @@ __global__ __launch_bounds__(256) void k_present(const float4 *__restrict__ src,
     out[i] = unorm8(c[0]) | (unorm8(c[1]) << 8) | (unorm8(c[2]) << 16) | (255u << 24);
 }
 
+// ---- TAA (taa.wgsl:1-127), as the oracle's vxo_taa ----
+// One wave per 8x8 tile. The 10x10 colour neighbourhood (edge texels repeated) is staged
+// in LDS as the trace output holds it (the resolve's f32 colour rounded to f16); each
+// lane reprojects its pixel through prev_view_proj, filters the f16 history at that
+// position (Catmull-Rom in five clamp-to-edge bilinear taps), clips it in YCoCg to the
+// neighbourhood's mean +- 1 sigma and mixes it with the colour by taa_feedback. The
+// output is stored as f16 RGBA (alpha 1) and is the next frame's history.
+struct TaaK {
+    Pools p;
+    FrameK f;
+    float prev[16];  // prev_view_proj, column major
+    float feedback;
+    const unsigned long long *__restrict__ vis;
+    const float4 *__restrict__ src;
+    const ushort4 *__restrict__ hist;
+    ushort4 *__restrict__ out;
+};
+
+__device__ __forceinline__ float f16r(float x) { return (float)(_Float16)x; }
+__device__ __forceinline__ float h2f(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
+__device__ __forceinline__ uint16_t f2h(float x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
+
+__device__ __forceinline__ void rgb_to_ycocg(const float c[3], float o[3]) {
+    o[0] = 0.25f * c[0] + 0.5f * c[1] + 0.25f * c[2];
+    o[1] = 0.5f * c[0] - 0.5f * c[2];
+    o[2] = -0.25f * c[0] + 0.5f * c[1] - 0.25f * c[2];
+}
+__device__ __forceinline__ void ycocg_to_rgb(const float c[3], float o[3]) {
+    const float t = c[0] - c[2];
+    o[0] = t + c[1];
+    o[1] = c[0] + c[2];
+    o[2] = t - c[1];
+}
+
+__device__ __forceinline__ void hist_bilinear(const ushort4 *hist, uint32_t w, uint32_t h, float u, float v,
+                                              float o[3]) {
+    const float x = u * (float)w - 0.5f, y = v * (float)h - 0.5f;
+    const float fx = floorf(x), fy = floorf(y);
+    const float a = x - fx, b = y - fy;
+    const int32_t x0 = clamp_texel((int32_t)fx, (int32_t)w), x1 = clamp_texel((int32_t)fx + 1, (int32_t)w);
+    const int32_t y0 = clamp_texel((int32_t)fy, (int32_t)h), y1 = clamp_texel((int32_t)fy + 1, (int32_t)h);
+    const ushort4 t00 = hist[(size_t)y0 * w + x0], t10 = hist[(size_t)y0 * w + x1];
+    const ushort4 t01 = hist[(size_t)y1 * w + x0], t11 = hist[(size_t)y1 * w + x1];
+    const float c00[3] = {h2f(t00.x), h2f(t00.y), h2f(t00.z)}, c10[3] = {h2f(t10.x), h2f(t10.y), h2f(t10.z)};
+    const float c01[3] = {h2f(t01.x), h2f(t01.y), h2f(t01.z)}, c11[3] = {h2f(t11.x), h2f(t11.y), h2f(t11.z)};
+#pragma unroll
+    for (int k = 0; k < 3; k++) o[k] = r_mix(r_mix(c00[k], c10[k], a), r_mix(c01[k], c11[k], a), b);
+}
+
+__global__ __launch_bounds__(64) void k_taa(TaaK k) {
+    __shared__ float tile[100][3];
+    const FrameK &f = k.f;
+    const uint32_t w = f.res[0], h = f.res[1];
+    const uint32_t t = blockIdx.x;
+    if (t >= f.tiles[0] * f.tiles[1]) return;
+    const int32_t gx = (int32_t)(t % f.tiles[0]), gy = (int32_t)(t / f.tiles[0]);
+    const uint32_t lane = threadIdx.x;
+    for (uint32_t q = lane; q < 100u; q += 64u) {
+        const int32_t x = min(max(gx * 8 - 1 + (int32_t)(q % 10u), 0), (int32_t)w - 1);
+        const int32_t y = min(max(gy * 8 - 1 + (int32_t)(q / 10u), 0), (int32_t)h - 1);
+        const float4 c = k.src[(size_t)y * w + x];
+        tile[q][0] = f16r(c.x);
+        tile[q][1] = f16r(c.y);
+        tile[q][2] = f16r(c.z);
+    }
+    __syncthreads();
+    const uint32_t lx = lane & 7u, ly = lane >> 3;
+    const uint32_t px = (uint32_t)gx * 8u + lx, py = (uint32_t)gy * 8u + ly;
+    if (px >= w || py >= h) return;
+    float cur[3], o[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) o[q] = cur[q] = tile[(ly + 1u) * 10u + lx + 1u][q];
+    const bool heat = (f.flags & VX_FLAG_HEATMAP) != 0u;
+    if (k.feedback < 1.0f && !heat) {
+        float rd[3];
+        ray_dir(f, (float)px + 0.5f, (float)py + 0.5f, rd[0], rd[1], rd[2]);
+        const uint64_t key = k.vis[(size_t)py * w + px];
+        float q4[4];
+        if (key == 0ull) {
+            q4[0] = rd[0]; q4[1] = rd[1]; q4[2] = rd[2]; q4[3] = 0.0f;
+        } else {
+            const uint32_t ci = (uint32_t)(key >> 24) & 0x1FFFu, low = (uint32_t)(key & 0xFFFFFFull);
+            const ChunkK c = ld_chunk(k.p, ci);
+            const float th = hit_t(f, c, (low >> 16) & 63u, (low >> 8) & 63u, low & 63u, (low >> 22) & 3u,
+                                   (low >> 14) & 3u, (low >> 6) & 3u, (uint32_t)(key >> 37) & 7u, rd);
+#pragma unroll
+            for (int q = 0; q < 3; q++) q4[q] = f.cam[q] + rd[q] * th;
+            q4[3] = 1.0f;
+        }
+        float clip[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+            clip[r] = ((k.prev[r] * q4[0] + k.prev[4 + r] * q4[1]) + k.prev[8 + r] * q4[2]) + k.prev[12 + r] * q4[3];
+        bool ok = clip[3] > 1e-6f;
+        float pr[2] = {0.0f, 0.0f};
+        if (ok) {
+            const float nx = clip[0] / clip[3], ny = clip[1] / clip[3];
+            const float rx = (float)w, ry = (float)h;
+            pr[0] = (nx * 0.5f + 0.5f) * rx - f.jitter[0];
+            pr[1] = (0.5f - ny * 0.5f) * ry - f.jitter[1];
+            ok = !(pr[0] < 0.0f || pr[1] < 0.0f || pr[0] > rx || pr[1] > ry);
+        }
+        if (ok) {
+            float m1[3] = {0.0f, 0.0f, 0.0f}, m2[3] = {0.0f, 0.0f, 0.0f};
+            for (uint32_t i = 0; i < 9u; i++) {
+                float y[3];
+                rgb_to_ycocg(tile[(ly + i / 3u) * 10u + lx + i % 3u], y);
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    m1[q] = m1[q] + y[q];
+                    m2[q] = m2[q] + y[q] * y[q];
+                }
+            }
+            // history_bicubic (taa.wgsl:28-58)
+            const float res[2] = {(float)w, (float)h};
+            float wt[5], tap[5][2], w0[2], w3[2], w12[2], t0[2], t3[2], t12[2];
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+                const float tex1 = floorf(pr[a] - 0.5f) + 0.5f;
+                const float fr = pr[a] - tex1, f2 = fr * fr, f3 = f2 * fr;
+                w0[a] = -0.5f * f3 + f2 - 0.5f * fr;
+                const float w1 = 1.5f * f3 - 2.5f * f2 + 1.0f;
+                const float w2 = -1.5f * f3 + 2.0f * f2 + 0.5f * fr;
+                w3[a] = 0.5f * f3 - 0.5f * f2;
+                w12[a] = w1 + w2;
+                t0[a] = (tex1 - 1.0f) / res[a];
+                t3[a] = (tex1 + 2.0f) / res[a];
+                t12[a] = (tex1 + w2 / w12[a]) / res[a];
+            }
+            tap[0][0] = t12[0]; tap[0][1] = t0[1];  wt[0] = w12[0] * w0[1];
+            tap[1][0] = t0[0];  tap[1][1] = t12[1]; wt[1] = w0[0] * w12[1];
+            tap[2][0] = t12[0]; tap[2][1] = t12[1]; wt[2] = w12[0] * w12[1];
+            tap[3][0] = t3[0];  tap[3][1] = t12[1]; wt[3] = w3[0] * w12[1];
+            tap[4][0] = t12[0]; tap[4][1] = t3[1];  wt[4] = w12[0] * w3[1];
+            float hb[3] = {0.0f, 0.0f, 0.0f};
+            for (int i = 0; i < 5; i++) {
+                float sm[3];
+                hist_bilinear(k.hist, w, h, tap[i][0], tap[i][1], sm);
+#pragma unroll
+                for (int q = 0; q < 3; q++) hb[q] = hb[q] + sm[q] * wt[i];
+            }
+            const float sum = wt[0] + wt[1] + wt[2] + wt[3] + wt[4];
+#pragma unroll
+            for (int q = 0; q < 3; q++) hb[q] = fmaxf(hb[q] / sum, 0.0f);
+            float hy[3], centre[3], v[3], n[3], cl[3], back[3];
+            rgb_to_ycocg(hb, hy);
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                const float mean = m1[q] / 9.0f;
+                const float sigma = sqrtf(fmaxf(m2[q] / 9.0f - mean * mean, 0.0f));
+                const float lo = mean - 1.0f * sigma, hi = mean + 1.0f * sigma;
+                centre[q] = 0.5f * (hi + lo);
+                const float extent = fmaxf(0.5f * (hi - lo), 1e-5f);
+                v[q] = hy[q] - centre[q];
+                n[q] = fabsf(v[q] / extent);
+            }
+            const float m = fmaxf(n[0], fmaxf(n[1], n[2]));
+#pragma unroll
+            for (int q = 0; q < 3; q++) cl[q] = m > 1.0f ? centre[q] + v[q] / m : hy[q];
+            ycocg_to_rgb(cl, back);
+#pragma unroll
+            for (int q = 0; q < 3; q++) o[q] = r_mix(fmaxf(back[q], 0.0f), cur[q], k.feedback);
+        }
+    }
+    k.out[(size_t)py * w + px] = make_ushort4(f2h(o[0]), f2h(o[1]), f2h(o[2]), (uint16_t)0x3C00u);
+}
+
+// The present of a TAA output: the f16 history read as the blit samples it.
+__global__ __launch_bounds__(256) void k_half_to_float(const ushort4 *__restrict__ src, uint32_t n,
+                                                       float4 *__restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const ushort4 v = src[i];
+    out[i] = make_float4(h2f(v.x), h2f(v.y), h2f(v.z), h2f(v.w));
+}
+
 }  // namespace vxd
 
 using namespace vxd;
@@ -722,6 +900,62 @@ int vx_resolve(vx_ctx *c) {
     VX_LAUNCHED();
     c->color_valid = true;
     c->image_valid = false;
+    c->taa_valid = false;
+    return VX_OK;
+}
+
+// TAA history buffers: two f16 RGBA targets of the context's size, zeroed when (re)made.
+static int taa_targets(vx_ctx *c) {
+    const size_t bytes = (size_t)c->w * c->h * 8;
+    if (c->hist[0].cap >= bytes && c->hist_len == bytes) return VX_OK;
+    for (int i = 0; i < 2; i++) {
+        int rc;
+        if ((rc = vxh::ensure(c->hist[i], bytes))) return rc;
+        HIPX(hipMemsetAsync(c->hist[i].p, 0, bytes, c->stream));
+    }
+    c->hist_len = bytes;
+    c->taa_last = 0;
+    return VX_OK;
+}
+
+int vx_taa(vx_ctx *c) {
+    if (!c || !c->color_valid || !c->have_frame) return VX_E_ARG;
+    HIPX(hipSetDevice(c->device));
+    int rc;
+    if ((rc = taa_targets(c))) return rc;
+    const vx_gpu_frame &g = c->gframe;
+    TaaK k;
+    k.p = vxh::make_pools(c);
+    k.f = c->frame;
+    std::memcpy(k.prev, g.prev_view_proj, sizeof k.prev);
+    k.feedback = g.taa_feedback;
+    k.vis = (const unsigned long long *)c->vis.p;
+    k.src = (const float4 *)c->color.p;
+    k.hist = (const ushort4 *)c->hist[c->taa_last].p;
+    k.out = (ushort4 *)c->hist[c->taa_last ^ 1u].p;
+    k_taa<<<c->tx * c->ty, 64, 0, c->stream>>>(k);
+    VX_LAUNCHED();
+    c->taa_last ^= 1u;
+    c->taa_valid = true;
+    c->image_valid = false;
+    return VX_OK;
+}
+
+int vx_read_taa(vx_ctx *c, uint16_t *rgba, size_t n) {
+    if (!c || !rgba || n != (size_t)c->w * c->h * 4 || !c->taa_valid || c->hist_len != n * 2) return VX_E_ARG;
+    HIPX(hipSetDevice(c->device));
+    HIPX(hipMemcpyAsync(rgba, c->hist[c->taa_last].p, n * 2, hipMemcpyDeviceToHost, c->stream));
+    HIPX(hipStreamSynchronize(c->stream));
+    return VX_OK;
+}
+
+int vx_write_taa_history(vx_ctx *c, const uint16_t *rgba, size_t n) {
+    if (!c || !rgba || n != (size_t)c->w * c->h * 4) return VX_E_ARG;
+    HIPX(hipSetDevice(c->device));
+    int rc;
+    if ((rc = taa_targets(c))) return rc;
+    HIPX(hipMemcpyAsync(c->hist[c->taa_last].p, rgba, n * 2, hipMemcpyHostToDevice, c->stream));
+    HIPX(hipStreamSynchronize(c->stream));
     return VX_OK;
 }
 
@@ -731,7 +965,16 @@ int vx_present(vx_ctx *c, const vx_present_desc *d) {
     const uint32_t n = c->w * c->h;
     int rc;
     if ((rc = vxh::ensure(c->image, (size_t)n * 4))) return rc;
-    k_present<<<(n + 255) / 256, 256, 0, c->stream>>>((const float4 *)c->color.p, n, d->encode, d->tone, d->grade,
+    const float4 *src = (const float4 *)c->color.p;
+    if (c->taa_valid) {
+        // After vx_taa the blit draws the TAA output (src/render/mod.rs:1908, last_blit).
+        if ((rc = vxh::ensure(c->taa_f32, (size_t)n * 16))) return rc;
+        k_half_to_float<<<(n + 255) / 256, 256, 0, c->stream>>>((const ushort4 *)c->hist[c->taa_last].p, n,
+                                                                (float4 *)c->taa_f32.p);
+        VX_LAUNCHED();
+        src = (const float4 *)c->taa_f32.p;
+    }
+    k_present<<<(n + 255) / 256, 256, 0, c->stream>>>(src, n, d->encode, d->tone, d->grade,
                                                       d->strength, (uint32_t *)c->image.p);
     VX_LAUNCHED();
     c->image_valid = true;

@@ -87,6 +87,8 @@ static int alloc_targets(vx_ctx *c, uint32_t w, uint32_t h) {
     if (w == 0 || h == 0) return VX_E_ARG;
     uint32_t tx = (w + 7) / 8, ty = (h + 7) / 8;
     if ((uint64_t)tx * ty > VX_MAX_TILES) return VX_E_OVERFLOW;  // warn_if_oversized
+    // Outputs of the old size are gone: nothing reads them until the next frame.
+    c->color_valid = c->image_valid = c->taa_valid = false;
     c->w = w;
     c->h = h;
     c->tx = tx;
@@ -234,7 +236,8 @@ int vx_ctx_destroy(vx_ctx *c) {
     DevBuf *bufs[] = {&c->pool[0], &c->pool[1],   &c->pool[2], &c->chunks,      &c->grid,     &c->vis,
                       &c->shadow,  &c->hiz,       &c->pairs,   &c->counters,    &c->cost,     &c->order,
                       &c->heat,    &c->build_dense, &c->build_leaf, &c->build_words, &c->build_l1, &c->build_out,
-                      &c->atlas,   &c->faces,     &c->color,  &c->image,       &c->flat};
+                      &c->atlas,   &c->faces,     &c->color,  &c->image,       &c->flat,
+                      &c->hist[0], &c->hist[1],   &c->taa_f32};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto &set : c->ring)

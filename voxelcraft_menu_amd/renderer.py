@@ -167,6 +167,23 @@ class Renderer:
         A.check(self._L.vx_read_color(self._c, A.ptr(out), out.size), "vx_read_color")
         return out
 
+    def taa(self):
+        """The TAA pass (taa.wgsl) over the last resolve, with the frame's taa_feedback and
+        prev_view_proj; its output becomes the history and what present draws."""
+        A.check(self._L.vx_taa(self._c), "vx_taa")
+
+    def read_taa(self) -> np.ndarray:
+        """(H, W, 4) f16 bit patterns (uint16) of the last TAA output."""
+        w, h = self.size
+        out = np.zeros((h, w, 4), dtype=np.uint16)
+        A.check(self._L.vx_read_taa(self._c, A.ptr(out), out.size), "vx_read_taa")
+        return out
+
+    def write_taa_history(self, hist: np.ndarray):
+        """Replace the history the next taa() reads ((H, W, 4) f16 bit patterns)."""
+        h = np.ascontiguousarray(hist, dtype=np.uint16)
+        A.check(self._L.vx_write_taa_history(self._c, A.ptr(h), h.size), "vx_write_taa_history")
+
     def present(self, encode: int = 1, tone: int = 0, grade: int = 0, strength: float = 1.0):
         """The present blit (blit.wgsl) of the last resolve into an RGBA8 image."""
         d = A.PresentDesc(encode, tone, grade, strength)

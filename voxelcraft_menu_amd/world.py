@@ -485,3 +485,32 @@ def relocate_view(view: "ViewData", table: np.ndarray, moved: np.ndarray) -> "Vi
         ch[j] = moved[index[key(view.chunks[j])]]
     return ViewData(ch, view.grid, view.frame, view.n_visible, view.n_total, view.camera_chunk, view.width,
                     view.height)
+
+
+NEAR = 0.05  # src/render/mod.rs:140
+
+
+def view_proj(frame: A.Frame) -> np.ndarray:
+    """math::view_proj (src/math.rs:42-67) of a frame's camera: the 4x4 matrix (row, col)
+    a later frame's prev_view_proj holds, in float64."""
+    pos = np.array(frame.cam_pos, np.float64)
+    fwd = np.array(frame.cam_fwd, np.float64)
+    right = np.array(frame.cam_right, np.float64)
+    up = np.array(frame.cam_up, np.float64)
+    view = np.array([[right[0], right[1], right[2], -right @ pos],
+                     [up[0], up[1], up[2], -up @ pos],
+                     [-fwd[0], -fwd[1], -fwd[2], fwd @ pos],
+                     [0.0, 0.0, 0.0, 1.0]])
+    f = 1.0 / float(frame.tan_half_fov)
+    far = float(frame.far_)
+    proj = np.array([[f / float(frame.aspect), 0, 0, 0], [0, f, 0, 0],
+                     [0, 0, far / (NEAR - far), NEAR * far / (NEAR - far)], [0, 0, -1.0, 0]])
+    return proj @ view
+
+
+def set_prev_view_proj(frame: A.Frame, m: np.ndarray):
+    """Store a (row, col) matrix in frame.prev_view_proj, column major as the GpuFrame
+    holds a Mat4."""
+    for c in range(4):
+        for r in range(4):
+            frame.prev_view_proj[c * 4 + r] = float(m[r, c])
