@@ -421,8 +421,9 @@ def run_moving(args, world, cams, specs, tuned, px):
 
 
 def run_resolve(r1, world, wl1, args):
-    """Resolve lane 0 (vx_resolve) of the 1080p view: the workload's flags less what the
-    resolve does not restate (probes, biome tint, distant terrain shadows), sky tint,
+    """The resolve (vx_resolve: lanes 0, 1 and 2) of the 1080p view, then TAA: the workload's
+    flags less what the resolve does not restate (probes, biome tint, distant terrain
+    shadows), sky tint,
     sky specular and fog on, a synthetic 27-layer 16x16 atlas with its mip chain. The
     march runs once; the resolve alone is timed over the steps."""
     from voxelcraft_menu_amd import workloads as W

@@ -254,20 +254,28 @@ int vx_march_stats_get(vx_ctx *ctx, vx_march_stats *out);
 /* The same summary of a caller-held counter (no device work). */
 int vx_march_stats_compute(const uint32_t *heat, size_t n, uint32_t chunks, vx_march_stats *out);
 
-/* ---- Resolve, lane 0 ---------------------------------------------------------------
- * The reference's resolve pass (src/render/shaders/resolve.wgsl:1378-1466, lane 0) over
- * the last vx_render frame, into an RGBA32F colour target (W*H*4 floats): a FLAG_HEATMAP
+/* ---- Resolve, lanes 0, 1 and 2 ------------------------------------------------------
+ * The reference's resolve passes (src/render/shaders/resolve.wgsl:1378-1466, the three
+ * RESOLVE_LANE pipelines, with the water_sec pass before them, :1469-1535) over the last
+ * vx_render frame, into an RGBA32F colour target (W*H*4 floats): a FLAG_HEATMAP
  * frame becomes heat(iterations / 300); a miss becomes the sky (sky_color without a cloud
  * deck); an opaque hit is shaded from its key as shade_hit_compact_cached /
  * shade_hit_legacy_cached do (resolve.wgsl:1537-2059: smooth light and AO from the 3x3
  * neighbourhood in front of the face, the sun visibility of the shadow passes -- 0 where
  * the shadow word holds a blocker distance, 1 elsewhere, shadow.wgsl:61-116 without soft
  * shadows --, sky specular, the atlas texel at the footprint's mip level) and mixed with
- * sky_base by the fog. Water and
- * glass hits are lanes 1 and 2 of the reference: their pixels keep alpha 0 here. The
- * frame's shading fields (fog_*, ambient, mip_bias, haze_warm, zenith_deep) are read as
- * given. Views that need what is not restated return VX_E_UNSUPPORTED: probes, biome tint
- * (FLAG_TINT with its spec bit), caustics, soft shadows, foliage-rich or canopy-relief albedo, distant
+ * sky_base by the fog. A glass hit (lane 1) is shade_glass (:1261-1319): the transmit leg
+ * traced and shaded (shade_water where it ends on water), the sky or, with
+ * SPEC_GLASS_REFLECT, the traced reflect leg, mixed by schlick_glass. A water hit (lane 2)
+ * is shade_water / shade_water_sec (:1078-1259): water_ctx (tint, medium light, wave
+ * normals, gloss, roughness), the refract and reflect legs -- per pixel, or with
+ * SPEC_WATER_SEC one pair per 2x2 block stored as f16 where water_share_safe holds --,
+ * shore foam, glint and schlick. Leg hits are shaded by shade_hit_legacy with the sun
+ * visibility of shadow_ray, without sky specular, with flat light under
+ * SPEC_FLAT_SECONDARY. The frame's shading fields (fog_*, ambient, mip_bias, haze_warm,
+ * zenith_deep, water_absorb, the wave fields) are read as given. Views that need what is
+ * not restated return VX_E_UNSUPPORTED: probes, biome tint (FLAG_TINT with its spec bit),
+ * caustics, soft shadows, foliage-rich or canopy-relief albedo, the water look, distant
  * terrain shadows, a cloud deck (cloud_cover > 0), terrain god-ray shafts
  * (godray_strength > 0 with FLAG_TERRAIN_SHAFTS), the underwater view.
  * vx_resolve_set_assets copies the atlas and block face table the reference's host builds

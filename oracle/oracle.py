@@ -150,9 +150,17 @@ def _p(a):
 
 
 class RSpec(C.Structure):
-    """vxo_rspec: the resolve's override switches."""
+    """vxo_rspec: the resolve's override switches; `lanes` adds the glass and water lanes,
+    whose traces read the march switches `march` points to."""
     _fields_ = [(k, C.c_uint32) for k in ("sky_specular", "lighting_repair", "sky_tint", "sky_cool", "sky_look",
-                                          "light_rgb", "emitter_gather", "shore_wet", "tex_variation", "glass")]
+                                          "light_rgb", "emitter_gather", "shore_wet", "tex_variation", "glass",
+                                          "lanes", "water_sec", "shore_foam", "flat_secondary")] + \
+        [("march", C.c_void_p)]
+
+    def with_march(self, spec: "Spec") -> "RSpec":
+        self._march = spec  # kept alive with the record
+        self.march = C.cast(C.pointer(spec), C.c_void_p).value
+        return self
 
 
 class ResolveAssets(C.Structure):
@@ -167,13 +175,16 @@ def rspec_from(flags: int, spec_hi: int) -> RSpec:
                  sky_tint=1 if flags & 2147483648 else 0, sky_cool=1 if spec_hi & 2048 else 0,
                  sky_look=1 if spec_hi & 1024 else 0, light_rgb=1 if flags & 268435456 else 0,
                  emitter_gather=1 if spec_hi & 4 else 0, shore_wet=1 if spec_hi & 8 else 0,
-                 tex_variation=1 if flags & 16384 else 0, glass=1 if flags & 131072 else 0)
+                 tex_variation=1 if flags & 16384 else 0, glass=1 if flags & 131072 else 0,
+                 lanes=1, water_sec=1 if spec_hi & 32 else 0, shore_foam=1 if spec_hi & 16 else 0,
+                 flat_secondary=1 if flags & 1048576 else 0).with_march(spec_from(flags, spec_hi))
 
 
 def resolve(scene: "Scene", frame_bytes: bytes, rspec: RSpec, atlas: np.ndarray, size: int, layers: int, mips: int,
             block_faces: np.ndarray, vis: np.ndarray, sun: np.ndarray | None, heat: np.ndarray | None = None,
             threads: int = 0) -> np.ndarray:
-    """Resolve lane 0 of a frame: (H, W, 4) float32 (alpha 0 where lane 0 writes nothing)."""
+    """The resolve of a frame: (H, W, 4) float32; lanes 1 and 2 (glass and water pixels) when
+    rspec.lanes is set, else alpha 0 there."""
     f = np.frombuffer(frame_bytes, dtype=np.uint8).copy()
     res = np.frombuffer(frame_bytes[80:88], dtype=np.uint32)
     w, h = int(res[0]), int(res[1])

@@ -812,6 +812,31 @@ float vxo_wave_cos(float x) {
     }
 }
 
+/* sin with the same reduction and polynomials as vxo_wave_cos (sin = the sine branch for
+ * k = 0 mod 4, then cos, -sin, -cos). */
+float vxo_wave_sin(float x) {
+    const float kf = rintf(x * 0x1.45f306p-1f);
+    const float r = ((x - kf * 0x1.922p0f) - kf * -0x1.2aep-18f) - kf * -0x1.de973ep-31f;
+    const float r2 = r * r;
+    float cs = 0x1.a01a02p-16f;
+    cs = cs * r2 - 0x1.6c16c2p-10f;
+    cs = cs * r2 + 0x1.555556p-5f;
+    cs = cs * r2 - 0.5f;
+    cs = cs * r2 + 1.0f;
+    float sn = 0x1.71de3ap-19f;
+    sn = sn * r2 - 0x1.a01a02p-13f;
+    sn = sn * r2 + 0x1.111112p-7f;
+    sn = sn * r2 - 0x1.555556p-3f;
+    sn = sn * r2 + 1.0f;
+    sn = sn * r;
+    switch ((int32_t)kf & 3) {
+        case 0: return sn;
+        case 1: return cs;
+        case 2: return -sn;
+        default: return -cs;
+    }
+}
+
 float vxo_wave_exp(float x) {
     if (x < -86.0f) return 0.0f;
     const float kf = rintf(x * 0x1.715476p0f);                        /* 1/ln 2 */
@@ -900,15 +925,24 @@ void vxo_wave_field(const vxo_frame *f, const vxo_spec *s, float px, float pz, f
  * normal, or on waved top faces about the wave normal at the reflect slope, blended back
  * toward the face normal by the optical roughness of the unresolved wave variance. The
  * water-look ripples (SPEC_WATER_LOOK) are not restated; callers keep them off. */
-static void water_mirror_t(ctx *c, uint32_t nid, const float hit[3], const float rd[3], float t, float out[3]) {
+/* water_ctx's normals (resolve.wgsl:1078-1167): n the face normal; on waved top faces
+ * nw the wave normal and nt the wave normal at the reflect slope (the shoal depth from a
+ * short trace straight down scales the waves), gloss from the unresolved variance; both
+ * blended back toward n by the optical roughness of that variance. */
+typedef struct wnormals {
+    float n[3], nw[3], nt[3], gloss, rough;
+    int waved;
+} wnormals;
+
+static void water_normals(ctx *c, uint32_t nid, const float hit[3], const float rd[3], float t, wnormals *o) {
     const vxo_frame *f = c->f;
     const vxo_spec *s = c->s;
-    float n[3], nt[3];
-    normal_of(nid, n);
-    for (int k = 0; k < 3; k++) nt[k] = n[k];
+    normal_of(nid, o->n);
+    for (int k = 0; k < 3; k++) o->nw[k] = o->nt[k] = o->n[k];
+    o->gloss = 1.0f;
     float unresolved = 0.0f;
-    const int waved = s->waves && (f->flags & FLAG_WAVES) != 0u && nid == NORMAL_PY;
-    if (waved) {
+    o->waved = s->waves && (f->flags & FLAG_WAVES) != 0u && nid == NORMAL_PY;
+    if (o->waved) {
         const float px_world = fmaxf(t, 1e-3f) * 2.0f * f->tan_half_fov / (float)f->res[1];
         float kk = 0.0f, vh[2] = {0.0f, 0.0f};
         const int aniso = s->wave_aniso && (f->flags & FLAG_WAVE_ANISO) != 0u;
@@ -930,12 +964,17 @@ static void water_mirror_t(ctx *c, uint32_t nid, const float hit[3], const float
         float w[3];
         vxo_wave_field(f, s, hit[0], hit[2], px_world, vh, kk, aniso, shoal, w);
         unresolved = fmaxf(w[2], 0.0f);
+        o->nw[0] = -w[0];
+        o->nw[1] = 1.0f;
+        o->nw[2] = -w[1];
+        wgsl_normalize(o->nw);
         const float len = sqrtf(w[0] * w[0] + w[1] * w[1]);
         const float m = fminf(1.0f, f->wave_reflect_slope / fmaxf(len, 1e-6f));
-        nt[0] = -(w[0] * m);
-        nt[1] = 1.0f;
-        nt[2] = -(w[1] * m);
-        wgsl_normalize(nt);
+        o->nt[0] = -(w[0] * m);
+        o->nt[1] = 1.0f;
+        o->nt[2] = -(w[1] * m);
+        wgsl_normalize(o->nt);
+        o->gloss = 1.0f / (1.0f + 160.0f * w[2]); /* 0.5 * GLINT_POWER */
     }
 #if VXO_PLANT == 10
     const float rough = 0.0f;
@@ -943,10 +982,21 @@ static void water_mirror_t(ctx *c, uint32_t nid, const float hit[3], const float
 #else
     const float rough = fminf(fmaxf(1.0f - vxo_wave_exp(-8.0f * unresolved), 0.0f), 1.0f);
 #endif
-    for (int k = 0; k < 3; k++) nt[k] = nt[k] * (1.0f - rough) + n[k] * rough;
-    wgsl_normalize(nt);
-    wgsl_reflect(rd, nt, out);
-    if (waved) {
+    o->rough = rough;
+    for (int k = 0; k < 3; k++) {
+        o->nw[k] = o->nw[k] * (1.0f - rough) + o->n[k] * rough;
+        o->nt[k] = o->nt[k] * (1.0f - rough) + o->n[k] * rough;
+    }
+    wgsl_normalize(o->nw);
+    wgsl_normalize(o->nt);
+}
+
+/* water_ctx's mirror_t: reflect(rd, nt), on waved faces kept above the horizon. */
+static void water_mirror_t(ctx *c, uint32_t nid, const float hit[3], const float rd[3], float t, float out[3]) {
+    wnormals w;
+    water_normals(c, nid, hit, rd, t, &w);
+    wgsl_reflect(rd, w.nt, out);
+    if (w.waved) {
         out[1] = fmaxf(out[1], 0.0f);
         wgsl_normalize(out);
     }

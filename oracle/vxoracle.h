@@ -151,6 +151,7 @@ void vxo_march_chunk(const vxo_pools *p, const vxo_chunk *chunks, uint32_t n_chu
  * WGSL leaves their accuracy to the implementation). */
 float vxo_wave_cos(float x);
 float vxo_wave_exp(float x);
+float vxo_wave_sin(float x);
 /* wave_field (resolve.wgsl:946-967): x, y = slope gradient, z = unresolved variance. */
 void vxo_wave_field(const vxo_frame *f, const vxo_spec *s, float px, float pz, float px_world,
                     const float vh[2], float kk, int aniso, float shoal, float out[3]);
@@ -239,6 +240,12 @@ float vxo_pow(float x, float y);
 typedef struct vxo_rspec {
     uint32_t sky_specular, lighting_repair, sky_tint, sky_cool, sky_look, light_rgb, emitter_gather,
         shore_wet, tex_variation, glass;
+    /* Lanes 1 and 2 (glass and water pixels; 0: lane 0 only, those pixels stay at alpha
+     * 0), SPEC_WATER_SEC with WATER_SEC_SHIFT 1, SPEC_SHORE_FOAM, SPEC_FLAT_SECONDARY, and the
+     * march switches the legs' traces and water_ctx read (waves, water repair, glass
+     * reflect); march may be NULL when lanes is 0. */
+    uint32_t lanes, water_sec, shore_foam, flat_secondary;
+    const vxo_spec *march;
 } vxo_rspec;
 
 /* The caller's atlas (RGBA8 sRGB: mip level 0 of every layer, then level 1, ...; level l

@@ -54,6 +54,13 @@ def _pillar_world():
     return w
 
 
+def _block_ids(world, v, vis):
+    """Block id of each pixel's key (0 for a miss), through the oracle's decode."""
+    sc = O.Scene(world.pool(0), world.pool(1), world.pool(2), v.chunks, v.grid)
+    dec = O.decode(sc, v.frame_bytes, vis)
+    return np.where(vis.reshape(dec.shape) != 0, dec["material"], 0).reshape(v.frame.res[1], v.frame.res[0])
+
+
 def _check(r, world, v, spec, hi, atlas, faces, tag, frames=1):
     sc = O.Scene(world.pool(0), world.pool(1), world.pool(2), v.chunks, v.grid)
     hz = np.zeros(v.tiles[0] * v.tiles[1], np.uint32)
@@ -114,9 +121,11 @@ def test_small_world_views_match_oracle(small_world):
         r.set_resolve_assets(atlas, SIZE, LAYERS, MIPS, faces)
         print(f"resolve view {i}", flush=True)
         got = _check(r, small_world, v, spec, hi, atlas, faces, f"view {i}", frames=2)
-        lane12 += int((got[..., 3] == 0.0).sum())
+        ids = _block_ids(small_world, v, r.read_vis())
+        lane12 += int(((ids == 13) | (ids == 10)).sum())
+        assert np.all(got[..., 3] == 1.0)  # lanes 1 and 2 shade the water and glass pixels
         r.close()
-    assert lane12 > 0  # water or glass pixels were left to lanes 1 and 2
+    assert lane12 > 0  # water or glass pixels were among them
 
 
 def test_heatmap_frame_matches_oracle(small_world):
@@ -251,3 +260,48 @@ def test_taa_frames_match_oracle(small_world):
     ref = O.present(hist.view(np.float16).astype(np.float32), encode=1)
     assert np.array_equal(img, ref)
     r.close()
+
+
+@pytest.mark.parametrize("case", ["default", "per_pixel", "waves", "glass_reflect", "smooth_legs", "legacy"])
+def test_water_and_glass_lanes_match_oracle(case):
+    """Lanes 1 and 2 on the pool / land / glass-pane chunk of the closed-form pins, from
+    three cameras: with SPEC_WATER_SEC and shore foam (the defaults), legs per pixel,
+    waves on, SPEC_GLASS_REFLECT, smooth light at the leg hits, the legacy lighting and
+    water; every colour bit-exact against the oracle."""
+    from tests.test_oracle_lanes_pins import ABSORB, FLAT_SECONDARY, _dense
+    world = World()
+    world.insert_dense((0, 0, 0), _dense())
+    flags = (A.FLAG_SHADOWS | 8 | A.FLAG_WATER_REFLECT | A.FLAG_WATER_REFRACT | A.FLAG_GLASS | A.FLAG_WATER_SHADOW_CUT
+             | FLAT_SECONDARY | SKY_TINT_BIT)
+    hi = A.FLAG_HI_FULL_MARCH | SKY_SPECULAR | 32 | 16
+    if case == "per_pixel":
+        hi &= ~32
+    if case == "waves":
+        flags |= 512 | 1024 | 2048 | 4096
+    if case == "glass_reflect":
+        hi |= A.FLAG_HI_GLASS_REFLECT
+    if case == "smooth_legs":
+        flags &= ~FLAT_SECONDARY
+    if case == "legacy":
+        hi |= LEGACY_LIGHTING | A.FLAG_HI_LEGACY_WATER
+    atlas, faces = _atlas(11), _faces_perm()
+    shaded = 0
+    for k, (pos, yaw, pitch) in enumerate([((30.5, 56.3, 2.5), 78.0, -42.0), ((16.5, 50.3, 30.5), 90.0, -60.0),
+                                           ((50.5, 44.3, 70.5), -100.0, -30.0)]):
+        cam = H.camera(pos, yaw, pitch, 160, 96, flags, sea=26.0)
+        v = build_view(world, cam)
+        f = v.frame
+        f.ambient, f.mip_bias, f.water_absorb = 0.05, -2.0, ABSORB
+        f.fog_density, f.fog_falloff, f.fog_height = 0.003, 0.01, 30.0
+        f.time, f.wave_amp, f.wave_scale, f.wave_speed, f.wave_reflect_slope = 2.5, 0.12, 32.0, 0.32, 0.35
+        r = Renderer(160, 96)
+        r.sync_world(world)
+        r.set_resolve_assets(atlas, SIZE, LAYERS, MIPS, faces)
+        print(f"lanes {case} camera {k}", flush=True)
+        got = _check(r, world, v, flags, hi, atlas, faces, f"lanes {case} camera {k}")
+        ids = _block_ids(world, v, r.read_vis())
+        lanes = (ids == 13) | (ids == 10)
+        assert np.all(got[..., 3][lanes] == 1.0)
+        shaded += int(lanes.sum())
+        r.close()
+    assert shaded > 2000

@@ -96,6 +96,7 @@ struct vx_ctx {
     // TAA: two f16 RGBA histories (the last output is hist[taa_last]) and the f32 copy of
     // one that the present reads; taa_valid while the last pass over the colour was vx_taa.
     vxh::DevBuf hist[2], taa_f32;
+    vxh::DevBuf sec_r, sec_m;  // the resolve's water_sec legs (2x2 blocks, f16 RGBA)
     size_t hist_len = 0;
     uint32_t taa_last = 0;
     bool taa_valid = false;

@@ -1204,19 +1204,24 @@ __device__ inline void wave_field(const FrameK &f, const Spec &s, float hx, floa
     }
 }
 
-// water_ctx's reflect-leg direction mirror_t (resolve.wgsl:1078-1167): about the face
-// normal, or on waved top faces about the wave normal at the reflect slope (the shoal
-// depth from a short trace straight down), blended back toward the face normal by the
-// optical roughness of the unresolved wave variance. Water-look ripples are refused.
-__device__ inline void water_mirror_t(const Pools &p, const FrameK &f, const Spec &s, uint32_t nid, const float hit[3],
-                                      const float rd[3], float t, float out[3]) {
-    float n[3], nt[3];
-    normal_of(nid, n);
+// water_ctx's normals (resolve.wgsl:1078-1167): n the face normal; on waved top faces nw
+// the wave normal and nt the wave normal at the reflect slope (the shoal depth from a
+// short trace straight down scales the waves) and gloss from the unresolved variance;
+// nw and nt blended back toward n by the optical roughness of that variance. Water-look
+// ripples are refused.
+struct WaterNormals {
+    float n[3], nw[3], nt[3], gloss, rough;
+    bool waved;
+};
+__device__ inline void water_normals(const Pools &p, const FrameK &f, const Spec &s, uint32_t nid, const float hit[3],
+                                     const float rd[3], float t, WaterNormals &o) {
+    normal_of(nid, o.n);
 #pragma unroll
-    for (int k = 0; k < 3; k++) nt[k] = n[k];
+    for (int k = 0; k < 3; k++) o.nw[k] = o.nt[k] = o.n[k];
+    o.gloss = 1.0f;
     float unresolved = 0.0f;
-    const bool waved = (s.bits & SPEC_WAVES) && (f.flags & FLAG_WAVES) && nid == NORMAL_PY;
-    if (waved) {
+    o.waved = (s.bits & SPEC_WAVES) && (f.flags & FLAG_WAVES) && nid == NORMAL_PY;
+    if (o.waved) {
         const float px_world = fmaxf(t, 1e-3f) * 2.0f * f.tan_half_fov / (float)f.res[1];
         float kk = 0.0f, vh[2] = {0.0f, 0.0f};
         const bool aniso = (s.bits & SPEC_WAVE_ANISO) && (f.flags & FLAG_WAVE_ANISO);
@@ -1237,19 +1242,37 @@ __device__ inline void water_mirror_t(const Pools &p, const FrameK &f, const Spe
         float w[3];
         wave_field(f, s, hit[0], hit[2], px_world, vh, kk, aniso, shoal, w);
         unresolved = fmaxf(w[2], 0.0f);
+        o.nw[0] = -w[0];
+        o.nw[1] = 1.0f;
+        o.nw[2] = -w[1];
+        wgsl_normalize(o.nw);
         const float len = sqrtf(w[0] * w[0] + w[1] * w[1]);
         const float m = fminf(1.0f, f.wave_reflect_slope / fmaxf(len, 1e-6f));
-        nt[0] = -(w[0] * m);
-        nt[1] = 1.0f;
-        nt[2] = -(w[1] * m);
-        wgsl_normalize(nt);
+        o.nt[0] = -(w[0] * m);
+        o.nt[1] = 1.0f;
+        o.nt[2] = -(w[1] * m);
+        wgsl_normalize(o.nt);
+        o.gloss = 1.0f / (1.0f + 160.0f * w[2]);  // 0.5 * GLINT_POWER
     }
     const float rough = fminf(fmaxf(1.0f - wave_exp(-8.0f * unresolved), 0.0f), 1.0f);
+    o.rough = rough;
 #pragma unroll
-    for (int k = 0; k < 3; k++) nt[k] = nt[k] * (1.0f - rough) + n[k] * rough;
-    wgsl_normalize(nt);
-    wgsl_reflect(rd, nt, out);
-    if (waved) {
+    for (int k = 0; k < 3; k++) {
+        o.nw[k] = o.nw[k] * (1.0f - rough) + o.n[k] * rough;
+        o.nt[k] = o.nt[k] * (1.0f - rough) + o.n[k] * rough;
+    }
+    wgsl_normalize(o.nw);
+    wgsl_normalize(o.nt);
+}
+
+// water_ctx's reflect-leg direction mirror_t: reflect(rd, nt), on waved faces kept above
+// the horizon.
+__device__ inline void water_mirror_t(const Pools &p, const FrameK &f, const Spec &s, uint32_t nid, const float hit[3],
+                                      const float rd[3], float t, float out[3]) {
+    WaterNormals w;
+    water_normals(p, f, s, nid, hit, rd, t, w);
+    wgsl_reflect(rd, w.nt, out);
+    if (w.waved) {
         out[1] = fmaxf(out[1], 0.0f);
         wgsl_normalize(out);
     }

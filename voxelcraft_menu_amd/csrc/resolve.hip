@@ -24,7 +24,7 @@ constexpr uint32_t PERM_NONE = 0u, PERM_D4 = 2u;
 // Resolve switches (pipeline-key overrides, src/render/mod.rs:2355-2689).
 constexpr uint32_t RS_SKY_SPECULAR = 1u, RS_LIGHTING_REPAIR = 2u, RS_SKY_TINT = 4u, RS_SKY_COOL = 8u, RS_SKY_LOOK = 16u,
                    RS_LIGHT_RGB = 32u, RS_EMITTER_GATHER = 64u, RS_SHORE_WET = 128u, RS_TEX_VARIATION = 256u,
-                   RS_GLASS = 512u;
+                   RS_GLASS = 512u, RS_WATER_SEC = 1024u, RS_SHORE_FOAM = 2048u, RS_FLAT_SECONDARY = 4096u;
 
 struct ResolveK {
     Pools p;
@@ -39,6 +39,13 @@ struct ResolveK {
     const uint16_t *__restrict__ sun;
     const uint32_t *__restrict__ heat;
     float4 *__restrict__ out;
+    // lanes 1 and 2: the march switches their traces read, and the water_sec legs
+    // (2x2 blocks, hw x hh texels of f16 RGBA: refract rgb + depth, reflect rgb + hit)
+    Spec s;
+    ushort4 *__restrict__ sec_r;
+    ushort4 *__restrict__ sec_m;
+    uint32_t hw, hh;
+    float water_absorb;
 };
 
 // ---- elementary functions (the oracle's vxo_log2 / vxo_exp2 / vxo_pow) ----
@@ -319,11 +326,14 @@ __device__ __forceinline__ uint32_t perm_key(const float b[3], uint32_t nid) {
                     (nid * 0x27d4eb2fu));
 }
 
-// shade_hit_compact_cached / shade_hit_legacy_cached, smooth light, do_spec, faces only
-// (resolve.wgsl:1537-2059).
+// shade_hit_compact_cached / shade_hit_legacy_cached (resolve.wgsl:1537-2059) with smooth
+// light and do_spec, for lane 0; shade_hit_legacy (:289-586) for the secondary legs, whose
+// sun visibility lit_word the caller takes from shadow_ray, without do_spec, and with flat
+// light (the light word in front of the face, no occlusion) when smooth is false. Faces
+// only (not foliage quads).
 __device__ inline void shade_opaque(const ResolveK &k, uint32_t ci, uint32_t vx, uint32_t vy, uint32_t vz, uint32_t id,
                                     uint32_t nid, const float hit[3], const float rd[3], float dist, float lit_word,
-                                    float out[3]) {
+                                    bool smooth, bool do_spec, float out[3]) {
     const Pools &p = k.p;
     const FrameK &f = k.f;
     const ChunkK ch = ld_chunk(p, ci);
@@ -332,7 +342,7 @@ __device__ inline void shade_opaque(const ResolveK &k, uint32_t ci, uint32_t vx,
     float n[3];
     normal_of(nid, n);
     float spec_pre[3] = {0.0f, 0.0f, 0.0f};
-    if (k.bits & RS_SKY_SPECULAR) {
+    if ((k.bits & RS_SKY_SPECULAR) && do_spec) {
         float m[3], sb[3];
         wgsl_reflect(rd, n, m);
         sky_base(k, m, 1.0f, sb);
@@ -358,98 +368,135 @@ __device__ inline void shade_opaque(const ResolveK &k, uint32_t ci, uint32_t vx,
     else { fa = local[0]; fb = local[1]; ta[0] = 1; tb[1] = 1; }
     float sky_tint[3];
     surface_sky_fill(k, n, sky_tint);
-    int32_t air_v[3];
-    bool local_only = true;
-#pragma unroll
-    for (int q = 0; q < 3; q++) {
-        air_v[q] = (int32_t)v[q] + (int32_t)n[q];
-        const int32_t span = abs(ta[q]) + abs(tb[q]);
-        if (!(air_v[q] - span >= 0 && air_v[q] + span <= 63)) local_only = false;
-    }
-    float sky_n[9], blk_n[9], open_n[9];
-    uint32_t light_w[9];
-    uint32_t any_blk = 0u;
-    const bool has_water = (ch.attr_flags & ATTR_HAS_WATER) != 0u;
-    for (uint32_t i = 0; i < 9u; i++) {
-        const int32_t da = (int32_t)(i % 3u) - 1, db = (int32_t)(i / 3u) - 1;
-        uint2 nb;
-        if (local_only) {  // gather_face (common.wgsl:1034-1097)
-            const int32_t x = air_v[0] + ta[0] * da + tb[0] * db, y = air_v[1] + ta[1] * da + tb[1] * db,
-                          z = air_v[2] + ta[2] * da + tb[2] * db;
-            bool solid = voxel_occupied(p, ch, x, y, z);
-            if (solid && has_water) solid = block_at(p, ch, (uint32_t)x, (uint32_t)y, (uint32_t)z) != WATER_ID;
-            nb = make_uint2((clight.y & 1u) ? clight.x : light_at(p, clight, x, y, z), solid ? 1u : 0u);
+    float ao = 1.0f, sky_sm, amb3[3];
+    if (!smooth) {
+        // flat light: the light word of the air voxel in front of the face
+        const int32_t a0 = (int32_t)v[0] + (int32_t)n[0], a1 = (int32_t)v[1] + (int32_t)n[1],
+                      a2 = (int32_t)v[2] + (int32_t)n[2];
+        uint32_t packed1;
+        if (a0 >= 0 && a1 >= 0 && a2 >= 0 && a0 <= 63 && a1 <= 63 && a2 <= 63) {
+            packed1 = light_at(p, clight, a0, a1, a2);
         } else {
             float pt[3];
 #pragma unroll
-            for (int q = 0; q < 3; q++)
-                pt[q] = ((box_min[q] + 0.5f * vs) + n[q] * vs) + ((float)ta[q] * vs) * ((float)(i % 3u) - 1.0f) +
-                        ((float)tb[q] * vs) * ((float)(i / 3u) - 1.0f);
-            nb = world_sample(p, f, pt);
+            for (int q = 0; q < 3; q++) pt[q] = (box_min[q] + 0.5f * vs) + n[q] * vs;
+            packed1 = world_sample(p, f, pt).x;
         }
-        light_w[i] = nb.x;
-        sky_n[i] = light_curve(light_sky(nb.x)) * f.daylight;
-        blk_n[i] = 0.0f;
+        const float sk1 = light_curve(light_sky(packed1)) * f.daylight;
+        sky_sm = sk1;
         if (k.bits & RS_LIGHT_RGB) {
-            if (k.bits & RS_EMITTER_GATHER) any_blk = any_blk | (nb.x & 0xFFFu);
-        } else {
-            blk_n[i] = light_curve(light_block_level(nb.x));
-        }
-        open_n[i] = 1.0f - (float)nb.y;
-    }
-    const bool rgb = (k.bits & RS_LIGHT_RGB) != 0u, gather = rgb && (k.bits & RS_EMITTER_GATHER) && any_blk != 0u;
-    float blk4[3] = {0.0f, 0.0f, 0.0f};
-    if (gather) {
-        float lv[3];
-        light_block_rgb(light_w[4], lv);
-        light_curve3(lv, blk4);
-    }
-    const bool ao_on = (f.flags & FLAG_AO) != 0u;
-    float amb_c[4][3], sky_c[4], ao_c[4];
+            float lv[3], bl[3];
+            light_block_rgb(packed1, lv);
+            light_curve3(lv, bl);
+            const float bk = fmaxf(bl[0], fmaxf(bl[1], bl[2]));
+            const float t = bk / fmaxf(sk1 + bk, 1e-4f);
+            const float m = fmaxf(sk1, bk);
 #pragma unroll
-    for (uint32_t kk = 0; kk < 4u; kk++) {
-        const uint32_t ia = (uint32_t)(4 + ((int32_t)(kk & 1u) * 2 - 1));
-        const uint32_t ib = (uint32_t)(4 + ((int32_t)(kk >> 1) * 2 - 1) * 3);
-        const uint32_t ic = ia + ib - 4u;
-        const float inv = 1.0f / (1.0f + open_n[ia] + open_n[ib] + open_n[ic]);
-        const float sk = (sky_n[4] + sky_n[ia] * open_n[ia] + sky_n[ib] * open_n[ib] + sky_n[ic] * open_n[ic]) * inv;
-        sky_c[kk] = sk;
-        if (rgb) {
-            if (gather) {
-                float ba[3], bb[3], bc[3], lv[3], bkv[3];
-                light_block_rgb(light_w[ia], lv);
-                light_curve3(lv, ba);
-                light_block_rgb(light_w[ib], lv);
-                light_curve3(lv, bb);
-                light_block_rgb(light_w[ic], lv);
-                light_curve3(lv, bc);
+            for (int q = 0; q < 3; q++) amb3[q] = r_mix(sky_tint[q], bl[q] / fmaxf(bk, 1e-4f), t) * m;
+        } else {
+            constexpr float BLOCK_TINT1[3] = {1.0000f, 0.6500f, 0.3200f};
+            const float bk1 = light_curve(light_block_level(packed1));
+#pragma unroll
+            for (int q = 0; q < 3; q++)
+                amb3[q] = r_mix(sky_tint[q], BLOCK_TINT1[q], bk1 / fmaxf(sk1 + bk1, 1e-4f)) * fmaxf(sk1, bk1);
+        }
+    } else {
+        int32_t air_v[3];
+        bool local_only = true;
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            air_v[q] = (int32_t)v[q] + (int32_t)n[q];
+            const int32_t span = abs(ta[q]) + abs(tb[q]);
+            if (!(air_v[q] - span >= 0 && air_v[q] + span <= 63)) local_only = false;
+        }
+        float sky_n[9], blk_n[9], open_n[9];
+        uint32_t light_w[9];
+        uint32_t any_blk = 0u;
+        const bool has_water = (ch.attr_flags & ATTR_HAS_WATER) != 0u;
+        for (uint32_t i = 0; i < 9u; i++) {
+            const int32_t da = (int32_t)(i % 3u) - 1, db = (int32_t)(i / 3u) - 1;
+            uint2 nb;
+            if (local_only) {  // gather_face (common.wgsl:1034-1097)
+                const int32_t x = air_v[0] + ta[0] * da + tb[0] * db, y = air_v[1] + ta[1] * da + tb[1] * db,
+                              z = air_v[2] + ta[2] * da + tb[2] * db;
+                bool solid = voxel_occupied(p, ch, x, y, z);
+                if (solid && has_water) solid = block_at(p, ch, (uint32_t)x, (uint32_t)y, (uint32_t)z) != WATER_ID;
+                nb = make_uint2((clight.y & 1u) ? clight.x : light_at(p, clight, x, y, z), solid ? 1u : 0u);
+            } else {
+                float pt[3];
 #pragma unroll
                 for (int q = 0; q < 3; q++)
-                    bkv[q] = (blk4[q] + ba[q] * open_n[ia] + bb[q] * open_n[ib] + bc[q] * open_n[ic]) * inv;
-                const float bk = fmaxf(bkv[0], fmaxf(bkv[1], bkv[2]));
+                    pt[q] = ((box_min[q] + 0.5f * vs) + n[q] * vs) + ((float)ta[q] * vs) * ((float)(i % 3u) - 1.0f) +
+                            ((float)tb[q] * vs) * ((float)(i / 3u) - 1.0f);
+                nb = world_sample(p, f, pt);
+            }
+            light_w[i] = nb.x;
+            sky_n[i] = light_curve(light_sky(nb.x)) * f.daylight;
+            blk_n[i] = 0.0f;
+            if (k.bits & RS_LIGHT_RGB) {
+                if (k.bits & RS_EMITTER_GATHER) any_blk = any_blk | (nb.x & 0xFFFu);
+            } else {
+                blk_n[i] = light_curve(light_block_level(nb.x));
+            }
+            open_n[i] = 1.0f - (float)nb.y;
+        }
+        const bool rgb = (k.bits & RS_LIGHT_RGB) != 0u, gather = rgb && (k.bits & RS_EMITTER_GATHER) && any_blk != 0u;
+        float blk4[3] = {0.0f, 0.0f, 0.0f};
+        if (gather) {
+            float lv[3];
+            light_block_rgb(light_w[4], lv);
+            light_curve3(lv, blk4);
+        }
+        const bool ao_on = (f.flags & FLAG_AO) != 0u;
+        float amb_c[4][3], sky_c[4], ao_c[4];
+#pragma unroll
+        for (uint32_t kk = 0; kk < 4u; kk++) {
+            const uint32_t ia = (uint32_t)(4 + ((int32_t)(kk & 1u) * 2 - 1));
+            const uint32_t ib = (uint32_t)(4 + ((int32_t)(kk >> 1) * 2 - 1) * 3);
+            const uint32_t ic = ia + ib - 4u;
+            const float inv = 1.0f / (1.0f + open_n[ia] + open_n[ib] + open_n[ic]);
+            const float sk = (sky_n[4] + sky_n[ia] * open_n[ia] + sky_n[ib] * open_n[ib] + sky_n[ic] * open_n[ic]) * inv;
+            sky_c[kk] = sk;
+            if (rgb) {
+                if (gather) {
+                    float ba[3], bb[3], bc[3], lv[3], bkv[3];
+                    light_block_rgb(light_w[ia], lv);
+                    light_curve3(lv, ba);
+                    light_block_rgb(light_w[ib], lv);
+                    light_curve3(lv, bb);
+                    light_block_rgb(light_w[ic], lv);
+                    light_curve3(lv, bc);
+#pragma unroll
+                    for (int q = 0; q < 3; q++)
+                        bkv[q] = (blk4[q] + ba[q] * open_n[ia] + bb[q] * open_n[ib] + bc[q] * open_n[ic]) * inv;
+                    const float bk = fmaxf(bkv[0], fmaxf(bkv[1], bkv[2]));
+                    const float t = bk / fmaxf(sk + bk, 1e-4f);
+                    const float m = fmaxf(sk, bk);
+#pragma unroll
+                    for (int q = 0; q < 3; q++) amb_c[kk][q] = r_mix(sky_tint[q], bkv[q] / fmaxf(bk, 1e-4f), t) * m;
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 3; q++) amb_c[kk][q] = sky_tint[q] * sk;
+                }
+            } else {
+                constexpr float BLOCK_TINT[3] = {1.0000f, 0.6500f, 0.3200f};
+                const float bk = (blk_n[4] + blk_n[ia] * open_n[ia] + blk_n[ib] * open_n[ib] + blk_n[ic] * open_n[ic]) * inv;
                 const float t = bk / fmaxf(sk + bk, 1e-4f);
                 const float m = fmaxf(sk, bk);
 #pragma unroll
-                for (int q = 0; q < 3; q++) amb_c[kk][q] = r_mix(sky_tint[q], bkv[q] / fmaxf(bk, 1e-4f), t) * m;
-            } else {
-#pragma unroll
-                for (int q = 0; q < 3; q++) amb_c[kk][q] = sky_tint[q] * sk;
+                for (int q = 0; q < 3; q++) amb_c[kk][q] = r_mix(sky_tint[q], BLOCK_TINT[q], t) * m;
             }
-        } else {
-            constexpr float BLOCK_TINT[3] = {1.0000f, 0.6500f, 0.3200f};
-            const float bk = (blk_n[4] + blk_n[ia] * open_n[ia] + blk_n[ib] * open_n[ib] + blk_n[ic] * open_n[ic]) * inv;
-            const float t = bk / fmaxf(sk + bk, 1e-4f);
-            const float m = fmaxf(sk, bk);
-#pragma unroll
-            for (int q = 0; q < 3; q++) amb_c[kk][q] = r_mix(sky_tint[q], BLOCK_TINT[q], t) * m;
+            const float s1 = 1.0f - open_n[ia];
+            const float s2 = 1.0f - open_n[ib];
+            const float occ = (s1 > 0.5f && s2 > 0.5f) ? 3.0f : s1 + s2 + 1.0f - open_n[ic];
+            ao_c[kk] = ao_on ? 1.0f - occ * AO_STRENGTH : 1.0f;
         }
-        const float s1 = 1.0f - open_n[ia];
-        const float s2 = 1.0f - open_n[ib];
-        const float occ = (s1 > 0.5f && s2 > 0.5f) ? 3.0f : s1 + s2 + 1.0f - open_n[ic];
-        ao_c[kk] = ao_on ? 1.0f - occ * AO_STRENGTH : 1.0f;
+        ao = r_mix(r_mix(ao_c[0], ao_c[1], fa), r_mix(ao_c[2], ao_c[3], fa), fb);
+        sky_sm = r_mix(r_mix(sky_c[0], sky_c[1], fa), r_mix(sky_c[2], sky_c[3], fa), fb);
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+            amb3[q] = r_mix(r_mix(amb_c[0][q], amb_c[1][q], fa), r_mix(amb_c[2][q], amb_c[3][q], fa), fb);
     }
-    const float ao = r_mix(r_mix(ao_c[0], ao_c[1], fa), r_mix(ao_c[2], ao_c[3], fa), fb);
-    const float sky_sm = r_mix(r_mix(sky_c[0], sky_c[1], fa), r_mix(sky_c[2], sky_c[3], fa), fb);
     float direct = 0.0f;
     const float ndl = dot3o(n, f.sun);
     if (ndl > 0.0f && f.daylight > 0.01f) {
@@ -465,8 +512,7 @@ __device__ inline void shade_opaque(const ResolveK &k, uint32_t ci, uint32_t vx,
     float light[3], spec_rgb[3];
 #pragma unroll
     for (int q = 0; q < 3; q++) {
-        const float amb = r_mix(r_mix(amb_c[0][q], amb_c[1][q], fa), r_mix(amb_c[2][q], amb_c[3][q], fa), fb);
-        const float ambient_rgb = amb * 0.50f;
+        const float ambient_rgb = amb3[q] * 0.50f;
         const float sun_rgb = SUN_RGB[q] * sun_k;
         const float floor_rgb = FLOOR_RGB[q] * k.ambient;
         spec_rgb[q] = spec_pre[q] * (sky_sm * SKY_SPEC_GAIN);
@@ -506,6 +552,438 @@ __device__ inline void shade_opaque(const ResolveK &k, uint32_t ci, uint32_t vx,
     }
 #pragma unroll
     for (int q = 0; q < 3; q++) out[q] = albedo[q] * (light[q] * 1.0f) + spec_rgb[q];
+}
+
+// ---- Lanes 1 and 2: glass and water pixels (resolve.wgsl:1029-1310, :1379-1466) ---------
+// As the oracle restates them: shade_glass traces the transmit leg (and the reflect leg
+// with SPEC_GLASS_REFLECT) and shades its hit (shade_water on water); shade_water builds
+// water_ctx, traces the refract and reflect legs and composes them; with SPEC_WATER_SEC the
+// legs of each 2x2 block come from k_water_sec (stored as f16) where water_share_safe
+// holds. Leg hits: shade_hit_legacy with the sun visibility of shadow_ray, no sky specular,
+// flat light under SPEC_FLAT_SECONDARY.
+constexpr float WATER_F0 = 0.02f, GLASS_F0 = 0.0426f, GLINT_POWER = 320.0f, GLINT_GAIN = 2.5f, FOAM_DEPTH = 2.5f,
+                FOAM_LO = 0.45f, FOAM_HI = 0.75f, FOAM_MAX = 0.85f, FOAM_RATE = 0.55f;
+constexpr uint32_t WATER_SEC_SPAN = 2u;
+
+__device__ __forceinline__ float r_sin(float x) {  // the oracle's vxo_wave_sin
+    const float kf = rintf(x * 0x1.45f306p-1f);
+    const float r = ((x - kf * 0x1.922p0f) - kf * -0x1.2aep-18f) - kf * -0x1.de973ep-31f;
+    const float r2 = r * r;
+    float cs = 0x1.a01a02p-16f;
+    cs = cs * r2 - 0x1.6c16c2p-10f;
+    cs = cs * r2 + 0x1.555556p-5f;
+    cs = cs * r2 - 0.5f;
+    cs = cs * r2 + 1.0f;
+    float sn = 0x1.71de3ap-19f;
+    sn = sn * r2 - 0x1.a01a02p-13f;
+    sn = sn * r2 + 0x1.111112p-7f;
+    sn = sn * r2 - 0x1.555556p-3f;
+    sn = sn * r2 + 1.0f;
+    sn = sn * r;
+    const int32_t q = (int32_t)kf & 3;
+    return q == 0 ? sn : (q == 1 ? cs : (q == 2 ? -sn : -cs));
+}
+
+// face_uv (resolve.wgsl:129-147)
+__device__ __forceinline__ void face_uv_of(uint32_t nid, const float local[3], float uv[2]) {
+    const uint32_t axis = nid >> 1;
+    if (axis == 0u) { uv[0] = local[2]; uv[1] = 1.0f - local[1]; }
+    else if (axis == 1u) { uv[0] = local[0]; uv[1] = local[2]; }
+    else { uv[0] = local[0]; uv[1] = 1.0f - local[1]; }
+    if (nid == 1u || nid == 4u) uv[0] = 1.0f - uv[0];
+}
+
+__device__ __forceinline__ float schlick_water(float cos_i) {
+    const float m = r_clamp(1.0f - cos_i, 0.0f, 1.0f);
+    const float m2 = m * m;
+    return WATER_F0 + (1.0f - WATER_F0) * m2 * m2 * m;
+}
+__device__ __forceinline__ float schlick_glass(float cos_i) {
+    const float m = r_clamp(1.0f - cos_i, 0.0f, 1.0f);
+    const float m2 = m * m;
+    return GLASS_F0 + (1.0f - GLASS_F0) * m2 * m2 * m;
+}
+
+__device__ __forceinline__ float medium_light(const ResolveK &k, uint32_t packed) {
+    const float sky_l = light_curve(light_sky(packed)) * k.f.daylight;
+    const float blk_l = light_curve(light_block_level(packed));
+    return fmaxf(sky_l, blk_l) * 0.50f + k.ambient;
+}
+
+__device__ __forceinline__ float foam_hash(float x, float y) {
+    const int32_t qx = (int32_t)x, qy = (int32_t)y;
+    return (float)((hash_u32((uint32_t)qx * 374761393u + (uint32_t)qy * 668265263u) >> 8) & 0xFFFFFFu) / 16777216.0f;
+}
+
+__device__ inline float foam_mottle(float x, float z, float time) {
+    const float sc = (float)(1.0 / 0.7);
+    const float p0 = x * sc, p1 = z * sc;
+    const float i0 = floorf(p0), i1 = floorf(p1);
+    const float w0 = p0 - floorf(p0), w1 = p1 - floorf(p1);
+    const float u0 = w0 * w0 * (3.0f - 2.0f * w0), u1 = w1 * w1 * (3.0f - 2.0f * w1);
+    const float m0 = r_mix(foam_hash(i0, i1), foam_hash(i0 + 1.0f, i1), u0);
+    const float m1 = r_mix(foam_hash(i0, i1 + 1.0f), foam_hash(i0 + 1.0f, i1 + 1.0f), u0);
+    const float m = r_mix(m0, m1, u1);
+    const float ph = time * FOAM_RATE * 4.0f + foam_hash(i0 * 2.23f, i1 * 2.71f) * 6.28318f;
+    return m * (0.5f + 0.5f * r_sin(ph));
+}
+
+__device__ __forceinline__ uint32_t face_layer_of(const ResolveK &k, uint32_t id, uint32_t nid) {
+    const uint32_t fi = id * 8u + nid;
+    return fi < k.n_faces ? (k.faces[fi] & 0xFFFFu) : 0u;
+}
+
+__device__ __forceinline__ void voxel_of(uint32_t w, uint32_t v[3]) {
+    v[0] = w & 63u;
+    v[1] = (w >> 6) & 63u;
+    v[2] = (w >> 12) & 63u;
+}
+
+// shade_hit at a leg's hit: the sun test of shadow_ray from hit + n * 0.02 vs, then
+// shade_hit_legacy without sky specular.
+__device__ inline void shade_leg_hit(const ResolveK &k, const WorldHit &w, const float pt[3], const float rd[3],
+                                     float dist, float shadow_dist, float out[3]) {
+    const FrameK &f = k.f;
+    uint32_t v[3];
+    voxel_of(w.voxel, v);
+    const ChunkK c = ld_chunk(k.p, w.ci);
+    const uint32_t id = block_at(k.p, c, v[0], v[1], v[2]);
+    float n[3];
+    normal_of(w.normal, n);
+    float lit = 1.0f;
+    if (dot3o(n, f.sun) > 0.0f && f.daylight > 0.01f && (f.flags & VX_FLAG_SHADOWS)) {
+        float o[3];
+#pragma unroll
+        for (int q = 0; q < 3; q++) o[q] = pt[q] + n[q] * (0.02f * c.vs);
+        lit = shadow_ray(k.p, f, k.s, o, shadow_dist) ? 0.0f : 1.0f;
+    }
+    shade_opaque(k, w.ci, v[0], v[1], v[2], id, w.normal, pt, rd, dist, lit, !(k.bits & RS_FLAT_SECONDARY), false,
+                 out);
+}
+
+struct WaterCtx {
+    WaterNormals nm;
+    float body[3], ml, refract_detail, reflect_detail, mirror[3], mirror_t[3];
+};
+
+// water_ctx (resolve.wgsl:1078-1167) without the water-look ripples
+__device__ inline void water_ctx(const ResolveK &k, uint32_t ci, const uint32_t v[3], uint32_t nid, const float hit[3],
+                                 const float rd[3], float t, WaterCtx &o) {
+    const FrameK &f = k.f;
+    const ChunkK ch = ld_chunk(k.p, ci);
+    const float vs = ch.vs;
+    float box_min[3], local[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) box_min[q] = ch.origin[q] + (float)v[q] * vs;
+#pragma unroll
+    for (int q = 0; q < 3; q++) local[q] = r_clamp((hit[q] - box_min[q]) / vs, 0.0f, 1.0f);
+    const float footprint = fmaxf(t, 1e-3f) * 2.0f * f.tan_half_fov / (float)f.res[1] * 16.0f / vs;
+    const float lod = r_clamp(r_log2(fmaxf(footprint, 1e-4f)) + k.mip_bias, 0.0f, 3.0f);
+    const uint32_t layer = face_layer_of(k, WATER_ID, nid);
+    float texel[4], uv[2];
+    if (k.s.bits & SPEC_WATER_REPAIR) {
+        uv[0] = uv[1] = 0.5f;
+        atlas_sample(k, layer, uv, 3.0f, texel);
+    } else {
+        face_uv_of(nid, local, uv);
+        atlas_sample(k, layer, uv, lod, texel);
+    }
+    o.ml = medium_light(k, light_at(k.p, ld_chunk_light(k.p, ci), (int32_t)v[0], (int32_t)v[1], (int32_t)v[2]));
+#pragma unroll
+    for (int q = 0; q < 3; q++) o.body[q] = texel[q] * o.ml;
+    o.refract_detail = 1.0f - wgsl_smoothstep(WATER_REFRACT_NEAR, WATER_REFRACT_FAR, t);
+    o.reflect_detail = 1.0f - wgsl_smoothstep(WATER_REFLECT_NEAR, WATER_REFLECT_FAR, t);
+    water_normals(k.p, f, k.s, nid, hit, rd, t, o.nm);
+    wgsl_reflect(rd, o.nm.nw, o.mirror);
+    wgsl_reflect(rd, o.nm.nt, o.mirror_t);
+    if (o.nm.waved) {
+        o.mirror_t[1] = fmaxf(o.mirror_t[1], 0.0f);
+        wgsl_normalize(o.mirror_t);
+    }
+}
+
+// water_refract_leg (resolve.wgsl:1029-1042): rgb and the column depth
+__device__ inline void water_refract_leg(const ResolveK &k, const float hit[3], const float n[3], const float rd[3],
+                                         float t, float out[4]) {
+    const FrameK &f = k.f;
+    float rdir[3], ro2[3];
+    wgsl_refract(rd, n, 1.0f / WATER_IOR, rdir);
+#pragma unroll
+    for (int q = 0; q < 3; q++) ro2[q] = hit[q] + rdir[q] * SURFACE_EPS;
+    const WorldHit h2 = trace_world<true>(k.p, f, k.s, ro2, rdir, WATER_REFRACT_DIST);
+    out[0] = out[1] = out[2] = 0.0f;
+    out[3] = WATER_REFRACT_DIST;
+    if (h2.hit) {
+        float p2[3];
+#pragma unroll
+        for (int q = 0; q < 3; q++) p2[q] = ro2[q] + rdir[q] * h2.t;
+        const float sd = (f.flags & FLAG_WATER_SHADOW_CUT) ? WATER_SHADOW_DIST : WATER_SHADOW_DIST_PRE41;
+        shade_leg_hit(k, h2, p2, rdir, t + h2.t, sd, out);
+        out[3] = h2.t;
+    }
+}
+
+// water_reflect_leg (resolve.wgsl:1044-1058): rgb and 1, or all 0 when it hits nothing
+__device__ inline void water_reflect_leg(const ResolveK &k, const float hit[3], const float n[3], const float mt[3],
+                                         float t, float out[4]) {
+    float ro3[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) ro3[q] = hit[q] + n[q] * SURFACE_EPS;
+    const WorldHit h3 = trace_world<true>(k.p, k.f, k.s, ro3, mt, WATER_REFLECT_DIST);
+    out[0] = out[1] = out[2] = out[3] = 0.0f;
+    if (!h3.hit) return;
+    float p3[3], lit[3], sky[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) p3[q] = ro3[q] + mt[q] * h3.t;
+    shade_leg_hit(k, h3, p3, mt, t + h3.t, k.f.shadow_dist, lit);
+    sky_base(k, mt, 1.0f, sky);
+    const float a = 1.0f - transmittance_from(k, hit[1], h3.t, mt);
+#pragma unroll
+    for (int q = 0; q < 3; q++) out[q] = r_mix(lit[q], sky[q], a);
+    out[3] = 1.0f;
+}
+
+// water_compose (resolve.wgsl:1169-1207) without the water look
+__device__ inline void water_compose(const ResolveK &k, const WaterCtx &x, const float hit[3], const float rd[3],
+                                     const float leg_r[4], const float leg_m[4], float out[3]) {
+    constexpr float SUN_TINT[3] = {1.0000f, 0.8469f, 0.5647f}, WATER_EXT[3] = {0.280f, 0.055f, 0.020f},
+                    FOAM_RGB[3] = {0.92f, 0.97f, 1.0f};
+    const FrameK &f = k.f;
+    float refr[3], refl[3], sky[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) refr[q] = x.body[q];
+    float column_depth = WATER_REFRACT_DIST;
+    if ((f.flags & FLAG_WATER_REFRACT) && x.refract_detail > 0.0f) {
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const float tr = wave_exp(-WATER_EXT[q] * k.water_absorb * fmaxf(leg_r[3], 0.0f));
+            const float med = leg_r[q] * tr + x.body[q] * (1.0f - tr);
+            refr[q] = r_mix(refr[q], med, x.refract_detail);
+        }
+        column_depth = leg_r[3];
+    }
+    if ((k.bits & RS_SHORE_FOAM) && x.refract_detail > 0.0f && column_depth < FOAM_DEPTH) {
+        const float shall = 1.0f - wgsl_smoothstep(0.0f, FOAM_DEPTH, column_depth);
+        const float m = foam_mottle(hit[0], hit[2], f.time * f.wave_speed);
+        const float foam = shall * wgsl_smoothstep(FOAM_LO, FOAM_HI, m) * x.refract_detail;
+        const float a = r_clamp(foam, 0.0f, 1.0f) * FOAM_MAX;
+#pragma unroll
+        for (int q = 0; q < 3; q++) refr[q] = r_mix(refr[q], FOAM_RGB[q] * x.ml, a);
+    }
+    sky_base(k, x.mirror, 1.0f, sky);
+    const float g = r_pow(fmaxf(dot3o(x.mirror, f.sun), 0.0f), GLINT_POWER * x.nm.gloss) * GLINT_GAIN * x.nm.gloss *
+                    f.daylight;
+#pragma unroll
+    for (int q = 0; q < 3; q++) refl[q] = sky[q] + SUN_TINT[q] * g;
+    if ((f.flags & FLAG_WATER_REFLECT) && x.reflect_detail > 0.0f && leg_m[3] > 0.0f)
+#pragma unroll
+        for (int q = 0; q < 3; q++) refl[q] = r_mix(refl[q], leg_m[q], x.reflect_detail);
+    float skn[3];
+    sky_base(k, x.nm.n, 1.0f, skn);
+#pragma unroll
+    for (int q = 0; q < 3; q++) refl[q] = r_mix(refl[q], skn[q], x.nm.rough * 0.65f);
+    const float fr = schlick_water(fmaxf(-dot3o(rd, x.nm.nw), 1e-3f));
+#pragma unroll
+    for (int q = 0; q < 3; q++) out[q] = r_mix(refr[q], refl[q], fr);
+}
+
+// shade_water (resolve.wgsl:1209-1221)
+__device__ inline void shade_water(const ResolveK &k, uint32_t ci, const uint32_t v[3], uint32_t nid,
+                                   const float hit[3], const float rd[3], float t, float out[3]) {
+    WaterCtx x;
+    water_ctx(k, ci, v, nid, hit, rd, t, x);
+    float leg_r[4] = {0.0f, 0.0f, 0.0f, WATER_REFRACT_DIST}, leg_m[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if ((k.f.flags & FLAG_WATER_REFRACT) && x.refract_detail > 0.0f) water_refract_leg(k, hit, x.nm.n, rd, t, leg_r);
+    if ((k.f.flags & FLAG_WATER_REFLECT) && x.reflect_detail > 0.0f)
+        water_reflect_leg(k, hit, x.nm.n, x.mirror_t, t, leg_m);
+    water_compose(k, x, hit, rd, leg_r, leg_m, out);
+}
+
+// The decoded primary key: chunk, voxel, micro cell, face.
+__device__ __forceinline__ bool key_decode(uint64_t key, uint32_t &ci, uint32_t v[3], uint32_t mc[3], uint32_t &nid) {
+    if (key == 0ull) return false;
+    nid = (uint32_t)(key >> 37) & 7u;
+    ci = (uint32_t)(key >> 24) & 0x1FFFu;
+    const uint32_t low = (uint32_t)(key & 0xFFFFFFull);
+    v[0] = (low >> 16) & 63u; v[1] = (low >> 8) & 63u; v[2] = low & 63u;
+    mc[0] = (low >> 22) & 3u; mc[1] = (low >> 14) & 3u; mc[2] = (low >> 6) & 3u;
+    return true;
+}
+
+// water_share_safe (resolve.wgsl:1223-1245)
+__device__ inline bool water_share_safe(const ResolveK &k, uint32_t px, uint32_t py, uint32_t nid, const float hit[3],
+                                        float t) {
+    const FrameK &f = k.f;
+    if (nid != NORMAL_PY) return false;
+    const uint32_t bx = (px >> 1) * WATER_SEC_SPAN, by = (py >> 1) * WATER_SEC_SPAN;
+    for (uint32_t y = 0; y < WATER_SEC_SPAN; y++)
+        for (uint32_t x = 0; x < WATER_SEC_SPAN; x++) {
+            const uint32_t qx = bx + x, qy = by + y;
+            if (qx >= f.res[0] || qy >= f.res[1]) continue;
+            uint32_t ci, v[3], mc[3], n2;
+            if (!key_decode(k.vis[(size_t)qy * f.res[0] + qx], ci, v, mc, n2)) return false;
+            if (n2 != nid || ci >= k.p.n_chunks) return false;
+            const ChunkK c = ld_chunk(k.p, ci);
+            if (block_at(k.p, c, v[0], v[1], v[2]) != WATER_ID) return false;
+            float rd[3];
+            ray_dir(f, (float)qx + 0.5f, (float)qy + 0.5f, rd[0], rd[1], rd[2]);
+            const float td = hit_t(f, c, v[0], v[1], v[2], mc[0], mc[1], mc[2], n2, rd);
+            const float hy = f.cam[1] + rd[1] * td;
+            if (fabsf(hy - hit[1]) > 0.01f || fabsf(td - t) > fmaxf(0.1f, fminf(1.0f, t * 0.02f))) return false;
+        }
+    return true;
+}
+
+__device__ __forceinline__ float h16f(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
+__device__ __forceinline__ uint16_t f16h(float x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
+
+// shade_water_sec (resolve.wgsl:1247-1259)
+__device__ inline void shade_water_sec(const ResolveK &k, uint32_t ci, const uint32_t v[3], uint32_t nid,
+                                       const float hit[3], const float rd[3], float t, uint32_t px, uint32_t py,
+                                       float out[3]) {
+    if ((k.s.bits & SPEC_WATER_REPAIR) && !water_share_safe(k, px, py, nid, hit, t)) {
+        shade_water(k, ci, v, nid, hit, rd, t, out);
+        return;
+    }
+    WaterCtx x;
+    water_ctx(k, ci, v, nid, hit, rd, t, x);
+    const size_t o = (size_t)(py >> 1) * k.hw + (px >> 1);
+    const ushort4 r = k.sec_r[o], m = k.sec_m[o];
+    const float leg_r[4] = {h16f(r.x), h16f(r.y), h16f(r.z), h16f(r.w)};
+    const float leg_m[4] = {h16f(m.x), h16f(m.y), h16f(m.z), h16f(m.w)};
+    water_compose(k, x, hit, rd, leg_r, leg_m, out);
+}
+
+// One leg of shade_glass from origin along dir: the hit shaded (shade_water on water) and
+// mixed toward the sky by the fog over the leg; out keeps its value when it hits nothing.
+__device__ inline void glass_leg(const ResolveK &k, const float hit[3], const float origin[3], const float dir[3],
+                                 float t, float out[3]) {
+    const WorldHit h = trace_world<false>(k.p, k.f, k.s, origin, dir, fmaxf(GLASS_TRANSMIT_DIST, k.f.far_));
+    if (!h.hit) return;
+    float p2[3], col[3], sky[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) p2[q] = origin[q] + dir[q] * h.t;
+    uint32_t v[3];
+    voxel_of(h.voxel, v);
+    const ChunkK c = ld_chunk(k.p, h.ci);
+    if (block_at(k.p, c, v[0], v[1], v[2]) == WATER_ID) shade_water(k, h.ci, v, h.normal, p2, dir, t + h.t, col);
+    else shade_leg_hit(k, h, p2, dir, t + h.t, GLASS_SHADOW_DIST, col);
+    sky_base(k, dir, 1.0f, sky);
+    const float a = 1.0f - transmittance_from(k, hit[1], h.t, dir);
+#pragma unroll
+    for (int q = 0; q < 3; q++) out[q] = r_mix(col[q], sky[q], a);
+}
+
+// shade_glass (resolve.wgsl:1261-1319)
+__device__ inline void shade_glass(const ResolveK &k, uint32_t ci, const uint32_t v[3], uint32_t nid,
+                                   const float hit[3], const float rd[3], float t, float out[3]) {
+    const FrameK &f = k.f;
+    const ChunkK ch = ld_chunk(k.p, ci);
+    const float vs = ch.vs;
+    float n[3];
+    normal_of(nid, n);
+    if (dot3o(n, rd) > 0.0f)
+#pragma unroll
+        for (int q = 0; q < 3; q++) n[q] = -n[q];
+    float box_min[3], local[3], uv[2], texel[4];
+#pragma unroll
+    for (int q = 0; q < 3; q++) box_min[q] = ch.origin[q] + (float)v[q] * vs;
+#pragma unroll
+    for (int q = 0; q < 3; q++) local[q] = r_clamp((hit[q] - box_min[q]) / vs, 0.0f, 1.0f);
+    const float footprint = fmaxf(t, 1e-3f) * 2.0f * f.tan_half_fov / (float)f.res[1] * 16.0f / vs;
+    const float lod = r_clamp(r_log2(fmaxf(footprint, 1e-4f)) + k.mip_bias, 0.0f, 3.0f);
+    face_uv_of(nid, local, uv);
+    atlas_sample(k, face_layer_of(k, GLASS_ID, nid), uv, lod, texel);
+    float ro2[3], through[3], mirror[3], refl[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) ro2[q] = hit[q] + rd[q] * SURFACE_EPS;
+    sky_base(k, rd, 1.0f, through);
+    glass_leg(k, hit, ro2, rd, t, through);
+    wgsl_reflect(rd, n, mirror);
+    sky_base(k, mirror, 1.0f, refl);
+    if (k.s.bits & SPEC_GLASS_REFLECT) {
+        float origin[3];
+#pragma unroll
+        for (int q = 0; q < 3; q++) origin[q] = hit[q] + n[q] * SURFACE_EPS;
+        glass_leg(k, hit, origin, mirror, t, refl);
+    }
+    const float fr = schlick_glass(fmaxf(-dot3o(rd, n), 1e-3f));
+#pragma unroll
+    for (int q = 0; q < 3; q++) out[q] = r_mix(through[q] * texel[q], refl[q], fr);
+}
+
+// water_sec (resolve.wgsl:1469-1535): one thread per 2x2 block; the legs from the block's
+// water pixel of least hit distance (the first in x-major order on ties), stored as f16.
+__global__ __launch_bounds__(64) void k_water_sec(ResolveK k) {
+    const FrameK &f = k.f;
+    const uint32_t bxs = (k.hw + 7u) / 8u;
+    const uint32_t hx = (blockIdx.x % bxs) * 8u + (threadIdx.x & 7u), hy = (blockIdx.x / bxs) * 8u + (threadIdx.x >> 3);
+    if (hx >= k.hw || hy >= k.hh) return;
+    float leg_r[4] = {0.0f, 0.0f, 0.0f, WATER_REFRACT_DIST}, leg_m[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float best_t = 3.4e38f;
+    bool has = false;
+    uint32_t bci = 0, bv[3] = {0, 0, 0}, bnid = 0, bpx = 0, bpy = 0;
+    for (uint32_t i = 0; i < WATER_SEC_SPAN; i++)
+        for (uint32_t j = 0; j < WATER_SEC_SPAN; j++) {
+            const uint32_t px = WATER_SEC_SPAN * hx + i, py = WATER_SEC_SPAN * hy + j;
+            if (px >= f.res[0] || py >= f.res[1]) continue;
+            uint32_t ci, v[3], mc[3], nid;
+            if (!key_decode(k.vis[(size_t)py * f.res[0] + px], ci, v, mc, nid) || ci >= k.p.n_chunks) continue;
+            const ChunkK c = ld_chunk(k.p, ci);
+            if (block_at(k.p, c, v[0], v[1], v[2]) != WATER_ID) continue;
+            float rd0[3];
+            ray_dir(f, (float)px + 0.5f, (float)py + 0.5f, rd0[0], rd0[1], rd0[2]);
+            const float ts = hit_t(f, c, v[0], v[1], v[2], mc[0], mc[1], mc[2], nid, rd0);
+            if (ts < best_t) {
+                best_t = ts; bci = ci; bnid = nid; bpx = px; bpy = py; has = true;
+                bv[0] = v[0]; bv[1] = v[1]; bv[2] = v[2];
+            }
+        }
+    if (has) {
+        float rd[3], hit[3];
+        ray_dir(f, (float)bpx + 0.5f, (float)bpy + 0.5f, rd[0], rd[1], rd[2]);
+#pragma unroll
+        for (int q = 0; q < 3; q++) hit[q] = f.cam[q] + rd[q] * best_t;
+        WaterCtx x;
+        water_ctx(k, bci, bv, bnid, hit, rd, best_t, x);
+        if ((f.flags & FLAG_WATER_REFRACT) && x.refract_detail > 0.0f) water_refract_leg(k, hit, x.nm.n, rd, best_t, leg_r);
+        if ((f.flags & FLAG_WATER_REFLECT) && x.reflect_detail > 0.0f)
+            water_reflect_leg(k, hit, x.nm.n, x.mirror_t, best_t, leg_m);
+    }
+    const size_t o = (size_t)hy * k.hw + hx;
+    k.sec_r[o] = make_ushort4(f16h(leg_r[0]), f16h(leg_r[1]), f16h(leg_r[2]), f16h(leg_r[3]));
+    k.sec_m[o] = make_ushort4(f16h(leg_m[0]), f16h(leg_m[1]), f16h(leg_m[2]), f16h(leg_m[3]));
+}
+
+// The glass and water pixels of the resolve (lanes 1 and 2), one wave per 8x8 tile; other
+// pixels are lane 0's.
+__global__ __launch_bounds__(64) void k_resolve_lanes(ResolveK k) {
+    const FrameK &f = k.f;
+    const uint32_t tile = blockIdx.x;
+    if (tile >= f.tiles[0] * f.tiles[1]) return;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t px = (tile % f.tiles[0]) * 8u + (lane & 7u), py = (tile / f.tiles[0]) * 8u + (lane >> 3);
+    if (px >= f.res[0] || py >= f.res[1]) return;
+    const size_t pix = (size_t)py * f.res[0] + px;
+    const uint64_t key = k.vis[pix];
+    if (key == 0ull) return;
+    const KeyHit h = key_hit(k.p, f, key, px, py);
+    if (!h.ok) return;
+    const bool glass = (k.bits & RS_GLASS) && h.id == GLASS_ID;
+    if (h.id != WATER_ID && !glass) return;
+    const uint32_t low = (uint32_t)(key & 0xFFFFFFull);
+    const uint32_t ci = (uint32_t)(key >> 24) & 0x1FFFu;
+    const uint32_t v[3] = {(low >> 16) & 63u, (low >> 8) & 63u, low & 63u};
+    float color[3];
+    if (glass) shade_glass(k, ci, v, h.nid, h.hit, h.rd, h.t, color);
+    else if (k.bits & RS_WATER_SEC) shade_water_sec(k, ci, v, h.nid, h.hit, h.rd, h.t, px, py, color);
+    else shade_water(k, ci, v, h.nid, h.hit, h.rd, h.t, color);
+    const float alpha = 1.0f - transmittance_from(k, f.cam[1] + 0.0f * h.rd[1], h.t - 0.0f, h.rd);
+    float sky[3];
+    sky_base(k, h.rd, 1.0f, sky);
+#pragma unroll
+    for (int q = 0; q < 3; q++) color[q] = r_mix(color[q], sky[q], alpha);
+    k.out[pix] = make_float4(color[0], color[1], color[2], 1.0f);
 }
 
 // heat (resolve.wgsl:36-44)
@@ -551,7 +1029,7 @@ __global__ __launch_bounds__(64) void k_resolve(ResolveK k) {
         const float d = f16_word(k.sun ? k.sun[pix] : (uint16_t)0);
         const float occ = d > 0.0f ? 1.0f : 0.0f;
         shade_opaque(k, ci, (low >> 16) & 63u, (low >> 8) & 63u, low & 63u, h.id, h.nid, h.hit, h.rd, h.t,
-                     r_clamp(1.0f - occ, 0.0f, 1.0f), color);
+                     r_clamp(1.0f - occ, 0.0f, 1.0f), true, true, color);
         const float alpha = 1.0f - transmittance_from(k, f.cam[1] + 0.0f * h.rd[1], h.t - 0.0f, h.rd);
         float sky[3];
         sky_base(k, h.rd, 1.0f, sky);
@@ -824,6 +1302,7 @@ int resolve_bits(uint32_t spec, uint32_t spec_hi, const vx_gpu_frame &g, uint32_
     if ((spec & 32768u) && (g.flags & 32768u)) return VX_E_UNSUPPORTED;    // distant terrain shadows
     if (spec_hi & (64u | 8192u | 16384u | 65536u)) return VX_E_UNSUPPORTED;  // caustics, foliage rich, canopy relief,
                                                                               // soft shadows
+    if (spec_hi & 4096u) return VX_E_UNSUPPORTED;                           // the water look (ripples, murk)
     if (g.cloud_cover > 0.0f) return VX_E_UNSUPPORTED;                      // cloud deck
     const bool terrain_shafts = (spec & 8192u) && (g.flags & 8192u);
     if (g.godray_strength > 0.0f && terrain_shafts) return VX_E_UNSUPPORTED;
@@ -838,6 +1317,9 @@ int resolve_bits(uint32_t spec, uint32_t spec_hi, const vx_gpu_frame &g, uint32_
     if (spec_hi & 8u) bits |= RS_SHORE_WET;
     if (spec & 16384u) bits |= RS_TEX_VARIATION;
     if (spec & VX_FLAG_GLASS) bits |= RS_GLASS;
+    if (spec_hi & 32u) bits |= RS_WATER_SEC;
+    if (spec_hi & 16u) bits |= RS_SHORE_FOAM;
+    if (spec & 1048576u) bits |= RS_FLAT_SECONDARY;
     return VX_OK;
 }
 
@@ -896,8 +1378,27 @@ int vx_resolve(vx_ctx *c) {
     k.sun = (const uint16_t *)c->shadow.p;
     k.heat = c->heat_valid ? (const uint32_t *)c->heat.p : nullptr;
     k.out = (float4 *)c->color.p;
+    k.s = c->spec;
+    k.water_absorb = g.water_absorb;
+    k.hw = (c->w + 1u) / 2u;
+    k.hh = (c->h + 1u) / 2u;
+    const bool lanes = !(g.flags & VX_FLAG_HEATMAP);
+    if (lanes && (bits & RS_WATER_SEC)) {
+        const size_t sec_bytes = (size_t)k.hw * k.hh * 8;
+        if ((rc = vxh::ensure(c->sec_r, sec_bytes)) || (rc = vxh::ensure(c->sec_m, sec_bytes))) return rc;
+        k.sec_r = (ushort4 *)c->sec_r.p;
+        k.sec_m = (ushort4 *)c->sec_m.p;
+        k_water_sec<<<((k.hw + 7u) / 8u) * ((k.hh + 7u) / 8u), 64, 0, c->stream>>>(k);
+        VX_LAUNCHED();
+    } else {
+        k.sec_r = k.sec_m = nullptr;
+    }
     k_resolve<<<c->tx * c->ty, 64, 0, c->stream>>>(k);
     VX_LAUNCHED();
+    if (lanes) {
+        k_resolve_lanes<<<c->tx * c->ty, 64, 0, c->stream>>>(k);
+        VX_LAUNCHED();
+    }
     c->color_valid = true;
     c->image_valid = false;
     c->taa_valid = false;

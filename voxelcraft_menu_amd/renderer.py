@@ -157,11 +157,12 @@ class Renderer:
         A.check(self._L.vx_resolve_set_assets(self._c, C.byref(a)), "vx_resolve_set_assets")
 
     def resolve(self):
-        """Resolve lane 0 (resolve.wgsl:1378-1466) of the last rendered frame."""
+        """The resolve (resolve.wgsl:1378-1535: water_sec, then lanes 0, 1 and 2) of the last
+        rendered frame."""
         A.check(self._L.vx_resolve(self._c), "vx_resolve")
 
     def read_color(self) -> np.ndarray:
-        """(H, W, 4) float32 colour of the last resolve (alpha 0: water / glass pixels)."""
+        """(H, W, 4) float32 colour of the last resolve."""
         w, h = self.size
         out = np.zeros((h, w, 4), dtype=np.float32)
         A.check(self._L.vx_read_color(self._c, A.ptr(out), out.size), "vx_read_color")
