@@ -271,6 +271,8 @@ def run_batch(args, out):
         out["chunk_build"] = run_chunk_build(args)
     if not args.no_view_setup:
         out["view_setup"] = run_view_setup(args)
+    if not args.no_device_edits:
+        out["device_edits"] = run_device_edits(args)
 
     if args.no_cpu_baseline:
         out["cpu_baseline"] = None
@@ -621,6 +623,66 @@ def run_chunk_build(args):
     return res
 
 
+def run_device_edits(args):
+    """Edit batches of 1000 random voxel set/clear ops on the 2048x2048x512 world (8192
+    chunks): applied on the GPU to the device-resident world (vx_dworld_edit: decode the
+    edited chunks, replay each voxel's edits, rebuild the chunks, place their blocks)
+    against applied to the host world (World::apply_edits on one core, without the upload
+    of the edited pools that follows it). Timed per batch, after two warm-up batches; the
+    device world is then checked against the host world: chunk records (box, dry mask,
+    flags) of every chunk and the dense ids of a sample of edited chunks."""
+    from voxelcraft_menu_amd import serving as S
+    from voxelcraft_menu_amd import workloads as W
+    from voxelcraft_menu_amd.renderer import DeviceWorld, Renderer
+    from voxelcraft_menu_amd.world import ViewSnapshot
+
+    wl = W.WORKLOADS[4]
+    world = W.make_world(wl)
+    r = Renderer(64, 64)
+    dw = DeviceWorld(r, world, capacity_factor=8.0)
+    rng = np.random.default_rng(12)
+    batches = [S.random_edits(rng, wl, 1000) for _ in range(12)]
+    dev_ms, dev_gpu_ms, host_ms, rebuilt = [], [], [], []
+    for k, e in enumerate(batches):
+        t0 = time.perf_counter()
+        dw.edit(e)
+        t1 = time.perf_counter()
+        world.apply_edits(e)
+        t2 = time.perf_counter()
+        if k >= 2:
+            st = dw.stats()
+            dev_ms.append((t1 - t0) * 1e3)
+            dev_gpu_ms.append(st["last_ms"])
+            rebuilt.append(st["chunks_rebuilt"])
+            host_ms.append((t2 - t1) * 1e3)
+    snap = ViewSnapshot(world)
+    host = snap.chunk_table().view(np.uint32).reshape(-1, 24)
+    snap.close()
+    dev = dw.chunks().view(np.uint32).reshape(-1, 24)
+    keep = np.uint32(0xFFFFFFFE)
+    if host.shape != dev.shape or not (np.array_equal(host[:, 0:11], dev[:, 0:11]) and np.array_equal(
+            host[:, 20:22], dev[:, 20:22]) and np.array_equal(host[:, 17] & keep, dev[:, 17] & keep)):
+        _fail("device world records differ from the host world's after the edit batches")
+    e = batches[-1]
+    keys = sorted({(int(x) // 64, int(y) // 64, int(z) // 64) for x, y, z in zip(e["x"], e["y"], e["z"])})
+    sample = keys[::max(1, len(keys) // 32)]
+    for key in sample:
+        if world.contains(key) and not np.array_equal(dw.extract(key), world.extract_dense(key)):
+            _fail(f"device world chunk {key} differs from the host world's after the edit batches")
+    st = dw.stats()
+    dw.close()
+    r.close()
+    med = lambda v: float(np.median(v))
+    res = {"workload": "1000-op-edit-batches-on-2048x2048x512", "batches": len(dev_ms), "ops_per_batch": 1000,
+           "chunks_rebuilt_per_batch": int(med(rebuilt)), "device_ms": round(med(dev_ms), 3),
+           "device_gpu_ms": round(med(dev_gpu_ms), 3), "host_ms_1core": round(med(host_ms), 3),
+           "host_over_device": round(med(host_ms) / med(dev_ms), 2), "used_bytes": int(st["used_bytes"]),
+           "checked_against_host": {"records": int(dev.shape[0]), "chunks_dense": len(sample)}}
+    _log(f"device edits: {res['device_ms']} ms per 1000-op batch ({res['device_gpu_ms']} ms GPU), "
+         f"host {res['host_ms_1core']} ms")
+    return res
+
+
 def run_incoherent(args):
     """The incoherent batch: the 2048x2048x512 world (caves, sea level 96, 1 % glass), 32
     seeded views at 1280x720, primary + sun shadow + one secondary bounce per water / glass
@@ -765,6 +827,7 @@ def main():
     ap.add_argument("--no-incoherent", action="store_true", help="skip the incoherent batch (workload 2 only)")
     ap.add_argument("--no-build", action="store_true", help="skip the device chunk build (workload 2 only)")
     ap.add_argument("--no-view-setup", action="store_true", help="skip the view-setup comparison (workload 2 only)")
+    ap.add_argument("--no-device-edits", action="store_true", help="skip the device edit batches (workload 2 only)")
     ap.add_argument("--no-moving", action="store_true", help="skip the turning-camera steps (workload 2 only)")
     ap.add_argument("--no-layout-ab", action="store_true", help="skip the pool-layout A/B (workload 2 only)")
     ap.add_argument("--pool-layout", choices=["reference", "flat"], default="reference",

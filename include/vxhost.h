@@ -189,6 +189,10 @@ int vxw_build_view_snapshot(const vxw_snapshot *s, const vxw_camera *cam, vx_gpu
  * record (fade 1) of every snapshot chunk with solid voxels, in the order vxw_build_view
  * takes them; returns the count (out may be NULL). */
 size_t vxw_snapshot_chunks(const vxw_snapshot *s, vx_gpu_chunk *out, size_t cap);
+/* Every resident LOD-0 chunk in world order, the empty ones included, with its solid
+ * voxel count (ChunkRecord::solid_count): the table a device world starts from
+ * (vx_dworld_create). */
+size_t vxw_snapshot_chunks_all(const vxw_snapshot *s, vx_gpu_chunk *out, uint32_t *solid, size_t cap);
 /* The three pools re-laid out as one contiguous, 128-byte aligned block per chunk
  * (no counterpart in the reference, whose pools share interned runs between chunks,
  * src/voxel/geometry.rs:12-52, attributes.rs:6-88): the chunk's L1 nodes, the leaf runs

@@ -591,6 +591,46 @@ int vx_build_local(vx_ctx *ctx, const uint16_t *dense, uint32_t n, void *out, si
 /* The same on device memory (dense and out on the context's device, 8-byte aligned),
  * without copies: for callers that keep chunks on the GPU, and to time the build alone. */
 int vx_build_local_device(vx_ctx *ctx, const uint16_t *dense, uint32_t n, void *out, size_t slot_bytes);
+/* ---- Device-resident world and edits on the GPU --------------------------------------
+ * No counterpart call in the reference, whose edits run World::set_block on the CPU
+ * (src/voxel/world.rs:288-529) and upload the dirty ranges (GpuMirror::sync). A device
+ * world holds the pools in the per-chunk block layout (vxw_flat_layout) and the table of
+ * every resident LOD-0 chunk (vxw_snapshot_chunks_all, empty chunks included, with their
+ * solid counts), and the context reads its pools from it until it is destroyed.
+ * vx_dworld_edit applies a batch of edit records on the GPU as set_block applies them one
+ * by one: the edited chunks are decoded to dense ids, each voxel's edits are replayed in
+ * order, the chunks are rebuilt (build_local on the device) and written as new blocks,
+ * with the record fields set_block only grows (box, dry mask, water / foliage / cutout /
+ * glass flags) grown and the water line of each L1 cell an edit changed set as set_block
+ * sets it. Keys and shadow words rendered from the device world equal those of a host
+ * world the same edits were applied to; the pool bytes differ (blocks are appended, runs
+ * are not shared). Edits in chunks outside the table do nothing. A batch runs in passes
+ * of up to 1024 chunks without waiting on the host; VX_E_OVERFLOW when a pass's blocks do
+ * not fit in capacity_words (its chunks and those of later passes keep their old blocks;
+ * chunks_rebuilt counts the placed ones). The space of replaced blocks is not reused:
+ * re-create the device world to compact it.
+ * One difference from set_block: a chunk whose last solid voxel is removed and which gets
+ * solid voxels again within one batch keeps the box of every voxel that became solid in
+ * the batch (set_block restarts the box at the first voxel after the chunk went empty).
+ * vx_dworld_chunks copies the records of the non-empty chunks in table order (the chunk
+ * table view setup starts from); vx_dworld_extract decodes one chunk to dense ids.
+ * A device world is destroyed before its context. */
+typedef struct vx_dworld vx_dworld;
+typedef struct vx_dworld_stats {
+    uint64_t edits, effective_edits;  /* last batch: records, and those that changed a voxel */
+    uint64_t used_bytes, capacity_bytes;
+    uint32_t chunks_rebuilt, chunks;  /* last batch; table */
+    float last_ms;                    /* GPU time of the last batch, decode to placement */
+    uint32_t pad;
+} vx_dworld_stats;
+int vx_dworld_create(vx_ctx *ctx, const uint32_t *flat, size_t flat_words, size_t capacity_words,
+                     const vx_gpu_chunk *table, const uint32_t *solid, uint32_t n, vx_dworld **out);
+int vx_dworld_destroy(vx_dworld *w);
+int vx_dworld_edit(vx_dworld *w, const vx_edit *edits, size_t n);
+int vx_dworld_chunks(vx_dworld *w, vx_gpu_chunk *out, uint32_t cap, uint32_t *n);
+int vx_dworld_extract(vx_dworld *w, int32_t cx, int32_t cy, int32_t cz, uint16_t *dense);
+int vx_dworld_stats_get(vx_dworld *w, vx_dworld_stats *out);
+
 /* Device memory on the context's device for the *_device calls (no counterpart in the
  * reference): allocate, free, and synchronous copies to and from host memory. */
 int vx_dev_alloc(vx_ctx *ctx, size_t bytes, void **out);

@@ -1,0 +1,140 @@
+"""Edit batches applied on the GPU (vx_dworld_*) against the same batches applied to the
+host world (World::set_block, src/voxel/world.rs:288-529).
+
+Three batches of edits on the 256x256x128 world: random set / clear operations over the
+whole world, operations on voxels the cameras see (dug out, built up, turned to water,
+glass, glowstone, leaves), and voxels set and set back within one batch. After each
+batch:
+- the device world's records of the non-empty chunks equal the host's in order, box,
+  dry mask and water / foliage / cutout / glass / emitter flags;
+- every edited chunk decodes to the host world's dense ids;
+- four views rendered from the device world (its records, its block-layout pools) give
+  the keys, Hi-Z words and shadow words the oracle gives on the host world, over two
+  frames.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from voxelcraft_menu_amd import _abi as A
+from voxelcraft_menu_amd import serving as S
+from voxelcraft_menu_amd import workloads as W
+from voxelcraft_menu_amd.renderer import DeviceWorld, Renderer
+from voxelcraft_menu_amd.world import ViewSnapshot, build_view, relocate_view
+
+pytestmark = pytest.mark.gpu
+
+IDS = np.array([0, 1, 2, 8, 10, 11, 13, 7], np.uint16)  # air, stone, dirt, planks, glass, glowstone, water, leaves
+
+
+def _seen_edits(world, cams, rng, n_per_view=250):
+    """Edits on and around voxels the cameras' rays hit (from the oracle's keys)."""
+    out = []
+    for cam in cams:
+        v = build_view(world, cam)
+        spec, hi = W.spec_words(cam.flags, world)
+        sc = O.Scene(world.pool(0), world.pool(1), world.pool(2), v.chunks, v.grid)
+        vis, _, _, _ = O.render(sc, v.frame_bytes, O.spec_from(spec, hi), np.zeros(v.tiles[0] * v.tiles[1], np.uint32),
+                                shadow=False)
+        keys = vis[vis != 0]
+        pick = rng.choice(keys, size=min(n_per_view, keys.size), replace=False)
+        org = v.chunks.view(np.float32).reshape(-1, 24)[:, 8:11]
+        for k in pick:
+            k = int(k)
+            ci, low = (k >> 24) & 0x1FFF, k & 0xFFFFFF
+            base = org[ci].astype(np.int64) + np.array([(low >> 16) & 63, (low >> 8) & 63, low & 63])
+            d = rng.integers(-1, 2, 3)
+            out.append((*(base + d), int(rng.choice(IDS))))
+    e = np.zeros(len(out), A.EDIT_DTYPE)
+    for i, (x, y, z, b) in enumerate(out):
+        e[i] = (x, y, z, b, 0)
+    return e
+
+
+def _set_and_back(world, rng, n=60):
+    """Voxels set to another id and back to their own within the batch."""
+    wl = W.WORKLOADS[0]
+    e = np.zeros(2 * n, A.EDIT_DTYPE)
+    g = wl.gen
+    for i in range(n):
+        x = int(rng.integers(g.chunk_min[0] * 64, g.chunk_max[0] * 64))
+        z = int(rng.integers(g.chunk_min[2] * 64, g.chunk_max[2] * 64))
+        y = int(g.height(x, z)) + int(rng.integers(-3, 3))
+        own = world.get_block(x, y, z)
+        other = 1 if own != 1 else 2
+        e[2 * i] = (x, y, z, other, 0)
+        e[2 * i + 1] = (x, y, z, own, 0)
+    return e
+
+
+def _records_match(world, dw):
+    snap = ViewSnapshot(world)
+    host = snap.chunk_table()
+    snap.close()
+    dev = dw.chunks()
+    assert host.shape == dev.shape
+    hw, dw_ = host.view(np.uint32).reshape(-1, 24), dev.view(np.uint32).reshape(-1, 24)
+    assert np.array_equal(hw[:, 8:11], dw_[:, 8:11])        # origins, in order
+    assert np.array_equal(hw[:, 0:3], dw_[:, 0:3])          # box
+    assert np.array_equal(hw[:, 4:7], dw_[:, 4:7])
+    assert np.array_equal(hw[:, 20:22], dw_[:, 20:22])      # dry mask
+    flags = np.uint32(0xFFFFFFFE)  # the uniform form may differ
+    assert np.array_equal(hw[:, 17] & flags, dw_[:, 17] & flags)
+    return dev
+
+
+def test_device_edits_render_as_host_edits():
+    wl = W.WORKLOADS[0]
+    world = W.make_world(wl)
+    rng = np.random.default_rng(7)
+    res = (320, 192)
+    cams = W.random_cameras(wl, 3, seed=31, resolutions=[res])
+    cams.append(W.default_camera(wl, width=res[0], height=res[1]))
+    r = Renderer(res[0], res[1], device=0)
+    r.sync_world(world)
+    dw = DeviceWorld(r, world)
+    st = dw.stats()
+    assert st["chunks"] > 0 and st["used_bytes"] < st["capacity_bytes"]
+    for batch in range(3):
+        edits = np.concatenate([S.random_edits(rng, wl, 600), _seen_edits(world, cams, rng),
+                                _set_and_back(world, rng)])
+        if batch == 1:
+            edits = edits[rng.permutation(len(edits))]
+        print(f"[dworld] batch {batch}: {len(edits)} edits", flush=True)
+        touched = {(int(x) // 64, int(y) // 64, int(z) // 64) for x, y, z in zip(edits["x"], edits["y"], edits["z"])}
+        world.apply_edits(edits)
+        dw.edit(edits)
+        st = dw.stats()
+        assert st["edits"] == len(edits) and 0 < st["effective_edits"] <= len(edits)
+        dev = _records_match(world, dw)
+        for key in sorted(touched):
+            if world.contains(key):
+                assert np.array_equal(dw.extract(key), world.extract_dense(key)), (batch, key)
+        for i, cam in enumerate(cams):
+            v = build_view(world, cam)
+            vd = relocate_view(v, dev, dev)
+            spec, hi = W.spec_words(cam.flags, world)
+            sc = O.Scene(world.pool(0), world.pool(1), world.pool(2), v.chunks, v.grid)
+            hz = np.zeros(v.tiles[0] * v.tiles[1], np.uint32)
+            r.set_view(vd)
+            r.write_hiz(hz)
+            for frame in range(2):
+                r.render(vd.frame, spec, hi)
+                r.sync()
+                vis, sh, hz, _ = O.render(sc, v.frame_bytes, O.spec_from(spec, hi), hz)
+                assert (vis != 0).sum() > 1000
+                assert np.array_equal(r.read_vis(), vis), (batch, i, frame)
+                assert np.array_equal(r.read_shadow(), sh), (batch, i, frame)
+                assert np.array_equal(r.read_hiz(), hz), (batch, i, frame)
+    # An empty batch and edits outside the world's chunks change nothing.
+    before = dw.chunks().copy()
+    dw.edit(np.zeros(0, A.EDIT_DTYPE))
+    out = np.zeros(3, A.EDIT_DTYPE)
+    out[0] = (-100, 10, 10, 1, 0)
+    out[1] = (10, 200, 10, 1, 0)
+    out[2] = (10, -1, 10, 1, 0)
+    dw.edit(out)
+    assert dw.stats()["effective_edits"] == 0 and dw.stats()["chunks_rebuilt"] == 0
+    assert np.array_equal(dw.chunks(), before)
+    dw.close()
+    r.close()

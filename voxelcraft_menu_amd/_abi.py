@@ -180,6 +180,12 @@ VXW_ABSENT, VXW_EMPTY, VXW_SOLID = 0, 1, 2
 VXW_MAX_LOD = 12
 
 
+class DWorldStats(C.Structure):  # vx_dworld_stats
+    _fields_ = [("edits", C.c_uint64), ("effective_edits", C.c_uint64), ("used_bytes", C.c_uint64),
+                ("capacity_bytes", C.c_uint64), ("chunks_rebuilt", C.c_uint32), ("chunks", C.c_uint32),
+                ("last_ms", C.c_float), ("pad", C.c_uint32)]
+
+
 class PoolMirror(C.Structure):  # vx_pool_mirror
     _fields_ = [("cap", C.c_uint64), ("len", C.c_uint64), ("uploaded_bytes", C.c_uint64),
                 ("generation", C.c_uint32), ("pad", C.c_uint32)]
@@ -351,6 +357,7 @@ _sig_host = {
     "vxw_snapshot_emitter_chunks": (C.c_uint32, [P]),
     "vxw_build_view_snapshot": (C.c_int, [P, C.POINTER(Camera), P, P, C.POINTER(View)]),
     "vxw_snapshot_chunks": (C.c_size_t, [P, P, C.c_size_t]),
+    "vxw_snapshot_chunks_all": (C.c_size_t, [P, P, P, C.c_size_t]),
     "vxw_frustum_planes": (C.c_int, [C.POINTER(Camera), P]),
     "vxw_flat_layout": (C.c_int, [P, C.c_size_t, P, C.c_size_t, P, C.c_size_t, P, C.c_uint32, P, P, C.c_size_t,
                                   C.POINTER(C.c_size_t)]),
@@ -373,6 +380,12 @@ _sig_march = {
     "vx_pool_read": (C.c_int, [P, C.c_int, P, C.c_size_t]),
     "vx_pool_set_flat": (C.c_int, [P, P, C.c_size_t]),
     "vx_taa": (C.c_int, [P]),
+    "vx_dworld_create": (C.c_int, [P, P, C.c_size_t, C.c_size_t, P, P, C.c_uint32, C.POINTER(P)]),
+    "vx_dworld_destroy": (C.c_int, [P]),
+    "vx_dworld_edit": (C.c_int, [P, P, C.c_size_t]),
+    "vx_dworld_chunks": (C.c_int, [P, P, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "vx_dworld_extract": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, P]),
+    "vx_dworld_stats_get": (C.c_int, [P, C.POINTER(DWorldStats)]),
     "vx_read_taa": (C.c_int, [P, P, C.c_size_t]),
     "vx_write_taa_history": (C.c_int, [P, P, C.c_size_t]),
     "vx_render": (C.c_int, [P, C.POINTER(Frame), C.c_uint32, C.c_uint32]),

@@ -449,17 +449,23 @@ int build_group(vx_ctx *c, const uint16_t *d_dense, uint32_t n, uint8_t *d_out, 
 
 }  // namespace
 
+int vxh::build_local_async(vx_ctx *c, const uint16_t *d_dense, uint32_t n, uint8_t *d_out, size_t slot_bytes) {
+    for (uint32_t b = 0; b < n; b += GROUP) {
+        const uint32_t g = std::min(GROUP, n - b);
+        int rc = build_group(c, d_dense + (size_t)b * VOL, g, d_out + (size_t)b * slot_bytes, slot_bytes);
+        if (rc) return rc;
+    }
+    return VX_OK;
+}
+
 extern "C" {
 
 int vx_build_local_device(vx_ctx *c, const uint16_t *dense, uint32_t n, void *out, size_t slot_bytes) {
     if (!c || (n && (!dense || !out)) || slot_bytes < VX_LOCAL_TREE_MAX || slot_bytes % 8u) return VX_E_ARG;
     if ((uintptr_t)dense % 8u || (uintptr_t)out % 8u) return VX_E_ARG;  // 8-byte row loads and record writes
     VX_HIPX(hipSetDevice(c->device));
-    for (uint32_t b = 0; b < n; b += GROUP) {
-        const uint32_t g = std::min(GROUP, n - b);
-        int rc = build_group(c, dense + (size_t)b * VOL, g, (uint8_t *)out + (size_t)b * slot_bytes, slot_bytes);
-        if (rc) return rc;
-    }
+    int rc = vxh::build_local_async(c, dense, n, (uint8_t *)out, slot_bytes);
+    if (rc) return rc;
     VX_HIPX(hipStreamSynchronize(c->stream));
     return VX_OK;
 }

@@ -106,4 +106,7 @@ struct vx_ctx {
 
 namespace vxh {
 vxd::Pools make_pools(const vx_ctx *c);
+// build_local of n dense chunks (device memory) into records (device memory) on the
+// context's stream, without waiting for it.
+int build_local_async(vx_ctx *c, const uint16_t *d_dense, uint32_t n, uint8_t *d_out, size_t slot_bytes);
 }

@@ -354,6 +354,19 @@ extern "C" size_t vxw_snapshot_chunks(const vxw_snapshot *s, vx_gpu_chunk *out, 
     return n;
 }
 
+extern "C" size_t vxw_snapshot_chunks_all(const vxw_snapshot *s, vx_gpu_chunk *out, uint32_t *solid, size_t cap) {
+    if (!s) return 0;
+    size_t n = 0;
+    for (const ViewRec &r : s->recs) {
+        if (n < cap) {
+            if (out) gpu_chunk_of(r, 1.0f, out[n]);
+            if (solid) solid[n] = r.solid_count;
+        }
+        n++;
+    }
+    return n;
+}
+
 extern "C" int vxw_pack_camera_frame(const vxw_camera *cam, uint32_t max_pairs, vx_gpu_frame *f) {
     if (!cam || !f || cam->width == 0 || cam->height == 0) return VXW_E_ARG;
     vxw_view v{};

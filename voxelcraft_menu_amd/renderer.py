@@ -536,3 +536,60 @@ def device_name() -> str:
     buf = C.create_string_buffer(256)
     rc = A.march().vx_device_name(buf, 256)
     return buf.value.decode() if rc == 0 else "none"
+
+
+class DeviceWorld:
+    """A world kept on the GPU in the per-chunk block layout, edited there (vx_dworld_*).
+    The renderer's context reads its pools from it while it lives; views use the chunk
+    records of `chunks()`."""
+
+    def __init__(self, renderer: "Renderer", world: World, capacity_factor: float = 4.0):
+        from .world import ViewSnapshot, flat_layout
+        self._L = A.march()
+        self._r = renderer
+        snap = ViewSnapshot(world)
+        table, solid = snap.chunk_table_all()
+        snap.close()
+        flat, moved = flat_layout(world.pool(0), world.pool(1), world.pool(2), table)
+        cap = int(max(flat.size * capacity_factor, flat.size + (1 << 22)) + 31) // 32 * 32
+        self._w = C.c_void_p()
+        A.check(self._L.vx_dworld_create(renderer._c, A.ptr(flat), flat.size, cap, A.ptr(moved), A.ptr(solid),
+                                         moved.shape[0], C.byref(self._w)), "vx_dworld_create")
+        renderer._batches.append(self)  # closed before the context it reads
+
+    def close(self):
+        if self._w:
+            self._L.vx_dworld_destroy(self._w)
+            self._w = None
+            if self in self._r._batches:
+                self._r._batches.remove(self)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def edit(self, edits: np.ndarray):
+        """Apply a batch of 16-byte edit records (x, y, z: i32; id: u16) in order."""
+        e = np.ascontiguousarray(edits)
+        A.check(self._L.vx_dworld_edit(self._w, A.ptr(e) if len(e) else None, len(e)), "vx_dworld_edit")
+
+    def chunks(self) -> np.ndarray:
+        """(n, 96) records of the non-empty chunks, in table order."""
+        n = C.c_uint32()
+        self._L.vx_dworld_chunks(self._w, None, 0, C.byref(n))
+        out = np.zeros((max(n.value, 1), 96), np.uint8)
+        A.check(self._L.vx_dworld_chunks(self._w, A.ptr(out), n.value, C.byref(n)), "vx_dworld_chunks")
+        return out[:n.value]
+
+    def extract(self, key) -> np.ndarray:
+        out = np.zeros(64 ** 3, np.uint16)
+        A.check(self._L.vx_dworld_extract(self._w, int(key[0]), int(key[1]), int(key[2]), A.ptr(out)),
+                "vx_dworld_extract")
+        return out
+
+    def stats(self) -> dict:
+        st = A.DWorldStats()
+        A.check(self._L.vx_dworld_stats_get(self._w, C.byref(st)), "vx_dworld_stats_get")
+        return {k: getattr(st, k) for k, _ in A.DWorldStats._fields_ if k != "pad"}

@@ -415,6 +415,16 @@ class ViewSnapshot:
             self._L.vxw_snapshot_chunks(self._h, A.ptr(out), n)
         return out[:n]
 
+    def chunk_table_all(self):
+        """Every resident LOD-0 chunk's (96-byte) record, the empty ones included, and its
+        solid voxel count: the table a device world starts from."""
+        n = self._L.vxw_snapshot_chunks_all(self._h, None, None, 0)
+        out = np.zeros((max(n, 1), 96), np.uint8)
+        solid = np.zeros(max(n, 1), np.uint32)
+        if n:
+            self._L.vxw_snapshot_chunks_all(self._h, A.ptr(out), A.ptr(solid), n)
+        return out[:n], solid[:n]
+
     def build_view(self, cam: CameraParams, max_pairs: int = 0) -> ViewData:
         L = A.host()
         c = cam.to_c()
