@@ -23,6 +23,7 @@
  *                         <- build_local / insert_local with a flat LocalTree record
  *                            (the device build's output format, vx_local_tree)
  *   vxw_flat_layout       <- (no counterpart) one pool block per chunk for vx_pool_set_flat
+ *   vxw_shaft_*           <- shaft::ShaftField (src/shaft.rs:1-154)
  *   vxw_journal_parse     <- EditJournal::load (src/journal.rs:391-475)
  *   vxw_journal_write     <- EditJournal::save (src/journal.rs:477-508)
  *   vxw_generate          <- a seeded terrain generator with the reference's
@@ -139,6 +140,31 @@ int vxw_generate_dense(const vxw_gen_params *p, int32_t cx, int32_t cy, int32_t 
  * the structure of WorldGen::generate_coarse (src/worldgen.rs:535-562): one height sample
  * per coarse column, each voxel the block at the top of its span. */
 int vxw_generate_dense_key(const vxw_gen_params *p, const vxw_key *key, uint16_t *out);
+
+/* ---- distant terrain shadows: the shaft height window (src/shaft.rs:1-154) ----
+ * A VXW_SHAFT_DIM^2 window of column heights (row z, then x), one texel per texel x texel
+ * blocks, centred on the camera column; vx_shaft_upload takes its heights and
+ * vx_shaft_scan scans them towards the sun for the resolve's terrain_shade_beyond. The
+ * frame carries the window: shaft_origin = vxw_shaft_origin_world, shaft_texel, and
+ * shaft_soft = VXW_SHAFT_SOFT. */
+#define VXW_SHAFT_DIM 512u    /* shaft::DIM */
+#define VXW_SHAFT_TEXEL 4     /* shaft::TEXEL, the config default (src/config.rs:310) */
+#define VXW_SHAFT_STEPS 8u    /* shaft::STEPS */
+#define VXW_SHAFT_SOFT 4.0f   /* shaft::SOFT */
+typedef struct vxw_shaft vxw_shaft;
+/* ShaftField::with_texel; NULL for a texel outside 1..64. */
+vxw_shaft *vxw_shaft_new(int32_t texel);
+void vxw_shaft_free(vxw_shaft *s);
+/* ShaftField::update with this generator's column heights (vxw_height); canopy_lift > 0
+ * (tree-line canopies, off by default) is refused with VXW_E_ARG. */
+int vxw_shaft_update(vxw_shaft *s, const vxw_gen_params *gen, float cam_x, float cam_z, float canopy_lift);
+const float *vxw_shaft_heights(const vxw_shaft *s);
+void vxw_shaft_origin_world(const vxw_shaft *s, float out[2]);
+int32_t vxw_shaft_texel(const vxw_shaft *s);
+/* 1 when the last update refilled the window (the heights need uploading). */
+int vxw_shaft_dirty(const vxw_shaft *s);
+/* ShaftField::envelope_reference: the brute-force horizon height at texel (tx, tz). */
+float vxw_shaft_envelope(const vxw_shaft *s, const float sun[3], uint32_t tx, uint32_t tz);
 
 /* ---- per-view setup ---- */
 typedef struct vxw_camera {

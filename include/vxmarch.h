@@ -25,6 +25,7 @@
  *   vx_timings                Readback per-pass timestamps   (src/render/timer.rs:150-176)
  *   vx_resolve                the resolve pass, lane 0       (src/render/shaders/resolve.wgsl:1378-1466,
  *                                                            dispatched at src/render/mod.rs:1853-1878)
+ *   vx_shaft_upload/scan      upload_shaft + the shaft scan  (src/render/mod.rs:1346-1352, :1726-1740)
  *   vx_present                the present blit               (src/render/shaders/blit.wgsl:18-93)
  *   vx_batch_*                no counterpart: many views per launch (the reference renders
  *                             one view per render() call, src/headless.rs:680-837)
@@ -84,6 +85,7 @@ enum {
 #define VX_FLAG_WAVE_SHOAL 2048u
 #define VX_FLAG_WAVE_FILL 4096u
 #define VX_FLAG_WATER_SHADOW_CUT 262144u
+#define VX_FLAG_DISTANT_SHADOWS 32768u
 /* spec_hi bits (src/render/mod.rs:220-270). */
 #define VX_FLAG_HI_FULL_MARCH 2u
 #define VX_FLAG_HI_GLASS_REFLECT 128u
@@ -272,12 +274,15 @@ int vx_march_stats_compute(const uint32_t *heat, size_t n, uint32_t chunks, vx_m
  * SPEC_WATER_SEC one pair per 2x2 block stored as f16 where water_share_safe holds --,
  * shore foam, glint and schlick. Leg hits are shaded by shade_hit_legacy with the sun
  * visibility of shadow_ray, without sky specular, with flat light under
- * SPEC_FLAT_SECONDARY. The frame's shading fields (fog_*, ambient, mip_bias, haze_warm,
- * zenith_deep, water_absorb, the wave fields) are read as given. Views that need what is
- * not restated return VX_E_UNSUPPORTED: probes, biome tint (FLAG_TINT with its spec bit),
- * caustics, soft shadows, foliage-rich or canopy-relief albedo, the water look, distant
- * terrain shadows, a cloud deck (cloud_cover > 0), terrain god-ray shafts
- * (godray_strength > 0 with FLAG_TERRAIN_SHAFTS), the underwater view.
+ * SPEC_FLAT_SECONDARY. With FLAG_DISTANT_SHADOWS the sun visibility of a lit point is
+ * multiplied by terrain_shade_beyond (common.wgsl:1571-1580) over the shaft field's last
+ * vx_shaft_scan (zeros before any upload). The frame's shading fields (fog_*, ambient,
+ * mip_bias, haze_warm, zenith_deep, water_absorb, the wave fields, shaft_origin,
+ * shaft_texel, shaft_soft) are read as given. Views that need what is not restated
+ * return VX_E_UNSUPPORTED: probes, biome tint (FLAG_TINT with its spec bit), caustics,
+ * soft shadows, foliage-rich or canopy-relief albedo, the water look, a cloud deck
+ * (cloud_cover > 0), terrain god-ray shafts (godray_strength > 0 with
+ * FLAG_TERRAIN_SHAFTS), the underwater view.
  * vx_resolve_set_assets copies the atlas and block face table the reference's host builds
  * (src/render/mod.rs:2159-2224, src/textures.rs; src/block.rs:96-102): RGBA8 sRGB texels,
  * mip level 0 of every layer, then level 1, ..., level l being (size >> l)^2 texels a
@@ -290,6 +295,15 @@ typedef struct vx_resolve_assets {
 } vx_resolve_assets;
 int vx_resolve_set_assets(vx_ctx *ctx, const vx_resolve_assets *a);
 int vx_resolve(vx_ctx *ctx);      /* asynchronous on the context stream */
+/* Distant terrain shadows. vx_shaft_upload copies a 512 x 512 window of column heights
+ * (Renderer::upload_shaft, src/render/mod.rs:1346-1352; n = 512 * 512, rows of z; the
+ * host's vxw_shaft_* keeps the window). vx_shaft_scan runs shaft_scan's eight passes
+ * (src/render/shaders/shaft.wgsl:1-24, encoded by Renderer::render when
+ * FrameParams::shaft_scan is set, src/render/mod.rs:1726-1740) with the frame's sun_dir
+ * and shaft_texel; vx_read_shaft copies the result (n = 512 * 512 floats). */
+int vx_shaft_upload(vx_ctx *ctx, const float *heights, size_t n);
+int vx_shaft_scan(vx_ctx *ctx, const vx_gpu_frame *frame);
+int vx_read_shaft(vx_ctx *ctx, float *out, size_t n);
 int vx_read_color(vx_ctx *ctx, float *rgba, size_t n);  /* n = W*H*4 */
 /* Present (src/render/shaders/blit.wgsl:18-93, src/render/blit.rs): the colour target
  * drawn onto a surface of the same size -- max(rgb, 0), the knee tonemap or the ACES fit

@@ -110,6 +110,10 @@ def _bind(L):
     L.vxo_resolve.restype = C.c_int
     L.vxo_resolve.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, C.POINTER(RSpec), C.POINTER(ResolveAssets),
                               V, V, V, V, C.POINTER(Stats), C.c_int]
+    L.vxo_shaft_scan.restype = C.c_int
+    L.vxo_shaft_scan.argtypes = [V, V, V]
+    L.vxo_terrain_shade.restype = C.c_float
+    L.vxo_terrain_shade.argtypes = [V, V, V, C.c_float]
     L.vxo_taa.restype = C.c_int
     L.vxo_taa.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, V, V, V]
     L.vxo_present.restype = None
@@ -155,11 +159,19 @@ class RSpec(C.Structure):
     _fields_ = [(k, C.c_uint32) for k in ("sky_specular", "lighting_repair", "sky_tint", "sky_cool", "sky_look",
                                           "light_rgb", "emitter_gather", "shore_wet", "tex_variation", "glass",
                                           "lanes", "water_sec", "shore_foam", "flat_secondary")] + \
-        [("march", C.c_void_p)]
+        [("march", C.c_void_p), ("distant", C.c_uint32), ("shaft", C.c_void_p)]
 
     def with_march(self, spec: "Spec") -> "RSpec":
         self._march = spec  # kept alive with the record
         self.march = C.cast(C.pointer(spec), C.c_void_p).value
+        return self
+
+    def with_shaft(self, result: np.ndarray) -> "RSpec":
+        """Distant terrain shadows from a shaft_scan result slice (512 x 512 f32)."""
+        self._shaft = np.ascontiguousarray(result, dtype=np.float32).reshape(-1)
+        assert self._shaft.size == SHAFT_DIM * SHAFT_DIM
+        self.shaft = self._shaft.ctypes.data
+        self.distant = 1
         return self
 
 
@@ -168,9 +180,41 @@ class ResolveAssets(C.Structure):
                 ("block_faces", C.c_void_p), ("n_faces", C.c_uint32)]
 
 
-def rspec_from(flags: int, spec_hi: int) -> RSpec:
+SHAFT_DIM = 512  # src/shaft.rs:3
+SHAFT_SOFT = 4.0  # src/shaft.rs:9
+
+
+def rspec_from(flags: int, spec_hi: int, shaft: np.ndarray | None = None) -> RSpec:
     """The resolve's override values for a (flags, spec_hi) pipeline key (src/render/mod.rs:
-    2355-2689)."""
+    2355-2689). A key with FLAG_DISTANT_SHADOWS needs the shaft field's scan result."""
+    distant = bool(flags & 32768)
+    if distant and shaft is None:
+        raise ValueError("distant terrain shadows read the shaft field: pass its scan result")
+    r = _rspec(flags, spec_hi)
+    return r.with_shaft(shaft) if distant else r
+
+
+def shaft_scan(heights: np.ndarray, frame_bytes: bytes) -> np.ndarray:
+    """shaft_scan's eight passes (shaft.wgsl) over a 512 x 512 height field with the frame's
+    sun_dir and shaft_texel: the SHAFT_RESULT slice, (512, 512) f32."""
+    hgt = np.ascontiguousarray(heights, dtype=np.float32).reshape(-1)
+    assert hgt.size == SHAFT_DIM * SHAFT_DIM
+    f = np.frombuffer(frame_bytes, dtype=np.uint8).copy()
+    out = np.zeros((SHAFT_DIM, SHAFT_DIM), dtype=np.float32)
+    if lib().vxo_shaft_scan(_p(hgt), _p(f), _p(out)) != 0:
+        raise RuntimeError("oracle shaft scan rejected its arguments")
+    return out
+
+
+def terrain_shade(result: np.ndarray, frame_bytes: bytes, p, d: float = 0.0) -> float:
+    """terrain_shade(p + sun_dir * d) (common.wgsl:1571-1580) over a scan result."""
+    r = np.ascontiguousarray(result, dtype=np.float32).reshape(-1)
+    f = np.frombuffer(frame_bytes, dtype=np.uint8).copy()
+    q = np.asarray(p, dtype=np.float32)
+    return float(lib().vxo_terrain_shade(_p(r), _p(f), _p(q), d))
+
+
+def _rspec(flags: int, spec_hi: int) -> RSpec:
     return RSpec(sky_specular=1 if spec_hi & 1 else 0, lighting_repair=0 if spec_hi & 1048576 else 1,
                  sky_tint=1 if flags & 2147483648 else 0, sky_cool=1 if spec_hi & 2048 else 0,
                  sky_look=1 if spec_hi & 1024 else 0, light_rgb=1 if flags & 268435456 else 0,

@@ -246,7 +246,22 @@ typedef struct vxo_rspec {
      * reflect); march may be NULL when lanes is 0. */
     uint32_t lanes, water_sec, shore_foam, flat_secondary;
     const vxo_spec *march;
+    /* SPEC_DISTANT_SHADOWS: terrain_shade_beyond from the shaft field's result slice
+     * (VXO_SHAFT_DIM^2 f32, the output of vxo_shaft_scan), read when the frame has
+     * FLAG_DISTANT_SHADOWS; shaft may be NULL when distant is 0. */
+    uint32_t distant;
+    const float *shaft;
 } vxo_rspec;
+
+/* Distant terrain shadows (src/shaft.rs, shaft.wgsl:1-24, common.wgsl:1557-1580).
+ * vxo_shaft_scan: the SHAFT_STEPS log-step passes of shaft_scan over a VXO_SHAFT_DIM^2
+ * height field with the frame's sun_dir and shaft_texel; result = the slice pass 7
+ * writes (SHAFT_RESULT). vxo_terrain_shade: terrain_shade(p + sun_dir * d) over such a
+ * result with the frame's shaft_origin, shaft_texel and shaft_soft (d = 0: terrain_shade). */
+#define VXO_SHAFT_DIM 512u
+#define VXO_SHAFT_STEPS 8u
+int vxo_shaft_scan(const float *heights, const vxo_frame *f, float *result);
+float vxo_terrain_shade(const float *result, const vxo_frame *f, const float p[3], float d);
 
 /* The caller's atlas (RGBA8 sRGB: mip level 0 of every layer, then level 1, ...; level l
  * is (size >> l)^2 texels a layer) and block_faces table (8 face words per block id:

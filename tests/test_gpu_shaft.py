@@ -1,0 +1,131 @@
+"""GPU parity of the distant terrain shadows: the shaft scan (vx_shaft_scan) and the resolve
+with FLAG_DISTANT_SHADOWS (terrain_shade_beyond on the sun term), against the oracle
+through the C ABI, bit for bit.
+
+- the scan of a height window of this build's generator, under two suns and two texel
+  sizes, and of a spike on flat ground;
+- resolved views of the 256x256x128 world under a low sun with the window around the
+  camera (opaque pixels, and water and glass pixels whose legs' sun terms take the same
+  factor), over two frames; the oracle without the factor differs on some pixels;
+- a resolve with the flag before any upload reads the zeroed field.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.test_gpu_resolve import BASE_FLAGS, LAYERS, MIPS, SIZE, SKY_SPECULAR, _atlas, _faces_perm
+from voxelcraft_menu_amd import _abi as A
+from voxelcraft_menu_amd import workloads as W
+from voxelcraft_menu_amd.renderer import Renderer
+from voxelcraft_menu_amd.world import ShaftField, build_view
+
+pytestmark = pytest.mark.gpu
+
+DISTANT = 32768
+
+
+def _frame(sun, texel):
+    f = A.Frame()
+    f.sun_dir[0], f.sun_dir[1], f.sun_dir[2] = sun
+    f.shaft_texel = float(texel)
+    return f
+
+
+@pytest.fixture(scope="module")
+def small_world():
+    return W.make_world(W.WORKLOADS[0])
+
+
+def test_scan_matches_oracle():
+    r = Renderer(64, 64)
+    for texel, cam, sun in ((4, (512.0, 300.0), (0.82, 0.04, 0.54)), (8, (-100.0, 2000.0), (-0.3, 0.03, 0.93))):
+        sf = ShaftField(texel)
+        sf.update(W.WORKLOADS[1].gen, *cam)
+        h = sf.heights()
+        fr = _frame(sun, texel)
+        print(f"[shaft] scan texel {texel} sun {sun}", flush=True)
+        r.upload_shaft(h)
+        r.shaft_scan(fr)
+        r.sync()
+        got = r.read_shaft()
+        want = O.shaft_scan(h, bytes(fr))
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (texel, int((got != want).sum()))
+        assert (got > h + 0.5).mean() > 0.01
+        sf.close()
+    spike = np.zeros((512, 512), np.float32)
+    spike[256, 300] = 200.0
+    fr = _frame((0.9396926, 0.34202015, 0.0), 4)
+    r.upload_shaft(spike)
+    r.shaft_scan(fr)
+    r.sync()
+    assert np.array_equal(r.read_shaft().view(np.uint32), O.shaft_scan(spike, bytes(fr)).view(np.uint32))
+    r.close()
+
+
+def test_resolve_with_distant_shadows_matches_oracle(small_world):
+    wl = W.WORKLOADS[0]
+    flags = BASE_FLAGS | DISTANT
+    cams = W.random_cameras(wl, 3, seed=41, resolutions=[(256, 144), (200, 120)], flags=flags)
+    cams[2].pitch = -0.6  # onto the terrain and the sea
+    atlas, faces = _atlas(9), _faces_perm()
+    changed = 0
+    for i, cam in enumerate(cams):
+        v = build_view(small_world, cam)
+        v.frame.sun_dir[0], v.frame.sun_dir[1], v.frame.sun_dir[2] = 0.7962, 0.0839, 0.5991  # a low sun
+        v.frame.ambient, v.frame.mip_bias = 0.04, -4.0
+        v.frame.fog_density, v.frame.fog_falloff, v.frame.fog_height = 0.002, 0.02, 70.0
+        sf = ShaftField()
+        sf.update(wl.gen, v.frame.cam_pos[0], v.frame.cam_pos[2])
+        sf.set_frame(v.frame)
+        spec, hi = W.spec_words(cam.flags, small_world)
+        hi = (hi | SKY_SPECULAR) & ~(64 | 8192 | 16384 | 65536)
+        r = Renderer(cam.width, cam.height)
+        r.sync_world(small_world)
+        r.set_resolve_assets(atlas, SIZE, LAYERS, MIPS, faces)
+        r.upload_shaft(sf.heights())
+        r.shaft_scan(v.frame)
+        r.set_view(v)
+        sc = O.Scene(small_world.pool(0), small_world.pool(1), small_world.pool(2), v.chunks, v.grid)
+        field = O.shaft_scan(sf.heights(), v.frame_bytes)
+        hz = np.zeros(v.tiles[0] * v.tiles[1], np.uint32)
+        print(f"[shaft] resolve view {i}", flush=True)
+        for frame in range(2):
+            r.render(v.frame, spec, hi)
+            vis, sun, hz, _ = O.render(sc, v.frame_bytes, O.spec_from(spec, hi), hz)
+            r.resolve()
+            r.sync()
+            got = r.read_color()
+            assert np.array_equal(r.read_vis(), vis)
+            ref = O.resolve(sc, v.frame_bytes, O.rspec_from(spec, hi, shaft=field), atlas, SIZE, LAYERS, MIPS, faces,
+                            vis, sun)
+            diff = got.view(np.uint32) != ref.view(np.uint32)
+            assert not diff.any(), f"view {i} frame {frame}: {diff.any(axis=-1).sum()} pixels differ"
+        plain = O.resolve(sc, v.frame_bytes, O.rspec_from(spec & ~DISTANT, hi), atlas, SIZE, LAYERS, MIPS, faces,
+                          vis, sun)
+        changed += int((plain.view(np.uint32) != ref.view(np.uint32)).any(axis=-1).sum())
+        r.close()
+        sf.close()
+    assert changed > 0  # the distant term darkened some pixels
+
+
+def test_resolve_before_any_upload_reads_the_zeroed_field(small_world):
+    wl = W.WORKLOADS[0]
+    cam = W.default_camera(wl, width=160, height=96, flags=BASE_FLAGS | DISTANT)
+    v = build_view(small_world, cam)
+    v.frame.shaft_texel, v.frame.shaft_soft = 4.0, 4.0
+    v.frame.shaft_origin[0], v.frame.shaft_origin[1] = -1024.0, -1024.0
+    spec, hi = W.spec_words(cam.flags, small_world)
+    atlas, faces = _atlas(2), _faces_perm()
+    r = Renderer(160, 96)
+    r.sync_world(small_world)
+    r.set_resolve_assets(atlas, SIZE, LAYERS, MIPS, faces)
+    r.set_view(v)
+    r.render(v.frame, spec, hi)
+    r.resolve()
+    r.sync()
+    sc = O.Scene(small_world.pool(0), small_world.pool(1), small_world.pool(2), v.chunks, v.grid)
+    vis, sun, _, _ = O.render(sc, v.frame_bytes, O.spec_from(spec, hi), np.zeros(v.tiles[0] * v.tiles[1], np.uint32))
+    zero = np.zeros((512, 512), np.float32)
+    ref = O.resolve(sc, v.frame_bytes, O.rspec_from(spec, hi, shaft=zero), atlas, SIZE, LAYERS, MIPS, faces, vis, sun)
+    assert np.array_equal(r.read_color().view(np.uint32), ref.view(np.uint32))
+    r.close()

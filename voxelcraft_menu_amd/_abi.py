@@ -362,6 +362,14 @@ _sig_host = {
     "vxw_flat_layout": (C.c_int, [P, C.c_size_t, P, C.c_size_t, P, C.c_size_t, P, C.c_uint32, P, P, C.c_size_t,
                                   C.POINTER(C.c_size_t)]),
     "vxw_pack_camera_frame": (C.c_int, [C.POINTER(Camera), C.c_uint32, C.POINTER(Frame)]),
+    "vxw_shaft_new": (P, [C.c_int32]),
+    "vxw_shaft_free": (None, [P]),
+    "vxw_shaft_update": (C.c_int, [P, C.POINTER(GenParams), C.c_float, C.c_float, C.c_float]),
+    "vxw_shaft_heights": (P, [P]),
+    "vxw_shaft_origin_world": (None, [P, P]),
+    "vxw_shaft_texel": (C.c_int32, [P]),
+    "vxw_shaft_dirty": (C.c_int, [P]),
+    "vxw_shaft_envelope": (C.c_float, [P, P, C.c_uint32, C.c_uint32]),
     "vxw_journal_parse": (C.c_int, [P, C.c_size_t, C.c_int32, C.c_int32, C.POINTER(JournalInfo), P, C.c_size_t,
                                     C.c_char_p, C.c_size_t]),
     "vxw_journal_write": (C.c_int, [P, C.c_size_t, C.c_int32, C.c_int32, C.POINTER(Player), P, P, C.c_size_t,
@@ -380,6 +388,9 @@ _sig_march = {
     "vx_pool_read": (C.c_int, [P, C.c_int, P, C.c_size_t]),
     "vx_pool_set_flat": (C.c_int, [P, P, C.c_size_t]),
     "vx_taa": (C.c_int, [P]),
+    "vx_shaft_upload": (C.c_int, [P, P, C.c_size_t]),
+    "vx_shaft_scan": (C.c_int, [P, C.POINTER(Frame)]),
+    "vx_read_shaft": (C.c_int, [P, P, C.c_size_t]),
     "vx_dworld_create": (C.c_int, [P, P, C.c_size_t, C.c_size_t, P, P, C.c_uint32, C.POINTER(P)]),
     "vx_dworld_destroy": (C.c_int, [P]),
     "vx_dworld_edit": (C.c_int, [P, P, C.c_size_t]),

@@ -97,6 +97,7 @@ struct vx_ctx {
     // one that the present reads; taa_valid while the last pass over the colour was vx_taa.
     vxh::DevBuf hist[2], taa_f32;
     vxh::DevBuf sec_r, sec_m;  // the resolve's water_sec legs (2x2 blocks, f16 RGBA)
+    vxh::DevBuf shaft;         // the distant-shadow field: scan slices 0 and 1, heights in slice 2
     size_t hist_len = 0;
     uint32_t taa_last = 0;
     bool taa_valid = false;

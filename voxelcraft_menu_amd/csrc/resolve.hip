@@ -24,7 +24,8 @@ constexpr uint32_t PERM_NONE = 0u, PERM_D4 = 2u;
 // Resolve switches (pipeline-key overrides, src/render/mod.rs:2355-2689).
 constexpr uint32_t RS_SKY_SPECULAR = 1u, RS_LIGHTING_REPAIR = 2u, RS_SKY_TINT = 4u, RS_SKY_COOL = 8u, RS_SKY_LOOK = 16u,
                    RS_LIGHT_RGB = 32u, RS_EMITTER_GATHER = 64u, RS_SHORE_WET = 128u, RS_TEX_VARIATION = 256u,
-                   RS_GLASS = 512u, RS_WATER_SEC = 1024u, RS_SHORE_FOAM = 2048u, RS_FLAT_SECONDARY = 4096u;
+                   RS_GLASS = 512u, RS_WATER_SEC = 1024u, RS_SHORE_FOAM = 2048u, RS_FLAT_SECONDARY = 4096u,
+                   RS_DISTANT = 8192u;
 
 struct ResolveK {
     Pools p;
@@ -46,6 +47,9 @@ struct ResolveK {
     ushort4 *__restrict__ sec_m;
     uint32_t hw, hh;
     float water_absorb;
+    // distant terrain shadows: the shaft field's result slice and the frame's window
+    const float *__restrict__ shaft;
+    float shaft_origin[2], shaft_texel, shaft_soft;
 };
 
 // ---- elementary functions (the oracle's vxo_log2 / vxo_exp2 / vxo_pow) ----
@@ -98,6 +102,41 @@ __device__ __forceinline__ float r_exp2(float x) {
 
 __device__ __forceinline__ float r_pow(float x, float y) { return r_exp2(y * r_log2(x)); }
 __device__ __forceinline__ float r_mix(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+
+// ---- distant terrain shadows: the shaft field (shaft.wgsl:1-24, common.wgsl:1557-1580) ----
+// The buffer holds three 512 x 512 f32 slices: 0 and 1 the scan's ping-pong, 2 the heights
+// (SHAFT_HEIGHTS); pass k reads the heights (k = 0) or slice (k - 1) & 1 and writes slice
+// k & 1, so the eighth pass leaves the result in slice 1 (SHAFT_RESULT).
+constexpr uint32_t SHAFT_DIM = 512u, SHAFT_SLICE = SHAFT_DIM * SHAFT_DIM, SHAFT_STEPS = 8u;
+
+// shaft_sample (common.wgsl:1557-1569)
+__device__ __forceinline__ float shaft_sample(const float *__restrict__ buf, float tx, float tz) {
+    const float hi = (float)SHAFT_DIM - 1.0f;
+    const float cx = fminf(fmaxf(tx, 0.0f), hi), cz = fminf(fmaxf(tz, 0.0f), hi);
+    const float f0x = floorf(cx), f0z = floorf(cz);
+    const float frx = cx - f0x, frz = cz - f0z;
+    const uint32_t i0x = (uint32_t)f0x, i0z = (uint32_t)f0z;
+    const uint32_t i1x = min(i0x + 1u, SHAFT_DIM - 1u), i1z = min(i0z + 1u, SHAFT_DIM - 1u);
+    const float a = buf[i0z * SHAFT_DIM + i0x], b = buf[i0z * SHAFT_DIM + i1x];
+    const float c0 = buf[i1z * SHAFT_DIM + i0x], d = buf[i1z * SHAFT_DIM + i1x];
+    return r_mix(r_mix(a, b, frx), r_mix(c0, d, frx), frz);
+}
+
+// shaft_scan (shaft.wgsl:3-24), pass k: 16 x 16 texels a block over the 512 x 512 slice.
+__global__ __launch_bounds__(256) void k_shaft_scan(float *__restrict__ shaft, uint32_t k, float sx, float sy,
+                                                    float sz, float texel) {
+    const uint32_t x = blockIdx.x * 16u + (threadIdx.x & 15u), z = blockIdx.y * 16u + (threadIdx.x >> 4);
+    if (x >= SHAFT_DIM || z >= SHAFT_DIM) return;
+    const float *src = shaft + (k > 0u ? ((k - 1u) & 1u) * SHAFT_SLICE : 2u * SHAFT_SLICE);
+    float *dst = shaft + (k & 1u) * SHAFT_SLICE;
+    const float span = (float)(1u << k);
+    const float horiz = fmaxf(sqrtf(sx * sx + sz * sz), 1e-3f);
+    const float dx = sx / horiz, dz = sz / horiz;
+    const float climb = span * texel * (sy / horiz);
+    const float here = src[z * SHAFT_DIM + x];
+    const float carried = shaft_sample(src, (float)x + dx * span, (float)z + dz * span) - climb;
+    dst[z * SHAFT_DIM + x] = fmaxf(here, carried);
+}
 __device__ __forceinline__ float r_clamp(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 
 __device__ __forceinline__ float srgb_decode(uint32_t u) {
@@ -326,6 +365,19 @@ __device__ __forceinline__ uint32_t perm_key(const float b[3], uint32_t nid) {
                     (nid * 0x27d4eb2fu));
 }
 
+// terrain_shade_beyond (common.wgsl:1571-1580): the scanned horizon at p + sun_dir * d
+// (texel centres at origin + (i + 0.5) texel), ramped over +-shaft_soft around its y.
+__device__ inline float terrain_shade_beyond(const ResolveK &k, const float p[3], float d) {
+    float q[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) q[i] = p[i] + k.f.sun[i] * d;
+    const float tx = (q[0] - k.shaft_origin[0]) / k.shaft_texel - 0.5f;
+    const float tz = (q[2] - k.shaft_origin[1]) / k.shaft_texel - 0.5f;
+    const float e = shaft_sample(k.shaft, tx, tz);
+    return wgsl_smoothstep(e - k.shaft_soft, e + k.shaft_soft, q[1]);
+}
+
+
 // shade_hit_compact_cached / shade_hit_legacy_cached (resolve.wgsl:1537-2059) with smooth
 // light and do_spec, for lane 0; shade_hit_legacy (:289-586) for the secondary legs, whose
 // sun visibility lit_word the caller takes from shadow_ray, without do_spec, and with flat
@@ -333,7 +385,7 @@ __device__ __forceinline__ uint32_t perm_key(const float b[3], uint32_t nid) {
 // only (not foliage quads).
 __device__ inline void shade_opaque(const ResolveK &k, uint32_t ci, uint32_t vx, uint32_t vy, uint32_t vz, uint32_t id,
                                     uint32_t nid, const float hit[3], const float rd[3], float dist, float lit_word,
-                                    bool smooth, bool do_spec, float out[3]) {
+                                    float shadow_dist, bool smooth, bool do_spec, float out[3]) {
     const Pools &p = k.p;
     const FrameK &f = k.f;
     const ChunkK ch = ld_chunk(p, ci);
@@ -501,7 +553,12 @@ __device__ inline void shade_opaque(const ResolveK &k, uint32_t ci, uint32_t vx,
     const float ndl = dot3o(n, f.sun);
     if (ndl > 0.0f && f.daylight > 0.01f) {
         float lit = 1.0f;
-        if (f.flags & VX_FLAG_SHADOWS) lit = lit_word;
+        if (f.flags & VX_FLAG_SHADOWS) {
+            lit = lit_word;
+            // terrain beyond the march's reach (resolve.wgsl:516-518, :1729-1731)
+            if ((k.bits & RS_DISTANT) && (f.flags & VX_FLAG_DISTANT_SHADOWS) && lit > 0.0f)
+                lit = lit * terrain_shade_beyond(k, hit, shadow_dist);
+        }
         direct = lit * ndl * sky_sm;
     }
     constexpr float FACE_SHADE[8] = {0.62f, 0.62f, 0.50f, 1.00f, 0.80f, 0.80f, 0.90f, 0.90f};
@@ -657,8 +714,8 @@ __device__ inline void shade_leg_hit(const ResolveK &k, const WorldHit &w, const
         for (int q = 0; q < 3; q++) o[q] = pt[q] + n[q] * (0.02f * c.vs);
         lit = shadow_ray(k.p, f, k.s, o, shadow_dist) ? 0.0f : 1.0f;
     }
-    shade_opaque(k, w.ci, v[0], v[1], v[2], id, w.normal, pt, rd, dist, lit, !(k.bits & RS_FLAT_SECONDARY), false,
-                 out);
+    shade_opaque(k, w.ci, v[0], v[1], v[2], id, w.normal, pt, rd, dist, lit, shadow_dist,
+                 !(k.bits & RS_FLAT_SECONDARY), false, out);
 }
 
 struct WaterCtx {
@@ -1029,7 +1086,7 @@ __global__ __launch_bounds__(64) void k_resolve(ResolveK k) {
         const float d = f16_word(k.sun ? k.sun[pix] : (uint16_t)0);
         const float occ = d > 0.0f ? 1.0f : 0.0f;
         shade_opaque(k, ci, (low >> 16) & 63u, (low >> 8) & 63u, low & 63u, h.id, h.nid, h.hit, h.rd, h.t,
-                     r_clamp(1.0f - occ, 0.0f, 1.0f), true, true, color);
+                     r_clamp(1.0f - occ, 0.0f, 1.0f), f.shadow_dist, true, true, color);
         const float alpha = 1.0f - transmittance_from(k, f.cam[1] + 0.0f * h.rd[1], h.t - 0.0f, h.rd);
         float sky[3];
         sky_base(k, h.rd, 1.0f, sky);
@@ -1290,6 +1347,14 @@ using namespace vxd;
 
 namespace {
 
+// The shaft buffer (three slices), zeroed when first made, as the reference's buffer starts.
+int shaft_buffer(vx_ctx *c) {
+    if (c->shaft.p) return VX_OK;
+    int rc;
+    if ((rc = vxh::ensure(c->shaft, (size_t)3 * SHAFT_SLICE * 4))) return rc;
+    return vxh::fill_u32(c->stream, c->shaft.p, 0u, (size_t)3 * SHAFT_SLICE);
+}
+
 // The resolve switches of a pipeline key, or VX_E_UNSUPPORTED for what is not restated.
 int resolve_bits(uint32_t spec, uint32_t spec_hi, const vx_gpu_frame &g, uint32_t &bits) {
     if (g.flags & VX_FLAG_HEATMAP) {
@@ -1299,7 +1364,6 @@ int resolve_bits(uint32_t spec, uint32_t spec_hi, const vx_gpu_frame &g, uint32_
     constexpr uint32_t PROBES = 16777216u | 33554432u | 67108864u | 134217728u | 536870912u | 1073741824u;
     if ((g.flags & VX_FLAG_UNDERWATER) || (spec & PROBES) || (spec & VX_FLAG_FOLIAGE)) return VX_E_UNSUPPORTED;
     if ((spec & 128u) && (g.flags & 128u)) return VX_E_UNSUPPORTED;        // biome tint
-    if ((spec & 32768u) && (g.flags & 32768u)) return VX_E_UNSUPPORTED;    // distant terrain shadows
     if (spec_hi & (64u | 8192u | 16384u | 65536u)) return VX_E_UNSUPPORTED;  // caustics, foliage rich, canopy relief,
                                                                               // soft shadows
     if (spec_hi & 4096u) return VX_E_UNSUPPORTED;                           // the water look (ripples, murk)
@@ -1320,6 +1384,7 @@ int resolve_bits(uint32_t spec, uint32_t spec_hi, const vx_gpu_frame &g, uint32_
     if (spec_hi & 32u) bits |= RS_WATER_SEC;
     if (spec_hi & 16u) bits |= RS_SHORE_FOAM;
     if (spec & 1048576u) bits |= RS_FLAT_SECONDARY;
+    if (spec & VX_FLAG_DISTANT_SHADOWS) bits |= RS_DISTANT;
     return VX_OK;
 }
 
@@ -1382,6 +1447,15 @@ int vx_resolve(vx_ctx *c) {
     k.water_absorb = g.water_absorb;
     k.hw = (c->w + 1u) / 2u;
     k.hh = (c->h + 1u) / 2u;
+    k.shaft = nullptr;
+    k.shaft_origin[0] = g.shaft_origin[0];
+    k.shaft_origin[1] = g.shaft_origin[1];
+    k.shaft_texel = g.shaft_texel;
+    k.shaft_soft = g.shaft_soft;
+    if ((bits & RS_DISTANT) && (g.flags & VX_FLAG_DISTANT_SHADOWS)) {
+        if ((rc = shaft_buffer(c))) return rc;  // never uploaded: the zeroed field
+        k.shaft = (const float *)c->shaft.p + SHAFT_SLICE;
+    }
     const bool lanes = !(g.flags & VX_FLAG_HEATMAP);
     if (lanes && (bits & RS_WATER_SEC)) {
         const size_t sec_bytes = (size_t)k.hw * k.hh * 8;
@@ -1402,6 +1476,41 @@ int vx_resolve(vx_ctx *c) {
     c->color_valid = true;
     c->image_valid = false;
     c->taa_valid = false;
+    return VX_OK;
+}
+
+int vx_shaft_upload(vx_ctx *c, const float *heights, size_t n) {
+    if (!c || !heights || n != SHAFT_SLICE) return VX_E_ARG;
+    HIPX(hipSetDevice(c->device));
+    int rc;
+    if ((rc = shaft_buffer(c))) return rc;
+    HIPX(hipMemcpyAsync((float *)c->shaft.p + 2u * SHAFT_SLICE, heights, (size_t)SHAFT_SLICE * 4,
+                        hipMemcpyHostToDevice, c->stream));
+    HIPX(hipStreamSynchronize(c->stream));
+    return VX_OK;
+}
+
+int vx_shaft_scan(vx_ctx *c, const vx_gpu_frame *frame) {
+    if (!c || !frame) return VX_E_ARG;
+    HIPX(hipSetDevice(c->device));
+    int rc;
+    if ((rc = shaft_buffer(c))) return rc;
+    const dim3 grid(SHAFT_DIM / 16u, SHAFT_DIM / 16u);
+    for (uint32_t k = 0; k < SHAFT_STEPS; k++)
+        k_shaft_scan<<<grid, 256, 0, c->stream>>>((float *)c->shaft.p, k, frame->sun_dir[0], frame->sun_dir[1],
+                                                  frame->sun_dir[2], frame->shaft_texel);
+    VX_LAUNCHED();
+    return VX_OK;
+}
+
+int vx_read_shaft(vx_ctx *c, float *out, size_t n) {
+    if (!c || !out || n != SHAFT_SLICE) return VX_E_ARG;
+    HIPX(hipSetDevice(c->device));
+    int rc;
+    if ((rc = shaft_buffer(c))) return rc;
+    HIPX(hipMemcpyAsync(out, (const float *)c->shaft.p + SHAFT_SLICE, (size_t)SHAFT_SLICE * 4,
+                        hipMemcpyDeviceToHost, c->stream));
+    HIPX(hipStreamSynchronize(c->stream));
     return VX_OK;
 }
 

@@ -168,6 +168,22 @@ class Renderer:
         A.check(self._L.vx_read_color(self._c, A.ptr(out), out.size), "vx_read_color")
         return out
 
+    def upload_shaft(self, heights: np.ndarray):
+        """The shaft field's 512 x 512 column heights (Renderer::upload_shaft)."""
+        h = np.ascontiguousarray(heights, dtype=np.float32).reshape(-1)
+        A.check(self._L.vx_shaft_upload(self._c, A.ptr(h), h.size), "vx_shaft_upload")
+
+    def shaft_scan(self, frame: A.Frame):
+        """shaft_scan's eight passes with the frame's sun_dir and shaft_texel; the distant
+        terrain shadows of the next resolve read the result."""
+        A.check(self._L.vx_shaft_scan(self._c, C.byref(frame)), "vx_shaft_scan")
+
+    def read_shaft(self) -> np.ndarray:
+        """(512, 512) f32: the last scan's result."""
+        out = np.zeros((512, 512), dtype=np.float32)
+        A.check(self._L.vx_read_shaft(self._c, A.ptr(out), out.size), "vx_read_shaft")
+        return out
+
     def taa(self):
         """The TAA pass (taa.wgsl) over the last resolve, with the frame's taa_feedback and
         prev_view_proj; its output becomes the history and what present draws."""

@@ -168,6 +168,63 @@ class GenParams:
         return out
 
 
+class ShaftField:
+    """shaft::ShaftField (src/shaft.rs): the 512 x 512 window of column heights, one texel
+    per texel x texel blocks, centred on the camera, that the distant terrain shadows scan
+    (vxw_shaft_*)."""
+
+    DIM = 512
+    SOFT = 4.0
+
+    def __init__(self, texel: int = 4):
+        self._L = A.host()
+        self._s = self._L.vxw_shaft_new(int(texel))
+        if not self._s:
+            raise ValueError(f"shaft texel {texel} outside 1..64")
+
+    def close(self):
+        if self._s:
+            self._L.vxw_shaft_free(self._s)
+            self._s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def update(self, gen: GenParams, cam_x: float, cam_z: float, canopy_lift: float = 0.0):
+        A.check(self._L.vxw_shaft_update(self._s, C.byref(gen.to_c()), cam_x, cam_z, canopy_lift),
+                "vxw_shaft_update")
+
+    def heights(self) -> np.ndarray:
+        p = self._L.vxw_shaft_heights(self._s)
+        return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_float)), shape=(self.DIM, self.DIM)).copy()
+
+    def origin_world(self):
+        out = (C.c_float * 2)()
+        self._L.vxw_shaft_origin_world(self._s, out)
+        return float(out[0]), float(out[1])
+
+    def texel(self) -> int:
+        return int(self._L.vxw_shaft_texel(self._s))
+
+    def dirty(self) -> bool:
+        return bool(self._L.vxw_shaft_dirty(self._s))
+
+    def envelope_reference(self, sun, tx: int, tz: int) -> float:
+        s = np.asarray(sun, dtype=np.float32)
+        return float(self._L.vxw_shaft_envelope(self._s, A.ptr(s), tx, tz))
+
+    def set_frame(self, frame: A.Frame):
+        """The frame's window fields, as Renderer::render writes them (src/render/mod.rs:
+        1695-1697)."""
+        o = self.origin_world()
+        frame.shaft_origin[0], frame.shaft_origin[1] = o
+        frame.shaft_texel = float(self.texel())
+        frame.shaft_soft = self.SOFT
+
+
 def chunk_key(key, lod: int = 0) -> A.Key:
     if not 0 <= lod <= A.VXW_MAX_LOD:
         raise ValueError(f"lod {lod} outside 0..{A.VXW_MAX_LOD}")
