@@ -278,8 +278,10 @@ int vx_march_stats_compute(const uint32_t *heat, size_t n, uint32_t chunks, vx_m
  * multiplied by terrain_shade_beyond (common.wgsl:1571-1580) over the shaft field's last
  * vx_shaft_scan (zeros before any upload). The frame's shading fields (fog_*, ambient,
  * mip_bias, haze_warm, zenith_deep, water_absorb, the wave fields, shaft_origin,
- * shaft_texel, shaft_soft) are read as given. Views that need what is not restated
- * return VX_E_UNSUPPORTED: probes, biome tint (FLAG_TINT with its spec bit), caustics,
+ * shaft_texel, shaft_soft, tint_*) are read as given. With FLAG_TINT the albedo takes the
+ * biome tint (resolve.wgsl:166-287: OpenSimplex2 temperature and humidity at the hit's
+ * x, z, band weights, the tint table) weighted by the texel's alpha and tint_strength.
+ * Views that need what is not restated return VX_E_UNSUPPORTED: probes, caustics,
  * soft shadows, foliage-rich or canopy-relief albedo, the water look, a cloud deck
  * (cloud_cover > 0), terrain god-ray shafts (godray_strength > 0 with
  * FLAG_TERRAIN_SHAFTS), the underwater view.

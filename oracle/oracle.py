@@ -114,6 +114,10 @@ def _bind(L):
     L.vxo_shaft_scan.argtypes = [V, V, V]
     L.vxo_terrain_shade.restype = C.c_float
     L.vxo_terrain_shade.argtypes = [V, V, V, C.c_float]
+    L.vxo_biome_noise.restype = C.c_float
+    L.vxo_biome_noise.argtypes = [C.c_int32, C.c_float, C.c_float, C.c_float]
+    L.vxo_biome_tint.restype = None
+    L.vxo_biome_tint.argtypes = [V, C.c_uint32, C.c_float, C.c_float, V]
     L.vxo_taa.restype = C.c_int
     L.vxo_taa.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, V, V, V]
     L.vxo_present.restype = None
@@ -159,7 +163,8 @@ class RSpec(C.Structure):
     _fields_ = [(k, C.c_uint32) for k in ("sky_specular", "lighting_repair", "sky_tint", "sky_cool", "sky_look",
                                           "light_rgb", "emitter_gather", "shore_wet", "tex_variation", "glass",
                                           "lanes", "water_sec", "shore_foam", "flat_secondary")] + \
-        [("march", C.c_void_p), ("distant", C.c_uint32), ("shaft", C.c_void_p)]
+        [("march", C.c_void_p), ("distant", C.c_uint32), ("shaft", C.c_void_p), ("tint", C.c_uint32),
+         ("tint_balance", C.c_uint32)]
 
     def with_march(self, spec: "Spec") -> "RSpec":
         self._march = spec  # kept alive with the record
@@ -214,6 +219,19 @@ def terrain_shade(result: np.ndarray, frame_bytes: bytes, p, d: float = 0.0) -> 
     return float(lib().vxo_terrain_shade(_p(r), _p(f), _p(q), d))
 
 
+def biome_noise(seed: int, freq: float, x: float, z: float) -> float:
+    """biome_noise (resolve.wgsl:201-257) at world (x, z)."""
+    return float(lib().vxo_biome_noise(seed, freq, x, z))
+
+
+def biome_tint(frame_bytes: bytes, x: float, z: float, balance: int = 0) -> np.ndarray:
+    """biome_tint (resolve.wgsl:274-287) at world (x, z) with the frame's tint fields."""
+    f = np.frombuffer(frame_bytes, dtype=np.uint8).copy()
+    out = np.zeros(3, np.float32)
+    lib().vxo_biome_tint(_p(f), balance, x, z, _p(out))
+    return out
+
+
 def _rspec(flags: int, spec_hi: int) -> RSpec:
     return RSpec(sky_specular=1 if spec_hi & 1 else 0, lighting_repair=0 if spec_hi & 1048576 else 1,
                  sky_tint=1 if flags & 2147483648 else 0, sky_cool=1 if spec_hi & 2048 else 0,
@@ -221,7 +239,8 @@ def _rspec(flags: int, spec_hi: int) -> RSpec:
                  emitter_gather=1 if spec_hi & 4 else 0, shore_wet=1 if spec_hi & 8 else 0,
                  tex_variation=1 if flags & 16384 else 0, glass=1 if flags & 131072 else 0,
                  lanes=1, water_sec=1 if spec_hi & 32 else 0, shore_foam=1 if spec_hi & 16 else 0,
-                 flat_secondary=1 if flags & 1048576 else 0).with_march(spec_from(flags, spec_hi))
+                 flat_secondary=1 if flags & 1048576 else 0, tint=1 if flags & 128 else 0,
+                 tint_balance=1 if spec_hi & 512 else 0).with_march(spec_from(flags, spec_hi))
 
 
 def resolve(scene: "Scene", frame_bytes: bytes, rspec: RSpec, atlas: np.ndarray, size: int, layers: int, mips: int,

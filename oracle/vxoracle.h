@@ -251,6 +251,8 @@ typedef struct vxo_rspec {
      * FLAG_DISTANT_SHADOWS; shaft may be NULL when distant is 0. */
     uint32_t distant;
     const float *shaft;
+    /* SPEC_TINT (the biome tint where the frame has FLAG_TINT), SPEC_TINT_BALANCE */
+    uint32_t tint, tint_balance;
 } vxo_rspec;
 
 /* Distant terrain shadows (src/shaft.rs, shaft.wgsl:1-24, common.wgsl:1557-1580).
@@ -261,6 +263,11 @@ typedef struct vxo_rspec {
 #define VXO_SHAFT_DIM 512u
 #define VXO_SHAFT_STEPS 8u
 int vxo_shaft_scan(const float *heights, const vxo_frame *f, float *result);
+/* The biome tint (resolve.wgsl:166-287): biome_noise (2D OpenSimplex2 as FastNoiseLite
+ * 1.1.1's SingleOpenSimplex2, normalised to [-1, 1]) and biome_tint at world (x, z) with
+ * the frame's tint_seed and tint frequencies (balance: TINT_PLAINS_BALANCED). */
+float vxo_biome_noise(int32_t seed, float freq, float x, float z);
+void vxo_biome_tint(const vxo_frame *f, uint32_t balance, float x, float z, float out[3]);
 float vxo_terrain_shade(const float *result, const vxo_frame *f, const float p[3], float d);
 
 /* The caller's atlas (RGBA8 sRGB: mip level 0 of every layer, then level 1, ...; level l

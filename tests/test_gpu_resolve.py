@@ -23,7 +23,8 @@ pytestmark = pytest.mark.gpu
 SKY_TINT_BIT, LIGHT_RGB_BIT, TEX_VARIATION_BIT, TINT_BIT = 2147483648, 268435456, 16384, 128
 SKY_SPECULAR, EMITTER, SHORE_WET, SKY_LOOK, SKY_COOL, LEGACY_LIGHTING = 1, 4, 8, 1024, 2048, 1048576
 PROBES = 16777216 | 33554432 | 67108864 | 134217728 | 536870912 | 1073741824
-# the workload flags without what the resolve does not restate (probes, biome tint, distant shadows)
+# the workload flags without probes (not restated) and without the biome tint and distant
+# shadows, whose own tests set up their frame fields
 BASE_FLAGS = W.DEFAULT_FLAGS & ~(PROBES | TINT_BIT | 32768)
 
 
@@ -166,7 +167,7 @@ def test_unrestated_configurations_are_refused(small_world):
         r.resolve()
     r.set_resolve_assets(_atlas(), SIZE, LAYERS, MIPS, _faces())
     r.resolve()  # the base configuration is restated
-    cases = [("biome tint", lambda f: setattr(f, "flags", f.flags | TINT_BIT), TINT_BIT, 0),
+    cases = [("caustics", lambda f: None, 0, 64),
              ("probes", lambda f: None, 67108864, 0),
              ("soft shadows", lambda f: None, 0, 65536),
              ("cloud deck", lambda f: setattr(f, "cloud_cover", 0.4), 0, 0)]

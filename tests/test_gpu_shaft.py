@@ -1,13 +1,15 @@
-"""GPU parity of the distant terrain shadows: the shaft scan (vx_shaft_scan) and the resolve
-with FLAG_DISTANT_SHADOWS (terrain_shade_beyond on the sun term), against the oracle
-through the C ABI, bit for bit.
+"""GPU parity of the distant terrain shadows -- the shaft scan (vx_shaft_scan) and the
+resolve with FLAG_DISTANT_SHADOWS (terrain_shade_beyond on the sun term) -- and of the biome
+tint, against the oracle through the C ABI, bit for bit.
 
 - the scan of a height window of this build's generator, under two suns and two texel
   sizes, and of a spike on flat ground;
 - resolved views of the 256x256x128 world under a low sun with the window around the
   camera (opaque pixels, and water and glass pixels whose legs' sun terms take the same
   factor), over two frames; the oracle without the factor differs on some pixels;
-- a resolve with the flag before any upload reads the zeroed field.
+- a resolve with the flag before any upload reads the zeroed field;
+- views with the workload's flags less probes (biome tint and distant shadows both on),
+  with and without SPEC_TINT_BALANCE, at tint strengths 1 and 0.6.
 """
 import numpy as np
 import pytest
@@ -17,7 +19,7 @@ from tests.test_gpu_resolve import BASE_FLAGS, LAYERS, MIPS, SIZE, SKY_SPECULAR,
 from voxelcraft_menu_amd import _abi as A
 from voxelcraft_menu_amd import workloads as W
 from voxelcraft_menu_amd.renderer import Renderer
-from voxelcraft_menu_amd.world import ShaftField, build_view
+from voxelcraft_menu_amd.world import ShaftField, build_view, set_tint
 
 pytestmark = pytest.mark.gpu
 
@@ -129,3 +131,47 @@ def test_resolve_before_any_upload_reads_the_zeroed_field(small_world):
     ref = O.resolve(sc, v.frame_bytes, O.rspec_from(spec, hi, shaft=zero), atlas, SIZE, LAYERS, MIPS, faces, vis, sun)
     assert np.array_equal(r.read_color().view(np.uint32), ref.view(np.uint32))
     r.close()
+
+
+def test_resolve_with_tint_and_distant_shadows_matches_oracle(small_world):
+    wl = W.WORKLOADS[0]
+    probes = 16777216 | 33554432 | 67108864 | 134217728 | 536870912 | 1073741824
+    flags = W.DEFAULT_FLAGS & ~probes
+    assert flags & 128 and flags & DISTANT
+    cams = W.random_cameras(wl, 3, seed=43, resolutions=[(224, 128)], flags=flags)
+    for c in cams[1:]:
+        c.pitch = -0.5
+    atlas, faces = _atlas(11), _faces_perm()
+    for i, cam in enumerate(cams):
+        v = build_view(small_world, cam)
+        v.frame.ambient, v.frame.mip_bias = 0.05, -3.0
+        set_tint(v.frame, 7 + i, (1.0, 0.6, 1.0)[i])
+        sf = ShaftField()
+        sf.update(wl.gen, v.frame.cam_pos[0], v.frame.cam_pos[2])
+        sf.set_frame(v.frame)
+        spec, hi = W.spec_words(cam.flags, small_world)
+        hi = (hi | SKY_SPECULAR | (512 if i == 2 else 0)) & ~(64 | 8192 | 16384 | 65536)
+        r = Renderer(cam.width, cam.height)
+        r.sync_world(small_world)
+        r.set_resolve_assets(atlas, SIZE, LAYERS, MIPS, faces)
+        r.upload_shaft(sf.heights())
+        r.shaft_scan(v.frame)
+        r.set_view(v)
+        r.render(v.frame, spec, hi)
+        r.resolve()
+        r.sync()
+        print(f"[tint] view {i}", flush=True)
+        sc = O.Scene(small_world.pool(0), small_world.pool(1), small_world.pool(2), v.chunks, v.grid)
+        vis, sun, _, _ = O.render(sc, v.frame_bytes, O.spec_from(spec, hi),
+                                  np.zeros(v.tiles[0] * v.tiles[1], np.uint32))
+        field = O.shaft_scan(sf.heights(), v.frame_bytes)
+        ref = O.resolve(sc, v.frame_bytes, O.rspec_from(spec, hi, shaft=field), atlas, SIZE, LAYERS, MIPS, faces, vis,
+                        sun)
+        got = r.read_color()
+        diff = got.view(np.uint32) != ref.view(np.uint32)
+        assert not diff.any(), f"view {i}: {diff.any(axis=-1).sum()} pixels differ"
+        untinted = O.resolve(sc, v.frame_bytes, O.rspec_from(spec & ~128, hi, shaft=field), atlas, SIZE, LAYERS, MIPS,
+                             faces, vis, sun)
+        assert (untinted.view(np.uint32) != ref.view(np.uint32)).any()  # the tint changed the image
+        r.close()
+        sf.close()

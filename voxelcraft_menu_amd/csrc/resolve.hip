@@ -25,7 +25,7 @@ constexpr uint32_t PERM_NONE = 0u, PERM_D4 = 2u;
 constexpr uint32_t RS_SKY_SPECULAR = 1u, RS_LIGHTING_REPAIR = 2u, RS_SKY_TINT = 4u, RS_SKY_COOL = 8u, RS_SKY_LOOK = 16u,
                    RS_LIGHT_RGB = 32u, RS_EMITTER_GATHER = 64u, RS_SHORE_WET = 128u, RS_TEX_VARIATION = 256u,
                    RS_GLASS = 512u, RS_WATER_SEC = 1024u, RS_SHORE_FOAM = 2048u, RS_FLAT_SECONDARY = 4096u,
-                   RS_DISTANT = 8192u;
+                   RS_DISTANT = 8192u, RS_TINT = 16384u, RS_TINT_BALANCE = 32768u;
 
 struct ResolveK {
     Pools p;
@@ -50,6 +50,9 @@ struct ResolveK {
     // distant terrain shadows: the shaft field's result slice and the frame's window
     const float *__restrict__ shaft;
     float shaft_origin[2], shaft_texel, shaft_soft;
+    // the biome tint
+    float tint_strength, tint_freq_t, tint_freq_h;
+    int32_t tint_seed;
 };
 
 // ---- elementary functions (the oracle's vxo_log2 / vxo_exp2 / vxo_pow) ----
@@ -365,6 +368,108 @@ __device__ __forceinline__ uint32_t perm_key(const float b[3], uint32_t nid) {
                     (nid * 0x27d4eb2fu));
 }
 
+// ---- biome tint (resolve.wgsl:166-287) ----
+constexpr uint32_t FLAG_TINT = 128u, BIOME_PRIME_X = 501125321u, BIOME_PRIME_Y = 1136930381u;
+constexpr int32_t BIOME_TEMP_SEED = 7, BIOME_HUMID_SEED = 8;
+constexpr float BIOME_BAND = 0.22f, BIOME_BLEND = 0.18f;
+constexpr float GRAD24_FIRST = 1.4398966328953218f, GRAD24_STEP = 0.26179938779914946f;
+constexpr float GRAD8_FIRST = 1.1780972450961724f, GRAD8_STEP = 0.7853981633974483f;
+
+__device__ __forceinline__ float r_sin(float x) {  // the oracle's vxo_wave_sin
+    const float kf = rintf(x * 0x1.45f306p-1f);
+    const float r = ((x - kf * 0x1.922p0f) - kf * -0x1.2aep-18f) - kf * -0x1.de973ep-31f;
+    const float r2 = r * r;
+    float cs = 0x1.a01a02p-16f;
+    cs = cs * r2 - 0x1.6c16c2p-10f;
+    cs = cs * r2 + 0x1.555556p-5f;
+    cs = cs * r2 - 0.5f;
+    cs = cs * r2 + 1.0f;
+    float sn = 0x1.71de3ap-19f;
+    sn = sn * r2 - 0x1.a01a02p-13f;
+    sn = sn * r2 + 0x1.111112p-7f;
+    sn = sn * r2 - 0x1.555556p-3f;
+    sn = sn * r2 + 1.0f;
+    sn = sn * r;
+    const int32_t q = (int32_t)kf & 3;
+    return q == 0 ? sn : (q == 1 ? cs : (q == 2 ? -sn : -cs));
+}
+
+// simplex_grad (resolve.wgsl:180-192)
+__device__ __forceinline__ float simplex_grad(uint32_t seed, uint32_t xp, uint32_t yp, float xd, float yd) {
+    const int32_t hs = (int32_t)((seed ^ xp ^ yp) * 0x27d4eb2du);
+    const uint32_t idx = (uint32_t)((hs ^ (hs >> 15)) & 254);
+    const float angle = idx < 240u ? GRAD24_FIRST - GRAD24_STEP * (float)((idx % 48u) >> 1)
+                                   : GRAD8_FIRST - GRAD8_STEP * (float)((idx - 240u) >> 1);
+    return xd * wave_cos(angle) + yd * r_sin(angle);
+}
+
+// biome_fast_floor (resolve.wgsl:194-199)
+__device__ __forceinline__ int32_t biome_fast_floor(float f) { return f >= 0.0f ? (int32_t)f : (int32_t)f - 1; }
+
+// biome_noise (resolve.wgsl:201-257): 2D OpenSimplex2, three corner kernels.
+__device__ inline float biome_noise(int32_t seed_i, float freq, float px, float pz) {
+    const uint32_t seed = (uint32_t)seed_i;
+    const float sqrt3 = 1.7320508075688772f;
+    const float g2 = (3.0f - sqrt3) / 6.0f;
+    const float f2 = 0.5f * (sqrt3 - 1.0f);
+    float x = px * freq, y = pz * freq;
+    const float s = (x + y) * f2;
+    x = x + s;
+    y = y + s;
+    const int32_t i0 = biome_fast_floor(x), j0 = biome_fast_floor(y);
+    const float xi = x - (float)i0, yi = y - (float)j0;
+    const float t = (xi + yi) * g2;
+    const float x0 = xi - t, y0 = yi - t;
+    const uint32_t i = (uint32_t)i0 * BIOME_PRIME_X, j = (uint32_t)j0 * BIOME_PRIME_Y;
+    const float a = 0.5f - x0 * x0 - y0 * y0;
+    float n0 = 0.0f;
+    if (a > 0.0f) n0 = (a * a) * (a * a) * simplex_grad(seed, i, j, x0, y0);
+    const float c = (2.0f * (1.0f - 2.0f * g2) * (1.0f / g2 - 2.0f)) * t +
+                    ((-2.0f * (1.0f - 2.0f * g2) * (1.0f - 2.0f * g2)) + a);
+    float n2 = 0.0f;
+    if (c > 0.0f) {
+        const float x2 = x0 + (2.0f * g2 - 1.0f), y2 = y0 + (2.0f * g2 - 1.0f);
+        n2 = (c * c) * (c * c) * simplex_grad(seed, i + BIOME_PRIME_X, j + BIOME_PRIME_Y, x2, y2);
+    }
+    const bool upper = y0 > x0;
+    const float x1 = upper ? x0 + g2 : x0 + (g2 - 1.0f), y1 = upper ? y0 + (g2 - 1.0f) : y0 + g2;
+    const float b = 0.5f - x1 * x1 - y1 * y1;
+    float n1 = 0.0f;
+    if (b > 0.0f)
+        n1 = (b * b) * (b * b) * simplex_grad(seed, upper ? i : i + BIOME_PRIME_X, upper ? j + BIOME_PRIME_Y : j, x1, y1);
+    return (n0 + n1 + n2) * 99.83685446303647f;
+}
+
+// biome_band_weights and biome_tint (resolve.wgsl:259-287)
+__device__ inline void biome_tint(const ResolveK &k, float x, float z, float out[3]) {
+    constexpr float PLAINS[3] = {1.000f, 1.000f, 1.000f}, PLAINS_BALANCED[3] = {0.935f, 1.067f, 0.892f};
+    constexpr float FOREST[3] = {0.664f, 1.045f, 1.000f}, SAVANNA[3] = {1.842f, 0.935f, 0.793f};
+    constexpr float DESERT[3] = {2.096f, 0.957f, 0.699f}, TAIGA[3] = {0.832f, 0.935f, 2.096f};
+    constexpr float TUNDRA[3] = {0.755f, 0.893f, 2.812f};
+    const float temp = biome_noise(k.tint_seed + BIOME_TEMP_SEED, k.tint_freq_t, x, z);
+    const float humid = biome_noise(k.tint_seed + BIOME_HUMID_SEED, k.tint_freq_h, x, z);
+    float wt[3], wh[3];
+    const float n2[2] = {temp, humid};
+    float *w2[2] = {wt, wh};
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+        const float hot = wgsl_smoothstep(BIOME_BAND - BIOME_BLEND, BIOME_BAND + BIOME_BLEND, n2[u]);
+        const float cold = 1.0f - wgsl_smoothstep(-BIOME_BAND - BIOME_BLEND, -BIOME_BAND + BIOME_BLEND, n2[u]);
+        w2[u][0] = cold;
+        w2[u][1] = 1.0f - cold - hot;
+        w2[u][2] = hot;
+    }
+    const bool balance = (k.bits & RS_TINT_BALANCE) != 0u;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const float pl = balance ? PLAINS_BALANCED[q] : PLAINS[q];
+        const float dry = wt[0] * TUNDRA[q] + wt[1] * pl + wt[2] * DESERT[q];
+        const float mid = wt[0] * TAIGA[q] + wt[1] * pl + wt[2] * SAVANNA[q];
+        const float wet = wt[0] * TAIGA[q] + wt[1] * FOREST[q] + wt[2] * FOREST[q];
+        out[q] = wh[0] * dry + wh[1] * mid + wh[2] * wet;
+    }
+}
+
 // terrain_shade_beyond (common.wgsl:1571-1580): the scanned horizon at p + sun_dir * d
 // (texel centres at origin + (i + 0.5) texel), ramped over +-shaft_soft around its y.
 __device__ inline float terrain_shade_beyond(const ResolveK &k, const float p[3], float d) {
@@ -595,6 +700,14 @@ __device__ inline void shade_opaque(const ResolveK &k, uint32_t ci, uint32_t vx,
     float texel[4];
     atlas_sample(k, face_word & 0xFFFFu, uv, lod, texel);
     float albedo[3] = {texel[0], texel[1], texel[2]};
+    // the biome tint, weighted by the texel's alpha (resolve.wgsl:337-343; faces only)
+    if ((k.bits & RS_TINT) && (f.flags & FLAG_TINT) && texel[3] > 0.0f) {
+        const float kt = texel[3] * k.tint_strength;
+        float tint[3];
+        biome_tint(k, hit[0], hit[2], tint);
+#pragma unroll
+        for (int q = 0; q < 3; q++) albedo[q] = albedo[q] * r_mix(1.0f, tint[q], kt);
+    }
     if ((k.bits & RS_SHORE_WET) && id == SAND_ID) {
         const float wet = 1.0f - wgsl_smoothstep(f.sea_level, f.sea_level + WET_BAND, hit[1]);
         const float m = r_mix(1.0f, WET_DARK, wet);
@@ -622,24 +735,6 @@ constexpr float WATER_F0 = 0.02f, GLASS_F0 = 0.0426f, GLINT_POWER = 320.0f, GLIN
                 FOAM_LO = 0.45f, FOAM_HI = 0.75f, FOAM_MAX = 0.85f, FOAM_RATE = 0.55f;
 constexpr uint32_t WATER_SEC_SPAN = 2u;
 
-__device__ __forceinline__ float r_sin(float x) {  // the oracle's vxo_wave_sin
-    const float kf = rintf(x * 0x1.45f306p-1f);
-    const float r = ((x - kf * 0x1.922p0f) - kf * -0x1.2aep-18f) - kf * -0x1.de973ep-31f;
-    const float r2 = r * r;
-    float cs = 0x1.a01a02p-16f;
-    cs = cs * r2 - 0x1.6c16c2p-10f;
-    cs = cs * r2 + 0x1.555556p-5f;
-    cs = cs * r2 - 0.5f;
-    cs = cs * r2 + 1.0f;
-    float sn = 0x1.71de3ap-19f;
-    sn = sn * r2 - 0x1.a01a02p-13f;
-    sn = sn * r2 + 0x1.111112p-7f;
-    sn = sn * r2 - 0x1.555556p-3f;
-    sn = sn * r2 + 1.0f;
-    sn = sn * r;
-    const int32_t q = (int32_t)kf & 3;
-    return q == 0 ? sn : (q == 1 ? cs : (q == 2 ? -sn : -cs));
-}
 
 // face_uv (resolve.wgsl:129-147)
 __device__ __forceinline__ void face_uv_of(uint32_t nid, const float local[3], float uv[2]) {
@@ -1363,7 +1458,6 @@ int resolve_bits(uint32_t spec, uint32_t spec_hi, const vx_gpu_frame &g, uint32_
     }
     constexpr uint32_t PROBES = 16777216u | 33554432u | 67108864u | 134217728u | 536870912u | 1073741824u;
     if ((g.flags & VX_FLAG_UNDERWATER) || (spec & PROBES) || (spec & VX_FLAG_FOLIAGE)) return VX_E_UNSUPPORTED;
-    if ((spec & 128u) && (g.flags & 128u)) return VX_E_UNSUPPORTED;        // biome tint
     if (spec_hi & (64u | 8192u | 16384u | 65536u)) return VX_E_UNSUPPORTED;  // caustics, foliage rich, canopy relief,
                                                                               // soft shadows
     if (spec_hi & 4096u) return VX_E_UNSUPPORTED;                           // the water look (ripples, murk)
@@ -1385,6 +1479,8 @@ int resolve_bits(uint32_t spec, uint32_t spec_hi, const vx_gpu_frame &g, uint32_
     if (spec_hi & 16u) bits |= RS_SHORE_FOAM;
     if (spec & 1048576u) bits |= RS_FLAT_SECONDARY;
     if (spec & VX_FLAG_DISTANT_SHADOWS) bits |= RS_DISTANT;
+    if (spec & FLAG_TINT) bits |= RS_TINT;
+    if (spec_hi & 512u) bits |= RS_TINT_BALANCE;
     return VX_OK;
 }
 
@@ -1452,6 +1548,10 @@ int vx_resolve(vx_ctx *c) {
     k.shaft_origin[1] = g.shaft_origin[1];
     k.shaft_texel = g.shaft_texel;
     k.shaft_soft = g.shaft_soft;
+    k.tint_strength = g.tint_strength;
+    k.tint_freq_t = g.tint_freq_t;
+    k.tint_freq_h = g.tint_freq_h;
+    k.tint_seed = g.tint_seed;
     if ((bits & RS_DISTANT) && (g.flags & VX_FLAG_DISTANT_SHADOWS)) {
         if ((rc = shaft_buffer(c))) return rc;  // never uploaded: the zeroed field
         k.shaft = (const float *)c->shaft.p + SHAFT_SLICE;

@@ -168,6 +168,18 @@ class GenParams:
         return out
 
 
+TINT_TEMP_FREQ, TINT_HUMID_FREQ = 0.00055, 0.0007  # biome::TEMP_FREQ / HUMID_FREQ (src/biome.rs:59-60)
+
+
+def set_tint(frame: A.Frame, seed: int, strength: float = 1.0):
+    """The frame's biome-tint fields as Renderer::render packs them (src/render/mod.rs:
+    1686-1689): strength clamped to [0, 1], the world seed, the biome frequencies."""
+    frame.tint_strength = min(max(float(strength), 0.0), 1.0)
+    frame.tint_seed = int(seed)
+    frame.tint_freq_t = TINT_TEMP_FREQ
+    frame.tint_freq_h = TINT_HUMID_FREQ
+
+
 class ShaftField:
     """shaft::ShaftField (src/shaft.rs): the 512 x 512 window of column heights, one texel
     per texel x texel blocks, centred on the camera, that the distant terrain shadows scan
