@@ -424,18 +424,24 @@ def run_moving(args, world, cams, specs, tuned, px):
 
 def run_resolve(r1, world, wl1, args):
     """The resolve (vx_resolve: lanes 0, 1 and 2) of the 1080p view, then TAA: the workload's
-    flags less what the resolve does not restate (probes): the biome tint (world seed,
-    strength 1), distant terrain shadows from the shaft window around the camera (its
-    eight scan passes timed apart), sky tint, sky specular and fog on, a synthetic
-    27-layer 16x16 atlas with its mip chain. The march runs once; the resolve alone is
-    timed over the steps."""
+    flags (all of them): the probe cube, bounce and sun over the probe field baked for the
+    chunks within the upload window, the biome tint (world seed, strength 1), distant
+    terrain shadows from the shaft window around the camera (its eight scan passes timed
+    apart), sky tint, sky specular and fog on, a synthetic 27-layer 16x16 atlas with its
+    mip chain. The march runs once; the resolve alone is timed over the steps."""
     from voxelcraft_menu_amd import workloads as W
-    from voxelcraft_menu_amd.world import ShaftField, build_view, set_tint
+    from voxelcraft_menu_amd.world import ShaftField, build_view, probe_bake, probe_uploadable, set_tint
 
-    probes = 16777216 | 33554432 | 67108864 | 134217728 | 536870912 | 1073741824
-    cam = W.default_camera(wl1, flags=W.DEFAULT_FLAGS & ~probes)
+    cam = W.default_camera(wl1, flags=W.DEFAULT_FLAGS)
     v = build_view(world, cam)
     set_tint(v.frame, wl1.gen.seed)
+    baked = 0
+    for key in world.keys():
+        origin = tuple(int(k) * 64 for k in key)
+        if probe_uploadable(origin, v.frame.cam_pos):
+            occl, bounce = probe_bake(wl1.gen, origin)
+            r1.write_probe(origin, occl, bounce)
+            baked += 1
     v.frame.ambient, v.frame.mip_bias, v.frame.fog_density, v.frame.fog_falloff = 0.05, 0.0, 0.002, 0.01
     v.frame.fog_scatter, v.frame.fog_g, v.frame.fog_height = 0.3, 0.6, 80.0
     sf = ShaftField()
@@ -485,9 +491,10 @@ def run_resolve(r1, world, wl1, args):
     res = {"ms_per_frame": round(dt / args.steps * 1e3, 4), "Gpix_per_s": round(px * args.steps / dt / 1e9, 3),
            "taa_ms_per_frame": round(dt_taa / args.steps * 1e3, 4),
            "shaft_scan_ms": round(dt_scan / args.steps * 1e3, 4),
-           "config": "1920x1080, workload flags less probes: biome tint (world seed, strength 1), distant "
-                     "shadows from the 512x512 shaft window (texel 4) around the camera; sky tint, sky specular "
-                     "and fog on; synthetic 16x16x27 atlas, 4 mips; TAA with feedback 0.1"}
+           "config": f"1920x1080, the workload's flags: probe cube / bounce / sun over {baked} baked chunks, "
+                     "biome tint (world seed, strength 1), distant shadows from the 512x512 shaft window (texel 4) "
+                     "around the camera; sky tint, sky specular and fog on; synthetic 16x16x27 atlas, 4 mips; "
+                     "TAA with feedback 0.1"}
     _log(f"resolve 1080p: {res['ms_per_frame']} ms, taa {res['taa_ms_per_frame']} ms, "
          f"shaft scan {res['shaft_scan_ms']} ms")
     return res

@@ -24,6 +24,7 @@
  *                            (the device build's output format, vx_local_tree)
  *   vxw_flat_layout       <- (no counterpart) one pool block per chunk for vx_pool_set_flat
  *   vxw_shaft_*           <- shaft::ShaftField (src/shaft.rs:1-154)
+ *   vxw_probe_*           <- probe::bake and its helpers (src/probe.rs:1-379)
  *   vxw_journal_parse     <- EditJournal::load (src/journal.rs:391-475)
  *   vxw_journal_write     <- EditJournal::save (src/journal.rs:477-508)
  *   vxw_generate          <- a seeded terrain generator with the reference's
@@ -165,6 +166,27 @@ int32_t vxw_shaft_texel(const vxw_shaft *s);
 int vxw_shaft_dirty(const vxw_shaft *s);
 /* ShaftField::envelope_reference: the brute-force horizon height at texel (tx, tz). */
 float vxw_shaft_envelope(const vxw_shaft *s, const float sun[3], uint32_t tx, uint32_t tz);
+
+/* ---- ambient probes: the bake (src/probe.rs:1-379) ----
+ * A chunk's 16^3 probes (4 blocks apart, from origin + 2): per cube axis (-X, +X, -Y, +Y,
+ * -Z, +Z) the sky occlusion occl[axis * 4096 + (pz * 16 + py) * 16 + px] and the ground
+ * bounce colour bounce[(axis * 4096 + i) * 3 + c], from this generator's column heights
+ * and surface blocks (vxw_height, vxw_surface_block) along 16 azimuths to 192 blocks;
+ * bounce_shadow weights each ground sample by its own sky exposure. vx_probe_write
+ * uploads the result. */
+#define VXW_PROBES 4096u
+int vxw_probe_bake(const vxw_gen_params *gen, const int32_t origin[3], int bounce_shadow, float *occl,
+                   float *bounce);
+/* probe::cube_from_tangents: the occlusion cube of one probe from its 16 horizon tangents. */
+void vxw_probe_cube_from_tangents(const float tan_h[16], float out[6]);
+/* probe::bounce_of_uniform_ground: the bounce of ground of one albedo all round (6 x rgb). */
+void vxw_probe_bounce_of_uniform_ground(const float albedo[3], float out[18]);
+/* probe::uploadable: the chunk lies within UPLOAD_REACH (224 blocks) of the camera in xz. */
+int vxw_probe_uploadable(const int32_t origin[3], const float cam[3]);
+/* The block at the top of column (x, z) of height h (the terrain, or the sea surface over it). */
+uint16_t vxw_surface_block(const vxw_gen_params *p, int32_t x, int32_t z, int32_t h);
+/* block::ALBEDO[id][c] (src/block.rs:187-214). */
+float vxw_albedo(uint16_t id, int c);
 
 /* ---- per-view setup ---- */
 typedef struct vxw_camera {

@@ -26,6 +26,7 @@
  *   vx_resolve                the resolve pass, lane 0       (src/render/shaders/resolve.wgsl:1378-1466,
  *                                                            dispatched at src/render/mod.rs:1853-1878)
  *   vx_shaft_upload/scan      upload_shaft + the shaft scan  (src/render/mod.rs:1346-1352, :1726-1740)
+ *   vx_probe_fill/write       make_probe + write_probe       (src/render/mod.rs:1398-1445, :2248-2315)
  *   vx_present                the present blit               (src/render/shaders/blit.wgsl:18-93)
  *   vx_batch_*                no counterpart: many views per launch (the reference renders
  *                             one view per render() call, src/headless.rs:680-837)
@@ -281,7 +282,8 @@ int vx_march_stats_compute(const uint32_t *heat, size_t n, uint32_t chunks, vx_m
  * shaft_texel, shaft_soft, tint_*) are read as given. With FLAG_TINT the albedo takes the
  * biome tint (resolve.wgsl:166-287: OpenSimplex2 temperature and humidity at the hit's
  * x, z, band weights, the tint table) weighted by the texel's alpha and tint_strength.
- * Views that need what is not restated return VX_E_UNSUPPORTED: probes, caustics,
+ * Views that need what is not restated return VX_E_UNSUPPORTED: the probe tap and probe
+ * ambient (FLAG_PROBE_TAP, FLAG_PROBE_AMBIENT), caustics,
  * soft shadows, foliage-rich or canopy-relief albedo, the water look, a cloud deck
  * (cloud_cover > 0), terrain god-ray shafts (godray_strength > 0 with
  * FLAG_TERRAIN_SHAFTS), the underwater view.
@@ -303,6 +305,22 @@ int vx_resolve(vx_ctx *ctx);      /* asynchronous on the context stream */
  * (src/render/shaders/shaft.wgsl:1-24, encoded by Renderer::render when
  * FrameParams::shaft_scan is set, src/render/mod.rs:1726-1740) with the frame's sun_dir
  * and shaft_texel; vx_read_shaft copies the result (n = 512 * 512 floats). */
+/* The ambient probe field (make_probe / write_probe, src/render/mod.rs:1398-1445,
+ * :2240-2315): 128 x (6 x 128) x 128 RGBA16F texels -- x and z the probe lattice on a
+ * 512-block torus, y the six cube-axis slabs of 128 levels -- texel (x, y, z) at
+ * ((z * 768 + y) * 128 + x) * 4 halves, bounce rgb and occlusion in a. vx_probe_fill sets
+ * every texel to *value (NULL: 1.0, the field a context starts with) or, with noise, to
+ * make_probe's hashed values; a fill with a value pins the field (writes are then
+ * ignored, as probe_fill pins it). vx_probe_write stores a chunk's bake (vxw_probe_bake:
+ * occl 6 x 4096, bounce 6 x 4096 x 3) as f16 into its 16^3 block of each slab at
+ * ((x / 4) mod 128, axis * 128 + clamp(y / 4, 0, 112), (z / 4) mod 128). vx_read_probe
+ * copies the field (n = 128 * 768 * 128 * 4 halves). With FLAG_PROBE_CUBE or
+ * FLAG_PROBE_BOUNCE the resolve samples it at hit + n * 2 (probe_field, trilinear,
+ * repeating in x and z): the face shade takes its alpha, the floor light its rgb
+ * (FLAG_PROBE_BOUNCE) and, with FLAG_PROBE_SUN, a sun bounce of its rgb. */
+int vx_probe_fill(vx_ctx *ctx, const float *value, int noise);
+int vx_probe_write(vx_ctx *ctx, const int32_t origin[3], const float *occl, const float *bounce);
+int vx_read_probe(vx_ctx *ctx, uint16_t *out, size_t n);
 int vx_shaft_upload(vx_ctx *ctx, const float *heights, size_t n);
 int vx_shaft_scan(vx_ctx *ctx, const vx_gpu_frame *frame);
 int vx_read_shaft(vx_ctx *ctx, float *out, size_t n);

@@ -114,6 +114,8 @@ def _bind(L):
     L.vxo_shaft_scan.argtypes = [V, V, V]
     L.vxo_terrain_shade.restype = C.c_float
     L.vxo_terrain_shade.argtypes = [V, V, V, C.c_float]
+    L.vxo_probe_field.restype = None
+    L.vxo_probe_field.argtypes = [V, V, C.c_uint32, V]
     L.vxo_biome_noise.restype = C.c_float
     L.vxo_biome_noise.argtypes = [C.c_int32, C.c_float, C.c_float, C.c_float]
     L.vxo_biome_tint.restype = None
@@ -164,11 +166,19 @@ class RSpec(C.Structure):
                                           "light_rgb", "emitter_gather", "shore_wet", "tex_variation", "glass",
                                           "lanes", "water_sec", "shore_foam", "flat_secondary")] + \
         [("march", C.c_void_p), ("distant", C.c_uint32), ("shaft", C.c_void_p), ("tint", C.c_uint32),
-         ("tint_balance", C.c_uint32)]
+         ("tint_balance", C.c_uint32), ("probe_cube", C.c_uint32), ("probe_bounce", C.c_uint32),
+         ("probe_sun", C.c_uint32), ("probe_sun_high", C.c_uint32), ("probe", C.c_void_p)]
 
     def with_march(self, spec: "Spec") -> "RSpec":
         self._march = spec  # kept alive with the record
         self.march = C.cast(C.pointer(spec), C.c_void_p).value
+        return self
+
+    def with_probe(self, field: np.ndarray) -> "RSpec":
+        """The probe field (PROBE_SHAPE f16 bit patterns) the cube and bounce terms sample."""
+        self._probe = np.ascontiguousarray(field, dtype=np.uint16)
+        assert self._probe.shape == PROBE_SHAPE
+        self.probe = self._probe.ctypes.data
         return self
 
     def with_shaft(self, result: np.ndarray) -> "RSpec":
@@ -186,17 +196,36 @@ class ResolveAssets(C.Structure):
 
 
 SHAFT_DIM = 512  # src/shaft.rs:3
+PROBE_SHAPE = (128, 768, 128, 4)  # probe_tex (z, y, x, rgba), src/render/mod.rs:2240-2268
+PROBE_BITS = {"cube": 67108864, "bounce": 134217728, "sun": 536870912, "sun_high": 1073741824}
 SHAFT_SOFT = 4.0  # src/shaft.rs:9
 
 
-def rspec_from(flags: int, spec_hi: int, shaft: np.ndarray | None = None) -> RSpec:
+def rspec_from(flags: int, spec_hi: int, shaft: np.ndarray | None = None, probe: np.ndarray | None = None) -> RSpec:
     """The resolve's override values for a (flags, spec_hi) pipeline key (src/render/mod.rs:
-    2355-2689). A key with FLAG_DISTANT_SHADOWS needs the shaft field's scan result."""
+    2355-2689). A key with FLAG_DISTANT_SHADOWS needs the shaft field's scan result, one
+    with FLAG_PROBE_CUBE or FLAG_PROBE_BOUNCE the probe field."""
     distant = bool(flags & 32768)
     if distant and shaft is None:
         raise ValueError("distant terrain shadows read the shaft field: pass its scan result")
+    sampled = bool(flags & (PROBE_BITS["cube"] | PROBE_BITS["bounce"]))
+    if sampled and probe is None:
+        raise ValueError("the probe cube and bounce read the probe field: pass it")
     r = _rspec(flags, spec_hi)
+    for k, bit in PROBE_BITS.items():
+        setattr(r, f"probe_{k}", 1 if flags & bit else 0)
+    if sampled:
+        r.with_probe(probe)
     return r.with_shaft(shaft) if distant else r
+
+
+def probe_field(field: np.ndarray, p, nid: int) -> np.ndarray:
+    """probe_field (resolve.wgsl:106-118) of a PROBE_SHAPE field at world p: rgba."""
+    fl = np.ascontiguousarray(field, dtype=np.uint16)
+    q = np.asarray(p, dtype=np.float32)
+    out = np.zeros(4, np.float32)
+    lib().vxo_probe_field(_p(fl), _p(q), nid, _p(out))
+    return out
 
 
 def shaft_scan(heights: np.ndarray, frame_bytes: bytes) -> np.ndarray:

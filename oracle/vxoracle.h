@@ -253,6 +253,11 @@ typedef struct vxo_rspec {
     const float *shaft;
     /* SPEC_TINT (the biome tint where the frame has FLAG_TINT), SPEC_TINT_BALANCE */
     uint32_t tint, tint_balance;
+    /* SPEC_PROBE_CUBE, _BOUNCE, _SUN, _SUN_HIGH over the probe field `probe` (128 x 768 x
+     * 128 RGBA16F texels, texel (x, y, z) at ((z * 768 + y) * 128 + x) * 4; NULL when
+     * neither cube nor bounce is on) */
+    uint32_t probe_cube, probe_bounce, probe_sun, probe_sun_high;
+    const uint16_t *probe;
 } vxo_rspec;
 
 /* Distant terrain shadows (src/shaft.rs, shaft.wgsl:1-24, common.wgsl:1557-1580).
@@ -267,6 +272,8 @@ int vxo_shaft_scan(const float *heights, const vxo_frame *f, float *result);
  * 1.1.1's SingleOpenSimplex2, normalised to [-1, 1]) and biome_tint at world (x, z) with
  * the frame's tint_seed and tint frequencies (balance: TINT_PLAINS_BALANCED). */
 float vxo_biome_noise(int32_t seed, float freq, float x, float z);
+/* probe_field (resolve.wgsl:106-118) at world p for face normal nid: rgba. */
+void vxo_probe_field(const uint16_t *probe, const float p[3], uint32_t nid, float out[4]);
 void vxo_biome_tint(const vxo_frame *f, uint32_t balance, float x, float z, float out[3]);
 float vxo_terrain_shade(const float *result, const vxo_frame *f, const float p[3], float d);
 

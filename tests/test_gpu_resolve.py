@@ -147,9 +147,9 @@ def test_heatmap_frame_matches_oracle(small_world):
     r.sync()
     got = r.read_color()
     assert np.array_equal(r.read_heat(), heat)
-    # a heatmap frame shades nothing: the distant-shadow field is not read
-    ref = O.resolve(sc, v.frame_bytes, O.rspec_from(spec & ~32768, hi), atlas, SIZE, LAYERS, MIPS, faces, vis, sun,
-                    heat)
+    # a heatmap frame shades nothing: the distant-shadow and probe fields are not read
+    ref = O.resolve(sc, v.frame_bytes, O.rspec_from(spec & ~(32768 | 67108864 | 134217728), hi), atlas, SIZE, LAYERS,
+                    MIPS, faces, vis, sun, heat)
     assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
     r.close()
 
@@ -168,7 +168,8 @@ def test_unrestated_configurations_are_refused(small_world):
     r.set_resolve_assets(_atlas(), SIZE, LAYERS, MIPS, _faces())
     r.resolve()  # the base configuration is restated
     cases = [("caustics", lambda f: None, 0, 64),
-             ("probes", lambda f: None, 67108864, 0),
+             ("probe tap", lambda f: None, 16777216, 0),
+             ("probe ambient", lambda f: None, 33554432, 0),
              ("soft shadows", lambda f: None, 0, 65536),
              ("cloud deck", lambda f: setattr(f, "cloud_cover", 0.4), 0, 0)]
     for name, frame_edit, spec_add, hi_add in cases:

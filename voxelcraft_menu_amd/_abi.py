@@ -362,6 +362,12 @@ _sig_host = {
     "vxw_flat_layout": (C.c_int, [P, C.c_size_t, P, C.c_size_t, P, C.c_size_t, P, C.c_uint32, P, P, C.c_size_t,
                                   C.POINTER(C.c_size_t)]),
     "vxw_pack_camera_frame": (C.c_int, [C.POINTER(Camera), C.c_uint32, C.POINTER(Frame)]),
+    "vxw_probe_bake": (C.c_int, [C.POINTER(GenParams), P, C.c_int, P, P]),
+    "vxw_probe_cube_from_tangents": (None, [P, P]),
+    "vxw_probe_bounce_of_uniform_ground": (None, [P, P]),
+    "vxw_probe_uploadable": (C.c_int, [P, P]),
+    "vxw_surface_block": (C.c_uint16, [C.POINTER(GenParams), C.c_int32, C.c_int32, C.c_int32]),
+    "vxw_albedo": (C.c_float, [C.c_uint16, C.c_int]),
     "vxw_shaft_new": (P, [C.c_int32]),
     "vxw_shaft_free": (None, [P]),
     "vxw_shaft_update": (C.c_int, [P, C.POINTER(GenParams), C.c_float, C.c_float, C.c_float]),
@@ -388,6 +394,9 @@ _sig_march = {
     "vx_pool_read": (C.c_int, [P, C.c_int, P, C.c_size_t]),
     "vx_pool_set_flat": (C.c_int, [P, P, C.c_size_t]),
     "vx_taa": (C.c_int, [P]),
+    "vx_probe_fill": (C.c_int, [P, P, C.c_int]),
+    "vx_probe_write": (C.c_int, [P, P, P, P]),
+    "vx_read_probe": (C.c_int, [P, P, C.c_size_t]),
     "vx_shaft_upload": (C.c_int, [P, P, C.c_size_t]),
     "vx_shaft_scan": (C.c_int, [P, C.POINTER(Frame)]),
     "vx_read_shaft": (C.c_int, [P, P, C.c_size_t]),

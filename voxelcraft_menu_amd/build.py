@@ -20,7 +20,8 @@ INC = os.path.join(ROOT, "include")
 CSRC = os.path.join(HERE, "csrc")
 
 HOST_SOURCES = ["host/world.cpp", "host/worldgen.cpp", "host/view.cpp", "host/capi.cpp", "host/journal.cpp",
-                "host/flat.cpp", "host/lod.cpp", "host/flat_layout.cpp", "host/shaft.cpp"]
+                "host/flat.cpp", "host/lod.cpp", "host/flat_layout.cpp", "host/shaft.cpp",
+                "host/probe.cpp"]
 HIP_SOURCES = ["kernels.hip", "tiles.hip", "secondary.hip", "vxmarch.hip", "batch.hip", "serve.hip", "build.hip", "noalloc.hip",
                "resolve.hip", "dworld.hip"]
 HIP_EXTRA = []  # further .hip translation units are appended by later modules

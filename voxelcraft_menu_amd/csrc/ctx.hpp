@@ -98,6 +98,8 @@ struct vx_ctx {
     vxh::DevBuf hist[2], taa_f32;
     vxh::DevBuf sec_r, sec_m;  // the resolve's water_sec legs (2x2 blocks, f16 RGBA)
     vxh::DevBuf shaft;         // the distant-shadow field: scan slices 0 and 1, heights in slice 2
+    vxh::DevBuf probe;         // the ambient probe field: 128 x 768 x 128 RGBA16F
+    bool probe_pinned = false; // filled with a value: writes are ignored
     size_t hist_len = 0;
     uint32_t taa_last = 0;
     bool taa_valid = false;

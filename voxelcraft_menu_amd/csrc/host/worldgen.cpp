@@ -261,6 +261,16 @@ extern "C" int32_t vxw_height(const vxw_gen_params *p, int32_t x, int32_t z) {
     return g.height(x, z);
 }
 
+extern "C" uint16_t vxw_surface_block(const vxw_gen_params *p, int32_t x, int32_t z, int32_t h) {
+    if (!p) return 0;
+    vxw_gen_params q = *p;
+    q.cave_fraction = 0.0f;
+    Gen g(q);
+    (void)x;
+    (void)z;
+    return g.column_block(std::max(h, q.sea_level) - 1, h);  // the column top: the terrain or the sea surface
+}
+
 extern "C" int vxw_generate_dense(const vxw_gen_params *p, int32_t cx, int32_t cy, int32_t cz, uint16_t *out) {
     if (!p || !out) return VXW_E_ARG;
     Gen g(*p);

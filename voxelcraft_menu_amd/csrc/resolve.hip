@@ -25,7 +25,8 @@ constexpr uint32_t PERM_NONE = 0u, PERM_D4 = 2u;
 constexpr uint32_t RS_SKY_SPECULAR = 1u, RS_LIGHTING_REPAIR = 2u, RS_SKY_TINT = 4u, RS_SKY_COOL = 8u, RS_SKY_LOOK = 16u,
                    RS_LIGHT_RGB = 32u, RS_EMITTER_GATHER = 64u, RS_SHORE_WET = 128u, RS_TEX_VARIATION = 256u,
                    RS_GLASS = 512u, RS_WATER_SEC = 1024u, RS_SHORE_FOAM = 2048u, RS_FLAT_SECONDARY = 4096u,
-                   RS_DISTANT = 8192u, RS_TINT = 16384u, RS_TINT_BALANCE = 32768u;
+                   RS_DISTANT = 8192u, RS_TINT = 16384u, RS_TINT_BALANCE = 32768u, RS_PROBE_CUBE = 65536u,
+                   RS_PROBE_BOUNCE = 131072u, RS_PROBE_SUN = 262144u, RS_PROBE_SUN_HIGH = 524288u;
 
 struct ResolveK {
     Pools p;
@@ -53,6 +54,8 @@ struct ResolveK {
     // the biome tint
     float tint_strength, tint_freq_t, tint_freq_h;
     int32_t tint_seed;
+    // the ambient probe field (RGBA16F texels)
+    const ushort4 *__restrict__ probe;
 };
 
 // ---- elementary functions (the oracle's vxo_log2 / vxo_exp2 / vxo_pow) ----
@@ -470,6 +473,51 @@ __device__ inline void biome_tint(const ResolveK &k, float x, float z, float out
     }
 }
 
+// ---- the ambient probe field (resolve.wgsl:95-118, src/render/mod.rs:2240-2315) ----
+constexpr uint32_t PROBE_XZ = 128u, PROBE_Y = 128u, PROBE_SLABS = 6u, PROBE_H = PROBE_Y * PROBE_SLABS;
+constexpr size_t PROBE_TEXELS = (size_t)PROBE_XZ * PROBE_H * PROBE_XZ;
+
+__device__ __forceinline__ void probe_texel(const ushort4 *__restrict__ t, uint32_t x, uint32_t y, uint32_t z,
+                                            float out[4]) {
+    const ushort4 v = t[((size_t)z * PROBE_H + y) * PROBE_XZ + x];
+    out[0] = (float)__builtin_bit_cast(_Float16, v.x);
+    out[1] = (float)__builtin_bit_cast(_Float16, v.y);
+    out[2] = (float)__builtin_bit_cast(_Float16, v.z);
+    out[3] = (float)__builtin_bit_cast(_Float16, v.w);
+}
+
+// probe_field: the slab of the face's normal at p / PROBE_SPACING, y kept half a texel
+// inside the slab, sampled as the probe sampler does (linear; repeat in x and z, clamp to
+// edge in y): texel coordinates uvw * size - 0.5, eight texels, weights in f32.
+__device__ inline void probe_field(const ResolveK &k, const float p[3], uint32_t nid, float out[4]) {
+    const float tx = p[0] * (1.0f / 4.0f), ty = p[1] * (1.0f / 4.0f), tz = p[2] * (1.0f / 4.0f);
+    const float y = fminf(fmaxf(ty, 0.5f), (float)PROBE_Y - 0.5f);
+    const float u = tx * (1.0f / (float)PROBE_XZ);
+    const float v = ((float)nid * (float)PROBE_Y + y) * (1.0f / ((float)PROBE_Y * (float)PROBE_SLABS));
+    const float w = tz * (1.0f / (float)PROBE_XZ);
+    const float X = u * (float)PROBE_XZ - 0.5f, Y = v * (float)PROBE_H - 0.5f, Z = w * (float)PROBE_XZ - 0.5f;
+    const float fx0 = floorf(X), fy0 = floorf(Y), fz0 = floorf(Z);
+    const float ax = X - fx0, ay = Y - fy0, az = Z - fz0;
+    const int32_t ix = (int32_t)fx0, iy = (int32_t)fy0, iz = (int32_t)fz0;
+    const uint32_t x0 = (uint32_t)(((ix % 128) + 128) % 128), x1 = (x0 + 1u) & 127u;
+    const uint32_t z0 = (uint32_t)(((iz % 128) + 128) % 128), z1 = (z0 + 1u) & 127u;
+    const uint32_t y0 = (uint32_t)min(max(iy, 0), (int32_t)PROBE_H - 1);
+    const uint32_t y1 = (uint32_t)min(max(iy + 1, 0), (int32_t)PROBE_H - 1);
+    float c[8][4];
+    probe_texel(k.probe, x0, y0, z0, c[0]);
+    probe_texel(k.probe, x1, y0, z0, c[1]);
+    probe_texel(k.probe, x0, y1, z0, c[2]);
+    probe_texel(k.probe, x1, y1, z0, c[3]);
+    probe_texel(k.probe, x0, y0, z1, c[4]);
+    probe_texel(k.probe, x1, y0, z1, c[5]);
+    probe_texel(k.probe, x0, y1, z1, c[6]);
+    probe_texel(k.probe, x1, y1, z1, c[7]);
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        out[q] = r_mix(r_mix(r_mix(c[0][q], c[1][q], ax), r_mix(c[2][q], c[3][q], ax), ay),
+                       r_mix(r_mix(c[4][q], c[5][q], ax), r_mix(c[6][q], c[7][q], ax), ay), az);
+}
+
 // terrain_shade_beyond (common.wgsl:1571-1580): the scanned horizon at p + sun_dir * d
 // (texel centres at origin + (i + 0.5) texel), ramped over +-shaft_soft around its y.
 __device__ inline float terrain_shade_beyond(const ResolveK &k, const float p[3], float d) {
@@ -668,7 +716,17 @@ __device__ inline void shade_opaque(const ResolveK &k, uint32_t ci, uint32_t vx,
     }
     constexpr float FACE_SHADE[8] = {0.62f, 0.62f, 0.50f, 1.00f, 0.80f, 0.80f, 0.90f, 0.90f};
     const bool repair = (k.bits & RS_LIGHTING_REPAIR) != 0u;
-    const float shade = repair ? 1.0f : FACE_SHADE[nid];
+    // the probe field at hit + n * PROBE_NORMAL_BIAS (resolve.wgsl:552-576)
+    float probe[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+    if (k.bits & (RS_PROBE_CUBE | RS_PROBE_BOUNCE)) {
+        float pp[3];
+#pragma unroll
+        for (int q = 0; q < 3; q++) pp[q] = hit[q] + n[q] * 2.0f;
+        probe_field(k, pp, nid, probe);
+    }
+    float shade = repair ? 1.0f : FACE_SHADE[nid];
+    if (k.bits & RS_PROBE_CUBE) shade = shade * probe[3];
+    const float sun_gain = (k.bits & RS_PROBE_SUN_HIGH) ? 0.25f : 0.10f;
     const float sun_k = direct * (repair ? 0.78f : 0.52f);
     constexpr float SUN_RGB[3] = {1.0000f, 0.8469f, 0.5647f}, FLOOR_RGB[3] = {0.85f, 0.90f, 1.00f};
     float light[3], spec_rgb[3];
@@ -676,7 +734,9 @@ __device__ inline void shade_opaque(const ResolveK &k, uint32_t ci, uint32_t vx,
     for (int q = 0; q < 3; q++) {
         const float ambient_rgb = amb3[q] * 0.50f;
         const float sun_rgb = SUN_RGB[q] * sun_k;
-        const float floor_rgb = FLOOR_RGB[q] * k.ambient;
+        float floor_rgb = FLOOR_RGB[q] * k.ambient;
+        if (k.bits & RS_PROBE_BOUNCE) floor_rgb = floor_rgb * probe[q];
+        if (k.bits & RS_PROBE_SUN) floor_rgb = floor_rgb + FLOOR_RGB[q] * probe[q] * (sun_gain * sky_sm);
         spec_rgb[q] = spec_pre[q] * (sky_sm * SKY_SPEC_GAIN);
         light[q] = repair ? ambient_rgb * ao * shade + sun_rgb + floor_rgb : ((ambient_rgb + sun_rgb) * ao + floor_rgb) * shade;
     }
@@ -1450,14 +1510,57 @@ int shaft_buffer(vx_ctx *c) {
     return vxh::fill_u32(c->stream, c->shaft.p, 0u, (size_t)3 * SHAFT_SLICE);
 }
 
+// make_probe's fill (src/render/mod.rs:2270-2287): value, or the hashed noise.
+__global__ __launch_bounds__(256) void k_probe_fill(ushort4 *__restrict__ t, float value, int noise) {
+    const size_t i = (size_t)blockIdx.x * 256u + threadIdx.x;
+    if (i >= PROBE_TEXELS) return;
+    uint16_t h4[4];
+#pragma unroll
+    for (uint32_t c = 0; c < 4u; c++) {
+        float v = value;
+        if (noise) {
+            const uint32_t h = (uint32_t)i * 2654435761u + c * 0x9E3779B9u;
+            v = 0.5f + (float)(h >> 8) / (float)(1u << 25);
+        }
+        h4[c] = __builtin_bit_cast(uint16_t, (_Float16)v);
+    }
+    t[i] = make_ushort4(h4[0], h4[1], h4[2], h4[3]);
+}
+
+// write_probe (src/render/mod.rs:1398-1445): one chunk's 6 x 16^3 probes into its block of
+// each slab, bounce rgb and occlusion as f16.
+__global__ __launch_bounds__(256) void k_probe_write(ushort4 *__restrict__ t, const float *__restrict__ occl,
+                                                     const float *__restrict__ bounce, uint32_t bx, uint32_t by,
+                                                     uint32_t bz) {
+    const uint32_t j = blockIdx.x * 256u + threadIdx.x;  // axis * 4096 + probe
+    if (j >= 6u * 4096u) return;
+    const uint32_t axis = j >> 12, i = j & 4095u;
+    const uint32_t px = i & 15u, py = (i >> 4) & 15u, pz = i >> 8;
+    const uint32_t x = bx + px, y = axis * PROBE_Y + by + py, z = bz + pz;
+    auto h = [](float v) { return __builtin_bit_cast(uint16_t, (_Float16)v); };  // round to nearest even
+    t[((size_t)z * PROBE_H + y) * PROBE_XZ + x] =
+        make_ushort4(h(bounce[j * 3u]), h(bounce[j * 3u + 1u]), h(bounce[j * 3u + 2u]), h(occl[j]));
+}
+
+// The probe field, made filled with ones as the default make_probe makes it.
+int probe_buffer(vx_ctx *c) {
+    if (c->probe.p) return VX_OK;
+    int rc;
+    if ((rc = vxh::ensure(c->probe, PROBE_TEXELS * 8))) return rc;
+    k_probe_fill<<<(unsigned)((PROBE_TEXELS + 255) / 256), 256, 0, c->stream>>>((ushort4 *)c->probe.p, 1.0f, 0);
+    VX_LAUNCHED();
+    return VX_OK;
+}
+
 // The resolve switches of a pipeline key, or VX_E_UNSUPPORTED for what is not restated.
 int resolve_bits(uint32_t spec, uint32_t spec_hi, const vx_gpu_frame &g, uint32_t &bits) {
     if (g.flags & VX_FLAG_HEATMAP) {
         bits = 0;
         return VX_OK;
     }
-    constexpr uint32_t PROBES = 16777216u | 33554432u | 67108864u | 134217728u | 536870912u | 1073741824u;
-    if ((g.flags & VX_FLAG_UNDERWATER) || (spec & PROBES) || (spec & VX_FLAG_FOLIAGE)) return VX_E_UNSUPPORTED;
+    constexpr uint32_t PROBE_TAP = 16777216u, PROBE_AMBIENT = 33554432u;
+    if ((g.flags & VX_FLAG_UNDERWATER) || (spec & (PROBE_TAP | PROBE_AMBIENT)) || (spec & VX_FLAG_FOLIAGE))
+        return VX_E_UNSUPPORTED;
     if (spec_hi & (64u | 8192u | 16384u | 65536u)) return VX_E_UNSUPPORTED;  // caustics, foliage rich, canopy relief,
                                                                               // soft shadows
     if (spec_hi & 4096u) return VX_E_UNSUPPORTED;                           // the water look (ripples, murk)
@@ -1481,6 +1584,10 @@ int resolve_bits(uint32_t spec, uint32_t spec_hi, const vx_gpu_frame &g, uint32_
     if (spec & VX_FLAG_DISTANT_SHADOWS) bits |= RS_DISTANT;
     if (spec & FLAG_TINT) bits |= RS_TINT;
     if (spec_hi & 512u) bits |= RS_TINT_BALANCE;
+    if (spec & 67108864u) bits |= RS_PROBE_CUBE;
+    if (spec & 134217728u) bits |= RS_PROBE_BOUNCE;
+    if (spec & 536870912u) bits |= RS_PROBE_SUN;
+    if (spec & 1073741824u) bits |= RS_PROBE_SUN_HIGH;
     return VX_OK;
 }
 
@@ -1552,6 +1659,11 @@ int vx_resolve(vx_ctx *c) {
     k.tint_freq_t = g.tint_freq_t;
     k.tint_freq_h = g.tint_freq_h;
     k.tint_seed = g.tint_seed;
+    k.probe = nullptr;
+    if (bits & (RS_PROBE_CUBE | RS_PROBE_BOUNCE)) {
+        if ((rc = probe_buffer(c))) return rc;  // never filled: the field of ones
+        k.probe = (const ushort4 *)c->probe.p;
+    }
     if ((bits & RS_DISTANT) && (g.flags & VX_FLAG_DISTANT_SHADOWS)) {
         if ((rc = shaft_buffer(c))) return rc;  // never uploaded: the zeroed field
         k.shaft = (const float *)c->shaft.p + SHAFT_SLICE;
@@ -1576,6 +1688,53 @@ int vx_resolve(vx_ctx *c) {
     c->color_valid = true;
     c->image_valid = false;
     c->taa_valid = false;
+    return VX_OK;
+}
+
+int vx_probe_fill(vx_ctx *c, const float *value, int noise) {
+    if (!c) return VX_E_ARG;
+    HIPX(hipSetDevice(c->device));
+    int rc;
+    if ((rc = probe_buffer(c))) return rc;
+    k_probe_fill<<<(unsigned)((PROBE_TEXELS + 255) / 256), 256, 0, c->stream>>>((ushort4 *)c->probe.p,
+                                                                                value ? *value : 1.0f, noise ? 1 : 0);
+    VX_LAUNCHED();
+    c->probe_pinned = value != nullptr;
+    HIPX(hipStreamSynchronize(c->stream));
+    return VX_OK;
+}
+
+int vx_probe_write(vx_ctx *c, const int32_t origin[3], const float *occl, const float *bounce) {
+    if (!c || !origin || !occl || !bounce) return VX_E_ARG;
+    if ((origin[0] | origin[1] | origin[2]) & 63) return VX_E_ARG;  // a chunk origin
+    HIPX(hipSetDevice(c->device));
+    int rc;
+    if ((rc = probe_buffer(c))) return rc;
+    if (c->probe_pinned) return VX_OK;
+    constexpr size_t N = 6u * 4096u;
+    float *d = nullptr;
+    HIPX(hipMalloc(&d, N * 4 * 4));
+    HIPX(hipMemcpyAsync(d, occl, N * 4, hipMemcpyHostToDevice, c->stream));
+    HIPX(hipMemcpyAsync(d + N, bounce, N * 12, hipMemcpyHostToDevice, c->stream));
+    const int32_t sp = 4;
+    auto rem = [](int32_t a, int32_t m) { return ((a % m) + m) % m; };
+    const uint32_t bx = (uint32_t)rem(origin[0] / sp, (int32_t)PROBE_XZ);
+    const uint32_t bz = (uint32_t)rem(origin[2] / sp, (int32_t)PROBE_XZ);
+    const uint32_t by = (uint32_t)std::min(std::max(origin[1] / sp, 0), (int32_t)PROBE_Y - 16);
+    k_probe_write<<<(unsigned)(N / 256), 256, 0, c->stream>>>((ushort4 *)c->probe.p, d, d + N, bx, by, bz);
+    VX_LAUNCHED();
+    HIPX(hipStreamSynchronize(c->stream));
+    HIPX(hipFree(d));
+    return VX_OK;
+}
+
+int vx_read_probe(vx_ctx *c, uint16_t *out, size_t n) {
+    if (!c || !out || n != PROBE_TEXELS * 4) return VX_E_ARG;
+    HIPX(hipSetDevice(c->device));
+    int rc;
+    if ((rc = probe_buffer(c))) return rc;
+    HIPX(hipMemcpyAsync(out, c->probe.p, PROBE_TEXELS * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPX(hipStreamSynchronize(c->stream));
     return VX_OK;
 }
 

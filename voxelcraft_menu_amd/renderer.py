@@ -168,6 +168,26 @@ class Renderer:
         A.check(self._L.vx_read_color(self._c, A.ptr(out), out.size), "vx_read_color")
         return out
 
+    def fill_probe(self, value: float | None = None, noise: bool = False):
+        """make_probe's fill of the whole probe field: 1.0, a pinned value, or its noise."""
+        v = C.c_float(value) if value is not None else None
+        A.check(self._L.vx_probe_fill(self._c, C.byref(v) if v is not None else None, 1 if noise else 0),
+                "vx_probe_fill")
+
+    def write_probe(self, origin, occl: np.ndarray, bounce: np.ndarray):
+        """Renderer::write_probe: a chunk's bake (world.probe_bake) into the probe field."""
+        o = np.asarray(origin, dtype=np.int32)
+        oc = np.ascontiguousarray(occl, dtype=np.float32)
+        bo = np.ascontiguousarray(bounce, dtype=np.float32)
+        assert oc.size == 6 * 4096 and bo.size == 6 * 4096 * 3
+        A.check(self._L.vx_probe_write(self._c, A.ptr(o), A.ptr(oc), A.ptr(bo)), "vx_probe_write")
+
+    def read_probe(self) -> np.ndarray:
+        """(128, 768, 128, 4) f16 bit patterns of the probe field, [z, y, x, rgba]."""
+        out = np.zeros((128, 768, 128, 4), np.uint16)
+        A.check(self._L.vx_read_probe(self._c, A.ptr(out), out.size), "vx_read_probe")
+        return out
+
     def upload_shaft(self, heights: np.ndarray):
         """The shaft field's 512 x 512 column heights (Renderer::upload_shaft)."""
         h = np.ascontiguousarray(heights, dtype=np.float32).reshape(-1)

@@ -168,6 +168,48 @@ class GenParams:
         return out
 
 
+PROBE_AXES, PROBE_PER_AXIS = 6, 16  # probe::AXES, probe::PER_AXIS (src/probe.rs:6-12)
+
+
+def probe_bake(gen: GenParams, origin, bounce_shadow: bool = True):
+    """probe::bake of the chunk at `origin` with this generator's heights and surface
+    blocks: (occl (6, 16, 16, 16), bounce (6, 16, 16, 16, 3)), indexed [axis, pz, py, px]."""
+    o = np.asarray(origin, dtype=np.int32)
+    occl = np.zeros((PROBE_AXES, 16, 16, 16), np.float32)
+    bounce = np.zeros((PROBE_AXES, 16, 16, 16, 3), np.float32)
+    A.check(A.host().vxw_probe_bake(C.byref(gen.to_c()), A.ptr(o), 1 if bounce_shadow else 0, A.ptr(occl),
+                                    A.ptr(bounce)), "vxw_probe_bake")
+    return occl, bounce
+
+
+def probe_cube_from_tangents(tan_h) -> np.ndarray:
+    t = np.ascontiguousarray(tan_h, dtype=np.float32)
+    out = np.zeros(6, np.float32)
+    A.host().vxw_probe_cube_from_tangents(A.ptr(t), A.ptr(out))
+    return out
+
+
+def probe_bounce_of_uniform_ground(albedo) -> np.ndarray:
+    a = np.ascontiguousarray(albedo, dtype=np.float32)
+    out = np.zeros((6, 3), np.float32)
+    A.host().vxw_probe_bounce_of_uniform_ground(A.ptr(a), A.ptr(out))
+    return out
+
+
+def probe_uploadable(origin, cam) -> bool:
+    o = np.asarray(origin, dtype=np.int32)
+    c = np.asarray(cam, dtype=np.float32)
+    return bool(A.host().vxw_probe_uploadable(A.ptr(o), A.ptr(c)))
+
+
+def surface_block(gen: GenParams, x: int, z: int, h: int) -> int:
+    return int(A.host().vxw_surface_block(C.byref(gen.to_c()), x, z, h))
+
+
+def albedo(block_id: int) -> np.ndarray:
+    return np.array([A.host().vxw_albedo(block_id, c) for c in range(3)], np.float32)
+
+
 TINT_TEMP_FREQ, TINT_HUMID_FREQ = 0.00055, 0.0007  # biome::TEMP_FREQ / HUMID_FREQ (src/biome.rs:59-60)
 
 
