@@ -662,7 +662,7 @@ def run_device_edits(args):
     wl = W.WORKLOADS[4]
     world = W.make_world(wl)
     r = Renderer(64, 64)
-    dw = DeviceWorld(r, world, capacity_factor=8.0)
+    dw = DeviceWorld(r, world, capacity_factor=8.0)  # compacts itself when the batches fill it
     rng = np.random.default_rng(12)
     batches = [S.random_edits(rng, wl, 1000) for _ in range(12)]
     dev_ms, dev_gpu_ms, host_ms, rebuilt = [], [], [], []

@@ -641,8 +641,11 @@ int vx_build_local_device(vx_ctx *ctx, const uint16_t *dense, uint32_t n, void *
  * are not shared). Edits in chunks outside the table do nothing. A batch runs in passes
  * of up to 1024 chunks without waiting on the host; VX_E_OVERFLOW when a pass's blocks do
  * not fit in capacity_words (its chunks and those of later passes keep their old blocks;
- * chunks_rebuilt counts the placed ones). The space of replaced blocks is not reused:
- * re-create the device world to compact it.
+ * chunks_rebuilt counts the placed ones). Replaced blocks stay until a compaction:
+ * vx_dworld_compact rebuilds every non-empty chunk into a fresh buffer of the same
+ * capacity, packed (records keep every field but their pointers), and vx_dworld_edit
+ * compacts first when the free space is under 1.5 x the words in use after the last
+ * compaction.
  * One difference from set_block: a chunk whose last solid voxel is removed and which gets
  * solid voxels again within one batch keeps the box of every voxel that became solid in
  * the batch (set_block restarts the box at the first voxel after the chunk went empty).
@@ -655,12 +658,13 @@ typedef struct vx_dworld_stats {
     uint64_t used_bytes, capacity_bytes;
     uint32_t chunks_rebuilt, chunks;  /* last batch; table */
     float last_ms;                    /* GPU time of the last batch, decode to placement */
-    uint32_t pad;
+    uint32_t compactions;             /* compactions so far (vx_dworld_compact or at an edit) */
 } vx_dworld_stats;
 int vx_dworld_create(vx_ctx *ctx, const uint32_t *flat, size_t flat_words, size_t capacity_words,
                      const vx_gpu_chunk *table, const uint32_t *solid, uint32_t n, vx_dworld **out);
 int vx_dworld_destroy(vx_dworld *w);
 int vx_dworld_edit(vx_dworld *w, const vx_edit *edits, size_t n);
+int vx_dworld_compact(vx_dworld *w);
 int vx_dworld_chunks(vx_dworld *w, vx_gpu_chunk *out, uint32_t cap, uint32_t *n);
 int vx_dworld_extract(vx_dworld *w, int32_t cx, int32_t cy, int32_t cz, uint16_t *dense);
 int vx_dworld_stats_get(vx_dworld *w, vx_dworld_stats *out);

@@ -138,3 +138,55 @@ def test_device_edits_render_as_host_edits():
     assert np.array_equal(dw.chunks(), before)
     dw.close()
     r.close()
+
+
+def _render_matches(world, dw, r, cams, tag):
+    dev = _records_match(world, dw)
+    for i, cam in enumerate(cams):
+        v = build_view(world, cam)
+        vd = relocate_view(v, dev, dev)
+        spec, hi = W.spec_words(cam.flags, world)
+        sc = O.Scene(world.pool(0), world.pool(1), world.pool(2), v.chunks, v.grid)
+        hz = np.zeros(v.tiles[0] * v.tiles[1], np.uint32)
+        r.set_view(vd)
+        r.write_hiz(hz)
+        r.render(vd.frame, spec, hi)
+        r.sync()
+        vis, sh, hz, _ = O.render(sc, v.frame_bytes, O.spec_from(spec, hi), hz)
+        assert np.array_equal(r.read_vis(), vis), (tag, i)
+        assert np.array_equal(r.read_shadow(), sh), (tag, i)
+
+
+def test_compaction_keeps_the_world():
+    """A device world with 6x its initial words compacts itself during the batches (the
+    free space falls under 1.5x the words in use after the last compaction) and on
+    request; every compaction leaves the records, the dense ids and the rendered bits of
+    the host world."""
+    wl = W.WORKLOADS[0]
+    world = W.make_world(wl)
+    rng = np.random.default_rng(17)
+    res = (256, 160)
+    cams = W.random_cameras(wl, 2, seed=5, resolutions=[res])
+    r = Renderer(res[0], res[1], device=0)
+    r.sync_world(world)
+    dw = DeviceWorld(r, world, capacity_factor=6.0, min_words=0)
+    cap = dw.stats()["capacity_bytes"]
+    for batch in range(4):
+        edits = np.concatenate([S.random_edits(rng, wl, 400), _set_and_back(world, rng, 30)])
+        print(f"[dworld] compaction batch {batch}", flush=True)
+        world.apply_edits(edits)
+        dw.edit(edits)
+        st = dw.stats()
+        assert st["used_bytes"] <= cap
+        _render_matches(world, dw, r, cams, f"batch {batch}")
+    assert dw.stats()["compactions"] >= 1
+    used = dw.stats()["used_bytes"]
+    dw.compact()
+    st = dw.stats()
+    assert st["used_bytes"] < used and st["compactions"] >= 2
+    _render_matches(world, dw, r, cams, "compact")
+    for key in world.keys():
+        key = tuple(int(k) for k in key)
+        assert np.array_equal(dw.extract(key), world.extract_dense(key)), key
+    dw.close()
+    r.close()

@@ -183,7 +183,7 @@ VXW_MAX_LOD = 12
 class DWorldStats(C.Structure):  # vx_dworld_stats
     _fields_ = [("edits", C.c_uint64), ("effective_edits", C.c_uint64), ("used_bytes", C.c_uint64),
                 ("capacity_bytes", C.c_uint64), ("chunks_rebuilt", C.c_uint32), ("chunks", C.c_uint32),
-                ("last_ms", C.c_float), ("pad", C.c_uint32)]
+                ("last_ms", C.c_float), ("compactions", C.c_uint32)]
 
 
 class PoolMirror(C.Structure):  # vx_pool_mirror
@@ -403,6 +403,7 @@ _sig_march = {
     "vx_dworld_create": (C.c_int, [P, P, C.c_size_t, C.c_size_t, P, P, C.c_uint32, C.POINTER(P)]),
     "vx_dworld_destroy": (C.c_int, [P]),
     "vx_dworld_edit": (C.c_int, [P, P, C.c_size_t]),
+    "vx_dworld_compact": (C.c_int, [P]),
     "vx_dworld_chunks": (C.c_int, [P, P, C.c_uint32, C.POINTER(C.c_uint32)]),
     "vx_dworld_extract": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, P]),
     "vx_dworld_stats_get": (C.c_int, [P, C.POINTER(DWorldStats)]),

@@ -34,6 +34,27 @@ HIPCC_FLAGS = [
 ]
 
 
+def _include_closure(src: str) -> list:
+    """The file and every quoted header it includes, transitively (looked up beside the
+    including file, in the library's sources and in include/)."""
+    import re
+
+    seen, todo = [], [src]
+    while todo:
+        f = todo.pop()
+        if f in seen:
+            continue
+        seen.append(f)
+        with open(f) as fh:
+            for name in re.findall(r'^\s*#\s*include\s+"([^"]+)"', fh.read(), re.M):
+                for d in (os.path.dirname(f), CSRC, INC):
+                    cand = os.path.join(d, name)
+                    if os.path.exists(cand):
+                        todo.append(os.path.normpath(cand))
+                        break
+    return seen
+
+
 def source_digest(kind: str) -> str:
     """Digest of what a library is built from: its sources, every header they can include
     and the compile flags. The build writes it beside the library (<lib>.srchash); the
@@ -46,10 +67,16 @@ def source_digest(kind: str) -> str:
         flags = "g++ -std=c++17 -O2"
     elif kind == "tile_kernels":
         # What the tile-march kernels (single view and batched) are compiled from: their
-        # translation unit, the headers and the flags. Counter profiles of those kernels
-        # carry this digest, so edits elsewhere in the library leave them valid.
-        files = [os.path.join(CSRC, "tiles.hip")] + glob.glob(os.path.join(CSRC, "*.hpp"))
-        flags = " ".join(HIPCC_FLAGS)
+        # translation unit, the headers it includes (transitively) and the flags. Counter
+        # profiles of those kernels carry this digest, so edits elsewhere in the library
+        # leave them valid.
+        files = _include_closure(os.path.join(CSRC, "tiles.hip"))
+        h = hashlib.sha256(" ".join(HIPCC_FLAGS).encode())
+        for f in sorted(set(files)):
+            h.update(os.path.relpath(f, ROOT).encode())
+            with open(f, "rb") as fh:
+                h.update(fh.read())
+        return h.hexdigest()
     elif kind == "calib":
         return hashlib.sha256(" ".join(HIPCC_FLAGS).encode() +
                               open(os.path.join(CSRC, "calib.hip"), "rb").read()).hexdigest()

@@ -196,11 +196,13 @@ __global__ __launch_bounds__(DW_GROUP) void k_dw_offsets(const uint8_t *__restri
     }
 }
 
-// Writes each slot's block at its offset and the chunk's new record; one wave per slot.
+// Writes each slot's block at its offset in flat and the chunk's new record, reading the
+// old block's L1 nodes from old_flat (the same buffer for an edit, the old one for a
+// compaction); one wave per slot.
 __global__ __launch_bounds__(64) void k_dw_place(const uint8_t *__restrict__ recs, size_t slot_bytes,
                                                  const DwSlot *__restrict__ slots, const DwState *st,
-                                                 uint32_t *__restrict__ flat, vx_gpu_chunk *__restrict__ table,
-                                                 uint32_t *__restrict__ solid) {
+                                                 const uint32_t *old_flat, uint32_t *flat,
+                                                 vx_gpu_chunk *__restrict__ table, uint32_t *__restrict__ solid) {
     if (st->overflow) return;  // the pass did not fit: its chunks keep their old blocks
     const uint32_t s = blockIdx.x, lane = threadIdx.x;
     const DwSlot sl = slots[s];
@@ -247,7 +249,7 @@ __global__ __launch_bounds__(64) void k_dw_place(const uint8_t *__restrict__ rec
                 wl = 15u;
             } else {
                 const uint32_t oi = old_run + mask_below(old_mask, bit);
-                wl = flat[4u * oi + 3u] >> 28;
+                wl = old_flat[4u * oi + 3u] >> 28;
             }
             uint32_t *o = flat + sl.offset + 4u * i;
             o[0] = (uint32_t)l1[i].mask;
@@ -305,6 +307,7 @@ struct vx_dworld {
     uint32_t n = 0;
     std::unordered_map<uint64_t, uint32_t> index;  // chunk key -> table index
     size_t used_words = 0, cap_words = 0;
+    size_t live_words = 0;             // the used words right after the last compaction (or creation)
     DevBuf batch, dense, recs, slots;  // batch inputs and state, per-pass scratch, extract's slot
     uint8_t *h_stage = nullptr;        // pinned staging of the batch inputs and state
     size_t h_stage_cap = 0;
@@ -342,104 +345,27 @@ void release_all(vx_dworld *w) {
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-int vx_dworld_create(vx_ctx *c, const uint32_t *flat, size_t flat_words, size_t capacity_words,
-                     const vx_gpu_chunk *table, const uint32_t *solid, uint32_t n, vx_dworld **out) {
-    if (!c || !out || !flat || !table || !solid || n == 0 || flat_words % 32u || capacity_words < flat_words ||
-        capacity_words * 4 > 0xFFFFFF00ull)
-        return VX_E_ARG;
-    for (uint32_t i = 0; i < n; i++)
-        if (table[i].lod != 0u || table[i].voxel_size != 1.0f) return VX_E_ARG;  // LOD-0 chunks only
-    VX_HIPX(hipSetDevice(c->device));
-    vx_dworld *w = new (std::nothrow) vx_dworld();
-    if (!w) return VX_E_OOM;
-    w->ctx = c;
-    int rc;
-    if ((rc = ensure(c->flat, capacity_words * 4)) || (rc = ensure(w->table, (size_t)n * sizeof(vx_gpu_chunk))) ||
-        (rc = ensure(w->solid, (size_t)n * 4))) {
-        release_all(w);
-        delete w;
-        return rc;
-    }
-    if (hipEventCreate(&w->e0) != hipSuccess || hipEventCreate(&w->e1) != hipSuccess) {
-        release_all(w);
-        delete w;
-        return VX_E_HIP;
-    }
-    VX_HIPX(hipMemcpyAsync(c->flat.p, flat, flat_words * 4, hipMemcpyHostToDevice, c->stream));
-    VX_HIPX(hipMemcpyAsync(w->table.p, table, (size_t)n * sizeof(vx_gpu_chunk), hipMemcpyHostToDevice, c->stream));
-    VX_HIPX(hipMemcpyAsync(w->solid.p, solid, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
-    VX_HIPX(hipStreamSynchronize(c->stream));
-    w->n = n;
-    w->h_solid.assign(solid, solid + n);
-    for (uint32_t i = 0; i < n; i++)
-        w->index[key_of((int32_t)floorf(table[i].origin[0] / 64.0f), (int32_t)floorf(table[i].origin[1] / 64.0f),
-                        (int32_t)floorf(table[i].origin[2] / 64.0f))] = i;
-    w->used_words = flat_words;
-    w->cap_words = capacity_words;
-    publish(w);
-    *out = w;
-    return VX_OK;
-}
+struct DwEdit {
+    uint32_t slot, di;  // slot, dense index
+    uint16_t id;
+};
 
-int vx_dworld_destroy(vx_dworld *w) {
-    if (!w) return VX_E_ARG;
-    (void)hipSetDevice(w->ctx->device);
-    (void)hipStreamSynchronize(w->ctx->stream);
-    release_all(w);
-    w->ctx->flat_len = 0;  // the context reads its mirrors again
-    w->ctx->pool_gen++;
-    delete w;
-    return VX_OK;
-}
-
-int vx_dworld_edit(vx_dworld *w, const vx_edit *edits, size_t n_edits) {
-    if (!w || (n_edits && !edits)) return VX_E_ARG;
+// The passes of one rebuild: the chunks slot_chunk (with the edits es, sorted by slot and
+// voxel) decoded from old_flat (the context's pools), rebuilt and placed into dst from
+// `used` words on. One staging image of every pass's inputs (slots, voxel runs, words =
+// slot << 18 | dense index, ids) after the batch state, one copy, then the passes run
+// without the host; passes take at most DW_GROUP chunks (dense ids and records are
+// per-chunk scratch). out: the batch state after the passes; ms: their GPU time.
+int run_passes(vx_dworld *w, const std::vector<uint32_t> &slot_chunk, const std::vector<DwEdit> &es,
+               const uint32_t *old_flat, uint32_t *dst, size_t used, DwState &out, float &ms) {
     vx_ctx *c = w->ctx;
-    VX_HIPX(hipSetDevice(c->device));
-    // Group the edits by chunk (slots in order of first appearance), then by voxel, keeping
-    // each voxel's edits in their order; edits outside the world's chunks do nothing, as
-    // set_block on a chunk that is not resident does nothing.
-    struct E { uint32_t slot, di; uint16_t id; };
-    std::vector<E> es;
-    es.reserve(n_edits);
-    std::vector<uint32_t> slot_chunk;
-    std::unordered_map<uint32_t, uint32_t> slot_of;
-    for (size_t i = 0; i < n_edits; i++) {
-        const vx_edit &e = edits[i];
-        const int32_t cx = floordiv64(e.x), cy = floordiv64(e.y), cz = floordiv64(e.z);
-        auto it = w->index.find(key_of(cx, cy, cz));
-        if (it == w->index.end()) continue;
-        auto sit = slot_of.find(it->second);
-        uint32_t s;
-        if (sit == slot_of.end()) {
-            s = (uint32_t)slot_chunk.size();
-            slot_of.emplace(it->second, s);
-            slot_chunk.push_back(it->second);
-        } else {
-            s = sit->second;
-        }
-        const uint32_t lx = (uint32_t)(e.x - cx * 64), ly = (uint32_t)(e.y - cy * 64), lz = (uint32_t)(e.z - cz * 64);
-        es.push_back({s, lx | (lz << 6) | (ly << 12), e.id});
-    }
-    std::stable_sort(es.begin(), es.end(), [](const E &a, const E &b) {
-        return a.slot != b.slot ? a.slot < b.slot : a.di < b.di;
-    });
     const uint32_t n_slots = (uint32_t)slot_chunk.size();
-    w->st.edits = n_edits;
-    w->st.effective_edits = 0;
-    w->st.chunks_rebuilt = 0;
-    w->st.last_ms = 0.0f;
-    if (n_slots == 0) return VX_OK;
-
-    // One staging image of every pass's inputs (slots, voxel runs, words = slot << 18 |
-    // dense index, ids), after the batch state; one copy, then the passes run without the
-    // host. Passes take at most DW_GROUP chunks (dense ids and records are per-chunk scratch).
     struct Pass { uint32_t s0, ns, n_runs; size_t slots, runs, words, ids; };
     std::vector<Pass> passes;
     std::vector<uint8_t> stage(align16(sizeof(DwState)));
-    DwState st0{w->used_words, w->cap_words, 0ull, 0u, 0u};
+    DwState st0{used, w->cap_words, 0ull, 0u, 0u};
     std::memcpy(stage.data(), &st0, sizeof st0);
     size_t eb = 0;
     for (uint32_t s0 = 0; s0 < n_slots; s0 += DW_GROUP) {
@@ -508,9 +434,8 @@ int vx_dworld_edit(vx_dworld *w, const vx_edit *edits, size_t n_edits) {
         if ((rc = vxh::build_local_async(c, (const uint16_t *)w->dense.p, ps.ns, (uint8_t *)w->recs.p, SLOT_BYTES)))
             break;
         k_dw_offsets<<<1, DW_GROUP, 0, c->stream>>>((const uint8_t *)w->recs.p, SLOT_BYTES, ps.ns, slots, d_state);
-        k_dw_place<<<ps.ns, 64, 0, c->stream>>>((const uint8_t *)w->recs.p, SLOT_BYTES, slots, d_state,
-                                                (uint32_t *)c->flat.p, (vx_gpu_chunk *)w->table.p,
-                                                (uint32_t *)w->solid.p);
+        k_dw_place<<<ps.ns, 64, 0, c->stream>>>((const uint8_t *)w->recs.p, SLOT_BYTES, slots, d_state, old_flat,
+                                                dst, (vx_gpu_chunk *)w->table.p, (uint32_t *)w->solid.p);
         VX_LAUNCHED();
     }
     VX_HIPX(hipEventRecord(w->e1, c->stream));
@@ -518,10 +443,139 @@ int vx_dworld_edit(vx_dworld *w, const vx_edit *edits, size_t n_edits) {
     VX_HIPX(hipMemcpyAsync(w->h_solid.data(), w->solid.p, (size_t)w->n * 4, hipMemcpyDeviceToHost, c->stream));
     VX_HIPX(hipStreamSynchronize(c->stream));
     if (rc) return rc;
-    DwState st1;
-    std::memcpy(&st1, w->h_stage, sizeof st1);
-    float ms = 0.0f;
+    std::memcpy(&out, w->h_stage, sizeof out);
     VX_HIPX(hipEventElapsedTime(&ms, w->e0, w->e1));
+    return VX_OK;
+}
+
+// Rebuild every non-empty chunk into a fresh buffer of the same capacity, packed from word
+// 0 (the blocks of replaced and emptied chunks are dropped), then make it the pools.
+// Records keep every field but their pointers: the rebuild replays no edit, so each L1
+// cell keeps its water line and the box, dry mask and flags stay as they were.
+int compact(vx_dworld *w) {
+    vx_ctx *c = w->ctx;
+    std::vector<uint32_t> chunks;
+    for (uint32_t i = 0; i < w->n; i++)
+        if (w->h_solid[i]) chunks.push_back(i);
+    DevBuf nf;
+    int rc;
+    if ((rc = ensure(nf, w->cap_words * 4))) return rc;
+    DwState st;
+    float ms = 0.0f;
+    rc = run_passes(w, chunks, {}, (const uint32_t *)c->flat.p, (uint32_t *)nf.p, 0, st, ms);
+    if (rc == VX_OK && st.overflow) rc = VX_E_OVERFLOW;
+    if (rc) {
+        release(nf);
+        return rc;
+    }
+    std::swap(c->flat, nf);
+    release(nf);
+    w->used_words = w->live_words = (size_t)st.used;
+    w->st.compactions++;
+    w->st.used_bytes = w->used_words * 4;
+    publish(w);
+    return VX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vx_dworld_create(vx_ctx *c, const uint32_t *flat, size_t flat_words, size_t capacity_words,
+                     const vx_gpu_chunk *table, const uint32_t *solid, uint32_t n, vx_dworld **out) {
+    if (!c || !out || !flat || !table || !solid || n == 0 || flat_words % 32u || capacity_words < flat_words ||
+        capacity_words * 4 > 0xFFFFFF00ull)
+        return VX_E_ARG;
+    for (uint32_t i = 0; i < n; i++)
+        if (table[i].lod != 0u || table[i].voxel_size != 1.0f) return VX_E_ARG;  // LOD-0 chunks only
+    VX_HIPX(hipSetDevice(c->device));
+    vx_dworld *w = new (std::nothrow) vx_dworld();
+    if (!w) return VX_E_OOM;
+    w->ctx = c;
+    int rc;
+    if ((rc = ensure(c->flat, capacity_words * 4)) || (rc = ensure(w->table, (size_t)n * sizeof(vx_gpu_chunk))) ||
+        (rc = ensure(w->solid, (size_t)n * 4))) {
+        release_all(w);
+        delete w;
+        return rc;
+    }
+    if (hipEventCreate(&w->e0) != hipSuccess || hipEventCreate(&w->e1) != hipSuccess) {
+        release_all(w);
+        delete w;
+        return VX_E_HIP;
+    }
+    VX_HIPX(hipMemcpyAsync(c->flat.p, flat, flat_words * 4, hipMemcpyHostToDevice, c->stream));
+    VX_HIPX(hipMemcpyAsync(w->table.p, table, (size_t)n * sizeof(vx_gpu_chunk), hipMemcpyHostToDevice, c->stream));
+    VX_HIPX(hipMemcpyAsync(w->solid.p, solid, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+    VX_HIPX(hipStreamSynchronize(c->stream));
+    w->n = n;
+    w->h_solid.assign(solid, solid + n);
+    for (uint32_t i = 0; i < n; i++)
+        w->index[key_of((int32_t)floorf(table[i].origin[0] / 64.0f), (int32_t)floorf(table[i].origin[1] / 64.0f),
+                        (int32_t)floorf(table[i].origin[2] / 64.0f))] = i;
+    w->used_words = w->live_words = flat_words;
+    w->cap_words = capacity_words;
+    publish(w);
+    *out = w;
+    return VX_OK;
+}
+
+int vx_dworld_destroy(vx_dworld *w) {
+    if (!w) return VX_E_ARG;
+    (void)hipSetDevice(w->ctx->device);
+    (void)hipStreamSynchronize(w->ctx->stream);
+    release_all(w);
+    w->ctx->flat_len = 0;  // the context reads its mirrors again
+    w->ctx->pool_gen++;
+    delete w;
+    return VX_OK;
+}
+
+int vx_dworld_edit(vx_dworld *w, const vx_edit *edits, size_t n_edits) {
+    if (!w || (n_edits && !edits)) return VX_E_ARG;
+    vx_ctx *c = w->ctx;
+    VX_HIPX(hipSetDevice(c->device));
+    // Group the edits by chunk (slots in order of first appearance), then by voxel, keeping
+    // each voxel's edits in their order; edits outside the world's chunks do nothing, as
+    // set_block on a chunk that is not resident does nothing.
+    std::vector<DwEdit> es;
+    es.reserve(n_edits);
+    std::vector<uint32_t> slot_chunk;
+    std::unordered_map<uint32_t, uint32_t> slot_of;
+    for (size_t i = 0; i < n_edits; i++) {
+        const vx_edit &e = edits[i];
+        const int32_t cx = floordiv64(e.x), cy = floordiv64(e.y), cz = floordiv64(e.z);
+        auto it = w->index.find(key_of(cx, cy, cz));
+        if (it == w->index.end()) continue;
+        auto sit = slot_of.find(it->second);
+        uint32_t s;
+        if (sit == slot_of.end()) {
+            s = (uint32_t)slot_chunk.size();
+            slot_of.emplace(it->second, s);
+            slot_chunk.push_back(it->second);
+        } else {
+            s = sit->second;
+        }
+        const uint32_t lx = (uint32_t)(e.x - cx * 64), ly = (uint32_t)(e.y - cy * 64), lz = (uint32_t)(e.z - cz * 64);
+        es.push_back({s, lx | (lz << 6) | (ly << 12), e.id});
+    }
+    std::stable_sort(es.begin(), es.end(), [](const DwEdit &a, const DwEdit &b) {
+        return a.slot != b.slot ? a.slot < b.slot : a.di < b.di;
+    });
+    const uint32_t n_slots = (uint32_t)slot_chunk.size();
+    w->st.edits = n_edits;
+    w->st.effective_edits = 0;
+    w->st.chunks_rebuilt = 0;
+    w->st.last_ms = 0.0f;
+    if (n_slots == 0) return VX_OK;
+    int rc;
+    // Room for a batch that rebuilds the whole world half over: else compact first.
+    if (w->used_words + w->live_words + w->live_words / 2 > w->cap_words && (rc = compact(w))) return rc;
+    DwState st1;
+    float ms = 0.0f;
+    if ((rc = run_passes(w, slot_chunk, es, (const uint32_t *)c->flat.p, (uint32_t *)c->flat.p, w->used_words, st1,
+                         ms)))
+        return rc;
     w->st.last_ms = ms;
     w->st.effective_edits = st1.effective;
     w->st.chunks_rebuilt = st1.placed;  // a pass that did not fit stops the rest
@@ -530,6 +584,12 @@ int vx_dworld_edit(vx_dworld *w, const vx_edit *edits, size_t n_edits) {
     w->st.capacity_bytes = w->cap_words * 4;
     publish(w);
     return st1.overflow ? VX_E_OVERFLOW : VX_OK;
+}
+
+int vx_dworld_compact(vx_dworld *w) {
+    if (!w) return VX_E_ARG;
+    VX_HIPX(hipSetDevice(w->ctx->device));
+    return compact(w);
 }
 
 int vx_dworld_chunks(vx_dworld *w, vx_gpu_chunk *out, uint32_t cap, uint32_t *n_out) {

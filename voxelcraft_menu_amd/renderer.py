@@ -579,7 +579,7 @@ class DeviceWorld:
     The renderer's context reads its pools from it while it lives; views use the chunk
     records of `chunks()`."""
 
-    def __init__(self, renderer: "Renderer", world: World, capacity_factor: float = 4.0):
+    def __init__(self, renderer: "Renderer", world: World, capacity_factor: float = 8.0, min_words: int = 1 << 22):
         from .world import ViewSnapshot, flat_layout
         self._L = A.march()
         self._r = renderer
@@ -587,7 +587,9 @@ class DeviceWorld:
         table, solid = snap.chunk_table_all()
         snap.close()
         flat, moved = flat_layout(world.pool(0), world.pool(1), world.pool(2), table)
-        cap = int(max(flat.size * capacity_factor, flat.size + (1 << 22)) + 31) // 32 * 32
+        # Edits that carve into full L1 cells expand their blocks (no run is shared), so a
+        # batch can need more words than the chunks it rebuilds had.
+        cap = int(max(flat.size * capacity_factor, flat.size + min_words) + 31) // 32 * 32
         self._w = C.c_void_p()
         A.check(self._L.vx_dworld_create(renderer._c, A.ptr(flat), flat.size, cap, A.ptr(moved), A.ptr(solid),
                                          moved.shape[0], C.byref(self._w)), "vx_dworld_create")
@@ -611,6 +613,10 @@ class DeviceWorld:
         e = np.ascontiguousarray(edits)
         A.check(self._L.vx_dworld_edit(self._w, A.ptr(e) if len(e) else None, len(e)), "vx_dworld_edit")
 
+    def compact(self):
+        """Rebuild every non-empty chunk into fresh, packed pools (vx_dworld_compact)."""
+        A.check(self._L.vx_dworld_compact(self._w), "vx_dworld_compact")
+
     def chunks(self) -> np.ndarray:
         """(n, 96) records of the non-empty chunks, in table order."""
         n = C.c_uint32()
@@ -628,4 +634,4 @@ class DeviceWorld:
     def stats(self) -> dict:
         st = A.DWorldStats()
         A.check(self._L.vx_dworld_stats_get(self._w, C.byref(st)), "vx_dworld_stats_get")
-        return {k: getattr(st, k) for k, _ in A.DWorldStats._fields_ if k != "pad"}
+        return {k: getattr(st, k) for k, _ in A.DWorldStats._fields_}
