@@ -282,8 +282,10 @@ int vx_march_stats_compute(const uint32_t *heat, size_t n, uint32_t chunks, vx_m
  * shaft_texel, shaft_soft, tint_*) are read as given. With FLAG_TINT the albedo takes the
  * biome tint (resolve.wgsl:166-287: OpenSimplex2 temperature and humidity at the hit's
  * x, z, band weights, the tint table) weighted by the texel's alpha and tint_strength.
+ * SPEC_CAUSTICS lifts the sun term of points up to 8 below the sea surface by the light
+ * sheets of the first two wave octaves (resolve.wgsl:530-549).
  * Views that need what is not restated return VX_E_UNSUPPORTED: the probe tap and probe
- * ambient (FLAG_PROBE_TAP, FLAG_PROBE_AMBIENT), caustics,
+ * ambient (FLAG_PROBE_TAP, FLAG_PROBE_AMBIENT),
  * soft shadows, foliage-rich or canopy-relief albedo, the water look, a cloud deck
  * (cloud_cover > 0), terrain god-ray shafts (godray_strength > 0 with
  * FLAG_TERRAIN_SHAFTS), the underwater view.

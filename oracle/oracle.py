@@ -167,7 +167,8 @@ class RSpec(C.Structure):
                                           "lanes", "water_sec", "shore_foam", "flat_secondary")] + \
         [("march", C.c_void_p), ("distant", C.c_uint32), ("shaft", C.c_void_p), ("tint", C.c_uint32),
          ("tint_balance", C.c_uint32), ("probe_cube", C.c_uint32), ("probe_bounce", C.c_uint32),
-         ("probe_sun", C.c_uint32), ("probe_sun_high", C.c_uint32), ("probe", C.c_void_p)]
+         ("probe_sun", C.c_uint32), ("probe_sun_high", C.c_uint32), ("probe", C.c_void_p),
+         ("caustics", C.c_uint32)]
 
     def with_march(self, spec: "Spec") -> "RSpec":
         self._march = spec  # kept alive with the record
@@ -269,7 +270,8 @@ def _rspec(flags: int, spec_hi: int) -> RSpec:
                  tex_variation=1 if flags & 16384 else 0, glass=1 if flags & 131072 else 0,
                  lanes=1, water_sec=1 if spec_hi & 32 else 0, shore_foam=1 if spec_hi & 16 else 0,
                  flat_secondary=1 if flags & 1048576 else 0, tint=1 if flags & 128 else 0,
-                 tint_balance=1 if spec_hi & 512 else 0).with_march(spec_from(flags, spec_hi))
+                 tint_balance=1 if spec_hi & 512 else 0, caustics=1 if spec_hi & 64 else 0).with_march(
+                     spec_from(flags, spec_hi))
 
 
 def resolve(scene: "Scene", frame_bytes: bytes, rspec: RSpec, atlas: np.ndarray, size: int, layers: int, mips: int,

@@ -258,6 +258,7 @@ typedef struct vxo_rspec {
      * neither cube nor bounce is on) */
     uint32_t probe_cube, probe_bounce, probe_sun, probe_sun_high;
     const uint16_t *probe;
+    uint32_t caustics;  /* SPEC_CAUSTICS */
 } vxo_rspec;
 
 /* Distant terrain shadows (src/shaft.rs, shaft.wgsl:1-24, common.wgsl:1557-1580).

@@ -167,7 +167,7 @@ def test_unrestated_configurations_are_refused(small_world):
         r.resolve()
     r.set_resolve_assets(_atlas(), SIZE, LAYERS, MIPS, _faces())
     r.resolve()  # the base configuration is restated
-    cases = [("caustics", lambda f: None, 0, 64),
+    cases = [("foliage-rich albedo", lambda f: None, 0, 8192),
              ("probe tap", lambda f: None, 16777216, 0),
              ("probe ambient", lambda f: None, 33554432, 0),
              ("soft shadows", lambda f: None, 0, 65536),

@@ -175,3 +175,38 @@ def test_resolve_with_tint_and_distant_shadows_matches_oracle(small_world):
         assert (untinted.view(np.uint32) != ref.view(np.uint32)).any()  # the tint changed the image
         r.close()
         sf.close()
+
+
+def test_resolve_with_caustics_matches_oracle(small_world):
+    """SPEC_CAUSTICS over the shallow sea floor of the 256x256x128 world, looking down."""
+    wl = W.WORKLOADS[0]
+    cams = W.random_cameras(wl, 3, seed=53, resolutions=[(224, 128)], flags=BASE_FLAGS)
+    for c in cams:
+        c.pitch = -0.9
+    atlas, faces = _atlas(17), _faces_perm()
+    changed = 0
+    for i, cam in enumerate(cams):
+        v = build_view(small_world, cam)
+        v.frame.ambient, v.frame.mip_bias = 0.05, -3.0
+        v.frame.wave_amp, v.frame.wave_scale, v.frame.wave_speed, v.frame.time = 0.4, 24.0, 0.3, 2.1
+        spec, hi = W.spec_words(cam.flags, small_world)
+        hi = (hi | SKY_SPECULAR | 64) & ~(8192 | 16384 | 65536)
+        r = Renderer(cam.width, cam.height)
+        r.sync_world(small_world)
+        r.set_resolve_assets(atlas, SIZE, LAYERS, MIPS, faces)
+        r.set_view(v)
+        r.render(v.frame, spec, hi)
+        r.resolve()
+        r.sync()
+        print(f"[caustics] view {i}", flush=True)
+        sc = O.Scene(small_world.pool(0), small_world.pool(1), small_world.pool(2), v.chunks, v.grid)
+        vis, sun, _, _ = O.render(sc, v.frame_bytes, O.spec_from(spec, hi),
+                                  np.zeros(v.tiles[0] * v.tiles[1], np.uint32))
+        ref = O.resolve(sc, v.frame_bytes, O.rspec_from(spec, hi), atlas, SIZE, LAYERS, MIPS, faces, vis, sun)
+        got = r.read_color()
+        diff = got.view(np.uint32) != ref.view(np.uint32)
+        assert not diff.any(), f"view {i}: {diff.any(axis=-1).sum()} pixels differ"
+        plain = O.resolve(sc, v.frame_bytes, O.rspec_from(spec, hi & ~64), atlas, SIZE, LAYERS, MIPS, faces, vis, sun)
+        changed += int((plain.view(np.uint32) != ref.view(np.uint32)).any(axis=-1).sum())
+        r.close()
+    assert changed > 0  # caustics lit some sea-floor pixels

@@ -26,7 +26,8 @@ constexpr uint32_t RS_SKY_SPECULAR = 1u, RS_LIGHTING_REPAIR = 2u, RS_SKY_TINT = 
                    RS_LIGHT_RGB = 32u, RS_EMITTER_GATHER = 64u, RS_SHORE_WET = 128u, RS_TEX_VARIATION = 256u,
                    RS_GLASS = 512u, RS_WATER_SEC = 1024u, RS_SHORE_FOAM = 2048u, RS_FLAT_SECONDARY = 4096u,
                    RS_DISTANT = 8192u, RS_TINT = 16384u, RS_TINT_BALANCE = 32768u, RS_PROBE_CUBE = 65536u,
-                   RS_PROBE_BOUNCE = 131072u, RS_PROBE_SUN = 262144u, RS_PROBE_SUN_HIGH = 524288u;
+                   RS_PROBE_BOUNCE = 131072u, RS_PROBE_SUN = 262144u, RS_PROBE_SUN_HIGH = 524288u,
+                   RS_CAUSTICS = 1048576u;
 
 struct ResolveK {
     Pools p;
@@ -713,6 +714,22 @@ __device__ inline void shade_opaque(const ResolveK &k, uint32_t ci, uint32_t vx,
                 lit = lit * terrain_shade_beyond(k, hit, shadow_dist);
         }
         direct = lit * ndl * sky_sm;
+        // caustics under the sea surface (resolve.wgsl:530-549): the two longest waves'
+        // slopes at the surface point above along the sun, squared into light sheets
+        const float cdepth = f.sea_level - hit[1];
+        if ((k.bits & RS_CAUSTICS) && cdepth > 0.0f && cdepth < 8.0f && direct > 0.0f) {
+            const float run = cdepth / fmaxf(f.sun[1], 1e-3f);
+            const float sp[2] = {hit[0] - f.sun[0] * run, hit[2] - f.sun[2] * run};
+            const float a = f.wave_amp, l0 = f.wave_scale, ph = f.time * f.wave_speed * WAVE_TAU;
+            const float zero[2] = {0.0f, 0.0f};
+            float g0[3], g1[3];
+            wave_octave(sp, 0.913089f, 0.407760f, l0, a * 0.42f, ph, 0.35f, zero, 0.0f, false, g0);
+            wave_octave(sp, -0.388685f, 0.921371f, l0 / 2.17f, a * 0.27f, ph * 1.4731f, 0.35f, zero, 0.0f, false,
+                        g1);
+            const float r0 = g0[0] * g0[0] + g0[1] * g0[1], r1 = g1[0] * g1[0] + g1[1] * g1[1];
+            const float sheets = r0 * r1 * 4000.0f;
+            direct = direct * (1.0f + sheets * wave_exp(-cdepth * 0.30f));
+        }
     }
     constexpr float FACE_SHADE[8] = {0.62f, 0.62f, 0.50f, 1.00f, 0.80f, 0.80f, 0.90f, 0.90f};
     const bool repair = (k.bits & RS_LIGHTING_REPAIR) != 0u;
@@ -1561,8 +1578,7 @@ int resolve_bits(uint32_t spec, uint32_t spec_hi, const vx_gpu_frame &g, uint32_
     constexpr uint32_t PROBE_TAP = 16777216u, PROBE_AMBIENT = 33554432u;
     if ((g.flags & VX_FLAG_UNDERWATER) || (spec & (PROBE_TAP | PROBE_AMBIENT)) || (spec & VX_FLAG_FOLIAGE))
         return VX_E_UNSUPPORTED;
-    if (spec_hi & (64u | 8192u | 16384u | 65536u)) return VX_E_UNSUPPORTED;  // caustics, foliage rich, canopy relief,
-                                                                              // soft shadows
+    if (spec_hi & (8192u | 16384u | 65536u)) return VX_E_UNSUPPORTED;  // foliage rich, canopy relief, soft shadows
     if (spec_hi & 4096u) return VX_E_UNSUPPORTED;                           // the water look (ripples, murk)
     if (g.cloud_cover > 0.0f) return VX_E_UNSUPPORTED;                      // cloud deck
     const bool terrain_shafts = (spec & 8192u) && (g.flags & 8192u);
@@ -1588,6 +1604,7 @@ int resolve_bits(uint32_t spec, uint32_t spec_hi, const vx_gpu_frame &g, uint32_
     if (spec & 134217728u) bits |= RS_PROBE_BOUNCE;
     if (spec & 536870912u) bits |= RS_PROBE_SUN;
     if (spec & 1073741824u) bits |= RS_PROBE_SUN_HIGH;
+    if (spec_hi & 64u) bits |= RS_CAUSTICS;
     return VX_OK;
 }
 
