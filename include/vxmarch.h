@@ -283,10 +283,13 @@ int vx_march_stats_compute(const uint32_t *heat, size_t n, uint32_t chunks, vx_m
  * biome tint (resolve.wgsl:166-287: OpenSimplex2 temperature and humidity at the hit's
  * x, z, band weights, the tint table) weighted by the texel's alpha and tint_strength.
  * SPEC_CAUSTICS lifts the sun term of points up to 8 below the sea surface by the light
- * sheets of the first two wave octaves (resolve.wgsl:530-549).
+ * sheets of the first two wave octaves (resolve.wgsl:530-549). SPEC_SOFT_SHADOWS runs
+ * shadow_horizontal and shadow_vertical (shadow.wgsl:49-116) over the frame's keys and
+ * shadow words first -- per pixel a guided average of the occlusion over the row, then
+ * the column, within the penumbra radius of each blocker distance (at most 4 pixels),
+ * stored as an R8 mask -- and lane 0 takes its sun visibility from that mask.
  * Views that need what is not restated return VX_E_UNSUPPORTED: the probe tap and probe
- * ambient (FLAG_PROBE_TAP, FLAG_PROBE_AMBIENT),
- * soft shadows, foliage-rich or canopy-relief albedo, the water look, a cloud deck
+ * ambient (FLAG_PROBE_TAP, FLAG_PROBE_AMBIENT), foliage-rich or canopy-relief albedo, the water look, a cloud deck
  * (cloud_cover > 0), terrain god-ray shafts (godray_strength > 0 with
  * FLAG_TERRAIN_SHAFTS), the underwater view.
  * vx_resolve_set_assets copies the atlas and block face table the reference's host builds

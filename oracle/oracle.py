@@ -114,6 +114,8 @@ def _bind(L):
     L.vxo_shaft_scan.argtypes = [V, V, V]
     L.vxo_terrain_shade.restype = C.c_float
     L.vxo_terrain_shade.argtypes = [V, V, V, C.c_float]
+    L.vxo_soft_mask.restype = C.c_int
+    L.vxo_soft_mask.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, C.POINTER(RSpec), V, V, V, V, C.c_int]
     L.vxo_probe_field.restype = None
     L.vxo_probe_field.argtypes = [V, V, C.c_uint32, V]
     L.vxo_biome_noise.restype = C.c_float
@@ -168,7 +170,7 @@ class RSpec(C.Structure):
         [("march", C.c_void_p), ("distant", C.c_uint32), ("shaft", C.c_void_p), ("tint", C.c_uint32),
          ("tint_balance", C.c_uint32), ("probe_cube", C.c_uint32), ("probe_bounce", C.c_uint32),
          ("probe_sun", C.c_uint32), ("probe_sun_high", C.c_uint32), ("probe", C.c_void_p),
-         ("caustics", C.c_uint32)]
+         ("caustics", C.c_uint32), ("soft_shadows", C.c_uint32)]
 
     def with_march(self, spec: "Spec") -> "RSpec":
         self._march = spec  # kept alive with the record
@@ -270,7 +272,8 @@ def _rspec(flags: int, spec_hi: int) -> RSpec:
                  tex_variation=1 if flags & 16384 else 0, glass=1 if flags & 131072 else 0,
                  lanes=1, water_sec=1 if spec_hi & 32 else 0, shore_foam=1 if spec_hi & 16 else 0,
                  flat_secondary=1 if flags & 1048576 else 0, tint=1 if flags & 128 else 0,
-                 tint_balance=1 if spec_hi & 512 else 0, caustics=1 if spec_hi & 64 else 0).with_march(
+                 tint_balance=1 if spec_hi & 512 else 0, caustics=1 if spec_hi & 64 else 0,
+                 soft_shadows=1 if spec_hi & 65536 else 0).with_march(
                      spec_from(flags, spec_hi))
 
 
@@ -295,6 +298,24 @@ def resolve(scene: "Scene", frame_bytes: bytes, rspec: RSpec, atlas: np.ndarray,
     if rc != 0:
         raise RuntimeError("oracle resolve rejected its arguments")
     return out
+
+
+def soft_mask(scene: "Scene", frame_bytes: bytes, rspec: RSpec, vis: np.ndarray, sun: np.ndarray,
+              threads: int = 0) -> tuple[np.ndarray, np.ndarray]:
+    """shadow_horizontal and shadow_vertical (shadow.wgsl:49-116) alone: the row pass's
+    (H, W, 2) f16 (occlusion, support) and the (H, W) u8 mask of clamp(1 - occlusion)."""
+    f = np.frombuffer(frame_bytes, dtype=np.uint8).copy()
+    res = np.frombuffer(frame_bytes[80:88], dtype=np.uint32)
+    w, h = int(res[0]), int(res[1])
+    v = np.ascontiguousarray(vis, dtype=np.uint64)
+    s = np.ascontiguousarray(sun, dtype=np.uint16)
+    hpass = np.zeros((h, w, 2), dtype=np.uint16)
+    mask = np.zeros((h, w), dtype=np.uint8)
+    rc = lib().vxo_soft_mask(C.byref(scene.pools), _p(scene.chunks), scene.n_chunks, _p(scene.grid), _p(f),
+                             C.byref(rspec), _p(v), _p(s), _p(hpass), _p(mask), threads)
+    if rc != 0:
+        raise RuntimeError("oracle soft mask rejected its arguments")
+    return hpass.view(np.float16), mask
 
 
 def taa(scene: "Scene", frame_bytes: bytes, vis: np.ndarray, src: np.ndarray, hist: np.ndarray) -> np.ndarray:

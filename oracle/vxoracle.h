@@ -259,6 +259,9 @@ typedef struct vxo_rspec {
     uint32_t probe_cube, probe_bounce, probe_sun, probe_sun_high;
     const uint16_t *probe;
     uint32_t caustics;  /* SPEC_CAUSTICS */
+    /* SPEC_SOFT_SHADOWS: lane-0 pixels take the sun visibility from the shadow_horizontal /
+     * shadow_vertical mask (R8) built over vis and sun first; sun must not be NULL */
+    uint32_t soft_shadows;
 } vxo_rspec;
 
 /* Distant terrain shadows (src/shaft.rs, shaft.wgsl:1-24, common.wgsl:1557-1580).
@@ -276,6 +279,12 @@ float vxo_biome_noise(int32_t seed, float freq, float x, float z);
 /* probe_field (resolve.wgsl:106-118) at world p for face normal nid: rgba. */
 void vxo_probe_field(const uint16_t *probe, const float p[3], uint32_t nid, float out[4]);
 void vxo_biome_tint(const vxo_frame *f, uint32_t balance, float x, float z, float out[3]);
+/* Soft shadows alone (shadow.wgsl:49-116): shadow_horizontal's RG16F output into h_out
+ * (W * H * 2 f16 words; may be NULL) and shadow_vertical's R8 mask into mask_out (W * H),
+ * for the frame, the rspec's glass switch, vis and the primary_shadow words sun. */
+int vxo_soft_mask(const vxo_pools *p, const vxo_chunk *chunks, uint32_t n_chunks, const uint32_t *grid,
+                  const vxo_frame *f, const vxo_rspec *r, const uint64_t *vis, const uint16_t *sun, uint16_t *h_out,
+                  uint8_t *mask_out, int threads);
 float vxo_terrain_shade(const float *result, const vxo_frame *f, const float p[3], float d);
 
 /* The caller's atlas (RGBA8 sRGB: mip level 0 of every layer, then level 1, ...; level l

@@ -9,7 +9,10 @@ tint, against the oracle through the C ABI, bit for bit.
   factor), over two frames; the oracle without the factor differs on some pixels;
 - a resolve with the flag before any upload reads the zeroed field;
 - views with the workload's flags less probes (biome tint and distant shadows both on),
-  with and without SPEC_TINT_BALANCE, at tint strengths 1 and 0.6.
+  with and without SPEC_TINT_BALANCE, at tint strengths 1 and 0.6;
+- caustics over the shallow sea floor, and soft shadows (SPEC_SOFT_SHADOWS: the row and
+  column passes and the mask read by lane 0) under a low sun, with distant shadows on in
+  one view; the oracle's hard-shadow resolve differs on some pixels.
 """
 import numpy as np
 import pytest
@@ -210,3 +213,52 @@ def test_resolve_with_caustics_matches_oracle(small_world):
         changed += int((plain.view(np.uint32) != ref.view(np.uint32)).any(axis=-1).sum())
         r.close()
     assert changed > 0  # caustics lit some sea-floor pixels
+
+
+def test_resolve_with_soft_shadows_matches_oracle(small_world):
+    """SPEC_SOFT_SHADOWS under a low sun (long blocker distances, penumbrae of several pixels)."""
+    wl = W.WORKLOADS[0]
+    cams = W.random_cameras(wl, 3, seed=59, resolutions=[(256, 144), (200, 120)], flags=BASE_FLAGS)
+    for c in cams:
+        c.pitch = -0.45
+    cams[2].flags |= DISTANT
+    atlas, faces = _atlas(19), _faces_perm()
+    changed = 0
+    for i, cam in enumerate(cams):
+        v = build_view(small_world, cam)
+        v.frame.sun_dir[0], v.frame.sun_dir[1], v.frame.sun_dir[2] = 0.7962, 0.0839, 0.5991
+        v.frame.ambient, v.frame.mip_bias = 0.04, -3.0
+        spec, hi = W.spec_words(cam.flags, small_world)
+        hi = (hi | SKY_SPECULAR | 65536) & ~(64 | 8192 | 16384)
+        r = Renderer(cam.width, cam.height)
+        r.sync_world(small_world)
+        r.set_resolve_assets(atlas, SIZE, LAYERS, MIPS, faces)
+        sf = None
+        if cam.flags & DISTANT:
+            sf = ShaftField()
+            sf.update(wl.gen, v.frame.cam_pos[0], v.frame.cam_pos[2])
+            sf.set_frame(v.frame)
+            r.upload_shaft(sf.heights())
+            r.shaft_scan(v.frame)
+        r.set_view(v)
+        print(f"[soft] view {i} {cam.width}x{cam.height} distant {bool(cam.flags & DISTANT)}", flush=True)
+        r.render(v.frame, spec, hi)
+        r.resolve()
+        r.sync()
+        sc = O.Scene(small_world.pool(0), small_world.pool(1), small_world.pool(2), v.chunks, v.grid)
+        vis, sun, _, _ = O.render(sc, v.frame_bytes, O.spec_from(spec, hi),
+                                  np.zeros(v.tiles[0] * v.tiles[1], np.uint32))
+        assert np.array_equal(r.read_vis(), vis)
+        field = O.shaft_scan(sf.heights(), v.frame_bytes) if sf else None
+        ref = O.resolve(sc, v.frame_bytes, O.rspec_from(spec, hi, shaft=field), atlas, SIZE, LAYERS, MIPS, faces,
+                        vis, sun)
+        got = r.read_color()
+        diff = got.view(np.uint32) != ref.view(np.uint32)
+        assert not diff.any(), f"view {i}: {diff.any(axis=-1).sum()} pixels differ"
+        hard = O.resolve(sc, v.frame_bytes, O.rspec_from(spec, hi & ~65536, shaft=field), atlas, SIZE, LAYERS, MIPS,
+                         faces, vis, sun)
+        changed += int((hard.view(np.uint32) != ref.view(np.uint32)).any(axis=-1).sum())
+        r.close()
+        if sf:
+            sf.close()
+    assert changed > 0  # the penumbrae changed some pixels

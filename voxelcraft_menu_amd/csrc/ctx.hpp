@@ -99,6 +99,7 @@ struct vx_ctx {
     vxh::DevBuf sec_r, sec_m;  // the resolve's water_sec legs (2x2 blocks, f16 RGBA)
     vxh::DevBuf shaft;         // the distant-shadow field: scan slices 0 and 1, heights in slice 2
     vxh::DevBuf probe;         // the ambient probe field: 128 x 768 x 128 RGBA16F
+    vxh::DevBuf soft_h, soft_mask;  // soft shadows: the row pass (RG16F) and the mask (R8)
     bool probe_pinned = false; // filled with a value: writes are ignored
     size_t hist_len = 0;
     uint32_t taa_last = 0;

@@ -27,7 +27,7 @@ constexpr uint32_t RS_SKY_SPECULAR = 1u, RS_LIGHTING_REPAIR = 2u, RS_SKY_TINT = 
                    RS_GLASS = 512u, RS_WATER_SEC = 1024u, RS_SHORE_FOAM = 2048u, RS_FLAT_SECONDARY = 4096u,
                    RS_DISTANT = 8192u, RS_TINT = 16384u, RS_TINT_BALANCE = 32768u, RS_PROBE_CUBE = 65536u,
                    RS_PROBE_BOUNCE = 131072u, RS_PROBE_SUN = 262144u, RS_PROBE_SUN_HIGH = 524288u,
-                   RS_CAUSTICS = 1048576u;
+                   RS_CAUSTICS = 1048576u, RS_SOFT_SHADOWS = 2097152u;
 
 struct ResolveK {
     Pools p;
@@ -57,6 +57,10 @@ struct ResolveK {
     int32_t tint_seed;
     // the ambient probe field (RGBA16F texels)
     const ushort4 *__restrict__ probe;
+    // soft shadows: shadow_horizontal's RG16F output and shadow_vertical's R8 mask (the
+    // primary sun visibility the resolve reads), or null without SPEC_SOFT_SHADOWS
+    ushort2 *__restrict__ soft_h;
+    uint8_t *__restrict__ soft_mask;
 };
 
 // ---- elementary functions (the oracle's vxo_log2 / vxo_exp2 / vxo_pow) ----
@@ -1257,8 +1261,9 @@ __global__ __launch_bounds__(64) void k_resolve(ResolveK k) {
         // (shadow.wgsl:61-116): clamp(1 - occ, 0, 1), occ = blocker distance > 0
         const float d = f16_word(k.sun ? k.sun[pix] : (uint16_t)0);
         const float occ = d > 0.0f ? 1.0f : 0.0f;
-        shade_opaque(k, ci, (low >> 16) & 63u, (low >> 8) & 63u, low & 63u, h.id, h.nid, h.hit, h.rd, h.t,
-                     r_clamp(1.0f - occ, 0.0f, 1.0f), f.shadow_dist, true, true, color);
+        const float lit = k.soft_mask ? (float)k.soft_mask[pix] / 255.0f : r_clamp(1.0f - occ, 0.0f, 1.0f);
+        shade_opaque(k, ci, (low >> 16) & 63u, (low >> 8) & 63u, low & 63u, h.id, h.nid, h.hit, h.rd, h.t, lit,
+                     f.shadow_dist, true, true, color);
         const float alpha = 1.0f - transmittance_from(k, f.cam[1] + 0.0f * h.rd[1], h.t - 0.0f, h.rd);
         float sky[3];
         sky_base(k, h.rd, 1.0f, sky);
@@ -1527,6 +1532,119 @@ int shaft_buffer(vx_ctx *c) {
     return vxh::fill_u32(c->stream, c->shaft.p, 0u, (size_t)3 * SHAFT_SLICE);
 }
 
+// ---- soft shadows: shadow_horizontal / shadow_vertical (shadow.wgsl:49-116) ----
+constexpr float SUN_ANGLE = 0.0093f, SOFT_PENUMBRA = 1.5f;
+constexpr int32_t SHADOW_RADIUS_LIMIT = 4;
+
+struct SGuide {
+    float hit[3], n[3], t, scale;
+    bool valid;
+};
+
+// shadow_guide (shadow.wgsl:8-30) at pixel (x, y)
+__device__ inline SGuide soft_guide(const ResolveK &k, int32_t x, int32_t y) {
+    SGuide g;
+    g.valid = false;
+    const FrameK &f = k.f;
+    if (x < 0 || y < 0 || x >= (int32_t)f.res[0] || y >= (int32_t)f.res[1]) return g;
+    const uint64_t key = k.vis[(size_t)y * f.res[0] + (uint32_t)x];
+    const KeyHit h = key_hit(k.p, f, key, (uint32_t)x, (uint32_t)y);
+    if (!h.ok || h.id == WATER_ID || ((k.bits & RS_GLASS) && h.id == GLASS_ID)) return g;
+    normal_of(h.nid, g.n);
+    if (h.nid >= NORMAL_CROSS_A && dot3o(g.n, h.rd) > 0.0f) {
+        g.n[0] = -g.n[0]; g.n[1] = -g.n[1]; g.n[2] = -g.n[2];
+    }
+    if (!(dot3o(g.n, f.sun) > 0.0f)) return g;
+    g.t = h.t;
+#pragma unroll
+    for (int q = 0; q < 3; q++) g.hit[q] = h.hit[q];
+    g.scale = ld_chunk(k.p, (uint32_t)(key >> 24) & 0x1FFFu).vs;
+    g.valid = g.t >= 0.0f;
+    return g;
+}
+
+__device__ __forceinline__ float shadow_radius(const FrameK &f, float distance, float depth) {
+    const float proj = (float)f.res[1] / (2.0f * fmaxf(f.tan_half_fov, 0.001f));
+    const float r = distance * SUN_ANGLE * SOFT_PENUMBRA * proj / fmaxf(depth, 0.01f);
+    return r >= 0.5f ? fminf(r, (float)SHADOW_RADIUS_LIMIT) : 0.0f;
+}
+
+__device__ __forceinline__ bool shadow_compatible(const SGuide &a, const SGuide &b) {
+    if (!b.valid || dot3o(a.n, b.n) < 0.95f) return false;
+    const float tol = fmaxf(0.01f, 0.01f * fminf(a.scale, b.scale));
+    const float d[3] = {b.hit[0] - a.hit[0], b.hit[1] - a.hit[1], b.hit[2] - a.hit[2]};
+    return fabsf(dot3o(d, a.n)) <= tol && fabsf(dot3o(d, b.n)) <= tol;
+}
+
+__device__ __forceinline__ bool soft_active(const FrameK &f) {
+    return (f.flags & VX_FLAG_SHADOWS) && !(f.flags & VX_FLAG_HEATMAP) && f.daylight > 0.01f;
+}
+
+// shadow_horizontal: per pixel the occlusion averaged along its row over compatible
+// neighbours within their penumbra radius, and the largest radius met (RG16F).
+__global__ __launch_bounds__(256) void k_soft_h(ResolveK k) {
+    const uint32_t pix = blockIdx.x * 256u + threadIdx.x;
+    const FrameK &f = k.f;
+    if (pix >= f.res[0] * f.res[1]) return;
+    const int32_t x = (int32_t)(pix % f.res[0]), y = (int32_t)(pix / f.res[0]);
+    const float d = f16_word(k.sun[pix]);
+    const float occ = d > 0.0f ? 1.0f : 0.0f;
+    float rx = occ, ry = 0.0f;
+    if (soft_active(f)) {
+        const SGuide center = soft_guide(k, x, y);
+        if (center.valid) {
+            const float rc = shadow_radius(f, d, center.t);
+            float sum = occ, weight = 1.0f, support = rc;
+            for (int32_t q = -SHADOW_RADIUS_LIMIT; q <= SHADOW_RADIUS_LIMIT; q++) {
+                if (q == 0) continue;
+                const SGuide g = soft_guide(k, x + q, y);
+                if (!shadow_compatible(center, g)) continue;
+                const float nd = f16_word(k.sun[pix + q]);
+                const float r = fmaxf(rc, shadow_radius(f, nd, g.t));
+                if (r < fabsf((float)q)) continue;
+                const float w = wave_exp(-2.0f * (float)(q * q) / fmaxf(r * r, 0.25f));
+                sum = sum + (nd > 0.0f ? 1.0f : 0.0f) * w;
+                weight = weight + w;
+                support = fmaxf(support, r);
+            }
+            rx = sum / weight;
+            ry = support;
+        }
+    }
+    k.soft_h[pix] = make_ushort2(f16h(rx), f16h(ry));
+}
+
+// shadow_vertical: the row averages averaged along the column likewise; the mask is
+// clamp(1 - occ, 0, 1) stored as R8 unorm.
+__global__ __launch_bounds__(256) void k_soft_v(ResolveK k) {
+    const uint32_t pix = blockIdx.x * 256u + threadIdx.x;
+    const FrameK &f = k.f;
+    if (pix >= f.res[0] * f.res[1]) return;
+    const int32_t x = (int32_t)(pix % f.res[0]), y = (int32_t)(pix / f.res[0]);
+    const ushort2 h = k.soft_h[pix];
+    const float hx = f16_word(h.x), hy = f16_word(h.y);
+    float occ = hx;
+    if (soft_active(f)) {
+        const SGuide center = soft_guide(k, x, y);
+        if (center.valid) {
+            float sum = hx, weight = 1.0f;
+            for (int32_t q = -SHADOW_RADIUS_LIMIT; q <= SHADOW_RADIUS_LIMIT; q++) {
+                if (q == 0) continue;
+                const SGuide g = soft_guide(k, x, y + q);
+                if (!shadow_compatible(center, g)) continue;
+                const ushort2 o = k.soft_h[(size_t)(y + q) * f.res[0] + (uint32_t)x];
+                const float r = fmaxf(hy, f16_word(o.y));
+                if (r < fabsf((float)q)) continue;
+                const float w = wave_exp(-2.0f * (float)(q * q) / fmaxf(r * r, 0.25f));
+                sum = sum + f16_word(o.x) * w;
+                weight = weight + w;
+            }
+            occ = sum / weight;
+        }
+    }
+    k.soft_mask[pix] = (uint8_t)floorf(r_clamp(1.0f - occ, 0.0f, 1.0f) * 255.0f + 0.5f);
+}
+
 // make_probe's fill (src/render/mod.rs:2270-2287): value, or the hashed noise.
 __global__ __launch_bounds__(256) void k_probe_fill(ushort4 *__restrict__ t, float value, int noise) {
     const size_t i = (size_t)blockIdx.x * 256u + threadIdx.x;
@@ -1578,7 +1696,7 @@ int resolve_bits(uint32_t spec, uint32_t spec_hi, const vx_gpu_frame &g, uint32_
     constexpr uint32_t PROBE_TAP = 16777216u, PROBE_AMBIENT = 33554432u;
     if ((g.flags & VX_FLAG_UNDERWATER) || (spec & (PROBE_TAP | PROBE_AMBIENT)) || (spec & VX_FLAG_FOLIAGE))
         return VX_E_UNSUPPORTED;
-    if (spec_hi & (8192u | 16384u | 65536u)) return VX_E_UNSUPPORTED;  // foliage rich, canopy relief, soft shadows
+    if (spec_hi & (8192u | 16384u)) return VX_E_UNSUPPORTED;  // foliage-rich and canopy-relief albedo
     if (spec_hi & 4096u) return VX_E_UNSUPPORTED;                           // the water look (ripples, murk)
     if (g.cloud_cover > 0.0f) return VX_E_UNSUPPORTED;                      // cloud deck
     const bool terrain_shafts = (spec & 8192u) && (g.flags & 8192u);
@@ -1605,6 +1723,7 @@ int resolve_bits(uint32_t spec, uint32_t spec_hi, const vx_gpu_frame &g, uint32_
     if (spec & 536870912u) bits |= RS_PROBE_SUN;
     if (spec & 1073741824u) bits |= RS_PROBE_SUN_HIGH;
     if (spec_hi & 64u) bits |= RS_CAUSTICS;
+    if (spec_hi & 65536u) bits |= RS_SOFT_SHADOWS;
     return VX_OK;
 }
 
@@ -1676,6 +1795,14 @@ int vx_resolve(vx_ctx *c) {
     k.tint_freq_t = g.tint_freq_t;
     k.tint_freq_h = g.tint_freq_h;
     k.tint_seed = g.tint_seed;
+    k.soft_h = nullptr;
+    k.soft_mask = nullptr;
+    if ((bits & RS_SOFT_SHADOWS) && c->shadow.p) {
+        const size_t np = (size_t)c->w * c->h;
+        if ((rc = vxh::ensure(c->soft_h, np * 4)) || (rc = vxh::ensure(c->soft_mask, np))) return rc;
+        k.soft_h = (ushort2 *)c->soft_h.p;
+        k.soft_mask = (uint8_t *)c->soft_mask.p;
+    }
     k.probe = nullptr;
     if (bits & (RS_PROBE_CUBE | RS_PROBE_BOUNCE)) {
         if ((rc = probe_buffer(c))) return rc;  // never filled: the field of ones
@@ -1695,6 +1822,12 @@ int vx_resolve(vx_ctx *c) {
         VX_LAUNCHED();
     } else {
         k.sec_r = k.sec_m = nullptr;
+    }
+    if (k.soft_mask) {
+        const unsigned blocks = (unsigned)(((size_t)c->w * c->h + 255) / 256);
+        k_soft_h<<<blocks, 256, 0, c->stream>>>(k);
+        k_soft_v<<<blocks, 256, 0, c->stream>>>(k);
+        VX_LAUNCHED();
     }
     k_resolve<<<c->tx * c->ty, 64, 0, c->stream>>>(k);
     VX_LAUNCHED();

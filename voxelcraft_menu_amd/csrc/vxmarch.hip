@@ -238,7 +238,7 @@ int vx_ctx_destroy(vx_ctx *c) {
                       &c->heat,    &c->build_dense, &c->build_leaf, &c->build_words, &c->build_l1, &c->build_out,
                       &c->atlas,   &c->faces,     &c->color,  &c->image,       &c->flat,
                       &c->hist[0], &c->hist[1],   &c->taa_f32,   &c->sec_r,       &c->sec_m,    &c->shaft,
-                      &c->probe};
+                      &c->probe,   &c->soft_h,    &c->soft_mask};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto &set : c->ring)
