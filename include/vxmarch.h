@@ -288,8 +288,11 @@ int vx_march_stats_compute(const uint32_t *heat, size_t n, uint32_t chunks, vx_m
  * shadow words first -- per pixel a guided average of the occlusion over the row, then
  * the column, within the penumbra radius of each blocker distance (at most 4 pixels),
  * stored as an R8 mask -- and lane 0 takes its sun visibility from that mask.
- * Views that need what is not restated return VX_E_UNSUPPORTED: the probe tap and probe
- * ambient (FLAG_PROBE_TAP, FLAG_PROBE_AMBIENT), foliage-rich or canopy-relief albedo, the water look, a cloud deck
+ * FLAG_PROBE_TAP scales the gathered ambient by probe_tap (the probe field at fract(p / 512)
+ * with clamp-to-edge linear filtering, resolve.wgsl:120-127); FLAG_PROBE_AMBIENT replaces it
+ * by probe_tap and drops the face shade and the floor light.
+ * Views that need what is not restated return VX_E_UNSUPPORTED: foliage-rich or
+ * canopy-relief albedo, the water look, a cloud deck
  * (cloud_cover > 0), terrain god-ray shafts (godray_strength > 0 with
  * FLAG_TERRAIN_SHAFTS), the underwater view.
  * vx_resolve_set_assets copies the atlas and block face table the reference's host builds

@@ -116,6 +116,8 @@ def _bind(L):
     L.vxo_terrain_shade.argtypes = [V, V, V, C.c_float]
     L.vxo_soft_mask.restype = C.c_int
     L.vxo_soft_mask.argtypes = [C.POINTER(Pools), V, C.c_uint32, V, V, C.POINTER(RSpec), V, V, V, V, C.c_int]
+    L.vxo_probe_tap.restype = None
+    L.vxo_probe_tap.argtypes = [V, V, V]
     L.vxo_probe_field.restype = None
     L.vxo_probe_field.argtypes = [V, V, C.c_uint32, V]
     L.vxo_biome_noise.restype = C.c_float
@@ -170,7 +172,8 @@ class RSpec(C.Structure):
         [("march", C.c_void_p), ("distant", C.c_uint32), ("shaft", C.c_void_p), ("tint", C.c_uint32),
          ("tint_balance", C.c_uint32), ("probe_cube", C.c_uint32), ("probe_bounce", C.c_uint32),
          ("probe_sun", C.c_uint32), ("probe_sun_high", C.c_uint32), ("probe", C.c_void_p),
-         ("caustics", C.c_uint32), ("soft_shadows", C.c_uint32)]
+         ("caustics", C.c_uint32), ("soft_shadows", C.c_uint32), ("probe_tap", C.c_uint32),
+         ("probe_ambient", C.c_uint32)]
 
     def with_march(self, spec: "Spec") -> "RSpec":
         self._march = spec  # kept alive with the record
@@ -200,7 +203,8 @@ class ResolveAssets(C.Structure):
 
 SHAFT_DIM = 512  # src/shaft.rs:3
 PROBE_SHAPE = (128, 768, 128, 4)  # probe_tex (z, y, x, rgba), src/render/mod.rs:2240-2268
-PROBE_BITS = {"cube": 67108864, "bounce": 134217728, "sun": 536870912, "sun_high": 1073741824}
+PROBE_BITS = {"cube": 67108864, "bounce": 134217728, "sun": 536870912, "sun_high": 1073741824,
+              "tap": 16777216, "ambient": 33554432}
 SHAFT_SOFT = 4.0  # src/shaft.rs:9
 
 
@@ -211,9 +215,9 @@ def rspec_from(flags: int, spec_hi: int, shaft: np.ndarray | None = None, probe:
     distant = bool(flags & 32768)
     if distant and shaft is None:
         raise ValueError("distant terrain shadows read the shaft field: pass its scan result")
-    sampled = bool(flags & (PROBE_BITS["cube"] | PROBE_BITS["bounce"]))
+    sampled = bool(flags & (PROBE_BITS["cube"] | PROBE_BITS["bounce"] | PROBE_BITS["tap"] | PROBE_BITS["ambient"]))
     if sampled and probe is None:
-        raise ValueError("the probe cube and bounce read the probe field: pass it")
+        raise ValueError("the probe cube, bounce, tap and ambient read the probe field: pass it")
     r = _rspec(flags, spec_hi)
     for k, bit in PROBE_BITS.items():
         setattr(r, f"probe_{k}", 1 if flags & bit else 0)
@@ -228,6 +232,15 @@ def probe_field(field: np.ndarray, p, nid: int) -> np.ndarray:
     q = np.asarray(p, dtype=np.float32)
     out = np.zeros(4, np.float32)
     lib().vxo_probe_field(_p(fl), _p(q), nid, _p(out))
+    return out
+
+
+def probe_tap(field: np.ndarray, p) -> np.ndarray:
+    """probe_tap (resolve.wgsl:120-127) of a PROBE_SHAPE field at world p: rgb."""
+    fl = np.ascontiguousarray(field, dtype=np.uint16)
+    q = np.asarray(p, dtype=np.float32)
+    out = np.zeros(3, np.float32)
+    lib().vxo_probe_tap(_p(fl), _p(q), _p(out))
     return out
 
 

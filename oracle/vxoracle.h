@@ -262,6 +262,9 @@ typedef struct vxo_rspec {
     /* SPEC_SOFT_SHADOWS: lane-0 pixels take the sun visibility from the shadow_horizontal /
      * shadow_vertical mask (R8) built over vis and sun first; sun must not be NULL */
     uint32_t soft_shadows;
+    /* SPEC_PROBE_TAP (the gathered ambient times probe_tap) and SPEC_PROBE_AMBIENT (probe_tap
+     * alone, face shade 1, no floor light); both read `probe` */
+    uint32_t probe_tap, probe_ambient;
 } vxo_rspec;
 
 /* Distant terrain shadows (src/shaft.rs, shaft.wgsl:1-24, common.wgsl:1557-1580).
@@ -278,6 +281,8 @@ int vxo_shaft_scan(const float *heights, const vxo_frame *f, float *result);
 float vxo_biome_noise(int32_t seed, float freq, float x, float z);
 /* probe_field (resolve.wgsl:106-118) at world p for face normal nid: rgba. */
 void vxo_probe_field(const uint16_t *probe, const float p[3], uint32_t nid, float out[4]);
+/* probe_tap (resolve.wgsl:120-127) at world p: rgb. */
+void vxo_probe_tap(const uint16_t *probe, const float p[3], float out[3]);
 void vxo_biome_tint(const vxo_frame *f, uint32_t balance, float x, float z, float out[3]);
 /* Soft shadows alone (shadow.wgsl:49-116): shadow_horizontal's RG16F output into h_out
  * (W * H * 2 f16 words; may be NULL) and shadow_vertical's R8 mask into mask_out (W * H),

@@ -61,11 +61,14 @@ def test_probe_field_layout():
     r.close()
 
 
-def test_resolve_with_probes_matches_oracle():
+@pytest.mark.parametrize("extra", [0, 16777216, 33554432], ids=["workload", "probe-tap", "probe-ambient"])
+def test_resolve_with_probes_matches_oracle(extra):
+    """The workload's flags, then with SPEC_PROBE_TAP and with SPEC_PROBE_AMBIENT added."""
     wl = W.WORKLOADS[0]
     world = W.make_world(wl)
-    flags = W.DEFAULT_FLAGS
-    assert flags & CUBE and flags & BOUNCE and flags & SUN and not flags & (16777216 | 33554432)
+    assert W.DEFAULT_FLAGS & CUBE and W.DEFAULT_FLAGS & BOUNCE and W.DEFAULT_FLAGS & SUN
+    assert not W.DEFAULT_FLAGS & (16777216 | 33554432)
+    flags = W.DEFAULT_FLAGS | extra
     cams = W.random_cameras(wl, 3, seed=47, resolutions=[(224, 128)], flags=flags)
     for c in cams[1:]:
         c.pitch = -0.5
@@ -95,7 +98,7 @@ def test_resolve_with_probes_matches_oracle():
         r.render(v.frame, spec, hi)
         r.resolve()
         r.sync()
-        print(f"[probe] resolve view {i}", flush=True)
+        print(f"[probe] resolve view {i} extra {extra}", flush=True)
         sc = O.Scene(world.pool(0), world.pool(1), world.pool(2), v.chunks, v.grid)
         vis, sun, _, _ = O.render(sc, v.frame_bytes, O.spec_from(spec, hi),
                                   np.zeros(v.tiles[0] * v.tiles[1], np.uint32))
@@ -105,7 +108,8 @@ def test_resolve_with_probes_matches_oracle():
         got = r.read_color()
         diff = got.view(np.uint32) != ref.view(np.uint32)
         assert not diff.any(), f"view {i}: {diff.any(axis=-1).sum()} pixels differ"
-        plain = O.resolve(sc, v.frame_bytes, O.rspec_from(spec & ~(CUBE | BOUNCE | SUN), hi, shaft=shaft), atlas,
+        plain = O.resolve(sc, v.frame_bytes, O.rspec_from(spec & ~(CUBE | BOUNCE | SUN | 16777216 | 33554432), hi,
+                                                          shaft=shaft), atlas,
                           SIZE, LAYERS, MIPS, faces, vis, sun)
         assert (plain.view(np.uint32) != ref.view(np.uint32)).any()  # the probes changed the image
         sf.close()

@@ -168,8 +168,6 @@ def test_unrestated_configurations_are_refused(small_world):
     r.set_resolve_assets(_atlas(), SIZE, LAYERS, MIPS, _faces())
     r.resolve()  # the base configuration is restated
     cases = [("foliage-rich albedo", lambda f: None, 0, 8192),
-             ("probe tap", lambda f: None, 16777216, 0),
-             ("probe ambient", lambda f: None, 33554432, 0),
              ("canopy-relief albedo", lambda f: None, 0, 16384),
              ("cloud deck", lambda f: setattr(f, "cloud_cover", 0.4), 0, 0)]
     for name, frame_edit, spec_add, hi_add in cases:

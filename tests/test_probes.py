@@ -16,7 +16,12 @@ f16): trilinear filtering returns the linear function at texel coordinates p / 4
 Resolve, on the floor-and-pillar chunk: a field of ones with the sun bounce off changes
 no bit; a constant field (rgb B, alpha a) gives the lane-0 closed form with the face shade
 times a and the floor light FLOOR (ambient B + B 0.10) under both lighting forms.
-Planted mistakes 57-61 each fail one of these.
+The probe tap (resolve.wgsl:120-127) on the same linear field returns the linear function
+at the clamped texel coordinates of fract(p / 512) over the whole texture (clamp to edge on
+every axis); cross faces sample the +y slab. Under SPEC_PROBE_TAP with a constant field B
+the floor's sky ambient is scaled by B; under SPEC_PROBE_AMBIENT the ambient is B itself,
+the face shade 1 and the floor light 0.
+Planted mistakes 57-61, 65 and 66 each fail one of these.
 """
 import dataclasses
 import math
@@ -30,7 +35,7 @@ from voxelcraft_menu_amd import workloads as W
 from voxelcraft_menu_amd.world import (albedo, probe_bake, probe_bounce_of_uniform_ground, probe_cube_from_tangents,
                                        probe_uploadable, surface_block)
 
-CUBE, BOUNCE, SUN = 67108864, 134217728, 536870912
+CUBE, BOUNCE, SUN, TAP, AMBIENT = 67108864, 134217728, 536870912, 16777216, 33554432
 GRASS, SNOW, WATER = 3, 14, 13
 FLOOR = np.array([0.85, 0.90, 1.00])
 
@@ -167,6 +172,27 @@ def test_oracle_probe_field_on_a_linear_field():
     assert _field_sample_misses() < 1e-4
 
 
+def _tap_misses():
+    field = _linear_field()
+    rng = np.random.default_rng(6)
+    worst = 0.0
+    for _ in range(300):
+        p = rng.uniform(-1500.0, 1500.0, 3).astype(np.float32)
+        got = O.probe_tap(field, p)
+        u = np.asarray(p, np.float64) / 512.0
+        u = u - np.floor(u)
+        t = np.clip(u * np.array([128.0, 768.0, 128.0]) - 0.5, 0.0, [127.0, 767.0, 127.0])
+        worst = max(worst, float(np.abs(got - t / 8.0).max()))
+    for nid in (6, 7):  # cross faces sample the +y slab
+        p = np.array([100.3, 60.7, 222.9], np.float32)
+        worst = max(worst, float(np.abs(O.probe_field(field, p, nid) - O.probe_field(field, p, 3)).max()))
+    return worst
+
+
+def test_oracle_probe_tap_on_a_linear_field():
+    assert _tap_misses() < 1e-3
+
+
 # ---- resolve ----
 
 def _pillar_probe(field, spec_extra, legacy=False):
@@ -226,6 +252,44 @@ def test_floor_pixels_under_a_constant_field_closed_form(legacy):
     assert _floor_probe_misses(legacy) == 0
 
 
+def _floor_tap_misses(mode, legacy=False):
+    B = float(np.float16(0.6))  # the field holds f16
+    dense, v, vis, sun, out = _pillar_probe(_constant_field(B, 0.5), TAP if mode == "tap" else AMBIENT, legacy)
+    out = out.reshape(-1, 4)
+    f = v.frame
+    d, sy = float(f.daylight), float(f.sun_dir[1])
+    lin = [RP._srgb64(c) for c in RP.LEVEL_RGB]
+    bad = n = 0
+    for i, t, hit, vox, rd in RP._floor_pixels(dense, v, vis):
+        if not RP._open(dense, vox[0], vox[2]):
+            continue
+        lit = 0.0 if float(np.array([sun[i]], np.uint16).view(np.float16)[0]) > 0 else 1.0
+        direct = lit * sy * d if sy > 0.0 and d > 0.01 else 0.0
+        sky = RP.SKY_TINT * d * (1.0 if legacy else 0.6)  # the gathered ambient on the open floor
+        amb = np.full(3, B) if mode == "ambient" else sky * B
+        floor = np.zeros(3) if mode == "ambient" else FLOOR * float(f.ambient)
+        if legacy:
+            light = (0.5 * amb + RP.SUN_RGB * direct * 0.52) * 1.0 + floor
+        else:
+            light = 0.5 * amb * 1.0 + RP.SUN_RGB * direct * 0.78 + floor
+        lod = RP._lod(v, t, rd[1])
+        l0 = int(math.floor(lod))
+        l1 = min(l0 + 1, RP.MIPS - 1)
+        alb = lin[l0] * (1.0 - (lod - l0)) + lin[l1] * (lod - l0)
+        want = alb * light
+        n += 1
+        if np.abs(out[i, :3] - want).max() > 2e-5 * max(1.0, np.abs(want).max()):
+            bad += 1
+    assert n > 400, n
+    return bad
+
+
+@pytest.mark.parametrize("mode", ["tap", "ambient"])
+@pytest.mark.parametrize("legacy", [False, True])
+def test_floor_pixels_under_the_probe_tap_closed_form(mode, legacy):
+    assert _floor_tap_misses(mode, legacy) == 0
+
+
 def _fails(fn, *args):
     try:
         r = fn(*args)
@@ -240,10 +304,16 @@ def _fails(fn, *args):
     (59, "field"),   # y not kept inside the slab
     (60, "floor"),   # the bounce left off the floor light
     (61, "floor"),   # the cube left off the face shade
+    (65, "tap"),     # the tap's x texel coordinate without the half-texel offset
+    (66, "ambient"), # the floor light kept under SPEC_PROBE_AMBIENT
 ])
 def test_planted_probe_mistakes_are_caught(plant, check, tmp_path):
     with O.planted(plant, tmp_path):
         if check == "field":
             assert _fails(lambda: _field_sample_misses() >= 1e-4)
+        elif check == "tap":
+            assert _fails(lambda: _tap_misses() >= 1e-3)
+        elif check == "ambient":
+            assert _fails(_floor_tap_misses, "ambient")
         else:
             assert _fails(_floor_probe_misses)

@@ -27,7 +27,7 @@ constexpr uint32_t RS_SKY_SPECULAR = 1u, RS_LIGHTING_REPAIR = 2u, RS_SKY_TINT = 
                    RS_GLASS = 512u, RS_WATER_SEC = 1024u, RS_SHORE_FOAM = 2048u, RS_FLAT_SECONDARY = 4096u,
                    RS_DISTANT = 8192u, RS_TINT = 16384u, RS_TINT_BALANCE = 32768u, RS_PROBE_CUBE = 65536u,
                    RS_PROBE_BOUNCE = 131072u, RS_PROBE_SUN = 262144u, RS_PROBE_SUN_HIGH = 524288u,
-                   RS_CAUSTICS = 1048576u, RS_SOFT_SHADOWS = 2097152u;
+                   RS_CAUSTICS = 1048576u, RS_PROBE_TAP = 4194304u, RS_PROBE_AMBIENT = 8388608u, RS_SOFT_SHADOWS = 2097152u;
 
 struct ResolveK {
     Pools p;
@@ -498,7 +498,8 @@ __device__ inline void probe_field(const ResolveK &k, const float p[3], uint32_t
     const float tx = p[0] * (1.0f / 4.0f), ty = p[1] * (1.0f / 4.0f), tz = p[2] * (1.0f / 4.0f);
     const float y = fminf(fmaxf(ty, 0.5f), (float)PROBE_Y - 0.5f);
     const float u = tx * (1.0f / (float)PROBE_XZ);
-    const float v = ((float)nid * (float)PROBE_Y + y) * (1.0f / ((float)PROBE_Y * (float)PROBE_SLABS));
+    const uint32_t slab = nid >= NORMAL_CROSS_A ? 3u : nid;
+    const float v = ((float)slab * (float)PROBE_Y + y) * (1.0f / ((float)PROBE_Y * (float)PROBE_SLABS));
     const float w = tz * (1.0f / (float)PROBE_XZ);
     const float X = u * (float)PROBE_XZ - 0.5f, Y = v * (float)PROBE_H - 0.5f, Z = w * (float)PROBE_XZ - 0.5f;
     const float fx0 = floorf(X), fy0 = floorf(Y), fz0 = floorf(Z);
@@ -521,6 +522,39 @@ __device__ inline void probe_field(const ResolveK &k, const float p[3], uint32_t
     for (int q = 0; q < 4; q++)
         out[q] = r_mix(r_mix(r_mix(c[0][q], c[1][q], ax), r_mix(c[2][q], c[3][q], ax), ay),
                        r_mix(r_mix(c[4][q], c[5][q], ax), r_mix(c[6][q], c[7][q], ax), ay), az);
+}
+
+// probe_tap (resolve.wgsl:120-127, one tap): the field sampled at fract(p / 512) across the
+// whole texture, all six slabs included, with the linear sampler (clamp to edge on every axis).
+__device__ inline void probe_tap(const ResolveK &k, const float p[3], float out[3]) {
+    float uvw[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const float s = p[q] * (1.0f / 512.0f) + 0.0f;
+        uvw[q] = s - floorf(s);
+    }
+    const float X = uvw[0] * (float)PROBE_XZ - 0.5f, Y = uvw[1] * (float)PROBE_H - 0.5f,
+                Z = uvw[2] * (float)PROBE_XZ - 0.5f;
+    const float fx0 = floorf(X), fy0 = floorf(Y), fz0 = floorf(Z);
+    const float ax = X - fx0, ay = Y - fy0, az = Z - fz0;
+    const int32_t ix = (int32_t)fx0, iy = (int32_t)fy0, iz = (int32_t)fz0;
+    const int32_t xz_max = (int32_t)PROBE_XZ - 1, h_max = (int32_t)PROBE_H - 1;
+    const uint32_t x0 = (uint32_t)min(max(ix, 0), xz_max), x1 = (uint32_t)min(max(ix + 1, 0), xz_max);
+    const uint32_t y0 = (uint32_t)min(max(iy, 0), h_max), y1 = (uint32_t)min(max(iy + 1, 0), h_max);
+    const uint32_t z0 = (uint32_t)min(max(iz, 0), xz_max), z1 = (uint32_t)min(max(iz + 1, 0), xz_max);
+    float c[8][4];
+    probe_texel(k.probe, x0, y0, z0, c[0]);
+    probe_texel(k.probe, x1, y0, z0, c[1]);
+    probe_texel(k.probe, x0, y1, z0, c[2]);
+    probe_texel(k.probe, x1, y1, z0, c[3]);
+    probe_texel(k.probe, x0, y0, z1, c[4]);
+    probe_texel(k.probe, x1, y0, z1, c[5]);
+    probe_texel(k.probe, x0, y1, z1, c[6]);
+    probe_texel(k.probe, x1, y1, z1, c[7]);
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+        out[q] = 1.0f * r_mix(r_mix(r_mix(c[0][q], c[1][q], ax), r_mix(c[2][q], c[3][q], ax), ay),
+                              r_mix(r_mix(c[4][q], c[5][q], ax), r_mix(c[6][q], c[7][q], ax), ay), az);
 }
 
 // terrain_shade_beyond (common.wgsl:1571-1580): the scanned horizon at p + sun_dir * d
@@ -707,6 +741,14 @@ __device__ inline void shade_opaque(const ResolveK &k, uint32_t ci, uint32_t vx,
         for (int q = 0; q < 3; q++)
             amb3[q] = r_mix(r_mix(amb_c[0][q], amb_c[1][q], fa), r_mix(amb_c[2][q], amb_c[3][q], fa), fb);
     }
+    // SPEC_PROBE_AMBIENT: the tap replaces the gathered ambient; SPEC_PROBE_TAP scales it
+    const bool probe_amb = (k.bits & RS_PROBE_AMBIENT) != 0u;
+    if (k.bits & (RS_PROBE_AMBIENT | RS_PROBE_TAP)) {
+        float tap[3];
+        probe_tap(k, hit, tap);
+#pragma unroll
+        for (int q = 0; q < 3; q++) amb3[q] = probe_amb ? tap[q] : amb3[q] * tap[q];
+    }
     float direct = 0.0f;
     const float ndl = dot3o(n, f.sun);
     if (ndl > 0.0f && f.daylight > 0.01f) {
@@ -739,14 +781,14 @@ __device__ inline void shade_opaque(const ResolveK &k, uint32_t ci, uint32_t vx,
     const bool repair = (k.bits & RS_LIGHTING_REPAIR) != 0u;
     // the probe field at hit + n * PROBE_NORMAL_BIAS (resolve.wgsl:552-576)
     float probe[4] = {1.0f, 1.0f, 1.0f, 1.0f};
-    if (k.bits & (RS_PROBE_CUBE | RS_PROBE_BOUNCE)) {
+    if (!probe_amb && (k.bits & (RS_PROBE_CUBE | RS_PROBE_BOUNCE))) {
         float pp[3];
 #pragma unroll
         for (int q = 0; q < 3; q++) pp[q] = hit[q] + n[q] * 2.0f;
         probe_field(k, pp, nid, probe);
     }
-    float shade = repair ? 1.0f : FACE_SHADE[nid];
-    if (k.bits & RS_PROBE_CUBE) shade = shade * probe[3];
+    float shade = probe_amb || repair ? 1.0f : FACE_SHADE[nid];
+    if (!probe_amb && (k.bits & RS_PROBE_CUBE)) shade = shade * probe[3];
     const float sun_gain = (k.bits & RS_PROBE_SUN_HIGH) ? 0.25f : 0.10f;
     const float sun_k = direct * (repair ? 0.78f : 0.52f);
     constexpr float SUN_RGB[3] = {1.0000f, 0.8469f, 0.5647f}, FLOOR_RGB[3] = {0.85f, 0.90f, 1.00f};
@@ -755,9 +797,12 @@ __device__ inline void shade_opaque(const ResolveK &k, uint32_t ci, uint32_t vx,
     for (int q = 0; q < 3; q++) {
         const float ambient_rgb = amb3[q] * 0.50f;
         const float sun_rgb = SUN_RGB[q] * sun_k;
-        float floor_rgb = FLOOR_RGB[q] * k.ambient;
-        if (k.bits & RS_PROBE_BOUNCE) floor_rgb = floor_rgb * probe[q];
-        if (k.bits & RS_PROBE_SUN) floor_rgb = floor_rgb + FLOOR_RGB[q] * probe[q] * (sun_gain * sky_sm);
+        float floor_rgb = 0.0f;
+        if (!probe_amb) {
+            floor_rgb = FLOOR_RGB[q] * k.ambient;
+            if (k.bits & RS_PROBE_BOUNCE) floor_rgb = floor_rgb * probe[q];
+            if (k.bits & RS_PROBE_SUN) floor_rgb = floor_rgb + FLOOR_RGB[q] * probe[q] * (sun_gain * sky_sm);
+        }
         spec_rgb[q] = spec_pre[q] * (sky_sm * SKY_SPEC_GAIN);
         light[q] = repair ? ambient_rgb * ao * shade + sun_rgb + floor_rgb : ((ambient_rgb + sun_rgb) * ao + floor_rgb) * shade;
     }
@@ -1693,9 +1738,7 @@ int resolve_bits(uint32_t spec, uint32_t spec_hi, const vx_gpu_frame &g, uint32_
         bits = 0;
         return VX_OK;
     }
-    constexpr uint32_t PROBE_TAP = 16777216u, PROBE_AMBIENT = 33554432u;
-    if ((g.flags & VX_FLAG_UNDERWATER) || (spec & (PROBE_TAP | PROBE_AMBIENT)) || (spec & VX_FLAG_FOLIAGE))
-        return VX_E_UNSUPPORTED;
+    if ((g.flags & VX_FLAG_UNDERWATER) || (spec & VX_FLAG_FOLIAGE)) return VX_E_UNSUPPORTED;
     if (spec_hi & (8192u | 16384u)) return VX_E_UNSUPPORTED;  // foliage-rich and canopy-relief albedo
     if (spec_hi & 4096u) return VX_E_UNSUPPORTED;                           // the water look (ripples, murk)
     if (g.cloud_cover > 0.0f) return VX_E_UNSUPPORTED;                      // cloud deck
@@ -1724,6 +1767,8 @@ int resolve_bits(uint32_t spec, uint32_t spec_hi, const vx_gpu_frame &g, uint32_
     if (spec & 1073741824u) bits |= RS_PROBE_SUN_HIGH;
     if (spec_hi & 64u) bits |= RS_CAUSTICS;
     if (spec_hi & 65536u) bits |= RS_SOFT_SHADOWS;
+    if (spec & 16777216u) bits |= RS_PROBE_TAP;
+    if (spec & 33554432u) bits |= RS_PROBE_AMBIENT;
     return VX_OK;
 }
 
@@ -1804,7 +1849,7 @@ int vx_resolve(vx_ctx *c) {
         k.soft_mask = (uint8_t *)c->soft_mask.p;
     }
     k.probe = nullptr;
-    if (bits & (RS_PROBE_CUBE | RS_PROBE_BOUNCE)) {
+    if (bits & (RS_PROBE_CUBE | RS_PROBE_BOUNCE | RS_PROBE_TAP | RS_PROBE_AMBIENT)) {
         if ((rc = probe_buffer(c))) return rc;  // never filled: the field of ones
         k.probe = (const ushort4 *)c->probe.p;
     }
